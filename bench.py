@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark: nonorthogonal-Wick H,S pair elements per second on one MI355X.
+
+Workload (the synthetic batched-Wick configuration): n_bas = nmo = 100,
+10 alpha + 10 beta electrons, orthonormal AO basis, ERIs from 300 seeded
+Cholesky-like vectors, 8 references C0 exp(kappa), one reference with an
+alpha occupied orbital exactly orthogonal to every other reference (m = 1
+pairs), 512 random singles/doubles per reference: 4096 determinants and
+8,390,656 upper-triangle pairs.
+
+One step = the whole hot path on inputs resident in HBM: the per-reference-
+pair build for every ordered pair the upper triangle needs (x <= w: 36 pairs)
+followed by the evaluation of all 8,390,656 H and S elements into device
+matrices.  value = pairs / second of that end-to-end step.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "noci-rs_amd"))
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sysm, dets, tol, mode):
+    """Time the CPU oracle on a bounded sample of the same workload: the
+    build of one ordered reference pair (0, 1) plus the evaluation of all
+    512 x 512 determinant pairs of that block."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes as ct
+
+    import numpy as np
+
+    import oracle as O
+
+    L = O.lib()
+    L.ow_hs_many.argtypes = [O.D, O.I64, O.D, ct.c_int, ct.c_longlong, O.U64, O.U64, O.D, ct.c_double,
+                             ct.c_double, ct.c_int, O.D, O.D]
+    L.ow_num_threads.restype = ct.c_int
+    threads = L.ow_num_threads()
+    t0 = time.perf_counter()
+    slab = O.Slab(sysm, tol=tol, pairs=[(0, 1)])
+    t1 = time.perf_counter()
+    bra = [d for d in dets if d.parent == 0]
+    ket = [d for d in dets if d.parent == 1]
+    npair = len(bra) * len(ket)
+    xex = np.zeros((npair, 8), dtype=np.uint64)
+    wex = np.zeros((npair, 8), dtype=np.uint64)
+    exph = np.zeros(npair)
+
+    def m8(d):
+        return np.concatenate([O.mask2(d.ha), O.mask2(d.pa), O.mask2(d.hb), O.mask2(d.pb)])
+    bm = np.stack([m8(d) for d in bra])
+    km = np.stack([m8(d) for d in ket])
+    bp = np.array([d.pha * d.phb for d in bra])
+    kp = np.array([d.pha * d.phb for d in ket])
+    xex[:] = np.repeat(bm, len(ket), axis=0)
+    wex[:] = np.tile(km, (len(bra), 1))
+    exph[:] = np.repeat(bp, len(ket)) * np.tile(kp, len(bra))
+    H = np.zeros(npair); S = np.zeros(npair)
+    sub, off, meta = slab.pair_view(0, 1)
+    t2 = time.perf_counter()
+    L.ow_hs_many(O._d(sub), off.ctypes.data_as(O.I64), O._d(meta), sysm.nmo, npair, O._u(xex), O._u(wex),
+                 O._d(exph), sysm.enuc, tol, mode, O._d(H), O._d(S))
+    t3 = time.perf_counter()
+    secs = (t1 - t0) + (t3 - t2)
+    return {"value": npair / secs, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"oracle build of ordered reference pair (0,1) ({t1 - t0:.1f} s) + evaluation of its "
+                      f"{len(bra)}x{len(ket)} determinant pairs ({t3 - t2:.2f} s), OpenMP threads={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", type=int, default=0, help="0: reference arithmetic, 1: corrected m>0 path")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=2024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the engine has no multi-GPU path in this build "
+              "(sharding is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import nwick
+    from nwick import basis, synth
+
+    tol = 1e-8
+    t = time.perf_counter()
+    sysm = synth.wick_batch_n100(seed=args.seed)
+    dets = basis.excited_basis(sysm.nref, sysm.nmo, sysm.nocc_a, sysm.nocc_b, per_ref=512, seed=args.seed)
+    rec = basis.to_records(dets)
+    ndet = len(dets)
+    npairs = ndet * (ndet + 1) // 2
+    log(f"setup: system + {ndet} determinants in {time.perf_counter() - t:.1f} s")
+    eng = nwick.Engine.from_system(sysm, device=0)
+    mask = np.triu(np.ones((sysm.nref, sysm.nref), dtype=np.uint8))
+    dev = torch.device("cuda:0")
+    Hd = torch.empty((ndet, ndet), dtype=torch.float64, device=dev)
+    Sd = torch.empty((ndet, ndet), dtype=torch.float64, device=dev)
+    rec_d = torch.from_numpy(rec.view(np.uint8).copy()).to(dev)
+    torch.cuda.synchronize()
+
+    def step():
+        meta = eng.build(tol=tol, pair_mask=mask)
+        tb = eng.timings()
+        eng.hs_matrix(rec_d, mode=args.mode, out=(Hd, Sd))
+        te = eng.timings()
+        return meta, tb, te
+
+    for w in range(args.warmup):
+        meta, tb, te = step()
+        log(f"warmup {w}: build {tb['total_ms']:.1f} ms (transforms {tb['transform_ms']:.1f} ms), "
+            f"eval {te['eval_ms']:.2f} ms")
+    torch.cuda.synchronize()
+    builds, transforms, evals = [], [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        meta, tb, te = step()
+        builds.append(tb["total_ms"]); transforms.append(tb["transform_ms"]); evals.append(te["eval_ms"])
+        log(f"step {k}: build {tb['total_ms']:.1f} ms (transforms {tb['transform_ms']:.1f} ms), "
+            f"eval {te['eval_ms']:.2f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = npairs * args.steps / elapsed
+    # dominant kernel: the rotating quarter transform, 2 * nmo * rows * n_bas flops per launch
+    n = sysm.n_bas
+    ntensor = 0
+    for (x, w), pm in meta.items():
+        ma, mb = pm.ma, pm.mb
+        slots = lambda m: sum(1 for b in [(0, 0, 0, 0), (0, 0, 0, 1), (0, 0, 1, 0), (0, 0, 1, 1), (0, 1, 0, 1),
+                                          (0, 1, 1, 0), (0, 1, 1, 1), (1, 0, 1, 0), (1, 0, 1, 1), (1, 1, 1, 1)]
+                              if sum(b) <= m)
+        nii = sum(1 for a0 in (0, 1) for aj in (0, 1) for b0 in (0, 1) for bj in (0, 1)
+                  if a0 + aj <= ma and b0 + bj <= mb)
+        ntensor += slots(ma) + slots(mb) + nii
+    launches = 4 * ntensor
+    flops_per_launch = 2.0 * n * n ** 3 * n
+    avg_ms = float(np.mean(transforms)) / launches
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    eval_rate = npairs / (float(np.mean(evals)) * 1e-3)
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline: oracle on pair (0,1) ...")
+        cpu = cpu_baseline(sysm, dets, tol, args.mode)
+        log(f"cpu baseline: {cpu['value']:.1f} pairs/s ({cpu['sample']})")
+    out = {
+        "metric": "Wick H,S pair elements/s (end-to-end: per-reference-pair build + all upper-triangle elements)",
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched Wick elements, n_bas=nmo=100, nocc=10+10, 8 refs (1 with an exactly "
+                               "orthogonal alpha orbital), 512 singles/doubles per ref, 4096 dets, 8390656 pairs",
+                   "mode": "reference" if args.mode == 0 else "corrected", "pairs_built": len(meta),
+                   "rank4_tensors": ntensor},
+        "eval_only_pairs_per_s": eval_rate,
+        "build_ms": float(np.mean(builds)),
+        "transform_ms": float(np.mean(transforms)),
+        "eval_ms": float(np.mean(evals)),
+        "roofline": {"bound": "mfma", "kernel": "k_qtransform (FP64 rotating quarter transform)",
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "launches_per_step": launches, "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,116 @@
+/*
+ * nwick.h -- C ABI of the MI355X nonorthogonal-Wick matrix-element engine.
+ *
+ * Every entry point replaces one piece of noci-rs's Rust surface for the
+ * nonorthogonal-Wick path (paths relative to the noci-rs source tree).  Types
+ * are plain C: 128-bit occupation strings are split into {lo, hi} uint64_t
+ * pairs; matrices are row-major double arrays.
+ *
+ * Buffers: every pointer argument may be host memory or device memory of the
+ * context's GPU; the library detects which and copies only host buffers.
+ * Calls on one context are serialised internally; distinct contexts are
+ * independent.  All functions return 0 (NW_OK) or an nw_status code;
+ * nw_last_error() describes the last failure on a context.
+ */
+#ifndef NWICK_H
+#define NWICK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nw_ctx nw_ctx;
+
+enum nw_status {
+    NW_OK = 0,
+    NW_E_ARG = 1,       /* null pointer / bad value */
+    NW_E_SHAPE = 2,     /* dimensions inconsistent or beyond supported limits */
+    NW_E_HIP = 3,       /* HIP runtime failure */
+    NW_E_MEM = 4,       /* device allocation failed */
+    NW_E_RANK = 5,      /* excitation rank per spin beyond 4 */
+    NW_E_STATE = 6,     /* call order (e.g. evaluation before nw_build) */
+    NW_E_NONFINITE = 7, /* non-finite input */
+    NW_E_ORDER = 8      /* references not occupied-first */
+};
+
+/* Evaluation modes for reference pairs with zero-overlap orbital pairs (m > 0).
+ * NW_MODE_REFERENCE reproduces the arithmetic of
+ * src/nonorthogonalwicks/eval/preparehamiltonianoverlap.rs:20550-20966
+ * literally; NW_MODE_CORRECTED is the exact extended-Wick result (agrees with
+ * the generalised Slater-Condon rules of src/noci/naive.rs).  Both give
+ * identical results when m_alpha = m_beta = 0. */
+enum nw_mode { NW_MODE_REFERENCE = 0, NW_MODE_CORRECTED = 1 };
+
+/* One determinant: the fields of DetState (src/scalar.rs:32-61) the Wick path
+ * reads -- parent reference, excitation phases, occupations and the
+ * excitation relative to the parent (Excitation, src/lib.rs:79-119). */
+typedef struct nw_det {
+    uint32_t parent;
+    uint32_t flags; /* reserved, 0 */
+    double pha, phb;
+    uint64_t oa[2], ob[2];
+    uint64_t holes_a[2], parts_a[2];
+    uint64_t holes_b[2], parts_b[2];
+} nw_det;
+
+/* Per ordered reference pair summary (PairMeta, src/nonorthogonalwicks/types.rs:146). */
+typedef struct nw_pair_meta {
+    int32_t ma, mb;           /* zero-overlap orbital pairs per spin */
+    double tilde_s_a, phase_a; /* SameSpinMeta tilde_s_prod / phase */
+    double tilde_s_b, phase_b;
+} nw_pair_meta;
+
+/* Create a context on `device` holding the AO problem (AoData, src/lib.rs:52-76):
+ * S, h: n_bas x n_bas; eri: dense chemists' (ab|cd) stored [a,b,c,d]
+ * (src/read.rs:31); enuc: nuclear repulsion. */
+int nw_create(nw_ctx **out, int device, int n_bas, const double *S, const double *h,
+              const double *eri, double enuc);
+
+/* Install the reference determinants (the `refs` of build_wicks_shared,
+ * src/noci/wicks.rs:133).  ca, cb: nref x n_bas x nmo row-major MO
+ * coefficients; occ: nref x 4 uint64 {oa.lo, oa.hi, ob.lo, ob.hi}.  Occupied
+ * columns must come first (src/scf/occupation.rs:75-100). */
+int nw_set_refs(nw_ctx *ctx, int nref, int nmo, const double *ca, const double *cb,
+                const uint64_t *occ);
+
+/* Build the per-ordered-reference-pair intermediates on the GPU: replaces
+ * build_wicks_shared (src/noci/wicks.rs:133-279) with plan_wicks_pairs
+ * (:90), assign_offsets (src/nonorthogonalwicks/layout.rs:21),
+ * SameSpinBuild::new (build.rs:86) x2, DiffSpinBuild::new (build.rs:728),
+ * write_same_spin/write_diff_spin/write_hcol0 (layout.rs:199-300).
+ * pair_mask: nref*nref bytes (x*nref + w), nonzero = build; NULL = all pairs.
+ * meta_out: nref*nref entries or NULL. */
+int nw_build(nw_ctx *ctx, double tol, const uint8_t *pair_mask, nw_pair_meta *meta_out);
+
+/* Batched H,S for arbitrary determinant pairs: replaces
+ * calculate_hs_pairs_wicks_batched (src/noci/hs.rs:72-130) and
+ * xw_hamiltonian_overlap_prepared_batched (preparehamiltonianoverlap.rs:121).
+ * req: nreq x 2 uint64 {bra index, ket index} into dets.  h_out, s_out: nreq. */
+int nw_hs_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t *req,
+                  uint64_t nreq, double *h_out, double *s_out, int mode);
+
+/* Full H and S over a determinant list: replaces calculate_matrix_elements
+ * (src/noci/matrix.rs:39-81) + scatter_matrix_elements (:93-110) as used by
+ * build_noci_hs (:214).  Evaluates pairs i <= j and mirrors them; H, S are
+ * ndet x ndet row-major. */
+int nw_hs_matrix(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double *H, double *S, int mode);
+
+/* Copy one ordered pair's slab block (layout of layout.rs:21-189) and its
+ * offsets (95 int64, relative to the block, -1 = absent, last = length) and
+ * scalar meta (32 doubles) out of device memory; out may be NULL to query
+ * the length through offsets[94].  For tests and disk interop. */
+int nw_pair_slab(nw_ctx *ctx, int x, int w, double *out, int64_t *offsets, double *meta);
+
+/* Device time in milliseconds of the kernels in the last nw_build /
+ * nw_hs_matrix / nw_hs_batched call, split as {total, transform, evaluate}. */
+int nw_last_timings(nw_ctx *ctx, double *ms3);
+
+const char *nw_last_error(nw_ctx *ctx);
+void nw_destroy(nw_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,863 @@
+// C ABI of the engine (include/nwick.h): context, per-reference-pair build
+// orchestration and the evaluation entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nwick.h"
+#include "nw_kernels.h"
+
+using namespace nw;
+
+namespace {
+
+struct NwError : std::runtime_error {
+    int code;
+    NwError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            throw NwError(NW_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+template <class T>
+T *dalloc(size_t n)
+{
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw NwError(NW_E_MEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    }
+    return (T *)p;
+}
+
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    template <class T>
+    T *get(size_t n)
+    {
+        size_t need = std::max<size_t>(n * sizeof(T), 8);
+        if (need > bytes) {
+            if (p) HIPCHK(hipFree(p));
+            p = nullptr;
+            bytes = 0;
+            HIPCHK(hipMalloc(&p, need));
+            bytes = need;
+        }
+        return (T *)p;
+    }
+};
+
+bool is_device_ptr(const void *p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice;
+}
+
+// assign_offsets (src/nonorthogonalwicks/layout.rs:21-189) for one pair.
+long long pair_offsets(int ma, int mb, int nmo, long long *off)
+{
+    const long long nn2 = (long long)nmo * nmo, nn4 = nn2 * nn2;
+    long long i = 0;
+    for (int k = 0; k < O_TOTAL; ++k) off[k] = -1;
+    for (int sp = 0; sp < 2; ++sp) {
+        long long *o = off + (sp ? O_BB : O_AA);
+        const int msp = sp ? mb : ma;
+        for (int mi = 0; mi < 2; ++mi) { o[O_X + mi] = i; i += nn2; }
+        for (int mi = 0; mi < 2; ++mi) { o[O_Y + mi] = i; i += nn2; }
+        for (int k = 0; k < 4; ++k) { o[O_FH + k] = i; i += nn2; }
+        for (int k = 0; k < 4; ++k) { o[O_FF + k] = i; i += nn2; }
+        for (int k = 0; k < 8; ++k) { o[O_V + k] = i; i += nn2; }
+        for (int s = 0; s < 10; ++s)
+            if (jbranch(s, 0) + jbranch(s, 1) + jbranch(s, 2) + jbranch(s, 3) <= msp) { o[O_J + s] = i; i += nn4; }
+    }
+    for (int k = 0; k < 8; ++k) { off[O_VAB + k] = i; i += nn2; }
+    for (int k = 0; k < 8; ++k) { off[O_VBA + k] = i; i += nn2; }
+    if (ma == 0 && mb == 0) {
+        off[O_AA + O_HCOL0] = i; i += nn2;
+        off[O_BB + O_HCOL0] = i; i += nn2;
+    }
+    for (int a0 = 0; a0 < 2; ++a0)
+        for (int aj = 0; aj < 2; ++aj)
+            for (int b0 = 0; b0 < 2; ++b0)
+                for (int bj = 0; bj < 2; ++bj)
+                    if (a0 + aj <= ma && b0 + bj <= mb) {
+                        off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj] = i;
+                        i += nn4;
+                    }
+    off[O_TOTAL] = i;
+    return i;
+}
+
+}  // namespace
+
+struct nw_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[4] = {};
+    std::mutex mu;
+    std::string err;
+    int n = 0;
+    double enuc = 0.0;
+    double *d_S = nullptr, *d_h = nullptr, *d_eri = nullptr, *d_easym = nullptr;
+    int nref = 0, nmo = 0, nocc[2] = {0, 0};
+    double *d_c[2] = {nullptr, nullptr};  // nref x n x nmo per spin
+    // build state
+    bool built = false;
+    double tol = 1e-8;
+    double *d_slab = nullptr;
+    size_t slab_cap = 0;
+    std::vector<PairInfo> pinfo;
+    PairInfo *d_pinfo = nullptr;
+    double t_ms[3] = {0, 0, 0};
+    // scratch
+    DBuf arena_desc, scratchA, scratchB, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
+    std::deque<std::vector<char>> keep;
+
+    ~nw_ctx()
+    {
+        (void)hipSetDevice(device);
+        if (st) (void)hipStreamSynchronize(st);
+        for (double *p : {d_S, d_h, d_eri, d_easym, d_c[0], d_c[1], d_slab})
+            if (p) (void)hipFree(p);
+        if (d_pinfo) (void)hipFree(d_pinfo);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace {
+
+// Upload host descriptor arrays into a per-call device arena (append only).
+struct Uploader {
+    nw_ctx *c;
+    char *base;
+    size_t cap, used = 0;
+    Uploader(nw_ctx *ctx, size_t bytes) : c(ctx)
+    {
+        base = c->arena_desc.get<char>(bytes);
+        cap = bytes;
+    }
+    template <class T>
+    T *put(const std::vector<T> &v)
+    {
+        size_t bytes = v.size() * sizeof(T);
+        size_t off = (used + 255) & ~size_t(255);
+        if (off + bytes > cap) throw NwError(NW_E_MEM, "descriptor arena exhausted");
+        if (bytes) {
+            c->keep.emplace_back((const char *)v.data(), (const char *)v.data() + bytes);
+            HIPCHK(hipMemcpyAsync(base + off, c->keep.back().data(), bytes, hipMemcpyHostToDevice, c->st));
+        }
+        used = off + bytes;
+        return (T *)(base + off);
+    }
+};
+
+void gemm(nw_ctx *c, Uploader &up, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc, double alpha,
+          double beta, const std::vector<GemmDesc> &descs)
+{
+    if (descs.empty()) return;
+    GemmDesc *d = up.put(descs);
+    launch_gemm(c->st, ta, tb, M, N, K, lda, ldb, ldc, alpha, beta, d, (int)descs.size());
+}
+
+int fail(nw_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(nw_ctx *c, F &&f)
+{
+    if (!c) return NW_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    try {
+        hipError_t e = hipSetDevice(c->device);
+        if (e != hipSuccess) return fail(c, NW_E_HIP, hipGetErrorString(e));
+        c->keep.clear();
+        int rc = f();
+        c->keep.clear();
+        return rc;
+    } catch (const NwError &e) {
+        (void)hipStreamSynchronize(c->st);
+        c->keep.clear();
+        return fail(c, e.code, e.what());
+    } catch (const std::exception &e) {
+        c->keep.clear();
+        return fail(c, NW_E_ARG, e.what());
+    }
+}
+
+// Copy count*size bytes to a device buffer unless the pointer is already device memory.
+const void *to_device(nw_ctx *c, DBuf &buf, const void *p, size_t bytes)
+{
+    if (is_device_ptr(p)) return p;
+    char *d = buf.get<char>(bytes);
+    HIPCHK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, c->st));
+    return d;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.f;
+    }
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nw_create(nw_ctx **out, int device, int n_bas, const double *S, const double *h, const double *eri, double enuc)
+{
+    if (!out || !S || !h || !eri || n_bas <= 0) return NW_E_ARG;
+    if (n_bas > MAX_NMO) return NW_E_SHAPE;
+    nw_ctx *c = new nw_ctx();
+    c->device = device;
+    c->n = n_bas;
+    c->enuc = enuc;
+    int rc = guarded(c, [&]() -> int {
+        HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+        for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+        const size_t n2 = (size_t)n_bas * n_bas, n4 = n2 * n2;
+        c->d_S = dalloc<double>(n2);
+        c->d_h = dalloc<double>(n2);
+        c->d_eri = dalloc<double>(n4);
+        c->d_easym = dalloc<double>(n4);
+        HIPCHK(hipMemcpyAsync(c->d_S, S, n2 * 8, hipMemcpyDefault, c->st));
+        HIPCHK(hipMemcpyAsync(c->d_h, h, n2 * 8, hipMemcpyDefault, c->st));
+        HIPCHK(hipMemcpyAsync(c->d_eri, eri, n4 * 8, hipMemcpyDefault, c->st));
+        launch_antisym(c->st, c->d_eri, c->d_easym, n_bas);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->st));
+        return NW_OK;
+    });
+    if (rc != NW_OK) {
+        std::fprintf(stderr, "nw_create: %s\n", c->err.c_str());
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return NW_OK;
+}
+
+int nw_set_refs(nw_ctx *c, int nref, int nmo, const double *ca, const double *cb, const uint64_t *occ)
+{
+    return guarded(c, [&]() -> int {
+        if (!ca || !cb || !occ || nref <= 0 || nmo <= 0) return fail(c, NW_E_ARG, "null reference data");
+        if (nmo > MAX_NMO || nmo > c->n) return fail(c, NW_E_SHAPE, "nmo must be <= n_bas and <= 128");
+        if (nref > 65535) return fail(c, NW_E_SHAPE, "too many references");
+        int no[2] = {-1, -1};
+        for (int r = 0; r < nref; ++r)
+            for (int sp = 0; sp < 2; ++sp) {
+                uint64_t lo = occ[r * 4 + sp * 2], hi = occ[r * 4 + sp * 2 + 1];
+                int cnt = __builtin_popcountll(lo) + __builtin_popcountll(hi);
+                // occupied-first: bits 0..cnt-1 set exactly
+                bool first = true;
+                for (int b = 0; b < 128; ++b) {
+                    bool set = b < 64 ? (lo >> b) & 1 : (hi >> (b - 64)) & 1;
+                    if (set != (b < cnt)) first = false;
+                }
+                if (!first) return fail(c, NW_E_ORDER, "reference occupations must be occupied-first");
+                if (no[sp] < 0) no[sp] = cnt;
+                else if (no[sp] != cnt) return fail(c, NW_E_SHAPE, "all references need the same electron count per spin");
+            }
+        if (no[0] > MAX_NOCC || no[1] > MAX_NOCC || no[0] >= nmo || no[1] >= nmo)
+            return fail(c, NW_E_SHAPE, "nocc must be < nmo and <= 64");
+        const size_t len = (size_t)nref * c->n * nmo;
+        for (int sp = 0; sp < 2; ++sp) {
+            if (c->d_c[sp]) HIPCHK(hipFree(c->d_c[sp]));
+            c->d_c[sp] = dalloc<double>(len);
+            HIPCHK(hipMemcpyAsync(c->d_c[sp], sp ? cb : ca, len * 8, hipMemcpyDefault, c->st));
+        }
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->nref = nref;
+        c->nmo = nmo;
+        c->nocc[0] = no[0];
+        c->nocc[1] = no[1];
+        c->built = false;
+        return NW_OK;
+    });
+}
+
+int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta_out)
+{
+    return guarded(c, [&]() -> int {
+        if (c->nref == 0) return fail(c, NW_E_STATE, "nw_set_refs first");
+        const int n = c->n, nm = c->nmo, nref = c->nref;
+        const size_t nn = (size_t)n * n, nnm = (size_t)n * nm, nm2 = (size_t)nm * nm;
+        std::vector<int> px, pw;
+        for (int x = 0; x < nref; ++x)
+            for (int w = 0; w < nref; ++w)
+                if (!pair_mask || pair_mask[x * nref + w]) { px.push_back(x); pw.push_back(w); }
+        const int P = (int)px.size();
+        c->built = false;
+        c->tol = tol;
+        HIPCHK(hipEventRecord(c->ev[0], c->st));
+        Uploader up(c, 64u << 20);
+        // ---- scratch per (pair, spin) -------------------------------------
+        int nomax = std::max(c->nocc[0], c->nocc[1]);
+        std::vector<double *> frees;
+        auto A = [&](size_t cnt) { double *p = dalloc<double>(cnt); frees.push_back(p); return p; };
+        struct Cleanup { std::vector<double *> &f; ~Cleanup() { for (double *p : f) (void)hipFree(p); } } cleanup{frees};
+        double *Tocc = A((size_t)2 * P * n * nomax), *Socc = A((size_t)2 * P * nomax * nomax);
+        double *U = A((size_t)2 * P * nomax * nomax), *V = A((size_t)2 * P * nomax * nomax);
+        double *sig = A((size_t)2 * P * nomax), *phase = A(2 * P), *tsp = A(2 * P);
+        int *mdev = (int *)A(P);
+        double *Lt = A((size_t)2 * P * n * nomax), *Gt = A((size_t)2 * P * n * nomax);
+        double *M = A((size_t)4 * P * nn);          // [(p*2+sp)*2+mi][n][n]
+        double *rowc = A((size_t)2 * P * nnm), *colc = A((size_t)2 * P * nnm);
+        double *cx = A((size_t)4 * P * nnm), *xc = A((size_t)4 * P * nnm);
+        double *MS = A((size_t)4 * P * nn), *MtS = A((size_t)4 * P * nn);
+        double *Q = A((size_t)4 * P * nnm), *R = A((size_t)4 * P * nnm);
+        double *Z = A((size_t)2 * P * nm2);
+        double *JK = A((size_t)4 * P * nn), *Jc = A((size_t)4 * P * nn);
+        double *hxc = A((size_t)4 * P * nnm);
+        double *JKxc = A((size_t)8 * P * nnm);
+        double *dmeta = A((size_t)P * M_LEN);
+        HIPCHK(hipMemsetAsync(dmeta, 0, (size_t)P * M_LEN * 8, c->st));
+        auto Cref = [&](int sp, int r) { return c->d_c[sp] + (size_t)r * nnm; };
+        auto prob = [&](int p, int sp) { return p * 2 + sp; };
+        // spin-local index for no-dependent arrays: (sp*P + p)
+        for (int sp = 0; sp < 2; ++sp) {
+            const int no = c->nocc[sp];
+            std::vector<GemmDesc> g1, g2;
+            for (int p = 0; p < P; ++p) {
+                size_t q = (size_t)sp * P + p;
+                g1.push_back({c->d_S, Cref(sp, pw[p]), Tocc + q * n * nomax});
+                g2.push_back({Cref(sp, px[p]), Tocc + q * n * nomax, Socc + q * nomax * nomax});
+            }
+            gemm(c, up, 0, 0, n, no, n, n, nm, no, 1.0, 0.0, g1);   // S Cw,occ
+            gemm(c, up, 1, 0, no, no, n, nm, no, no, 1.0, 0.0, g2);  // Cx,occ^T (S Cw,occ)
+        }
+        // The SVD kernel reads problem q at A + q*no*no; repack into U as staging.
+        for (int sp = 0; sp < 2; ++sp) {
+            const int no = c->nocc[sp];
+            std::vector<LinDesc> cp;
+            double *stage = R;  // reuse R as staging (size 4P*n*nm >= P*no*no)
+            for (int p = 0; p < P; ++p) {
+                size_t q = (size_t)sp * P + p;
+                cp.push_back({Socc + q * nomax * nomax, nullptr, nullptr, stage + (size_t)p * no * no, 1.0, 0.0, 0.0});
+            }
+            launch_lincomb(c->st, up.put(cp), P, (long long)no * no);
+            double *Usp = U + (size_t)sp * P * nomax * nomax, *Vsp = V + (size_t)sp * P * nomax * nomax;
+            launch_svd(c->st, stage, no, P, Usp, Vsp, sig + (size_t)sp * P * nomax, phase + (size_t)sp * P,
+                       mdev + 0, tsp + (size_t)sp * P, tol);
+            // copy m out per spin (mdev has P ints)
+            std::vector<int> mh(P);
+            HIPCHK(hipMemcpyAsync(mh.data(), mdev, P * sizeof(int), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            if (sp == 0) c->pinfo.assign((size_t)nref * nref, PairInfo{});
+            for (int p = 0; p < P; ++p) {
+                PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
+                if (sp == 0) pi.ma = mh[p]; else pi.mb = mh[p];
+            }
+        }
+        std::vector<double> ph_h(2 * P), tsp_h(2 * P);
+        HIPCHK(hipMemcpyAsync(ph_h.data(), phase, 2 * P * 8, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(tsp_h.data(), tsp, 2 * P * 8, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        // ---- slab layout ------------------------------------------------------
+        long long total = 0;
+        for (int p = 0; p < P; ++p) {
+            PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
+            long long sz = pair_offsets(pi.ma, pi.mb, nm, pi.off);
+            pi.base = total;
+            pi.built = 1;
+            total += sz;
+        }
+        if ((size_t)total > c->slab_cap) {
+            if (c->d_slab) HIPCHK(hipFree(c->d_slab));
+            c->d_slab = nullptr;
+            c->slab_cap = 0;
+            c->d_slab = dalloc<double>((size_t)total);
+            c->slab_cap = (size_t)total;
+        }
+        double *slab = c->d_slab;
+        auto SL = [&](int p, long long rel) { return slab + c->pinfo[(size_t)px[p] * nref + pw[p]].base + rel; };
+        auto OFF = [&](int p) { return c->pinfo[(size_t)px[p] * nref + pw[p]].off; };
+        // ---- rotated occupied orbitals and co-densities ----------------------
+        for (int sp = 0; sp < 2; ++sp) {
+            const int no = c->nocc[sp];
+            std::vector<GemmDesc> gl, gg;
+            for (int p = 0; p < P; ++p) {
+                size_t q = (size_t)sp * P + p;
+                gl.push_back({Cref(sp, px[p]), U + (size_t)sp * P * nomax * nomax + (size_t)p * no * no, Lt + q * n * nomax});
+                gg.push_back({Cref(sp, pw[p]), V + (size_t)sp * P * nomax * nomax + (size_t)p * no * no, Gt + q * n * nomax});
+            }
+            gemm(c, up, 0, 0, n, no, no, nm, no, no, 1.0, 0.0, gl);
+            gemm(c, up, 0, 0, n, no, no, nm, no, no, 1.0, 0.0, gg);
+            // codensity per (pair) for this spin: write into M[(p*2+sp)*2 + mi]
+            // launch per pair so the output stride can interleave spins
+            for (int p = 0; p < P; ++p) {
+                size_t q = (size_t)sp * P + p;
+                launch_codensity(c->st, Lt + q * n * nomax, Gt + q * n * nomax, sig + (size_t)sp * P * nomax + (size_t)p * no,
+                                 n, no, tol, M + (size_t)prob(p, sp) * 2 * nn, 1);
+            }
+            std::vector<const double *> cxp, cwp;
+            for (int p = 0; p < P; ++p) { cxp.push_back(Cref(sp, px[p])); cwp.push_back(Cref(sp, pw[p])); }
+            // contraction orbitals into rowc/colc at prob(p, sp): launch per pair
+            const double *const *dcx = up.put(cxp);
+            const double *const *dcw = up.put(cwp);
+            for (int p = 0; p < P; ++p)
+                launch_contraction_orbitals(c->st, dcx + p, dcw + p, n, nm, no, rowc + (size_t)prob(p, sp) * nnm,
+                                            colc + (size_t)prob(p, sp) * nnm, 1);
+            for (int p = 0; p < P; ++p)
+                launch_prefill_cxxc(c->st, rowc + (size_t)prob(p, sp) * nnm, colc + (size_t)prob(p, sp) * nnm, n, nm, no,
+                                    cx + (size_t)prob(p, sp) * 2 * nnm, xc + (size_t)prob(p, sp) * 2 * nnm, 1);
+        }
+        // ---- M S, M^T S, cx, xc (build_cx_xc, build.rs:944-991) ---------------
+        {
+            std::vector<GemmDesc> gms, gmts, gcx, gxc, gq, gr, gsc, gz;
+            for (int k = 0; k < 4 * P; ++k) {
+                gms.push_back({M + (size_t)k * nn, c->d_S, MS + (size_t)k * nn});
+                gmts.push_back({M + (size_t)k * nn, c->d_S, MtS + (size_t)k * nn});
+                const int pr = k / 2;
+                gcx.push_back({MtS + (size_t)k * nn, rowc + (size_t)pr * nnm, cx + (size_t)k * nnm});
+                gxc.push_back({MS + (size_t)k * nn, colc + (size_t)pr * nnm, xc + (size_t)k * nnm});
+                gq.push_back({MS + (size_t)k * nn, colc + (size_t)pr * nnm, Q + (size_t)k * nnm});
+                gr.push_back({c->d_S, Q + (size_t)k * nnm, R + (size_t)k * nnm});
+            }
+            gemm(c, up, 0, 0, n, n, n, n, n, n, 1.0, 0.0, gms);
+            gemm(c, up, 1, 0, n, n, n, n, n, n, 1.0, 0.0, gmts);
+            gemm(c, up, 0, 0, n, nm, n, n, nm, nm, 1.0, 1.0, gcx);
+            gemm(c, up, 0, 0, n, nm, n, n, nm, nm, 1.0, 1.0, gxc);
+            gemm(c, up, 0, 0, n, nm, n, n, nm, nm, 1.0, 0.0, gq);
+            gemm(c, up, 0, 0, n, nm, n, n, nm, nm, 1.0, 0.0, gr);
+            // X[mi] = rowc^T (S M S colc) into the slab; Z = rowc^T S colc
+            std::vector<GemmDesc> gx;
+            for (int p = 0; p < P; ++p)
+                for (int sp = 0; sp < 2; ++sp) {
+                    const long long *o = OFF(p) + (sp ? O_BB : O_AA);
+                    const int pr = prob(p, sp);
+                    for (int mi = 0; mi < 2; ++mi)
+                        gx.push_back({rowc + (size_t)pr * nnm, R + (size_t)(pr * 2 + mi) * nnm, SL(p, o[O_X + mi])});
+                    gsc.push_back({c->d_S, colc + (size_t)pr * nnm, Q + (size_t)pr * nnm});  // reuse Q rows as S colc
+                }
+            gemm(c, up, 1, 0, nm, nm, n, nm, nm, nm, 1.0, 0.0, gx);
+            // Q is free now (its contents were consumed by R); compute S colc into Q[pr]
+            gemm(c, up, 0, 0, n, nm, n, n, nm, nm, 1.0, 0.0, gsc);
+            for (int p = 0; p < P; ++p)
+                for (int sp = 0; sp < 2; ++sp) {
+                    const int pr = prob(p, sp);
+                    gz.push_back({rowc + (size_t)pr * nnm, Q + (size_t)pr * nnm, Z + (size_t)pr * nm2});
+                }
+            gemm(c, up, 1, 0, nm, nm, n, nm, nm, nm, 1.0, 0.0, gz);
+            std::vector<LinDesc> ly;
+            for (int p = 0; p < P; ++p)
+                for (int sp = 0; sp < 2; ++sp) {
+                    const long long *o = OFF(p) + (sp ? O_BB : O_AA);
+                    const int pr = prob(p, sp);
+                    ly.push_back({SL(p, o[O_X]), Z + (size_t)pr * nm2, nullptr, SL(p, o[O_Y]), 1.0, -1.0, 0.0});
+                    ly.push_back({SL(p, o[O_X + 1]), nullptr, nullptr, SL(p, o[O_Y + 1]), 1.0, 0.0, 0.0});
+                    for (int k = 0; k < 4; ++k)  // ff intermediates start at zero (build.rs:216-220)
+                        ly.push_back({nullptr, nullptr, nullptr, SL(p, o[O_FF + k]), 0.0, 0.0, 0.0});
+                }
+            launch_lincomb(c->st, up.put(ly), (int)ly.size(), (long long)nm2);
+        }
+        // ---- J - K and J for every co-density (build.rs:179-191, 812-820) ----
+        {
+            std::vector<GemmDesc> g1 = {{M, c->d_easym, JK}}, g2 = {{M, c->d_eri, Jc}};
+            gemm(c, up, 0, 1, 4 * P, (int)nn, (int)nn, (int)nn, (int)nn, (int)nn, 1.0, 0.0, g1);
+            gemm(c, up, 0, 1, 4 * P, (int)nn, (int)nn, (int)nn, (int)nn, (int)nn, 1.0, 0.0, g2);
+        }
+        // ---- scalar intermediates --------------------------------------------
+        {
+            std::vector<TraceDesc> tr;
+            for (int p = 0; p < P; ++p) {
+                const PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
+                double *mt = dmeta + (size_t)p * M_LEN;
+                for (int sp = 0; sp < 2; ++sp) {
+                    double *ms = mt + (sp ? M_BB : M_AA);
+                    const int m = sp ? pi.mb : pi.ma;
+                    const int pr = prob(p, sp);
+                    const double *M0 = M + (size_t)(pr * 2) * nn, *M1 = M + (size_t)(pr * 2 + 1) * nn;
+                    const double *JK0 = JK + (size_t)(pr * 2) * nn, *JK1 = JK + (size_t)(pr * 2 + 1) * nn;
+                    tr.push_back({M0, c->d_h, ms + M_F0H, 1.0});
+                    tr.push_back({M1, c->d_h, ms + M_F0H + 1, 1.0});
+                    tr.push_back({JK0, M0, ms + M_V0, 1.0});
+                    if (m >= 1) tr.push_back({JK0, M1, ms + M_V0 + 1, 2.0});
+                    if (m >= 2) tr.push_back({JK1, M1, ms + M_V0 + 2, 1.0});
+                }
+                const int pa = prob(p, 0), pb = prob(p, 1);
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j) {
+                        tr.push_back({Jc + (size_t)(pa * 2 + i) * nn, M + (size_t)(pb * 2 + j) * nn, mt + M_VAB0 + i * 2 + j, 1.0});
+                        tr.push_back({Jc + (size_t)(pb * 2 + j) * nn, M + (size_t)(pa * 2 + i) * nn, mt + M_VBA0 + j * 2 + i, 1.0});
+                    }
+            }
+            launch_einsum(c->st, up.put(tr), (int)tr.size(), n);
+        }
+        // ---- one-column intermediates (transposed, write2t layout.rs:339) ----
+        {
+            std::vector<GemmDesc> gh, gfh, gjx, gv, gva, gvb;
+            for (int k = 0; k < 4 * P; ++k) gh.push_back({c->d_h, xc + (size_t)k * nnm, hxc + (size_t)k * nnm});
+            gemm(c, up, 0, 0, n, nm, n, n, nm, nm, 1.0, 0.0, gh);
+            for (int p = 0; p < P; ++p)
+                for (int sp = 0; sp < 2; ++sp) {
+                    const long long *o = OFF(p) + (sp ? O_BB : O_AA);
+                    const int pr = prob(p, sp);
+                    for (int mi = 0; mi < 2; ++mi)
+                        for (int mj = 0; mj < 2; ++mj)
+                            gfh.push_back({hxc + (size_t)(pr * 2 + mj) * nnm, cx + (size_t)(pr * 2 + mi) * nnm,
+                                           SL(p, o[O_FH + mi * 2 + mj])});
+                    for (int mk = 0; mk < 2; ++mk)
+                        for (int mj = 0; mj < 2; ++mj)
+                            gjx.push_back({JK + (size_t)(pr * 2 + mk) * nn, xc + (size_t)(pr * 2 + mj) * nnm,
+                                           JKxc + (size_t)((pr * 2 + mk) * 2 + mj) * nnm});
+                    for (int mi = 0; mi < 2; ++mi)
+                        for (int mj = 0; mj < 2; ++mj)
+                            for (int mk = 0; mk < 2; ++mk)
+                                gv.push_back({JKxc + (size_t)((pr * 2 + mk) * 2 + mj) * nnm, cx + (size_t)(pr * 2 + mi) * nnm,
+                                              SL(p, o[O_V + (mi * 2 + mj) * 2 + mk])});
+                }
+            gemm(c, up, 1, 0, nm, nm, n, nm, nm, nm, 1.0, 0.0, gfh);
+            gemm(c, up, 0, 0, n, nm, n, n, nm, nm, 1.0, 0.0, gjx);
+            gemm(c, up, 1, 0, nm, nm, n, nm, nm, nm, 1.0, 0.0, gv);
+            // different-spin: reuse JKxc as Jc xc staging
+            std::vector<GemmDesc> gja;
+            for (int p = 0; p < P; ++p) {
+                const int pa = prob(p, 0), pb = prob(p, 1);
+                for (int b0 = 0; b0 < 2; ++b0)
+                    for (int k = 0; k < 2; ++k) {
+                        // Jc_b[b0] xc_a[k]  -> slot ((pa*2+b0)*2+k)
+                        gja.push_back({Jc + (size_t)(pb * 2 + b0) * nn, xc + (size_t)(pa * 2 + k) * nnm,
+                                       JKxc + (size_t)((pa * 2 + b0) * 2 + k) * nnm});
+                        // Jc_a[b0] xc_b[k]  -> slot ((pb*2+b0)*2+k)
+                        gja.push_back({Jc + (size_t)(pa * 2 + b0) * nn, xc + (size_t)(pb * 2 + k) * nnm,
+                                       JKxc + (size_t)((pb * 2 + b0) * 2 + k) * nnm});
+                    }
+            }
+            gemm(c, up, 0, 0, n, nm, n, n, nm, nm, 1.0, 0.0, gja);
+            for (int p = 0; p < P; ++p) {
+                const int pa = prob(p, 0), pb = prob(p, 1);
+                const long long *o = OFF(p);
+                for (int a0 = 0; a0 < 2; ++a0)
+                    for (int b0 = 0; b0 < 2; ++b0)
+                        for (int k = 0; k < 2; ++k) {
+                            // vab[a0][b0][k] = cx_a[a0]^T Jb[b0] xc_a[k]
+                            gva.push_back({JKxc + (size_t)((pa * 2 + b0) * 2 + k) * nnm, cx + (size_t)(pa * 2 + a0) * nnm,
+                                           SL(p, o[O_VAB + (a0 * 2 + b0) * 2 + k])});
+                            // vba[b0][a0][k] = cx_b[b0]^T Ja[a0] xc_b[k]
+                            gvb.push_back({JKxc + (size_t)((pb * 2 + a0) * 2 + k) * nnm, cx + (size_t)(pb * 2 + b0) * nnm,
+                                           SL(p, o[O_VBA + (b0 * 2 + a0) * 2 + k])});
+                        }
+            }
+            gemm(c, up, 1, 0, nm, nm, n, nm, nm, nm, 1.0, 0.0, gva);
+            gemm(c, up, 1, 0, nm, nm, n, nm, nm, nm, 1.0, 0.0, gvb);
+            std::vector<LinDesc> lh;
+            for (int p = 0; p < P; ++p) {
+                const long long *o = OFF(p);
+                if (o[O_AA + O_HCOL0] < 0) continue;
+                lh.push_back({SL(p, o[O_AA + O_FH]), SL(p, o[O_AA + O_V]), SL(p, o[O_VAB]), SL(p, o[O_AA + O_HCOL0]), 1, 1, 1});
+                lh.push_back({SL(p, o[O_BB + O_FH]), SL(p, o[O_BB + O_V]), SL(p, o[O_VBA]), SL(p, o[O_BB + O_HCOL0]), 1, 1, 1});
+            }
+            launch_lincomb(c->st, up.put(lh), (int)lh.size(), (long long)nm2);
+        }
+        HIPCHK(hipGetLastError());
+        // ---- four-index transforms (build.rs:261-313, 877-917) ----------------
+        HIPCHK(hipEventRecord(c->ev[1], c->st));
+        {
+            const size_t big = std::max((size_t)n * n * n * n, (size_t)nm * n * n * n);
+            double *TA = c->scratchA.get<double>(big), *TB = c->scratchB.get<double>(big);
+            auto transform = [&](const double *E, const double *c1, const double *c2, const double *c3, const double *c4,
+                                 double *out) {
+                launch_qtransform(c->st, E, c4, TA, (long long)n * n * n, n, nm);
+                launch_qtransform(c->st, TA, c3, TB, (long long)nm * n * n, n, nm);
+                launch_qtransform(c->st, TB, c2, TA, (long long)nm * nm * n, n, nm);
+                launch_qtransform(c->st, TA, c1, out, (long long)nm * nm * nm, n, nm);
+            };
+            for (int p = 0; p < P; ++p) {
+                const PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
+                for (int sp = 0; sp < 2; ++sp) {
+                    const long long *o = pi.off + (sp ? O_BB : O_AA);
+                    const int pr = prob(p, sp);
+                    const double *cxs[2] = {cx + (size_t)(pr * 2) * nnm, cx + (size_t)(pr * 2 + 1) * nnm};
+                    const double *xcs[2] = {xc + (size_t)(pr * 2) * nnm, xc + (size_t)(pr * 2 + 1) * nnm};
+                    for (int s = 0; s < 10; ++s) {
+                        if (o[O_J + s] < 0) continue;
+                        const int i = jbranch(s, 0), j = jbranch(s, 1), k = jbranch(s, 2), l = jbranch(s, 3);
+                        // J slab block [a,b,c,d] = blk[c,d,a,b] with blk = (cx_i xc_j || cx_k xc_l);
+                        // the pair symmetry of (ab||cd) lets one transform produce it directly.
+                        transform(c->d_easym, cxs[k], xcs[l], cxs[i], xcs[j], slab + pi.base + o[O_J + s]);
+                    }
+                }
+                const int pa = prob(p, 0), pb = prob(p, 1);
+                for (int a0 = 0; a0 < 2; ++a0)
+                    for (int aj = 0; aj < 2; ++aj)
+                        for (int b0 = 0; b0 < 2; ++b0)
+                            for (int bj = 0; bj < 2; ++bj) {
+                                long long oo = pi.off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj];
+                                if (oo < 0) continue;
+                                transform(c->d_eri, cx + (size_t)(pa * 2 + a0) * nnm, xc + (size_t)(pa * 2 + aj) * nnm,
+                                          cx + (size_t)(pb * 2 + b0) * nnm, xc + (size_t)(pb * 2 + bj) * nnm,
+                                          slab + pi.base + oo);
+                            }
+            }
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[2], c->st));
+        // ---- meta to host, pair table to device ------------------------------
+        std::vector<double> mh((size_t)P * M_LEN);
+        HIPCHK(hipMemcpyAsync(mh.data(), dmeta, mh.size() * 8, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (int p = 0; p < P; ++p) {
+            PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
+            for (int k = 0; k < M_LEN; ++k) pi.meta[k] = mh[(size_t)p * M_LEN + k];
+            for (int sp = 0; sp < 2; ++sp) {
+                double *ms = pi.meta + (sp ? M_BB : M_AA);
+                ms[M_TSP] = tsp_h[(size_t)sp * P + p];
+                ms[M_PH] = ph_h[(size_t)sp * P + p];
+                ms[M_M] = sp ? pi.mb : pi.ma;
+                ms[M_NOCC] = c->nocc[sp];
+            }
+            if (meta_out) {
+                nw_pair_meta &o = meta_out[(size_t)px[p] * nref + pw[p]];
+                o.ma = pi.ma; o.mb = pi.mb;
+                o.tilde_s_a = pi.meta[M_AA + M_TSP]; o.phase_a = pi.meta[M_AA + M_PH];
+                o.tilde_s_b = pi.meta[M_BB + M_TSP]; o.phase_b = pi.meta[M_BB + M_PH];
+            }
+        }
+        if (c->d_pinfo) HIPCHK(hipFree(c->d_pinfo));
+        c->d_pinfo = nullptr;
+        c->d_pinfo = (PairInfo *)dalloc<char>(sizeof(PairInfo) * c->pinfo.size());
+        HIPCHK(hipMemcpyAsync(c->d_pinfo, c->pinfo.data(), sizeof(PairInfo) * c->pinfo.size(), hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
+        c->t_ms[1] = elapsed(c->ev[1], c->ev[2]);
+        c->t_ms[2] = 0.0;
+        c->built = true;
+        return NW_OK;
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+struct Run { long long start, len; int parent, ra, rb; };
+
+// Pack determinants on the device and return the host copy of the packed records.
+std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked)
+{
+    const void *draw = to_device(c, c->dets_raw, dets, ndet * sizeof(nw_det));
+    dpacked = c->dets_packed.get<PackedDet>(ndet);
+    int *dflag = c->flag.get<int>(1);
+    HIPCHK(hipMemsetAsync(dflag, 0, sizeof(int), c->st));
+    launch_pack_dets(c->st, draw, (long long)ndet, dpacked, dflag);
+    std::vector<PackedDet> h(ndet);
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(h.data(), dpacked, ndet * sizeof(PackedDet), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (flag) throw NwError(NW_E_RANK, "excitation rank per spin must be <= 4 with equal holes and particles");
+    for (uint64_t i = 0; i < ndet; ++i)
+        if ((int)h[i].parent >= c->nref) throw NwError(NW_E_ARG, "determinant parent out of range");
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nw_hs_matrix(nw_ctx *c, const nw_det *dets, uint64_t ndet, double *H, double *S, int mode)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->built) return fail(c, NW_E_STATE, "nw_build first");
+        if (!dets || !H || !S || ndet == 0) return fail(c, NW_E_ARG, "null arguments");
+        if (ndet > (1ull << 31)) return fail(c, NW_E_SHAPE, "too many determinants");
+        HIPCHK(hipEventRecord(c->ev[0], c->st));
+        PackedDet *dp = nullptr;
+        std::vector<PackedDet> h = pack(c, dets, ndet, dp);
+        // runs of equal (parent, rank_a, rank_b)
+        std::vector<Run> runs;
+        for (long long i = 0; i < (long long)ndet; ++i) {
+            const PackedDet &d = h[i];
+            if (!runs.empty() && runs.back().parent == d.parent && runs.back().ra == d.ra && runs.back().rb == d.rb)
+                runs.back().len++;
+            else
+                runs.push_back({i, 1, d.parent, d.ra, d.rb});
+        }
+        std::map<int, std::vector<EvalTile>> bins;  // key = (la*5+lb)*2 + m0
+        for (size_t r1 = 0; r1 < runs.size(); ++r1)
+            for (size_t r2 = r1; r2 < runs.size(); ++r2) {
+                const Run &A = runs[r1], &B = runs[r2];
+                const int pair = A.parent * c->nref + B.parent;
+                const PairInfo &pi = c->pinfo[pair];
+                if (!pi.built) throw NwError(NW_E_STATE, "reference pair (" + std::to_string(A.parent) + "," +
+                                                             std::to_string(B.parent) + ") was not built");
+                const int la = A.ra + B.ra, lb = A.rb + B.rb;
+                const bool m0 = pi.ma == 0 && pi.mb == 0;
+                auto &v = bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)];
+                for (long long i0 = A.start; i0 < A.start + A.len; i0 += 16) {
+                    const int ni = (int)std::min<long long>(16, A.start + A.len - i0);
+                    for (long long j0 = B.start; j0 < B.start + B.len; j0 += 16) {
+                        const int nj = (int)std::min<long long>(16, B.start + B.len - j0);
+                        if (r1 == r2 && j0 + nj - 1 < i0) continue;
+                        v.push_back({(int)i0, (int)j0, ni, nj, pair, r1 == r2 ? 1 : 0});
+                    }
+                }
+            }
+        const bool hdev = is_device_ptr(H), sdev = is_device_ptr(S);
+        double *dH = hdev ? H : c->outH.get<double>(ndet * ndet);
+        double *dS = sdev ? S : c->outS.get<double>(ndet * ndet);
+        Uploader up(c, 64u << 20);
+        HIPCHK(hipEventRecord(c->ev[1], c->st));
+        for (auto &kv : bins) {
+            const int key = kv.first;
+            const bool m0 = key & 1;
+            const int lab = key >> 1, la = lab / 5, lb = lab % 5;
+            EvalTile *dt = up.put(kv.second);
+            launch_eval_tiles(c->st, la, lb, m0, mode, dt, (int)kv.second.size(), dp, c->d_pinfo, c->d_slab, nullptr,
+                              c->nmo, c->enuc, c->tol, dH, dS, (long long)ndet);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[2], c->st));
+        if (!hdev) HIPCHK(hipMemcpyAsync(H, dH, ndet * ndet * 8, hipMemcpyDeviceToHost, c->st));
+        if (!sdev) HIPCHK(hipMemcpyAsync(S, dS, ndet * ndet * 8, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
+        c->t_ms[1] = 0.0;
+        c->t_ms[2] = elapsed(c->ev[1], c->ev[2]);
+        return NW_OK;
+    });
+}
+
+int nw_hs_batched(nw_ctx *c, const nw_det *dets, uint64_t ndet, const uint64_t *req, uint64_t nreq, double *h_out,
+                  double *s_out, int mode)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->built) return fail(c, NW_E_STATE, "nw_build first");
+        if (!dets || !req || !h_out || !s_out) return fail(c, NW_E_ARG, "null arguments");
+        if (nreq == 0) return NW_OK;
+        if (nreq > (1ull << 31)) return fail(c, NW_E_SHAPE, "too many requests");
+        HIPCHK(hipEventRecord(c->ev[0], c->st));
+        PackedDet *dp = nullptr;
+        std::vector<PackedDet> h = pack(c, dets, ndet, dp);
+        std::vector<uint64_t> rq(nreq * 2);
+        if (is_device_ptr(req)) {
+            HIPCHK(hipMemcpyAsync(rq.data(), req, nreq * 16, hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        } else {
+            std::memcpy(rq.data(), req, nreq * 16);
+        }
+        // Host copies of the full occupations for the same-parent shortcut.
+        std::vector<nw_det> hd;
+        const nw_det *hdp = dets;
+        if (is_device_ptr(dets)) {
+            hd.resize(ndet);
+            HIPCHK(hipMemcpyAsync(hd.data(), dets, ndet * sizeof(nw_det), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            hdp = hd.data();
+        }
+        // calculate_hs_pairs_wicks_batched (src/noci/hs.rs:86-109): same-parent pairs
+        // differing in more than four spin orbitals are exactly zero; the rest are
+        // grouped by contraction-rank bin and reference-pair zero class.
+        std::map<int, std::vector<EvalReq>> bins;
+        std::vector<long long> zeros;
+        for (uint64_t q = 0; q < nreq; ++q) {
+            const uint64_t a = rq[2 * q], b = rq[2 * q + 1];
+            if (a >= ndet || b >= ndet) throw NwError(NW_E_ARG, "request index out of range");
+            const nw_det &A = hdp[a], &B = hdp[b];
+            if (A.parent == B.parent) {
+                int diff = __builtin_popcountll(A.oa[0] ^ B.oa[0]) + __builtin_popcountll(A.oa[1] ^ B.oa[1]) +
+                           __builtin_popcountll(A.ob[0] ^ B.ob[0]) + __builtin_popcountll(A.ob[1] ^ B.ob[1]);
+                if (diff > 4) { zeros.push_back((long long)q); continue; }
+            }
+            const int pair = (int)A.parent * c->nref + (int)B.parent;
+            const PairInfo &pi = c->pinfo[pair];
+            if (!pi.built) throw NwError(NW_E_STATE, "reference pair not built");
+            const int la = h[a].ra + h[b].ra, lb = h[a].rb + h[b].rb;
+            const bool m0 = pi.ma == 0 && pi.mb == 0;
+            bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)].push_back({(int)a, (int)b, (long long)q});
+        }
+        const bool hdev = is_device_ptr(h_out), sdev = is_device_ptr(s_out);
+        double *dH = hdev ? h_out : c->outH.get<double>(nreq);
+        double *dS = sdev ? s_out : c->outS.get<double>(nreq);
+        Uploader up(c, 64u << 20 > nreq * 40 ? 64u << 20 : nreq * 40 + (1u << 20));
+        if (!zeros.empty()) {
+            // zero outputs through the linear-combination kernel on single elements
+            std::vector<LinDesc> lz;
+            for (long long q : zeros) {
+                lz.push_back({nullptr, nullptr, nullptr, dH + q, 0, 0, 0});
+                lz.push_back({nullptr, nullptr, nullptr, dS + q, 0, 0, 0});
+            }
+            launch_lincomb(c->st, up.put(lz), (int)lz.size(), 1);
+        }
+        HIPCHK(hipEventRecord(c->ev[1], c->st));
+        for (auto &kv : bins) {
+            const int key = kv.first;
+            const bool m0 = key & 1;
+            const int lab = key >> 1, la = lab / 5, lb = lab % 5;
+            EvalReq *dr = up.put(kv.second);
+            launch_eval_reqs(c->st, la, lb, m0, mode, dr, (int)kv.second.size(), dp, c->d_pinfo, c->nref, c->d_slab,
+                             nullptr, c->nmo, c->enuc, c->tol, dH, dS);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[2], c->st));
+        if (!hdev) HIPCHK(hipMemcpyAsync(h_out, dH, nreq * 8, hipMemcpyDeviceToHost, c->st));
+        if (!sdev) HIPCHK(hipMemcpyAsync(s_out, dS, nreq * 8, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
+        c->t_ms[1] = 0.0;
+        c->t_ms[2] = elapsed(c->ev[1], c->ev[2]);
+        return NW_OK;
+    });
+}
+
+int nw_pair_slab(nw_ctx *c, int x, int w, double *out, int64_t *offsets, double *meta)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->built) return fail(c, NW_E_STATE, "nw_build first");
+        if (x < 0 || w < 0 || x >= c->nref || w >= c->nref) return fail(c, NW_E_ARG, "pair out of range");
+        const PairInfo &pi = c->pinfo[(size_t)x * c->nref + w];
+        if (!pi.built) return fail(c, NW_E_STATE, "pair not built");
+        if (offsets)
+            for (int k = 0; k < O_LEN; ++k) offsets[k] = pi.off[k];
+        if (meta)
+            for (int k = 0; k < M_LEN; ++k) meta[k] = pi.meta[k];
+        if (out) {
+            HIPCHK(hipMemcpyAsync(out, c->d_slab + pi.base, (size_t)pi.off[O_TOTAL] * 8, hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
+        return NW_OK;
+    });
+}
+
+int nw_last_timings(nw_ctx *c, double *ms3)
+{
+    if (!c || !ms3) return NW_E_ARG;
+    for (int k = 0; k < 3; ++k) ms3[k] = c->t_ms[k];
+    return NW_OK;
+}
+
+const char *nw_last_error(nw_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+void nw_destroy(nw_ctx *c) { delete c; }
+
+}  // extern "C"

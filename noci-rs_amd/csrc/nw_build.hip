@@ -1,0 +1,439 @@
+// Per-reference-pair build kernels (FP64): occupied-overlap SVD, co-densities,
+// batched small GEMMs and the four-index AO->MO transforms.
+//
+// The algorithm follows src/nonorthogonalwicks/build.rs; the decomposition
+// into GPU launches is this engine's own: every step runs batched over all
+// requested ordered reference pairs and both spins at once.
+#include "nw_kernels.h"
+
+namespace nw {
+
+// ---------------------------------------------------------------------------
+// Batched FP64 GEMM, C = alpha op(A) op(B) + beta C, row-major operands.
+// op(A) is M x K: TA=0 reads A[i*lda+k], TA=1 reads A[k*lda+i].
+// op(B) is K x N: TB=0 reads B[k*ldb+j], TB=1 reads B[j*ldb+k].
+// 64x64 tile per 256-thread workgroup, 16-deep K steps staged in LDS, each
+// thread accumulating a 4x4 strided sub-tile.
+// ---------------------------------------------------------------------------
+template <int TA, int TB>
+__device__ inline void gemm_tile(const GemmDesc g, int M, int N, int K, int lda, int ldb, int ldc, double alpha,
+                                 double beta)
+{
+    constexpr int BM = 64, BN = 64, BK = 16;
+    __shared__ double As[BK][BM + 1];
+    __shared__ double Bs[BK][BN + 1];
+    const int tid = threadIdx.x;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int tx = tid & 15, ty = tid >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+
+    for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+        for (int e4 = 0; e4 < 4; ++e4) {
+            int e = tid + e4 * 256;
+            int i, k;
+            if (TA == 0) { i = e / BK; k = e % BK; } else { k = e / BM; i = e % BM; }
+            int gi = m0 + i, gk = k0 + k;
+            double v = 0.0;
+            if (gi < M && gk < K) v = TA == 0 ? g.A[(size_t)gi * lda + gk] : g.A[(size_t)gk * lda + gi];
+            As[k][i] = v;
+            int j;
+            if (TB == 0) { k = e / BN; j = e % BN; } else { j = e / BK; k = e % BK; }
+            int gj = n0 + j;
+            gk = k0 + k;
+            double w = 0.0;
+            if (gj < N && gk < K) w = TB == 0 ? g.B[(size_t)gk * ldb + gj] : g.B[(size_t)gj * ldb + gk];
+            Bs[k][j] = w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < BK; ++k) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[k][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[k][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int gi = m0 + ty + 16 * i;
+        if (gi >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int gj = n0 + tx + 16 * j;
+            if (gj >= N) continue;
+            double *c = g.C + (size_t)gi * ldc + gj;
+            double v = alpha * acc[i][j];
+            if (beta != 0.0) v += beta * *c;
+            *c = v;
+        }
+    }
+}
+
+template <int TA, int TB>
+__global__ __launch_bounds__(256) void k_gemm(const GemmDesc *__restrict__ descs, int M, int N, int K,
+                                               int lda, int ldb, int ldc, double alpha, double beta)
+{
+    gemm_tile<TA, TB>(descs[blockIdx.z], M, N, K, lda, ldb, ldc, alpha, beta);
+}
+
+// Rotating quarter transform of the four-index AO->MO transformation
+// (eri_ao2mo_into, src/maths/eri.rs:258-313): Y[l', m] = sum_l c[l, l'] X[m, l].
+// The contracted index is the fastest of X and the new index becomes the
+// slowest of Y, so four passes return the original index order.
+__global__ __launch_bounds__(256) void k_qtransform(const double *__restrict__ X, const double *__restrict__ c,
+                                                     double *__restrict__ Y, int rows, int nl, int nout)
+{
+    GemmDesc g;
+    g.A = c;
+    g.B = X;
+    g.C = Y;
+    gemm_tile<1, 1>(g, nout, rows, nl, nout, nl, rows, 1.0, 0.0);
+}
+
+void launch_qtransform(hipStream_t st, const double *X, const double *c, double *Y, long long rows, int nl, int nout)
+{
+    dim3 grid((unsigned)((rows + 63) / 64), (nout + 63) / 64, 1);
+    hipLaunchKernelGGL(k_qtransform, grid, dim3(256), 0, st, X, c, Y, (int)rows, nl, nout);
+}
+
+void launch_gemm(hipStream_t st, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc,
+                 double alpha, double beta, const GemmDesc *d_descs, int nbatch)
+{
+    if (nbatch <= 0 || M <= 0 || N <= 0) return;
+    int nz = nbatch;
+    while (nz > 0) {
+        int chunk = nz > 65535 ? 65535 : nz;
+        dim3 grid((N + 63) / 64, (M + 63) / 64, chunk);
+        const GemmDesc *d = d_descs + (nbatch - nz);
+        if (ta == 0 && tb == 0) hipLaunchKernelGGL((k_gemm<0, 0>), grid, dim3(256), 0, st, d, M, N, K, lda, ldb, ldc, alpha, beta);
+        else if (ta == 0 && tb == 1) hipLaunchKernelGGL((k_gemm<0, 1>), grid, dim3(256), 0, st, d, M, N, K, lda, ldb, ldc, alpha, beta);
+        else if (ta == 1 && tb == 0) hipLaunchKernelGGL((k_gemm<1, 0>), grid, dim3(256), 0, st, d, M, N, K, lda, ldb, ldc, alpha, beta);
+        else hipLaunchKernelGGL((k_gemm<1, 1>), grid, dim3(256), 0, st, d, M, N, K, lda, ldb, ldc, alpha, beta);
+        nz -= chunk;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One-sided Jacobi SVD of one n x n occupied-overlap matrix per 64-lane
+// workgroup (n <= 64), FP64 throughout.  Produces singular values in
+// descending order, U (left vectors; zero-singular-value columns completed
+// orthonormally), V, phase = det(U) det(V), m = #{sigma <= tol} and the
+// product of the nonzero singular values (build.rs:123-142, 401-424).
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return __shfl(v, 0, 64);
+}
+
+__global__ __launch_bounds__(64) void k_svd(const double *__restrict__ A, int n, double *__restrict__ U,
+                                             double *__restrict__ V, double *__restrict__ sig,
+                                             double *__restrict__ phase, int *__restrict__ mout,
+                                             double *__restrict__ tsp, double tol)
+{
+    __shared__ double W[MAX_NOCC * MAX_NOCC];   // column-major: W[c*64 + r]
+    __shared__ double Vs[MAX_NOCC * MAX_NOCC];
+    __shared__ double Us[MAX_NOCC * MAX_NOCC];
+    __shared__ double sv[MAX_NOCC];
+    __shared__ int ord[MAX_NOCC];
+    __shared__ int piv_s;
+    const int prob = blockIdx.x;
+    const int r = threadIdx.x;
+    const double *a = A + (size_t)prob * n * n;
+    for (int c = 0; c < n; ++c) {
+        W[c * 64 + r] = (r < n) ? a[r * n + c] : 0.0;
+        Vs[c * 64 + r] = (r == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 80; ++sweep) {
+        int rotated = 0;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double wp = W[p * 64 + r], wq = W[q * 64 + r];
+                double al = wave_sum(wp * wp), be = wave_sum(wq * wq), ga = wave_sum(wp * wq);
+                if (ga == 0.0 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
+                rotated = 1;
+                double zeta = (be - al) / (2.0 * ga);
+                double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                W[p * 64 + r] = c * wp - s * wq;
+                W[q * 64 + r] = s * wp + c * wq;
+                double vp = Vs[p * 64 + r], vq = Vs[q * 64 + r];
+                Vs[p * 64 + r] = c * vp - s * vq;
+                Vs[q * 64 + r] = s * vp + c * vq;
+                __syncthreads();
+            }
+        if (!rotated) break;
+    }
+    __syncthreads();
+    if (r < n) {
+        double acc = 0.0;
+        for (int i = 0; i < n; ++i) acc += W[r * 64 + i] * W[r * 64 + i];
+        sv[r] = sqrt(acc);
+    }
+    __syncthreads();
+    if (r == 0) {
+        for (int k = 0; k < n; ++k) ord[k] = k;
+        for (int x = 0; x < n; ++x)
+            for (int y = x + 1; y < n; ++y)
+                if (sv[ord[y]] > sv[ord[x]]) { int t = ord[x]; ord[x] = ord[y]; ord[y] = t; }
+    }
+    __syncthreads();
+    const double smax = n > 0 ? sv[ord[0]] : 0.0;
+    // U columns (normalised W columns, or completion for numerically zero ones)
+    for (int k = 0; k < n; ++k) {
+        int src = ord[k];
+        double s = sv[src];
+        bool good = s > 1e-13 * (smax > 0 ? smax : 1.0);
+        if (good) {
+            Us[k * 64 + r] = (r < n) ? W[src * 64 + r] / s : 0.0;
+        } else {
+            for (int e = 0; e < n; ++e) {
+                double cand = (r == e) ? 1.0 : 0.0;
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int j = 0; j < k; ++j) {
+                        double d = wave_sum(Us[j * 64 + r] * cand);
+                        cand -= d * Us[j * 64 + r];
+                    }
+                double nn = wave_sum(cand * cand);
+                if (nn > 1e-2) {
+                    Us[k * 64 + r] = cand / sqrt(nn);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // outputs: row-major U, V (n x n), sigma
+    double *u = U + (size_t)prob * n * n, *v = V + (size_t)prob * n * n;
+    if (r < n) {
+        for (int k = 0; k < n; ++k) {
+            u[r * n + k] = Us[k * 64 + r];
+            v[r * n + k] = Vs[ord[k] * 64 + r];
+        }
+        sig[(size_t)prob * n + r] = sv[ord[r]];
+    }
+    __syncthreads();
+    // det(U) and det(V) by partial-pivot LU; lane r owns row r (W reused for U, Vs permuted into W? use Us/W)
+    double dets[2];
+    for (int which = 0; which < 2; ++which) {
+        // copy into W as row-major-by-lane: W[c*64 + r] = M[r][c]
+        for (int c = 0; c < n; ++c) W[c * 64 + r] = (r < n) ? (which == 0 ? Us[c * 64 + r] : Vs[ord[c] * 64 + r]) : 0.0;
+        __syncthreads();
+        double sign = 1.0, d = 1.0;
+        for (int k = 0; k < n; ++k) {
+            if (r == 0) {
+                int p = k;
+                double pa = fabs(W[k * 64 + k]);
+                for (int i = k + 1; i < n; ++i)
+                    if (fabs(W[k * 64 + i]) > pa) { pa = fabs(W[k * 64 + i]); p = i; }
+                piv_s = p;
+            }
+            __syncthreads();
+            int p = piv_s;
+            if (p != k) {
+                sign = -sign;
+                if (r < n) {
+                    // swap rows k and p across columns (lane r handles column r)
+                    double t = W[r * 64 + k];
+                    W[r * 64 + k] = W[r * 64 + p];
+                    W[r * 64 + p] = t;
+                }
+            }
+            __syncthreads();
+            double pivot = W[k * 64 + k];
+            d *= pivot;
+            if (pivot != 0.0 && r > k && r < n) {
+                double f = W[k * 64 + r] / pivot;
+                for (int c = k + 1; c < n; ++c) W[c * 64 + r] -= f * W[c * 64 + k];
+            }
+            __syncthreads();
+        }
+        dets[which] = sign * d;
+        __syncthreads();
+    }
+    if (r == 0) {
+        phase[prob] = dets[0] * dets[1];
+        int m = 0;
+        double prod = 1.0;
+        for (int k = 0; k < n; ++k) {
+            double s = sv[ord[k]];
+            if (fabs(s) > tol) prod *= s; else ++m;
+        }
+        mout[prob] = m;
+        tsp[prob] = prod;
+    }
+}
+
+void launch_svd(hipStream_t st, const double *A, int n, int nprob, double *U, double *V, double *sig,
+                double *phase, int *m, double *tsp, double tol)
+{
+    if (nprob <= 0) return;
+    hipLaunchKernelGGL(k_svd, dim3(nprob), dim3(64), 0, st, A, n, U, V, sig, phase, m, tsp, tol);
+}
+
+// ---------------------------------------------------------------------------
+// Co-densities M0, M1 (construct_m, build.rs:447-503), one problem per z.
+// lt, gt: n x no rotated occupied orbitals (row-major, ld = no).
+// ---------------------------------------------------------------------------
+__global__ void k_codensity(const double *__restrict__ lt, const double *__restrict__ gt,
+                            const double *__restrict__ sig, int n, int no, double tol,
+                            double *__restrict__ M /* per prob: [2][n][n] */)
+{
+    const int prob = blockIdx.z;
+    const int mu = blockIdx.y, nu = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nu >= n) return;
+    const double *L = lt + (size_t)prob * n * no, *G = gt + (size_t)prob * n * no;
+    const double *s = sig + (size_t)prob * no;
+    double m0 = 0.0, m1 = 0.0, zz = 0.0;
+    for (int k = 0; k < no; ++k) {
+        double g = G[mu * no + k], l = L[nu * no + k];
+        if (fabs(s[k]) > tol) m0 += (g * (1.0 / s[k])) * l;
+    }
+    for (int k = 0; k < no; ++k) {
+        if (fabs(s[k]) > tol) continue;
+        double g = G[mu * no + k];
+        zz += g * G[nu * no + k];
+        m1 += g * L[nu * no + k];
+    }
+    double *out = M + (size_t)prob * 2 * n * n;
+    out[mu * n + nu] = m0 + zz + m1;
+    out[n * n + mu * n + nu] = m1;
+}
+
+void launch_codensity(hipStream_t st, const double *lt, const double *gt, const double *sig, int n, int no,
+                      double tol, double *M, int nprob)
+{
+    if (nprob <= 0) return;
+    dim3 grid((n + 63) / 64, n, nprob);
+    hipLaunchKernelGGL(k_codensity, grid, dim3(64), 0, st, lt, gt, sig, n, no, tol, M);
+}
+
+// rowc = [Cx[:, no:] | Cw[:, :no]], colc = [Cx[:, :no] | Cw[:, no:]]
+// (contraction_orbitals, build.rs:1008-1031) and the masked copies used for
+// the (1 - m_i) subtractions of build_cx_xc (build.rs:944-991).
+__global__ void k_contraction_orbitals(const double *const *__restrict__ cx_ptrs, const double *const *__restrict__ cw_ptrs,
+                                       int n, int nm, int no, double *__restrict__ rowc, double *__restrict__ colc)
+{
+    const int prob = blockIdx.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * nm) return;
+    const int mu = e / nm, j = e % nm;
+    const double *Cx = cx_ptrs[prob], *Cw = cw_ptrs[prob];
+    const int nv = nm - no;
+    rowc[(size_t)prob * n * nm + e] = (j < nv) ? Cx[mu * nm + no + j] : Cw[mu * nm + (j - nv)];
+    colc[(size_t)prob * n * nm + e] = (j < no) ? Cx[mu * nm + j] : Cw[mu * nm + j];
+}
+
+void launch_contraction_orbitals(hipStream_t st, const double *const *cx, const double *const *cw, int n, int nm,
+                                 int no, double *rowc, double *colc, int nprob)
+{
+    if (nprob <= 0) return;
+    dim3 grid((n * nm + 255) / 256, nprob);
+    hipLaunchKernelGGL(k_contraction_orbitals, grid, dim3(256), 0, st, cx, cw, n, nm, no, rowc, colc);
+}
+
+// cx[mi] prefill = -(1-mi) rowc on columns < nv, 0 elsewhere;
+// xc[mi] prefill = -(1-mi) colc on columns >= no, 0 elsewhere.
+__global__ void k_prefill_cxxc(const double *__restrict__ rowc, const double *__restrict__ colc, int n, int nm, int no,
+                               double *__restrict__ cx, double *__restrict__ xc)
+{
+    const int prob = blockIdx.y;  // (pair, spin)
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * nm) return;
+    const int j = e % nm;
+    const int nv = nm - no;
+    const size_t nn = (size_t)n * nm;
+    double rc = rowc[prob * nn + e], cc = colc[prob * nn + e];
+    cx[(prob * 2 + 0) * nn + e] = (j < nv) ? -rc : 0.0;
+    cx[(prob * 2 + 1) * nn + e] = 0.0;
+    xc[(prob * 2 + 0) * nn + e] = (j >= no) ? -cc : 0.0;
+    xc[(prob * 2 + 1) * nn + e] = 0.0;
+}
+
+void launch_prefill_cxxc(hipStream_t st, const double *rowc, const double *colc, int n, int nm, int no, double *cx,
+                         double *xc, int nprob)
+{
+    if (nprob <= 0) return;
+    dim3 grid((n * nm + 255) / 256, nprob);
+    hipLaunchKernelGGL(k_prefill_cxxc, grid, dim3(256), 0, st, rowc, colc, n, nm, no, cx, xc);
+}
+
+// out = ca*a + cb*b + cc*c elementwise (null operands skipped), batch over y.
+__global__ void k_lincomb(const LinDesc *__restrict__ d, long long n)
+{
+    const LinDesc s = d[blockIdx.y];
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        if (s.a) v += s.ca * s.a[e];
+        if (s.b) v += s.cb * s.b[e];
+        if (s.c) v += s.cc * s.c[e];
+        s.out[e] = v;
+    }
+}
+
+void launch_lincomb(hipStream_t st, const LinDesc *d, int nb, long long n)
+{
+    if (nb <= 0 || n <= 0) return;
+    long long gx = (n + 255) / 256;
+    if (gx > 1024) gx = 1024;
+    int y = 0;
+    while (y < nb) {
+        int chunk = nb - y > 65535 ? 65535 : nb - y;
+        hipLaunchKernelGGL(k_lincomb, dim3((unsigned)gx, chunk), dim3(256), 0, st, d + y, n);
+        y += chunk;
+    }
+}
+
+// sum_{a,b} g[b,a] h[a,b] * scale  (einsum_ba_ab, src/maths/einsum.rs:25)
+__global__ __launch_bounds__(256) void k_einsum_ba_ab(const TraceDesc *__restrict__ d, int n)
+{
+    const TraceDesc t = d[blockIdx.x];
+    double acc = 0.0;
+    for (int e = threadIdx.x; e < n * n; e += 256) {
+        int a = e / n, b = e % n;
+        acc += t.g[b * n + a] * t.h[a * n + b];
+    }
+    __shared__ double red[4];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) *t.out = t.scale * (red[0] + red[1] + red[2] + red[3]);
+}
+
+void launch_einsum(hipStream_t st, const TraceDesc *d, int nd, int n)
+{
+    if (nd <= 0) return;
+    hipLaunchKernelGGL(k_einsum_ba_ab, dim3(nd), dim3(256), 0, st, d, n);
+}
+
+// eri_asym[a,b,c,d] = (ab|cd) - (ac|bd)  (src/read.rs:17-20)
+__global__ void k_antisym(const double *__restrict__ eri, double *__restrict__ out, int n)
+{
+    const long long total = (long long)n * n * n * n;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+        long long d = e % n, t = e / n;
+        long long c = t % n; t /= n;
+        long long b = t % n, a = t / n;
+        out[e] = eri[e] - eri[((a * n + c) * n + b) * n + d];
+    }
+}
+
+void launch_antisym(hipStream_t st, const double *eri, double *out, int n)
+{
+    hipLaunchKernelGGL(k_antisym, dim3(2048), dim3(256), 0, st, eri, out, n);
+}
+
+}  // namespace nw

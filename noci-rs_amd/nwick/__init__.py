@@ -1,0 +1,175 @@
+"""Python host mirror of the MI355X nonorthogonal-Wick engine.
+
+Thin ctypes layer over the C ABI in include/nwick.h (libnwick.so, built from
+noci-rs_amd/csrc).  The object model mirrors noci-rs: an AO problem
+(AoData), reference determinants, a per-reference-pair build
+(build_wicks_shared) and the element evaluators used by the NOCI H/S build
+(calculate_matrix_elements) and batched QMC spawning
+(calculate_hs_pairs_wicks_batched).
+
+There is no fallback: if the shared library is missing or fails to load,
+every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+from .basis import DET_DTYPE  # noqa: F401  (re-export)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libnwick.so")
+
+MODE_REFERENCE = 0
+MODE_CORRECTED = 1
+
+STATUS = {0: "NW_OK", 1: "NW_E_ARG", 2: "NW_E_SHAPE", 3: "NW_E_HIP", 4: "NW_E_MEM", 5: "NW_E_RANK",
+          6: "NW_E_STATE", 7: "NW_E_NONFINITE", 8: "NW_E_ORDER"}
+
+# Every symbol the header declares.
+EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matrix", "nw_pair_slab",
+           "nw_last_timings", "nw_last_error", "nw_destroy"]
+
+_lib = None
+
+
+class NwickError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class PairMeta(ct.Structure):
+    _fields_ = [("ma", ct.c_int32), ("mb", ct.c_int32), ("tilde_s_a", ct.c_double), ("phase_a", ct.c_double),
+                ("tilde_s_b", ct.c_double), ("phase_b", ct.c_double)]
+
+
+def load():
+    """Load libnwick.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NwickError(3, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ct.CDLL(LIB_PATH)
+    vp, dp, u64p = ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_uint64)
+    L.nw_create.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, vp, vp, vp, ct.c_double]
+    L.nw_set_refs.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
+    L.nw_build.argtypes = [vp, ct.c_double, vp, vp]
+    L.nw_hs_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, vp, vp, ct.c_int]
+    L.nw_hs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
+    L.nw_pair_slab.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
+    L.nw_last_timings.argtypes = [vp, dp]
+    L.nw_last_error.argtypes = [vp]
+    L.nw_last_error.restype = ct.c_char_p
+    L.nw_destroy.argtypes = [vp]
+    L.nw_destroy.restype = None
+    for name in EXPORTS:
+        getattr(L, name)
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    """Host numpy array or torch tensor (device or host) -> void pointer."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return ct.c_void_p(a.data_ptr())
+    return ct.c_void_p(a.ctypes.data)
+
+
+class Engine:
+    """One context on one GPU: AO integrals + references + slab."""
+
+    def __init__(self, S, h, eri, enuc=0.0, device=0):
+        self.L = load()
+        self.n_bas = S.shape[0]
+        self._keep = [np.ascontiguousarray(x, dtype=np.float64) for x in (S, h, eri)]
+        ctx = ct.c_void_p()
+        rc = self.L.nw_create(ct.byref(ctx), device, self.n_bas, *[_ptr(x) for x in self._keep], float(enuc))
+        if rc != 0:
+            raise NwickError(rc, "nw_create failed")
+        self.ctx = ctx
+        self.nref = 0
+
+    @classmethod
+    def from_system(cls, sysm, device=0):
+        eng = cls(sysm.S, sysm.h, sysm.eri, sysm.enuc, device)
+        eng.set_refs(sysm.refs, sysm.nocc_a, sysm.nocc_b)
+        return eng
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NwickError(rc, self.L.nw_last_error(self.ctx).decode())
+
+    def set_refs(self, refs, nocc_a, nocc_b):
+        nref = len(refs)
+        nmo = refs[0][0].shape[1]
+        ca = np.ascontiguousarray(np.stack([r[0] for r in refs]), dtype=np.float64)
+        cb = np.ascontiguousarray(np.stack([r[1] for r in refs]), dtype=np.float64)
+        occ = np.zeros((nref, 4), dtype=np.uint64)
+        for i in range(nref):
+            for k, no in enumerate((nocc_a, nocc_b)):
+                v = (1 << no) - 1
+                occ[i, 2 * k] = v & 0xFFFFFFFFFFFFFFFF
+                occ[i, 2 * k + 1] = v >> 64
+        self._check(self.L.nw_set_refs(self.ctx, nref, nmo, _ptr(ca), _ptr(cb), _ptr(occ)))
+        self.nref, self.nmo = nref, nmo
+
+    def build(self, tol=1e-8, pair_mask=None):
+        """build_wicks_shared: returns {(x, w): PairMeta} for the built pairs."""
+        meta = (PairMeta * (self.nref * self.nref))()
+        mask = None
+        if pair_mask is not None:
+            mask = np.ascontiguousarray(pair_mask, dtype=np.uint8).reshape(-1)
+        self._check(self.L.nw_build(self.ctx, float(tol), _ptr(mask), ct.cast(meta, ct.c_void_p)))
+        out = {}
+        for x in range(self.nref):
+            for w in range(self.nref):
+                if mask is None or mask[x * self.nref + w]:
+                    out[(x, w)] = meta[x * self.nref + w]
+        return out
+
+    def hs_matrix(self, dets, mode=MODE_REFERENCE, out=None):
+        """Full H, S (pairs i <= j, mirrored).  dets: DET_DTYPE array or a device tensor of records."""
+        n = len(dets) if not hasattr(dets, "data_ptr") else dets.numel() * dets.element_size() // DET_DTYPE.itemsize
+        if out is None:
+            H = np.empty((n, n)); S = np.empty((n, n))
+        else:
+            H, S = out
+        self._check(self.L.nw_hs_matrix(self.ctx, _ptr(dets), n, _ptr(H), _ptr(S), int(mode)))
+        return H, S
+
+    def hs_batched(self, dets, req, mode=MODE_REFERENCE):
+        req = np.ascontiguousarray(req, dtype=np.uint64).reshape(-1, 2)
+        h = np.empty(len(req)); s = np.empty(len(req))
+        self._check(self.L.nw_hs_batched(self.ctx, _ptr(dets), len(dets), _ptr(req), len(req), _ptr(h), _ptr(s),
+                                         int(mode)))
+        return h, s
+
+    def pair_slab(self, x, w):
+        off = np.zeros(95, dtype=np.int64)
+        meta = np.zeros(32)
+        self._check(self.L.nw_pair_slab(self.ctx, x, w, None, _ptr(off), _ptr(meta)))
+        out = np.empty(int(off[94]))
+        self._check(self.L.nw_pair_slab(self.ctx, x, w, _ptr(out), _ptr(off), _ptr(meta)))
+        return out, off, meta
+
+    def timings(self):
+        t = (ct.c_double * 3)()
+        self.L.nw_last_timings(self.ctx, t)
+        return {"total_ms": t[0], "transform_ms": t[1], "eval_ms": t[2]}
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.nw_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

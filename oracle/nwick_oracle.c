@@ -34,6 +34,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef unsigned __int128 u128;
 
@@ -242,43 +245,58 @@ int ow_svd(int n, const double *A, double *U, double *s, double *V)
 static void ao2mo(int nb, int nm, const double *eri, const double *c1, const double *c2,
                   const double *c3, const double *c4, double *out)
 {
-    size_t nb2 = (size_t)nb * nb, nb3 = nb2 * nb;
+    size_t nb2 = (size_t)nb * nb, nb3 = nb2 * nb, nm2 = (size_t)nm * nm, nm3 = nm2 * nm;
+    double *c4t = (double *)malloc(sizeof(double) * nb * nm);
+    for (int sg = 0; sg < nb; ++sg)
+        for (int s = 0; s < nm; ++s) c4t[(size_t)s * nb + sg] = c4[(size_t)sg * nm + s];
     double *t1 = (double *)calloc(nb3 * nm, sizeof(double));
-    /* t1[mu,nu,lam,s] */
-    for (size_t a = 0; a < nb3; ++a)
+    /* t1[mu,nu,lam,s] = sum_sig eri[mu,nu,lam,sig] c4[sig,s] */
+#pragma omp parallel for schedule(static)
+    for (long long a = 0; a < (long long)nb3; ++a)
         for (int s = 0; s < nm; ++s) {
             double acc = 0;
-            for (int sg = 0; sg < nb; ++sg) acc += eri[a * nb + sg] * c4[(size_t)sg * nm + s];
-            t1[a * nm + s] = acc;
+            const double *e = eri + (size_t)a * nb, *c = c4t + (size_t)s * nb;
+            for (int sg = 0; sg < nb; ++sg) acc += e[sg] * c[sg];
+            t1[(size_t)a * nm + s] = acc;
         }
-    double *t2 = (double *)calloc(nb2 * nm * nm, sizeof(double));
-    /* t2[mu,nu,r,s] */
-    for (size_t mn = 0; mn < nb2; ++mn)
-        for (int r = 0; r < nm; ++r)
-            for (int s = 0; s < nm; ++s) {
-                double acc = 0;
-                for (int l = 0; l < nb; ++l) acc += c3[(size_t)l * nm + r] * t1[(mn * nb + l) * nm + s];
-                t2[(mn * nm + r) * nm + s] = acc;
+    free(c4t);
+    double *t2 = (double *)calloc(nb2 * nm2, sizeof(double));
+    /* t2[mu,nu,r,s] = sum_lam c3[lam,r] t1[mu,nu,lam,s] */
+#pragma omp parallel for schedule(static)
+    for (long long mn = 0; mn < (long long)nb2; ++mn)
+        for (int l = 0; l < nb; ++l) {
+            const double *src = t1 + ((size_t)mn * nb + l) * nm;
+            for (int r = 0; r < nm; ++r) {
+                double cr = c3[(size_t)l * nm + r];
+                double *dst = t2 + ((size_t)mn * nm + r) * nm;
+                for (int s = 0; s < nm; ++s) dst[s] += cr * src[s];
             }
+        }
     free(t1);
-    double *t3 = (double *)calloc((size_t)nb * nm * nm * nm, sizeof(double));
-    /* t3[mu,q,r,s] */
+    double *t3 = (double *)calloc((size_t)nb * nm3, sizeof(double));
+    /* t3[mu,q,r,s] = sum_nu c2[nu,q] t2[mu,nu,r,s] */
+#pragma omp parallel for schedule(static)
     for (int mu = 0; mu < nb; ++mu)
-        for (int q = 0; q < nm; ++q)
-            for (size_t rs = 0; rs < (size_t)nm * nm; ++rs) {
-                double acc = 0;
-                for (int nu = 0; nu < nb; ++nu)
-                    acc += c2[(size_t)nu * nm + q] * t2[((size_t)mu * nb + nu) * nm * nm + rs];
-                t3[((size_t)mu * nm + q) * nm * nm + rs] = acc;
+        for (int nu = 0; nu < nb; ++nu) {
+            const double *src = t2 + ((size_t)mu * nb + nu) * nm2;
+            for (int q = 0; q < nm; ++q) {
+                double cq = c2[(size_t)nu * nm + q];
+                double *dst = t3 + ((size_t)mu * nm + q) * nm2;
+                for (size_t rs = 0; rs < nm2; ++rs) dst[rs] += cq * src[rs];
             }
-    free(t2);
-    size_t nm3 = (size_t)nm * nm * nm;
-    for (int p = 0; p < nm; ++p)
-        for (size_t qrs = 0; qrs < nm3; ++qrs) {
-            double acc = 0;
-            for (int mu = 0; mu < nb; ++mu) acc += c1[(size_t)mu * nm + p] * t3[(size_t)mu * nm3 + qrs];
-            out[(size_t)p * nm3 + qrs] = acc;
         }
+    free(t2);
+    /* out[p,q,r,s] = sum_mu c1[mu,p] t3[mu,q,r,s] */
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < nm; ++p) {
+        double *dst = out + (size_t)p * nm3;
+        for (size_t k = 0; k < nm3; ++k) dst[k] = 0.0;
+        for (int mu = 0; mu < nb; ++mu) {
+            double cp = c1[(size_t)mu * nm + p];
+            const double *src = t3 + (size_t)mu * nm3;
+            for (size_t k = 0; k < nm3; ++k) dst[k] += cp * src[k];
+        }
+    }
     free(t3);
 }
 
@@ -432,6 +450,7 @@ static void spin_core_build(int nb, int nm, const double *S, const double *lc, u
 static void coulomb(int n, const double *eri, const double *M, double *J)
 {
     size_t n2 = (size_t)n * n;
+#pragma omp parallel for schedule(static)
     for (int s = 0; s < n; ++s)
         for (int t = 0; t < n; ++t) {
             double acc = 0;
@@ -442,6 +461,7 @@ static void coulomb(int n, const double *eri, const double *M, double *J)
 }
 static void exchange(int n, const double *eri, const double *M, double *K)
 {
+#pragma omp parallel for schedule(static)
     for (int s = 0; s < n; ++s)
         for (int t = 0; t < n; ++t) {
             double acc = 0;
@@ -1055,6 +1075,27 @@ int ow_hs(const double *slab, const long long *off, const double *meta, int nm,
         if (mode == 1) *H += enuc * *S;
     }
     return 0;
+}
+
+/* Evaluate a block of determinant pairs on one ordered reference pair:
+ * xex/wex: per pair 8 uint64 masks; exph: per pair excitation phase. */
+int ow_hs_many(const double *slab, const long long *off, const double *meta, int nm, long long npair,
+               const uint64_t *xex, const uint64_t *wex, const double *exph, double enuc, double tol, int mode,
+               double *H, double *S)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long long q = 0; q < npair; ++q)
+        ow_hs(slab, off, meta, nm, xex + 8 * q, wex + 8 * q, exph[q], enuc, tol, mode, H + q, S + q);
+    return 0;
+}
+
+int ow_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* excitation_phase_bits (src/basis/excitation.rs:79-116) */
