@@ -101,8 +101,96 @@ __global__ __launch_bounds__(256) void k_qtransform(const double *__restrict__ X
     gemm_tile<1, 1>(g, nout, rows, nl, nout, nl, rows, 1.0, 0.0);
 }
 
+// MFMA form of the rotating quarter transform (v_mfma_f64_16x16x4f64).
+// One workgroup = NW waves; wave w owns output rows l' in [16w, 16w+16) and
+// keeps its 16 x nl slice of c in registers as MFMA A fragments (lane ->
+// c[4kk + lane/16][16w + lane%16]).  The workgroup walks 64-column tiles of
+// X (64 rows m of X, contiguous in HBM) staged in LDS; each wave issues
+// KS x 4 MFMAs per tile and stores its 16 x 64 block of Y with 128-byte
+// coalesced row segments.  Persistent grid: tiles strided by gridDim.x.
+typedef double nw_d4 __attribute__((ext_vector_type(4)));
+
+template <int KS>
+__global__ __launch_bounds__(512) void k_qtransform_mfma(const double *__restrict__ X, const double *__restrict__ c,
+                                                          double *__restrict__ Y, long long rows, int nl, int nout,
+                                                          long long ntiles)
+{
+    extern __shared__ double xs[];
+    constexpr int NLP = KS * 4;
+    constexpr int LDX = NLP + 1;
+    const int nwave = blockDim.x >> 6;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int lp = wave * 16 + lr;
+    double afr[KS];
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk) {
+        const int l = kk * 4 + lk;
+        afr[kk] = (l < nl && lp < nout) ? c[(size_t)l * nout + lp] : 0.0;
+    }
+    const bool active = wave * 16 < nout;
+    for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const long long m0 = t * 64;
+        const int mcount = (int)((rows - m0) < 64 ? (rows - m0) : 64);
+        __syncthreads();
+        for (int r = wave; r < 64; r += nwave) {
+            const double *src = X + (size_t)(m0 + r) * nl;
+#pragma unroll
+            for (int k0 = 0; k0 < NLP; k0 += 64) {
+                const int k = k0 + lane;
+                if (k < NLP) xs[r * LDX + k] = (r < mcount && k < nl) ? src[k] : 0.0;
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+        nw_d4 acc[4];
+#pragma unroll
+        for (int ns = 0; ns < 4; ++ns) acc[ns] = nw_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            const double a = afr[kk];
+#pragma unroll
+            for (int ns = 0; ns < 4; ++ns) {
+                const double b = xs[(ns * 16 + lr) * LDX + kk * 4 + lk];
+                acc[ns] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[ns], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int ns = 0; ns < 4; ++ns) {
+            const int mm = ns * 16 + lr;
+            if (mm >= mcount) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = wave * 16 + 4 * r + lk;  // D row = 4*reg + lane/16 (probed on gfx950)
+                if (row < nout) Y[(size_t)row * rows + m0 + mm] = acc[ns][r];
+            }
+        }
+    }
+}
+
 void launch_qtransform(hipStream_t st, const double *X, const double *c, double *Y, long long rows, int nl, int nout)
 {
+    const int ks = (nl + 3) / 4;
+    const int nw = (nout + 15) / 16;
+    const long long ntiles = (rows + 63) / 64;
+    if (ks <= 31 && nw <= 8) {  // LDS tile <= 64 KB
+        const size_t lds = (size_t)64 * (ks * 4 + 1) * sizeof(double);
+        long long grid = 256LL * 4;
+        if (grid > ntiles) grid = ntiles;
+        dim3 g((unsigned)grid), b(64 * nw);
+#define NW_QT(K)                                                                                            \
+    case K:                                                                                                 \
+        hipLaunchKernelGGL((k_qtransform_mfma<K>), g, b, lds, st, X, c, Y, rows, nl, nout, ntiles);       \
+        return;
+        switch (ks) {
+            NW_QT(1) NW_QT(2) NW_QT(3) NW_QT(4) NW_QT(5) NW_QT(6) NW_QT(7) NW_QT(8) NW_QT(9) NW_QT(10)
+            NW_QT(11) NW_QT(12) NW_QT(13) NW_QT(14) NW_QT(15) NW_QT(16) NW_QT(17) NW_QT(18) NW_QT(19)
+            NW_QT(20) NW_QT(21) NW_QT(22) NW_QT(23) NW_QT(24) NW_QT(25) NW_QT(26) NW_QT(27) NW_QT(28)
+            NW_QT(29) NW_QT(30) NW_QT(31) NW_QT(32)
+        default: break;
+        }
+#undef NW_QT
+    }
     dim3 grid((unsigned)((rows + 63) / 64), (nout + 63) / 64, 1);
     hipLaunchKernelGGL(k_qtransform, grid, dim3(256), 0, st, X, c, Y, (int)rows, nl, nout);
 }
@@ -146,6 +234,7 @@ __global__ __launch_bounds__(64) void k_svd(const double *__restrict__ A, int n,
     __shared__ double W[MAX_NOCC * MAX_NOCC];   // column-major: W[c*64 + r]
     __shared__ double Vs[MAX_NOCC * MAX_NOCC];
     __shared__ double Us[MAX_NOCC * MAX_NOCC];
+    __shared__ double Vo[MAX_NOCC * MAX_NOCC];  // V columns in descending-sigma order
     __shared__ double sv[MAX_NOCC];
     __shared__ int ord[MAX_NOCC];
     __shared__ int piv_s;
@@ -192,25 +281,32 @@ __global__ __launch_bounds__(64) void k_svd(const double *__restrict__ A, int n,
     }
     __syncthreads();
     const double smax = n > 0 ? sv[ord[0]] : 0.0;
-    // U columns (normalised W columns, or completion for numerically zero ones)
+    // U columns are the normalised W columns; numerically zero singular values
+    // get canonical left and right null vectors (Gram-Schmidt of unit vectors
+    // against the preceding columns) so the u_k <-> v_k pairing is independent
+    // of the rotation history.
     for (int k = 0; k < n; ++k) {
         int src = ord[k];
         double s = sv[src];
         bool good = s > 1e-13 * (smax > 0 ? smax : 1.0);
         if (good) {
             Us[k * 64 + r] = (r < n) ? W[src * 64 + r] / s : 0.0;
+            Vo[k * 64 + r] = (r < n) ? Vs[src * 64 + r] : 0.0;
         } else {
-            for (int e = 0; e < n; ++e) {
-                double cand = (r == e) ? 1.0 : 0.0;
-                for (int pass = 0; pass < 2; ++pass)
-                    for (int j = 0; j < k; ++j) {
-                        double d = wave_sum(Us[j * 64 + r] * cand);
-                        cand -= d * Us[j * 64 + r];
+            for (int side = 0; side < 2; ++side) {
+                double *Q = side ? Vo : Us;
+                for (int e = 0; e < n; ++e) {
+                    double cand = (r == e) ? 1.0 : 0.0;
+                    for (int pass = 0; pass < 2; ++pass)
+                        for (int j = 0; j < k; ++j) {
+                            double d = wave_sum(Q[j * 64 + r] * cand);
+                            cand -= d * Q[j * 64 + r];
+                        }
+                    double nn = wave_sum(cand * cand);
+                    if (nn > 1e-2) {
+                        Q[k * 64 + r] = cand / sqrt(nn);
+                        break;
                     }
-                double nn = wave_sum(cand * cand);
-                if (nn > 1e-2) {
-                    Us[k * 64 + r] = cand / sqrt(nn);
-                    break;
                 }
             }
         }
@@ -221,7 +317,7 @@ __global__ __launch_bounds__(64) void k_svd(const double *__restrict__ A, int n,
     if (r < n) {
         for (int k = 0; k < n; ++k) {
             u[r * n + k] = Us[k * 64 + r];
-            v[r * n + k] = Vs[ord[k] * 64 + r];
+            v[r * n + k] = Vo[k * 64 + r];
         }
         sig[(size_t)prob * n + r] = sv[ord[r]];
     }
@@ -230,7 +326,7 @@ __global__ __launch_bounds__(64) void k_svd(const double *__restrict__ A, int n,
     double dets[2];
     for (int which = 0; which < 2; ++which) {
         // copy into W as row-major-by-lane: W[c*64 + r] = M[r][c]
-        for (int c = 0; c < n; ++c) W[c * 64 + r] = (r < n) ? (which == 0 ? Us[c * 64 + r] : Vs[ord[c] * 64 + r]) : 0.0;
+        for (int c = 0; c < n; ++c) W[c * 64 + r] = (r < n) ? (which == 0 ? Us[c * 64 + r] : Vo[c * 64 + r]) : 0.0;
         __syncthreads();
         double sign = 1.0, d = 1.0;
         for (int k = 0; k < n; ++k) {

@@ -210,26 +210,34 @@ int ow_svd(int n, const double *A, double *U, double *s, double *V)
     for (int k = 0; k < n; ++k) {
         int src = ord[k];
         s[k] = sv[src];
-        for (int i = 0; i < n; ++i) V[i * n + k] = Vt[i * n + src];
         int good = sv[src] > 1e-13 * (smax > 0 ? smax : 1.0);
         for (int i = 0; i < n; ++i) U[i * n + k] = good ? W[i * n + src] / sv[src] : 0.0;
+        for (int i = 0; i < n; ++i) V[i * n + k] = good ? Vt[i * n + src] : 0.0;
         if (!good) {
-            /* complete: orthonormal direction against previous columns */
-            for (int e = 0; e < n; ++e) {
-                double cand[64];
-                for (int i = 0; i < n; ++i) cand[i] = (i == e) ? 1.0 : 0.0;
-                for (int pass = 0; pass < 2; ++pass)
-                    for (int j = 0; j < k; ++j) {
-                        double d = 0;
-                        for (int i = 0; i < n; ++i) d += U[i * n + j] * cand[i];
-                        for (int i = 0; i < n; ++i) cand[i] -= d * U[i * n + j];
+            /* Numerically zero singular values: the left and right null
+             * spaces are completed canonically (Gram-Schmidt of unit vectors
+             * against the preceding columns), so the pairing u_k <-> v_k does
+             * not depend on the rotation history.  Exact matrix elements are
+             * gauge invariant; the reference's m>0 arithmetic is not for
+             * m >= 2, which makes a fixed gauge necessary to compare it. */
+            for (int side = 0; side < 2; ++side) {
+                double *Q = side ? V : U;
+                for (int e = 0; e < n; ++e) {
+                    double cand[64];
+                    for (int i = 0; i < n; ++i) cand[i] = (i == e) ? 1.0 : 0.0;
+                    for (int pass = 0; pass < 2; ++pass)
+                        for (int j = 0; j < k; ++j) {
+                            double d = 0;
+                            for (int i = 0; i < n; ++i) d += Q[i * n + j] * cand[i];
+                            for (int i = 0; i < n; ++i) cand[i] -= d * Q[i * n + j];
+                        }
+                    double nn = 0;
+                    for (int i = 0; i < n; ++i) nn += cand[i] * cand[i];
+                    if (nn > 1e-2) {
+                        nn = sqrt(nn);
+                        for (int i = 0; i < n; ++i) Q[i * n + k] = cand[i] / nn;
+                        break;
                     }
-                double nn = 0;
-                for (int i = 0; i < n; ++i) nn += cand[i] * cand[i];
-                if (nn > 1e-2) {
-                    nn = sqrt(nn);
-                    for (int i = 0; i < n; ++i) U[i * n + k] = cand[i] / nn;
-                    break;
                 }
             }
         }
