@@ -168,11 +168,107 @@ __global__ __launch_bounds__(512) void k_qtransform_mfma(const double *__restric
     }
 }
 
+// Pipelined form for nl % 4 == 0: the next 64-row tile of X is fetched into
+// registers (16-byte loads, all in flight at once) while the current tile is
+// multiplied out of LDS, so HBM latency hides behind the MFMA work.
+template <int KS>
+__global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const double *__restrict__ X,
+                                                               const double *__restrict__ c,
+                                                               double *__restrict__ Y, long long rows, int nout,
+                                                               long long ntiles)
+{
+    constexpr int NL = KS * 4;
+    constexpr int LDX = NL + 1;
+    constexpr int NT = 448;
+    constexpr int TILE2 = 64 * NL / 2;          // double2 per tile
+    constexpr int PER = (TILE2 + NT - 1) / NT;  // double2 per thread
+    extern __shared__ double xs[];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int lp = wave * 16 + lr;
+    double afr[KS];
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk) afr[kk] = (lp < nout) ? c[(size_t)(kk * 4 + lk) * nout + lp] : 0.0;
+    const bool active = wave * 16 < nout;
+    double2 buf[PER];
+    auto gload = [&](long long t) {
+        const double2 *src = reinterpret_cast<const double2 *>(X + (size_t)t * 64 * NL);
+        const long long rem = rows - t * 64;
+        const int lim2 = (int)((rem < 64 ? rem : 64) * NL / 2);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e2 = tid + j * NT;
+            buf[j] = (e2 < lim2) ? src[e2] : double2{0.0, 0.0};
+        }
+    };
+    long long t = blockIdx.x;
+    if (t < ntiles) gload(t);
+    for (; t < ntiles; t += gridDim.x) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = 2 * (tid + j * NT);
+            if (e < 2 * TILE2) {
+                const int r = e / NL, k = e % NL;
+                xs[r * LDX + k] = buf[j].x;
+                xs[r * LDX + k + 1] = buf[j].y;
+            }
+        }
+        __syncthreads();
+        const long long tn = t + gridDim.x;
+        if (tn < ntiles) gload(tn);
+        if (!active) continue;
+        const long long m0 = t * 64;
+        const int mcount = (int)((rows - m0) < 64 ? (rows - m0) : 64);
+        nw_d4 acc[4];
+#pragma unroll
+        for (int ns = 0; ns < 4; ++ns) acc[ns] = nw_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            const double a = afr[kk];
+#pragma unroll
+            for (int ns = 0; ns < 4; ++ns) {
+                const double b = xs[(ns * 16 + lr) * LDX + kk * 4 + lk];
+                acc[ns] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[ns], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int ns = 0; ns < 4; ++ns) {
+            const int mm = ns * 16 + lr;
+            if (mm >= mcount) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = wave * 16 + 4 * r + lk;
+                if (row < nout) Y[(size_t)row * rows + m0 + mm] = acc[ns][r];
+            }
+        }
+    }
+}
+
 void launch_qtransform(hipStream_t st, const double *X, const double *c, double *Y, long long rows, int nl, int nout)
 {
     const int ks = (nl + 3) / 4;
     const int nw = (nout + 15) / 16;
     const long long ntiles = (rows + 63) / 64;
+    if (nl % 4 == 0 && ks <= 31 && nw <= 7) {
+        const size_t lds = (size_t)64 * (ks * 4 + 1) * sizeof(double);
+        long long grid = 256LL * 2;
+        if (grid > ntiles) grid = ntiles;
+        dim3 g((unsigned)grid), b(448);
+#define NW_QP(K)                                                                                            \
+    case K:                                                                                                 \
+        hipLaunchKernelGGL((k_qtransform_mfma_pipe<K>), g, b, lds, st, X, c, Y, rows, nout, ntiles);      \
+        return;
+        switch (ks) {
+            NW_QP(1) NW_QP(2) NW_QP(3) NW_QP(4) NW_QP(5) NW_QP(6) NW_QP(7) NW_QP(8) NW_QP(9) NW_QP(10)
+            NW_QP(11) NW_QP(12) NW_QP(13) NW_QP(14) NW_QP(15) NW_QP(16) NW_QP(17) NW_QP(18) NW_QP(19)
+            NW_QP(20) NW_QP(21) NW_QP(22) NW_QP(23) NW_QP(24) NW_QP(25) NW_QP(26) NW_QP(27) NW_QP(28)
+            NW_QP(29) NW_QP(30) NW_QP(31)
+        default: break;
+        }
+#undef NW_QP
+    }
     if (ks <= 31 && nw <= 8) {  // LDS tile <= 64 KB
         const size_t lds = (size_t)64 * (ks * 4 + 1) * sizeof(double);
         long long grid = 256LL * 4;
