@@ -152,6 +152,12 @@ def main():
     avg_ms = float(np.mean(transforms)) / launches
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     eval_rate = npairs / (float(np.mean(evals)) * 1e-3)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "qtransform_n100_hbm_traffic.txt")
+    if os.path.exists(tfile):
+        for line in open(tfile):
+            if line.startswith("TRAFFIC_BYTES"):
+                traffic = float(line.split()[1])
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: oracle on pair (0,1) ...")
@@ -178,9 +184,10 @@ def main():
         "build_ms": float(np.mean(builds)),
         "transform_ms": float(np.mean(transforms)),
         "eval_ms": float(np.mean(evals)),
-        "roofline": {"bound": "mfma", "kernel": "k_qtransform (FP64 rotating quarter transform)",
+        "roofline": {"bound": "mfma", "kernel": "k_qtransform_mfma_pipe (FP64 rotating quarter transform)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "traffic_source": "profiles/qtransform_n100_hbm_traffic.txt (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                      "launches_per_step": launches, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
     }
