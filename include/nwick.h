@@ -43,6 +43,14 @@ enum nw_status {
  * identical results when m_alpha = m_beta = 0. */
 enum nw_mode { NW_MODE_REFERENCE = 0, NW_MODE_CORRECTED = 1 };
 
+/* Arithmetic of the per-reference-pair build.  NW_PREC_FP64: FP64 throughout
+ * (the reference's arithmetic).  NW_PREC_MIXED: the four-index transforms
+ * producing the same-spin J and different-spin II tensors run with FP32
+ * operands, FP32 intermediates and FP32 MFMA accumulation; the SVD, zero
+ * classification, co-densities, one-body and one-column intermediates and
+ * every evaluation stay FP64. */
+enum nw_precision { NW_PREC_FP64 = 0, NW_PREC_MIXED = 1 };
+
 /* One determinant: the fields of DetState (src/scalar.rs:32-61) the Wick path
  * reads -- parent reference, excitation phases, occupations and the
  * excitation relative to the parent (Excitation, src/lib.rs:79-119). */
@@ -96,6 +104,10 @@ int nw_hs_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t
  * build_noci_hs (:214).  Evaluates pairs i <= j and mirrors them; H, S are
  * ndet x ndet row-major. */
 int nw_hs_matrix(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double *H, double *S, int mode);
+
+/* Select the build arithmetic (nw_precision) for subsequent nw_build calls.
+ * No reference counterpart: the reference computes in FP64 only. */
+int nw_set_precision(nw_ctx *ctx, int precision);
 
 /* Copy one ordered pair's slab block (layout of layout.rs:21-189) and its
  * offsets (95 int64, relative to the block, -1 = absent, last = length) and

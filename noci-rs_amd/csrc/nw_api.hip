@@ -120,6 +120,8 @@ struct nw_ctx {
     int n = 0;
     double enuc = 0.0;
     double *d_S = nullptr, *d_h = nullptr, *d_eri = nullptr, *d_easym = nullptr;
+    float *d_eri32 = nullptr, *d_easym32 = nullptr;  // FP32 copies for the mixed build
+    int precision = NW_PREC_FP64;
     int nref = 0, nmo = 0, nocc[2] = {0, 0};
     double *d_c[2] = {nullptr, nullptr};  // nref x n x nmo per spin
     // build state
@@ -131,7 +133,7 @@ struct nw_ctx {
     PairInfo *d_pinfo = nullptr;
     double t_ms[3] = {0, 0, 0};
     // scratch
-    DBuf arena_desc, scratchA, scratchB, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
+    DBuf arena_desc, scratchA, scratchB, cx32, xc32, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
     std::deque<std::vector<char>> keep;
 
     ~nw_ctx()
@@ -139,6 +141,8 @@ struct nw_ctx {
         (void)hipSetDevice(device);
         if (st) (void)hipStreamSynchronize(st);
         for (double *p : {d_S, d_h, d_eri, d_easym, d_c[0], d_c[1], d_slab})
+            if (p) (void)hipFree(p);
+        for (float *p : {d_eri32, d_easym32})
             if (p) (void)hipFree(p);
         if (d_pinfo) (void)hipFree(d_pinfo);
         for (auto &e : ev)
@@ -582,7 +586,56 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         HIPCHK(hipGetLastError());
         // ---- four-index transforms (build.rs:261-313, 877-917) ----------------
         HIPCHK(hipEventRecord(c->ev[1], c->st));
-        {
+        const bool mixed = c->precision == NW_PREC_MIXED && n % 4 == 0;
+        if (mixed) {
+            // Mixed form: FP32 operands, intermediates and MFMA accumulation for the
+            // four-index transforms; the slab blocks they produce are stored in FP64.
+            const size_t n4 = (size_t)n * n * n * n;
+            if (!c->d_eri32) {
+                c->d_eri32 = (float *)dalloc<char>(n4 * sizeof(float));
+                c->d_easym32 = (float *)dalloc<char>(n4 * sizeof(float));
+                launch_to_f32(c->st, c->d_eri, c->d_eri32, (long long)n4);
+                launch_to_f32(c->st, c->d_easym, c->d_easym32, (long long)n4);
+            }
+            float *cx32 = c->cx32.get<float>((size_t)4 * P * nnm), *xc32 = c->xc32.get<float>((size_t)4 * P * nnm);
+            launch_to_f32(c->st, cx, cx32, (long long)4 * P * nnm);
+            launch_to_f32(c->st, xc, xc32, (long long)4 * P * nnm);
+            const size_t big = std::max(n4, (size_t)nm * n * n * n);
+            float *TA = c->scratchA.get<float>(big), *TB = c->scratchB.get<float>(big);
+            auto transform32 = [&](const float *E, const float *c1, const float *c2, const float *c3, const float *c4,
+                                   double *out) {
+                bool ok = launch_qtransform_f32(c->st, E, c4, TA, (long long)n * n * n, n, nm) &&
+                          launch_qtransform_f32(c->st, TA, c3, TB, (long long)nm * n * n, n, nm) &&
+                          launch_qtransform_f32(c->st, TB, c2, TA, (long long)nm * nm * n, n, nm) &&
+                          launch_qtransform_f32_out64(c->st, TA, c1, out, (long long)nm * nm * nm, n, nm);
+                if (!ok) throw NwError(NW_E_SHAPE, "mixed-precision transform needs n_bas % 4 == 0 and nmo <= 112");
+            };
+            for (int p = 0; p < P; ++p) {
+                const PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
+                for (int sp = 0; sp < 2; ++sp) {
+                    const long long *o = pi.off + (sp ? O_BB : O_AA);
+                    const int pr = prob(p, sp);
+                    const float *cxs[2] = {cx32 + (size_t)(pr * 2) * nnm, cx32 + (size_t)(pr * 2 + 1) * nnm};
+                    const float *xcs[2] = {xc32 + (size_t)(pr * 2) * nnm, xc32 + (size_t)(pr * 2 + 1) * nnm};
+                    for (int s = 0; s < 10; ++s) {
+                        if (o[O_J + s] < 0) continue;
+                        const int i = jbranch(s, 0), j = jbranch(s, 1), k = jbranch(s, 2), l = jbranch(s, 3);
+                        transform32(c->d_easym32, cxs[k], xcs[l], cxs[i], xcs[j], slab + pi.base + o[O_J + s]);
+                    }
+                }
+                const int pa = prob(p, 0), pb = prob(p, 1);
+                for (int a0 = 0; a0 < 2; ++a0)
+                    for (int aj = 0; aj < 2; ++aj)
+                        for (int b0 = 0; b0 < 2; ++b0)
+                            for (int bj = 0; bj < 2; ++bj) {
+                                long long oo = pi.off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj];
+                                if (oo < 0) continue;
+                                transform32(c->d_eri32, cx32 + (size_t)(pa * 2 + a0) * nnm, xc32 + (size_t)(pa * 2 + aj) * nnm,
+                                            cx32 + (size_t)(pb * 2 + b0) * nnm, xc32 + (size_t)(pb * 2 + bj) * nnm,
+                                            slab + pi.base + oo);
+                            }
+            }
+        } else {
             const size_t big = std::max((size_t)n * n * n * n, (size_t)nm * n * n * n);
             double *TA = c->scratchA.get<double>(big), *TB = c->scratchB.get<double>(big);
             auto transform = [&](const double *E, const double *c1, const double *c2, const double *c3, const double *c4,
@@ -826,6 +879,16 @@ int nw_hs_batched(nw_ctx *c, const nw_det *dets, uint64_t ndet, const uint64_t *
         c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
         c->t_ms[1] = 0.0;
         c->t_ms[2] = elapsed(c->ev[1], c->ev[2]);
+        return NW_OK;
+    });
+}
+
+int nw_set_precision(nw_ctx *c, int precision)
+{
+    return guarded(c, [&]() -> int {
+        if (precision != NW_PREC_FP64 && precision != NW_PREC_MIXED) return fail(c, NW_E_ARG, "unknown precision");
+        c->precision = precision;
+        c->built = false;
         return NW_OK;
     });
 }

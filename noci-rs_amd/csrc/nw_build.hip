@@ -109,6 +109,9 @@ __global__ __launch_bounds__(256) void k_qtransform(const double *__restrict__ X
 // KS x 4 MFMAs per tile and stores its 16 x 64 block of Y with 128-byte
 // coalesced row segments.  Persistent grid: tiles strided by gridDim.x.
 typedef double nw_d4 __attribute__((ext_vector_type(4)));
+// Row of the 16x16 FP32 MFMA result held in accumulator register r of lane
+// (lane/16 = lk): 4*lk + r on CDNA3/4.
+#define NW_F32_ROW(lk, r) (4 * (lk) + (r))
 
 template <int KS>
 __global__ __launch_bounds__(512) void k_qtransform_mfma(const double *__restrict__ X, const double *__restrict__ c,
@@ -244,6 +247,137 @@ __global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const double *__re
             }
         }
     }
+}
+
+// FP32 form used by the mixed-precision build: FP32 operands and
+// intermediates, FP32 MFMA accumulation (v_mfma_f32_16x16x4f32), output
+// either FP32 (intermediate passes) or FP64 (the final pass, into the slab).
+// 128-row tiles of X, staged in LDS, next tile prefetched into registers.
+typedef float nw_f4 __attribute__((ext_vector_type(4)));
+
+template <int KS, typename OutT>
+__global__ __launch_bounds__(448) void k_qtransform_f32_pipe(const float *__restrict__ X, const float *__restrict__ c,
+                                                              OutT *__restrict__ Y, long long rows, int nout,
+                                                              long long ntiles)
+{
+    constexpr int NL = KS * 4;
+    constexpr int LDX = NL + 1;
+    constexpr int NT = 448;
+    constexpr int TR = 128;                     // rows of X per tile
+    constexpr int TILE4 = TR * NL / 4;          // float4 per tile
+    constexpr int PER = (TILE4 + NT - 1) / NT;  // float4 per thread
+    extern __shared__ float xsf[];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int lp = wave * 16 + lr;
+    float afr[KS];
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk) afr[kk] = (lp < nout) ? c[(size_t)(kk * 4 + lk) * nout + lp] : 0.0f;
+    const bool active = wave * 16 < nout;
+    float4 buf[PER];
+    auto gload = [&](long long t) {
+        const float4 *src = reinterpret_cast<const float4 *>(X + (size_t)t * TR * NL);
+        const long long rem = rows - t * TR;
+        const int lim4 = (int)((rem < TR ? rem : TR) * NL / 4);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e4 = tid + j * NT;
+            buf[j] = (e4 < lim4) ? src[e4] : float4{0.f, 0.f, 0.f, 0.f};
+        }
+    };
+    long long t = blockIdx.x;
+    if (t < ntiles) gload(t);
+    for (; t < ntiles; t += gridDim.x) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = 4 * (tid + j * NT);
+            if (e < 4 * TILE4) {
+                const int r = e / NL, k = e % NL;
+                float *d = xsf + r * LDX + k;
+                d[0] = buf[j].x; d[1] = buf[j].y; d[2] = buf[j].z; d[3] = buf[j].w;
+            }
+        }
+        __syncthreads();
+        const long long tn = t + gridDim.x;
+        if (tn < ntiles) gload(tn);
+        if (!active) continue;
+        const long long m0 = t * TR;
+        const int mcount = (int)((rows - m0) < TR ? (rows - m0) : TR);
+        nw_f4 acc[TR / 16];
+#pragma unroll
+        for (int ns = 0; ns < TR / 16; ++ns) acc[ns] = nw_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            const float a = afr[kk];
+#pragma unroll
+            for (int ns = 0; ns < TR / 16; ++ns) {
+                const float b = xsf[(ns * 16 + lr) * LDX + kk * 4 + lk];
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[ns], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int ns = 0; ns < TR / 16; ++ns) {
+            const int mm = ns * 16 + lr;
+            if (mm >= mcount) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = wave * 16 + NW_F32_ROW(lk, r);
+                if (row < nout) Y[(size_t)row * rows + m0 + mm] = (OutT)acc[ns][r];
+            }
+        }
+    }
+}
+
+__global__ void k_to_f32(const double *__restrict__ in, float *__restrict__ out, long long n)
+{
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+        out[e] = (float)in[e];
+}
+
+void launch_to_f32(hipStream_t st, const double *in, float *out, long long n)
+{
+    if (n <= 0) return;
+    long long g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)g), dim3(256), 0, st, in, out, n);
+}
+
+template <typename OutT>
+static bool launch_qtransform_f32_t(hipStream_t st, const float *X, const float *c, OutT *Y, long long rows, int nl,
+                                    int nout)
+{
+    const int ks = nl / 4;
+    if (nl % 4 != 0 || ks > 31 || (nout + 15) / 16 > 7) return false;
+    const long long ntiles = (rows + 127) / 128;
+    const size_t lds = (size_t)128 * (nl + 1) * sizeof(float);
+    long long grid = 256LL * 2;
+    if (grid > ntiles) grid = ntiles;
+    dim3 g((unsigned)grid), b(448);
+#define NW_QF(K)                                                                                            \
+    case K:                                                                                                 \
+        hipLaunchKernelGGL((k_qtransform_f32_pipe<K, OutT>), g, b, lds, st, X, c, Y, rows, nout, ntiles); \
+        return true;
+    switch (ks) {
+        NW_QF(1) NW_QF(2) NW_QF(3) NW_QF(4) NW_QF(5) NW_QF(6) NW_QF(7) NW_QF(8) NW_QF(9) NW_QF(10)
+        NW_QF(11) NW_QF(12) NW_QF(13) NW_QF(14) NW_QF(15) NW_QF(16) NW_QF(17) NW_QF(18) NW_QF(19)
+        NW_QF(20) NW_QF(21) NW_QF(22) NW_QF(23) NW_QF(24) NW_QF(25) NW_QF(26) NW_QF(27) NW_QF(28)
+        NW_QF(29) NW_QF(30) NW_QF(31)
+    default: return false;
+    }
+#undef NW_QF
+}
+
+bool launch_qtransform_f32(hipStream_t st, const float *X, const float *c, float *Y, long long rows, int nl, int nout)
+{
+    return launch_qtransform_f32_t<float>(st, X, c, Y, rows, nl, nout);
+}
+
+bool launch_qtransform_f32_out64(hipStream_t st, const float *X, const float *c, double *Y, long long rows, int nl,
+                                 int nout)
+{
+    return launch_qtransform_f32_t<double>(st, X, c, Y, rows, nl, nout);
 }
 
 void launch_qtransform(hipStream_t st, const double *X, const double *c, double *Y, long long rows, int nl, int nout)

@@ -25,6 +25,11 @@ void launch_antisym(hipStream_t st, const double *eri, double *out, int n);
 // Rotating quarter transform Y[l', m] = sum_l c[l, l'] X[m, l]
 // (X: rows x nl row-major, c: nl x nout row-major, Y: nout x rows row-major).
 void launch_qtransform(hipStream_t st, const double *X, const double *c, double *Y, long long rows, int nl, int nout);
+// FP32 variants (mixed-precision build); return false when the shape is unsupported.
+bool launch_qtransform_f32(hipStream_t st, const float *X, const float *c, float *Y, long long rows, int nl, int nout);
+bool launch_qtransform_f32_out64(hipStream_t st, const float *X, const float *c, double *Y, long long rows, int nl,
+                                 int nout);
+void launch_to_f32(hipStream_t st, const double *in, float *out, long long n);
 
 // Evaluation.
 struct EvalTile {

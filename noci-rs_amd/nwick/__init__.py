@@ -29,8 +29,10 @@ STATUS = {0: "NW_OK", 1: "NW_E_ARG", 2: "NW_E_SHAPE", 3: "NW_E_HIP", 4: "NW_E_ME
           6: "NW_E_STATE", 7: "NW_E_NONFINITE", 8: "NW_E_ORDER"}
 
 # Every symbol the header declares.
-EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matrix", "nw_pair_slab",
-           "nw_last_timings", "nw_last_error", "nw_destroy"]
+EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matrix", "nw_set_precision",
+           "nw_pair_slab", "nw_last_timings", "nw_last_error", "nw_destroy"]
+PREC_FP64 = 0
+PREC_MIXED = 1
 
 _lib = None
 
@@ -61,6 +63,7 @@ def load():
     L.nw_hs_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, vp, vp, ct.c_int]
     L.nw_hs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
     L.nw_pair_slab.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
+    L.nw_set_precision.argtypes = [vp, ct.c_int]
     L.nw_last_timings.argtypes = [vp, dp]
     L.nw_last_error.argtypes = [vp]
     L.nw_last_error.restype = ct.c_char_p
@@ -118,6 +121,10 @@ class Engine:
                 occ[i, 2 * k + 1] = v >> 64
         self._check(self.L.nw_set_refs(self.ctx, nref, nmo, _ptr(ca), _ptr(cb), _ptr(occ)))
         self.nref, self.nmo = nref, nmo
+
+    def set_precision(self, prec):
+        """PREC_FP64 or PREC_MIXED (FP32 MFMA four-index transforms)."""
+        self._check(self.L.nw_set_precision(self.ctx, int(prec)))
 
     def build(self, tol=1e-8, pair_mask=None):
         """build_wicks_shared: returns {(x, w): PairMeta} for the built pairs."""

@@ -111,3 +111,24 @@ def test_pair_mask_and_unbuilt_pair_error():
     assert np.max(np.abs(H - Ho)) < TOL
     with pytest.raises(nwick.NwickError):
         eng.hs_batched(rec, np.array([[len(dets) - 1, 0]], dtype=np.uint64))
+
+
+def test_mixed_precision_build_close_to_fp64():
+    """Mixed build (FP32 MFMA transforms): elements within 1e-6 of FP64 and the
+    lowest NOCI energy within 1 uHa; FP64 quantities (S, m) unchanged."""
+    sysm, dets = _case(16, 3, 3, 3, seed=51, orth_refs=(2,), orth_count=1, orthonormal=False, per_ref=40)
+    rec = basis.to_records(dets)
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    H64, S64 = eng.hs_matrix(rec, mode=1)
+    eng.set_precision(nwick.PREC_MIXED)
+    meta = eng.build()
+    assert meta[(0, 2)].ma == 1
+    Hmx, Smx = eng.hs_matrix(rec, mode=1)
+    assert np.max(np.abs(Smx - S64)) < 1e-14
+    assert np.max(np.abs(Hmx - H64)) < 1e-6
+    import scipy.linalg
+    e64 = scipy.linalg.eigh(H64, S64, eigvals_only=True)[0]
+    emx = scipy.linalg.eigh(Hmx, Smx, eigvals_only=True)[0]
+    print(f"mixed: max|dH|={np.max(np.abs(Hmx - H64)):.3e} dE={emx - e64:.3e}", flush=True)
+    assert abs(emx - e64) < 1e-6
