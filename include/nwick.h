@@ -105,6 +105,27 @@ int nw_hs_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t
  * ndet x ndet row-major. */
 int nw_hs_matrix(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double *H, double *S, int mode);
 
+/* Rebuild the generalised-Fock intermediates of every built pair from the
+ * spin Fock matrices fa, fb (n_bas x n_bas): replaces update_wicks_fock
+ * (src/noci/wicks.rs:290-330) / construct_f_scalar (build.rs:564-617). */
+int nw_update_fock(nw_ctx *ctx, const double *fa, const double *fb);
+
+/* Full generalised-Fock and overlap matrices over a determinant list (pairs
+ * i <= j, mirrored): F_ab = calculate_f_pair_wicks (src/noci/fock.rs:175),
+ * S_ab = calculate_s_pair_wicks (src/noci/overlap.rs:106).  M = F - E0 S is
+ * calculate_m_pair_wicks (src/noci/m.rs:160). */
+int nw_fs_matrix(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double *F, double *S, int mode);
+
+/* Batched F, S for arbitrary determinant pairs (req as in nw_hs_batched). */
+int nw_fs_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t *req,
+                  uint64_t nreq, double *f_out, double *s_out, int mode);
+
+/* Matrix-free y = (F - e0 S) x over all pairs a <= b with the mirror
+ * contribution (apply_candidate_m, src/snoci/operators.rs:375-488); if ys is
+ * not NULL also ys = S x (apply_candidate_s).  x, y, ys: ndet doubles. */
+int nw_matvec_m(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double e0, const double *x,
+                double *y, double *ys, int mode);
+
 /* Select the build arithmetic (nw_precision) for subsequent nw_build calls.
  * No reference counterpart: the reference computes in FP64 only. */
 int nw_set_precision(nw_ctx *ctx, int precision);

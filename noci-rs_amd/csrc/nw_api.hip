@@ -133,7 +133,7 @@ struct nw_ctx {
     PairInfo *d_pinfo = nullptr;
     double t_ms[3] = {0, 0, 0};
     // scratch
-    DBuf arena_desc, scratchA, scratchB, cx32, xc32, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
+    DBuf arena_desc, scratchA, scratchB, cx32, xc32, keepM, keepCX, keepXC, fockbuf, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
     std::deque<std::vector<char>> keep;
 
     ~nw_ctx()
@@ -334,9 +334,10 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         double *sig = A((size_t)2 * P * nomax), *phase = A(2 * P), *tsp = A(2 * P);
         int *mdev = (int *)A(P);
         double *Lt = A((size_t)2 * P * n * nomax), *Gt = A((size_t)2 * P * n * nomax);
-        double *M = A((size_t)4 * P * nn);          // [(p*2+sp)*2+mi][n][n]
+        // co-densities and the cx/xc contractions stay resident for nw_update_fock
+        double *M = c->keepM.get<double>((size_t)4 * P * nn);  // [(p*2+sp)*2+mi][n][n]
         double *rowc = A((size_t)2 * P * nnm), *colc = A((size_t)2 * P * nnm);
-        double *cx = A((size_t)4 * P * nnm), *xc = A((size_t)4 * P * nnm);
+        double *cx = c->keepCX.get<double>((size_t)4 * P * nnm), *xc = c->keepXC.get<double>((size_t)4 * P * nnm);
         double *MS = A((size_t)4 * P * nn), *MtS = A((size_t)4 * P * nn);
         double *Q = A((size_t)4 * P * nnm), *R = A((size_t)4 * P * nnm);
         double *Z = A((size_t)2 * P * nm2);
@@ -393,6 +394,7 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
             long long sz = pair_offsets(pi.ma, pi.mb, nm, pi.off);
             pi.base = total;
             pi.built = 1;
+            pi.bidx = p;
             total += sz;
         }
         if ((size_t)total > c->slab_cap) {
@@ -734,147 +736,295 @@ std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, Packed
     return h;
 }
 
+// Upper-triangle pairs (i <= j) of a determinant list grouped into blocks of
+// equal (parent, rank_a, rank_b) runs: calculate_matrix_elements' pair list
+// (src/noci/matrix.rs:55-59), partitioned by contraction-rank bin.
+std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vector<PackedDet> &h, bool need_hcol)
+{
+    std::vector<Run> runs;
+    for (long long i = 0; i < (long long)h.size(); ++i) {
+        const PackedDet &d = h[i];
+        if (!runs.empty() && runs.back().parent == d.parent && runs.back().ra == d.ra && runs.back().rb == d.rb)
+            runs.back().len++;
+        else
+            runs.push_back({i, 1, d.parent, d.ra, d.rb});
+    }
+    std::map<int, std::vector<EvalBlock>> bins;  // key = ((la*5+lb) << 1) | m0
+    for (size_t r1 = 0; r1 < runs.size(); ++r1)
+        for (size_t r2 = r1; r2 < runs.size(); ++r2) {
+            const Run &A = runs[r1], &B = runs[r2];
+            const int pair = A.parent * c->nref + B.parent;
+            const PairInfo &pi = c->pinfo[pair];
+            if (!pi.built)
+                throw NwError(NW_E_STATE, "reference pair (" + std::to_string(A.parent) + "," + std::to_string(B.parent) +
+                                              ") was not built");
+            const int la = A.ra + B.ra, lb = A.rb + B.rb;
+            const bool m0 = need_hcol && pi.ma == 0 && pi.mb == 0;
+            auto &v = bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)];
+            v.push_back({A.start, B.start, (int)A.len, (int)B.len, pair, r1 == r2 ? 1 : 0, 0});
+        }
+    for (auto &kv : bins) {
+        long long t = 0;
+        for (auto &b : kv.second) {
+            b.tstart = t;
+            t += (long long)((b.ni + 15) / 16) * ((b.nj + 15) / 16);
+        }
+    }
+    return bins;
+}
+
+long long tiles_of(const std::vector<EvalBlock> &v)
+{
+    const EvalBlock &b = v.back();
+    return b.tstart + (long long)((b.ni + 15) / 16) * ((b.nj + 15) / 16);
+}
+
+EvalArgs base_args(nw_ctx *c, const PackedDet *dp, int mode)
+{
+    EvalArgs a{};
+    a.dets = dp;
+    a.pairs = c->d_pinfo;
+    a.slab = c->d_slab;
+    a.nref = c->nref;
+    a.nmo = c->nmo;
+    a.mode = mode;
+    a.enuc = c->enuc;
+    a.tol = c->tol;
+    return a;
+}
+
+// Shared body of nw_hs_matrix / nw_fs_matrix.
+int eval_matrix(nw_ctx *c, int kind, const nw_det *dets, uint64_t ndet, double *A, double *B, int mode)
+{
+    if (!c->built) return fail(c, NW_E_STATE, "nw_build first");
+    if (!dets || !A || !B || ndet == 0) return fail(c, NW_E_ARG, "null arguments");
+    if (ndet > (1ull << 31)) return fail(c, NW_E_SHAPE, "too many determinants");
+    HIPCHK(hipEventRecord(c->ev[0], c->st));
+    PackedDet *dp = nullptr;
+    std::vector<PackedDet> h = pack(c, dets, ndet, dp);
+    auto bins = triangle_blocks(c, h, kind == EVAL_HS);
+    const bool adev = is_device_ptr(A), bdev = is_device_ptr(B);
+    double *dA = adev ? A : c->outH.get<double>(ndet * ndet);
+    double *dB = bdev ? B : c->outS.get<double>(ndet * ndet);
+    EvalArgs args = base_args(c, dp, mode);
+    args.A = dA;
+    args.B = dB;
+    args.ldo = (long long)ndet;
+    Uploader up(c, 64u << 20);
+    HIPCHK(hipEventRecord(c->ev[1], c->st));
+    for (auto &kv : bins) {
+        const int key = kv.first, lab = key >> 1;
+        EvalBlock *db = up.put(kv.second);
+        launch_eval_blocks(c->st, kind, lab / 5, lab % 5, key & 1, db, (int)kv.second.size(), tiles_of(kv.second), args);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[2], c->st));
+    if (!adev) HIPCHK(hipMemcpyAsync(A, dA, ndet * ndet * 8, hipMemcpyDeviceToHost, c->st));
+    if (!bdev) HIPCHK(hipMemcpyAsync(B, dB, ndet * ndet * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
+    c->t_ms[1] = 0.0;
+    c->t_ms[2] = elapsed(c->ev[1], c->ev[2]);
+    return NW_OK;
+}
+
+// Shared body of nw_hs_batched / nw_fs_batched.
+int eval_batched(nw_ctx *c, int kind, const nw_det *dets, uint64_t ndet, const uint64_t *req, uint64_t nreq, double *A,
+                 double *B, int mode)
+{
+    if (!c->built) return fail(c, NW_E_STATE, "nw_build first");
+    if (!dets || !req || !A || !B) return fail(c, NW_E_ARG, "null arguments");
+    if (nreq == 0) return NW_OK;
+    if (nreq > (1ull << 31)) return fail(c, NW_E_SHAPE, "too many requests");
+    HIPCHK(hipEventRecord(c->ev[0], c->st));
+    PackedDet *dp = nullptr;
+    std::vector<PackedDet> h = pack(c, dets, ndet, dp);
+    std::vector<uint64_t> rq(nreq * 2);
+    if (is_device_ptr(req)) {
+        HIPCHK(hipMemcpyAsync(rq.data(), req, nreq * 16, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    } else {
+        std::memcpy(rq.data(), req, nreq * 16);
+    }
+    std::vector<nw_det> hd;
+    const nw_det *hdp = dets;
+    if (is_device_ptr(dets)) {
+        hd.resize(ndet);
+        HIPCHK(hipMemcpyAsync(hd.data(), dets, ndet * sizeof(nw_det), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        hdp = hd.data();
+    }
+    // calculate_hs_pairs_wicks_batched (src/noci/hs.rs:86-109): same-parent pairs
+    // differing in more than four spin orbitals are exactly zero; the rest are
+    // grouped by contraction-rank bin and reference-pair zero class.
+    std::map<int, std::vector<EvalReq>> bins;
+    std::vector<long long> zeros;
+    for (uint64_t q = 0; q < nreq; ++q) {
+        const uint64_t a = rq[2 * q], b = rq[2 * q + 1];
+        if (a >= ndet || b >= ndet) throw NwError(NW_E_ARG, "request index out of range");
+        const nw_det &Ad = hdp[a], &Bd = hdp[b];
+        if (kind == EVAL_HS && Ad.parent == Bd.parent) {
+            int diff = __builtin_popcountll(Ad.oa[0] ^ Bd.oa[0]) + __builtin_popcountll(Ad.oa[1] ^ Bd.oa[1]) +
+                       __builtin_popcountll(Ad.ob[0] ^ Bd.ob[0]) + __builtin_popcountll(Ad.ob[1] ^ Bd.ob[1]);
+            if (diff > 4) { zeros.push_back((long long)q); continue; }
+        }
+        const int pair = (int)Ad.parent * c->nref + (int)Bd.parent;
+        const PairInfo &pi = c->pinfo[pair];
+        if (!pi.built) throw NwError(NW_E_STATE, "reference pair not built");
+        const int la = h[a].ra + h[b].ra, lb = h[a].rb + h[b].rb;
+        const bool m0 = kind == EVAL_HS && pi.ma == 0 && pi.mb == 0;
+        bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)].push_back({(int)a, (int)b, (long long)q});
+    }
+    const bool adev = is_device_ptr(A), bdev = is_device_ptr(B);
+    double *dA = adev ? A : c->outH.get<double>(nreq);
+    double *dB = bdev ? B : c->outS.get<double>(nreq);
+    Uploader up(c, std::max<size_t>(64u << 20, nreq * 40 + (1u << 20)));
+    if (!zeros.empty()) {
+        std::vector<LinDesc> lz;
+        for (long long q : zeros) {
+            lz.push_back({nullptr, nullptr, nullptr, dA + q, 0, 0, 0});
+            lz.push_back({nullptr, nullptr, nullptr, dB + q, 0, 0, 0});
+        }
+        launch_lincomb(c->st, up.put(lz), (int)lz.size(), 1);
+    }
+    EvalArgs args = base_args(c, dp, mode);
+    args.A = dA;
+    args.B = dB;
+    HIPCHK(hipEventRecord(c->ev[1], c->st));
+    for (auto &kv : bins) {
+        const int key = kv.first, lab = key >> 1;
+        EvalReq *dr = up.put(kv.second);
+        launch_eval_reqs(c->st, kind, lab / 5, lab % 5, key & 1, dr, (int)kv.second.size(), args);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[2], c->st));
+    if (!adev) HIPCHK(hipMemcpyAsync(A, dA, nreq * 8, hipMemcpyDeviceToHost, c->st));
+    if (!bdev) HIPCHK(hipMemcpyAsync(B, dB, nreq * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
+    c->t_ms[1] = 0.0;
+    c->t_ms[2] = elapsed(c->ev[1], c->ev[2]);
+    return NW_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 int nw_hs_matrix(nw_ctx *c, const nw_det *dets, uint64_t ndet, double *H, double *S, int mode)
 {
-    return guarded(c, [&]() -> int {
-        if (!c->built) return fail(c, NW_E_STATE, "nw_build first");
-        if (!dets || !H || !S || ndet == 0) return fail(c, NW_E_ARG, "null arguments");
-        if (ndet > (1ull << 31)) return fail(c, NW_E_SHAPE, "too many determinants");
-        HIPCHK(hipEventRecord(c->ev[0], c->st));
-        PackedDet *dp = nullptr;
-        std::vector<PackedDet> h = pack(c, dets, ndet, dp);
-        // runs of equal (parent, rank_a, rank_b)
-        std::vector<Run> runs;
-        for (long long i = 0; i < (long long)ndet; ++i) {
-            const PackedDet &d = h[i];
-            if (!runs.empty() && runs.back().parent == d.parent && runs.back().ra == d.ra && runs.back().rb == d.rb)
-                runs.back().len++;
-            else
-                runs.push_back({i, 1, d.parent, d.ra, d.rb});
-        }
-        std::map<int, std::vector<EvalTile>> bins;  // key = (la*5+lb)*2 + m0
-        for (size_t r1 = 0; r1 < runs.size(); ++r1)
-            for (size_t r2 = r1; r2 < runs.size(); ++r2) {
-                const Run &A = runs[r1], &B = runs[r2];
-                const int pair = A.parent * c->nref + B.parent;
-                const PairInfo &pi = c->pinfo[pair];
-                if (!pi.built) throw NwError(NW_E_STATE, "reference pair (" + std::to_string(A.parent) + "," +
-                                                             std::to_string(B.parent) + ") was not built");
-                const int la = A.ra + B.ra, lb = A.rb + B.rb;
-                const bool m0 = pi.ma == 0 && pi.mb == 0;
-                auto &v = bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)];
-                for (long long i0 = A.start; i0 < A.start + A.len; i0 += 16) {
-                    const int ni = (int)std::min<long long>(16, A.start + A.len - i0);
-                    for (long long j0 = B.start; j0 < B.start + B.len; j0 += 16) {
-                        const int nj = (int)std::min<long long>(16, B.start + B.len - j0);
-                        if (r1 == r2 && j0 + nj - 1 < i0) continue;
-                        v.push_back({(int)i0, (int)j0, ni, nj, pair, r1 == r2 ? 1 : 0});
-                    }
-                }
-            }
-        const bool hdev = is_device_ptr(H), sdev = is_device_ptr(S);
-        double *dH = hdev ? H : c->outH.get<double>(ndet * ndet);
-        double *dS = sdev ? S : c->outS.get<double>(ndet * ndet);
-        Uploader up(c, 64u << 20);
-        HIPCHK(hipEventRecord(c->ev[1], c->st));
-        for (auto &kv : bins) {
-            const int key = kv.first;
-            const bool m0 = key & 1;
-            const int lab = key >> 1, la = lab / 5, lb = lab % 5;
-            EvalTile *dt = up.put(kv.second);
-            launch_eval_tiles(c->st, la, lb, m0, mode, dt, (int)kv.second.size(), dp, c->d_pinfo, c->d_slab, nullptr,
-                              c->nmo, c->enuc, c->tol, dH, dS, (long long)ndet);
-        }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(c->ev[2], c->st));
-        if (!hdev) HIPCHK(hipMemcpyAsync(H, dH, ndet * ndet * 8, hipMemcpyDeviceToHost, c->st));
-        if (!sdev) HIPCHK(hipMemcpyAsync(S, dS, ndet * ndet * 8, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipStreamSynchronize(c->st));
-        c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
-        c->t_ms[1] = 0.0;
-        c->t_ms[2] = elapsed(c->ev[1], c->ev[2]);
-        return NW_OK;
-    });
+    return guarded(c, [&]() -> int { return eval_matrix(c, EVAL_HS, dets, ndet, H, S, mode); });
+}
+
+int nw_fs_matrix(nw_ctx *c, const nw_det *dets, uint64_t ndet, double *F, double *S, int mode)
+{
+    return guarded(c, [&]() -> int { return eval_matrix(c, EVAL_FS, dets, ndet, F, S, mode); });
 }
 
 int nw_hs_batched(nw_ctx *c, const nw_det *dets, uint64_t ndet, const uint64_t *req, uint64_t nreq, double *h_out,
                   double *s_out, int mode)
 {
+    return guarded(c, [&]() -> int { return eval_batched(c, EVAL_HS, dets, ndet, req, nreq, h_out, s_out, mode); });
+}
+
+int nw_fs_batched(nw_ctx *c, const nw_det *dets, uint64_t ndet, const uint64_t *req, uint64_t nreq, double *f_out,
+                  double *s_out, int mode)
+{
+    return guarded(c, [&]() -> int { return eval_batched(c, EVAL_FS, dets, ndet, req, nreq, f_out, s_out, mode); });
+}
+
+int nw_update_fock(nw_ctx *c, const double *fa, const double *fb)
+{
     return guarded(c, [&]() -> int {
         if (!c->built) return fail(c, NW_E_STATE, "nw_build first");
-        if (!dets || !req || !h_out || !s_out) return fail(c, NW_E_ARG, "null arguments");
-        if (nreq == 0) return NW_OK;
-        if (nreq > (1ull << 31)) return fail(c, NW_E_SHAPE, "too many requests");
+        if (!fa || !fb) return fail(c, NW_E_ARG, "null Fock matrix");
+        const int n = c->n, nm = c->nmo, nref = c->nref;
+        const size_t nn = (size_t)n * n, nnm = (size_t)n * nm;
+        double *F = c->fockbuf.get<double>(2 * nn);
+        HIPCHK(hipMemcpyAsync(F, fa, nn * 8, hipMemcpyDefault, c->st));
+        HIPCHK(hipMemcpyAsync(F + nn, fb, nn * 8, hipMemcpyDefault, c->st));
+        std::vector<int> built;
+        for (int k = 0; k < nref * nref; ++k)
+            if (c->pinfo[k].built) built.push_back(k);
+        const int P = (int)built.size();
+        int maxb = 0;
+        for (int k : built) maxb = std::max(maxb, c->pinfo[k].bidx + 1);
+        double *M = (double *)c->keepM.p, *cx = (double *)c->keepCX.p, *xc = (double *)c->keepXC.p;
+        Uploader up(c, 64u << 20);
+        double *tmp = c->scratchA.get<double>((size_t)8 * maxb * nnm);
+        double *dmeta = c->scratchB.get<double>((size_t)P * 4);
+        std::vector<GemmDesc> gfx, gff;
+        std::vector<TraceDesc> tr;
+        for (int q = 0; q < P; ++q) {
+            const PairInfo &pi = c->pinfo[built[q]];
+            const int p = pi.bidx;
+            for (int sp = 0; sp < 2; ++sp) {
+                const int pr = p * 2 + sp;
+                const double *Fs = F + sp * nn;
+                const long long *o = pi.off + (sp ? O_BB : O_AA);
+                for (int mi = 0; mi < 2; ++mi)
+                    tr.push_back({M + (size_t)(pr * 2 + mi) * nn, Fs, dmeta + (size_t)q * 4 + sp * 2 + mi, 1.0});
+                for (int mj = 0; mj < 2; ++mj)
+                    gfx.push_back({Fs, xc + (size_t)(pr * 2 + mj) * nnm, tmp + (size_t)(pr * 2 + mj) * nnm});
+                for (int mi = 0; mi < 2; ++mi)
+                    for (int mj = 0; mj < 2; ++mj)
+                        gff.push_back({tmp + (size_t)(pr * 2 + mj) * nnm, cx + (size_t)(pr * 2 + mi) * nnm,
+                                       c->d_slab + pi.base + o[O_FF + mi * 2 + mj]});
+            }
+        }
+        // f0f[mi] = sum M[b,a] F[a,b]; ff[mi][mj] = cx[mi]^T F xc[mj] (construct_f_scalar, build.rs:564-617)
+        launch_einsum(c->st, up.put(tr), (int)tr.size(), n);
+        gemm(c, up, 0, 0, n, nm, n, n, nm, nm, 1.0, 0.0, gfx);
+        gemm(c, up, 1, 0, nm, nm, n, nm, nm, nm, 1.0, 0.0, gff);
+        std::vector<double> mh((size_t)P * 4);
+        HIPCHK(hipMemcpyAsync(mh.data(), dmeta, mh.size() * 8, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (int q = 0; q < P; ++q) {
+            PairInfo &pi = c->pinfo[built[q]];
+            for (int sp = 0; sp < 2; ++sp)
+                for (int mi = 0; mi < 2; ++mi) pi.meta[(sp ? M_BB : M_AA) + M_F0F + mi] = mh[(size_t)q * 4 + sp * 2 + mi];
+        }
+        HIPCHK(hipMemcpyAsync(c->d_pinfo, c->pinfo.data(), sizeof(PairInfo) * c->pinfo.size(), hipMemcpyHostToDevice,
+                              c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        return NW_OK;
+    });
+}
+
+int nw_matvec_m(nw_ctx *c, const nw_det *dets, uint64_t ndet, double e0, const double *x, double *y, double *ys,
+                int mode)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->built) return fail(c, NW_E_STATE, "nw_build first");
+        if (!dets || !x || !y || ndet == 0) return fail(c, NW_E_ARG, "null arguments");
         HIPCHK(hipEventRecord(c->ev[0], c->st));
         PackedDet *dp = nullptr;
         std::vector<PackedDet> h = pack(c, dets, ndet, dp);
-        std::vector<uint64_t> rq(nreq * 2);
-        if (is_device_ptr(req)) {
-            HIPCHK(hipMemcpyAsync(rq.data(), req, nreq * 16, hipMemcpyDeviceToHost, c->st));
-            HIPCHK(hipStreamSynchronize(c->st));
-        } else {
-            std::memcpy(rq.data(), req, nreq * 16);
-        }
-        // Host copies of the full occupations for the same-parent shortcut.
-        std::vector<nw_det> hd;
-        const nw_det *hdp = dets;
-        if (is_device_ptr(dets)) {
-            hd.resize(ndet);
-            HIPCHK(hipMemcpyAsync(hd.data(), dets, ndet * sizeof(nw_det), hipMemcpyDeviceToHost, c->st));
-            HIPCHK(hipStreamSynchronize(c->st));
-            hdp = hd.data();
-        }
-        // calculate_hs_pairs_wicks_batched (src/noci/hs.rs:86-109): same-parent pairs
-        // differing in more than four spin orbitals are exactly zero; the rest are
-        // grouped by contraction-rank bin and reference-pair zero class.
-        std::map<int, std::vector<EvalReq>> bins;
-        std::vector<long long> zeros;
-        for (uint64_t q = 0; q < nreq; ++q) {
-            const uint64_t a = rq[2 * q], b = rq[2 * q + 1];
-            if (a >= ndet || b >= ndet) throw NwError(NW_E_ARG, "request index out of range");
-            const nw_det &A = hdp[a], &B = hdp[b];
-            if (A.parent == B.parent) {
-                int diff = __builtin_popcountll(A.oa[0] ^ B.oa[0]) + __builtin_popcountll(A.oa[1] ^ B.oa[1]) +
-                           __builtin_popcountll(A.ob[0] ^ B.ob[0]) + __builtin_popcountll(A.ob[1] ^ B.ob[1]);
-                if (diff > 4) { zeros.push_back((long long)q); continue; }
-            }
-            const int pair = (int)A.parent * c->nref + (int)B.parent;
-            const PairInfo &pi = c->pinfo[pair];
-            if (!pi.built) throw NwError(NW_E_STATE, "reference pair not built");
-            const int la = h[a].ra + h[b].ra, lb = h[a].rb + h[b].rb;
-            const bool m0 = pi.ma == 0 && pi.mb == 0;
-            bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)].push_back({(int)a, (int)b, (long long)q});
-        }
-        const bool hdev = is_device_ptr(h_out), sdev = is_device_ptr(s_out);
-        double *dH = hdev ? h_out : c->outH.get<double>(nreq);
-        double *dS = sdev ? s_out : c->outS.get<double>(nreq);
-        Uploader up(c, 64u << 20 > nreq * 40 ? 64u << 20 : nreq * 40 + (1u << 20));
-        if (!zeros.empty()) {
-            // zero outputs through the linear-combination kernel on single elements
-            std::vector<LinDesc> lz;
-            for (long long q : zeros) {
-                lz.push_back({nullptr, nullptr, nullptr, dH + q, 0, 0, 0});
-                lz.push_back({nullptr, nullptr, nullptr, dS + q, 0, 0, 0});
-            }
-            launch_lincomb(c->st, up.put(lz), (int)lz.size(), 1);
-        }
+        auto bins = triangle_blocks(c, h, false);
+        const double *dx = (const double *)to_device(c, c->outH, x, ndet * 8);
+        const bool ydev = is_device_ptr(y), ysdev = ys && is_device_ptr(ys);
+        double *dy = ydev ? y : c->outS.get<double>(2 * ndet);
+        double *dys = ys ? (ysdev ? ys : c->outS.get<double>(2 * ndet) + ndet) : nullptr;
+        std::vector<LinDesc> lz = {{nullptr, nullptr, nullptr, dy, 0, 0, 0}};
+        if (dys) lz.push_back({nullptr, nullptr, nullptr, dys, 0, 0, 0});
+        Uploader up(c, 64u << 20);
+        launch_lincomb(c->st, up.put(lz), (int)lz.size(), (long long)ndet);
+        EvalArgs args = base_args(c, dp, mode);
+        args.e0 = e0;
+        args.x = dx;
+        args.y = dy;
+        args.ys = dys;
         HIPCHK(hipEventRecord(c->ev[1], c->st));
         for (auto &kv : bins) {
-            const int key = kv.first;
-            const bool m0 = key & 1;
-            const int lab = key >> 1, la = lab / 5, lb = lab % 5;
-            EvalReq *dr = up.put(kv.second);
-            launch_eval_reqs(c->st, la, lb, m0, mode, dr, (int)kv.second.size(), dp, c->d_pinfo, c->nref, c->d_slab,
-                             nullptr, c->nmo, c->enuc, c->tol, dH, dS);
+            const int lab = kv.first >> 1;
+            EvalBlock *db = up.put(kv.second);
+            launch_eval_blocks(c->st, EVAL_MATVEC, lab / 5, lab % 5, false, db, (int)kv.second.size(),
+                               tiles_of(kv.second), args);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c->ev[2], c->st));
-        if (!hdev) HIPCHK(hipMemcpyAsync(h_out, dH, nreq * 8, hipMemcpyDeviceToHost, c->st));
-        if (!sdev) HIPCHK(hipMemcpyAsync(s_out, dS, nreq * 8, hipMemcpyDeviceToHost, c->st));
+        if (!ydev) HIPCHK(hipMemcpyAsync(y, dy, ndet * 8, hipMemcpyDeviceToHost, c->st));
+        if (ys && !ysdev) HIPCHK(hipMemcpyAsync(ys, dys, ndet * 8, hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
         c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
         c->t_ms[1] = 0.0;

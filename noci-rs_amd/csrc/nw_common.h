@@ -35,7 +35,7 @@ struct alignas(16) PairInfo {
     long long base;        // slab offset of the block
     long long off[O_LEN];  // relative offsets
     double meta[M_LEN];
-    int built, ma, mb, pad;
+    int built, ma, mb, bidx;  // bidx: position of the pair in the last build
 };
 
 // Compact determinant for evaluation kernels (ExcitationCache, src/lib.rs:128-167):
@@ -43,7 +43,8 @@ struct alignas(16) PairInfo {
 struct alignas(8) PackedDet {
     uint8_t ra, rb;
     uint16_t parent;
-    float phase;            // pha * phb, exactly +-1
+    int8_t pha, phb;        // excitation phases per spin, +-1
+    uint8_t pad[2];
     uint8_t ia[8], ib[8];
 };
 

@@ -48,7 +48,9 @@ __global__ void k_pack_dets(const RawDet *__restrict__ in, long long n, PackedDe
     p.ra = (uint8_t)(ra < 0 ? 0 : ra);
     p.rb = (uint8_t)(rb < 0 ? 0 : rb);
     p.parent = (uint16_t)d.parent;
-    p.phase = (float)(d.pha * d.phb);
+    p.pha = d.pha < 0 ? -1 : 1;
+    p.phb = d.phb < 0 ? -1 : 1;
+    p.pad[0] = p.pad[1] = 0;
     out[i] = p;
 }
 
@@ -139,7 +141,7 @@ __device__ inline long long idx4(int n, int a, int b, int c, int d)
 }
 
 struct SpinScal {
-    double f0h[2], v0[3];
+    double f0h[2], f0f[2], v0[3];
     double pref;  // phase * tilde_s_prod
     int m, nocc;
 };
@@ -150,6 +152,8 @@ __device__ inline SpinScal load_spin(const PairInfo &P, int sp)
     SpinScal s;
     s.f0h[0] = mt[M_F0H];
     s.f0h[1] = mt[M_F0H + 1];
+    s.f0f[0] = mt[M_F0F];
+    s.f0f[1] = mt[M_F0F + 1];
     s.v0[0] = mt[M_V0];
     s.v0[1] = mt[M_V0 + 1];
     s.v0[2] = mt[M_V0 + 2];
@@ -274,7 +278,7 @@ __device__ inline void hs_m0(const PairInfo &P, const double *slab, const Packed
                 h2ab += cofa[e * LA + z] * inner;
             }
     }
-    double pref = (double)bra.phase * (double)ket.phase * sa.pref * sb.pref;
+    double pref = (double)(bra.pha * bra.phb * ket.pha * ket.phb) * sa.pref * sb.pref;
     double s = pref * deta * detb;
     S = s;
     H = enuc * s + pref * (same_a * detb + same_b * deta + h2ab);
@@ -481,97 +485,255 @@ __device__ inline void hs_gen(const PairInfo &P, const double *slab, const Packe
             h2ab += contrib;
         }
     }
-    const double pref = (double)bra.phase * (double)ket.phase * sa.pref * sb.pref;
+    const double pref = (double)(bra.pha * bra.phb * ket.pha * ket.phb) * sa.pref * sb.pref;
     S = pref * s_a * s_b;
     H = pref * (h1a * s_b + h1b * s_a + 0.5 * h2aa * s_b + 0.5 * h2bb * s_a + h2ab);
     if (mode == 1) H += enuc * S;
 }
 
 // ---------------------------------------------------------------------------
-// Kernels: 16 x 16 determinant tiles (matrix mode) and request lists.
+// Overlap and generalised-Fock elements per spin: xw_overlap
+// (eval/overlap.rs:42-66, sum over L-bit distributions of det D_mix) and
+// xw_f (eval/onebody.rs:40-496, sum over (L+1)-bit distributions of
+// det f0f[m_op] - sum_b det(D, column b -> ff[m_op][m_b])).  Returned without
+// the phase * tilde_s_prod factor.
 // ---------------------------------------------------------------------------
-template <int LA, int LB, bool M0>
-__global__ __launch_bounds__(256) void k_eval_tiles(const EvalTile *__restrict__ tiles, const PackedDet *__restrict__ dets,
-                                                     const PairInfo *__restrict__ pairs, const double *__restrict__ slab0,
-                                                     int nmo, double enuc, double tol, int mode, double *__restrict__ H,
-                                                     double *__restrict__ S, long long ld)
+template <int L>
+__device__ inline void fs_spin(const double *slab, const long long *o, const SpinScal &sc, int n, const int *rows,
+                               const int *cols, const double *d0, const double *d1, double tol, int mode, double &s_out,
+                               double &f_out)
 {
-    const EvalTile t = tiles[blockIdx.x];
+    constexpr int L1 = L > 0 ? L : 1;
+    if (sc.m == 0) {
+        double cof[L1 * L1];
+        double dd = 1.0;
+        if constexpr (L > 0) dd = cofactors<L>(d0, cof);
+        double f = dd * sc.f0f[0];
+        if constexpr (L > 0) {
+            const double *ff = slab + o[O_FF];
+#pragma unroll
+            for (int z = 0; z < L; ++z)
+#pragma unroll
+                for (int e = 0; e < L; ++e) f -= cof[e * L + z] * ff[cols[z] * n + rows[e]];
+        }
+        s_out = dd;
+        f_out = f;
+        return;
+    }
+    double s = 0.0, f = 0.0;
+    if (sc.m <= L) {
+        for (unsigned bits = 0; bits < (1u << L); ++bits) {
+            if (__popc(bits) != sc.m) continue;
+            double mix[L1 * L1];
+#pragma unroll
+            for (int r = 0; r < L; ++r)
+#pragma unroll
+                for (int c = 0; c < L; ++c) mix[r * L + c] = bitk(bits, c) ? d1[r * L + c] : d0[r * L + c];
+            s += det_n<L>(mix);
+        }
+    }
+    for (unsigned bits = 0; bits < (1u << (L + 1)); ++bits) {
+        if (__popc(bits) != sc.m) continue;
+        const int mi = bitk(bits, 0);
+        double mix[L1 * L1], cof[L1 * L1];
+        double dd = 1.0;
+        if constexpr (L > 0) {
+#pragma unroll
+            for (int r = 0; r < L; ++r)
+#pragma unroll
+                for (int c = 0; c < L; ++c) mix[r * L + c] = bitk(bits >> 1, c) ? d1[r * L + c] : d0[r * L + c];
+            dd = cofactors<L>(mix, cof);
+            if (mode == 0 && !(fabs(dd) > tol)) continue;
+        }
+        double contrib = dd * sc.f0f[mi];
+#pragma unroll
+        for (int b = 0; b < L; ++b) {
+            const int mj = bitk(bits, b + 1);
+            const double *fsl = slab + o[O_FF + mi * 2 + mj];
+            double corr = 0.0;
+#pragma unroll
+            for (int r = 0; r < L; ++r) corr += (fsl[cols[b] * n + rows[r]] - mix[r * L + b]) * cof[r * L + b];
+            contrib -= dd + corr;
+        }
+        f += contrib;
+    }
+    s_out = s;
+    f_out = f;
+}
+
+// calculate_f_pair_wicks (src/noci/fock.rs:175-216) and the overlap
+// calculate_s_pair_wicks (src/noci/overlap.rs:106-141) of one pair.
+template <int LA, int LB>
+__device__ inline void fs_elem(const PairInfo &P, const double *slab, const PackedDet &bra, const PackedDet &ket,
+                               int nmo, double tol, int mode, double &F, double &S)
+{
+    constexpr int LA1 = LA > 0 ? LA : 1, LB1 = LB > 0 ? LB : 1;
+    const SpinScal sa = load_spin(P, 0), sb = load_spin(P, 1);
+    const long long *oa = P.off + O_AA, *ob = P.off + O_BB;
+    int ra[LA1], ca[LA1], rb[LB1], cb[LB1];
+    double da0[LA1 * LA1], da1[LA1 * LA1], db0[LB1 * LB1], db1[LB1 * LB1];
+    if constexpr (LA > 0) {
+        labels<LA>(bra.ia, bra.ra, ket.ia, sa.nocc, nmo - sa.nocc, ra, ca);
+        build_d<LA>(slab + oa[O_X], slab + oa[O_Y], nmo, ra, ca, da0);
+        if (sa.m > 0) build_d<LA>(slab + oa[O_X + 1], slab + oa[O_Y + 1], nmo, ra, ca, da1);
+    }
+    if constexpr (LB > 0) {
+        labels<LB>(bra.ib, bra.rb, ket.ib, sb.nocc, nmo - sb.nocc, rb, cb);
+        build_d<LB>(slab + ob[O_X], slab + ob[O_Y], nmo, rb, cb, db0);
+        if (sb.m > 0) build_d<LB>(slab + ob[O_X + 1], slab + ob[O_Y + 1], nmo, rb, cb, db1);
+    }
+    double ssa, fsa, ssb, fsb;
+    fs_spin<LA>(slab, oa, sa, nmo, ra, ca, da0, da1, tol, mode, ssa, fsa);
+    fs_spin<LB>(slab, ob, sb, nmo, rb, cb, db0, db1, tol, mode, ssb, fsb);
+    const double pha = (double)(bra.pha * ket.pha), phb = (double)(bra.phb * ket.phb);
+    const double s_a = pha * sa.pref * ssa, s_b = phb * sb.pref * ssb;
+    double f = 0.0;
+    if (s_b != 0.0) f += pha * sa.pref * fsa * s_b;
+    if (s_a != 0.0) f += phb * sb.pref * fsb * s_a;
+    F = f;
+    S = s_a * s_b;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels.  Block mode: one 256-lane workgroup = one 16 x 16 tile of a block.
+// ---------------------------------------------------------------------------
+__device__ inline int find_block(const EvalBlock *__restrict__ blocks, int nb, long long t)
+{
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blocks[mid].tstart <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int KIND, int LA, int LB, bool M0>
+__global__ __launch_bounds__(256) void k_eval_blocks(const EvalBlock *__restrict__ blocks, int nblocks, EvalArgs a)
+{
+    const long long t = blockIdx.x;
+    const EvalBlock B = blocks[find_block(blocks, nblocks, t)];
+    const int ntj = (B.nj + 15) >> 4;
+    const long long loc = t - B.tstart;
+    const int ti = (int)(loc / ntj), tj = (int)(loc % ntj);
     const int li = threadIdx.x >> 4, lj = threadIdx.x & 15;
-    if (li >= t.ni || lj >= t.nj) return;
-    const int i = t.i0 + li, j = t.j0 + lj;
-    if (t.tri && j < i) return;
-    const PairInfo &P = pairs[t.pair];
-    const double *slab = slab0 + P.base;
-    const PackedDet bra = dets[i], ket = dets[j];
-    double h, s;
-    if constexpr (M0) hs_m0<LA, LB>(P, slab, bra, ket, nmo, enuc, h, s);
-    else hs_gen<LA, LB>(P, slab, bra, ket, nmo, enuc, tol, mode, h, s);
-    H[(long long)i * ld + j] = h;
-    S[(long long)i * ld + j] = s;
-    if (i != j) {
-        H[(long long)j * ld + i] = h;
-        S[(long long)j * ld + i] = s;
+    const long long i = B.i0 + ti * 16 + li, j = B.j0 + tj * 16 + lj;
+    bool valid = (ti * 16 + li < B.ni) && (tj * 16 + lj < B.nj) && !(B.tri && j < i);
+    if constexpr (KIND != EVAL_MATVEC) {
+        if (!valid) return;
+    }
+    const PairInfo &P = a.pairs[B.pair];
+    const double *slab = a.slab + P.base;
+    double v0 = 0.0, v1 = 0.0;
+    if (valid) {
+        const PackedDet bra = a.dets[i], ket = a.dets[j];
+        if constexpr (KIND == EVAL_HS) {
+            if constexpr (M0) hs_m0<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);
+            else hs_gen<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, a.tol, a.mode, v0, v1);
+        } else {
+            fs_elem<LA, LB>(P, slab, bra, ket, a.nmo, a.tol, a.mode, v0, v1);
+        }
+    }
+    if constexpr (KIND != EVAL_MATVEC) {
+        a.A[i * a.ldo + j] = v0;
+        a.B[i * a.ldo + j] = v1;
+        if (i != j) {
+            a.A[j * a.ldo + i] = v0;
+            a.B[j * a.ldo + i] = v1;
+        }
+    } else {
+        // apply_candidate_m (src/snoci/operators.rs:430-476): y_a += M_ab x_b,
+        // y_b += M_ab x_a (b != a), M = F - e0 S; optionally the same for S.
+        __shared__ double colm[16][17], cols_[16][17];
+        const double m = valid ? v0 - a.e0 * v1 : 0.0;
+        const double s = valid ? v1 : 0.0;
+        const double xj = valid ? a.x[j] : 0.0, xi = valid ? a.x[i] : 0.0;
+        double rm = m * xj, rs = s * xj;
+#pragma unroll
+        for (int o = 8; o >= 1; o >>= 1) {
+            rm += __shfl_xor(rm, o, 16);
+            rs += __shfl_xor(rs, o, 16);
+        }
+        if (lj == 0 && ti * 16 + li < B.ni) {
+            unsafeAtomicAdd(a.y + B.i0 + ti * 16 + li, rm);
+            if (a.ys) unsafeAtomicAdd(a.ys + B.i0 + ti * 16 + li, rs);
+        }
+        const bool off = valid && i != j;
+        colm[li][lj] = off ? m * xi : 0.0;
+        cols_[li][lj] = off ? s * xi : 0.0;
+        __syncthreads();
+        if (threadIdx.x < 16 && tj * 16 + (int)threadIdx.x < B.nj) {
+            double cm = 0.0, cs = 0.0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                cm += colm[r][threadIdx.x];
+                cs += cols_[r][threadIdx.x];
+            }
+            unsafeAtomicAdd(a.y + B.j0 + tj * 16 + threadIdx.x, cm);
+            if (a.ys) unsafeAtomicAdd(a.ys + B.j0 + tj * 16 + threadIdx.x, cs);
+        }
     }
 }
 
-template <int LA, int LB, bool M0>
-__global__ __launch_bounds__(256) void k_eval_reqs(const EvalReq *__restrict__ reqs, int nreq,
-                                                    const PackedDet *__restrict__ dets, const PairInfo *__restrict__ pairs,
-                                                    int nref, const double *__restrict__ slab0, int nmo, double enuc,
-                                                    double tol, int mode, double *__restrict__ H, double *__restrict__ S)
+template <int KIND, int LA, int LB, bool M0>
+__global__ __launch_bounds__(256) void k_eval_reqs(const EvalReq *__restrict__ reqs, int nreq, EvalArgs a)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nreq) return;
     const EvalReq r = reqs[q];
-    const PackedDet bra = dets[r.a], ket = dets[r.b];
-    const PairInfo &P = pairs[(int)bra.parent * nref + ket.parent];
-    const double *slab = slab0 + P.base;
-    double h, s;
-    if constexpr (M0) hs_m0<LA, LB>(P, slab, bra, ket, nmo, enuc, h, s);
-    else hs_gen<LA, LB>(P, slab, bra, ket, nmo, enuc, tol, mode, h, s);
-    H[r.out] = h;
-    S[r.out] = s;
+    const PackedDet bra = a.dets[r.a], ket = a.dets[r.b];
+    const PairInfo &P = a.pairs[(int)bra.parent * a.nref + ket.parent];
+    const double *slab = a.slab + P.base;
+    double v0, v1;
+    if constexpr (KIND == EVAL_HS) {
+        if constexpr (M0) hs_m0<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);
+        else hs_gen<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, a.tol, a.mode, v0, v1);
+    } else {
+        fs_elem<LA, LB>(P, slab, bra, ket, a.nmo, a.tol, a.mode, v0, v1);
+    }
+    a.A[r.out] = v0;
+    a.B[r.out] = v1;
 }
 
-// (L_alpha, L_beta) with each <= 4 and sum <= 8: dispatch table
+// (L_alpha, L_beta) with each <= 4: dispatch table
 #define NW_BIN_CASES(X)                                                                               \
     X(0, 0) X(0, 1) X(0, 2) X(0, 3) X(0, 4) X(1, 0) X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(2, 0) X(2, 1) \
         X(2, 2) X(2, 3) X(2, 4) X(3, 0) X(3, 1) X(3, 2) X(3, 3) X(3, 4) X(4, 0) X(4, 1) X(4, 2) X(4, 3) X(4, 4)
 
-void launch_eval_tiles(hipStream_t st, int la, int lb, bool m0, int mode, const EvalTile *tiles, int ntiles,
-                       const PackedDet *dets, const PairInfo *pairs, const double *slab, const int *nocc_ab, int nmo,
-                       double enuc, double tol, double *H, double *S, long long ld)
+void launch_eval_blocks(hipStream_t st, int kind, int la, int lb, bool m0, const EvalBlock *blocks, int nblocks,
+                        long long ntiles, const EvalArgs &args)
 {
-    (void)nocc_ab;
     if (ntiles <= 0) return;
-    dim3 grid(ntiles), block(256);
-#define NW_T(A, B)                                                                                                   \
-    if (la == A && lb == B) {                                                                                        \
-        if (m0) hipLaunchKernelGGL((k_eval_tiles<A, B, true>), grid, block, 0, st, tiles, dets, pairs, slab, nmo, enuc, \
-                                   tol, mode, H, S, ld);                                                             \
-        else hipLaunchKernelGGL((k_eval_tiles<A, B, false>), grid, block, 0, st, tiles, dets, pairs, slab, nmo, enuc,   \
-                                tol, mode, H, S, ld);                                                                \
-        return;                                                                                                      \
+    dim3 grid((unsigned)ntiles), block(256);
+#define NW_T(A, B)                                                                                             \
+    if (la == A && lb == B) {                                                                                  \
+        if (kind == EVAL_HS) {                                                                                 \
+            if (m0) hipLaunchKernelGGL((k_eval_blocks<EVAL_HS, A, B, true>), grid, block, 0, st, blocks, nblocks, args); \
+            else hipLaunchKernelGGL((k_eval_blocks<EVAL_HS, A, B, false>), grid, block, 0, st, blocks, nblocks, args); \
+        } else if (kind == EVAL_FS) {                                                                          \
+            hipLaunchKernelGGL((k_eval_blocks<EVAL_FS, A, B, false>), grid, block, 0, st, blocks, nblocks, args);  \
+        } else {                                                                                               \
+            hipLaunchKernelGGL((k_eval_blocks<EVAL_MATVEC, A, B, false>), grid, block, 0, st, blocks, nblocks, args); \
+        }                                                                                                      \
+        return;                                                                                                \
     }
     NW_BIN_CASES(NW_T)
 #undef NW_T
 }
 
-void launch_eval_reqs(hipStream_t st, int la, int lb, bool m0, int mode, const EvalReq *reqs, int nreq,
-                      const PackedDet *dets, const PairInfo *pairs, int nref, const double *slab, const int *nocc_ab,
-                      int nmo, double enuc, double tol, double *H, double *S)
+void launch_eval_reqs(hipStream_t st, int kind, int la, int lb, bool m0, const EvalReq *reqs, int nreq,
+                      const EvalArgs &args)
 {
-    (void)nocc_ab;
     if (nreq <= 0) return;
     dim3 grid((nreq + 255) / 256), block(256);
-#define NW_R(A, B)                                                                                                   \
-    if (la == A && lb == B) {                                                                                        \
-        if (m0) hipLaunchKernelGGL((k_eval_reqs<A, B, true>), grid, block, 0, st, reqs, nreq, dets, pairs, nref, slab, \
-                                   nmo, enuc, tol, mode, H, S);                                                      \
-        else hipLaunchKernelGGL((k_eval_reqs<A, B, false>), grid, block, 0, st, reqs, nreq, dets, pairs, nref, slab,   \
-                                nmo, enuc, tol, mode, H, S);                                                         \
-        return;                                                                                                      \
+#define NW_R(A, B)                                                                                             \
+    if (la == A && lb == B) {                                                                                  \
+        if (kind == EVAL_HS) {                                                                                 \
+            if (m0) hipLaunchKernelGGL((k_eval_reqs<EVAL_HS, A, B, true>), grid, block, 0, st, reqs, nreq, args);  \
+            else hipLaunchKernelGGL((k_eval_reqs<EVAL_HS, A, B, false>), grid, block, 0, st, reqs, nreq, args);   \
+        } else {                                                                                               \
+            hipLaunchKernelGGL((k_eval_reqs<EVAL_FS, A, B, false>), grid, block, 0, st, reqs, nreq, args);        \
+        }                                                                                                      \
+        return;                                                                                                \
     }
     NW_BIN_CASES(NW_R)
 #undef NW_R

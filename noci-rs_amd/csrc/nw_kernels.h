@@ -31,21 +31,37 @@ bool launch_qtransform_f32_out64(hipStream_t st, const float *X, const float *c,
                                  int nout);
 void launch_to_f32(hipStream_t st, const double *in, float *out, long long n);
 
-// Evaluation.
-struct EvalTile {
-    int i0, j0;      // first bra / ket determinant
-    int ni, nj;      // extent (<= 16 each)
-    int pair;        // x * nref + w
-    int tri;         // 1: keep only j >= i
+// Evaluation.  Work is described by blocks: a rectangle of determinant pairs
+// (bra rows i0..i0+ni, ket columns j0..j0+nj) sharing one ordered reference
+// pair and one contraction-rank bin; tri = 1 keeps only j >= i.  Blocks of one
+// bin are launched together; tstart is the prefix sum of their 16x16 tile
+// counts, and each workgroup binary-searches its block.
+struct EvalBlock {
+    long long i0, j0;
+    int ni, nj;
+    int pair, tri;
+    long long tstart;
 };
 struct EvalReq { int a, b; long long out; };
 
+enum { EVAL_HS = 0, EVAL_FS = 1, EVAL_MATVEC = 2 };
+
+struct EvalArgs {
+    const PackedDet *dets;
+    const PairInfo *pairs;
+    const double *slab;
+    int nref, nmo, mode;
+    double enuc, tol, e0;
+    double *A, *B;        // H,S or F,S outputs (matrix: ld = ldo; batched: indexed by req.out)
+    long long ldo;
+    const double *x;      // matvec input
+    double *y, *ys;       // matvec outputs (ys may be null)
+};
+
 void launch_pack_dets(hipStream_t st, const void *dets, long long ndet, PackedDet *out, int *err);
-void launch_eval_tiles(hipStream_t st, int la, int lb, bool m0, int mode, const EvalTile *tiles, int ntiles,
-                       const PackedDet *dets, const PairInfo *pairs, const double *slab, const int *nocc_ab, int nmo,
-                       double enuc, double tol, double *H, double *S, long long ld);
-void launch_eval_reqs(hipStream_t st, int la, int lb, bool m0, int mode, const EvalReq *reqs, int nreq,
-                      const PackedDet *dets, const PairInfo *pairs, int nref, const double *slab, const int *nocc_ab,
-                      int nmo, double enuc, double tol, double *H, double *S);
+void launch_eval_blocks(hipStream_t st, int kind, int la, int lb, bool m0, const EvalBlock *blocks, int nblocks,
+                        long long ntiles, const EvalArgs &args);
+void launch_eval_reqs(hipStream_t st, int kind, int la, int lb, bool m0, const EvalReq *reqs, int nreq,
+                      const EvalArgs &args);
 
 }  // namespace nw

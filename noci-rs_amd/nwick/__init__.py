@@ -29,8 +29,9 @@ STATUS = {0: "NW_OK", 1: "NW_E_ARG", 2: "NW_E_SHAPE", 3: "NW_E_HIP", 4: "NW_E_ME
           6: "NW_E_STATE", 7: "NW_E_NONFINITE", 8: "NW_E_ORDER"}
 
 # Every symbol the header declares.
-EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matrix", "nw_set_precision",
-           "nw_pair_slab", "nw_last_timings", "nw_last_error", "nw_destroy"]
+EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matrix", "nw_update_fock",
+           "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_set_precision", "nw_pair_slab", "nw_last_timings",
+           "nw_last_error", "nw_destroy"]
 PREC_FP64 = 0
 PREC_MIXED = 1
 
@@ -64,6 +65,10 @@ def load():
     L.nw_hs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
     L.nw_pair_slab.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
     L.nw_set_precision.argtypes = [vp, ct.c_int]
+    L.nw_update_fock.argtypes = [vp, vp, vp]
+    L.nw_fs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
+    L.nw_fs_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, vp, vp, ct.c_int]
+    L.nw_matvec_m.argtypes = [vp, vp, ct.c_uint64, ct.c_double, vp, vp, vp, ct.c_int]
     L.nw_last_timings.argtypes = [vp, dp]
     L.nw_last_error.argtypes = [vp]
     L.nw_last_error.restype = ct.c_char_p
@@ -156,6 +161,37 @@ class Engine:
         self._check(self.L.nw_hs_batched(self.ctx, _ptr(dets), len(dets), _ptr(req), len(req), _ptr(h), _ptr(s),
                                          int(mode)))
         return h, s
+
+    def update_fock(self, fa, fb):
+        """update_wicks_fock: generalised-Fock intermediates from spin Fock matrices."""
+        fa = np.ascontiguousarray(fa, dtype=np.float64)
+        fb = np.ascontiguousarray(fb, dtype=np.float64)
+        self._check(self.L.nw_update_fock(self.ctx, _ptr(fa), _ptr(fb)))
+
+    @staticmethod
+    def _ndet(dets):
+        return len(dets) if not hasattr(dets, "data_ptr") else dets.numel() * dets.element_size() // DET_DTYPE.itemsize
+
+    def fs_matrix(self, dets, mode=MODE_REFERENCE, out=None):
+        n = self._ndet(dets)
+        F, S = (np.empty((n, n)), np.empty((n, n))) if out is None else out
+        self._check(self.L.nw_fs_matrix(self.ctx, _ptr(dets), n, _ptr(F), _ptr(S), int(mode)))
+        return F, S
+
+    def fs_batched(self, dets, req, mode=MODE_REFERENCE):
+        req = np.ascontiguousarray(req, dtype=np.uint64).reshape(-1, 2)
+        f = np.empty(len(req)); s = np.empty(len(req))
+        self._check(self.L.nw_fs_batched(self.ctx, _ptr(dets), self._ndet(dets), _ptr(req), len(req), _ptr(f),
+                                         _ptr(s), int(mode)))
+        return f, s
+
+    def matvec_m(self, dets, e0, x, y=None, ys=None, mode=MODE_REFERENCE):
+        """y = (F - e0 S) x (and ys = S x if ys is given); arrays may be torch device tensors."""
+        n = self._ndet(dets)
+        if y is None:
+            y = np.empty(n)
+        self._check(self.L.nw_matvec_m(self.ctx, _ptr(dets), n, float(e0), _ptr(x), _ptr(y), _ptr(ys), int(mode)))
+        return y if ys is None else (y, ys)
 
     def pair_slab(self, x, w):
         off = np.zeros(95, dtype=np.int64)

@@ -687,8 +687,8 @@ int ow_build_pair(int nb, int nm, const double *S, const double *h, const double
 /* ------------------------------------------------------------------------- */
 typedef struct {
     int n, nocc, m;
-    double tsp, phase, f0h[2], v0[3];
-    const double *x[2], *y[2], *fh[2][2], *v[2][2][2], *j[10], *hcol0;
+    double tsp, phase, f0h[2], f0f[2], v0[3];
+    const double *x[2], *y[2], *fh[2][2], *ff[2][2], *v[2][2][2], *j[10], *hcol0;
 } same_view;
 typedef struct {
     int n;
@@ -708,9 +708,11 @@ static void make_views(const double *slab, const long long *off, const double *m
         s->n = nm; s->nocc = (int)mt[M_NOCC]; s->m = (int)mt[M_M];
         s->tsp = mt[M_TSP]; s->phase = mt[M_PH];
         s->f0h[0] = mt[M_F0H]; s->f0h[1] = mt[M_F0H + 1];
+        s->f0f[0] = mt[M_F0F]; s->f0f[1] = mt[M_F0F + 1];
         for (int k = 0; k < 3; ++k) s->v0[k] = mt[M_V0 + k];
         for (int mi = 0; mi < 2; ++mi) { s->x[mi] = slab + o[O_X + mi]; s->y[mi] = slab + o[O_Y + mi]; }
         for (int k = 0; k < 4; ++k) s->fh[k / 2][k % 2] = slab + o[O_FH + k];
+        for (int k = 0; k < 4; ++k) s->ff[k / 2][k % 2] = slab + o[O_FF + k];
         for (int k = 0; k < 8; ++k) s->v[k / 4][(k / 2) % 2][k % 2] = slab + o[O_V + k];
         for (int k = 0; k < 10; ++k) s->j[k] = ptr_or_null(slab, o[O_J + k]);
         s->hcol0 = ptr_or_null(slab, o[O_HCOL0]);
@@ -1082,6 +1084,111 @@ int ow_hs(const double *slab, const long long *off, const double *meta, int nm,
         /* the reference's m>0 path omits E_nuc S (pho.rs:20555,20963) */
         if (mode == 1) *H += enuc * *S;
     }
+    return 0;
+}
+
+/* construct_f_scalar (build.rs:564-617) as applied by update_wicks_fock
+ * (src/noci/wicks.rs:290-330): generalised-Fock scalar and one-column
+ * intermediates of one ordered pair from the spin Fock matrices. */
+int ow_update_fock_pair(int nb, int nm, const double *S, const double *fa, const double *fb,
+                        const double *lca, const double *lcb, const uint64_t *loa, const uint64_t *lob,
+                        const double *gca, const double *gcb, const uint64_t *goa, const uint64_t *gob,
+                        double tol, const long long *off, double *slab, double *meta)
+{
+    u128 la = mk128(loa), lb = mk128(lob), ga = mk128(goa), gb = mk128(gob);
+    const double *F[2] = {fa, fb};
+    const double *lc[2] = {lca, lcb}, *gc[2] = {gca, gcb};
+    u128 lo[2] = {la, lb}, go[2] = {ga, gb};
+    double *tmp = (double *)calloc((size_t)nb * nm, sizeof(double));
+    double *blk = (double *)calloc((size_t)nm * nm, sizeof(double));
+    for (int sp = 0; sp < 2; ++sp) {
+        spin_core c;
+        spin_core_build(nb, nm, S, lc[sp], lo[sp], gc[sp], go[sp], tol, &c);
+        const long long *o = off + (sp ? O_BB : O_AA);
+        double *mt = meta + (sp ? M_BB : M_AA);
+        mt[M_F0F + 0] = einsum_ba_ab(nb, c.mao[0], F[sp]);
+        mt[M_F0F + 1] = einsum_ba_ab(nb, c.mao[1], F[sp]);
+        for (int mi = 0; mi < 2; ++mi)
+            for (int mj = 0; mj < 2; ++mj) {
+                mm(nb, nm, nb, F[sp], nb, 0, c.xc[mj], nm, 0, tmp, nm);
+                mm(nm, nm, nb, c.cx[mi], nm, 1, tmp, nm, 0, blk, nm);
+                write2t(slab, o[O_FF + mi * 2 + mj], blk, nm);
+            }
+        free_core(&c);
+    }
+    free(tmp); free(blk);
+    return 0;
+}
+
+/* xw_overlap (eval/overlap.rs:42-66): phase * tsp * sum over L-bit
+ * distributions with popcount m of det D_mix (no tolerance). */
+static double xw_overlap_o(const same_view *w, const prep *p)
+{
+    int l = p->l;
+    if (w->m > l) return 0.0;
+    uint64_t combos[4096];
+    int nc = m_combinations(l, w->m, combos);
+    double mix[256], acc = 0.0;
+    for (int ci = 0; ci < nc; ++ci) {
+        mix_columns(mix, p->det0, p->det1, l, combos[ci]);
+        acc += det_small(mix, l);
+    }
+    return w->phase * w->tsp * acc;
+}
+
+/* xw_f (eval/onebody.rs:40-496): phase * tsp * sum over (L+1)-bit
+ * distributions (bit 0 = operator) of det f0f[mi] - sum_b det(D, col b ->
+ * ff[mi][m_b]); m > 0 distributions with |det| <= tol are dropped in
+ * mode 0 (get_det_adjt_same, helpers.rs:521). */
+static double xw_f_o(const same_view *w, const prep *p, double tol, int mode)
+{
+    int l = p->l, n = w->n;
+    uint64_t combos[4096];
+    int nc = m_combinations(l + 1, w->m, combos);
+    double mix[256], cof[256], acc = 0.0;
+    for (int ci = 0; ci < nc; ++ci) {
+        uint64_t bits = combos[ci];
+        int mi = bit(bits, 0);
+        mix_columns(mix, p->det0, p->det1, l, bits >> 1);
+        double dd = 1.0;
+        if (l > 0) {
+            dd = adjugate_transpose(cof, mix, l);
+            if (w->m > 0 && mode == 0 && !(fabs(dd) > tol)) continue;
+        }
+        double contrib = dd * w->f0f[mi];
+        for (int b = 0; b < l; ++b) {
+            int mj = bit(bits, b + 1);
+            const double *fsl = w->ff[mi][mj];
+            int cb = p->cols[b];
+            double corr = 0.0;
+            for (int r = 0; r < l; ++r) corr += (fsl[cb * n + p->rows[r]] - mix[r * l + b]) * cof[r * l + b];
+            contrib -= dd + corr;
+        }
+        acc += contrib;
+    }
+    return w->phase * w->tsp * acc;
+}
+
+/* calculate_f_pair_wicks (src/noci/fock.rs:175-216) and
+ * calculate_s_pair_wicks (src/noci/overlap.rs:106-141) for one pair;
+ * M = F - e0 S is calculate_m_pair_wicks (src/noci/m.rs:160-197). */
+int ow_fs(const double *slab, const long long *off, const double *meta, int nm,
+          const uint64_t *xex, const uint64_t *wex, double pha, double phb, double tol, int mode,
+          double *F, double *Sout)
+{
+    same_view sv[2];
+    diff_view dv;
+    make_views(slab, off, meta, nm, sv, &dv);
+    prep pa, pb;
+    prepare_same(&sv[0], mk128(xex + 0), mk128(xex + 2), mk128(wex + 0), mk128(wex + 2), &pa);
+    prepare_same(&sv[1], mk128(xex + 4), mk128(xex + 6), mk128(wex + 4), mk128(wex + 6), &pb);
+    double sa = pha * xw_overlap_o(&sv[0], &pa);
+    double sb = phb * xw_overlap_o(&sv[1], &pb);
+    double f = 0.0;
+    if (sb != 0.0) f += pha * xw_f_o(&sv[0], &pa, tol, mode) * sb;
+    if (sa != 0.0) f += phb * xw_f_o(&sv[1], &pb, tol, mode) * sa;
+    *F = f;
+    *Sout = sa * sb;
     return 0;
 }
 

@@ -41,6 +41,9 @@ def lib():
         L.ow_excitation_phase.restype = ct.c_double
         L.ow_hs_naive.argtypes = [ct.c_int, ct.c_int, D, D, D, D, D, D, U64, U64, D, D, U64, U64,
                                   ct.c_double, ct.c_double, D, D]
+        L.ow_update_fock_pair.argtypes = [ct.c_int, ct.c_int, D, D, D, D, D, U64, U64, D, D, U64, U64,
+                                          ct.c_double, I64, D, D]
+        L.ow_fs.argtypes = [D, I64, D, ct.c_int, U64, U64, ct.c_double, ct.c_double, ct.c_double, ct.c_int, D, D]
         L.ow_meta_len.restype = ct.c_int
         L.ow_offsets_len.restype = ct.c_int
         _lib = L
@@ -121,6 +124,28 @@ class Slab:
                             tol, self.off[(x, w)].ctypes.data_as(I64), _d(sub), _d(meta))
             self.meta[(x, w)] = meta
 
+    def update_fock(self, sysm, fa, fb):
+        """update_wicks_fock (src/noci/wicks.rs:290): f0f and ff for every built pair."""
+        nb, nm = sysm.n_bas, sysm.nmo
+        oa, ob = mask2((1 << sysm.nocc_a) - 1), mask2((1 << sysm.nocc_b) - 1)
+        for (x, w) in self.pairs:
+            Cx, Cw = sysm.refs[x], sysm.refs[w]
+            sub = self.slab[self.base[(x, w)]:]
+            lib().ow_update_fock_pair(nb, nm, _d(_c(sysm.S)), _d(_c(fa)), _d(_c(fb)),
+                                      _d(_c(Cx[0])), _d(_c(Cx[1])), _u(oa), _u(ob),
+                                      _d(_c(Cw[0])), _d(_c(Cw[1])), _u(oa), _u(ob),
+                                      self.tol, self.off[(x, w)].ctypes.data_as(I64), _d(sub), _d(self.meta[(x, w)]))
+
+    def fs(self, bra, ket, mode=0):
+        """(F, S) of one determinant pair; M = F - e0 S."""
+        sub, off, meta = self.pair_view(bra.parent, ket.parent)
+        xex = np.concatenate([mask2(bra.ha), mask2(bra.pa), mask2(bra.hb), mask2(bra.pb)])
+        wex = np.concatenate([mask2(ket.ha), mask2(ket.pa), mask2(ket.hb), mask2(ket.pb)])
+        F = ct.c_double(); S = ct.c_double()
+        lib().ow_fs(_d(sub), off.ctypes.data_as(I64), _d(meta), self.nmo, _u(xex), _u(wex),
+                    bra.pha * ket.pha, bra.phb * ket.phb, self.tol, mode, ct.byref(F), ct.byref(S))
+        return F.value, S.value
+
     def pair_view(self, x, w):
         b = self.base[(x, w)]
         return self.slab[b:], self.off[(x, w)], self.meta[(x, w)]
@@ -154,6 +179,41 @@ def hs_naive(sysm, bra, ket, tol=1e-8, eri_asym=None):
                       _d(_c(Cg[0])), _d(_c(Cg[1])), _u(mask2(ket.oa)), _u(mask2(ket.ob)),
                       sysm.enuc, tol, ct.byref(H), ct.byref(S))
     return H.value, S.value
+
+
+def fs_matrix(slab: Slab, dets, mode=0):
+    n = len(dets)
+    F = np.zeros((n, n)); S = np.zeros((n, n))
+    for i in range(n):
+        for j in range(i, n):
+            f, s = slab.fs(dets[i], dets[j], mode)
+            F[i, j] = F[j, i] = f
+            S[i, j] = S[j, i] = s
+    return F, S
+
+
+def fs_naive(sysm, bra, ket, fa, fb, tol=1e-8):
+    """One-body generalised Slater-Condon route for F (one_electron_scalar,
+    src/noci/naive.rs:270 via calculate_m_pair_naive, src/noci/m.rs:114):
+    F = S_b f_a + S_a f_b on the excited determinants."""
+    import numpy as _np
+    res = []
+    for sp, (Fm, occ_l, occ_g) in enumerate(((fa, bra.oa, ket.oa), (fb, bra.ob, ket.ob))):
+        Cl = sysm.refs[bra.parent][sp][:, [i for i in range(sysm.nmo) if occ_l >> i & 1]]
+        Cg = sysm.refs[ket.parent][sp][:, [i for i in range(sysm.nmo) if occ_g >> i & 1]]
+        Socc = Cl.T @ sysm.S @ Cg
+        s = _np.linalg.det(Socc)
+        # one-body transition element <l|f|g> = sum_ij (Cl^T F Cg)_ij adj(Socc)_ji
+        A = Cl.T @ Fm @ Cg
+        n = Socc.shape[0]
+        cof = _np.zeros_like(Socc)
+        for i in range(n):
+            for j in range(n):
+                mnr = _np.delete(_np.delete(Socc, i, 0), j, 1)
+                cof[i, j] = (-1) ** (i + j) * (_np.linalg.det(mnr) if n > 1 else 1.0)
+        res.append((s, float(_np.sum(A * cof))))
+    (sa, fa_), (sb, fb_) = res
+    return sb * fa_ + sa * fb_, sa * sb
 
 
 def hs_matrix(slab: Slab, dets, symmetric=True, mode=0):

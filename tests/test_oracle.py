@@ -157,3 +157,23 @@ def test_generation_order_matches_reference_enumeration():
     sub = basis.random_excitations(rng, 8, 3, 2, {"a": 3, "b": 3, "aa": 5, "bb": 1, "ab": 6})
     idx = [pos[e] for e in sub]
     assert idx == sorted(idx)
+
+
+def test_fock_elements_vs_generalised_slater_condon():
+    """Generalised-Fock and overlap elements (corrected mode) against the
+    one-body Loewdin route on the excited determinants, incl. m = 1 pairs."""
+    for orth in ((), (2,)):
+        sysm = synth.make_system(10, 3, 2, 3, seed=4, orth_refs=orth, orthonormal_ao=False, n_aux=20)
+        rng = np.random.default_rng(0)
+        fa = rng.normal(size=(10, 10)); fa = fa + fa.T
+        fb = rng.normal(size=(10, 10)); fb = fb + fb.T
+        slab = O.Slab(sysm)
+        slab.update_fock(sysm, fa, fb)
+        dets = basis.excited_basis(3, 10, 3, 2, per_ref=15, seed=2, include_refs=True)
+        for i in range(0, len(dets), 3):
+            for j in range(0, len(dets), 2):
+                f, s = slab.fs(dets[i], dets[j], 1)
+                fn, sn = O.fs_naive(sysm, dets[i], dets[j], fa, fb)
+                assert abs(f - fn) < 1e-10 and abs(s - sn) < 1e-10
+                if slab.m[(dets[i].parent, dets[j].parent)] == (0, 0):
+                    assert slab.fs(dets[i], dets[j], 0) == (f, s)
