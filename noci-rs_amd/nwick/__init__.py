@@ -56,6 +56,12 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise NwickError(3, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    try:
+        # One HIP runtime per process: when torch is present its bundled
+        # libamdhip64.so.7 must be the one libnwick binds to, so load it first.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ct.CDLL(LIB_PATH)
     vp, dp, u64p = ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_uint64)
     L.nw_create.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, vp, vp, vp, ct.c_double]

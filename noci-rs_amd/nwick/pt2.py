@@ -1,0 +1,264 @@
+"""NOCI-PT2 on the GPU: projected shifted generalised-Fock operator and GMRES.
+
+Mirrors the reference's PT2 step (src/snoci/step.rs:178-440):
+
+  |Omega_a> = |Phi_a> - |Psi_0><Psi_0|Phi_a>
+  M^Omega x = M x - f_a0 (s_0a . x) - s_a0 (f_0a . x) + 2 E0 s_a0 (s_0a . x)
+              (apply_omega_m / apply_projection_correction, operators.rs:696-955)
+  M_ab = F_ab - E0 S_ab, evaluated on the fly by the engine (nw_matvec_m)
+  solve M^Omega a = -V^Omega with right-preconditioned restarted GMRES
+  (gmres.rs:406-544), E_PT2 = a.M^Omega a + a.V + V.a  (step.rs:414-429).
+
+Everything that scales with the candidate count lives on the GPU in FP64:
+the matvec (HIP kernels through the C ABI) and the Krylov vectors, Givens
+rotations and the true residual (torch tensors on the device).  The NOCI
+density and Fock matrices of the small current space are host-side set-up,
+restated from src/noci/naive.rs:231 (noci_density) and src/scf/kernels.rs:46.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from .basis import to_records
+
+SMALL = 1e-14
+
+
+def general_evp(H, S, tol=1e-8):
+    """Lowest-first generalised eigenpairs by canonical orthogonalisation,
+    discarding overlap eigenvalues <= tol (src/maths/linalg.rs:239)."""
+    w, U = np.linalg.eigh(S)
+    keep = w > tol
+    X = U[:, keep] / np.sqrt(w[keep])
+    e, v = np.linalg.eigh(X.T @ H @ X)
+    return e, X @ v
+
+
+def _pair_density(Cl, Cg, S, tol):
+    """Loewdin-paired transition density (naive.rs:73-229): returns
+    (s_pair, rho) with rho = phase * s_red * (W or P) as pair_density."""
+    Socc = Cl.T @ S @ Cg
+    U, s, Vt = np.linalg.svd(Socc)
+    V = Vt.T
+    phase = np.linalg.det(U) * np.linalg.det(V)
+    zeros = [k for k in range(len(s)) if abs(s[k]) <= tol]
+    s_red = float(np.prod([x for x in s if abs(x) > tol])) if len(s) else 1.0
+    lt, gt = Cl @ U, Cg @ V
+    s_pair = phase * float(np.prod(s)) if not zeros else 0.0
+    if len(zeros) == 0:
+        w = np.array([1.0 / x if abs(x) > tol else 0.0 for x in s])
+        rho = (gt * w) @ lt.T
+    elif len(zeros) == 1:
+        k = zeros[0]
+        rho = np.outer(gt[:, k], lt[:, k])
+    else:
+        rho = np.zeros((Cl.shape[0], Cl.shape[0]))
+    return s_pair, phase * s_red * rho
+
+
+def noci_density(sysm, dets, c, tol=1e-8):
+    """noci_density (src/noci/naive.rs:231-289) for a real NOCI state."""
+    n = sysm.n_bas
+    Da = np.zeros((n, n)); Db = np.zeros((n, n))
+
+    def occ(d, sp):
+        C = sysm.refs[d.parent][sp]
+        o = d.oa if sp == 0 else d.ob
+        return C[:, [i for i in range(sysm.nmo) if o >> i & 1]]
+
+    for i, di in enumerate(dets):
+        for j, dj in enumerate(dets):
+            sa, ra = _pair_density(occ(di, 0), occ(dj, 0), sysm.S, tol)
+            sb, rb = _pair_density(occ(di, 1), occ(dj, 1), sysm.S, tol)
+            ph = (di.pha * dj.pha) * (di.phb * dj.phb)
+            cij = c[i] * c[j] * ph
+            Da += cij * sb * ra
+            Db += cij * sa * rb
+    return Da, Db
+
+
+def fock(h, eri, Da, Db):
+    """fock (src/scf/kernels.rs:46): F_sigma = h + J[Da + Db] - K[D_sigma]."""
+    D = Da + Db
+    J = np.einsum("pqrs,rs->pq", eri, D)
+    Ka = np.einsum("prqs,rs->pq", eri, Da)
+    Kb = np.einsum("prqs,rs->pq", eri, Db)
+    Fa, Fb = h + J - Ka, h + J - Kb
+    # the reference fills q <= p and mirrors
+    return np.tril(Fa) + np.tril(Fa, -1).T, np.tril(Fb) + np.tril(Fb, -1).T
+
+
+class Preconditioner:
+    """Diagonal or rank-2 Woodbury preconditioner (src/snoci/types.rs:161-279), real case."""
+
+    def __init__(self, torch, m_diag, proj, kind="diag"):
+        self.t = torch
+        dmax = float(m_diag.abs().max())
+        dfloor = max(1e-12 * dmax, 1e-14)
+        self.dinv = torch.where(m_diag.abs() > dfloor, 1.0 / m_diag, m_diag / (dfloor * dfloor))
+        self.active = False
+        if kind != "diag":
+            u0 = -proj["f_a0"] + 2.0 * proj["e0"] * proj["s_a0"]
+            u1 = -proj["s_a0"]
+            self.v0, self.v1 = proj["s_0a"], proj["f_0a"]
+            self.z0, self.z1 = self.dinv * u0, self.dinv * u1
+            c00 = 1.0 + float(self.v0 @ self.z0); c01 = float(self.v0 @ self.z1)
+            c10 = float(self.v1 @ self.z0); c11 = 1.0 + float(self.v1 @ self.z1)
+            det = c00 * c11 - c01 * c10
+            self.active = abs(det) > 1e-14
+            if self.active:
+                self.w = (c11 / det, -c01 / det, -c10 / det, c00 / det)
+
+    def __call__(self, v):
+        y = v * self.dinv
+        if not self.active:
+            return y
+        t0, t1 = float(self.v0 @ y), float(self.v1 @ y)
+        w00, w01, w10, w11 = self.w
+        return y - self.z0 * (w00 * t0 + w01 * t1) - self.z1 * (w10 * t0 + w11 * t1)
+
+
+def gmres(torch, apply, precondition, b, res_tol=1e-8, restart=30, max_iter=300, log=None):
+    """Right-preconditioned restarted GMRES (src/snoci/gmres.rs:406-544):
+    modified Gram-Schmidt with two passes, Givens rotations, convergence on
+    the RMS of the true residual ||b - A x|| / sqrt(n) recomputed at restarts."""
+    n = b.shape[0]
+    rms = math.sqrt(n)
+    x = torch.zeros_like(b)
+    rtrue = b - apply(x)
+    res = float(torch.linalg.vector_norm(rtrue)) / rms
+    total = 0
+    if res <= res_tol:
+        return x, res, 0, True
+    while total < max_iter:
+        beta = float(torch.linalg.vector_norm(rtrue))
+        if beta <= SMALL:
+            return x, beta / rms, total, beta / rms <= res_tol
+        inner = min(restart, max_iter - total)
+        q = [rtrue / beta]
+        Hm = np.zeros((inner + 1, inner))
+        cs = np.zeros(inner); sn = np.zeros(inner)
+        g = np.zeros(inner + 1); g[0] = beta
+        kfinal = 0
+        for k in range(inner):
+            w = apply(precondition(q[k]))
+            for _ in range(2):
+                for j in range(k + 1):
+                    hjk = float(q[j] @ w)
+                    Hm[j, k] += hjk
+                    w = w - hjk * q[j]
+            hn = float(torch.linalg.vector_norm(w))
+            Hm[k + 1, k] = hn
+            if hn > SMALL:
+                q.append(w / hn)
+            for j in range(k):
+                tmp = cs[j] * Hm[j, k] + sn[j] * Hm[j + 1, k]
+                Hm[j + 1, k] = -sn[j] * Hm[j, k] + cs[j] * Hm[j + 1, k]
+                Hm[j, k] = tmp
+            xk, yk = Hm[k, k], Hm[k + 1, k]
+            den = math.hypot(xk, yk)
+            if den > SMALL:
+                cs[k] = abs(xk) / den
+                sn[k] = cs[k] * yk / xk if abs(xk) > SMALL else 1.0
+            else:
+                cs[k], sn[k] = 1.0, 0.0
+            h0, h1 = Hm[k, k], Hm[k + 1, k]
+            Hm[k, k] = cs[k] * h0 + sn[k] * h1
+            Hm[k + 1, k] = -sn[k] * h0 + cs[k] * h1
+            g0, g1 = g[k], g[k + 1]
+            g[k] = cs[k] * g0 + sn[k] * g1
+            g[k + 1] = -sn[k] * g0 + cs[k] * g1
+            kfinal = k + 1
+            est = abs(g[k + 1]) / rms
+            if log:
+                log(f"  gmres iter {total + k + 1}: residual estimate {est:.3e}")
+            if est <= res_tol or hn <= SMALL:
+                break
+        y = np.zeros(kfinal)
+        for i in reversed(range(kfinal)):
+            rhs = g[i] - sum(Hm[i, j] * y[j] for j in range(i + 1, kfinal))
+            y[i] = rhs / Hm[i, i] if abs(Hm[i, i]) > SMALL else 0.0
+        for j in range(kfinal):
+            x = x + y[j] * precondition(q[j])
+        total += kfinal
+        rtrue = b - apply(x)
+        res = float(torch.linalg.vector_norm(rtrue)) / rms
+        if res <= res_tol:
+            return x, res, total, True
+    return x, res, total, res <= res_tol
+
+
+class PT2:
+    """NOCI-PT2 correction of a current NOCI space over a candidate space.
+
+    eng must hold a build covering every ordered reference pair the
+    candidate-current and candidate-candidate pairs touch."""
+
+    def __init__(self, eng, sysm, current, candidates, tol=1e-8, mode=0, device="cuda:0"):
+        import torch
+        self.torch, self.eng, self.sysm = torch, eng, sysm
+        self.current, self.candidates = list(current), list(candidates)
+        self.tol, self.mode = tol, mode
+        self.dev = torch.device(device)
+        self.timings = {}
+
+    def setup(self):
+        torch, eng = self.torch, self.eng
+        t0 = time.perf_counter()
+        cur_rec = to_records(self.current)
+        Hc, Sc = eng.hs_matrix(cur_rec, mode=self.mode)
+        e, C = general_evp(Hc, Sc, self.tol)
+        self.ecurrent, c = float(e[0]), C[:, 0]
+        Da, Db = noci_density(self.sysm, self.current, c, self.tol)
+        fa, fb = fock(self.sysm.h, self.sysm.eri, Da, Db)
+        eng.update_fock(fa, fb)
+        Fc, _ = eng.fs_matrix(cur_rec, mode=self.mode)
+        self.e0 = float(c @ Fc @ c)
+        # candidate-current blocks through one combined determinant list
+        allrec = to_records(self.current + self.candidates)
+        ncur, nc = len(self.current), len(self.candidates)
+        a_idx = np.repeat(np.arange(ncur, ncur + nc), ncur)
+        i_idx = np.tile(np.arange(ncur), nc)
+        req = np.stack([a_idx, i_idx], axis=1).astype(np.uint64)
+        f_ai, s_ai = eng.fs_batched(allrec, req, mode=self.mode)
+        h_ai, _ = eng.hs_batched(allrec, req, mode=self.mode)
+        F_ai, S_ai, H_ai = (v.reshape(nc, ncur) for v in (f_ai, s_ai, h_ai))
+        diag = np.stack([np.arange(ncur, ncur + nc)] * 2, axis=1).astype(np.uint64)
+        f_aa, s_aa = eng.fs_batched(allrec, diag, mode=self.mode)
+        T = lambda v: torch.as_tensor(np.ascontiguousarray(v), dtype=torch.float64, device=self.dev)
+        s_a0, f_a0 = S_ai @ c, F_ai @ c
+        self.proj = {"e0": self.e0, "s_a0": T(s_a0), "s_0a": T(s_a0), "f_a0": T(f_a0), "f_0a": T(f_a0)}
+        self.v_omega = T(H_ai @ c - self.ecurrent * s_a0)
+        self.m_diag = T(f_aa - self.e0 * s_aa)
+        self.cand_rec = torch.from_numpy(to_records(self.candidates).view(np.uint8).copy()).to(self.dev)
+        self.y = torch.empty(nc, dtype=torch.float64, device=self.dev)
+        self.timings["setup_s"] = time.perf_counter() - t0
+
+    def apply(self, x):
+        """M^Omega x (apply_omega_m with the projection correction)."""
+        x = x.contiguous()
+        p = self.proj
+        sx, fx = float(p["s_0a"] @ x), float(p["f_0a"] @ x)
+        y = self.torch.empty_like(self.y)
+        self.eng.matvec_m(self.cand_rec, self.e0, x, y=y, mode=self.mode)
+        self.nmatvec += 1
+        return y - p["f_a0"] * sx - p["s_a0"] * fx + 2.0 * self.e0 * p["s_a0"] * sx
+
+    def solve(self, res_tol=1e-8, restart=30, max_iter=300, preconditioner="diag", log=None):
+        torch = self.torch
+        self.nmatvec = 0
+        prec = Preconditioner(torch, self.m_diag, self.proj, preconditioner)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        a, res, iters, conv = gmres(torch, self.apply, prec, -self.v_omega, res_tol, restart, max_iter, log)
+        ma = self.apply(a)
+        torch.cuda.synchronize()
+        ept2 = float(a @ ma) + 2.0 * float(a @ self.v_omega)
+        self.timings["gmres_s"] = time.perf_counter() - t0
+        return {"e_noci": self.ecurrent, "e0": self.e0, "ept2": ept2, "e_total": self.ecurrent + ept2,
+                "residual_rms": res, "iterations": iters, "converged": conv, "matvecs": self.nmatvec,
+                "a": a}
+

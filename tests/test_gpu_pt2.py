@@ -72,3 +72,47 @@ def test_fock_refresh_replaces_previous(pt2_case):
     Fo, _ = O.fs_matrix(slab, dets, mode=1)
     assert np.max(np.abs(F1 - Fo)) < TOL
     assert np.max(np.abs(F2 - F1)) > 1e-3
+
+
+def test_pt2_gmres_vs_dense_solve():
+    """GPU PT2 (matrix-free matvec + GPU GMRES) against a dense solve built
+    from the oracle's F, S, H blocks with the same Fock matrices."""
+    from nwick.pt2 import PT2, fock, general_evp, noci_density
+    sysm = synth.make_system(10, 2, 2, 3, seed=71, orthonormal_ao=False, n_aux=20)
+    refs = basis.excited_basis(3, 10, 2, 2, per_ref=0, include_refs=True, counts={})
+    allx = basis.excited_basis(3, 10, 2, 2, per_ref=None, include_refs=False)
+    # ref-0 singles and doubles plus ref-1 singles: a linearly independent candidate space
+    cands = [d for d in allx if d.parent == 0 or (d.parent == 1 and d.rank_a + d.rank_b == 1)]
+    print(f"pt2 dense check: ncur={len(refs)} ncand={len(cands)}", flush=True)
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    pt = PT2(eng, sysm, refs, cands, mode=1)
+    pt.setup()
+    # the synthetic Fock operator makes M^Omega indefinite (cond ~2e4): restarted
+    # GMRES(30) stagnates, so the test runs unrestarted GMRES (restart >= n)
+    r = pt.solve(res_tol=1e-10, restart=400, max_iter=800)
+    print(f"pt2 gmres: converged={r['converged']} residual={r['residual_rms']:.3e} iters={r['iterations']}", flush=True)
+    assert r["converged"]
+    # dense reference
+    slab = O.Slab(sysm)
+    Hc, Sc = O.hs_matrix(slab, refs, mode=1)
+    e, C = general_evp(Hc, Sc)
+    c = C[:, 0]
+    fa, fb = fock(sysm.h, sysm.eri, *noci_density(sysm, refs, c))
+    slab.update_fock(sysm, fa, fb)
+    Fc, _ = O.fs_matrix(slab, refs, mode=1)
+    e0 = c @ Fc @ c
+    allm = refs + cands
+    nc, ncur = len(cands), len(refs)
+    Fall, Sall = O.fs_matrix(slab, allm, mode=1)
+    Hall, _ = O.hs_matrix(slab, allm, mode=1)
+    F_ai, S_ai, H_ai = Fall[ncur:, :ncur], Sall[ncur:, :ncur], Hall[ncur:, :ncur]
+    Faa, Saa = Fall[ncur:, ncur:], Sall[ncur:, ncur:]
+    s0, f0 = S_ai @ c, F_ai @ c
+    M = Faa - e0 * Saa - np.outer(f0, s0) - np.outer(s0, f0) + 2 * e0 * np.outer(s0, s0)
+    v = H_ai @ c - e[0] * s0
+    a = np.linalg.solve(M, -v)
+    ept2 = a @ M @ a + 2 * a @ v
+    print(f"pt2: gpu {r['ept2']:.12f} dense {ept2:.12f} iters {r['iterations']}", flush=True)
+    assert abs(r["e0"] - e0) < 1e-10 and abs(r["e_noci"] - e[0]) < 1e-10
+    assert abs(r["ept2"] - ept2) < 1e-8
