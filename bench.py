@@ -78,6 +78,58 @@ def cpu_baseline(sysm, dets, tol, mode):
                       f"{len(bra)}x{len(ket)} determinant pairs ({t3 - t2:.2f} s), OpenMP threads={threads}"}
 
 
+def pt2_workload(args):
+    """NOCI-PT2 matvec sized like F2/cc-pVDZ: n_bas = 28, 9 + 9 electrons,
+    6 references (kappa ~ N(0, 0.4)), candidates = all singles + 20,000 seeded
+    doubles per reference; current space = the references.  Times the
+    matrix-free (F - E0 S) x product over all candidate pairs."""
+    import numpy as np
+    import torch
+
+    import nwick
+    from nwick import basis, synth
+    from nwick.pt2 import PT2
+
+    sysm = synth.make_system(28, 9, 9, 6, seed=args.seed, kappa_sigma=0.4)
+    refs = basis.excited_basis(6, 28, 9, 9, per_ref=0, include_refs=True, counts={})
+    rng = np.random.default_rng(args.seed)
+    cands = []
+    for r in range(6):
+        singles = [e for e in basis.all_excitations(28, 9, 9, orders=(1,))]
+        doubles = basis.random_excitations(rng, 28, 9, 9, {"aa": 5000, "bb": 5000, "ab": 10000})
+        exc = sorted(singles + doubles, key=lambda e: basis.generation_key(9, 9, *e))
+        cands += [basis.Det(r, (1 << 9) - 1, (1 << 9) - 1, *e) for e in exc]
+    n = len(cands)
+    log(f"pt2 workload: {n} candidates, {n * (n + 1) // 2} pairs per matvec")
+    eng = nwick.Engine.from_system(sysm, device=0)
+    eng.build()
+    pt = PT2(eng, sysm, refs, cands, mode=args.mode)
+    pt.setup()
+    log(f"pt2 setup {pt.timings['setup_s']:.1f} s, E_NOCI {pt.ecurrent:.10f}, E0 {pt.e0:.10f}")
+    x = torch.randn(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        eng.matvec_m(pt.cand_rec, pt.e0, x, y=y, mode=args.mode)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kern = []
+    for k in range(args.steps):
+        eng.matvec_m(pt.cand_rec, pt.e0, x, y=y, mode=args.mode)
+        kern.append(eng.timings()["eval_ms"])
+        log(f"matvec {k}: {kern[-1]:.1f} ms kernels")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    pairs = n * (n + 1) // 2
+    out = {"metric": "NOCI-PT2 matrix-free M = F - E0 S matvec, pair elements/s", "value": pairs * args.steps / el,
+           "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "NOCI-PT2 matvec, n_bas=28, nocc=9+9, 6 refs, all singles + 20000 doubles "
+                                  "per ref", "candidates": n, "mode": args.mode},
+           "kernel_ms": float(np.mean(kern)), "e_noci": pt.ecurrent, "e0": pt.e0}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,11 +140,15 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
                     help="mixed: also run the mixed-precision build and report it when |dE| < 1 uHa")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--workload", choices=["wick", "pt2"], default="wick",
+                    help="wick: batched H,S elements (headline); pt2: NOCI-PT2 matvec")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the engine has no multi-GPU path in this build "
               "(sharding is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "pt2":
+        return pt2_workload(args)
 
     import numpy as np
     import torch
