@@ -212,6 +212,7 @@ class PT2:
         Hc, Sc = eng.hs_matrix(cur_rec, mode=self.mode)
         e, C = general_evp(Hc, Sc, self.tol)
         self.ecurrent, c = float(e[0]), C[:, 0]
+        self.coeffs = c
         Da, Db = noci_density(self.sysm, self.current, c, self.tol)
         fa, fb = fock(self.sysm.h, self.sysm.eri, Da, Db)
         eng.update_fock(fa, fb)
