@@ -143,6 +143,51 @@ int nw_last_timings(nw_ctx *ctx, double *ms3);
 const char *nw_last_error(nw_ctx *ctx);
 void nw_destroy(nw_ctx *ctx);
 
+/* ---------------------------------------------------------------------------
+ * Multi-system batches.  No single reference counterpart: noci-rs evaluates
+ * one system per build_noci_hs call (src/noci/matrix.rs:214, pair list and
+ * Rayon loop of calculate_matrix_elements :39-81).  A batch takes several
+ * built contexts (each keeps its own slab; none may be rebuilt or destroyed
+ * while the batch exists) with one determinant list each, and evaluates the
+ * upper-triangle H, S of every system in one interleaved plan of persistent
+ * fused launches, captured into a HIP graph and replayed.
+ * ------------------------------------------------------------------------- */
+typedef struct nw_batch nw_batch;
+
+enum nw_batch_flags { NW_BATCH_TUNE = 1, NW_BATCH_GRAPH = 2 };
+/* Kernel grouping of the plan: one launch per contraction-rank code, per
+ * register class of codes, or a single launch for every code. */
+enum nw_batch_grouping { NW_GROUP_CODE = 0, NW_GROUP_CLASS = 1, NW_GROUP_ALL = 2 };
+
+/* One tuned bucket: systems with n_occ = nocc (max over spins) and
+ * contraction-rank code ((l_alpha * 5 + l_beta) << 1 | m0).  shape = log2 of
+ * the tile width TJ (tiles are 256/TJ bra x TJ ket determinants). */
+typedef struct nw_tune_entry {
+    int32_t nocc, code, shape, pad;
+    double ms_best, ms_default; /* sampled bucket time with the chosen / 16x16 tiles */
+} nw_tune_entry;
+
+int nw_batch_create(nw_batch **out, int device);
+/* Add one system: a built context and its determinants (as nw_hs_matrix). */
+int nw_batch_add(nw_batch *b, nw_ctx *ctx, const nw_det *dets, uint64_t ndet, int mode, int *index);
+/* Plan the batch.  NW_BATCH_TUNE times tile shapes per bucket and then the
+ * schedule (grouping, static/dynamic distribution, persistent grid) on the
+ * batch itself; NW_BATCH_GRAPH captures the plan into a HIP graph. */
+int nw_batch_plan(nw_batch *b, int flags);
+/* Evaluate every system (graph replay when captured); blocks until done. */
+int nw_batch_run(nw_batch *b);
+/* Copy system `index`'s ndet x ndet H and S (host or device buffers). */
+int nw_batch_result(nw_batch *b, int index, double *H, double *S);
+/* The schedule in force and the tuner's per-bucket report. */
+int nw_batch_schedule(nw_batch *b, int *grouping, int *dyn, int *grid_per_cu, nw_tune_entry *entries, int cap,
+                      int *count);
+/* Impose a schedule (e.g. one cached from an earlier tuning run). */
+int nw_batch_set_schedule(nw_batch *b, int grouping, int dyn, int grid_per_cu, const nw_tune_entry *entries, int n);
+/* Device time of the last nw_batch_run and the number of pairs it evaluated. */
+int nw_batch_last_ms(nw_batch *b, double *ms, long long *npairs);
+const char *nw_batch_last_error(nw_batch *b);
+void nw_batch_destroy(nw_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,202 @@
+// Internal host-side definitions shared by the C ABI translation units:
+// error handling, device buffers, the context object and the determinant
+// packing / pair-block planning used by every evaluation entry point.
+#pragma once
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nwick.h"
+#include "nw_kernels.h"
+
+namespace nwi {
+using namespace nw;
+
+
+struct NwError : std::runtime_error {
+    int code;
+    NwError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            throw NwError(NW_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+template <class T>
+T *dalloc(size_t n)
+{
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw NwError(NW_E_MEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    }
+    return (T *)p;
+}
+
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    template <class T>
+    T *get(size_t n)
+    {
+        size_t need = std::max<size_t>(n * sizeof(T), 8);
+        if (need > bytes) {
+            if (p) HIPCHK(hipFree(p));
+            p = nullptr;
+            bytes = 0;
+            HIPCHK(hipMalloc(&p, need));
+            bytes = need;
+        }
+        return (T *)p;
+    }
+};
+
+inline bool is_device_ptr(const void *p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice;
+}
+
+}  // namespace nwi
+
+
+struct nw_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[4] = {};
+    std::mutex mu;
+    std::string err;
+    int n = 0;
+    double enuc = 0.0;
+    double *d_S = nullptr, *d_h = nullptr, *d_eri = nullptr, *d_easym = nullptr;
+    float *d_eri32 = nullptr, *d_easym32 = nullptr;  // FP32 copies for the mixed build
+    int precision = NW_PREC_FP64;
+    int nref = 0, nmo = 0, nocc[2] = {0, 0};
+    double *d_c[2] = {nullptr, nullptr};  // nref x n x nmo per spin
+    // build state
+    bool built = false;
+    double tol = 1e-8;
+    double *d_slab = nullptr;
+    size_t slab_cap = 0;
+    std::vector<nw::PairInfo> pinfo;
+    nw::PairInfo *d_pinfo = nullptr;
+    double t_ms[3] = {0, 0, 0};
+    // scratch
+    nwi::DBuf arena_desc, scratchA, scratchB, cx32, xc32, keepM, keepCX, keepXC, fockbuf, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
+    std::deque<std::vector<char>> keep;
+
+    ~nw_ctx()
+    {
+        (void)hipSetDevice(device);
+        if (st) (void)hipStreamSynchronize(st);
+        for (double *p : {d_S, d_h, d_eri, d_easym, d_c[0], d_c[1], d_slab})
+            if (p) (void)hipFree(p);
+        for (float *p : {d_eri32, d_easym32})
+            if (p) (void)hipFree(p);
+        if (d_pinfo) (void)hipFree(d_pinfo);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+
+namespace nwi {
+
+
+// Upload host descriptor arrays into a per-call device arena (append only).
+struct Uploader {
+    nw_ctx *c;
+    char *base;
+    size_t cap, used = 0;
+    Uploader(nw_ctx *ctx, size_t bytes) : c(ctx)
+    {
+        base = c->arena_desc.get<char>(bytes);
+        cap = bytes;
+    }
+    template <class T>
+    T *put(const std::vector<T> &v)
+    {
+        size_t bytes = v.size() * sizeof(T);
+        size_t off = (used + 255) & ~size_t(255);
+        if (off + bytes > cap) throw NwError(NW_E_MEM, "descriptor arena exhausted");
+        if (bytes) {
+            c->keep.emplace_back((const char *)v.data(), (const char *)v.data() + bytes);
+            HIPCHK(hipMemcpyAsync(base + off, c->keep.back().data(), bytes, hipMemcpyHostToDevice, c->st));
+        }
+        used = off + bytes;
+        return (T *)(base + off);
+    }
+};
+
+inline int fail(nw_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(nw_ctx *c, F &&f)
+{
+    if (!c) return NW_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    try {
+        hipError_t e = hipSetDevice(c->device);
+        if (e != hipSuccess) return fail(c, NW_E_HIP, hipGetErrorString(e));
+        c->keep.clear();
+        int rc = f();
+        c->keep.clear();
+        return rc;
+    } catch (const NwError &e) {
+        (void)hipStreamSynchronize(c->st);
+        c->keep.clear();
+        return fail(c, e.code, e.what());
+    } catch (const std::exception &e) {
+        c->keep.clear();
+        return fail(c, NW_E_ARG, e.what());
+    }
+}
+
+// Copy count*size bytes to a device buffer unless the pointer is already device memory.
+inline const void *to_device(nw_ctx *c, DBuf &buf, const void *p, size_t bytes)
+{
+    if (is_device_ptr(p)) return p;
+    char *d = buf.get<char>(bytes);
+    HIPCHK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, c->st));
+    return d;
+}
+
+inline float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.f;
+    }
+    return ms;
+}
+
+
+struct Run { long long start, len; int parent, ra, rb; };
+std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked);
+std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vector<PackedDet> &h, bool need_hcol);
+long long tiles_of(const std::vector<EvalBlock> &v);
+EvalArgs base_args(nw_ctx *c, const PackedDet *dp, int mode);
+
+}  // namespace nwi

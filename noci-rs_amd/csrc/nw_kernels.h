@@ -64,4 +64,43 @@ void launch_eval_blocks(hipStream_t st, int kind, int la, int lb, bool m0, const
 void launch_eval_reqs(hipStream_t st, int kind, int la, int lb, bool m0, const EvalReq *reqs, int nreq,
                       const EvalArgs &args);
 
+// Multi-system fused evaluation (nw_batch_*).  Per-system addressing and one
+// work item per tile; see k_eval_fused.
+struct BatchSys {
+    const PackedDet *dets;
+    const PairInfo *pairs;
+    const double *slab;
+    double *H, *S;
+    long long ldo;
+    int nref, nmo, mode, pad;
+    double enuc, tol;
+};
+struct BatchTile {
+    long long i0, j0;
+    int sys, pair;
+    short ni, nj;
+    unsigned char code, shape, tri, pad;
+};
+
+// Register class of a contraction-rank code ((la*5+lb) << 1 | m0): 0 = light
+// (<= 64 VGPRs per lane in the per-bin kernels), 1 = medium (<= 128), 2 = heavy.
+__host__ __device__ constexpr int fused_class(int code)
+{
+    const int m0 = code & 1, lab = code >> 1, la = lab / 5, lb = lab % 5;
+    const int lo = la < lb ? la : lb, hi = la < lb ? lb : la;
+    if (m0) {
+        if (hi <= 2 && lo <= 1) return 0;
+        if ((lo == 0 && hi <= 4) || (lo == 1 && hi == 3) || (lo == 2 && hi == 2)) return 1;
+        return 2;
+    }
+    if (hi <= 1) return 0;
+    if (hi == 2 && lo <= 2) return 1;
+    return 2;
+}
+
+constexpr int FUSE_ALL = -5;
+// sel: a code (>= 0), a register class (-1 - class) or FUSE_ALL.
+void launch_eval_fused(hipStream_t st, int sel, bool dyn, const BatchTile *tiles, long long ntiles,
+                       const BatchSys *sys, unsigned long long *counter, int grid);
+
 }  // namespace nw

@@ -31,9 +31,14 @@ STATUS = {0: "NW_OK", 1: "NW_E_ARG", 2: "NW_E_SHAPE", 3: "NW_E_HIP", 4: "NW_E_ME
 # Every symbol the header declares.
 EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matrix", "nw_update_fock",
            "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_set_precision", "nw_pair_slab", "nw_last_timings",
-           "nw_last_error", "nw_destroy"]
+           "nw_last_error", "nw_destroy", "nw_batch_create", "nw_batch_add", "nw_batch_plan", "nw_batch_run",
+           "nw_batch_result", "nw_batch_schedule", "nw_batch_set_schedule", "nw_batch_last_ms",
+           "nw_batch_last_error", "nw_batch_destroy"]
 PREC_FP64 = 0
 PREC_MIXED = 1
+BATCH_TUNE = 1
+BATCH_GRAPH = 2
+GROUP_CODE, GROUP_CLASS, GROUP_ALL = 0, 1, 2
 
 _lib = None
 
@@ -42,6 +47,11 @@ class NwickError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{STATUS.get(code, code)}: {msg}")
         self.code = code
+
+
+class TuneEntry(ct.Structure):
+    _fields_ = [("nocc", ct.c_int32), ("code", ct.c_int32), ("shape", ct.c_int32), ("pad", ct.c_int32),
+                ("ms_best", ct.c_double), ("ms_default", ct.c_double)]
 
 
 class PairMeta(ct.Structure):
@@ -80,6 +90,19 @@ def load():
     L.nw_last_error.restype = ct.c_char_p
     L.nw_destroy.argtypes = [vp]
     L.nw_destroy.restype = None
+    L.nw_batch_create.argtypes = [ct.POINTER(vp), ct.c_int]
+    L.nw_batch_add.argtypes = [vp, vp, vp, ct.c_uint64, ct.c_int, ct.POINTER(ct.c_int)]
+    L.nw_batch_plan.argtypes = [vp, ct.c_int]
+    L.nw_batch_run.argtypes = [vp]
+    L.nw_batch_result.argtypes = [vp, ct.c_int, vp, vp]
+    L.nw_batch_schedule.argtypes = [vp, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), vp, ct.c_int,
+                                    ct.POINTER(ct.c_int)]
+    L.nw_batch_set_schedule.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, vp, ct.c_int]
+    L.nw_batch_last_ms.argtypes = [vp, dp, ct.POINTER(ct.c_longlong)]
+    L.nw_batch_last_error.argtypes = [vp]
+    L.nw_batch_last_error.restype = ct.c_char_p
+    L.nw_batch_destroy.argtypes = [vp]
+    L.nw_batch_destroy.restype = None
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -216,6 +239,79 @@ class Engine:
         if getattr(self, "ctx", None):
             self.L.nw_destroy(self.ctx)
             self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Several built engines evaluated together (nw_batch_*): one interleaved
+    plan of persistent fused launches over every system's pairs, optionally
+    tuned and captured into a HIP graph."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        b = ct.c_void_p()
+        rc = self.L.nw_batch_create(ct.byref(b), device)
+        if rc != 0:
+            raise NwickError(rc, "nw_batch_create failed")
+        self.b = b
+        self.engines, self.sizes = [], []
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NwickError(rc, self.L.nw_batch_last_error(self.b).decode())
+
+    def add(self, eng, dets, mode=MODE_REFERENCE):
+        idx = ct.c_int()
+        n = Engine._ndet(dets)
+        self._check(self.L.nw_batch_add(self.b, eng.ctx, _ptr(dets), n, int(mode), ct.byref(idx)))
+        self.engines.append(eng)
+        self.sizes.append(n)
+        return idx.value
+
+    def plan(self, tune=False, graph=True):
+        self._check(self.L.nw_batch_plan(self.b, (BATCH_TUNE if tune else 0) | (BATCH_GRAPH if graph else 0)))
+
+    def run(self):
+        self._check(self.L.nw_batch_run(self.b))
+
+    def result(self, index, out=None):
+        n = self.sizes[index]
+        H, S = (np.empty((n, n)), np.empty((n, n))) if out is None else out
+        self._check(self.L.nw_batch_result(self.b, index, _ptr(H), _ptr(S)))
+        return H, S
+
+    def schedule(self):
+        g, d, gp, cnt = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_int()
+        self._check(self.L.nw_batch_schedule(self.b, ct.byref(g), ct.byref(d), ct.byref(gp), None, 0, ct.byref(cnt)))
+        ent = (TuneEntry * max(cnt.value, 1))()
+        self._check(self.L.nw_batch_schedule(self.b, None, None, None, ct.cast(ent, ct.c_void_p), cnt.value, None))
+        return {"grouping": g.value, "dynamic": d.value, "grid_per_cu": gp.value,
+                "buckets": [{"nocc": e.nocc, "code": e.code, "la": (e.code >> 1) // 5, "lb": (e.code >> 1) % 5,
+                             "m0": e.code & 1, "shape": e.shape, "ms_best": e.ms_best, "ms_default": e.ms_default}
+                            for e in ent[:cnt.value]]}
+
+    def set_schedule(self, sched):
+        bk = sched.get("buckets", [])
+        ent = (TuneEntry * max(len(bk), 1))()
+        for k, e in enumerate(bk):
+            ent[k].nocc, ent[k].code, ent[k].shape = e["nocc"], e["code"], e["shape"]
+        self._check(self.L.nw_batch_set_schedule(self.b, sched["grouping"], sched["dynamic"], sched["grid_per_cu"],
+                                                 ct.cast(ent, ct.c_void_p), len(bk)))
+
+    def last(self):
+        ms, n = ct.c_double(), ct.c_longlong()
+        self._check(self.L.nw_batch_last_ms(self.b, ct.byref(ms), ct.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.L.nw_batch_destroy(self.b)
+            self.b = None
 
     def __del__(self):
         try:

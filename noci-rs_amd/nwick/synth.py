@@ -153,3 +153,42 @@ def wick_batch_n100(seed: int = 2024, dense_eri: bool = True) -> System:
     occupied orbital exactly orthogonal to every other reference."""
     return make_system(100, 10, 10, 8, seed, n_aux=300, orth_refs=(7,), orth_count=1,
                        dense_eri=dense_eri)
+
+
+# Uneven multi-system batch: (n, references, excitations per reference) with
+# n_bas = 5 n and n_occ = n / 2 per spin, n = 4..10 (n_bas 20..50).
+C3_SYSTEMS = [(4, 4, 1024), (4, 16, 64), (6, 8, 512), (6, 12, 256),
+              (8, 16, 256), (8, 6, 768), (10, 8, 512), (10, 4, 1024)]
+
+
+def capped_counts(nmo: int, nocc: int, per_ref: int) -> dict:
+    """The default excitation mix (10% alpha singles, 10% beta singles, doubles
+    split over aa/bb/ab) with every type capped at the number that exists; the
+    remainder goes to alpha-beta doubles."""
+    from math import comb
+    v = nmo - nocc
+    pool = {"a": nocc * v, "b": nocc * v, "aa": comb(nocc, 2) * comb(v, 2), "bb": comb(nocc, 2) * comb(v, 2),
+            "ab": (nocc * v) ** 2}
+    ns = per_ref // 10
+    nd = per_ref - 2 * ns
+    want = {"a": ns, "b": ns, "aa": nd // 3, "bb": nd // 3}
+    c = {k: min(w, pool[k]) for k, w in want.items()}
+    c["ab"] = min(per_ref - sum(c.values()), pool["ab"])
+    return c
+
+
+def batch_systems(seed: int = 3030, systems=C3_SYSTEMS):
+    """Seeded systems and determinant lists of the uneven batch.  References
+    of systems with at least 6 references get one exactly orthogonal
+    occupied alpha orbital on the last reference (m > 0 pairs) when the
+    basis has room for it."""
+    from . import basis
+    out = []
+    for k, (n, nref, per) in enumerate(systems):
+        nb, no = 5 * n, n // 2
+        orth = (nref - 1,) if nref >= 6 and nref * no <= nb else ()
+        sysm = make_system(nb, no, no, nref, seed=seed + k, orth_refs=orth, orthonormal_ao=True)
+        dets = basis.excited_basis(nref, nb, no, no, per_ref=per, seed=seed + k, include_refs=True,
+                                   counts=capped_counts(nb, no, per))
+        out.append((sysm, dets))
+    return out
