@@ -68,6 +68,8 @@ typedef struct nw_pair_meta {
     int32_t ma, mb;           /* zero-overlap orbital pairs per spin */
     double tilde_s_a, phase_a; /* SameSpinMeta tilde_s_prod / phase */
     double tilde_s_b, phase_b;
+    double sigma_min;         /* smallest singular value above tol, both spins (no reference field) */
+    int32_t fp32, pad;        /* 1 when this pair's four-index transforms ran in FP32 (NW_PREC_MIXED) */
 } nw_pair_meta;
 
 /* Create a context on `device` holding the AO problem (AoData, src/lib.rs:52-76):
@@ -129,6 +131,13 @@ int nw_matvec_m(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double e0, const
 /* Select the build arithmetic (nw_precision) for subsequent nw_build calls.
  * No reference counterpart: the reference computes in FP64 only. */
 int nw_set_precision(nw_ctx *ctx, int precision);
+
+/* Under NW_PREC_MIXED, reference pairs whose occupied-overlap matrices have a
+ * nonzero singular value below sigma_floor (per spin, values <= tol excluded)
+ * keep FP64 transforms: the co-densities carry 1/sigma, which amplifies FP32
+ * rounding in the matrix elements roughly as 1/sigma^4.  Default 0.1.  No
+ * reference counterpart. */
+int nw_set_mixed_floor(nw_ctx *ctx, double sigma_floor);
 
 /* Copy one ordered pair's slab block (layout of layout.rs:21-189) and its
  * offsets (95 int64, relative to the block, -1 = absent, last = length) and

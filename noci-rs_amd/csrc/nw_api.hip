@@ -152,6 +152,7 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                 if (!pair_mask || pair_mask[x * nref + w]) { px.push_back(x); pw.push_back(w); }
         const int P = (int)px.size();
         c->built = false;
+        c->build_gen++;
         c->tol = tol;
         HIPCHK(hipEventRecord(c->ev[0], c->st));
         Uploader up(c, 64u << 20);
@@ -214,14 +215,27 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                 if (sp == 0) pi.ma = mh[p]; else pi.mb = mh[p];
             }
         }
-        std::vector<double> ph_h(2 * P), tsp_h(2 * P);
+        std::vector<double> ph_h(2 * P), tsp_h(2 * P), sig_h((size_t)2 * P * nomax);
         HIPCHK(hipMemcpyAsync(ph_h.data(), phase, 2 * P * 8, hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipMemcpyAsync(tsp_h.data(), tsp, 2 * P * 8, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(sig_h.data(), sig, sig_h.size() * 8, hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
+        // smallest singular value above the zero threshold per pair (both spins):
+        // the co-densities carry 1/sigma, so FP32 rounding in the four-index
+        // transforms is amplified roughly as 1/sigma^4 in the matrix elements
+        std::vector<double> smin(P, 1.0);
+        for (int sp = 0; sp < 2; ++sp)
+            for (int p = 0; p < P; ++p)
+                for (int k = 0; k < c->nocc[sp]; ++k) {
+                    const double sv = std::fabs(sig_h[(size_t)sp * P * nomax + (size_t)p * c->nocc[sp] + k]);
+                    if (sv > tol) smin[p] = std::min(smin[p], sv);
+                }
         // ---- slab layout ------------------------------------------------------
         long long total = 0;
         for (int p = 0; p < P; ++p) {
             PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
+            pi.smin = smin[p];
+            pi.fp32 = 0;
             long long sz = pair_offsets(pi.ma, pi.mb, nm, pi.off);
             pi.base = total;
             pi.built = 1;
@@ -420,9 +434,42 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         // ---- four-index transforms (build.rs:261-313, 877-917) ----------------
         HIPCHK(hipEventRecord(c->ev[1], c->st));
         const bool mixed = c->precision == NW_PREC_MIXED && n % 4 == 0;
-        if (mixed) {
-            // Mixed form: FP32 operands, intermediates and MFMA accumulation for the
-            // four-index transforms; the slab blocks they produce are stored in FP64.
+        // One job per rank-4 tensor: the J slots per spin and the II blocks.
+        // J slab block [a,b,c,d] = blk[c,d,a,b] with blk = (cx_i xc_j || cx_k xc_l);
+        // the pair symmetry of (ab||cd) lets one transform produce it directly.
+        struct TJob { bool f32; bool asym; int c1, c2, c3, c4; double *out; };  // cN: matrix index into cx/xc
+        std::vector<TJob> jobs;
+        auto CX = [&](int pr, int mi) { return 2 * (pr * 2 + mi); };       // even: cx, odd: xc
+        auto XC = [&](int pr, int mi) { return 2 * (pr * 2 + mi) + 1; };
+        for (int p = 0; p < P; ++p) {
+            PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
+            const bool f32 = mixed && smin[p] >= c->mixed_floor;
+            pi.fp32 = f32 ? 1 : 0;
+            for (int sp = 0; sp < 2; ++sp) {
+                const long long *o = pi.off + (sp ? O_BB : O_AA);
+                const int pr = prob(p, sp);
+                for (int s = 0; s < 10; ++s) {
+                    if (o[O_J + s] < 0) continue;
+                    const int i = jbranch(s, 0), j = jbranch(s, 1), k = jbranch(s, 2), l = jbranch(s, 3);
+                    jobs.push_back({f32, true, CX(pr, k), XC(pr, l), CX(pr, i), XC(pr, j), slab + pi.base + o[O_J + s]});
+                }
+            }
+            const int pa = prob(p, 0), pb = prob(p, 1);
+            for (int a0 = 0; a0 < 2; ++a0)
+                for (int aj = 0; aj < 2; ++aj)
+                    for (int b0 = 0; b0 < 2; ++b0)
+                        for (int bj = 0; bj < 2; ++bj) {
+                            long long oo = pi.off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj];
+                            if (oo < 0) continue;
+                            jobs.push_back({f32, false, CX(pa, a0), XC(pa, aj), CX(pb, b0), XC(pb, bj), slab + pi.base + oo});
+                        }
+        }
+        float *cx32 = nullptr, *xc32 = nullptr;
+        bool any32 = false;
+        for (auto &jb : jobs) any32 |= jb.f32;
+        if (any32) {
+            // FP32 operands, intermediates and MFMA accumulation; the slab blocks
+            // they produce are stored in FP64.
             const size_t n4 = (size_t)n * n * n * n;
             if (!c->d_eri32) {
                 c->d_eri32 = (float *)dalloc<char>(n4 * sizeof(float));
@@ -430,80 +477,62 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                 launch_to_f32(c->st, c->d_eri, c->d_eri32, (long long)n4);
                 launch_to_f32(c->st, c->d_easym, c->d_easym32, (long long)n4);
             }
-            float *cx32 = c->cx32.get<float>((size_t)4 * P * nnm), *xc32 = c->xc32.get<float>((size_t)4 * P * nnm);
+            cx32 = c->cx32.get<float>((size_t)4 * P * nnm);
+            xc32 = c->xc32.get<float>((size_t)4 * P * nnm);
             launch_to_f32(c->st, cx, cx32, (long long)4 * P * nnm);
             launch_to_f32(c->st, xc, xc32, (long long)4 * P * nnm);
-            const size_t big = std::max(n4, (size_t)nm * n * n * n);
-            float *TA = c->scratchA.get<float>(big), *TB = c->scratchB.get<float>(big);
-            auto transform32 = [&](const float *E, const float *c1, const float *c2, const float *c3, const float *c4,
-                                   double *out) {
-                bool ok = launch_qtransform_f32(c->st, E, c4, TA, (long long)n * n * n, n, nm) &&
-                          launch_qtransform_f32(c->st, TA, c3, TB, (long long)nm * n * n, n, nm) &&
-                          launch_qtransform_f32(c->st, TB, c2, TA, (long long)nm * nm * n, n, nm) &&
-                          launch_qtransform_f32_out64(c->st, TA, c1, out, (long long)nm * nm * nm, n, nm);
-                if (!ok) throw NwError(NW_E_SHAPE, "mixed-precision transform needs n_bas % 4 == 0 and nmo <= 112");
-            };
-            for (int p = 0; p < P; ++p) {
-                const PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
-                for (int sp = 0; sp < 2; ++sp) {
-                    const long long *o = pi.off + (sp ? O_BB : O_AA);
-                    const int pr = prob(p, sp);
-                    const float *cxs[2] = {cx32 + (size_t)(pr * 2) * nnm, cx32 + (size_t)(pr * 2 + 1) * nnm};
-                    const float *xcs[2] = {xc32 + (size_t)(pr * 2) * nnm, xc32 + (size_t)(pr * 2 + 1) * nnm};
-                    for (int s = 0; s < 10; ++s) {
-                        if (o[O_J + s] < 0) continue;
-                        const int i = jbranch(s, 0), j = jbranch(s, 1), k = jbranch(s, 2), l = jbranch(s, 3);
-                        transform32(c->d_easym32, cxs[k], xcs[l], cxs[i], xcs[j], slab + pi.base + o[O_J + s]);
+        }
+        // Tensors in flight per launch group: each needs two scratch slots of
+        // max(n^4, nmo n^3) elements; the group is sized to 1 GiB per buffer.
+        const size_t big = std::max((size_t)n * n * n * n, (size_t)nm * n * n * n);
+        for (int f32 = 0; f32 < 2; ++f32) {
+            std::vector<const TJob *> todo;
+            for (auto &jb : jobs)
+                if (jb.f32 == (f32 == 1)) todo.push_back(&jb);
+            if (todo.empty()) continue;
+            const size_t slot = big * (f32 ? sizeof(float) : sizeof(double));
+            const int g = (int)std::max<size_t>(1, std::min<size_t>(QMAX, (size_t(1) << 30) / slot));
+            char *A = c->scratchA.get<char>(slot * std::min<size_t>(g, todo.size()));
+            char *B = c->scratchB.get<char>(slot * std::min<size_t>(g, todo.size()));
+            for (size_t j0 = 0; j0 < todo.size(); j0 += g) {
+                const int nb = (int)std::min<size_t>(g, todo.size() - j0);
+                if (f32) {
+                    std::vector<const float *> X(nb), C1(nb), C2(nb), C3(nb), C4(nb);
+                    std::vector<float *> TA(nb), TB(nb);
+                    std::vector<double *> OUT(nb);
+                    auto cm = [&](int idx) { return (idx & 1 ? xc32 : cx32) + (size_t)(idx >> 1) * nnm; };
+                    for (int k = 0; k < nb; ++k) {
+                        const TJob &jb = *todo[j0 + k];
+                        X[k] = jb.asym ? c->d_easym32 : c->d_eri32;
+                        C1[k] = cm(jb.c1); C2[k] = cm(jb.c2); C3[k] = cm(jb.c3); C4[k] = cm(jb.c4);
+                        TA[k] = (float *)(A + k * slot);
+                        TB[k] = (float *)(B + k * slot);
+                        OUT[k] = jb.out;
                     }
-                }
-                const int pa = prob(p, 0), pb = prob(p, 1);
-                for (int a0 = 0; a0 < 2; ++a0)
-                    for (int aj = 0; aj < 2; ++aj)
-                        for (int b0 = 0; b0 < 2; ++b0)
-                            for (int bj = 0; bj < 2; ++bj) {
-                                long long oo = pi.off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj];
-                                if (oo < 0) continue;
-                                transform32(c->d_eri32, cx32 + (size_t)(pa * 2 + a0) * nnm, xc32 + (size_t)(pa * 2 + aj) * nnm,
-                                            cx32 + (size_t)(pb * 2 + b0) * nnm, xc32 + (size_t)(pb * 2 + bj) * nnm,
-                                            slab + pi.base + oo);
-                            }
-            }
-        } else {
-            const size_t big = std::max((size_t)n * n * n * n, (size_t)nm * n * n * n);
-            double *TA = c->scratchA.get<double>(big), *TB = c->scratchB.get<double>(big);
-            auto transform = [&](const double *E, const double *c1, const double *c2, const double *c3, const double *c4,
-                                 double *out) {
-                launch_qtransform(c->st, E, c4, TA, (long long)n * n * n, n, nm);
-                launch_qtransform(c->st, TA, c3, TB, (long long)nm * n * n, n, nm);
-                launch_qtransform(c->st, TB, c2, TA, (long long)nm * nm * n, n, nm);
-                launch_qtransform(c->st, TA, c1, out, (long long)nm * nm * nm, n, nm);
-            };
-            for (int p = 0; p < P; ++p) {
-                const PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
-                for (int sp = 0; sp < 2; ++sp) {
-                    const long long *o = pi.off + (sp ? O_BB : O_AA);
-                    const int pr = prob(p, sp);
-                    const double *cxs[2] = {cx + (size_t)(pr * 2) * nnm, cx + (size_t)(pr * 2 + 1) * nnm};
-                    const double *xcs[2] = {xc + (size_t)(pr * 2) * nnm, xc + (size_t)(pr * 2 + 1) * nnm};
-                    for (int s = 0; s < 10; ++s) {
-                        if (o[O_J + s] < 0) continue;
-                        const int i = jbranch(s, 0), j = jbranch(s, 1), k = jbranch(s, 2), l = jbranch(s, 3);
-                        // J slab block [a,b,c,d] = blk[c,d,a,b] with blk = (cx_i xc_j || cx_k xc_l);
-                        // the pair symmetry of (ab||cd) lets one transform produce it directly.
-                        transform(c->d_easym, cxs[k], xcs[l], cxs[i], xcs[j], slab + pi.base + o[O_J + s]);
+                    std::vector<const float *> TAc(TA.begin(), TA.end()), TBc(TB.begin(), TB.end());
+                    bool ok = launch_qtransform_f32_many(c->st, X.data(), C4.data(), TA.data(), nb, (long long)n * n * n, n, nm) &&
+                              launch_qtransform_f32_many(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm) &&
+                              launch_qtransform_f32_many(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm) &&
+                              launch_qtransform_f32_out64_many(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
+                    if (!ok) throw NwError(NW_E_SHAPE, "mixed-precision transform needs n_bas % 4 == 0 and nmo <= 112");
+                } else {
+                    std::vector<const double *> X(nb), C1(nb), C2(nb), C3(nb), C4(nb);
+                    std::vector<double *> TA(nb), TB(nb), OUT(nb);
+                    auto cm = [&](int idx) { return (idx & 1 ? xc : cx) + (size_t)(idx >> 1) * nnm; };
+                    for (int k = 0; k < nb; ++k) {
+                        const TJob &jb = *todo[j0 + k];
+                        X[k] = jb.asym ? c->d_easym : c->d_eri;
+                        C1[k] = cm(jb.c1); C2[k] = cm(jb.c2); C3[k] = cm(jb.c3); C4[k] = cm(jb.c4);
+                        TA[k] = (double *)(A + k * slot);
+                        TB[k] = (double *)(B + k * slot);
+                        OUT[k] = jb.out;
                     }
+                    std::vector<const double *> TAc(TA.begin(), TA.end()), TBc(TB.begin(), TB.end());
+                    launch_qtransform_many(c->st, X.data(), C4.data(), TA.data(), nb, (long long)n * n * n, n, nm);
+                    launch_qtransform_many(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm);
+                    launch_qtransform_many(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm);
+                    launch_qtransform_many(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
                 }
-                const int pa = prob(p, 0), pb = prob(p, 1);
-                for (int a0 = 0; a0 < 2; ++a0)
-                    for (int aj = 0; aj < 2; ++aj)
-                        for (int b0 = 0; b0 < 2; ++b0)
-                            for (int bj = 0; bj < 2; ++bj) {
-                                long long oo = pi.off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj];
-                                if (oo < 0) continue;
-                                transform(c->d_eri, cx + (size_t)(pa * 2 + a0) * nnm, xc + (size_t)(pa * 2 + aj) * nnm,
-                                          cx + (size_t)(pb * 2 + b0) * nnm, xc + (size_t)(pb * 2 + bj) * nnm,
-                                          slab + pi.base + oo);
-                            }
             }
         }
         HIPCHK(hipGetLastError());
@@ -527,6 +556,7 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                 o.ma = pi.ma; o.mb = pi.mb;
                 o.tilde_s_a = pi.meta[M_AA + M_TSP]; o.phase_a = pi.meta[M_AA + M_PH];
                 o.tilde_s_b = pi.meta[M_BB + M_TSP]; o.phase_b = pi.meta[M_BB + M_PH];
+                o.sigma_min = pi.smin; o.fp32 = pi.fp32; o.pad = 0;
             }
         }
         if (c->d_pinfo) HIPCHK(hipFree(c->d_pinfo));
@@ -858,6 +888,16 @@ int nw_matvec_m(nw_ctx *c, const nw_det *dets, uint64_t ndet, double e0, const d
         c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
         c->t_ms[1] = 0.0;
         c->t_ms[2] = elapsed(c->ev[1], c->ev[2]);
+        return NW_OK;
+    });
+}
+
+int nw_set_mixed_floor(nw_ctx *c, double sigma_floor)
+{
+    return guarded(c, [&]() -> int {
+        if (!(sigma_floor >= 0.0) || !std::isfinite(sigma_floor)) return fail(c, NW_E_ARG, "bad sigma floor");
+        c->mixed_floor = sigma_floor;
+        c->built = false;
         return NW_OK;
     });
 }

@@ -36,6 +36,7 @@ double elem_bytes(int code)
 
 struct BSys {
     nw_ctx *ctx = nullptr;
+    unsigned long long gen = 0;  // the context's build generation at nw_batch_add
     uint64_t ndet = 0;
     int mode = 0, nocc = 0;
     PackedDet *dets = nullptr;
@@ -102,6 +103,15 @@ int bfail(nw_batch *b, int code, const std::string &m)
 {
     if (b) b->err = m;
     return code;
+}
+
+// The plan addresses each context's slab and pair table directly, so a
+// context rebuilt after nw_batch_add would leave it pointing at freed memory.
+void check_generations(nw_batch *b)
+{
+    for (size_t k = 0; k < b->sys.size(); ++k)
+        if (b->sys[k].ctx->build_gen != b->sys[k].gen || !b->sys[k].ctx->built)
+            throw NwError(NW_E_STATE, "system " + std::to_string(k) + ": context rebuilt after nw_batch_add");
 }
 
 template <class F>
@@ -380,6 +390,7 @@ int nw_batch_add(nw_batch *b, nw_ctx *ctx, const nw_det *dets, uint64_t ndet, in
         if (!ctx->built) return bfail(b, NW_E_STATE, "context not built");
         BSys S;
         S.ctx = ctx;
+        S.gen = ctx->build_gen;
         S.ndet = ndet;
         S.mode = mode;
         S.nocc = std::max(ctx->nocc[0], ctx->nocc[1]);
@@ -423,6 +434,7 @@ int nw_batch_plan(nw_batch *b, int flags)
 {
     return bguard(b, [&]() -> int {
         if (b->sys.empty()) return bfail(b, NW_E_STATE, "no systems added");
+        check_generations(b);
         if (flags & NW_BATCH_TUNE) tune(b);
         make_plan(b);
         if (flags & NW_BATCH_GRAPH) {
@@ -446,6 +458,7 @@ int nw_batch_run(nw_batch *b)
 {
     return bguard(b, [&]() -> int {
         if (!b->planned) return bfail(b, NW_E_STATE, "nw_batch_plan first");
+        check_generations(b);
         HIPCHK(hipEventRecord(b->ev[0], b->st[0]));
         if (b->exec) HIPCHK(hipGraphLaunch(b->exec, b->st[0]));
         else enqueue(b);

@@ -91,13 +91,12 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc *__restrict__ descs
 // (eri_ao2mo_into, src/maths/eri.rs:258-313): Y[l', m] = sum_l c[l, l'] X[m, l].
 // The contracted index is the fastest of X and the new index becomes the
 // slowest of Y, so four passes return the original index order.
-__global__ __launch_bounds__(256) void k_qtransform(const double *__restrict__ X, const double *__restrict__ c,
-                                                     double *__restrict__ Y, int rows, int nl, int nout)
+__global__ __launch_bounds__(256) void k_qtransform(const QPtrs q, int rows, int nl, int nout)
 {
     GemmDesc g;
-    g.A = c;
-    g.B = X;
-    g.C = Y;
+    g.A = (const double *)q.c[blockIdx.z];
+    g.B = (const double *)q.X[blockIdx.z];
+    g.C = (double *)q.Y[blockIdx.z];
     gemm_tile<1, 1>(g, nout, rows, nl, nout, nl, rows, 1.0, 0.0);
 }
 
@@ -114,10 +113,12 @@ typedef double nw_d4 __attribute__((ext_vector_type(4)));
 #define NW_F32_ROW(lk, r) (4 * (lk) + (r))
 
 template <int KS>
-__global__ __launch_bounds__(512) void k_qtransform_mfma(const double *__restrict__ X, const double *__restrict__ c,
-                                                          double *__restrict__ Y, long long rows, int nl, int nout,
+__global__ __launch_bounds__(512) void k_qtransform_mfma(const QPtrs q, long long rows, int nl, int nout,
                                                           long long ntiles)
 {
+    const double *__restrict__ X = (const double *)q.X[blockIdx.y];
+    const double *__restrict__ c = (const double *)q.c[blockIdx.y];
+    double *__restrict__ Y = (double *)q.Y[blockIdx.y];
     extern __shared__ double xs[];
     constexpr int NLP = KS * 4;
     constexpr int LDX = NLP + 1;
@@ -175,11 +176,12 @@ __global__ __launch_bounds__(512) void k_qtransform_mfma(const double *__restric
 // registers (16-byte loads, all in flight at once) while the current tile is
 // multiplied out of LDS, so HBM latency hides behind the MFMA work.
 template <int KS>
-__global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const double *__restrict__ X,
-                                                               const double *__restrict__ c,
-                                                               double *__restrict__ Y, long long rows, int nout,
+__global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, long long rows, int nout,
                                                                long long ntiles)
 {
+    const double *__restrict__ X = (const double *)q.X[blockIdx.y];
+    const double *__restrict__ c = (const double *)q.c[blockIdx.y];
+    double *__restrict__ Y = (double *)q.Y[blockIdx.y];
     constexpr int NL = KS * 4;
     constexpr int LDX = NL + 1;
     constexpr int NT = 448;
@@ -256,10 +258,12 @@ __global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const double *__re
 typedef float nw_f4 __attribute__((ext_vector_type(4)));
 
 template <int KS, typename OutT>
-__global__ __launch_bounds__(448) void k_qtransform_f32_pipe(const float *__restrict__ X, const float *__restrict__ c,
-                                                              OutT *__restrict__ Y, long long rows, int nout,
+__global__ __launch_bounds__(448) void k_qtransform_f32_pipe(const QPtrs q, long long rows, int nout,
                                                               long long ntiles)
 {
+    const float *__restrict__ X = (const float *)q.X[blockIdx.y];
+    const float *__restrict__ c = (const float *)q.c[blockIdx.y];
+    OutT *__restrict__ Y = (OutT *)q.Y[blockIdx.y];
     constexpr int NL = KS * 4;
     constexpr int LDX = NL + 1;
     constexpr int NT = 448;
@@ -344,85 +348,122 @@ void launch_to_f32(hipStream_t st, const double *in, float *out, long long n)
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)g), dim3(256), 0, st, in, out, n);
 }
 
+// Number of tensors of one batched launch and the persistent grid per tensor.
+static unsigned grid_x(long long ntiles, int count, long long target)
+{
+    long long g = target / count;
+    if (g < 1) g = 1;
+    if (g > ntiles) g = ntiles;
+    return (unsigned)g;
+}
+
 template <typename OutT>
-static bool launch_qtransform_f32_t(hipStream_t st, const float *X, const float *c, OutT *Y, long long rows, int nl,
-                                    int nout)
+static bool launch_qtransform_f32_t(hipStream_t st, const float *const *X, const float *const *c, OutT *const *Y,
+                                    int count, long long rows, int nl, int nout)
 {
     const int ks = nl / 4;
     if (nl % 4 != 0 || ks > 31 || (nout + 15) / 16 > 7) return false;
     const long long ntiles = (rows + 127) / 128;
     const size_t lds = (size_t)128 * (nl + 1) * sizeof(float);
-    long long grid = 256LL * 2;
-    if (grid > ntiles) grid = ntiles;
-    dim3 g((unsigned)grid), b(448);
-#define NW_QF(K)                                                                                            \
-    case K:                                                                                                 \
-        hipLaunchKernelGGL((k_qtransform_f32_pipe<K, OutT>), g, b, lds, st, X, c, Y, rows, nout, ntiles); \
-        return true;
-    switch (ks) {
-        NW_QF(1) NW_QF(2) NW_QF(3) NW_QF(4) NW_QF(5) NW_QF(6) NW_QF(7) NW_QF(8) NW_QF(9) NW_QF(10)
-        NW_QF(11) NW_QF(12) NW_QF(13) NW_QF(14) NW_QF(15) NW_QF(16) NW_QF(17) NW_QF(18) NW_QF(19)
-        NW_QF(20) NW_QF(21) NW_QF(22) NW_QF(23) NW_QF(24) NW_QF(25) NW_QF(26) NW_QF(27) NW_QF(28)
-        NW_QF(29) NW_QF(30) NW_QF(31)
-    default: return false;
-    }
+    for (int b0 = 0; b0 < count; b0 += QMAX) {
+        const int nb = count - b0 < QMAX ? count - b0 : QMAX;
+        QPtrs q{};
+        for (int k = 0; k < nb; ++k) { q.X[k] = X[b0 + k]; q.c[k] = c[b0 + k]; q.Y[k] = Y[b0 + k]; }
+        dim3 g(grid_x(ntiles, nb, 1024), (unsigned)nb), b(448);
+#define NW_QF(K)                                                                                        \
+    case K:                                                                                             \
+        hipLaunchKernelGGL((k_qtransform_f32_pipe<K, OutT>), g, b, lds, st, q, rows, nout, ntiles);    \
+        break;
+        switch (ks) {
+            NW_QF(1) NW_QF(2) NW_QF(3) NW_QF(4) NW_QF(5) NW_QF(6) NW_QF(7) NW_QF(8) NW_QF(9) NW_QF(10)
+            NW_QF(11) NW_QF(12) NW_QF(13) NW_QF(14) NW_QF(15) NW_QF(16) NW_QF(17) NW_QF(18) NW_QF(19)
+            NW_QF(20) NW_QF(21) NW_QF(22) NW_QF(23) NW_QF(24) NW_QF(25) NW_QF(26) NW_QF(27) NW_QF(28)
+            NW_QF(29) NW_QF(30) NW_QF(31)
+        default: return false;
+        }
 #undef NW_QF
+    }
+    return true;
+}
+
+bool launch_qtransform_f32_many(hipStream_t st, const float *const *X, const float *const *c, float *const *Y,
+                                int count, long long rows, int nl, int nout)
+{
+    return launch_qtransform_f32_t<float>(st, X, c, Y, count, rows, nl, nout);
+}
+
+bool launch_qtransform_f32_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
+                                      int count, long long rows, int nl, int nout)
+{
+    return launch_qtransform_f32_t<double>(st, X, c, Y, count, rows, nl, nout);
 }
 
 bool launch_qtransform_f32(hipStream_t st, const float *X, const float *c, float *Y, long long rows, int nl, int nout)
 {
-    return launch_qtransform_f32_t<float>(st, X, c, Y, rows, nl, nout);
+    return launch_qtransform_f32_many(st, &X, &c, &Y, 1, rows, nl, nout);
 }
 
 bool launch_qtransform_f32_out64(hipStream_t st, const float *X, const float *c, double *Y, long long rows, int nl,
                                  int nout)
 {
-    return launch_qtransform_f32_t<double>(st, X, c, Y, rows, nl, nout);
+    return launch_qtransform_f32_out64_many(st, &X, &c, &Y, 1, rows, nl, nout);
 }
 
-void launch_qtransform(hipStream_t st, const double *X, const double *c, double *Y, long long rows, int nl, int nout)
+void launch_qtransform_many(hipStream_t st, const double *const *X, const double *const *c, double *const *Y, int count,
+                            long long rows, int nl, int nout)
 {
     const int ks = (nl + 3) / 4;
     const int nw = (nout + 15) / 16;
     const long long ntiles = (rows + 63) / 64;
-    if (nl % 4 == 0 && ks <= 31 && nw <= 7) {
-        const size_t lds = (size_t)64 * (ks * 4 + 1) * sizeof(double);
-        long long grid = 256LL * 2;
-        if (grid > ntiles) grid = ntiles;
-        dim3 g((unsigned)grid), b(448);
-#define NW_QP(K)                                                                                            \
-    case K:                                                                                                 \
-        hipLaunchKernelGGL((k_qtransform_mfma_pipe<K>), g, b, lds, st, X, c, Y, rows, nout, ntiles);      \
-        return;
-        switch (ks) {
-            NW_QP(1) NW_QP(2) NW_QP(3) NW_QP(4) NW_QP(5) NW_QP(6) NW_QP(7) NW_QP(8) NW_QP(9) NW_QP(10)
-            NW_QP(11) NW_QP(12) NW_QP(13) NW_QP(14) NW_QP(15) NW_QP(16) NW_QP(17) NW_QP(18) NW_QP(19)
-            NW_QP(20) NW_QP(21) NW_QP(22) NW_QP(23) NW_QP(24) NW_QP(25) NW_QP(26) NW_QP(27) NW_QP(28)
-            NW_QP(29) NW_QP(30) NW_QP(31)
-        default: break;
-        }
+    for (int b0 = 0; b0 < count; b0 += QMAX) {
+        const int nb = count - b0 < QMAX ? count - b0 : QMAX;
+        QPtrs q{};
+        for (int k = 0; k < nb; ++k) { q.X[k] = X[b0 + k]; q.c[k] = c[b0 + k]; q.Y[k] = Y[b0 + k]; }
+        bool done = false;
+        if (nl % 4 == 0 && ks <= 31 && nw <= 7) {
+            const size_t lds = (size_t)64 * (ks * 4 + 1) * sizeof(double);
+            dim3 g(grid_x(ntiles, nb, 512), (unsigned)nb), b(448);
+            done = true;
+#define NW_QP(K)                                                                                        \
+    case K:                                                                                             \
+        hipLaunchKernelGGL((k_qtransform_mfma_pipe<K>), g, b, lds, st, q, rows, nout, ntiles);         \
+        break;
+            switch (ks) {
+                NW_QP(1) NW_QP(2) NW_QP(3) NW_QP(4) NW_QP(5) NW_QP(6) NW_QP(7) NW_QP(8) NW_QP(9) NW_QP(10)
+                NW_QP(11) NW_QP(12) NW_QP(13) NW_QP(14) NW_QP(15) NW_QP(16) NW_QP(17) NW_QP(18) NW_QP(19)
+                NW_QP(20) NW_QP(21) NW_QP(22) NW_QP(23) NW_QP(24) NW_QP(25) NW_QP(26) NW_QP(27) NW_QP(28)
+                NW_QP(29) NW_QP(30) NW_QP(31)
+            default: done = false; break;
+            }
 #undef NW_QP
-    }
-    if (ks <= 31 && nw <= 8) {  // LDS tile <= 64 KB
-        const size_t lds = (size_t)64 * (ks * 4 + 1) * sizeof(double);
-        long long grid = 256LL * 4;
-        if (grid > ntiles) grid = ntiles;
-        dim3 g((unsigned)grid), b(64 * nw);
-#define NW_QT(K)                                                                                            \
-    case K:                                                                                                 \
-        hipLaunchKernelGGL((k_qtransform_mfma<K>), g, b, lds, st, X, c, Y, rows, nl, nout, ntiles);       \
-        return;
-        switch (ks) {
-            NW_QT(1) NW_QT(2) NW_QT(3) NW_QT(4) NW_QT(5) NW_QT(6) NW_QT(7) NW_QT(8) NW_QT(9) NW_QT(10)
-            NW_QT(11) NW_QT(12) NW_QT(13) NW_QT(14) NW_QT(15) NW_QT(16) NW_QT(17) NW_QT(18) NW_QT(19)
-            NW_QT(20) NW_QT(21) NW_QT(22) NW_QT(23) NW_QT(24) NW_QT(25) NW_QT(26) NW_QT(27) NW_QT(28)
-            NW_QT(29) NW_QT(30) NW_QT(31) NW_QT(32)
-        default: break;
         }
+        if (!done && ks <= 31 && nw <= 8) {  // LDS tile <= 64 KB
+            const size_t lds = (size_t)64 * (ks * 4 + 1) * sizeof(double);
+            dim3 g(grid_x(ntiles, nb, 1024), (unsigned)nb), b(64 * nw);
+            done = true;
+#define NW_QT(K)                                                                                        \
+    case K:                                                                                             \
+        hipLaunchKernelGGL((k_qtransform_mfma<K>), g, b, lds, st, q, rows, nl, nout, ntiles);          \
+        break;
+            switch (ks) {
+                NW_QT(1) NW_QT(2) NW_QT(3) NW_QT(4) NW_QT(5) NW_QT(6) NW_QT(7) NW_QT(8) NW_QT(9) NW_QT(10)
+                NW_QT(11) NW_QT(12) NW_QT(13) NW_QT(14) NW_QT(15) NW_QT(16) NW_QT(17) NW_QT(18) NW_QT(19)
+                NW_QT(20) NW_QT(21) NW_QT(22) NW_QT(23) NW_QT(24) NW_QT(25) NW_QT(26) NW_QT(27) NW_QT(28)
+                NW_QT(29) NW_QT(30) NW_QT(31) NW_QT(32)
+            default: done = false; break;
+            }
 #undef NW_QT
+        }
+        if (!done) {
+            dim3 grid((unsigned)((rows + 63) / 64), (nout + 63) / 64, (unsigned)nb);
+            hipLaunchKernelGGL(k_qtransform, grid, dim3(256), 0, st, q, (int)rows, nl, nout);
+        }
     }
-    dim3 grid((unsigned)((rows + 63) / 64), (nout + 63) / 64, 1);
-    hipLaunchKernelGGL(k_qtransform, grid, dim3(256), 0, st, X, c, Y, (int)rows, nl, nout);
+}
+
+void launch_qtransform(hipStream_t st, const double *X, const double *c, double *Y, long long rows, int nl, int nout)
+{
+    launch_qtransform_many(st, &X, &c, &Y, 1, rows, nl, nout);
 }
 
 void launch_gemm(hipStream_t st, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc,

@@ -36,6 +36,8 @@ struct alignas(16) PairInfo {
     long long off[O_LEN];  // relative offsets
     double meta[M_LEN];
     int built, ma, mb, bidx;  // bidx: position of the pair in the last build
+    double smin;              // smallest singular value above the zero threshold (both spins)
+    int fp32, pad;            // 1: four-index transforms of this pair ran in FP32
 };
 
 // Compact determinant for evaluation kernels (ExcitationCache, src/lib.rs:128-167):

@@ -87,10 +87,12 @@ struct nw_ctx {
     double *d_S = nullptr, *d_h = nullptr, *d_eri = nullptr, *d_easym = nullptr;
     float *d_eri32 = nullptr, *d_easym32 = nullptr;  // FP32 copies for the mixed build
     int precision = NW_PREC_FP64;
+    double mixed_floor = 0.1;  // pairs with a nonzero singular value below this build in FP64
     int nref = 0, nmo = 0, nocc[2] = {0, 0};
     double *d_c[2] = {nullptr, nullptr};  // nref x n x nmo per spin
     // build state
     bool built = false;
+    unsigned long long build_gen = 0;  // incremented by every nw_build / nw_set_refs
     double tol = 1e-8;
     double *d_slab = nullptr;
     size_t slab_cap = 0;

@@ -25,6 +25,20 @@ void launch_antisym(hipStream_t st, const double *eri, double *out, int n);
 // Rotating quarter transform Y[l', m] = sum_l c[l, l'] X[m, l]
 // (X: rows x nl row-major, c: nl x nout row-major, Y: nout x rows row-major).
 void launch_qtransform(hipStream_t st, const double *X, const double *c, double *Y, long long rows, int nl, int nout);
+// Up to QMAX tensors per launch travel as kernel arguments (indexed by a grid
+// dimension); the *_many launchers split longer lists.
+constexpr int QMAX = 32;
+struct QPtrs {
+    const void *X[QMAX];
+    const void *c[QMAX];
+    void *Y[QMAX];
+};
+void launch_qtransform_many(hipStream_t st, const double *const *X, const double *const *c, double *const *Y, int count,
+                            long long rows, int nl, int nout);
+bool launch_qtransform_f32_many(hipStream_t st, const float *const *X, const float *const *c, float *const *Y,
+                                int count, long long rows, int nl, int nout);
+bool launch_qtransform_f32_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
+                                      int count, long long rows, int nl, int nout);
 // FP32 variants (mixed-precision build); return false when the shape is unsupported.
 bool launch_qtransform_f32(hipStream_t st, const float *X, const float *c, float *Y, long long rows, int nl, int nout);
 bool launch_qtransform_f32_out64(hipStream_t st, const float *X, const float *c, double *Y, long long rows, int nl,

@@ -30,7 +30,7 @@ STATUS = {0: "NW_OK", 1: "NW_E_ARG", 2: "NW_E_SHAPE", 3: "NW_E_HIP", 4: "NW_E_ME
 
 # Every symbol the header declares.
 EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matrix", "nw_update_fock",
-           "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_set_precision", "nw_pair_slab", "nw_last_timings",
+           "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_set_precision", "nw_set_mixed_floor", "nw_pair_slab", "nw_last_timings",
            "nw_last_error", "nw_destroy", "nw_batch_create", "nw_batch_add", "nw_batch_plan", "nw_batch_run",
            "nw_batch_result", "nw_batch_schedule", "nw_batch_set_schedule", "nw_batch_last_ms",
            "nw_batch_last_error", "nw_batch_destroy"]
@@ -56,7 +56,8 @@ class TuneEntry(ct.Structure):
 
 class PairMeta(ct.Structure):
     _fields_ = [("ma", ct.c_int32), ("mb", ct.c_int32), ("tilde_s_a", ct.c_double), ("phase_a", ct.c_double),
-                ("tilde_s_b", ct.c_double), ("phase_b", ct.c_double)]
+                ("tilde_s_b", ct.c_double), ("phase_b", ct.c_double), ("sigma_min", ct.c_double),
+                ("fp32", ct.c_int32), ("pad", ct.c_int32)]
 
 
 def load():
@@ -81,6 +82,7 @@ def load():
     L.nw_hs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
     L.nw_pair_slab.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
     L.nw_set_precision.argtypes = [vp, ct.c_int]
+    L.nw_set_mixed_floor.argtypes = [vp, ct.c_double]
     L.nw_update_fock.argtypes = [vp, vp, vp]
     L.nw_fs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
     L.nw_fs_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, vp, vp, ct.c_int]
@@ -159,6 +161,10 @@ class Engine:
     def set_precision(self, prec):
         """PREC_FP64 or PREC_MIXED (FP32 MFMA four-index transforms)."""
         self._check(self.L.nw_set_precision(self.ctx, int(prec)))
+
+    def set_mixed_floor(self, sigma_floor):
+        """Pairs with a nonzero occupied-overlap singular value below sigma_floor keep FP64 transforms."""
+        self._check(self.L.nw_set_mixed_floor(self.ctx, float(sigma_floor)))
 
     def build(self, tol=1e-8, pair_mask=None):
         """build_wicks_shared: returns {(x, w): PairMeta} for the built pairs."""

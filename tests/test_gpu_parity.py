@@ -132,3 +132,34 @@ def test_mixed_precision_build_close_to_fp64():
     emx = scipy.linalg.eigh(Hmx, Smx, eigvals_only=True)[0]
     print(f"mixed: max|dH|={np.max(np.abs(Hmx - H64)):.3e} dE={emx - e64:.3e}", flush=True)
     assert abs(emx - e64) < 1e-6
+
+
+def test_mixed_floor_selects_pairs():
+    """The sigma floor decides per reference pair: floor 0 puts every pair on
+    FP32 transforms, a floor above 1 keeps every pair FP64 (then the mixed
+    build equals the FP64 build bit for bit), and the default keeps exactly the
+    pairs whose smallest nonzero singular value is below 0.1 in FP64."""
+    sysm, dets = _case(16, 3, 3, 4, seed=52, orthonormal=False, per_ref=24)
+    rec = basis.to_records(dets)
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    H64, S64 = eng.hs_matrix(rec, mode=0)
+    eng.set_precision(nwick.PREC_MIXED)
+    eng.set_mixed_floor(2.0)
+    meta = eng.build()
+    assert all(m.fp32 == 0 for m in meta.values())
+    Hk, Sk = eng.hs_matrix(rec, mode=0)
+    assert np.array_equal(Hk, H64) and np.array_equal(Sk, S64)
+    eng.set_mixed_floor(0.0)
+    meta = eng.build()
+    assert all(m.fp32 == 1 for m in meta.values())
+    eng.set_mixed_floor(0.1)
+    meta = eng.build()
+    for (x, w), m in meta.items():
+        sv = []
+        for sp in (0, 1):
+            A = sysm.refs[x][sp][:, :3].T @ sysm.S @ sysm.refs[w][sp][:, :3]
+            s = np.linalg.svd(A, compute_uv=False)
+            sv += [v for v in s if v > 1e-8]
+        assert abs(m.sigma_min - min(sv)) < 1e-10
+        assert m.fp32 == (1 if min(sv) >= 0.1 else 0)
