@@ -128,6 +128,15 @@ int nw_fs_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t
 int nw_matvec_m(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double e0, const double *x,
                 double *y, double *ys, int mode);
 
+/* Projection of candidates on a current NOCI state.  dets = the ncur
+ * current-space determinants followed by the candidates; for every candidate
+ * a (index ncur + k): v[k] = sum_i H_ai coef_i and s[k] = sum_i S_ai coef_i
+ * over the current space i < ncur.  Replaces build_candidate_current_h with
+ * build_candidate_v (src/snoci/operators.rs:151-180, 991-999) and the
+ * candidate-current overlap of build_snoci_overlaps (:106-149). */
+int nw_project_hs(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, uint64_t ncur, const double *coef, double *v,
+                  double *s, int mode);
+
 /* Select the build arithmetic (nw_precision) for subsequent nw_build calls.
  * No reference counterpart: the reference computes in FP64 only. */
 int nw_set_precision(nw_ctx *ctx, int precision);
@@ -196,6 +205,33 @@ int nw_batch_set_schedule(nw_batch *b, int grouping, int dyn, int grid_per_cu, c
 int nw_batch_last_ms(nw_batch *b, double *ms, long long *npairs);
 const char *nw_batch_last_error(nw_batch *b);
 void nw_batch_destroy(nw_batch *b);
+
+/* ---------------------------------------------------------------------------
+ * Learned candidate screener.  No reference counterpart: noci-rs ranks SNOCI
+ * and PT2 candidates by exact evaluation only.  A perceptron n_in-128-128-1
+ * (ReLU) with BF16 weight copies, FP32 activations and accumulation, FP32
+ * master weights trained with Adam on the weighted mean squared error
+ * L = sum_r w_r (y_r - t_r)^2 / (2 n).  Parameters are one FP32 vector laid out
+ * W1 [128 x n_in], b1 [128], W2 [128 x 128], b2 [128], W3 [128], b3 [1].
+ * ------------------------------------------------------------------------- */
+typedef struct nw_mlp nw_mlp;
+
+int nw_mlp_create(nw_mlp **out, int device, int n_in, uint64_t seed); /* n_in <= 32; He-uniform init */
+int nw_mlp_num_params(nw_mlp *net, long long *n);
+int nw_mlp_set_params(nw_mlp *net, const float *params, long long n); /* also resets Adam state */
+int nw_mlp_get_params(nw_mlp *net, float *params, long long n);
+/* y[r] = network(X[r, :]) for n rows of X (n x n_in, row-major). */
+int nw_mlp_forward(nw_mlp *net, const float *X, long long n, float *y);
+/* Loss and gradient over all n rows (w may be NULL for unit weights). */
+int nw_mlp_grad(nw_mlp *net, const float *X, const float *t, const float *w, long long n, float *loss, float *grad);
+/* One full-batch Adam step (lr; beta1 0.9, beta2 0.999, eps 1e-8). */
+int nw_mlp_step(nw_mlp *net, const float *X, const float *t, const float *w, long long n, float lr, float *loss);
+/* Mini-batch Adam over shuffled rows for `epochs` epochs; epoch_loss[e] =
+ * mean batch loss of epoch e (may be NULL). */
+int nw_mlp_train(nw_mlp *net, const float *X, const float *t, const float *w, long long n, int epochs, int batch,
+                 float lr, uint64_t seed, float *epoch_loss);
+const char *nw_mlp_last_error(nw_mlp *net);
+void nw_mlp_destroy(nw_mlp *net);
 
 #ifdef __cplusplus
 }

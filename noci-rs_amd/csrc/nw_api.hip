@@ -632,6 +632,46 @@ std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vect
     return bins;
 }
 
+// Rows [nrow0, n) against columns [0, ncol) of one determinant list: every
+// pair (row, col) once, grouped by (parent, rank_a, rank_b) runs as in
+// triangle_blocks (no triangle restriction).
+std::map<int, std::vector<EvalBlock>> rect_blocks(nw_ctx *c, const std::vector<PackedDet> &h, long long nrow0,
+                                                  long long ncol, bool need_hcol)
+{
+    auto runs_of = [&](long long a, long long b) {
+        std::vector<Run> runs;
+        for (long long i = a; i < b; ++i) {
+            const PackedDet &d = h[i];
+            if (!runs.empty() && runs.back().parent == d.parent && runs.back().ra == d.ra && runs.back().rb == d.rb)
+                runs.back().len++;
+            else
+                runs.push_back({i, 1, d.parent, d.ra, d.rb});
+        }
+        return runs;
+    };
+    const std::vector<Run> rows = runs_of(nrow0, (long long)h.size()), cols = runs_of(0, ncol);
+    std::map<int, std::vector<EvalBlock>> bins;
+    for (const Run &A : rows)
+        for (const Run &B : cols) {
+            const int pair = A.parent * c->nref + B.parent;
+            const PairInfo &pi = c->pinfo[pair];
+            if (!pi.built)
+                throw NwError(NW_E_STATE, "reference pair (" + std::to_string(A.parent) + "," + std::to_string(B.parent) +
+                                              ") was not built");
+            const int la = A.ra + B.ra, lb = A.rb + B.rb;
+            const bool m0 = need_hcol && pi.ma == 0 && pi.mb == 0;
+            bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)].push_back({A.start, B.start, (int)A.len, (int)B.len, pair, 0, 0});
+        }
+    for (auto &kv : bins) {
+        long long t = 0;
+        for (auto &b : kv.second) {
+            b.tstart = t;
+            t += (long long)((b.ni + 15) / 16) * ((b.nj + 15) / 16);
+        }
+    }
+    return bins;
+}
+
 long long tiles_of(const std::vector<EvalBlock> &v)
 {
     const EvalBlock &b = v.back();
@@ -884,6 +924,49 @@ int nw_matvec_m(nw_ctx *c, const nw_det *dets, uint64_t ndet, double e0, const d
         HIPCHK(hipEventRecord(c->ev[2], c->st));
         if (!ydev) HIPCHK(hipMemcpyAsync(y, dy, ndet * 8, hipMemcpyDeviceToHost, c->st));
         if (ys && !ysdev) HIPCHK(hipMemcpyAsync(ys, dys, ndet * 8, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
+        c->t_ms[1] = 0.0;
+        c->t_ms[2] = elapsed(c->ev[1], c->ev[2]);
+        return NW_OK;
+    });
+}
+
+int nw_project_hs(nw_ctx *c, const nw_det *dets, uint64_t ndet, uint64_t ncur, const double *coef, double *v,
+                  double *s, int mode)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->built) return fail(c, NW_E_STATE, "nw_build first");
+        if (!dets || !coef || !v || !s || ncur == 0 || ncur >= ndet) return fail(c, NW_E_ARG, "bad arguments");
+        if (ndet > (1ull << 31)) return fail(c, NW_E_SHAPE, "too many determinants");
+        const uint64_t nc = ndet - ncur;
+        HIPCHK(hipEventRecord(c->ev[0], c->st));
+        PackedDet *dp = nullptr;
+        std::vector<PackedDet> h = pack(c, dets, ndet, dp);
+        auto bins = rect_blocks(c, h, (long long)ncur, (long long)ncur, true);
+        const double *dx = (const double *)to_device(c, c->outH, coef, ncur * 8);
+        const bool vdev = is_device_ptr(v), sdev = is_device_ptr(s);
+        double *dv = vdev ? v : c->fockbuf.get<double>(2 * nc);
+        double *ds = sdev ? s : (vdev ? c->fockbuf.get<double>(nc) : dv + nc);
+        std::vector<LinDesc> lz = {{nullptr, nullptr, nullptr, dv, 0, 0, 0}, {nullptr, nullptr, nullptr, ds, 0, 0, 0}};
+        Uploader up(c, 64u << 20);
+        launch_lincomb(c->st, up.put(lz), 2, (long long)nc);
+        EvalArgs args = base_args(c, dp, mode);
+        args.x = dx;
+        args.y = dv;
+        args.ys = ds;
+        args.ldo = (long long)ncur;  // row index offset of the outputs
+        HIPCHK(hipEventRecord(c->ev[1], c->st));
+        for (auto &kv : bins) {
+            const int key = kv.first, lab = key >> 1;
+            EvalBlock *db = up.put(kv.second);
+            launch_eval_blocks(c->st, EVAL_PROJECT, lab / 5, lab % 5, key & 1, db, (int)kv.second.size(),
+                               tiles_of(kv.second), args);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[2], c->st));
+        if (!vdev) HIPCHK(hipMemcpyAsync(v, dv, nc * 8, hipMemcpyDeviceToHost, c->st));
+        if (!sdev) HIPCHK(hipMemcpyAsync(s, ds, nc * 8, hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
         c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
         c->t_ms[1] = 0.0;

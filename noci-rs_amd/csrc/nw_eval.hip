@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_eval_blocks(const EvalBlock *__restrict
     const int li = threadIdx.x >> 4, lj = threadIdx.x & 15;
     const long long i = B.i0 + ti * 16 + li, j = B.j0 + tj * 16 + lj;
     bool valid = (ti * 16 + li < B.ni) && (tj * 16 + lj < B.nj) && !(B.tri && j < i);
-    if constexpr (KIND != EVAL_MATVEC) {
+    if constexpr (KIND == EVAL_HS || KIND == EVAL_FS) {
         if (!valid) return;
     }
     const PairInfo &P = a.pairs[B.pair];
@@ -93,14 +93,28 @@ __global__ __launch_bounds__(256) void k_eval_blocks(const EvalBlock *__restrict
     double v0 = 0.0, v1 = 0.0;
     if (valid) {
         const PackedDet bra = a.dets[i], ket = a.dets[j];
-        if constexpr (KIND == EVAL_HS) {
+        if constexpr (KIND == EVAL_HS || KIND == EVAL_PROJECT) {
             if constexpr (M0) hs_m0<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);
             else hs_gen<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, a.tol, a.mode, v0, v1);
         } else {
             fs_elem<LA, LB>(P, slab, bra, ket, a.nmo, a.tol, a.mode, v0, v1);
         }
     }
-    if constexpr (KIND != EVAL_MATVEC) {
+    if constexpr (KIND == EVAL_PROJECT) {
+        // build_candidate_current_h + build_candidate_v (src/snoci/operators.rs:151, 991):
+        // v_a += H_ai c_i and s_a += S_ai c_i over the current-space columns.
+        const double xj = valid ? a.x[j] : 0.0;
+        double rh = valid ? v0 * xj : 0.0, rs = valid ? v1 * xj : 0.0;
+#pragma unroll
+        for (int o = 8; o >= 1; o >>= 1) {
+            rh += __shfl_xor(rh, o, 16);
+            rs += __shfl_xor(rs, o, 16);
+        }
+        if (lj == 0 && ti * 16 + li < B.ni) {
+            unsafeAtomicAdd(a.y + (i - a.ldo), rh);
+            unsafeAtomicAdd(a.ys + (i - a.ldo), rs);
+        }
+    } else if constexpr (KIND != EVAL_MATVEC) {
         a.A[i * a.ldo + j] = v0;
         a.B[i * a.ldo + j] = v1;
         if (i != j) {
@@ -175,6 +189,9 @@ void launch_eval_blocks(hipStream_t st, int kind, int la, int lb, bool m0, const
             else hipLaunchKernelGGL((k_eval_blocks<EVAL_HS, A, B, false>), grid, block, 0, st, blocks, nblocks, args); \
         } else if (kind == EVAL_FS) {                                                                          \
             hipLaunchKernelGGL((k_eval_blocks<EVAL_FS, A, B, false>), grid, block, 0, st, blocks, nblocks, args);  \
+        } else if (kind == EVAL_PROJECT) {                                                                     \
+            if (m0) hipLaunchKernelGGL((k_eval_blocks<EVAL_PROJECT, A, B, true>), grid, block, 0, st, blocks, nblocks, args); \
+            else hipLaunchKernelGGL((k_eval_blocks<EVAL_PROJECT, A, B, false>), grid, block, 0, st, blocks, nblocks, args); \
         } else {                                                                                               \
             hipLaunchKernelGGL((k_eval_blocks<EVAL_MATVEC, A, B, false>), grid, block, 0, st, blocks, nblocks, args); \
         }                                                                                                      \

@@ -199,6 +199,8 @@ struct Run { long long start, len; int parent, ra, rb; };
 std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked);
 std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vector<PackedDet> &h, bool need_hcol);
 long long tiles_of(const std::vector<EvalBlock> &v);
+std::map<int, std::vector<EvalBlock>> rect_blocks(nw_ctx *c, const std::vector<PackedDet> &h, long long nrow0,
+                                                  long long ncol, bool need_hcol);
 EvalArgs base_args(nw_ctx *c, const PackedDet *dp, int mode);
 
 }  // namespace nwi

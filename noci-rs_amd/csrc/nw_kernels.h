@@ -58,7 +58,7 @@ struct EvalBlock {
 };
 struct EvalReq { int a, b; long long out; };
 
-enum { EVAL_HS = 0, EVAL_FS = 1, EVAL_MATVEC = 2 };
+enum { EVAL_HS = 0, EVAL_FS = 1, EVAL_MATVEC = 2, EVAL_PROJECT = 3 };
 
 struct EvalArgs {
     const PackedDet *dets;

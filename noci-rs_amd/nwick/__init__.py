@@ -30,10 +30,12 @@ STATUS = {0: "NW_OK", 1: "NW_E_ARG", 2: "NW_E_SHAPE", 3: "NW_E_HIP", 4: "NW_E_ME
 
 # Every symbol the header declares.
 EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matrix", "nw_update_fock",
-           "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_set_precision", "nw_set_mixed_floor", "nw_pair_slab", "nw_last_timings",
+           "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_project_hs", "nw_set_precision", "nw_set_mixed_floor", "nw_pair_slab", "nw_last_timings",
            "nw_last_error", "nw_destroy", "nw_batch_create", "nw_batch_add", "nw_batch_plan", "nw_batch_run",
            "nw_batch_result", "nw_batch_schedule", "nw_batch_set_schedule", "nw_batch_last_ms",
-           "nw_batch_last_error", "nw_batch_destroy"]
+           "nw_batch_last_error", "nw_batch_destroy", "nw_mlp_create", "nw_mlp_num_params", "nw_mlp_set_params",
+           "nw_mlp_get_params", "nw_mlp_forward", "nw_mlp_grad", "nw_mlp_step", "nw_mlp_train", "nw_mlp_last_error",
+           "nw_mlp_destroy"]
 PREC_FP64 = 0
 PREC_MIXED = 1
 BATCH_TUNE = 1
@@ -82,6 +84,7 @@ def load():
     L.nw_hs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
     L.nw_pair_slab.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
     L.nw_set_precision.argtypes = [vp, ct.c_int]
+    L.nw_project_hs.argtypes = [vp, vp, ct.c_uint64, ct.c_uint64, vp, vp, vp, ct.c_int]
     L.nw_set_mixed_floor.argtypes = [vp, ct.c_double]
     L.nw_update_fock.argtypes = [vp, vp, vp]
     L.nw_fs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
@@ -227,6 +230,17 @@ class Engine:
             y = np.empty(n)
         self._check(self.L.nw_matvec_m(self.ctx, _ptr(dets), n, float(e0), _ptr(x), _ptr(y), _ptr(ys), int(mode)))
         return y if ys is None else (y, ys)
+
+    def project_hs(self, dets, ncur, coef, v=None, s=None, mode=MODE_REFERENCE):
+        """v_a = sum_i H_ai c_i, s_a = sum_i S_ai c_i for every candidate a (dets[ncur:]) over the
+        current space dets[:ncur]; arrays may be torch device tensors."""
+        n = self._ndet(dets)
+        nc = n - ncur
+        v = np.empty(nc) if v is None else v
+        s = np.empty(nc) if s is None else s
+        coef = coef if hasattr(coef, "data_ptr") else np.ascontiguousarray(coef, dtype=np.float64)
+        self._check(self.L.nw_project_hs(self.ctx, _ptr(dets), n, int(ncur), _ptr(coef), _ptr(v), _ptr(s), int(mode)))
+        return v, s
 
     def pair_slab(self, x, w):
         off = np.zeros(95, dtype=np.int64)

@@ -1,0 +1,414 @@
+"""Learned screening of selected-NOCI candidates.
+
+Selection in noci-rs (src/snoci/step.rs:178-440, select_candidates
+operators.rs:1064) scores every candidate exactly.  Here the exact score is
+the Epstein-Nesbet-like contribution
+
+    score(D) = |<D|H - E S|Psi>|^2 / |E - H_DD|,   <D|H - E S|Psi> = v_D - E s_D
+
+with v, s from the engine's projection of the candidates on the current state
+(nw_project_hs).  A small network (nwick.mlp, trained on the GPU by the
+project's own kernels) learns log10 score from 24 cheap pair descriptors on
+the first selection iterations; later iterations evaluate exactly only the
+candidates in the network's order until a chunk no longer holds one close to
+the current k-th best exact score, plus a random audit, and fall back to
+exhaustive exact screening when the audit shows that the network missed a
+candidate that would have been selected.
+
+Candidate determinants are generated vectorised (occupied-first references,
+nmo <= 64): the excitation phase of holes h (removed highest first) and
+particles p (added lowest first) on the occupied-first string is
+(-1)^(sum h + n_h (n_occ - n_h) + n_h (n_h - 1) / 2), the closed form of
+excitation_phase_bits for that case.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .basis import DET_DTYPE, to_records
+
+N_FEATURES = 24
+_TYPES = (("a", 1, 0), ("b", 0, 1), ("aa", 2, 0), ("bb", 0, 2), ("ab", 1, 1))
+
+
+@dataclass
+class Candidates:
+    """Vectorised candidate set: parent, per spin holes/particles (-1 padded, 2 slots)."""
+    parent: np.ndarray
+    ah: np.ndarray
+    ap: np.ndarray
+    bh: np.ndarray
+    bp: np.ndarray
+    nocc_a: int
+    nocc_b: int
+
+    def __len__(self):
+        return len(self.parent)
+
+    def subset(self, idx):
+        return Candidates(self.parent[idx], self.ah[idx], self.ap[idx], self.bh[idx], self.bp[idx], self.nocc_a,
+                          self.nocc_b)
+
+    def ranks(self):
+        return (self.ah >= 0).sum(1), (self.bh >= 0).sum(1)
+
+
+def _mask(idx):
+    m = np.zeros(len(idx), dtype=np.uint64)
+    for c in range(idx.shape[1]):
+        v = idx[:, c]
+        m |= np.where(v >= 0, np.left_shift(np.uint64(1), np.maximum(v, 0).astype(np.uint64)), np.uint64(0))
+    return m
+
+
+def _phase(h, nocc):
+    nh = (h >= 0).sum(1)
+    odd = (np.where(h >= 0, h, 0).sum(1) + nh * (nocc - nh) + nh * (nh - 1) // 2) & 1
+    return np.where(odd == 1, -1.0, 1.0)
+
+
+def records(c: Candidates) -> np.ndarray:
+    """nw_det records of the candidates (low 64-bit words; nmo <= 64)."""
+    n = len(c)
+    out = np.zeros(n, dtype=DET_DTYPE)
+    ref_a = np.uint64((1 << c.nocc_a) - 1)
+    ref_b = np.uint64((1 << c.nocc_b) - 1)
+    ha, pa, hb, pb = _mask(c.ah), _mask(c.ap), _mask(c.bh), _mask(c.bp)
+    out["parent"] = c.parent
+    out["pha"] = _phase(c.ah, c.nocc_a)
+    out["phb"] = _phase(c.bh, c.nocc_b)
+    out["oa"][:, 0] = (ref_a & ~ha) | pa
+    out["ob"][:, 0] = (ref_b & ~hb) | pb
+    out["ha"][:, 0], out["pa"][:, 0], out["hb"][:, 0], out["pb"][:, 0] = ha, pa, hb, pb
+    return out
+
+
+def random_candidates(nref, nmo, nocc_a, nocc_b, per_ref, seed, mix=(0.1, 0.1, 0.8 / 3, 0.8 / 3, 0.8 / 3)):
+    """Seeded unique singles and doubles per reference (default 10% alpha
+    singles, 10% beta singles, doubles split over aa/bb/ab), each reference's
+    block ordered by rank then indices (generation-like order)."""
+    assert nmo <= 64
+    rng = np.random.default_rng(seed)
+    parts = []
+    for r in range(nref):
+        blocks = []
+        from math import comb
+        va, vb = nmo - nocc_a, nmo - nocc_b
+        pool = [nocc_a * va, nocc_b * vb, comb(nocc_a, 2) * comb(va, 2), comb(nocc_b, 2) * comb(vb, 2),
+                nocc_a * va * nocc_b * vb]
+        # each type capped at half of what exists (rejection sampling stays
+        # cheap); the remainder goes to alpha-beta doubles
+        wants = [min(int(per_ref * f), p // 2) for f, p in zip(mix[:-1], pool[:-1])]
+        wants.append(per_ref - sum(wants))
+        if wants[-1] > pool[-1] // 2:
+            raise ValueError("per_ref too large for this orbital space")
+        for (name, ka, kb), want in zip(_TYPES, wants):
+            if want == 0:
+                continue
+            got = np.zeros((0, 8), dtype=np.int64)
+            tries = 0
+            while len(got) < want:
+                tries += 1
+                if tries > 50:
+                    raise ValueError(f"cannot draw {want} unique {name} excitations")
+                m = 2 * (want - len(got)) + 64
+                cols = []
+                for k, no, lo, hi in ((ka, nocc_a, 0, nocc_a), (ka, nocc_a, nocc_a, nmo), (kb, nocc_b, 0, nocc_b),
+                                      (kb, nocc_b, nocc_b, nmo)):
+                    if k == 0:
+                        cols.append(np.full((m, 2), -1))
+                        continue
+                    x = np.sort(np.stack([rng.choice(np.arange(lo, hi), size=k, replace=False) for _ in range(m)]), 1)
+                    if k == 1:
+                        x = np.concatenate([x, np.full((m, 1), -1)], 1)
+                    cols.append(x)
+                new = np.concatenate(cols, 1)
+                got = np.unique(np.concatenate([got, new]), axis=0)
+            sel = rng.choice(len(got), want, replace=False)
+            blocks.append(got[np.sort(sel)])
+        e = np.concatenate(blocks)
+        ra = (e[:, 0:2] >= 0).sum(1)
+        rb = (e[:, 4:6] >= 0).sum(1)
+        order = np.lexsort(tuple(e[:, k] for k in range(7, -1, -1)) + (ra, ra + rb))
+        e = e[order]
+        parts.append(np.concatenate([np.full((len(e), 1), r), e], 1))
+    a = np.concatenate(parts)
+    return Candidates(a[:, 0].astype(np.uint32), a[:, 1:3], a[:, 3:5], a[:, 5:7], a[:, 7:9], nocc_a, nocc_b)
+
+
+# ---------------------------------------------------------------------------
+# Descriptors
+# ---------------------------------------------------------------------------
+def reference_fock(sysm):
+    """Per reference and spin: the reference determinant's own spin Fock matrix
+    in its MO basis (h + J - K of its occupied densities, src/scf/kernels.rs:46)."""
+    from .pt2 import fock
+    out = []
+    for Ca, Cb in sysm.refs:
+        Da = Ca[:, :sysm.nocc_a] @ Ca[:, :sysm.nocc_a].T
+        Db = Cb[:, :sysm.nocc_b] @ Cb[:, :sysm.nocc_b].T
+        fa, fb = fock(sysm.h, sysm.eri, Da, Db)
+        out.append((Ca.T @ fa @ Ca, Cb.T @ fb @ Cb))
+    return out
+
+
+def _pair_sv(sysm, p, d, sp):
+    no = sysm.nocc_a if sp == 0 else sysm.nocc_b
+    A = sysm.refs[p][sp][:, :no].T @ sysm.S @ sysm.refs[d][sp][:, :no]
+    return np.sort(np.linalg.svd(A, compute_uv=False))
+
+
+def features(sysm, cand: Candidates, fmo, state, couplings):
+    """24 pair descriptors of each candidate with the current state:
+    excitation ranks; hole/particle orbital energies of the parent's Fock
+    (2+2 per spin); the orbital-energy gap; the exact couplings |H_Dp| to the
+    parent reference and |H_Dd| to the dominant reference (two Wick elements
+    per candidate, against the whole current space for the exact score) and
+    the first-order amplitude |H_Dp| / |H_DD - E|; the one-body Fock coupling
+    of the excitation; the parent's weight and diagonal energy in the current
+    state; and the occupied-overlap singular values, zero count and reduced
+    overlap between the parent and the dominant reference."""
+    nref = sysm.nref
+    wpar = state["parent_weight"]
+    dom = int(np.argmax(wpar))
+    pf = np.zeros((nref, 10))
+    for p in range(nref):
+        sa, sb = _pair_sv(sysm, p, dom, 0), _pair_sv(sysm, p, dom, 1)
+        nz = np.concatenate([sa, sb])
+        zc = (nz <= 1e-8).sum()
+        red = np.prod(np.where(nz > 1e-8, nz, 1.0))
+        pf[p] = [wpar[p], state["parent_energy"][p] - state["energy"], np.log10(max(red, 1e-30)), zc,
+                 sa[0], sa[1] if len(sa) > 1 else 1.0, sb[0], sb[1] if len(sb) > 1 else 1.0, nz.mean(),
+                 1.0 if p == dom else 0.0]
+    n = len(cand)
+    F = np.zeros((n, N_FEATURES), dtype=np.float32)
+    ra, rb = cand.ranks()
+    F[:, 0], F[:, 1] = ra, rb
+    par = cand.parent.astype(np.int64)
+    eps = [np.stack([np.diag(fmo[p][sp]) for p in range(nref)]) for sp in (0, 1)]
+    de = np.zeros(n)
+    coup = np.zeros(n)
+    for sp, (h, q, col) in enumerate(((cand.ah, cand.ap, 2), (cand.bh, cand.bp, 6))):
+        e = eps[sp]
+        for k in range(2):
+            hk, qk = h[:, k], q[:, k]
+            eh = np.where(hk >= 0, e[par, np.maximum(hk, 0)], 0.0)
+            ep = np.where(qk >= 0, e[par, np.maximum(qk, 0)], 0.0)
+            F[:, col + k], F[:, col + 2 + k] = eh, ep
+            de += ep - eh
+        fm = np.stack([fmo[p][sp] for p in range(nref)])
+        h0, q0, h1, q1 = h[:, 0], q[:, 0], h[:, 1], q[:, 1]
+        single = (h0 >= 0) & (h1 < 0)
+        double = h1 >= 0
+        f00 = fm[par, np.maximum(h0, 0), np.maximum(q0, 0)]
+        f11 = fm[par, np.maximum(h1, 0), np.maximum(q1, 0)]
+        f01 = fm[par, np.maximum(h0, 0), np.maximum(q1, 0)]
+        f10 = fm[par, np.maximum(h1, 0), np.maximum(q0, 0)]
+        coup += np.where(single, np.abs(f00), 0.0) + np.where(double, np.abs(f00 * f11 - f01 * f10), 0.0)
+    hp, hd, hdiag = couplings
+    F[:, 10] = de
+    F[:, 11] = np.log10(np.abs(hp) + 1e-14)
+    F[:, 12:20] = pf[par][:, :8]
+    F[:, 20] = np.log10(np.abs(hd) + 1e-14)
+    F[:, 21] = np.log10(np.abs(hp) / np.maximum(np.abs(hdiag - state["energy"]), 1e-3) + 1e-14)
+    F[:, 22] = np.log10(coup + 1e-12)
+    F[:, 23] = pf[par][:, 8]
+    return F
+
+
+# ---------------------------------------------------------------------------
+# Exact scores through the engine
+# ---------------------------------------------------------------------------
+class ExactScorer:
+    """Candidate projections v, s on the current state (nw_project_hs) and the
+    candidates' diagonal H_DD (nw_hs_batched), all on the GPU."""
+
+    def __init__(self, eng, cand: Candidates, device="cuda:0", mode=0):
+        import torch
+        self.torch, self.eng, self.mode = torch, eng, mode
+        self.dev = torch.device(device)
+        rec = records(cand)
+        self.rec = rec
+        self.rec_d = torch.from_numpy(rec.view(np.uint8).copy()).to(self.dev).view(-1, DET_DTYPE.itemsize)
+        n = len(cand)
+        req = np.stack([np.arange(n), np.arange(n)], 1).astype(np.uint64)
+        self.hdiag, self.sdiag = eng.hs_batched(rec, req, mode=mode)
+        self.n_exact = 0
+
+    def coupling(self, refdet_rec):
+        """H between every candidate (bra) and one determinant (ket), as a vector."""
+        n = len(self.rec)
+        allrec = np.concatenate([self.rec, refdet_rec])
+        req = np.stack([np.arange(n), np.full(n, n)], 1).astype(np.uint64)
+        h, _ = self.eng.hs_batched(allrec, req, mode=self.mode)
+        return h
+
+    def scores(self, cur_rec, coef, energy, idx=None):
+        """Exact scores of the candidates idx (all when None), in idx order."""
+        torch = self.torch
+        cur = torch.from_numpy(cur_rec.view(np.uint8).copy()).to(self.dev).view(-1, DET_DTYPE.itemsize)
+        if idx is None:
+            sub = self.rec_d
+            n = sub.shape[0]
+            hd = self.hdiag
+        else:
+            it = torch.as_tensor(np.asarray(idx, dtype=np.int64), device=self.dev)
+            sub = self.rec_d.index_select(0, it)
+            n = len(idx)
+            hd = self.hdiag[idx]
+        allrec = torch.cat([cur, sub]).contiguous()
+        v = torch.empty(n, dtype=torch.float64, device=self.dev)
+        s = torch.empty(n, dtype=torch.float64, device=self.dev)
+        c = torch.as_tensor(np.ascontiguousarray(coef), dtype=torch.float64, device=self.dev)
+        self.eng.project_hs(allrec, cur.shape[0], c, v=v, s=s, mode=self.mode)
+        num = (v - energy * s).cpu().numpy()
+        self.n_exact += n
+        return num * num / np.maximum(np.abs(energy - hd), 1e-3)
+
+
+def solve_current(eng, cur_rec, mode=0, tol=1e-8):
+    """Lowest root of the current space (general_evp of src/maths/linalg.rs:239)."""
+    from .pt2 import general_evp
+    H, S = eng.hs_matrix(cur_rec, mode=mode)
+    e, C = general_evp(H, S, tol)
+    return float(e[0]), C[:, 0], H
+
+
+@dataclass
+class SelectionResult:
+    energies: list = field(default_factory=list)
+    selected: list = field(default_factory=list)
+    screen_s: list = field(default_factory=list)
+    exact_evals: list = field(default_factory=list)
+    audit_misses: list = field(default_factory=list)
+    fallbacks: int = 0
+    recall: list = field(default_factory=list)
+    captured: list = field(default_factory=list)  # exact score of the picks / of the true best picks
+    train: dict = field(default_factory=dict)
+
+
+def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_iters=2, top_frac=0.05,
+           audit_frac=0.01, chunk_frac=0.01, chunk_ratio=0.1, max_frac=0.5, epochs=8, batch=4096, lr=2e-3, seed=0,
+           mode=0, measure_recall=False, log=None):
+    """Selected NOCI from the references: each iteration adds the k_add
+    best-scoring candidates.  learned=True trains the network on the exact
+    scores of the first learn_iters iterations and screens the later ones.
+    Timings in screen_s cover scoring and picking (features, inference,
+    exact evaluation, audit and fallback); measure_recall adds an untimed
+    exhaustive scoring of each screened iteration to report recall."""
+    from .basis import Det
+    from .mlp import MLP
+    refs = [Det(i, (1 << sysm.nocc_a) - 1, (1 << sysm.nocc_b) - 1) for i in range(sysm.nref)]
+    cur_rec = to_records(refs)
+    scorer = ExactScorer(eng, cand, mode=mode)
+    fmo = reference_fock(sysm)
+    rng = np.random.default_rng(seed)
+    taken = np.zeros(len(cand), dtype=bool)
+    res = SelectionResult()
+    Xs, ys = [], []
+    net, mu, sd = None, None, None
+    e_ref0 = None
+    ref_rec = to_records(refs)
+    # exact coupling of every candidate with its own parent reference
+    hpar = np.zeros(len(cand))
+    for p in range(sysm.nref):
+        sel = cand.parent == p
+        h = scorer.coupling(ref_rec[p:p + 1])
+        hpar[sel] = h[sel]
+    for it in range(iters):
+        e, c, H = solve_current(eng, cur_rec, mode)
+        if e_ref0 is None:
+            e_ref0 = e
+        res.energies.append(e)
+        par = cur_rec["parent"].astype(int)
+        wpar = np.bincount(par, weights=c * c, minlength=sysm.nref)
+        isref = (cur_rec["ha"][:, 0] == 0) & (cur_rec["hb"][:, 0] == 0)
+        penergy = np.zeros(sysm.nref)
+        penergy[par[isref]] = np.diag(H)[isref]
+        state = {"parent_weight": wpar, "parent_energy": penergy, "energy": e, "reference_energy": e_ref0}
+        t0 = time.perf_counter()
+        cpl = None
+        if learned:
+            hdom = scorer.coupling(ref_rec[int(np.argmax(wpar)):int(np.argmax(wpar)) + 1])
+            cpl = (hpar, hdom, scorer.hdiag)
+        use_net = learned and it >= learn_iters
+        if not use_net:
+            sc = scorer.scores(cur_rec, c, e)
+            n_eval = len(cand)
+            if learned:
+                Xs.append(features(sysm, cand, fmo, state, cpl))
+                ys.append(np.log10(sc + 1e-20).astype(np.float32))
+        else:
+            Xf = features(sysm, cand, fmo, state, cpl)
+            pred = net.forward((Xf - mu) / sd)
+            order = np.argsort(-np.where(taken, -np.inf, pred), kind="stable")
+            order = order[:len(cand) - int(taken.sum())]
+            sc = np.full(len(cand), -1.0)
+            # exact evaluation in order of predicted score: the top fraction,
+            # then further chunks while a chunk still holds a candidate within
+            # chunk_ratio of the current k_add-th best exact score
+            chunk = max(k_add, int(chunk_frac * len(cand)))
+            done = min(len(order), max(k_add, int(top_frac * len(cand))))
+            first = np.sort(order[:done])
+            sc[first] = scorer.scores(cur_rec, c, e, first)
+            while done < len(order) and done < max_frac * len(cand):
+                valid = np.sort(sc[sc >= 0])
+                kth = valid[-k_add] if len(valid) >= k_add else 0.0
+                nxt = np.sort(order[done:done + chunk])
+                sc[nxt] = scorer.scores(cur_rec, c, e, nxt)
+                done += len(nxt)
+                if sc[nxt].max() < chunk_ratio * kth:
+                    break
+            top = order[:done]
+            rest = order[done:]
+            audit = rng.choice(rest, min(len(rest), max(1, int(audit_frac * len(cand)))), replace=False) \
+                if len(rest) else np.zeros(0, dtype=np.int64)
+            audit = np.sort(audit)
+            if len(audit):
+                sc[audit] = scorer.scores(cur_rec, c, e, audit)
+            n_eval = done + len(audit)
+            # the exact evaluation judges the network: an audited candidate
+            # it ranked out that beats the k_add-th pick is a miss
+            valid = np.sort(sc[(sc >= 0) & ~taken])
+            kth = valid[-k_add] if len(valid) >= k_add else 0.0
+            misses = int((sc[audit] > kth).sum()) if len(audit) else 0
+            res.audit_misses.append(misses)
+            if misses > 0 or done >= max_frac * len(cand):
+                res.fallbacks += 1
+                sc = scorer.scores(cur_rec, c, e)
+                n_eval += len(cand)
+        sc = np.where(taken, -1.0, sc)
+        pick = np.argsort(-sc, kind="stable")[:k_add]
+        pick = pick[sc[pick] > 0]
+        res.screen_s.append(time.perf_counter() - t0)
+        res.exact_evals.append(int(n_eval))
+        if use_net and measure_recall:
+            full = scorer.scores(cur_rec, c, e)
+            scorer.n_exact -= len(cand)
+            best = np.argsort(-np.where(taken, -1.0, full))[:k_add]
+            res.recall.append(float(np.isin(best, top).mean()))
+            res.captured.append(float(full[np.isin(np.arange(len(cand)), pick)].sum() / full[best].sum()))
+        taken[pick] = True
+        res.selected.append(pick)
+        cur_rec = np.concatenate([cur_rec, scorer.rec[np.sort(pick)]])
+        cur_rec = cur_rec[np.argsort(cur_rec["parent"], kind="stable")]  # runs of one parent together
+        if log:
+            log(f"selection iteration {it}: E={e:.10f} evaluated={n_eval} picked={len(pick)} "
+                f"screen {res.screen_s[-1]:.3f} s" + (f" misses={res.audit_misses[-1]}" if use_net else ""))
+        if learned and it == learn_iters - 1:
+            X = np.concatenate(Xs)
+            y = np.concatenate(ys)
+            mu, sd = X.mean(0), X.std(0) + 1e-6
+            net = MLP(N_FEATURES, seed=seed)
+            t1 = time.perf_counter()
+            # emphasise the top of the distribution: the selection only needs its order
+            wt = np.where(y >= np.quantile(y, 0.9), 5.0, 1.0).astype(np.float32)
+            el = net.train((X - mu) / sd, y - y.mean(), wt, epochs=epochs, batch=batch, lr=lr, seed=seed)
+            res.train = {"samples": int(len(y)), "epochs": epochs, "loss": [float(v) for v in el],
+                         "seconds": time.perf_counter() - t1}
+    e, c, H = solve_current(eng, cur_rec, mode)
+    res.energies.append(e)
+    return res
