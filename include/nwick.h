@@ -8,6 +8,9 @@
  *
  * Buffers: every pointer argument may be host memory or device memory of the
  * context's GPU; the library detects which and copies only host buffers.
+ * The library works on its own HIP stream: device inputs must be complete
+ * when a call is made (synchronise the stream that produced them), and every
+ * output is complete when the call returns.
  * Calls on one context are serialised internally; distinct contexts are
  * independent.  All functions return 0 (NW_OK) or an nw_status code;
  * nw_last_error() describes the last failure on a context.

@@ -114,6 +114,18 @@ def load():
     return L
 
 
+def sync_torch(*arrays):
+    """The engine runs on its own HIP stream and returns only after that stream
+    has finished; torch device tensors handed to it may still be in flight on
+    torch's current stream, so finish that stream first (reads of inputs and
+    writes of outputs both depend on it)."""
+    for a in arrays:
+        if a is not None and getattr(a, "is_cuda", False):
+            import torch
+            torch.cuda.current_stream(a.device).synchronize()
+            return
+
+
 def _ptr(a):
     """Host numpy array or torch tensor (device or host) -> void pointer."""
     if a is None:
@@ -190,12 +202,14 @@ class Engine:
             H = np.empty((n, n)); S = np.empty((n, n))
         else:
             H, S = out
+        sync_torch(dets, H, S)
         self._check(self.L.nw_hs_matrix(self.ctx, _ptr(dets), n, _ptr(H), _ptr(S), int(mode)))
         return H, S
 
     def hs_batched(self, dets, req, mode=MODE_REFERENCE):
         req = np.ascontiguousarray(req, dtype=np.uint64).reshape(-1, 2)
         h = np.empty(len(req)); s = np.empty(len(req))
+        sync_torch(dets)
         self._check(self.L.nw_hs_batched(self.ctx, _ptr(dets), len(dets), _ptr(req), len(req), _ptr(h), _ptr(s),
                                          int(mode)))
         return h, s
@@ -213,12 +227,14 @@ class Engine:
     def fs_matrix(self, dets, mode=MODE_REFERENCE, out=None):
         n = self._ndet(dets)
         F, S = (np.empty((n, n)), np.empty((n, n))) if out is None else out
+        sync_torch(dets, F, S)
         self._check(self.L.nw_fs_matrix(self.ctx, _ptr(dets), n, _ptr(F), _ptr(S), int(mode)))
         return F, S
 
     def fs_batched(self, dets, req, mode=MODE_REFERENCE):
         req = np.ascontiguousarray(req, dtype=np.uint64).reshape(-1, 2)
         f = np.empty(len(req)); s = np.empty(len(req))
+        sync_torch(dets)
         self._check(self.L.nw_fs_batched(self.ctx, _ptr(dets), self._ndet(dets), _ptr(req), len(req), _ptr(f),
                                          _ptr(s), int(mode)))
         return f, s
@@ -228,6 +244,7 @@ class Engine:
         n = self._ndet(dets)
         if y is None:
             y = np.empty(n)
+        sync_torch(dets, x, y, ys)
         self._check(self.L.nw_matvec_m(self.ctx, _ptr(dets), n, float(e0), _ptr(x), _ptr(y), _ptr(ys), int(mode)))
         return y if ys is None else (y, ys)
 
@@ -239,6 +256,7 @@ class Engine:
         v = np.empty(nc) if v is None else v
         s = np.empty(nc) if s is None else s
         coef = coef if hasattr(coef, "data_ptr") else np.ascontiguousarray(coef, dtype=np.float64)
+        sync_torch(dets, coef, v, s)
         self._check(self.L.nw_project_hs(self.ctx, _ptr(dets), n, int(ncur), _ptr(coef), _ptr(v), _ptr(s), int(mode)))
         return v, s
 
@@ -288,6 +306,7 @@ class Batch:
     def add(self, eng, dets, mode=MODE_REFERENCE):
         idx = ct.c_int()
         n = Engine._ndet(dets)
+        sync_torch(dets)
         self._check(self.L.nw_batch_add(self.b, eng.ctx, _ptr(dets), n, int(mode), ct.byref(idx)))
         self.engines.append(eng)
         self.sizes.append(n)
@@ -302,6 +321,7 @@ class Batch:
     def result(self, index, out=None):
         n = self.sizes[index]
         H, S = (np.empty((n, n)), np.empty((n, n))) if out is None else out
+        sync_torch(H, S)
         self._check(self.L.nw_batch_result(self.b, index, _ptr(H), _ptr(S)))
         return H, S
 

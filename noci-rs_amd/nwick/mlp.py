@@ -5,7 +5,7 @@ import ctypes as ct
 
 import numpy as np
 
-from . import NwickError, _ptr, load
+from . import NwickError, _ptr, load, sync_torch
 
 N_HIDDEN = 128
 
@@ -69,6 +69,7 @@ class MLP:
         X = _f32(X)
         n = X.shape[0]
         y = np.empty(n, dtype=np.float32) if out is None else out
+        sync_torch(X, y)
         self._check(self.L.nw_mlp_forward(self.h, _ptr(X), n, _ptr(y)))
         return y
 
@@ -76,18 +77,21 @@ class MLP:
         X, t, w = _f32(X), _f32(t), _f32(w)
         loss = np.zeros(1, dtype=np.float32)
         g = np.empty(self.n_params, dtype=np.float32)
+        sync_torch(X, t, w)
         self._check(self.L.nw_mlp_grad(self.h, _ptr(X), _ptr(t), _ptr(w), X.shape[0], _ptr(loss), _ptr(g)))
         return float(loss[0]), g
 
     def step(self, X, t, w=None, lr=1e-3):
         X, t, w = _f32(X), _f32(t), _f32(w)
         loss = np.zeros(1, dtype=np.float32)
+        sync_torch(X, t, w)
         self._check(self.L.nw_mlp_step(self.h, _ptr(X), _ptr(t), _ptr(w), X.shape[0], float(lr), _ptr(loss)))
         return float(loss[0])
 
     def train(self, X, t, w=None, epochs=5, batch=4096, lr=1e-3, seed=0):
         X, t, w = _f32(X), _f32(t), _f32(w)
         el = np.zeros(epochs, dtype=np.float32)
+        sync_torch(X, t, w)
         self._check(self.L.nw_mlp_train(self.h, _ptr(X), _ptr(t), _ptr(w), X.shape[0], int(epochs), int(batch),
                                         float(lr), int(seed), _ptr(el)))
         return el

@@ -121,7 +121,8 @@ def random_candidates(nref, nmo, nocc_a, nocc_b, per_ref, seed, mix=(0.1, 0.1, 0
                     if k == 0:
                         cols.append(np.full((m, 2), -1))
                         continue
-                    x = np.sort(np.stack([rng.choice(np.arange(lo, hi), size=k, replace=False) for _ in range(m)]), 1)
+                    # k distinct values of [lo, hi) per row: the k smallest of i.i.d. keys
+                    x = np.sort(np.argpartition(rng.random((m, hi - lo)), k - 1, axis=1)[:, :k], 1) + lo
                     if k == 1:
                         x = np.concatenate([x, np.full((m, 1), -1)], 1)
                     cols.append(x)
@@ -161,16 +162,11 @@ def _pair_sv(sysm, p, d, sp):
     return np.sort(np.linalg.svd(A, compute_uv=False))
 
 
-def features(sysm, cand: Candidates, fmo, state, couplings):
-    """24 pair descriptors of each candidate with the current state:
-    excitation ranks; hole/particle orbital energies of the parent's Fock
-    (2+2 per spin); the orbital-energy gap; the exact couplings |H_Dp| to the
-    parent reference and |H_Dd| to the dominant reference (two Wick elements
-    per candidate, against the whole current space for the exact score) and
-    the first-order amplitude |H_Dp| / |H_DD - E|; the one-body Fock coupling
-    of the excitation; the parent's weight and diagonal energy in the current
-    state; and the occupied-overlap singular values, zero count and reduced
-    overlap between the parent and the dominant reference."""
+def parent_table(sysm, state):
+    """Per parent reference: weight in the current state, diagonal energy
+    relative to E, log10 reduced overlap, zero count, the two smallest
+    occupied-overlap singular values per spin and their mean, all with the
+    dominant reference (largest weight)."""
     nref = sysm.nref
     wpar = state["parent_weight"]
     dom = int(np.argmax(wpar))
@@ -183,6 +179,15 @@ def features(sysm, cand: Candidates, fmo, state, couplings):
         pf[p] = [wpar[p], state["parent_energy"][p] - state["energy"], np.log10(max(red, 1e-30)), zc,
                  sa[0], sa[1] if len(sa) > 1 else 1.0, sb[0], sb[1] if len(sb) > 1 else 1.0, nz.mean(),
                  1.0 if p == dom else 0.0]
+    return pf
+
+
+def static_features(sysm, cand: Candidates, fmo, hpar):
+    """The state-independent descriptor columns: excitation ranks (0-1), hole
+    and particle orbital energies of the parent's Fock per spin (2-9), the
+    orbital-energy gap (10), the exact coupling |H_Dp| with the parent
+    reference (11) and the one-body Fock coupling of the excitation (22)."""
+    nref = sysm.nref
     n = len(cand)
     F = np.zeros((n, N_FEATURES), dtype=np.float32)
     ra, rb = cand.ranks()
@@ -208,15 +213,34 @@ def features(sysm, cand: Candidates, fmo, state, couplings):
         f01 = fm[par, np.maximum(h0, 0), np.maximum(q1, 0)]
         f10 = fm[par, np.maximum(h1, 0), np.maximum(q0, 0)]
         coup += np.where(single, np.abs(f00), 0.0) + np.where(double, np.abs(f00 * f11 - f01 * f10), 0.0)
-    hp, hd, hdiag = couplings
     F[:, 10] = de
-    F[:, 11] = np.log10(np.abs(hp) + 1e-14)
-    F[:, 12:20] = pf[par][:, :8]
-    F[:, 20] = np.log10(np.abs(hd) + 1e-14)
-    F[:, 21] = np.log10(np.abs(hp) / np.maximum(np.abs(hdiag - state["energy"]), 1e-3) + 1e-14)
+    F[:, 11] = np.log10(np.abs(hpar) + 1e-14)
     F[:, 22] = np.log10(coup + 1e-12)
-    F[:, 23] = pf[par][:, 8]
     return F
+
+
+def dynamic_features(xp, Fs, par, pf, hpar, hdom, hdiag, energy):
+    """Complete the descriptor rows with the state-dependent columns: the
+    parent table (12-19, 23), the exact coupling |H_Dd| with the dominant
+    reference (20) and the first-order amplitude |H_Dp| / |H_DD - E| (21).
+    xp is numpy or torch (then every array is a device tensor)."""
+    F = Fs.clone() if hasattr(Fs, "clone") else Fs.copy()
+    t = pf[par]
+    F[:, 12:20] = t[:, :8]
+    F[:, 23] = t[:, 8]
+    F[:, 20] = xp.log10(xp.abs(hdom) + 1e-14)
+    den = xp.clip(xp.abs(hdiag - energy), 1e-3, None) if xp is np else (hdiag - energy).abs().clamp(min=1e-3)
+    F[:, 21] = xp.log10(xp.abs(hpar) / den + 1e-14)
+    return F
+
+
+def features(sysm, cand: Candidates, fmo, state, couplings):
+    """All 24 pair descriptors on the host (numpy): static_features then
+    dynamic_features.  couplings = (H_Dp, H_Dd, H_DD)."""
+    hp, hd, hdiag = couplings
+    Fs = static_features(sysm, cand, fmo, hp)
+    pf = parent_table(sysm, state).astype(np.float32)
+    return dynamic_features(np, Fs, cand.parent.astype(np.int64), pf, hp, hd, hdiag, state["energy"])
 
 
 # ---------------------------------------------------------------------------
@@ -238,13 +262,26 @@ class ExactScorer:
         self.hdiag, self.sdiag = eng.hs_batched(rec, req, mode=mode)
         self.n_exact = 0
 
-    def coupling(self, refdet_rec):
-        """H between every candidate (bra) and one determinant (ket), as a vector."""
+    def coupling(self, ket_rec, ket_of):
+        """H_{D, k(D)} for every candidate D (bra) with ket ket_rec[ket_of[D]]."""
         n = len(self.rec)
-        allrec = np.concatenate([self.rec, refdet_rec])
-        req = np.stack([np.arange(n), np.full(n, n)], 1).astype(np.uint64)
+        allrec = np.concatenate([self.rec, ket_rec])
+        req = np.stack([np.arange(n), n + np.asarray(ket_of, dtype=np.int64)], 1).astype(np.uint64)
         h, _ = self.eng.hs_batched(allrec, req, mode=self.mode)
         return h
+
+    def project_single(self, ket_rec):
+        """H_{D,k} for every candidate D against one determinant k, as a device
+        tensor (one projection launch with a one-determinant current space)."""
+        torch = self.torch
+        k = torch.from_numpy(ket_rec.view(np.uint8).copy()).to(self.dev).view(-1, DET_DTYPE.itemsize)
+        allrec = torch.cat([k, self.rec_d]).contiguous()
+        n = self.rec_d.shape[0]
+        v = torch.empty(n, dtype=torch.float64, device=self.dev)
+        s = torch.empty(n, dtype=torch.float64, device=self.dev)
+        one = torch.ones(1, dtype=torch.float64, device=self.dev)
+        self.eng.project_hs(allrec, 1, one, v=v, s=s, mode=self.mode)
+        return v
 
     def scores(self, cur_rec, coef, energy, idx=None):
         """Exact scores of the candidates idx (all when None), in idx order."""
@@ -296,14 +333,16 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
     """Selected NOCI from the references: each iteration adds the k_add
     best-scoring candidates.  learned=True trains the network on the exact
     scores of the first learn_iters iterations and screens the later ones.
-    Timings in screen_s cover scoring and picking (features, inference,
+    Timings in screen_s cover scoring and picking (descriptors, inference,
     exact evaluation, audit and fallback); measure_recall adds an untimed
     exhaustive scoring of each screened iteration to report recall."""
     from .basis import Det
     from .mlp import MLP
     refs = [Det(i, (1 << sysm.nocc_a) - 1, (1 << sysm.nocc_b) - 1) for i in range(sysm.nref)]
     cur_rec = to_records(refs)
+    ref_rec = to_records(refs)
     scorer = ExactScorer(eng, cand, mode=mode)
+    torch, dev = scorer.torch, scorer.dev
     fmo = reference_fock(sysm)
     rng = np.random.default_rng(seed)
     taken = np.zeros(len(cand), dtype=bool)
@@ -311,13 +350,15 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
     Xs, ys = [], []
     net, mu, sd = None, None, None
     e_ref0 = None
-    ref_rec = to_records(refs)
-    # exact coupling of every candidate with its own parent reference
-    hpar = np.zeros(len(cand))
-    for p in range(sysm.nref):
-        sel = cand.parent == p
-        h = scorer.coupling(ref_rec[p:p + 1])
-        hpar[sel] = h[sel]
+    N = len(cand)
+    if learned:
+        t0 = time.perf_counter()
+        hpar = scorer.coupling(ref_rec, cand.parent.astype(np.int64))
+        Fs = torch.from_numpy(static_features(sysm, cand, fmo, hpar)).to(dev)
+        par_d = torch.from_numpy(cand.parent.astype(np.int64)).to(dev)
+        hpar_d = torch.from_numpy(hpar).to(dev)
+        hdiag_d = torch.from_numpy(np.ascontiguousarray(scorer.hdiag)).to(dev)
+        res.train["static_descriptor_s"] = time.perf_counter() - t0
     for it in range(iters):
         e, c, H = solve_current(eng, cur_rec, mode)
         if e_ref0 is None:
@@ -330,31 +371,34 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
         penergy[par[isref]] = np.diag(H)[isref]
         state = {"parent_weight": wpar, "parent_energy": penergy, "energy": e, "reference_energy": e_ref0}
         t0 = time.perf_counter()
-        cpl = None
-        if learned:
-            hdom = scorer.coupling(ref_rec[int(np.argmax(wpar)):int(np.argmax(wpar)) + 1])
-            cpl = (hpar, hdom, scorer.hdiag)
         use_net = learned and it >= learn_iters
+        F = None
+        if learned:
+            dom = int(np.argmax(wpar))
+            pf = torch.from_numpy(parent_table(sysm, state).astype(np.float32)).to(dev)
+            hdom = scorer.project_single(ref_rec[dom:dom + 1])
+            F = dynamic_features(torch, Fs, par_d, pf, hpar_d, hdom, hdiag_d, e)
         if not use_net:
             sc = scorer.scores(cur_rec, c, e)
-            n_eval = len(cand)
+            n_eval = N
             if learned:
-                Xs.append(features(sysm, cand, fmo, state, cpl))
+                Xs.append(F)
                 ys.append(np.log10(sc + 1e-20).astype(np.float32))
         else:
-            Xf = features(sysm, cand, fmo, state, cpl)
-            pred = net.forward((Xf - mu) / sd)
-            order = np.argsort(-np.where(taken, -np.inf, pred), kind="stable")
-            order = order[:len(cand) - int(taken.sum())]
-            sc = np.full(len(cand), -1.0)
+            Xn = ((F - mu) / sd).contiguous()
+            pred = torch.empty(N, dtype=torch.float32, device=dev)
+            net.forward(Xn, out=pred)
+            pred[torch.from_numpy(taken).to(dev)] = -float("inf")
+            order = torch.argsort(pred, descending=True).cpu().numpy()[:N - int(taken.sum())]
+            sc = np.full(N, -1.0)
             # exact evaluation in order of predicted score: the top fraction,
             # then further chunks while a chunk still holds a candidate within
             # chunk_ratio of the current k_add-th best exact score
-            chunk = max(k_add, int(chunk_frac * len(cand)))
-            done = min(len(order), max(k_add, int(top_frac * len(cand))))
+            chunk = max(k_add, int(chunk_frac * N))
+            done = min(len(order), max(k_add, int(top_frac * N)))
             first = np.sort(order[:done])
             sc[first] = scorer.scores(cur_rec, c, e, first)
-            while done < len(order) and done < max_frac * len(cand):
+            while done < len(order) and done < max_frac * N:
                 valid = np.sort(sc[sc >= 0])
                 kth = valid[-k_add] if len(valid) >= k_add else 0.0
                 nxt = np.sort(order[done:done + chunk])
@@ -364,9 +408,8 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
                     break
             top = order[:done]
             rest = order[done:]
-            audit = rng.choice(rest, min(len(rest), max(1, int(audit_frac * len(cand)))), replace=False) \
+            audit = np.sort(rng.choice(rest, min(len(rest), max(1, int(audit_frac * N))), replace=False)) \
                 if len(rest) else np.zeros(0, dtype=np.int64)
-            audit = np.sort(audit)
             if len(audit):
                 sc[audit] = scorer.scores(cur_rec, c, e, audit)
             n_eval = done + len(audit)
@@ -376,10 +419,10 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
             kth = valid[-k_add] if len(valid) >= k_add else 0.0
             misses = int((sc[audit] > kth).sum()) if len(audit) else 0
             res.audit_misses.append(misses)
-            if misses > 0 or done >= max_frac * len(cand):
+            if misses > 0 or done >= max_frac * N:
                 res.fallbacks += 1
                 sc = scorer.scores(cur_rec, c, e)
-                n_eval += len(cand)
+                n_eval += N
         sc = np.where(taken, -1.0, sc)
         pick = np.argsort(-sc, kind="stable")[:k_add]
         pick = pick[sc[pick] > 0]
@@ -387,10 +430,10 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
         res.exact_evals.append(int(n_eval))
         if use_net and measure_recall:
             full = scorer.scores(cur_rec, c, e)
-            scorer.n_exact -= len(cand)
+            scorer.n_exact -= N
             best = np.argsort(-np.where(taken, -1.0, full))[:k_add]
             res.recall.append(float(np.isin(best, top).mean()))
-            res.captured.append(float(full[np.isin(np.arange(len(cand)), pick)].sum() / full[best].sum()))
+            res.captured.append(float(full[pick].sum() / full[best].sum()))
         taken[pick] = True
         res.selected.append(pick)
         cur_rec = np.concatenate([cur_rec, scorer.rec[np.sort(pick)]])
@@ -399,16 +442,20 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
             log(f"selection iteration {it}: E={e:.10f} evaluated={n_eval} picked={len(pick)} "
                 f"screen {res.screen_s[-1]:.3f} s" + (f" misses={res.audit_misses[-1]}" if use_net else ""))
         if learned and it == learn_iters - 1:
-            X = np.concatenate(Xs)
+            X = torch.cat(Xs)
             y = np.concatenate(ys)
             mu, sd = X.mean(0), X.std(0) + 1e-6
-            net = MLP(N_FEATURES, seed=seed)
-            t1 = time.perf_counter()
+            Xn = ((X - mu) / sd).contiguous()
             # emphasise the top of the distribution: the selection only needs its order
             wt = np.where(y >= np.quantile(y, 0.9), 5.0, 1.0).astype(np.float32)
-            el = net.train((X - mu) / sd, y - y.mean(), wt, epochs=epochs, batch=batch, lr=lr, seed=seed)
-            res.train = {"samples": int(len(y)), "epochs": epochs, "loss": [float(v) for v in el],
-                         "seconds": time.perf_counter() - t1}
+            yd = torch.from_numpy(y - y.mean()).to(dev)
+            wd = torch.from_numpy(wt).to(dev)
+            net = MLP(N_FEATURES, seed=seed)
+            t1 = time.perf_counter()
+            el = net.train(Xn, yd, wd, epochs=epochs, batch=batch, lr=lr, seed=seed)
+            res.train.update({"samples": int(len(y)), "epochs": epochs, "loss": [float(v) for v in el],
+                              "seconds": time.perf_counter() - t1})
+            del Xs, X, Xn
     e, c, H = solve_current(eng, cur_rec, mode)
     res.energies.append(e)
     return res
