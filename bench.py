@@ -316,6 +316,29 @@ def screen_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def copy_workload(args):
+    """Calibration for HBM traffic counters: a plain device-to-device copy of
+    256 MiB (torch's elementwise copy kernel), repeated steps times.  The
+    per-dispatch FETCH_SIZE / WRITE_SIZE of this kernel, compared with the
+    268,435,456 bytes it moves each way, scale the same counters read for the
+    engine's kernels."""
+    import torch
+    n = (256 << 20) // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    out = {"metric": "device copy bandwidth (counter calibration)", "value": 2 * n * 8 * args.steps / el / 1e9,
+           "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": 1e3 * el / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "copy of 256 MiB", "bytes_each_way": n * 8}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -326,9 +349,11 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
                     help="mixed: also run the mixed-precision build and report it when |dE| < 1 uHa")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--workload", choices=["wick", "pt2", "batch", "screen"], default="wick",
+    ap.add_argument("--no-energy", action="store_true",
+                    help="skip the NOCI energies (eigensolver) -- for profiler runs; then |dE| is not checked")
+    ap.add_argument("--workload", choices=["wick", "pt2", "batch", "screen", "copy"], default="wick",
                     help="wick: batched H,S elements (headline); pt2: NOCI-PT2 matvec; batch: uneven "
-                         "multi-system batch; screen: learned SNOCI screening")
+                         "multi-system batch; screen: learned SNOCI screening; copy: counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the engine has no multi-GPU path in this build "
@@ -340,6 +365,8 @@ def main():
         return batch_workload(args)
     if args.workload == "screen":
         return screen_workload(args)
+    if args.workload == "copy":
+        return copy_workload(args)
 
     import numpy as np
     import torch
@@ -392,7 +419,7 @@ def main():
                 f"(transforms {tb['transform_ms']:.1f} ms), eval {te['eval_ms']:.2f} ms")
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
-        energy = lowest_energy()
+        energy = float("nan") if args.no_energy else lowest_energy()
         log(f"[{PREC_NAME[prec]}] {npairs * args.steps / elapsed:.4e} pairs/s, lowest NOCI energy {energy:.12f}")
         return {"elapsed": elapsed, "meta": meta, "builds": builds, "transforms": transforms, "evals": evals,
                 "energy": energy, "rate": npairs * args.steps / elapsed}
@@ -400,28 +427,39 @@ def main():
     PREC_NAME = {nwick.PREC_FP64: "fp64", nwick.PREC_MIXED: "mixed"}
     res64 = phase(nwick.PREC_FP64)
     resmx = phase(nwick.PREC_MIXED) if args.precision == "mixed" else None
-    dE = None if resmx is None else resmx["energy"] - res64["energy"]
-    use_mixed = resmx is not None and abs(dE) < 1e-6
+    dE = None if resmx is None or args.no_energy else resmx["energy"] - res64["energy"]
+    use_mixed = resmx is not None and (args.no_energy or abs(dE) < 1e-6)
     res = resmx if use_mixed else res64
     meta = res["meta"]
     ms_per_step = 1e3 * res["elapsed"] / args.steps
     value = res["rate"]
     transforms, evals, builds = res["transforms"], res["evals"], res["builds"]
-    # dominant kernel: the rotating quarter transform, 2 * nmo * rows * n_bas flops per launch
+    # dominant kernel: the rotating quarter transform, 2 * n_bas * rows * nmo = 2 n^5 flop per tensor pass;
+    # the build launches up to g tensors per dispatch (g tensors of scratch fit in 1 GiB)
     n = sysm.n_bas
-    ntensor = 0
-    for (x, w), pm in meta.items():
-        ma, mb = pm.ma, pm.mb
-        slots = lambda m: sum(1 for b in [(0, 0, 0, 0), (0, 0, 0, 1), (0, 0, 1, 0), (0, 0, 1, 1), (0, 1, 0, 1),
-                                          (0, 1, 1, 0), (0, 1, 1, 1), (1, 0, 1, 0), (1, 0, 1, 1), (1, 1, 1, 1)]
-                              if sum(b) <= m)
+    slots = lambda m: sum(1 for b in [(0, 0, 0, 0), (0, 0, 0, 1), (0, 0, 1, 0), (0, 0, 1, 1), (0, 1, 0, 1),
+                                      (0, 1, 1, 0), (0, 1, 1, 1), (1, 0, 1, 0), (1, 0, 1, 1), (1, 1, 1, 1)]
+                          if sum(b) <= m)
+    ntensor = n32 = 0
+    for (x, w), pm in res64["meta"].items():
         nii = sum(1 for a0 in (0, 1) for aj in (0, 1) for b0 in (0, 1) for bj in (0, 1)
-                  if a0 + aj <= ma and b0 + bj <= mb)
-        ntensor += slots(ma) + slots(mb) + nii
-    launches = 4 * ntensor
-    flops_per_launch = 2.0 * n * n ** 3 * n
-    avg_ms = float(np.mean(transforms)) / launches
-    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+                  if a0 + aj <= pm.ma and b0 + bj <= pm.mb)
+        k = slots(pm.ma) + slots(pm.mb) + nii
+        ntensor += k
+        if use_mixed and res["meta"][(x, w)].fp32:
+            n32 += k
+    flops_tensor = 4 * 2.0 * n ** 5
+    t64 = float(np.mean(res64["transforms"])) / ntensor  # ms per FP64 tensor
+    group = lambda elem: max(1, min(32, (1 << 30) // (n ** 4 * elem)))
+    if use_mixed:
+        t_k = float(np.mean(transforms)) - (ntensor - n32) * t64
+        nk, g = n32, group(4)
+    else:
+        t_k = float(np.mean(transforms))
+        nk, g = ntensor, group(8)
+    dispatches = 4 * ((nk + g - 1) // g)
+    achieved = nk * flops_tensor / (t_k * 1e-3) / 1e12
+    avg_ms = t_k / dispatches
     eval_rate = npairs / (float(np.mean(evals)) * 1e-3)
     kname = "k_qtransform_f32_pipe" if use_mixed else "k_qtransform_mfma_pipe"
     peak = PEAK_FP32_TFLOPS if use_mixed else PEAK_FP64_TFLOPS
@@ -453,7 +491,7 @@ def main():
                                "orthogonal alpha orbital), 512 singles/doubles per ref, 4096 dets, 8390656 pairs",
                    "mode": "reference" if args.mode == 0 else "corrected", "pairs_built": len(meta),
                    "precision": "mixed" if use_mixed else "fp64",
-                   "rank4_tensors": ntensor},
+                   "rank4_tensors": ntensor, "fp32_tensors": n32},
         "fp64_pairs_per_s": res64["rate"],
         "mixed_pairs_per_s": None if resmx is None else resmx["rate"],
         "mixed_over_fp64": None if resmx is None else resmx["rate"] / res64["rate"],
@@ -468,7 +506,8 @@ def main():
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "traffic_source": os.path.relpath(tfile, ROOT) + " (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
-                     "launches_per_step": launches, "avg_launch_ms": avg_ms},
+                     "dispatches_per_step": dispatches, "tensors_per_dispatch": g, "avg_launch_ms": avg_ms,
+                     "flops_per_dispatch": g * flops_tensor / 4},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
