@@ -263,24 +263,21 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
             }
             gemm(c, up, 0, 0, n, no, no, nm, no, no, 1.0, 0.0, gl);
             gemm(c, up, 0, 0, n, no, no, nm, no, no, 1.0, 0.0, gg);
-            // codensity per (pair) for this spin: write into M[(p*2+sp)*2 + mi]
-            // launch per pair so the output stride can interleave spins
-            for (int p = 0; p < P; ++p) {
-                size_t q = (size_t)sp * P + p;
-                launch_codensity(c->st, Lt + q * n * nomax, Gt + q * n * nomax, sig + (size_t)sp * P * nomax + (size_t)p * no,
-                                 n, no, tol, M + (size_t)prob(p, sp) * 2 * nn, 1);
-            }
+            // codensity of every pair for this spin, into M[(p*2+sp)*2 + mi]
+            launch_codensity(c->st, Lt + (size_t)sp * P * n * nomax, Gt + (size_t)sp * P * n * nomax,
+                             sig + (size_t)sp * P * nomax, n, no, tol, M + (size_t)prob(0, sp) * 2 * nn, P,
+                             (long long)n * nomax, no, (long long)(prob(1, sp) - prob(0, sp)) * 2 * nn);
             std::vector<const double *> cxp, cwp;
             for (int p = 0; p < P; ++p) { cxp.push_back(Cref(sp, px[p])); cwp.push_back(Cref(sp, pw[p])); }
-            // contraction orbitals into rowc/colc at prob(p, sp): launch per pair
+            // contraction orbitals of every pair into rowc/colc at prob(p, sp)
             const double *const *dcx = up.put(cxp);
             const double *const *dcw = up.put(cwp);
-            for (int p = 0; p < P; ++p)
-                launch_contraction_orbitals(c->st, dcx + p, dcw + p, n, nm, no, rowc + (size_t)prob(p, sp) * nnm,
-                                            colc + (size_t)prob(p, sp) * nnm, 1);
-            for (int p = 0; p < P; ++p)
-                launch_prefill_cxxc(c->st, rowc + (size_t)prob(p, sp) * nnm, colc + (size_t)prob(p, sp) * nnm, n, nm, no,
-                                    cx + (size_t)prob(p, sp) * 2 * nnm, xc + (size_t)prob(p, sp) * 2 * nnm, 1);
+            const long long pstride = (long long)(prob(1, sp) - prob(0, sp));
+            launch_contraction_orbitals(c->st, dcx, dcw, n, nm, no, rowc + (size_t)prob(0, sp) * nnm,
+                                        colc + (size_t)prob(0, sp) * nnm, P, pstride * (long long)nnm);
+            launch_prefill_cxxc(c->st, rowc + (size_t)prob(0, sp) * nnm, colc + (size_t)prob(0, sp) * nnm, n, nm, no,
+                                cx + (size_t)prob(0, sp) * 2 * nnm, xc + (size_t)prob(0, sp) * 2 * nnm, P,
+                                pstride * (long long)nnm, pstride * 2 * (long long)nnm);
         }
         // ---- M S, M^T S, cx, xc (build_cx_xc, build.rs:944-991) ---------------
         {
@@ -433,7 +430,7 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         HIPCHK(hipGetLastError());
         // ---- four-index transforms (build.rs:261-313, 877-917) ----------------
         HIPCHK(hipEventRecord(c->ev[1], c->st));
-        const bool mixed = c->precision == NW_PREC_MIXED && n % 4 == 0;
+        const bool mixed = c->precision == NW_PREC_MIXED && n % 2 == 0;
         // One job per rank-4 tensor: the J slots per spin and the II blocks.
         // J slab block [a,b,c,d] = blk[c,d,a,b] with blk = (cx_i xc_j || cx_k xc_l);
         // the pair symmetry of (ab||cd) lets one transform produce it directly.
@@ -514,7 +511,7 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                               launch_qtransform_f32_many(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm) &&
                               launch_qtransform_f32_many(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm) &&
                               launch_qtransform_f32_out64_many(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
-                    if (!ok) throw NwError(NW_E_SHAPE, "mixed-precision transform needs n_bas % 4 == 0 and nmo <= 112");
+                    if (!ok) throw NwError(NW_E_SHAPE, "mixed-precision transform needs an even n_bas and nmo <= 112");
                 } else {
                     std::vector<const double *> X(nb), C1(nb), C2(nb), C3(nb), C4(nb);
                     std::vector<double *> TA(nb), TB(nb), OUT(nb);

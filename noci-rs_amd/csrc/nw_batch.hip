@@ -234,11 +234,12 @@ void enqueue(nw_batch *b)
     BatchTile *dt = (BatchTile *)b->tiles.p;
     unsigned long long *dc = (unsigned long long *)b->counters.p;
     const int grid = b->grid_per_cu * b->ncu;
+    launch_reset_counters(b->st[0], dc, (int)b->launches.size());  // one zero-fill for every launch's counter
     HIPCHK(hipEventRecord(b->fork, b->st[0]));
     for (int s = 1; s < NSTREAM; ++s) HIPCHK(hipStreamWaitEvent(b->st[s], b->fork, 0));
     for (size_t k = 0; k < b->launches.size(); ++k) {
         const Launch &L = b->launches[k];
-        launch_eval_fused(b->st[L.stream], L.sel, b->dyn != 0, dt + L.off, L.n, b->d_sys, dc + k, grid);
+        launch_eval_fused(b->st[L.stream], L.sel, b->dyn != 0, dt + L.off, L.n, b->d_sys, dc + k, grid, false);
     }
     HIPCHK(hipGetLastError());
     for (int s = 1; s < NSTREAM; ++s) {

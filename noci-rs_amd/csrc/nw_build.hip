@@ -4,6 +4,8 @@
 // The algorithm follows src/nonorthogonalwicks/build.rs; the decomposition
 // into GPU launches is this engine's own: every step runs batched over all
 // requested ordered reference pairs and both spins at once.
+#include <type_traits>
+
 #include "nw_kernels.h"
 
 namespace nw {
@@ -172,35 +174,43 @@ __global__ __launch_bounds__(512) void k_qtransform_mfma(const QPtrs q, long lon
     }
 }
 
-// Pipelined form for nl % 4 == 0: the next 64-row tile of X is fetched into
+// Pipelined form for even nl: the next 64-row tile of X is fetched into
 // registers (16-byte loads, all in flight at once) while the current tile is
-// multiplied out of LDS, so HBM latency hides behind the MFMA work.
-template <int KS>
-__global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, long long rows, int nout,
-                                                               long long ntiles)
+// multiplied out of LDS, so HBM latency hides behind the MFMA work.  Rows of
+// X are nl long; the LDS tile is padded to NL = 4 KS columns with zeros so
+// the K = 4 MFMA steps need no guards.
+template <int KS, bool PAD2>
+__global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, long long rows, int nout, long long ntiles)
 {
+    constexpr int nl = KS * 4 - (PAD2 ? 2 : 0);  // row length of X (even)
     const double *__restrict__ X = (const double *)q.X[blockIdx.y];
     const double *__restrict__ c = (const double *)q.c[blockIdx.y];
     double *__restrict__ Y = (double *)q.Y[blockIdx.y];
     constexpr int NL = KS * 4;
     constexpr int LDX = NL + 1;
     constexpr int NT = 448;
-    constexpr int TILE2 = 64 * NL / 2;          // double2 per tile
-    constexpr int PER = (TILE2 + NT - 1) / NT;  // double2 per thread
+    constexpr int PER = (64 * NL / 2 + NT - 1) / NT;  // double2 per thread (upper bound)
     extern __shared__ double xs[];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lk = lane >> 4;
     const int lp = wave * 16 + lr;
+    constexpr int tile2 = 64 * nl / 2;
+    constexpr int npad = NL - nl;
+    if constexpr (npad > 0)
+        for (int e = tid; e < 64 * npad; e += NT) xs[(e / npad) * LDX + nl + e % npad] = 0.0;
     double afr[KS];
 #pragma unroll
-    for (int kk = 0; kk < KS; ++kk) afr[kk] = (lp < nout) ? c[(size_t)(kk * 4 + lk) * nout + lp] : 0.0;
+    for (int kk = 0; kk < KS; ++kk) {
+        const int l = kk * 4 + lk;
+        afr[kk] = (lp < nout && l < nl) ? c[(size_t)l * nout + lp] : 0.0;
+    }
     const bool active = wave * 16 < nout;
     double2 buf[PER];
     auto gload = [&](long long t) {
-        const double2 *src = reinterpret_cast<const double2 *>(X + (size_t)t * 64 * NL);
+        const double2 *src = reinterpret_cast<const double2 *>(X + (size_t)t * 64 * nl);
         const long long rem = rows - t * 64;
-        const int lim2 = (int)((rem < 64 ? rem : 64) * NL / 2);
+        const int lim2 = (int)((rem < 64 ? rem : 64) * nl / 2);
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const int e2 = tid + j * NT;
@@ -213,9 +223,9 @@ __global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, lon
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const int e = 2 * (tid + j * NT);
-            if (e < 2 * TILE2) {
-                const int r = e / NL, k = e % NL;
+            const int e2 = tid + j * NT;
+            if (e2 < tile2) {
+                const int e = 2 * e2, r = e / nl, k = e % nl;
                 xs[r * LDX + k] = buf[j].x;
                 xs[r * LDX + k + 1] = buf[j].y;
             }
@@ -257,37 +267,44 @@ __global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, lon
 // 128-row tiles of X, staged in LDS, next tile prefetched into registers.
 typedef float nw_f4 __attribute__((ext_vector_type(4)));
 
-template <int KS, typename OutT>
-__global__ __launch_bounds__(448) void k_qtransform_f32_pipe(const QPtrs q, long long rows, int nout,
-                                                              long long ntiles)
+template <int KS, typename OutT, int VEC>
+__global__ __launch_bounds__(448) void k_qtransform_f32_pipe(const QPtrs q, long long rows, int nout, long long ntiles)
 {
+    constexpr int nl = KS * 4 - (VEC == 4 ? 0 : 2);  // row length of X: float4 rows when nl % 4 == 0
     const float *__restrict__ X = (const float *)q.X[blockIdx.y];
     const float *__restrict__ c = (const float *)q.c[blockIdx.y];
     OutT *__restrict__ Y = (OutT *)q.Y[blockIdx.y];
+    using VT = typename std::conditional<VEC == 4, float4, float2>::type;
     constexpr int NL = KS * 4;
     constexpr int LDX = NL + 1;
     constexpr int NT = 448;
-    constexpr int TR = 128;                     // rows of X per tile
-    constexpr int TILE4 = TR * NL / 4;          // float4 per tile
-    constexpr int PER = (TILE4 + NT - 1) / NT;  // float4 per thread
+    constexpr int TR = 128;                                // rows of X per tile
+    constexpr int PER = (TR * NL / VEC + NT - 1) / NT;     // vectors per thread (upper bound)
     extern __shared__ float xsf[];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lk = lane >> 4;
     const int lp = wave * 16 + lr;
+    constexpr int tilev = TR * nl / VEC;
+    constexpr int npad = NL - nl;
+    if constexpr (npad > 0)
+        for (int e = tid; e < TR * npad; e += NT) xsf[(e / npad) * LDX + nl + e % npad] = 0.f;
     float afr[KS];
 #pragma unroll
-    for (int kk = 0; kk < KS; ++kk) afr[kk] = (lp < nout) ? c[(size_t)(kk * 4 + lk) * nout + lp] : 0.0f;
+    for (int kk = 0; kk < KS; ++kk) {
+        const int l = kk * 4 + lk;
+        afr[kk] = (lp < nout && l < nl) ? c[(size_t)l * nout + lp] : 0.0f;
+    }
     const bool active = wave * 16 < nout;
-    float4 buf[PER];
+    VT buf[PER];
     auto gload = [&](long long t) {
-        const float4 *src = reinterpret_cast<const float4 *>(X + (size_t)t * TR * NL);
+        const VT *src = reinterpret_cast<const VT *>(X + (size_t)t * TR * nl);
         const long long rem = rows - t * TR;
-        const int lim4 = (int)((rem < TR ? rem : TR) * NL / 4);
+        const int limv = (int)((rem < TR ? rem : TR) * nl / VEC);
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const int e4 = tid + j * NT;
-            buf[j] = (e4 < lim4) ? src[e4] : float4{0.f, 0.f, 0.f, 0.f};
+            const int ev = tid + j * NT;
+            buf[j] = (ev < limv) ? src[ev] : VT{};
         }
     };
     long long t = blockIdx.x;
@@ -296,11 +313,16 @@ __global__ __launch_bounds__(448) void k_qtransform_f32_pipe(const QPtrs q, long
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const int e = 4 * (tid + j * NT);
-            if (e < 4 * TILE4) {
-                const int r = e / NL, k = e % NL;
+            const int ev = tid + j * NT;
+            if (ev < tilev) {
+                const int e = VEC * ev, r = e / nl, k = e % nl;
                 float *d = xsf + r * LDX + k;
-                d[0] = buf[j].x; d[1] = buf[j].y; d[2] = buf[j].z; d[3] = buf[j].w;
+                d[0] = buf[j].x;
+                d[1] = buf[j].y;
+                if constexpr (VEC == 4) {
+                    d[2] = buf[j].z;
+                    d[3] = buf[j].w;
+                }
             }
         }
         __syncthreads();
@@ -361,18 +383,20 @@ template <typename OutT>
 static bool launch_qtransform_f32_t(hipStream_t st, const float *const *X, const float *const *c, OutT *const *Y,
                                     int count, long long rows, int nl, int nout)
 {
-    const int ks = nl / 4;
-    if (nl % 4 != 0 || ks > 31 || (nout + 15) / 16 > 7) return false;
+    const int ks = (nl + 3) / 4;
+    if (nl % 2 != 0 || ks > 31 || (nout + 15) / 16 > 7) return false;
+    const bool v4 = nl % 4 == 0;
     const long long ntiles = (rows + 127) / 128;
-    const size_t lds = (size_t)128 * (nl + 1) * sizeof(float);
+    const size_t lds = (size_t)128 * (ks * 4 + 1) * sizeof(float);
     for (int b0 = 0; b0 < count; b0 += QMAX) {
         const int nb = count - b0 < QMAX ? count - b0 : QMAX;
         QPtrs q{};
         for (int k = 0; k < nb; ++k) { q.X[k] = X[b0 + k]; q.c[k] = c[b0 + k]; q.Y[k] = Y[b0 + k]; }
         dim3 g(grid_x(ntiles, nb, 1024), (unsigned)nb), b(448);
-#define NW_QF(K)                                                                                        \
-    case K:                                                                                             \
-        hipLaunchKernelGGL((k_qtransform_f32_pipe<K, OutT>), g, b, lds, st, q, rows, nout, ntiles);    \
+#define NW_QF(K)                                                                                               \
+    case K:                                                                                                    \
+        if (v4) hipLaunchKernelGGL((k_qtransform_f32_pipe<K, OutT, 4>), g, b, lds, st, q, rows, nout, ntiles); \
+        else hipLaunchKernelGGL((k_qtransform_f32_pipe<K, OutT, 2>), g, b, lds, st, q, rows, nout, ntiles);    \
         break;
         switch (ks) {
             NW_QF(1) NW_QF(2) NW_QF(3) NW_QF(4) NW_QF(5) NW_QF(6) NW_QF(7) NW_QF(8) NW_QF(9) NW_QF(10)
@@ -420,13 +444,14 @@ void launch_qtransform_many(hipStream_t st, const double *const *X, const double
         QPtrs q{};
         for (int k = 0; k < nb; ++k) { q.X[k] = X[b0 + k]; q.c[k] = c[b0 + k]; q.Y[k] = Y[b0 + k]; }
         bool done = false;
-        if (nl % 4 == 0 && ks <= 31 && nw <= 7) {
+        if (nl % 2 == 0 && ks <= 31 && nw <= 7) {
             const size_t lds = (size_t)64 * (ks * 4 + 1) * sizeof(double);
             dim3 g(grid_x(ntiles, nb, 512), (unsigned)nb), b(448);
             done = true;
 #define NW_QP(K)                                                                                        \
     case K:                                                                                             \
-        hipLaunchKernelGGL((k_qtransform_mfma_pipe<K>), g, b, lds, st, q, rows, nout, ntiles);         \
+        if (nl % 4) hipLaunchKernelGGL((k_qtransform_mfma_pipe<K, true>), g, b, lds, st, q, rows, nout, ntiles); \
+        else hipLaunchKernelGGL((k_qtransform_mfma_pipe<K, false>), g, b, lds, st, q, rows, nout, ntiles);      \
         break;
             switch (ks) {
                 NW_QP(1) NW_QP(2) NW_QP(3) NW_QP(4) NW_QP(5) NW_QP(6) NW_QP(7) NW_QP(8) NW_QP(9) NW_QP(10)
@@ -657,13 +682,14 @@ void launch_svd(hipStream_t st, const double *A, int n, int nprob, double *U, do
 // ---------------------------------------------------------------------------
 __global__ void k_codensity(const double *__restrict__ lt, const double *__restrict__ gt,
                             const double *__restrict__ sig, int n, int no, double tol,
-                            double *__restrict__ M /* per prob: [2][n][n] */)
+                            double *__restrict__ M /* per prob: [2][n][n] */, long long in_stride,
+                            long long sig_stride, long long out_stride)
 {
     const int prob = blockIdx.z;
     const int mu = blockIdx.y, nu = blockIdx.x * blockDim.x + threadIdx.x;
     if (nu >= n) return;
-    const double *L = lt + (size_t)prob * n * no, *G = gt + (size_t)prob * n * no;
-    const double *s = sig + (size_t)prob * no;
+    const double *L = lt + (size_t)prob * in_stride, *G = gt + (size_t)prob * in_stride;
+    const double *s = sig + (size_t)prob * sig_stride;
     double m0 = 0.0, m1 = 0.0, zz = 0.0;
     for (int k = 0; k < no; ++k) {
         double g = G[mu * no + k], l = L[nu * no + k];
@@ -675,24 +701,26 @@ __global__ void k_codensity(const double *__restrict__ lt, const double *__restr
         zz += g * G[nu * no + k];
         m1 += g * L[nu * no + k];
     }
-    double *out = M + (size_t)prob * 2 * n * n;
+    double *out = M + (size_t)prob * out_stride;
     out[mu * n + nu] = m0 + zz + m1;
     out[n * n + mu * n + nu] = m1;
 }
 
 void launch_codensity(hipStream_t st, const double *lt, const double *gt, const double *sig, int n, int no,
-                      double tol, double *M, int nprob)
+                      double tol, double *M, int nprob, long long in_stride, long long sig_stride, long long out_stride)
 {
     if (nprob <= 0) return;
     dim3 grid((n + 63) / 64, n, nprob);
-    hipLaunchKernelGGL(k_codensity, grid, dim3(64), 0, st, lt, gt, sig, n, no, tol, M);
+    hipLaunchKernelGGL(k_codensity, grid, dim3(64), 0, st, lt, gt, sig, n, no, tol, M, in_stride, sig_stride,
+                       out_stride);
 }
 
 // rowc = [Cx[:, no:] | Cw[:, :no]], colc = [Cx[:, :no] | Cw[:, no:]]
 // (contraction_orbitals, build.rs:1008-1031) and the masked copies used for
 // the (1 - m_i) subtractions of build_cx_xc (build.rs:944-991).
 __global__ void k_contraction_orbitals(const double *const *__restrict__ cx_ptrs, const double *const *__restrict__ cw_ptrs,
-                                       int n, int nm, int no, double *__restrict__ rowc, double *__restrict__ colc)
+                                       int n, int nm, int no, double *__restrict__ rowc, double *__restrict__ colc,
+                                       long long out_stride)
 {
     const int prob = blockIdx.y;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -700,42 +728,44 @@ __global__ void k_contraction_orbitals(const double *const *__restrict__ cx_ptrs
     const int mu = e / nm, j = e % nm;
     const double *Cx = cx_ptrs[prob], *Cw = cw_ptrs[prob];
     const int nv = nm - no;
-    rowc[(size_t)prob * n * nm + e] = (j < nv) ? Cx[mu * nm + no + j] : Cw[mu * nm + (j - nv)];
-    colc[(size_t)prob * n * nm + e] = (j < no) ? Cx[mu * nm + j] : Cw[mu * nm + j];
+    rowc[(size_t)prob * out_stride + e] = (j < nv) ? Cx[mu * nm + no + j] : Cw[mu * nm + (j - nv)];
+    colc[(size_t)prob * out_stride + e] = (j < no) ? Cx[mu * nm + j] : Cw[mu * nm + j];
 }
 
 void launch_contraction_orbitals(hipStream_t st, const double *const *cx, const double *const *cw, int n, int nm,
-                                 int no, double *rowc, double *colc, int nprob)
+                                 int no, double *rowc, double *colc, int nprob, long long out_stride)
 {
     if (nprob <= 0) return;
     dim3 grid((n * nm + 255) / 256, nprob);
-    hipLaunchKernelGGL(k_contraction_orbitals, grid, dim3(256), 0, st, cx, cw, n, nm, no, rowc, colc);
+    hipLaunchKernelGGL(k_contraction_orbitals, grid, dim3(256), 0, st, cx, cw, n, nm, no, rowc, colc, out_stride);
 }
 
 // cx[mi] prefill = -(1-mi) rowc on columns < nv, 0 elsewhere;
 // xc[mi] prefill = -(1-mi) colc on columns >= no, 0 elsewhere.
 __global__ void k_prefill_cxxc(const double *__restrict__ rowc, const double *__restrict__ colc, int n, int nm, int no,
-                               double *__restrict__ cx, double *__restrict__ xc)
+                               double *__restrict__ cx, double *__restrict__ xc, long long in_stride,
+                               long long out_stride)
 {
-    const int prob = blockIdx.y;  // (pair, spin)
+    const int prob = blockIdx.y;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * nm) return;
     const int j = e % nm;
     const int nv = nm - no;
     const size_t nn = (size_t)n * nm;
-    double rc = rowc[prob * nn + e], cc = colc[prob * nn + e];
-    cx[(prob * 2 + 0) * nn + e] = (j < nv) ? -rc : 0.0;
-    cx[(prob * 2 + 1) * nn + e] = 0.0;
-    xc[(prob * 2 + 0) * nn + e] = (j >= no) ? -cc : 0.0;
-    xc[(prob * 2 + 1) * nn + e] = 0.0;
+    double rc = rowc[prob * in_stride + e], cc = colc[prob * in_stride + e];
+    double *cxp = cx + prob * out_stride, *xcp = xc + prob * out_stride;
+    cxp[e] = (j < nv) ? -rc : 0.0;
+    cxp[nn + e] = 0.0;
+    xcp[e] = (j >= no) ? -cc : 0.0;
+    xcp[nn + e] = 0.0;
 }
 
 void launch_prefill_cxxc(hipStream_t st, const double *rowc, const double *colc, int n, int nm, int no, double *cx,
-                         double *xc, int nprob)
+                         double *xc, int nprob, long long in_stride, long long out_stride)
 {
     if (nprob <= 0) return;
     dim3 grid((n * nm + 255) / 256, nprob);
-    hipLaunchKernelGGL(k_prefill_cxxc, grid, dim3(256), 0, st, rowc, colc, n, nm, no, cx, xc);
+    hipLaunchKernelGGL(k_prefill_cxxc, grid, dim3(256), 0, st, rowc, colc, n, nm, no, cx, xc, in_stride, out_stride);
 }
 
 // out = ca*a + cb*b + cc*c elementwise (null operands skipped), batch over y.

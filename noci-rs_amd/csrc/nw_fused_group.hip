@@ -4,16 +4,21 @@
 
 namespace nw {
 
-__global__ void k_reset_counter(unsigned long long *counter)
+__global__ void k_reset_counter(unsigned long long *counter, int n)
 {
-    if (threadIdx.x == 0) counter[0] = 0ull;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) counter[k] = 0ull;
+}
+
+void launch_reset_counters(hipStream_t st, unsigned long long *counters, int n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(256), 0, st, counters, n);
 }
 
 void launch_eval_fused(hipStream_t st, int sel, bool dyn, const BatchTile *tiles, long long ntiles,
-                       const BatchSys *sys, unsigned long long *counter, int grid)
+                       const BatchSys *sys, unsigned long long *counter, int grid, bool reset)
 {
     if (ntiles <= 0) return;
-    if (dyn || sel >= 0) hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, counter);
+    if (reset && (dyn || sel >= 0)) launch_reset_counters(st, counter, 1);
     if ((long long)grid > ntiles) grid = (int)ntiles;
     if (sel >= 0) {  // single-code kernels exist in the dynamic-schedule form only
         launch_eval_fused_code(st, sel, tiles, ntiles, sys, counter, grid);

@@ -12,12 +12,13 @@ void launch_gemm(hipStream_t st, int ta, int tb, int M, int N, int K, int lda, i
                  double alpha, double beta, const GemmDesc *d_descs, int nbatch);
 void launch_svd(hipStream_t st, const double *A, int n, int nprob, double *U, double *V, double *sig,
                 double *phase, int *m, double *tsp, double tol);
+// Strides (in doubles) between consecutive problems of one launch.
 void launch_codensity(hipStream_t st, const double *lt, const double *gt, const double *sig, int n, int no,
-                      double tol, double *M, int nprob);
+                      double tol, double *M, int nprob, long long in_stride, long long sig_stride, long long out_stride);
 void launch_contraction_orbitals(hipStream_t st, const double *const *cx, const double *const *cw, int n, int nm,
-                                 int no, double *rowc, double *colc, int nprob);
+                                 int no, double *rowc, double *colc, int nprob, long long out_stride);
 void launch_prefill_cxxc(hipStream_t st, const double *rowc, const double *colc, int n, int nm, int no, double *cx,
-                         double *xc, int nprob);
+                         double *xc, int nprob, long long in_stride, long long out_stride);
 void launch_lincomb(hipStream_t st, const LinDesc *d, int nb, long long n);
 void launch_einsum(hipStream_t st, const TraceDesc *d, int nd, int n);
 void launch_antisym(hipStream_t st, const double *eri, double *out, int n);
@@ -114,7 +115,10 @@ __host__ __device__ constexpr int fused_class(int code)
 
 constexpr int FUSE_ALL = -5;
 // sel: a code (>= 0), a register class (-1 - class) or FUSE_ALL.
+// reset = zero the work counter first (false when the caller zeroed all
+// counters of a plan with one launch_reset_counters).
 void launch_eval_fused(hipStream_t st, int sel, bool dyn, const BatchTile *tiles, long long ntiles,
-                       const BatchSys *sys, unsigned long long *counter, int grid);
+                       const BatchSys *sys, unsigned long long *counter, int grid, bool reset = true);
+void launch_reset_counters(hipStream_t st, unsigned long long *counters, int n);
 
 }  // namespace nw

@@ -163,3 +163,35 @@ def test_mixed_floor_selects_pairs():
             sv += [v for v in s if v > 1e-8]
         assert abs(m.sigma_min - min(sv)) < 1e-10
         assert m.fp32 == (1 if min(sv) >= 0.1 else 0)
+
+
+@pytest.mark.parametrize("n", [11, 14])
+def test_transform_shapes_odd_and_two_mod_four(n):
+    """n_bas odd (generic / non-pipelined transform) and n_bas = 2 mod 4
+    (pipelined transform with a zero-padded K tile, FP64 and the float2 FP32
+    form) against the oracle."""
+    sysm, dets = _case(n, 3, 2, 3, seed=90 + n, orthonormal=False, per_ref=20)
+    rec = basis.to_records(dets)
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    H, S = eng.hs_matrix(rec, mode=1)
+    Ho, So = O.hs_matrix(O.Slab(sysm, tol=1e-8), dets, mode=1)
+    assert np.max(np.abs(H - Ho)) < TOL and np.max(np.abs(S - So)) < TOL
+    if n % 2 == 0:
+        # well-conditioned references (small rotations: every singular value
+        # near 1) so FP32 rounding is not amplified and the FP32 kernel's
+        # indexing is what the comparison tests
+        sysm = synth.make_system(n, 3, 2, 3, seed=90 + n, kappa_sigma=0.05, n_aux=2 * n)
+        dets = basis.excited_basis(3, n, 3, 2, per_ref=20, seed=90 + n, include_refs=True)
+        rec = basis.to_records(dets)
+        eng = nwick.Engine.from_system(sysm)
+        eng.build()
+        H, S = eng.hs_matrix(rec, mode=1)
+        eng.set_precision(nwick.PREC_MIXED)
+        eng.set_mixed_floor(0.0)
+        meta = eng.build()
+        assert all(m.fp32 == 1 for m in meta.values())
+        Hm, Sm = eng.hs_matrix(rec, mode=1)
+        print(f"n={n} mixed: max|dH|={np.max(np.abs(Hm - H)):.3e}", flush=True)
+        assert np.array_equal(Sm, S)
+        assert np.max(np.abs(Hm - H)) < 1e-5 * max(1.0, np.max(np.abs(H)))
