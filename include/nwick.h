@@ -140,6 +140,14 @@ int nw_matvec_m(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double e0, const
 int nw_project_hs(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, uint64_t ncur, const double *coef, double *v,
                   double *s, int mode);
 
+/* NOCI one-particle densities of a state sum_i coef_i |det_i> and its spin
+ * Fock matrices, the refresh that precedes nw_update_fock in every SNOCI
+ * iteration: noci_density (src/noci/naive.rs:231-289, pair densities
+ * :73-229) with zero threshold tol, then fock (src/scf/kernels.rs:46).
+ * fa, fb (n_bas x n_bas) required; da, db optional (NULL). */
+int nw_noci_fock(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const double *coef, double tol, double *fa,
+                 double *fb, double *da, double *db);
+
 /* Select the build arithmetic (nw_precision) for subsequent nw_build calls.
  * No reference counterpart: the reference computes in FP64 only. */
 int nw_set_precision(nw_ctx *ctx, int precision);

@@ -30,7 +30,7 @@ STATUS = {0: "NW_OK", 1: "NW_E_ARG", 2: "NW_E_SHAPE", 3: "NW_E_HIP", 4: "NW_E_ME
 
 # Every symbol the header declares.
 EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matrix", "nw_update_fock",
-           "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_project_hs", "nw_set_precision", "nw_set_mixed_floor", "nw_pair_slab", "nw_last_timings",
+           "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_project_hs", "nw_noci_fock", "nw_set_precision", "nw_set_mixed_floor", "nw_pair_slab", "nw_last_timings",
            "nw_last_error", "nw_destroy", "nw_batch_create", "nw_batch_add", "nw_batch_plan", "nw_batch_run",
            "nw_batch_result", "nw_batch_schedule", "nw_batch_set_schedule", "nw_batch_last_ms",
            "nw_batch_last_error", "nw_batch_destroy", "nw_mlp_create", "nw_mlp_num_params", "nw_mlp_set_params",
@@ -85,6 +85,7 @@ def load():
     L.nw_pair_slab.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
     L.nw_set_precision.argtypes = [vp, ct.c_int]
     L.nw_project_hs.argtypes = [vp, vp, ct.c_uint64, ct.c_uint64, vp, vp, vp, ct.c_int]
+    L.nw_noci_fock.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_double, vp, vp, vp, vp]
     L.nw_set_mixed_floor.argtypes = [vp, ct.c_double]
     L.nw_update_fock.argtypes = [vp, vp, vp]
     L.nw_fs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
@@ -259,6 +260,18 @@ class Engine:
         sync_torch(dets, coef, v, s)
         self._check(self.L.nw_project_hs(self.ctx, _ptr(dets), n, int(ncur), _ptr(coef), _ptr(v), _ptr(s), int(mode)))
         return v, s
+
+    def noci_fock(self, dets, coef, tol=1e-8, densities=False):
+        """Spin Fock matrices (and optionally NOCI densities) of the state sum_i coef_i |dets_i>."""
+        n = self._ndet(dets)
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        fa = np.empty((self.n_bas, self.n_bas)); fb = np.empty((self.n_bas, self.n_bas))
+        da = np.empty_like(fa) if densities else None
+        db = np.empty_like(fa) if densities else None
+        sync_torch(dets)
+        self._check(self.L.nw_noci_fock(self.ctx, _ptr(dets), n, _ptr(coef), float(tol), _ptr(fa), _ptr(fb),
+                                        _ptr(da), _ptr(db)))
+        return (fa, fb, da, db) if densities else (fa, fb)
 
     def pair_slab(self, x, w):
         off = np.zeros(95, dtype=np.int64)

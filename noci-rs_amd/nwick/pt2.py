@@ -12,8 +12,9 @@ Mirrors the reference's PT2 step (src/snoci/step.rs:178-440):
 Everything that scales with the candidate count lives on the GPU in FP64:
 the matvec (HIP kernels through the C ABI) and the Krylov vectors, Givens
 rotations and the true residual (torch tensors on the device).  The NOCI
-density and Fock matrices of the small current space are host-side set-up,
-restated from src/noci/naive.rs:231 (noci_density) and src/scf/kernels.rs:46.
+density and Fock matrices of the current space come from nw_noci_fock; the
+host functions noci_density / fock below restate src/noci/naive.rs:231 and
+src/scf/kernels.rs:46 for checking it.
 """
 from __future__ import annotations
 
@@ -213,8 +214,8 @@ class PT2:
         e, C = general_evp(Hc, Sc, self.tol)
         self.ecurrent, c = float(e[0]), C[:, 0]
         self.coeffs = c
-        Da, Db = noci_density(self.sysm, self.current, c, self.tol)
-        fa, fb = fock(self.sysm.h, self.sysm.eri, Da, Db)
+        # NOCI density and Fock on the GPU (noci_density + fock), then the slab's Fock intermediates
+        fa, fb = eng.noci_fock(cur_rec, c, self.tol)
         eng.update_fock(fa, fb)
         Fc, _ = eng.fs_matrix(cur_rec, mode=self.mode)
         self.e0 = float(c @ Fc @ c)

@@ -116,3 +116,19 @@ def test_pt2_gmres_vs_dense_solve():
     print(f"pt2: gpu {r['ept2']:.12f} dense {ept2:.12f} iters {r['iterations']}", flush=True)
     assert abs(r["e0"] - e0) < 1e-10 and abs(r["e_noci"] - e[0]) < 1e-10
     assert abs(r["ept2"] - ept2) < 1e-8
+
+
+def test_noci_fock_matches_host_restatement():
+    """nw_noci_fock (GPU pair densities, D = CC T CC^T, J/K by GEMV) against
+    noci_density + fock on the host, with m = 1 reference pairs present."""
+    from nwick.pt2 import fock, noci_density
+    sysm = synth.make_system(12, 3, 2, 3, seed=75, orthonormal_ao=False, n_aux=24, orth_refs=(2,))
+    dets = basis.excited_basis(3, 12, 3, 2, per_ref=6, seed=75, include_refs=True)
+    c = np.random.default_rng(1).normal(size=len(dets))
+    eng = nwick.Engine.from_system(sysm)
+    fa, fb, da, db = eng.noci_fock(basis.to_records(dets), c, tol=1e-8, densities=True)
+    Da, Db = noci_density(sysm, dets, c, 1e-8)
+    Fa, Fb = fock(sysm.h, sysm.eri, Da, Db)
+    print(f"noci_fock: max|dDa|={np.max(np.abs(da - Da)):.3e} max|dFa|={np.max(np.abs(fa - Fa)):.3e}", flush=True)
+    for got, want in ((da, Da), (db, Db), (fa, Fa), (fb, Fb)):
+        assert np.max(np.abs(got - want)) < 1e-10 * max(1.0, np.max(np.abs(want)))
