@@ -165,6 +165,18 @@ int nw_set_mixed_floor(nw_ctx *ctx, double sigma_floor);
  * the length through offsets[94].  For tests and disk interop. */
 int nw_pair_slab(nw_ctx *ctx, int x, int w, double *out, int64_t *offsets, double *meta);
 
+/* Wick disk cache in the reference's format (src/nonorthogonalwicks/storage.rs:
+ * WICKS_DISK_CACHE_VERSION 2): `dir/wicks.bin` = the slab (f64, pairs
+ * x * nref + w back to back, create_wicks_mmap) and `dir/wicks.meta` = the
+ * bincode serialisation of WicksDiskMeta<f64> (offsets and PairMeta).
+ * nw_save_wicks needs a build of every ordered pair.  nw_load_wicks replaces
+ * nw_build (load_wicks_mmap, :227-255): it checks the version, nref, nmo,
+ * occupations and that every pair's layout is assign_offsets'; tol is the
+ * zero threshold for later evaluations.  A loaded slab has no co-densities,
+ * so nw_update_fock needs a real nw_build. */
+int nw_save_wicks(nw_ctx *ctx, const char *dir);
+int nw_load_wicks(nw_ctx *ctx, const char *dir, double tol);
+
 /* Device time in milliseconds of the kernels in the last nw_build /
  * nw_hs_matrix / nw_hs_batched call, split as {total, transform, evaluate}. */
 int nw_last_timings(nw_ctx *ctx, double *ms3);

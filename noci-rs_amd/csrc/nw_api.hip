@@ -18,40 +18,6 @@ using namespace nwi;
 
 namespace {
 
-// assign_offsets (src/nonorthogonalwicks/layout.rs:21-189) for one pair.
-long long pair_offsets(int ma, int mb, int nmo, long long *off)
-{
-    const long long nn2 = (long long)nmo * nmo, nn4 = nn2 * nn2;
-    long long i = 0;
-    for (int k = 0; k < O_TOTAL; ++k) off[k] = -1;
-    for (int sp = 0; sp < 2; ++sp) {
-        long long *o = off + (sp ? O_BB : O_AA);
-        const int msp = sp ? mb : ma;
-        for (int mi = 0; mi < 2; ++mi) { o[O_X + mi] = i; i += nn2; }
-        for (int mi = 0; mi < 2; ++mi) { o[O_Y + mi] = i; i += nn2; }
-        for (int k = 0; k < 4; ++k) { o[O_FH + k] = i; i += nn2; }
-        for (int k = 0; k < 4; ++k) { o[O_FF + k] = i; i += nn2; }
-        for (int k = 0; k < 8; ++k) { o[O_V + k] = i; i += nn2; }
-        for (int s = 0; s < 10; ++s)
-            if (jbranch(s, 0) + jbranch(s, 1) + jbranch(s, 2) + jbranch(s, 3) <= msp) { o[O_J + s] = i; i += nn4; }
-    }
-    for (int k = 0; k < 8; ++k) { off[O_VAB + k] = i; i += nn2; }
-    for (int k = 0; k < 8; ++k) { off[O_VBA + k] = i; i += nn2; }
-    if (ma == 0 && mb == 0) {
-        off[O_AA + O_HCOL0] = i; i += nn2;
-        off[O_BB + O_HCOL0] = i; i += nn2;
-    }
-    for (int a0 = 0; a0 < 2; ++a0)
-        for (int aj = 0; aj < 2; ++aj)
-            for (int b0 = 0; b0 < 2; ++b0)
-                for (int bj = 0; bj < 2; ++bj)
-                    if (a0 + aj <= ma && b0 + bj <= mb) {
-                        off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj] = i;
-                        i += nn4;
-                    }
-    off[O_TOTAL] = i;
-    return i;
-}
 
 
 void gemm(nw_ctx *c, Uploader &up, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc, double alpha,
@@ -565,6 +531,7 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         c->t_ms[1] = elapsed(c->ev[1], c->ev[2]);
         c->t_ms[2] = 0.0;
         c->built = true;
+        c->have_codensity = true;
         return NW_OK;
     });
 }
@@ -572,6 +539,42 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
 }  // extern "C"
 
 namespace nwi {
+
+// assign_offsets (src/nonorthogonalwicks/layout.rs:21-189) for one pair.
+long long pair_offsets(int ma, int mb, int nmo, long long *off)
+{
+    const long long nn2 = (long long)nmo * nmo, nn4 = nn2 * nn2;
+    long long i = 0;
+    for (int k = 0; k < O_TOTAL; ++k) off[k] = -1;
+    for (int sp = 0; sp < 2; ++sp) {
+        long long *o = off + (sp ? O_BB : O_AA);
+        const int msp = sp ? mb : ma;
+        for (int mi = 0; mi < 2; ++mi) { o[O_X + mi] = i; i += nn2; }
+        for (int mi = 0; mi < 2; ++mi) { o[O_Y + mi] = i; i += nn2; }
+        for (int k = 0; k < 4; ++k) { o[O_FH + k] = i; i += nn2; }
+        for (int k = 0; k < 4; ++k) { o[O_FF + k] = i; i += nn2; }
+        for (int k = 0; k < 8; ++k) { o[O_V + k] = i; i += nn2; }
+        for (int s = 0; s < 10; ++s)
+            if (jbranch(s, 0) + jbranch(s, 1) + jbranch(s, 2) + jbranch(s, 3) <= msp) { o[O_J + s] = i; i += nn4; }
+    }
+    for (int k = 0; k < 8; ++k) { off[O_VAB + k] = i; i += nn2; }
+    for (int k = 0; k < 8; ++k) { off[O_VBA + k] = i; i += nn2; }
+    if (ma == 0 && mb == 0) {
+        off[O_AA + O_HCOL0] = i; i += nn2;
+        off[O_BB + O_HCOL0] = i; i += nn2;
+    }
+    for (int a0 = 0; a0 < 2; ++a0)
+        for (int aj = 0; aj < 2; ++aj)
+            for (int b0 = 0; b0 < 2; ++b0)
+                for (int bj = 0; bj < 2; ++bj)
+                    if (a0 + aj <= ma && b0 + bj <= mb) {
+                        off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj] = i;
+                        i += nn4;
+                    }
+    off[O_TOTAL] = i;
+    return i;
+}
+
 
 // Pack determinants on the device and return the host copy of the packed records.
 std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked)
@@ -833,6 +836,8 @@ int nw_update_fock(nw_ctx *c, const double *fa, const double *fb)
 {
     return guarded(c, [&]() -> int {
         if (!c->built) return fail(c, NW_E_STATE, "nw_build first");
+        if (!c->have_codensity)
+            return fail(c, NW_E_STATE, "the Fock refresh needs the co-densities of an nw_build (not a loaded cache)");
         if (!fa || !fb) return fail(c, NW_E_ARG, "null Fock matrix");
         const int n = c->n, nm = c->nmo, nref = c->nref;
         const size_t nn = (size_t)n * n, nnm = (size_t)n * nm;

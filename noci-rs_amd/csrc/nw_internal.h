@@ -93,6 +93,7 @@ struct nw_ctx {
     // build state
     bool built = false;
     unsigned long long build_gen = 0;  // incremented by every nw_build / nw_set_refs
+    bool have_codensity = false;       // co-densities and cx/xc of the last nw_build are kept (nw_update_fock)
     double tol = 1e-8;
     double *d_slab = nullptr;
     size_t slab_cap = 0;
@@ -196,6 +197,7 @@ inline float elapsed(hipEvent_t a, hipEvent_t b)
 
 
 struct Run { long long start, len; int parent, ra, rb; };
+long long pair_offsets(int ma, int mb, int nmo, long long *off);
 std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked);
 std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vector<PackedDet> &h, bool need_hcol);
 long long tiles_of(const std::vector<EvalBlock> &v);

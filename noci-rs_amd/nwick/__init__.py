@@ -30,7 +30,7 @@ STATUS = {0: "NW_OK", 1: "NW_E_ARG", 2: "NW_E_SHAPE", 3: "NW_E_HIP", 4: "NW_E_ME
 
 # Every symbol the header declares.
 EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matrix", "nw_update_fock",
-           "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_project_hs", "nw_noci_fock", "nw_set_precision", "nw_set_mixed_floor", "nw_pair_slab", "nw_last_timings",
+           "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_project_hs", "nw_noci_fock", "nw_set_precision", "nw_set_mixed_floor", "nw_save_wicks", "nw_load_wicks", "nw_pair_slab", "nw_last_timings",
            "nw_last_error", "nw_destroy", "nw_batch_create", "nw_batch_add", "nw_batch_plan", "nw_batch_run",
            "nw_batch_result", "nw_batch_schedule", "nw_batch_set_schedule", "nw_batch_last_ms",
            "nw_batch_last_error", "nw_batch_destroy", "nw_mlp_create", "nw_mlp_num_params", "nw_mlp_set_params",
@@ -86,6 +86,8 @@ def load():
     L.nw_set_precision.argtypes = [vp, ct.c_int]
     L.nw_project_hs.argtypes = [vp, vp, ct.c_uint64, ct.c_uint64, vp, vp, vp, ct.c_int]
     L.nw_noci_fock.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_double, vp, vp, vp, vp]
+    L.nw_save_wicks.argtypes = [vp, ct.c_char_p]
+    L.nw_load_wicks.argtypes = [vp, ct.c_char_p, ct.c_double]
     L.nw_set_mixed_floor.argtypes = [vp, ct.c_double]
     L.nw_update_fock.argtypes = [vp, vp, vp]
     L.nw_fs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
@@ -272,6 +274,14 @@ class Engine:
         self._check(self.L.nw_noci_fock(self.ctx, _ptr(dets), n, _ptr(coef), float(tol), _ptr(fa), _ptr(fb),
                                         _ptr(da), _ptr(db)))
         return (fa, fb, da, db) if densities else (fa, fb)
+
+    def save_wicks(self, directory):
+        """Write wicks.bin + wicks.meta (the reference's disk cache, version 2) into directory."""
+        self._check(self.L.nw_save_wicks(self.ctx, os.fsencode(directory)))
+
+    def load_wicks(self, directory, tol=1e-8):
+        """Replace the build by a reference-format disk cache from directory."""
+        self._check(self.L.nw_load_wicks(self.ctx, os.fsencode(directory), float(tol)))
 
     def pair_slab(self, x, w):
         off = np.zeros(95, dtype=np.int64)
