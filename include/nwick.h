@@ -185,6 +185,63 @@ const char *nw_last_error(nw_ctx *ctx);
 void nw_destroy(nw_ctx *ctx);
 
 /* ---------------------------------------------------------------------------
+ * Spin-factorised one-body and overlap operators (src/noci/factorise).
+ * A determinant list is split per parent reference P into its unique alpha
+ * and beta excitation strings (SpinFactorisation::new, mod.rs:75-93;
+ * assign_aids/assign_bids :185-278; build_parent_spin_spaces :315-410).  For
+ * every ordered parent pair (Q target, P source) and spin the per-spin
+ * overlap and generalised-Fock factors between Q's and P's strings are
+ * evaluated from the built Wick intermediates, bra = the parent whose first
+ * determinant comes first (ordered_parent_pair, mod.rs:163-183), and
+ * y = (F + lambda S) x is applied as two table contractions per pair.
+ * Components are numbered per parent by (rank, holes, particles, phase).
+ * Every determinant's per-spin rank plus any other parent's must be <= 4.
+ * ------------------------------------------------------------------------- */
+typedef struct nw_factor nw_factor;
+
+/* Table storage: NW_FACTOR_DEVICE keeps every ordered pair's tables in
+ * device memory (SNOCIStorage::RAM in OneBodyFactorisation::new,
+ * onebody.rs:152-210); NW_FACTOR_TRANSIENT regenerates one pair's tables
+ * per use (SNOCIStorage::None, the Transient blocks). */
+enum nw_factor_storage { NW_FACTOR_DEVICE = 0, NW_FACTOR_TRANSIENT = 1 };
+
+/* Factorise dets and build the tables from the context's current build and
+ * Fock: SpinFactorisation::new + OneBodyFactorisation::new
+ * (build_one_body_factor_tables, onebody.rs:938-1017;
+ * build_spin_one_body_factor_row :1300-1569) and the overlap factors of
+ * OverlapFactors::build_overlap_factors (overlap.rs:280-397), which are the S
+ * halves of the same tables.  mode: nw_mode for m > 0 pairs. */
+int nw_factor_create(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, int mode, int storage, nw_factor **out);
+/* Rebuild the tables after nw_update_fock (a new OneBodyFactorisation per
+ * SNOCI iteration, src/snoci/step.rs:292-305).  Applying stale tables fails
+ * with NW_E_STATE. */
+int nw_factor_refresh(nw_factor *f);
+/* y = (F + lambda S) x over every determinant pair (apply_one_body,
+ * onebody.rs:290-345, with apply_one_body_a_first/_b_first :491-648 chosen by
+ * select_one_body_contraction :1583); ys = S x if not NULL (the overlap
+ * action of apply_overlap_sparse, overlap.rs:434-496, on a dense vector).
+ * lambda = -E0 gives the PT2 operator of apply_factorised_shifted_omega_m
+ * (src/snoci/operators.rs:762-778) before the projection correction. */
+int nw_factor_apply(nw_factor *f, double lambda, const double *x, double *y, double *ys);
+/* Diagonals of F + lambda S and of S (one_body_diagonals, onebody.rs:358-405);
+ * either output may be NULL. */
+int nw_factor_diagonals(nw_factor *f, double lambda, double *mdiag, double *sdiag);
+/* Component counts of one parent (parent_component_counts, mod.rs:132-146;
+ * parent < 0 skips them) and the bytes of the resident tables
+ * (OneBodyFactorisation::storage_bytes, onebody.rs:231-281, counted over
+ * every ordered pair). */
+int nw_factor_info(const nw_factor *f, int parent, uint64_t *ncomp_a, uint64_t *ncomp_b, uint64_t *table_bytes);
+/* Per-determinant component ids (SpinFactorisation::aid/bid, mod.rs:106-130). */
+int nw_factor_ids(const nw_factor *f, uint32_t *aids, uint32_t *bids);
+/* Copy one table: ncomp(source) x ncomp(target) interleaved {S, F} doubles,
+ * element [s][t] = the spin factor between target component t of parent
+ * `target` and source component s of parent `source`.  For tests. */
+int nw_factor_table(nw_factor *f, int target, int source, int spin, double *out);
+/* Device time of the last nw_factor_apply {total, table regeneration} in ms. */
+int nw_factor_last_ms(const nw_factor *f, double *ms2);
+void nw_factor_destroy(nw_factor *f);
+
+/* ---------------------------------------------------------------------------
  * Multi-system batches.  No single reference counterpart: noci-rs evaluates
  * one system per build_noci_hs call (src/noci/matrix.rs:214, pair list and
  * Rayon loop of calculate_matrix_elements :39-81).  A batch takes several

@@ -119,6 +119,7 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         const int P = (int)px.size();
         c->built = false;
         c->build_gen++;
+        c->fock_gen++;
         c->tol = tol;
         HIPCHK(hipEventRecord(c->ev[0], c->st));
         Uploader up(c, 64u << 20);
@@ -841,6 +842,7 @@ int nw_update_fock(nw_ctx *c, const double *fa, const double *fb)
         if (!fa || !fb) return fail(c, NW_E_ARG, "null Fock matrix");
         const int n = c->n, nm = c->nmo, nref = c->nref;
         const size_t nn = (size_t)n * n, nnm = (size_t)n * nm;
+        c->fock_gen++;
         double *F = c->fockbuf.get<double>(2 * nn);
         HIPCHK(hipMemcpyAsync(F, fa, nn * 8, hipMemcpyDefault, c->st));
         HIPCHK(hipMemcpyAsync(F + nn, fb, nn * 8, hipMemcpyDefault, c->st));

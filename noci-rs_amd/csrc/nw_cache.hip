@@ -228,6 +228,7 @@ int nw_load_wicks(nw_ctx *c, const char *dir, double tol)
         fb.seekg(0);
         c->built = false;
         c->build_gen++;
+        c->fock_gen++;
         if (slab_len > c->slab_cap) {
             if (c->d_slab) HIPCHK(hipFree(c->d_slab));
             c->d_slab = nullptr;

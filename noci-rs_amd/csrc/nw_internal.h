@@ -93,6 +93,7 @@ struct nw_ctx {
     // build state
     bool built = false;
     unsigned long long build_gen = 0;  // incremented by every nw_build / nw_set_refs
+    unsigned long long fock_gen = 0;   // incremented by every nw_build / nw_update_fock / nw_load_wicks
     bool have_codensity = false;       // co-densities and cx/xc of the last nw_build are kept (nw_update_fock)
     double tol = 1e-8;
     double *d_slab = nullptr;

@@ -35,12 +35,14 @@ EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matri
            "nw_batch_result", "nw_batch_schedule", "nw_batch_set_schedule", "nw_batch_last_ms",
            "nw_batch_last_error", "nw_batch_destroy", "nw_mlp_create", "nw_mlp_num_params", "nw_mlp_set_params",
            "nw_mlp_get_params", "nw_mlp_forward", "nw_mlp_grad", "nw_mlp_step", "nw_mlp_train", "nw_mlp_last_error",
-           "nw_mlp_destroy"]
+           "nw_mlp_destroy", "nw_factor_create", "nw_factor_refresh", "nw_factor_apply", "nw_factor_diagonals",
+           "nw_factor_info", "nw_factor_ids", "nw_factor_table", "nw_factor_last_ms", "nw_factor_destroy"]
 PREC_FP64 = 0
 PREC_MIXED = 1
 BATCH_TUNE = 1
 BATCH_GRAPH = 2
 GROUP_CODE, GROUP_CLASS, GROUP_ALL = 0, 1, 2
+FACTOR_DEVICE, FACTOR_TRANSIENT = 0, 1
 
 _lib = None
 
@@ -98,6 +100,16 @@ def load():
     L.nw_last_error.restype = ct.c_char_p
     L.nw_destroy.argtypes = [vp]
     L.nw_destroy.restype = None
+    L.nw_factor_create.argtypes = [vp, vp, ct.c_uint64, ct.c_int, ct.c_int, ct.POINTER(vp)]
+    L.nw_factor_refresh.argtypes = [vp]
+    L.nw_factor_apply.argtypes = [vp, ct.c_double, vp, vp, vp]
+    L.nw_factor_diagonals.argtypes = [vp, ct.c_double, vp, vp]
+    L.nw_factor_info.argtypes = [vp, ct.c_int, u64p, u64p, u64p]
+    L.nw_factor_ids.argtypes = [vp, vp, vp]
+    L.nw_factor_table.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, vp]
+    L.nw_factor_last_ms.argtypes = [vp, dp]
+    L.nw_factor_destroy.argtypes = [vp]
+    L.nw_factor_destroy.restype = None
     L.nw_batch_create.argtypes = [ct.POINTER(vp), ct.c_int]
     L.nw_batch_add.argtypes = [vp, vp, vp, ct.c_uint64, ct.c_int, ct.POINTER(ct.c_int)]
     L.nw_batch_plan.argtypes = [vp, ct.c_int]
@@ -251,6 +263,10 @@ class Engine:
         self._check(self.L.nw_matvec_m(self.ctx, _ptr(dets), n, float(e0), _ptr(x), _ptr(y), _ptr(ys), int(mode)))
         return y if ys is None else (y, ys)
 
+    def factor(self, dets, mode=MODE_REFERENCE, storage=FACTOR_DEVICE):
+        """Spin-factorised one-body / overlap operator over dets (nw_factor_*)."""
+        return Factor(self, dets, mode, storage)
+
     def project_hs(self, dets, ncur, coef, v=None, s=None, mode=MODE_REFERENCE):
         """v_a = sum_i H_ai c_i, s_a = sum_i S_ai c_i for every candidate a (dets[ncur:]) over the
         current space dets[:ncur]; arrays may be torch device tensors."""
@@ -300,6 +316,75 @@ class Engine:
         if getattr(self, "ctx", None):
             self.L.nw_destroy(self.ctx)
             self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Factor:
+    """Spin-factorised one-body and overlap operators of one determinant list
+    (nw_factor_*): per-parent alpha/beta components and per-spin {S, F}
+    tables for every ordered parent pair, applied as y = (F + lambda S) x."""
+
+    def __init__(self, eng, dets, mode=MODE_REFERENCE, storage=FACTOR_DEVICE):
+        self.eng, self.L = eng, eng.L
+        self.n = Engine._ndet(dets)
+        h = ct.c_void_p()
+        sync_torch(dets)
+        eng._check(self.L.nw_factor_create(eng.ctx, _ptr(dets), self.n, int(mode), int(storage), ct.byref(h)))
+        self.h = h
+
+    def _check(self, rc):
+        self.eng._check(rc)
+
+    def refresh(self):
+        self._check(self.L.nw_factor_refresh(self.h))
+
+    def apply(self, lam, x, y=None, ys=None):
+        """y = (F + lam S) x, and ys = S x if ys is given; arrays may be torch device tensors."""
+        if y is None:
+            y = np.empty(self.n)
+        sync_torch(x, y, ys)
+        self._check(self.L.nw_factor_apply(self.h, float(lam), _ptr(x), _ptr(y), _ptr(ys)))
+        return y if ys is None else (y, ys)
+
+    def diagonals(self, lam, mdiag=None, sdiag=None):
+        mdiag = np.empty(self.n) if mdiag is None else mdiag
+        sdiag = np.empty(self.n) if sdiag is None else sdiag
+        sync_torch(mdiag, sdiag)
+        self._check(self.L.nw_factor_diagonals(self.h, float(lam), _ptr(mdiag), _ptr(sdiag)))
+        return mdiag, sdiag
+
+    def info(self, parent=-1):
+        na, nb, tb = ct.c_uint64(), ct.c_uint64(), ct.c_uint64()
+        self._check(self.L.nw_factor_info(self.h, int(parent), ct.byref(na), ct.byref(nb), ct.byref(tb)))
+        return {"ncomp_a": na.value, "ncomp_b": nb.value, "table_bytes": tb.value}
+
+    def ids(self):
+        a = np.empty(self.n, dtype=np.uint32); b = np.empty(self.n, dtype=np.uint32)
+        self._check(self.L.nw_factor_ids(self.h, _ptr(a), _ptr(b)))
+        return a, b
+
+    def table(self, target, source, spin):
+        """[source component, target component, {S, F}] factor table."""
+        nt = self.info(target)["ncomp_a" if spin == 0 else "ncomp_b"]
+        ns = self.info(source)["ncomp_a" if spin == 0 else "ncomp_b"]
+        out = np.empty((ns, nt, 2))
+        self._check(self.L.nw_factor_table(self.h, int(target), int(source), int(spin), _ptr(out)))
+        return out
+
+    def last_ms(self):
+        ms = (ct.c_double * 2)()
+        self._check(self.L.nw_factor_last_ms(self.h, ms))
+        return {"total_ms": ms[0], "regen_ms": ms[1]}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.nw_factor_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

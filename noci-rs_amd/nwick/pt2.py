@@ -5,7 +5,9 @@ Mirrors the reference's PT2 step (src/snoci/step.rs:178-440):
   |Omega_a> = |Phi_a> - |Psi_0><Psi_0|Phi_a>
   M^Omega x = M x - f_a0 (s_0a . x) - s_a0 (f_0a . x) + 2 E0 s_a0 (s_0a . x)
               (apply_omega_m / apply_projection_correction, operators.rs:696-955)
-  M_ab = F_ab - E0 S_ab, evaluated on the fly by the engine (nw_matvec_m)
+  M_ab = F_ab - E0 S_ab, applied through spin-factorised per-spin tables
+  (nw_factor_apply; apply_factorised_shifted_omega_m, operators.rs:762) or
+  evaluated pairwise on the fly (nw_matvec_m; apply_candidate_m :375)
   solve M^Omega a = -V^Omega with right-preconditioned restarted GMRES
   (gmres.rs:406-544), E_PT2 = a.M^Omega a + a.V + V.a  (step.rs:414-429).
 
@@ -198,13 +200,21 @@ class PT2:
     eng must hold a build covering every ordered reference pair the
     candidate-current and candidate-candidate pairs touch."""
 
-    def __init__(self, eng, sysm, current, candidates, tol=1e-8, mode=0, device="cuda:0"):
+    def __init__(self, eng, sysm, current, candidates, tol=1e-8, mode=0, device="cuda:0", factor_tables="device"):
+        """factor_tables: "device" (tables resident in device memory, the
+        reference's default snoci.gmres.factor_tables = RAM with full_m = None,
+        src/input/snoci.rs:25-33), "transient" (regenerated per matvec) or None
+        (pairwise on-the-fly matvec, nw_matvec_m).  Candidate lists whose
+        per-spin ranks exceed the factor tables' limit, or whose tables do not
+        fit in device memory, use the pairwise matvec."""
         import torch
         self.torch, self.eng, self.sysm = torch, eng, sysm
         self.current, self.candidates = list(current), list(candidates)
         self.tol, self.mode = tol, mode
         self.dev = torch.device(device)
         self.timings = {}
+        self.factor_tables = factor_tables
+        self.fac = None
 
     def setup(self):
         torch, eng = self.torch, self.eng
@@ -234,9 +244,28 @@ class PT2:
         s_a0, f_a0 = S_ai @ c, F_ai @ c
         self.proj = {"e0": self.e0, "s_a0": T(s_a0), "s_0a": T(s_a0), "f_a0": T(f_a0), "f_0a": T(f_a0)}
         self.v_omega = T(H_ai @ c - self.ecurrent * s_a0)
-        self.m_diag = T(f_aa - self.e0 * s_aa)
         self.cand_rec = torch.from_numpy(to_records(self.candidates).view(np.uint8).copy()).to(self.dev)
         self.y = torch.empty(nc, dtype=torch.float64, device=self.dev)
+        self.fac = None
+        if self.factor_tables is not None and nc > 0:
+            from . import FACTOR_DEVICE, FACTOR_TRANSIENT, NwickError
+            storage = FACTOR_DEVICE if self.factor_tables == "device" else FACTOR_TRANSIENT
+            t1 = time.perf_counter()
+            try:
+                self.fac = eng.factor(self.cand_rec, mode=self.mode, storage=storage)
+            except NwickError as e:
+                if e.code not in (4, 5):  # NW_E_MEM, NW_E_RANK: pairwise matvec instead
+                    raise
+                self.fac = None
+            self.timings["factor_s"] = time.perf_counter() - t1
+        if self.fac is not None:
+            # build_factorised_candidate_diags (src/snoci/operators.rs:354-365)
+            md = torch.empty(nc, dtype=torch.float64, device=self.dev)
+            self.fac.diagonals(-self.e0, mdiag=md, sdiag=torch.empty_like(md))
+            self.m_diag = md
+        else:
+            self.m_diag = T(f_aa - self.e0 * s_aa)
+        self.operator = "factorised" if self.fac is not None else "pairwise"
         self.timings["setup_s"] = time.perf_counter() - t0
 
     def apply(self, x):
@@ -245,7 +274,11 @@ class PT2:
         p = self.proj
         sx, fx = float(p["s_0a"] @ x), float(p["f_0a"] @ x)
         y = self.torch.empty_like(self.y)
-        self.eng.matvec_m(self.cand_rec, self.e0, x, y=y, mode=self.mode)
+        if self.fac is not None:
+            # apply_factorised_shifted_omega_m (src/snoci/operators.rs:762-778)
+            self.fac.apply(-self.e0, x, y=y)
+        else:
+            self.eng.matvec_m(self.cand_rec, self.e0, x, y=y, mode=self.mode)
         self.nmatvec += 1
         return y - p["f_a0"] * sx - p["s_a0"] * fx + 2.0 * self.e0 * p["s_a0"] * sx
 

@@ -1192,6 +1192,24 @@ int ow_fs(const double *slab, const long long *off, const double *meta, int nm,
     return 0;
 }
 
+/* One spin's factor of a determinant pair: {S, F} = ph * (xw_overlap, xw_f)
+ * of that spin (the generic path of build_spin_one_body_factor_row,
+ * src/noci/factorise/onebody.rs:1548-1568: phase * xw_f_overlap_prepared).
+ * xh, xp, wh, wp: hole / particle masks {lo, hi} of the bra (x) and ket (w). */
+int ow_fs_spin(const double *slab, const long long *off, const double *meta, int nm, int spin,
+               const uint64_t *xh, const uint64_t *xp, const uint64_t *wh, const uint64_t *wp, double ph,
+               double tol, int mode, double *F, double *Sout)
+{
+    same_view sv[2];
+    diff_view dv;
+    make_views(slab, off, meta, nm, sv, &dv);
+    prep p;
+    prepare_same(&sv[spin], mk128(xh), mk128(xp), mk128(wh), mk128(wp), &p);
+    *Sout = ph * xw_overlap_o(&sv[spin], &p);
+    *F = ph * xw_f_o(&sv[spin], &p, tol, mode);
+    return 0;
+}
+
 /* Evaluate a block of determinant pairs on one ordered reference pair:
  * xex/wex: per pair 8 uint64 masks; exph: per pair excitation phase. */
 int ow_hs_many(const double *slab, const long long *off, const double *meta, int nm, long long npair,

@@ -44,6 +44,8 @@ def lib():
         L.ow_update_fock_pair.argtypes = [ct.c_int, ct.c_int, D, D, D, D, D, U64, U64, D, D, U64, U64,
                                           ct.c_double, I64, D, D]
         L.ow_fs.argtypes = [D, I64, D, ct.c_int, U64, U64, ct.c_double, ct.c_double, ct.c_double, ct.c_int, D, D]
+        L.ow_fs_spin.argtypes = [D, I64, D, ct.c_int, ct.c_int, U64, U64, U64, U64, ct.c_double, ct.c_double,
+                                 ct.c_int, D, D]
         L.ow_meta_len.restype = ct.c_int
         L.ow_offsets_len.restype = ct.c_int
         _lib = L
@@ -145,6 +147,21 @@ class Slab:
         lib().ow_fs(_d(sub), off.ctypes.data_as(I64), _d(meta), self.nmo, _u(xex), _u(wex),
                     bra.pha * ket.pha, bra.phb * ket.phb, self.tol, mode, ct.byref(F), ct.byref(S))
         return F.value, S.value
+
+    def fs_spin(self, bra, ket, spin, mode=0):
+        """One spin's {S, F} factor of a pair (bra = x side), times that spin's
+        excitation phases: the per-spin value the factor tables hold."""
+        sub, off, meta = self.pair_view(bra.parent, ket.parent)
+        if spin == 0:
+            m = [mask2(bra.ha), mask2(bra.pa), mask2(ket.ha), mask2(ket.pa)]
+            ph = bra.pha * ket.pha
+        else:
+            m = [mask2(bra.hb), mask2(bra.pb), mask2(ket.hb), mask2(ket.pb)]
+            ph = bra.phb * ket.phb
+        F = ct.c_double(); S = ct.c_double()
+        lib().ow_fs_spin(_d(sub), off.ctypes.data_as(I64), _d(meta), self.nmo, int(spin), *[_u(v) for v in m],
+                         float(ph), self.tol, mode, ct.byref(F), ct.byref(S))
+        return S.value, F.value
 
     def pair_view(self, x, w):
         b = self.base[(x, w)]

@@ -1,0 +1,161 @@
+"""GPU parity of the spin-factorised one-body / overlap operator (nw_factor_*)
+against the oracle restatement (oracle/spin_factor.py) and the pairwise
+matvec (nw_matvec_m): component ids, per-spin tables for m = 0 and m > 0
+reference pairs in both arithmetic modes, y = (F + lambda S) x and S x with
+resident and regenerated tables, diagonals, the stale-table guard, split
+source groups, and NOCI-PT2 energies through the factorised operator."""
+import numpy as np
+import pytest
+
+import nwick
+import oracle as O
+import spin_factor as OF
+from nwick import basis, synth
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def fcase():
+    sysm = synth.make_system(12, 3, 2, 3, seed=61, orth_refs=(1,), orth_count=1, orthonormal_ao=False, n_aux=24)
+    dets = basis.excited_basis(3, 12, 3, 2, per_ref=30, seed=6, include_refs=True)
+    rng = np.random.default_rng(7)
+    fa = rng.normal(size=(12, 12)); fa = 0.5 * (fa + fa.T)
+    fb = rng.normal(size=(12, 12)); fb = 0.5 * (fb + fb.T)
+    print(f"factor case: n=12 ndet={len(dets)}", flush=True)
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    eng.update_fock(fa, fb)
+    slab = O.Slab(sysm)
+    slab.update_fock(sysm, fa, fb)
+    fac = OF.SpinFactorisation(dets, sysm.nref)
+    return sysm, dets, eng, slab, fac, fa, fb
+
+
+def test_ids_match_oracle(fcase):
+    sysm, dets, eng, slab, fac, fa, fb = fcase
+    f = eng.factor(basis.to_records(dets))
+    a, b = f.ids()
+    assert np.array_equal(a, fac.aid) and np.array_equal(b, fac.bid)
+    for p in range(sysm.nref):
+        info = f.info(p)
+        assert (info["ncomp_a"], info["ncomp_b"]) == (fac.ncomp(p, 0), fac.ncomp(p, 1))
+    f.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_tables_vs_oracle(fcase, mode):
+    sysm, dets, eng, slab, fac, fa, fb = fcase
+    f = eng.factor(basis.to_records(dets), mode=mode)
+    assert any(m != (0, 0) for m in slab.m.values())
+    tabs = OF.tables(slab, dets, fac, mode)
+    worst = 0.0
+    for (q, p, sp), ref in tabs.items():
+        got = f.table(q, p, sp)
+        worst = max(worst, float(np.max(np.abs(got - ref) / np.maximum(1.0, np.abs(ref)))))
+    print(f"mode {mode}: max table deviation {worst:.2e}", flush=True)
+    assert worst < TOL
+    f.close()
+
+
+@pytest.mark.parametrize("storage", [nwick.FACTOR_DEVICE, nwick.FACTOR_TRANSIENT])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_apply_vs_oracle_and_pairwise(fcase, mode, storage):
+    sysm, dets, eng, slab, fac, fa, fb = fcase
+    rec = basis.to_records(dets)
+    f = eng.factor(rec, mode=mode, storage=storage)
+    tabs = OF.tables(slab, dets, fac, mode)
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=len(dets))
+    x[::5] = 0.0
+    lam = 2.75
+    y, ys = f.apply(lam, x, ys=np.empty(len(dets)))
+    yo, yso = OF.apply(fac, tabs, lam, x)
+    scale = max(1.0, float(np.max(np.abs(yo))))
+    assert np.max(np.abs(y - yo)) < 1e-9 * scale
+    assert np.max(np.abs(ys - yso)) < 1e-9 * scale
+    # the pairwise operator (bra = lower index) agrees: same-parent pairs have m = 0
+    yp, ysp = eng.matvec_m(rec, -lam, x, ys=np.empty(len(dets)), mode=mode)
+    assert np.max(np.abs(y - yp)) < 1e-9 * scale
+    assert np.max(np.abs(ys - ysp)) < 1e-9 * scale
+    f.close()
+
+
+def test_diagonals(fcase):
+    sysm, dets, eng, slab, fac, fa, fb = fcase
+    f = eng.factor(basis.to_records(dets))
+    lam = -1.5
+    md, sd = f.diagonals(lam)
+    for i in range(len(dets)):
+        fo, so = slab.fs(dets[i], dets[i], 0)
+        assert abs(sd[i] - so) < TOL and abs(md[i] - (fo + lam * so)) < TOL * max(1.0, abs(fo))
+    f.close()
+
+
+def test_stale_tables_and_refresh(fcase):
+    sysm, dets, eng, slab, fac, fa, fb = fcase
+    rec = basis.to_records(dets)
+    f = eng.factor(rec)
+    x = np.random.default_rng(2).normal(size=len(dets))
+    eng.update_fock(2.0 * fa, fb)
+    with pytest.raises(nwick.NwickError) as ei:
+        f.apply(0.0, x)
+    assert ei.value.code == 6
+    f.refresh()
+    y = f.apply(0.0, x)
+    yp = eng.matvec_m(rec, 0.0, x)
+    assert np.max(np.abs(y - yp)) < 1e-9 * max(1.0, float(np.max(np.abs(yp))))
+    eng.update_fock(fa, fb)
+    f.refresh()
+    f.close()
+
+
+@pytest.mark.parametrize("storage", [nwick.FACTOR_DEVICE, nwick.FACTOR_TRANSIENT])
+def test_split_groups_vs_pairwise(storage):
+    """All singles and doubles of two references: the beta-reference group of
+    every parent holds several hundred entries, so stage 1 splits it into
+    pieces and reduces them; checked against the pairwise matvec."""
+    sysm = synth.make_system(16, 4, 4, 2, seed=5, kappa_sigma=0.3)
+    dets = basis.excited_basis(2, 16, 4, 4, per_ref=None, include_refs=True)
+    print(f"split case: ndet={len(dets)}", flush=True)
+    rec = basis.to_records(dets)
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    f = eng.factor(rec, storage=storage)
+    counts = [f.info(p) for p in range(2)]
+    assert all(c["ncomp_a"] > 64 for c in counts)
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=len(dets))
+    lam = -0.8
+    y, ys = f.apply(lam, x, ys=np.empty(len(dets)))
+    yp, ysp = eng.matvec_m(rec, -lam, x, ys=np.empty(len(dets)))
+    scale = max(1.0, float(np.max(np.abs(yp))))
+    assert np.max(np.abs(y - yp)) < 1e-9 * scale
+    assert np.max(np.abs(ys - ysp)) < 1e-9 * scale
+    f.close()
+
+
+def test_pt2_factorised_equals_pairwise():
+    """E_PT2 through resident tables, regenerated tables and the pairwise
+    matvec.  The synthetic Fock operator makes M^Omega indefinite, so the
+    solve is unrestarted GMRES (restart >= n) as in the dense PT2 check."""
+    from nwick.pt2 import PT2
+    sysm = synth.make_system(10, 2, 2, 3, seed=71, orthonormal_ao=False, n_aux=20)
+    refs = basis.excited_basis(3, 10, 2, 2, per_ref=0, include_refs=True, counts={})
+    allx = basis.excited_basis(3, 10, 2, 2, per_ref=None, include_refs=False)
+    cands = [d for d in allx if d.parent == 0 or (d.parent == 1 and d.rank_a + d.rank_b == 1)]
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    out = {}
+    for ft in ("device", "transient", None):
+        pt = PT2(eng, sysm, refs, cands, mode=1, factor_tables=ft)
+        pt.setup()
+        assert pt.operator == ("pairwise" if ft is None else "factorised")
+        r = pt.solve(res_tol=1e-10, restart=400, max_iter=800)
+        print(f"factor_tables={ft}: E_PT2 {r['ept2']:.12f} iters {r['iterations']} converged {r['converged']}",
+              flush=True)
+        assert r["converged"]
+        out[ft] = r["ept2"]
+    assert abs(out["device"] - out[None]) < 1e-9
+    assert abs(out["transient"] - out[None]) < 1e-9
