@@ -15,15 +15,23 @@
 // Tables are double2 {S, F} indexed [source component][target component]
 // (target contiguous) and held in device memory for every ordered parent
 // pair, or regenerated per application (the reference's RAM and None
-// storage).  y = (F + lambda S) x is then, per ordered pair and with
-// (1, 2) = (alpha, beta) or (beta, alpha) picked by the reference's cost model
-// (select_one_body_contraction, onebody.rs:1583-1606):
-//   stage 1  T[s2][t1] = sum_{e in P, c2_e = s2} x_e Tab1[c1_e][t1]           (sparse x dense)
-//   stage 2  y_I += sum_s2 T.F[s2][t1_I] S2[s2][t2_I]
-//                  + T.S[s2][t1_I] (F2 + lambda S2)[s2][t2_I]                   (sampled dense x dense)
-// (apply_one_body_a_first / _b_first, onebody.rs:491-648); ys = S x comes
-// from the same pass.  Stage 1 streams Tab1 once per pair with coalesced
-// rows; stage 2 streams Tab2 and T with the target entries on the lanes.
+// storage).  y = (F + lambda S) x = sum over parent pairs of
+// sample_{(a_I, b_I)} (Tab_a^T X_P Tab_b), X_P the sparse (alpha x beta)
+// component matrix of the source entries (apply_one_body_a_first / _b_first,
+// onebody.rs:491-648).  The reference picks, per parent pair, whether to sum
+// the source over alpha first (rows grouped by beta component) or beta first
+// (select_one_body_contraction, :1583-1606).  Here every source entry is
+// assigned to a group keyed by its beta or its alpha component through a
+// greedy vertex cover of the bipartite (alpha, beta) entry graph, so that an
+// arrow-shaped X_P (reference row and column full, a sparse block) needs only
+// a few hundred rows where either single order needs thousands:
+//   stage 1  U_g = sum_{e in g} x_e Tab_a[a_e]  (group keyed by beta b_g),
+//            V_g = sum_{e in g} x_e Tab_b[b_e]  (group keyed by alpha a_g);
+//   stage 2  y_I += sum_g U_g[a_I] . Tab_b[b_g][b_I]  or  Tab_a[a_g][a_I] . V_g[b_I],
+//            + x_e Tab_a[a_e][a_I] . Tab_b[b_e][b_I] for one-entry groups,
+// with u . v = u.F v.S + u.S (v.F + lambda v.S); ys = S x from the same pass.
+// Stage 1 streams each table row about once; stage 2 is one lane per target
+// entry over the group rows.
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -48,12 +56,21 @@ struct SpinComp {
 };
 static_assert(sizeof(SpinComp) == 12, "SpinComp layout");
 
-struct S1Piece { int e0, e1, row, part; };  // entry range -> output row (part >= 0: partial row)
-struct S1Unit { int p0, p1; };              // pieces of one workgroup
-struct S1Red { int row, p0, p1; };          // partial rows [p0, p1) summed into row
+// Stage-1 work item: out[t] = sum_{k in [e0, e1)} x[mdet[k]] tab[midx[k] nt + t], t < nt.
+struct S1Item {
+    const double2 *tab;
+    double2 *out;
+    int nt, e0, e1, pad;
+};
+// Stage-2 row: alpha-side row u (over target alpha components), beta-side row
+// v (over target beta components), weight x[wdet] (wdet >= 0) or 1.
+struct S2Row {
+    const double2 *u, *v;
+    long long wdet;
+};
 
-constexpr int S1_CHUNK = 64;   // entries per piece of a split group
-constexpr int S1_UNIT = 64;    // entries per workgroup (packing target)
+constexpr int S1_CHUNK = 128;  // entries per stage-1 item (pieces of a large group)
+constexpr int S2_BATCH = 256;  // rows staged in LDS at a time
 
 template <int L>
 __device__ inline void spin_fs(const double *slab, const long long *o, const SpinScal &sc, int nmo,
@@ -100,74 +117,70 @@ __global__ __launch_bounds__(256) void k_fac_table(const PairInfo *__restrict__ 
     tab[(size_t)s * nt + t] = make_double2(ph * vs, ph * vf);
 }
 
-// xs[e] = x[det[e]] for every source entry of every parent and order.
-__global__ void k_fac_gather(const uint32_t *__restrict__ det, long long n, const double *__restrict__ x,
-                             double *__restrict__ xs)
+// Stage 1: one item per workgroup (x values and table row indices staged in LDS).
+__global__ __launch_bounds__(256) void k_fac_mat(const S1Item *__restrict__ items, const uint32_t *__restrict__ midx,
+                                                 const uint32_t *__restrict__ mdet, const double *__restrict__ x)
 {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) xs[e] = x[det[e]];
-}
-
-// Stage 1: T[row][t] (or a partial row) = sum_e xs[e] Tab1[c1[e]][t].
-__global__ __launch_bounds__(256) void k_fac_spmm(const S1Unit *__restrict__ units, const S1Piece *__restrict__ pieces,
-                                                  const uint32_t *__restrict__ c1, const double *__restrict__ xs,
-                                                  const double2 *__restrict__ tab1, int nt1, double2 *__restrict__ tt,
-                                                  double2 *__restrict__ part)
-{
-    const S1Unit u = units[blockIdx.x];
-    const int t = blockIdx.y * 256 + threadIdx.x;
-    if (t >= nt1) return;
-    for (int p = u.p0; p < u.p1; ++p) {
-        const S1Piece pc = pieces[p];
-        double s = 0.0, f = 0.0;
-#pragma unroll 4
-        for (int e = pc.e0; e < pc.e1; ++e) {
-            const double xe = xs[e];
-            const double2 v = tab1[(size_t)c1[e] * nt1 + t];
-            s = fma(xe, v.x, s);
-            f = fma(xe, v.y, f);
-        }
-        double2 *out = pc.part < 0 ? tt + (size_t)pc.row * nt1 : part + (size_t)pc.part * nt1;
-        out[t] = make_double2(s, f);
+    __shared__ int s_r[S1_CHUNK];
+    __shared__ double s_x[S1_CHUNK];
+    const S1Item it = items[blockIdx.x];
+    const int ne = it.e1 - it.e0;
+    if ((int)threadIdx.x < ne) {
+        s_r[threadIdx.x] = (int)midx[it.e0 + threadIdx.x];
+        s_x[threadIdx.x] = x[mdet[it.e0 + threadIdx.x]];
     }
-}
-
-__global__ __launch_bounds__(256) void k_fac_spmm_reduce(const S1Red *__restrict__ reds, const double2 *__restrict__ part,
-                                                         int nt1, double2 *__restrict__ tt)
-{
-    const S1Red r = reds[blockIdx.x];
+    __syncthreads();
     const int t = blockIdx.y * 256 + threadIdx.x;
-    if (t >= nt1) return;
+    if (t >= it.nt) return;
+    const double2 *col = it.tab + t;
     double s = 0.0, f = 0.0;
-    for (int p = r.p0; p < r.p1; ++p) {
-        const double2 v = part[(size_t)p * nt1 + t];
-        s += v.x;
-        f += v.y;
+#pragma unroll 8
+    for (int k = 0; k < ne; ++k) {
+        const double2 v = col[(size_t)s_r[k] * it.nt];
+        s = fma(s_x[k], v.x, s);
+        f = fma(s_x[k], v.y, f);
     }
-    tt[(size_t)r.row * nt1 + t] = make_double2(s, f);
+    it.out[t] = make_double2(s, f);
 }
 
-// Stage 2 over the source components [k chunk, (k+1) chunk): one lane per target entry.
-__global__ __launch_bounds__(256) void k_fac_sddmm(const uint32_t *__restrict__ t1i, const uint32_t *__restrict__ t2i,
-                                                   int nt, const double2 *__restrict__ tt, int nt1,
-                                                   const double2 *__restrict__ tab2, int nt2, int ns2, int chunk,
-                                                   double lambda, double2 *__restrict__ part2)
+// Stage 2 over the rows [k chunk, (k+1) chunk): one lane per target entry.
+__global__ __launch_bounds__(256) void k_fac_rows(const uint32_t *__restrict__ ta, const uint32_t *__restrict__ tb,
+                                                  int nt, const S2Row *__restrict__ rows, int nrows, int chunk,
+                                                  const double *__restrict__ x, double lambda,
+                                                  double2 *__restrict__ part2)
 {
+    __shared__ const double2 *s_u[S2_BATCH];
+    __shared__ const double2 *s_v[S2_BATCH];
+    __shared__ double s_w[S2_BATCH];
     const int I = blockIdx.x * 256 + threadIdx.x;
-    if (I >= nt) return;
-    const int s0 = blockIdx.y * chunk, s1 = min(ns2, s0 + chunk);
-    const double2 *pa = tt + t1i[I];
-    const double2 *pb = tab2 + t2i[I];
+    const bool live = I < nt;
+    const int r0 = blockIdx.y * chunk, r1 = min(nrows, r0 + chunk);
+    const uint32_t a = live ? ta[I] : 0, b = live ? tb[I] : 0;
     double m = 0.0, s = 0.0;
-#pragma unroll 4
-    for (int k = s0; k < s1; ++k) {
-        const double2 u = pa[(size_t)k * nt1];  // {T.S, T.F}
-        const double2 v = pb[(size_t)k * nt2];  // {S2, F2}
-        m = fma(u.y, v.x, m);
-        m = fma(u.x, fma(lambda, v.x, v.y), m);
-        s = fma(u.x, v.x, s);
+    for (int base = r0; base < r1; base += S2_BATCH) {
+        const int nb = min(S2_BATCH, r1 - base);
+        __syncthreads();
+        if ((int)threadIdx.x < nb) {
+            const S2Row r = rows[base + threadIdx.x];
+            s_u[threadIdx.x] = r.u;
+            s_v[threadIdx.x] = r.v;
+            s_w[threadIdx.x] = r.wdet >= 0 ? x[r.wdet] : 1.0;
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll 8
+            for (int k = 0; k < nb; ++k) {
+                const double2 u = s_u[k][a];  // alpha side {S, F}
+                const double2 v = s_v[k][b];  // beta side {S, F}
+                const double w = s_w[k];
+                const double us = w * u.x, uf = w * u.y;
+                m = fma(uf, v.x, m);
+                m = fma(us, fma(lambda, v.x, v.y), m);
+                s = fma(us, v.x, s);
+            }
+        }
     }
-    part2[(size_t)blockIdx.y * nt + I] = make_double2(m, s);
+    if (live) part2[(size_t)blockIdx.y * nt + I] = make_double2(m, s);
 }
 
 // y[det_I] += sum_k part2[k][I] (chunks in order); the same for ys.
@@ -233,15 +246,14 @@ struct DVec {
     }
 };
 
-// Source entries of one parent grouped by the second spin's component.
-struct SrcOrder {
-    std::vector<uint32_t> c1, det;  // host copies
-    DVec<uint32_t> d_c1;
-    DVec<S1Piece> d_pieces;
-    DVec<S1Unit> d_units;
-    DVec<S1Red> d_reds;
-    long long xs_off = 0;  // offset of this order's entries in the gathered x
-    int nunits = 0, nreds = 0, npart = 0;
+// A group of source entries of one parent sharing one component: keyed by the
+// beta component (side 1: alpha rows summed) or the alpha component (side 0:
+// beta rows summed).  Entries [m0, m1) of the parent's materialisation list;
+// a one-entry group keeps its entry in `single` instead.
+struct Group {
+    int side, key, m0, m1;
+    int single_a, single_b;
+    uint32_t single_det;
 };
 
 struct ParentSpace {
@@ -251,8 +263,79 @@ struct ParentSpace {
     std::vector<uint32_t> tdet, ta, tb;  // target entries in determinant order
     DVec<uint32_t> d_tdet, d_ta, d_tb;
     int maxrank[2] = {0, 0};
-    SrcOrder ord[2];  // 0: grouped by beta (alpha contracted first), 1: grouped by alpha
+    std::vector<Group> groups;
+    long long m_off = 0;  // offset of this parent's materialisation entries in the global list
+    int ngroup_a = 0, ngroup_b = 0, nsingle = 0;
 };
+
+// Greedy vertex cover of the bipartite (alpha component, beta component)
+// graph whose edges are the parent's entries: repeatedly take the component
+// with the most uncovered entries and group those entries under it.
+void cover_groups(ParentSpace &P, std::vector<uint32_t> &midx, std::vector<uint32_t> &mdet)
+{
+    const int na = P.ncomp[0], nb = P.ncomp[1], n = P.nt;
+    std::vector<std::vector<int>> by[2];
+    by[0].resize(na);
+    by[1].resize(nb);
+    for (int k = 0; k < n; ++k) {
+        by[0][P.ta[k]].push_back(k);
+        by[1][P.tb[k]].push_back(k);
+    }
+    std::vector<int> deg[2];
+    deg[0].resize(na);
+    deg[1].resize(nb);
+    std::vector<std::tuple<int, int, int>> heap;  // (degree, -side, -key): max-heap pops the largest degree
+    for (int sd = 0; sd < 2; ++sd)
+        for (int c = 0; c < (sd ? nb : na); ++c) {
+            deg[sd][c] = (int)by[sd][c].size();
+            heap.emplace_back(deg[sd][c], -sd, -c);
+        }
+    std::make_heap(heap.begin(), heap.end());
+    std::vector<char> alive(n, 1);
+    P.m_off = (long long)midx.size();
+    while (!heap.empty()) {
+        std::pop_heap(heap.begin(), heap.end());
+        auto [d, nsd, nc] = heap.back();
+        heap.pop_back();
+        const int sd = -nsd, c = -nc;
+        if (d == 0 || d != deg[sd][c]) continue;  // stale entry
+        std::vector<int> mem;
+        for (int k : by[sd][c])
+            if (alive[k]) mem.push_back(k);
+        deg[sd][c] = 0;
+        for (int k : mem) {
+            alive[k] = 0;
+            const int osd = 1 - sd, oc = (int)(osd ? P.tb[k] : P.ta[k]);
+            if (--deg[osd][oc] > 0) {
+                heap.emplace_back(deg[osd][oc], -osd, -oc);
+                std::push_heap(heap.begin(), heap.end());
+            }
+        }
+        Group g{};
+        g.side = sd == 0 ? 0 : 1;
+        g.key = c;
+        if (mem.size() == 1) {
+            g.m0 = g.m1 = -1;
+            g.single_a = (int)P.ta[mem[0]];
+            g.single_b = (int)P.tb[mem[0]];
+            g.single_det = P.tdet[mem[0]];
+            ++P.nsingle;
+        } else {
+            // entries ordered by the summed component (table rows in ascending order)
+            std::sort(mem.begin(), mem.end(), [&](int u, int v) {
+                return sd == 1 ? P.ta[u] < P.ta[v] : P.tb[u] < P.tb[v];
+            });
+            g.m0 = (int)(midx.size() - P.m_off);
+            for (int k : mem) {
+                midx.push_back(sd == 1 ? P.ta[k] : P.tb[k]);
+                mdet.push_back(P.tdet[k]);
+            }
+            g.m1 = (int)(midx.size() - P.m_off);
+            (sd == 1 ? P.ngroup_a : P.ngroup_b)++;
+        }
+        P.groups.push_back(g);
+    }
+}
 
 uint64_t spin_key(const PackedDet &d, int sp)
 {
@@ -281,6 +364,14 @@ SpinComp spin_comp(const PackedDet &d, int sp)
     return c;
 }
 
+// Per ordered parent pair: its stage-1 items and stage-2 rows (ranges into the
+// uploaded arrays) and its materialisation footprint.
+struct PairPlan {
+    int q = -1, p = -1;
+    long long item0 = 0, item1 = 0, row0 = 0, row1 = 0;
+    size_t mat_elems = 0;
+};
+
 }  // namespace
 
 struct nw_factor {
@@ -293,15 +384,15 @@ struct nw_factor {
     DVec<SpinComp> d_comp[2];
     // tables: per (Q * nref + P) * 2 + spin, offset into d_tab (-1 = none)
     std::vector<long long> tab_off;
-    std::vector<int> order;  // per (Q, P): 0 alpha first, 1 beta first
     DVec<double2> d_tab;
     size_t tab_elems = 0;
-    // scratch
-    DVec<uint32_t> d_srcdet;
-    DVec<double> d_xs, d_y;
-    DVec<double2> d_tt, d_part1, d_part2;
-    size_t tt_elems = 0, part1_elems = 0, part2_elems = 0;
-    long long nsrc = 0;
+    // plan
+    DVec<uint32_t> d_midx, d_mdet;
+    DVec<S1Item> d_items;
+    DVec<S2Row> d_rows;
+    std::vector<PairPlan> plans;  // ordered by target, then source
+    DVec<double2> d_mat, d_part2;
+    int maxnt = 0;
     double t_ms[2] = {0, 0};  // last apply: total, table regeneration
 };
 
@@ -359,6 +450,94 @@ double2 *table_ptr(nw_factor *f, int q, int p, int sp, double2 *scratch)
     if (f->storage == NW_FACTOR_DEVICE) return f->d_tab.p + f->tab_off[(q * f->nref + p) * 2 + sp];
     launch_table(f, q, p, sp, scratch);
     return scratch;
+}
+
+// Stage-1 items and stage-2 rows of every ordered pair.  Resident tables:
+// the materialisation buffer holds one target's pairs at a time (pairs of a
+// target are contiguous); regenerated tables: one pair at a time, both
+// tables in the scratch table slot.
+void make_plans(nw_factor *f)
+{
+    const int nref = f->nref;
+    std::vector<S1Item> items;
+    std::vector<S2Row> rows;
+    size_t mat_need = 0;
+    const bool dev = f->storage == NW_FACTOR_DEVICE;
+    for (int q = 0; q < nref; ++q) {
+        const ParentSpace &Q = f->par[q];
+        if (Q.nt == 0) continue;
+        size_t mat_q = 0;
+        for (int p = 0; p < nref; ++p) {
+            const ParentSpace &P = f->par[p];
+            if (P.nt == 0) continue;
+            PairPlan pl;
+            pl.q = q;
+            pl.p = p;
+            const int nta = Q.ncomp[0], ntb = Q.ncomp[1];
+            const double2 *taba, *tabb;
+            if (dev) {
+                taba = f->d_tab.p + f->tab_off[(q * nref + p) * 2 + 0];
+                tabb = f->d_tab.p + f->tab_off[(q * nref + p) * 2 + 1];
+            } else {
+                taba = f->d_tab.p;
+                tabb = f->d_tab.p + table_elems(f, q, p, 0);
+            }
+            size_t mo = dev ? mat_q : 0;
+            pl.item0 = (long long)items.size();
+            pl.row0 = (long long)rows.size();
+            for (const Group &g : P.groups) {
+                if (g.m0 < 0) {
+                    rows.push_back({taba + (size_t)g.single_a * nta, tabb + (size_t)g.single_b * ntb,
+                                    (long long)g.single_det});
+                    continue;
+                }
+                const int len = g.side == 1 ? nta : ntb;
+                for (int e = g.m0; e < g.m1; e += S1_CHUNK) {
+                    double2 *out = f->d_mat.p + mo;
+                    items.push_back({g.side == 1 ? taba : tabb, out, len, (int)(P.m_off + e),
+                                     (int)(P.m_off + std::min(g.m1, e + S1_CHUNK)), 0});
+                    if (g.side == 1) rows.push_back({out, tabb + (size_t)g.key * ntb, -1});
+                    else rows.push_back({taba + (size_t)g.key * nta, out, -1});
+                    mo += (size_t)len;
+                }
+            }
+            pl.item1 = (long long)items.size();
+            pl.row1 = (long long)rows.size();
+            pl.mat_elems = mo - (dev ? mat_q : 0);
+            if (dev) mat_q = mo;
+            else mat_need = std::max(mat_need, pl.mat_elems);
+            f->plans.push_back(pl);
+        }
+        if (dev) mat_need = std::max(mat_need, mat_q);
+    }
+    f->d_items.upload(items);
+    f->d_rows.upload(rows);
+    (void)mat_need;
+}
+
+size_t mat_elems_needed(const nw_factor *f)
+{
+    // the same walk as make_plans without pointers
+    size_t need = 0;
+    for (int q = 0; q < f->nref; ++q) {
+        const ParentSpace &Q = f->par[q];
+        if (Q.nt == 0) continue;
+        size_t mq = 0;
+        for (int p = 0; p < f->nref; ++p) {
+            const ParentSpace &P = f->par[p];
+            if (P.nt == 0) continue;
+            size_t mp = 0;
+            for (const Group &g : P.groups)
+                if (g.m0 >= 0) {
+                    const size_t len = g.side == 1 ? Q.ncomp[0] : Q.ncomp[1];
+                    mp += len * (size_t)((g.m1 - g.m0 + S1_CHUNK - 1) / S1_CHUNK);
+                }
+            if (f->storage == NW_FACTOR_DEVICE) mq += mp;
+            else need = std::max(need, mp);
+        }
+        need = std::max(need, mq);
+    }
+    return need;
 }
 
 }  // namespace
@@ -435,94 +614,26 @@ extern "C" int nw_factor_create(nw_ctx *c, const nw_det *dets, uint64_t ndet, in
             if (P.ncomp[0] > 262140 || P.ncomp[1] > 262140)
                 return fail(c, NW_E_SHAPE, "more than 262140 spin components in one parent");
         for (int sp = 0; sp < 2; ++sp) f->d_comp[sp].upload(comps[sp]);
-        // ---- source orders: entries grouped by the second spin, pieces, units, reductions
-        std::vector<uint32_t> srcdet;
-        for (int p = 0; p < c->nref; ++p) {
-            ParentSpace &P = f->par[p];
+        // ---- source groups (vertex cover) and target lists
+        std::vector<uint32_t> midx, mdet;
+        for (ParentSpace &P : f->par) {
             if (P.nt == 0) continue;
             P.d_tdet.upload(P.tdet);
             P.d_ta.upload(P.ta);
             P.d_tb.upload(P.tb);
-            for (int o = 0; o < 2; ++o) {
-                SrcOrder &S = P.ord[o];
-                const std::vector<uint32_t> &c1v = o == 0 ? P.ta : P.tb, &c2v = o == 0 ? P.tb : P.ta;
-                std::vector<int> idx(P.nt);
-                std::iota(idx.begin(), idx.end(), 0);
-                std::stable_sort(idx.begin(), idx.end(), [&](int u, int v) {
-                    return c2v[u] != c2v[v] ? c2v[u] < c2v[v] : c1v[u] < c1v[v];
-                });
-                for (int k : idx) {
-                    S.c1.push_back(c1v[k]);
-                    S.det.push_back(P.tdet[k]);
-                }
-                std::vector<S1Piece> pieces;
-                std::vector<S1Red> reds;
-                int npart = 0;
-                for (int e = 0; e < P.nt;) {
-                    int g = e;
-                    while (g < P.nt && c2v[idx[g]] == c2v[idx[e]]) ++g;
-                    const int row = (int)c2v[idx[e]];
-                    if (g - e <= S1_CHUNK) {
-                        pieces.push_back({e, g, row, -1});
-                    } else {
-                        const int p0 = npart;
-                        for (int q0 = e; q0 < g; q0 += S1_CHUNK)
-                            pieces.push_back({q0, std::min(g, q0 + S1_CHUNK), row, npart++});
-                        reds.push_back({row, p0, npart});
-                    }
-                    e = g;
-                }
-                std::vector<S1Unit> units;
-                for (int a = 0; a < (int)pieces.size();) {
-                    int b = a, cnt = 0;
-                    while (b < (int)pieces.size() && (b == a || cnt + pieces[b].e1 - pieces[b].e0 <= S1_UNIT)) {
-                        cnt += pieces[b].e1 - pieces[b].e0;
-                        ++b;
-                    }
-                    units.push_back({a, b});
-                    a = b;
-                }
-                S.nunits = (int)units.size();
-                S.nreds = (int)reds.size();
-                S.npart = npart;
-                S.xs_off = (long long)srcdet.size();
-                srcdet.insert(srcdet.end(), S.det.begin(), S.det.end());
-                S.d_c1.upload(S.c1);
-                S.d_pieces.upload(pieces);
-                S.d_units.upload(units);
-                S.d_reds.upload(reds);
-            }
+            cover_groups(P, midx, mdet);
+            f->maxnt = std::max(f->maxnt, P.nt);
         }
-        // part1 rows are indexed by the target's first-spin components: size for the largest parent
-        int maxcomp = 0, maxnt = 0;
-        for (const ParentSpace &P : f->par) {
-            maxcomp = std::max({maxcomp, P.ncomp[0], P.ncomp[1]});
-            maxnt = std::max(maxnt, P.nt);
-        }
-        size_t maxpart = 0;
-        for (const ParentSpace &P : f->par)
-            for (int o = 0; o < 2; ++o) maxpart = std::max(maxpart, (size_t)P.ord[o].npart);
-        f->part1_elems = std::max<size_t>(1, maxpart * (size_t)maxcomp);
-        f->nsrc = (long long)srcdet.size();
-        f->d_srcdet.upload(srcdet);
-        f->d_xs.alloc(std::max<long long>(1, f->nsrc));
-        // ---- contraction order per pair and table placement
+        if (midx.size() >= (1ull << 31)) return fail(c, NW_E_SHAPE, "too many source entries");
+        f->d_midx.upload(midx);
+        f->d_mdet.upload(mdet);
+        // ---- tables
         const int nref = c->nref;
-        f->order.assign((size_t)nref * nref, 0);
         f->tab_off.assign((size_t)nref * nref * 2, -1);
         size_t off = 0, maxpair = 0;
         for (int q = 0; q < nref; ++q)
             for (int p = 0; p < nref; ++p) {
-                const ParentSpace &Q = f->par[q], &P = f->par[p];
-                if (Q.nt == 0 || P.nt == 0) continue;
-                const double nta = Q.ncomp[0], ntb = Q.ncomp[1], nsa = P.ncomp[0], nsb = P.ncomp[1];
-                const double ca = 2.0 * nta * P.nt + 2.0 * Q.nt * nsb;
-                const double cb = 2.0 * ntb * P.nt + 2.0 * Q.nt * nsa;
-                const int o = ca <= cb ? 0 : 1;
-                f->order[q * nref + p] = o;
-                // stage-1 output: second-spin source components x first-spin target components
-                const size_t tt = o == 0 ? (size_t)nsb * (size_t)nta : (size_t)nsa * (size_t)ntb;
-                f->tt_elems = std::max(f->tt_elems, tt);
+                if (f->par[q].nt == 0 || f->par[p].nt == 0) continue;
                 size_t pair_elems = 0;
                 for (int sp = 0; sp < 2; ++sp) {
                     f->tab_off[(q * nref + p) * 2 + sp] = (long long)off;
@@ -532,16 +643,10 @@ extern "C" int nw_factor_create(nw_ctx *c, const nw_det *dets, uint64_t ndet, in
                 maxpair = std::max(maxpair, pair_elems);
             }
         f->tab_elems = off;
-        if (storage == NW_FACTOR_DEVICE) {
-            f->d_tab.alloc(std::max<size_t>(1, off));
-        } else {
-            // one pair's two tables, regenerated per use
-            f->d_tab.alloc(std::max<size_t>(1, maxpair));
-        }
-        f->d_tt.alloc(std::max<size_t>(1, f->tt_elems));
-        f->d_part1.alloc(f->part1_elems);
-        f->part2_elems = std::max<size_t>(1, (size_t)64 * (size_t)std::max(1, maxnt));
-        f->d_part2.alloc(f->part2_elems);
+        f->d_tab.alloc(std::max<size_t>(1, storage == NW_FACTOR_DEVICE ? off : maxpair));
+        f->d_mat.alloc(std::max<size_t>(1, mat_elems_needed(f.get())));
+        f->d_part2.alloc((size_t)64 * (size_t)std::max(1, f->maxnt));
+        make_plans(f.get());
         if (storage == NW_FACTOR_DEVICE) build_tables(f.get());
         else {
             f->build_gen = c->build_gen;
@@ -567,6 +672,34 @@ extern "C" int nw_factor_refresh(nw_factor *f)
     });
 }
 
+namespace {
+
+// Stage 1 over items [i0, i1), stage 2 over rows [r0, r1) for the targets of parent q.
+void run_range(nw_factor *f, int q, long long i0, long long i1, long long r0, long long r1, const double *dx,
+               double lambda, double *dy, double *dys)
+{
+    nw_ctx *c = f->c;
+    const ParentSpace &Q = f->par[q];
+    if (i1 > i0) {
+        const int maxlen = std::max(Q.ncomp[0], Q.ncomp[1]);
+        hipLaunchKernelGGL(k_fac_mat, dim3((unsigned)(i1 - i0), blocks_of(maxlen, 256)), dim3(256), 0, c->st,
+                           f->d_items.p + i0, f->d_midx.p, f->d_mdet.p, dx);
+    }
+    const int nr = (int)(r1 - r0);
+    if (nr == 0) return;
+    const long long waves = (Q.nt + 63) / 64;
+    int nk = (int)std::min<long long>(64, std::max<long long>(1, (4096 + waves - 1) / waves));
+    nk = std::min(nk, std::max(1, nr / 16));
+    const int chunk = (nr + nk - 1) / nk;
+    nk = (nr + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_fac_rows, dim3(blocks_of(Q.nt, 256), (unsigned)nk), dim3(256), 0, c->st, Q.d_ta.p, Q.d_tb.p,
+                       Q.nt, f->d_rows.p + r0, nr, chunk, dx, lambda, f->d_part2.p);
+    hipLaunchKernelGGL(k_fac_scatter, dim3(blocks_of(Q.nt, 256)), dim3(256), 0, c->st, Q.d_tdet.p, Q.nt, nk,
+                       f->d_part2.p, dy, dys);
+}
+
+}  // namespace
+
 extern "C" int nw_factor_apply(nw_factor *f, double lambda, const double *x, double *y, double *ys)
 {
     if (!f) return NW_E_ARG;
@@ -582,55 +715,25 @@ extern "C" int nw_factor_apply(nw_factor *f, double lambda, const double *x, dou
         double *dys = ys ? (ysdev ? ys : c->outS.get<double>(2 * n) + n) : nullptr;
         hipLaunchKernelGGL(k_fac_zero, dim3(blocks_of(n, 256)), dim3(256), 0, c->st, dy, n);
         if (dys) hipLaunchKernelGGL(k_fac_zero, dim3(blocks_of(n, 256)), dim3(256), 0, c->st, dys, n);
-        if (f->nsrc > 0)
-            hipLaunchKernelGGL(k_fac_gather, dim3(blocks_of(f->nsrc, 256)), dim3(256), 0, c->st, f->d_srcdet.p,
-                               f->nsrc, dx, f->d_xs.p);
-        const int nref = f->nref;
         float regen = 0.f;
-        for (int q = 0; q < nref; ++q) {
-            const ParentSpace &Q = f->par[q];
-            if (Q.nt == 0) continue;
-            for (int p = 0; p < nref; ++p) {
-                const ParentSpace &P = f->par[p];
-                if (P.nt == 0) continue;
-                const int o = f->order[q * nref + p];
-                const int s1 = o == 0 ? 0 : 1, s2 = 1 - s1;  // spin contracted in stage 1 / stage 2
-                double2 *tab1, *tab2;
-                if (f->storage == NW_FACTOR_DEVICE) {
-                    tab1 = f->d_tab.p + f->tab_off[(q * nref + p) * 2 + s1];
-                    tab2 = f->d_tab.p + f->tab_off[(q * nref + p) * 2 + s2];
-                } else {
-                    HIPCHK(hipEventRecord(c->ev[2], c->st));
-                    tab1 = f->d_tab.p;
-                    tab2 = f->d_tab.p + table_elems(f, q, p, s1);
-                    launch_table(f, q, p, s1, tab1);
-                    launch_table(f, q, p, s2, tab2);
-                    HIPCHK(hipEventRecord(c->ev[3], c->st));
-                }
-                const SrcOrder &S = P.ord[o];
-                const int nt1 = Q.ncomp[s1], nt2 = Q.ncomp[s2], ns2 = P.ncomp[s2];
-                // stage 1
-                hipLaunchKernelGGL(k_fac_spmm, dim3((unsigned)S.nunits, blocks_of(nt1, 256)), dim3(256), 0, c->st,
-                                   S.d_units.p, S.d_pieces.p, S.d_c1.p, f->d_xs.p + S.xs_off, tab1, nt1, f->d_tt.p,
-                                   f->d_part1.p);
-                if (S.nreds > 0)
-                    hipLaunchKernelGGL(k_fac_spmm_reduce, dim3((unsigned)S.nreds, blocks_of(nt1, 256)), dim3(256), 0,
-                                       c->st, S.d_reds.p, f->d_part1.p, nt1, f->d_tt.p);
-                // stage 2: split the source components so the grid covers the chip
-                const long long waves = (Q.nt + 63) / 64;
-                int nk = (int)std::min<long long>(64, std::max<long long>(1, (4096 + waves - 1) / waves));
-                nk = std::min(nk, std::max(1, ns2 / 32));
-                const int chunk = (ns2 + nk - 1) / nk;
-                nk = (ns2 + chunk - 1) / chunk;
-                const uint32_t *t1i = o == 0 ? Q.d_ta.p : Q.d_tb.p, *t2i = o == 0 ? Q.d_tb.p : Q.d_ta.p;
-                hipLaunchKernelGGL(k_fac_sddmm, dim3(blocks_of(Q.nt, 256), (unsigned)nk), dim3(256), 0, c->st, t1i,
-                                   t2i, Q.nt, f->d_tt.p, nt1, tab2, nt2, ns2, chunk, lambda, f->d_part2.p);
-                hipLaunchKernelGGL(k_fac_scatter, dim3(blocks_of(Q.nt, 256)), dim3(256), 0, c->st, Q.d_tdet.p, Q.nt,
-                                   nk, f->d_part2.p, dy, dys);
-                if (f->storage == NW_FACTOR_TRANSIENT) {
-                    HIPCHK(hipStreamSynchronize(c->st));
-                    regen += elapsed(c->ev[2], c->ev[3]);
-                }
+        if (f->storage == NW_FACTOR_DEVICE) {
+            // the pairs of one target are contiguous in the plan
+            for (size_t k = 0; k < f->plans.size();) {
+                size_t j = k;
+                while (j < f->plans.size() && f->plans[j].q == f->plans[k].q) ++j;
+                run_range(f, f->plans[k].q, f->plans[k].item0, f->plans[j - 1].item1, f->plans[k].row0,
+                          f->plans[j - 1].row1, dx, lambda, dy, dys);
+                k = j;
+            }
+        } else {
+            for (const PairPlan &pl : f->plans) {
+                HIPCHK(hipEventRecord(c->ev[2], c->st));
+                launch_table(f, pl.q, pl.p, 0, f->d_tab.p);
+                launch_table(f, pl.q, pl.p, 1, f->d_tab.p + table_elems(f, pl.q, pl.p, 0));
+                HIPCHK(hipEventRecord(c->ev[3], c->st));
+                run_range(f, pl.q, pl.item0, pl.item1, pl.row0, pl.row1, dx, lambda, dy, dys);
+                HIPCHK(hipStreamSynchronize(c->st));
+                regen += elapsed(c->ev[2], c->ev[3]);
             }
         }
         HIPCHK(hipGetLastError());
