@@ -117,20 +117,24 @@ __global__ __launch_bounds__(256) void k_fac_table(const PairInfo *__restrict__ 
     tab[(size_t)s * nt + t] = make_double2(ph * vs, ph * vf);
 }
 
-// Stage 1: one item per workgroup (x values and table row indices staged in LDS).
+// Stage 1: one item per workgroup row of the grid (x values and table row
+// indices staged in LDS); blockIdx.x is the 256-wide target column block.
+// The host pads grid.x to a multiple of 8, so a column block always runs on
+// the same XCD (workgroups are dealt round-robin over the 8 XCDs) and the
+// table rows that several groups share are re-read from that XCD's L2.
 __global__ __launch_bounds__(256) void k_fac_mat(const S1Item *__restrict__ items, const uint32_t *__restrict__ midx,
                                                  const uint32_t *__restrict__ mdet, const double *__restrict__ x)
 {
     __shared__ int s_r[S1_CHUNK];
     __shared__ double s_x[S1_CHUNK];
-    const S1Item it = items[blockIdx.x];
+    const S1Item it = items[blockIdx.y];
     const int ne = it.e1 - it.e0;
     if ((int)threadIdx.x < ne) {
         s_r[threadIdx.x] = (int)midx[it.e0 + threadIdx.x];
         s_x[threadIdx.x] = x[mdet[it.e0 + threadIdx.x]];
     }
     __syncthreads();
-    const int t = blockIdx.y * 256 + threadIdx.x;
+    const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= it.nt) return;
     const double2 *col = it.tab + t;
     double s = 0.0, f = 0.0;
@@ -680,10 +684,12 @@ void run_range(nw_factor *f, int q, long long i0, long long i1, long long r0, lo
 {
     nw_ctx *c = f->c;
     const ParentSpace &Q = f->par[q];
-    if (i1 > i0) {
-        const int maxlen = std::max(Q.ncomp[0], Q.ncomp[1]);
-        hipLaunchKernelGGL(k_fac_mat, dim3((unsigned)(i1 - i0), blocks_of(maxlen, 256)), dim3(256), 0, c->st,
-                           f->d_items.p + i0, f->d_midx.p, f->d_mdet.p, dx);
+    const int maxlen = std::max(Q.ncomp[0], Q.ncomp[1]);
+    const unsigned gx = (blocks_of(maxlen, 256) + 7u) & ~7u;
+    for (long long b = i0; b < i1; b += 65535) {
+        const unsigned ni = (unsigned)std::min<long long>(65535, i1 - b);
+        hipLaunchKernelGGL(k_fac_mat, dim3(gx, ni), dim3(256), 0, c->st, f->d_items.p + b, f->d_midx.p, f->d_mdet.p,
+                           dx);
     }
     const int nr = (int)(r1 - r0);
     if (nr == 0) return;
