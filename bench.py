@@ -147,6 +147,12 @@ def pt2_workload(args):
            "dtype": "f64", "data": "synthetic",
            "config": {"workload": "NOCI-PT2 matvec, n_bas=28, nocc=9+9, 6 refs, all singles + 20000 doubles "
                                   "per ref", "candidates": n, "mode": args.mode, "operator": "spin-factorised"},
+           "roofline": {"bound": "hbm", "kernel": "k_fac_mat + k_fac_rows + k_fac_scatter (factorised matvec)",
+                        "achieved": info["table_bytes"] / (float(np.mean(kern)) * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                        "unit": "GB/s",
+                        "frac": info["table_bytes"] / (float(np.mean(kern)) * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                        "traffic": None, "traffic_source": "profiles/factor_pt2_c4_hbm_traffic.txt",
+                        "algorithmic_bytes": info["table_bytes"]},
            "kernel_ms": float(np.mean(kern)), "table_build_s": pt.timings["factor_s"],
            "table_bytes": info["table_bytes"], "pairwise_kernel_ms": float(np.mean(pk)),
            "pairwise_pairs_per_s": pairs / el_p, "factorised_vs_pairwise_max_rel": rel,
