@@ -88,33 +88,46 @@ __device__ inline void spin_fs(const double *slab, const long long *o, const Spi
 }
 
 // Factor table of one ordered parent pair and spin: tab[s][t] = phase * {S, F}
-// with the bra = target when target_left, else the source.
+// with the bra = target when target_left, else the source; 16 x 16 tiles.
+// The table of the reverse pair is the transpose (the same Wick element with
+// the same bra), so when tabT is not null the tile is also written there,
+// tabT[t][s], through LDS.
 __global__ __launch_bounds__(256) void k_fac_table(const PairInfo *__restrict__ pairs, const double *__restrict__ slab0,
                                                    int pair, int spin, const SpinComp *__restrict__ tc, int nt,
                                                    const SpinComp *__restrict__ scomp, int ns, int target_left, int nmo,
-                                                   double tol, int mode, double2 *__restrict__ tab)
+                                                   double tol, int mode, double2 *__restrict__ tab,
+                                                   double2 *__restrict__ tabT)
 {
-    const int t = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int s = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (t >= nt || s >= ns) return;
-    const PairInfo &P = pairs[pair];
-    const double *slab = slab0 + P.base;
-    const SpinScal sc = load_spin(P, spin);
-    const long long *o = P.off + (spin ? O_BB : O_AA);
-    const SpinComp T = tc[t], S = scomp[s];
-    const SpinComp &bra = target_left ? T : S;
-    const SpinComp &ket = target_left ? S : T;
-    double vs = 0.0, vf = 0.0;
-    switch (bra.r + ket.r) {
-        case 0: spin_fs<0>(slab, o, sc, nmo, bra, ket, tol, mode, vs, vf); break;
-        case 1: spin_fs<1>(slab, o, sc, nmo, bra, ket, tol, mode, vs, vf); break;
-        case 2: spin_fs<2>(slab, o, sc, nmo, bra, ket, tol, mode, vs, vf); break;
-        case 3: spin_fs<3>(slab, o, sc, nmo, bra, ket, tol, mode, vs, vf); break;
-        case 4: spin_fs<4>(slab, o, sc, nmo, bra, ket, tol, mode, vs, vf); break;
-        default: break;  // excluded on the host
+    __shared__ double2 sh[16][17];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int t = blockIdx.x * 16 + tx, s = blockIdx.y * 16 + ty;
+    double2 out = make_double2(0.0, 0.0);
+    if (t < nt && s < ns) {
+        const PairInfo &P = pairs[pair];
+        const double *slab = slab0 + P.base;
+        const SpinScal sc = load_spin(P, spin);
+        const long long *o = P.off + (spin ? O_BB : O_AA);
+        const SpinComp T = tc[t], S = scomp[s];
+        const SpinComp &bra = target_left ? T : S;
+        const SpinComp &ket = target_left ? S : T;
+        double vs = 0.0, vf = 0.0;
+        switch (bra.r + ket.r) {
+            case 0: spin_fs<0>(slab, o, sc, nmo, bra, ket, tol, mode, vs, vf); break;
+            case 1: spin_fs<1>(slab, o, sc, nmo, bra, ket, tol, mode, vs, vf); break;
+            case 2: spin_fs<2>(slab, o, sc, nmo, bra, ket, tol, mode, vs, vf); break;
+            case 3: spin_fs<3>(slab, o, sc, nmo, bra, ket, tol, mode, vs, vf); break;
+            case 4: spin_fs<4>(slab, o, sc, nmo, bra, ket, tol, mode, vs, vf); break;
+            default: break;  // excluded on the host
+        }
+        const double ph = (double)(T.ph * S.ph) * sc.pref;
+        out = make_double2(ph * vs, ph * vf);
+        tab[(size_t)s * nt + t] = out;
     }
-    const double ph = (double)(T.ph * S.ph) * sc.pref;
-    tab[(size_t)s * nt + t] = make_double2(ph * vs, ph * vf);
+    if (!tabT) return;  // uniform over the launch
+    sh[ty][tx] = out;
+    __syncthreads();
+    const int t2 = blockIdx.x * 16 + ty, s2 = blockIdx.y * 16 + tx;
+    if (t2 < nt && s2 < ns) tabT[(size_t)t2 * ns + s2] = sh[tx][ty];
 }
 
 // Stage 1: one item per workgroup row of the grid (x values and table row
@@ -410,7 +423,9 @@ size_t table_elems(const nw_factor *f, int q, int p, int sp)
     return (size_t)comps_of(f, p, sp) * (size_t)comps_of(f, q, sp);
 }
 
-void launch_table(nw_factor *f, int q, int p, int sp, double2 *out)
+// Table (q target, p source, spin sp) into out; when outT is not null also
+// its transpose, the table (p target, q source).
+void launch_table(nw_factor *f, int q, int p, int sp, double2 *out, double2 *outT = nullptr)
 {
     nw_ctx *c = f->c;
     const ParentSpace &Q = f->par[q], &P = f->par[p];
@@ -418,20 +433,25 @@ void launch_table(nw_factor *f, int q, int p, int sp, double2 *out)
     const int pair = target_left ? q * c->nref + p : p * c->nref + q;
     const int nt = Q.ncomp[sp], ns = P.ncomp[sp];
     if (nt == 0 || ns == 0) return;
-    dim3 grid(blocks_of(nt, 64), blocks_of(ns, 4));
+    dim3 grid(blocks_of(nt, 16), blocks_of(ns, 16));
     hipLaunchKernelGGL(k_fac_table, grid, dim3(256), 0, c->st, c->d_pinfo, c->d_slab, pair, sp,
                        f->d_comp[sp].p + Q.comp_off[sp], nt, f->d_comp[sp].p + P.comp_off[sp], ns,
-                       target_left ? 1 : 0, c->nmo, c->tol, f->mode, out);
+                       target_left ? 1 : 0, c->nmo, c->tol, f->mode, out, outT);
 }
 
 void build_tables(nw_factor *f)
 {
     const int nref = f->nref;
+    // each unordered pair once, with the earlier parent as target (bra); the
+    // reverse pair's table is written as the transpose
     for (int q = 0; q < nref; ++q)
         for (int p = 0; p < nref; ++p)
             for (int sp = 0; sp < 2; ++sp) {
                 const long long off = f->tab_off[(q * nref + p) * 2 + sp];
-                if (off >= 0) launch_table(f, q, p, sp, f->d_tab.p + off);
+                if (off < 0) continue;
+                if (q == p) launch_table(f, q, p, sp, f->d_tab.p + off);
+                else if (f->par[q].first_det < f->par[p].first_det)
+                    launch_table(f, q, p, sp, f->d_tab.p + off, f->d_tab.p + f->tab_off[(p * nref + q) * 2 + sp]);
             }
     HIPCHK(hipGetLastError());
     f->build_gen = f->c->build_gen;
