@@ -24,6 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "noci-rs_amd"))
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
 
@@ -454,12 +455,21 @@ def main():
         return {"elapsed": elapsed, "meta": meta, "builds": builds, "transforms": transforms, "evals": evals,
                 "energy": energy, "rate": npairs * args.steps / elapsed}
 
-    PREC_NAME = {nwick.PREC_FP64: "fp64", nwick.PREC_MIXED: "mixed"}
+    PREC_NAME = {nwick.PREC_FP64: "fp64", nwick.PREC_MIXED: "mixed", nwick.PREC_BF16X3: "bf16x3"}
     res64 = phase(nwick.PREC_FP64)
-    resmx = phase(nwick.PREC_MIXED) if args.precision == "mixed" else None
-    dE = None if resmx is None or args.no_energy else resmx["energy"] - res64["energy"]
-    use_mixed = resmx is not None and (args.no_energy or abs(dE) < 1e-6)
-    res = resmx if use_mixed else res64
+    mixed_runs = {}
+    if args.precision == "mixed":
+        for prec in (nwick.PREC_MIXED, nwick.PREC_BF16X3):
+            r = phase(prec)
+            r["dE"] = None if args.no_energy else r["energy"] - res64["energy"]
+            mixed_runs[prec] = r
+    # report the faster mixed form whose energy is within 1 uHa of FP64
+    ok = [p for p, r in mixed_runs.items() if args.no_energy or abs(r["dE"]) < 1e-6]
+    best = max(ok, key=lambda p: mixed_runs[p]["rate"]) if ok else None
+    resmx = mixed_runs.get(nwick.PREC_MIXED)
+    dE = None if resmx is None else resmx["dE"]
+    use_mixed = best is not None
+    res = mixed_runs[best] if use_mixed else res64
     meta = res["meta"]
     ms_per_step = 1e3 * res["elapsed"] / args.steps
     value = res["rate"]
@@ -491,8 +501,13 @@ def main():
     achieved = nk * flops_tensor / (t_k * 1e-3) / 1e12
     avg_ms = t_k / dispatches
     eval_rate = npairs / (float(np.mean(evals)) * 1e-3)
-    kname = "k_qtransform_f32_pipe" if use_mixed else "k_qtransform_mfma_pipe"
-    peak = PEAK_FP32_TFLOPS if use_mixed else PEAK_FP64_TFLOPS
+    if use_mixed and best == nwick.PREC_BF16X3:
+        # six BF16 products per FP32-equivalent product: the emulated peak is dense BF16 MFMA / 6
+        kname, peak = "k_qtransform_bf3", PEAK_BF16_TFLOPS / 6.0
+    elif use_mixed:
+        kname, peak = "k_qtransform_f32_pipe", PEAK_FP32_TFLOPS
+    else:
+        kname, peak = "k_qtransform_mfma_pipe", PEAK_FP64_TFLOPS
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"{'qtransform_f32' if use_mixed else 'qtransform'}_n100_hbm_traffic.txt")
     if os.path.exists(tfile):
@@ -515,19 +530,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if use_mixed else "f64",
+        "dtype": ("bf16x3+f64" if best == nwick.PREC_BF16X3 else "f32+f64") if use_mixed else "f64",
         "data": "synthetic",
         "config": {"workload": "batched Wick elements, n_bas=nmo=100, nocc=10+10, 8 refs (1 with an exactly "
                                "orthogonal alpha orbital), 512 singles/doubles per ref, 4096 dets, 8390656 pairs",
                    "mode": "reference" if args.mode == 0 else "corrected", "pairs_built": len(meta),
-                   "precision": "mixed" if use_mixed else "fp64",
+                   "precision": PREC_NAME[best] if use_mixed else "fp64",
                    "rank4_tensors": ntensor, "fp32_tensors": n32},
         "fp64_pairs_per_s": res64["rate"],
         "mixed_pairs_per_s": None if resmx is None else resmx["rate"],
         "mixed_over_fp64": None if resmx is None else resmx["rate"] / res64["rate"],
+        "bf16x3_pairs_per_s": mixed_runs[nwick.PREC_BF16X3]["rate"] if nwick.PREC_BF16X3 in mixed_runs else None,
+        "bf16x3_over_fp64": (mixed_runs[nwick.PREC_BF16X3]["rate"] / res64["rate"]
+                             if nwick.PREC_BF16X3 in mixed_runs else None),
         "energy_fp64": res64["energy"],
         "energy_mixed": None if resmx is None else resmx["energy"],
         "dE_mixed_minus_fp64": dE,
+        "dE_bf16x3_minus_fp64": mixed_runs[nwick.PREC_BF16X3]["dE"] if nwick.PREC_BF16X3 in mixed_runs else None,
+        "value_over_fp64": value / res64["rate"],
         "eval_only_pairs_per_s": eval_rate,
         "build_ms": float(np.mean(builds)),
         "transform_ms": float(np.mean(transforms)),

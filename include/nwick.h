@@ -51,8 +51,10 @@ enum nw_mode { NW_MODE_REFERENCE = 0, NW_MODE_CORRECTED = 1 };
  * producing the same-spin J and different-spin II tensors run with FP32
  * operands, FP32 intermediates and FP32 MFMA accumulation; the SVD, zero
  * classification, co-densities, one-body and one-column intermediates and
- * every evaluation stay FP64. */
-enum nw_precision { NW_PREC_FP64 = 0, NW_PREC_MIXED = 1 };
+ * every evaluation stay FP64.  NW_PREC_BF16X3: as NW_PREC_MIXED, with the
+ * transforms on BF16 MFMA: every FP32 operand split into three BF16 parts,
+ * six cross products accumulated in FP32 (FP32-level error). */
+enum nw_precision { NW_PREC_FP64 = 0, NW_PREC_MIXED = 1, NW_PREC_BF16X3 = 2 };
 
 /* One determinant: the fields of DetState (src/scalar.rs:32-61) the Wick path
  * reads -- parent reference, excitation phases, occupations and the

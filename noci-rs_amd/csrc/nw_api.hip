@@ -397,7 +397,8 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         HIPCHK(hipGetLastError());
         // ---- four-index transforms (build.rs:261-313, 877-917) ----------------
         HIPCHK(hipEventRecord(c->ev[1], c->st));
-        const bool mixed = c->precision == NW_PREC_MIXED && n % 2 == 0;
+        const bool mixed = c->precision != NW_PREC_FP64 && n % 2 == 0;
+        const bool bf3 = c->precision == NW_PREC_BF16X3;
         // One job per rank-4 tensor: the J slots per spin and the II blocks.
         // J slab block [a,b,c,d] = blk[c,d,a,b] with blk = (cx_i xc_j || cx_k xc_l);
         // the pair symmetry of (ab||cd) lets one transform produce it directly.
@@ -474,10 +475,12 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                         OUT[k] = jb.out;
                     }
                     std::vector<const float *> TAc(TA.begin(), TA.end()), TBc(TB.begin(), TB.end());
-                    bool ok = launch_qtransform_f32_many(c->st, X.data(), C4.data(), TA.data(), nb, (long long)n * n * n, n, nm) &&
-                              launch_qtransform_f32_many(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm) &&
-                              launch_qtransform_f32_many(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm) &&
-                              launch_qtransform_f32_out64_many(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
+                    auto pass = bf3 ? launch_qtransform_bf3_many : launch_qtransform_f32_many;
+                    auto last = bf3 ? launch_qtransform_bf3_out64_many : launch_qtransform_f32_out64_many;
+                    bool ok = pass(c->st, X.data(), C4.data(), TA.data(), nb, (long long)n * n * n, n, nm) &&
+                              pass(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm) &&
+                              pass(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm) &&
+                              last(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
                     if (!ok) throw NwError(NW_E_SHAPE, "mixed-precision transform needs an even n_bas and nmo <= 112");
                 } else {
                     std::vector<const double *> X(nb), C1(nb), C2(nb), C3(nb), C4(nb);
@@ -992,7 +995,8 @@ int nw_set_mixed_floor(nw_ctx *c, double sigma_floor)
 int nw_set_precision(nw_ctx *c, int precision)
 {
     return guarded(c, [&]() -> int {
-        if (precision != NW_PREC_FP64 && precision != NW_PREC_MIXED) return fail(c, NW_E_ARG, "unknown precision");
+        if (precision != NW_PREC_FP64 && precision != NW_PREC_MIXED && precision != NW_PREC_BF16X3)
+            return fail(c, NW_E_ARG, "unknown precision");
         c->precision = precision;
         c->built = false;
         return NW_OK;

@@ -4,7 +4,9 @@
 // The algorithm follows src/nonorthogonalwicks/build.rs; the decomposition
 // into GPU launches is this engine's own: every step runs batched over all
 // requested ordered reference pairs and both spins at once.
+#include <mutex>
 #include <type_traits>
+#include <unordered_map>
 
 #include "nw_kernels.h"
 
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(512) void k_qtransform_mfma(const QPtrs q, long lon
 // multiplied out of LDS, so HBM latency hides behind the MFMA work.  Rows of
 // X are nl long; the LDS tile is padded to NL = 4 KS columns with zeros so
 // the K = 4 MFMA steps need no guards.
-template <int KS, bool PAD2>
+template <int KS, bool PAD2, int TRD>
 __global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, long long rows, int nout, long long ntiles)
 {
     constexpr int nl = KS * 4 - (PAD2 ? 2 : 0);  // row length of X (even)
@@ -189,16 +191,16 @@ __global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, lon
     constexpr int NL = KS * 4;
     constexpr int LDX = NL + 1;
     constexpr int NT = 448;
-    constexpr int PER = (64 * NL / 2 + NT - 1) / NT;  // double2 per thread (upper bound)
+    constexpr int PER = (TRD * NL / 2 + NT - 1) / NT;  // double2 per thread (upper bound)
     extern __shared__ double xs[];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lk = lane >> 4;
     const int lp = wave * 16 + lr;
-    constexpr int tile2 = 64 * nl / 2;
+    constexpr int tile2 = TRD * nl / 2;
     constexpr int npad = NL - nl;
     if constexpr (npad > 0)
-        for (int e = tid; e < 64 * npad; e += NT) xs[(e / npad) * LDX + nl + e % npad] = 0.0;
+        for (int e = tid; e < TRD * npad; e += NT) xs[(e / npad) * LDX + nl + e % npad] = 0.0;
     double afr[KS];
 #pragma unroll
     for (int kk = 0; kk < KS; ++kk) {
@@ -208,9 +210,9 @@ __global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, lon
     const bool active = wave * 16 < nout;
     double2 buf[PER];
     auto gload = [&](long long t) {
-        const double2 *src = reinterpret_cast<const double2 *>(X + (size_t)t * 64 * nl);
-        const long long rem = rows - t * 64;
-        const int lim2 = (int)((rem < 64 ? rem : 64) * nl / 2);
+        const double2 *src = reinterpret_cast<const double2 *>(X + (size_t)t * TRD * nl);
+        const long long rem = rows - t * TRD;
+        const int lim2 = (int)((rem < TRD ? rem : TRD) * nl / 2);
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const int e2 = tid + j * NT;
@@ -234,22 +236,22 @@ __global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, lon
         const long long tn = t + gridDim.x;
         if (tn < ntiles) gload(tn);
         if (!active) continue;
-        const long long m0 = t * 64;
-        const int mcount = (int)((rows - m0) < 64 ? (rows - m0) : 64);
-        nw_d4 acc[4];
+        const long long m0 = t * TRD;
+        const int mcount = (int)((rows - m0) < TRD ? (rows - m0) : TRD);
+        nw_d4 acc[TRD / 16];
 #pragma unroll
-        for (int ns = 0; ns < 4; ++ns) acc[ns] = nw_d4{0.0, 0.0, 0.0, 0.0};
+        for (int ns = 0; ns < TRD / 16; ++ns) acc[ns] = nw_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kk = 0; kk < KS; ++kk) {
             const double a = afr[kk];
 #pragma unroll
-            for (int ns = 0; ns < 4; ++ns) {
+            for (int ns = 0; ns < TRD / 16; ++ns) {
                 const double b = xs[(ns * 16 + lr) * LDX + kk * 4 + lk];
                 acc[ns] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[ns], 0, 0, 0);
             }
         }
 #pragma unroll
-        for (int ns = 0; ns < 4; ++ns) {
+        for (int ns = 0; ns < TRD / 16; ++ns) {
             const int mm = ns * 16 + lr;
             if (mm >= mcount) continue;
 #pragma unroll
@@ -267,7 +269,15 @@ __global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, lon
 // 128-row tiles of X, staged in LDS, next tile prefetched into registers.
 typedef float nw_f4 __attribute__((ext_vector_type(4)));
 
-template <int KS, typename OutT, int VEC>
+// Rows of X per LDS tile of the pipelined FP64 / FP32 transforms.
+#ifndef NW_TR_F64
+#define NW_TR_F64 32
+#endif
+#ifndef NW_TR_F32
+#define NW_TR_F32 64
+#endif
+
+template <int KS, typename OutT, int VEC, int TR>
 __global__ __launch_bounds__(448) void k_qtransform_f32_pipe(const QPtrs q, long long rows, int nout, long long ntiles)
 {
     constexpr int nl = KS * 4 - (VEC == 4 ? 0 : 2);  // row length of X: float4 rows when nl % 4 == 0
@@ -278,7 +288,6 @@ __global__ __launch_bounds__(448) void k_qtransform_f32_pipe(const QPtrs q, long
     constexpr int NL = KS * 4;
     constexpr int LDX = NL + 1;
     constexpr int NT = 448;
-    constexpr int TR = 128;                                // rows of X per tile
     constexpr int PER = (TR * NL / VEC + NT - 1) / NT;     // vectors per thread (upper bound)
     extern __shared__ float xsf[];
     const int tid = threadIdx.x;
@@ -370,6 +379,27 @@ void launch_to_f32(hipStream_t st, const double *in, float *out, long long n)
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)g), dim3(256), 0, st, in, out, n);
 }
 
+// Workgroups of one kernel instance that are resident on the whole device at
+// once (occupancy x CU count).  A persistent grid larger than this leaves a
+// second round of workgroups that starts only as the first drains.
+template <typename F>
+static long long resident_grid(F kern, int block, size_t lds)
+{
+    static std::mutex mu;
+    static std::unordered_map<const void *, long long> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find((const void *)kern);
+    if (it != cache.end()) return it->second;
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1) ncu = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, block, lds) != hipSuccess || per < 1) per = 1;
+    (void)hipGetLastError();
+    const long long g = (long long)per * ncu;
+    cache[(const void *)kern] = g;
+    return g;
+}
+
 // Number of tensors of one batched launch and the persistent grid per tensor.
 static unsigned grid_x(long long ntiles, int count, long long target)
 {
@@ -386,18 +416,22 @@ static bool launch_qtransform_f32_t(hipStream_t st, const float *const *X, const
     const int ks = (nl + 3) / 4;
     if (nl % 2 != 0 || ks > 31 || (nout + 15) / 16 > 7) return false;
     const bool v4 = nl % 4 == 0;
-    const long long ntiles = (rows + 127) / 128;
-    const size_t lds = (size_t)128 * (ks * 4 + 1) * sizeof(float);
+    constexpr int TR = NW_TR_F32;
+    const long long ntiles = (rows + TR - 1) / TR;
+    const size_t lds = (size_t)TR * (ks * 4 + 1) * sizeof(float);
     for (int b0 = 0; b0 < count; b0 += QMAX) {
         const int nb = count - b0 < QMAX ? count - b0 : QMAX;
         QPtrs q{};
         for (int k = 0; k < nb; ++k) { q.X[k] = X[b0 + k]; q.c[k] = c[b0 + k]; q.Y[k] = Y[b0 + k]; }
-        dim3 g(grid_x(ntiles, nb, 1024), (unsigned)nb), b(448);
+        const dim3 b(448);
 #define NW_QF(K)                                                                                               \
-    case K:                                                                                                    \
-        if (v4) hipLaunchKernelGGL((k_qtransform_f32_pipe<K, OutT, 4>), g, b, lds, st, q, rows, nout, ntiles); \
-        else hipLaunchKernelGGL((k_qtransform_f32_pipe<K, OutT, 2>), g, b, lds, st, q, rows, nout, ntiles);    \
-        break;
+    case K: {                                                                                                  \
+        void (*kern)(const QPtrs, long long, int, long long) =                                                 \
+            v4 ? k_qtransform_f32_pipe<K, OutT, 4, TR> : k_qtransform_f32_pipe<K, OutT, 2, TR>;                \
+        const dim3 g(grid_x(ntiles, nb, resident_grid(kern, 448, lds)), (unsigned)nb);                        \
+        hipLaunchKernelGGL(kern, g, b, lds, st, q, rows, nout, ntiles);                                        \
+        break;                                                                                                 \
+    }
         switch (ks) {
             NW_QF(1) NW_QF(2) NW_QF(3) NW_QF(4) NW_QF(5) NW_QF(6) NW_QF(7) NW_QF(8) NW_QF(9) NW_QF(10)
             NW_QF(11) NW_QF(12) NW_QF(13) NW_QF(14) NW_QF(15) NW_QF(16) NW_QF(17) NW_QF(18) NW_QF(19)
@@ -433,26 +467,222 @@ bool launch_qtransform_f32_out64(hipStream_t st, const float *X, const float *c,
     return launch_qtransform_f32_out64_many(st, &X, &c, &Y, 1, rows, nl, nout);
 }
 
+// ---------------------------------------------------------------------------
+// The same quarter transform with BF16 MFMA on split operands ("bf16x3"):
+// every FP32 value v is written as h + m + l with h = bf16(v),
+// m = bf16(v - h), l = bf16(v - h - m) (round to nearest even), which keeps
+// 24 significant bits, and a product as the six terms whose weight is at
+// least 2^-16 of h_a h_b:
+//   ab ~ h_a h_b + h_a m_b + m_a h_b + h_a l_b + l_a h_b + m_a m_b,
+// each an exact BF16 x BF16 product accumulated in FP32 by
+// v_mfma_f32_16x16x32_bf16 (K = 32 per instruction; a lane holds 8
+// consecutive K values of one row of A and one column of B, the same K
+// assignment for both).  The dropped terms (m l, l m, l l) are below
+// 2^-24 relative, so the result has FP32-level error at 6/16 of the cost of
+// FP32 MFMA per product.  X tiles of 64 rows are split once when staged in
+// LDS (three bf16 planes, K padded to a multiple of 32 with zeros); the
+// coefficient fragments are split once per wave.
+typedef __bf16 nw_bf8 __attribute__((ext_vector_type(8)));
+
+__device__ inline void nw_split3(float v, __bf16 &h, __bf16 &m, __bf16 &l)
+{
+    h = (__bf16)v;
+    const float r = v - (float)h;
+    m = (__bf16)r;
+    l = (__bf16)(r - (float)m);
+}
+
+template <int KS, typename OutT, int VEC>
+__global__ __launch_bounds__(448) void k_qtransform_bf3(const QPtrs q, long long rows, int nl, int nout, long long ntiles)
+{
+    constexpr int NT = 448;
+    constexpr int TR = 64;            // rows of X per tile
+    constexpr int KP = KS * 32;       // padded K
+    constexpr int LDB = KP + 8;       // bf16 per LDS row (16 B pad)
+    constexpr int PLANE = TR * LDB;
+    const float *__restrict__ X = (const float *)q.X[blockIdx.y];
+    const float *__restrict__ c = (const float *)q.c[blockIdx.y];
+    OutT *__restrict__ Y = (OutT *)q.Y[blockIdx.y];
+    using VT = typename std::conditional<VEC == 4, float4, float2>::type;
+    extern __shared__ __bf16 xb[];  // planes h, m, l
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int lp = wave * 16 + lr;
+    // zero the K padding of every plane once; tiles only write k < nl
+    for (int e = tid; e < 3 * TR * (KP - nl); e += NT) {
+        const int pl = e / (TR * (KP - nl)), rem = e % (TR * (KP - nl));
+        xb[pl * PLANE + (rem / (KP - nl)) * LDB + nl + rem % (KP - nl)] = (__bf16)0.0f;
+    }
+    // coefficient fragments: row lp of A = c[:, lp], K = 32 ks + 8 lk + j
+    nw_bf8 ah[KS], am[KS], al[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int l = ks * 32 + lk * 8 + j;
+            const float v = (lp < nout && l < nl) ? c[(size_t)l * nout + lp] : 0.0f;
+            __bf16 h, m, lo;
+            nw_split3(v, h, m, lo);
+            ah[ks][j] = h;
+            am[ks][j] = m;
+            al[ks][j] = lo;
+        }
+    // this thread's vectors of a tile: element e = VEC (tid + j NT) -> row r, column k
+    const int tilev = TR * nl / VEC;
+    constexpr int PER = (TR * KP / VEC + NT - 1) / NT;  // upper bound (nl <= KP)
+    const int step = VEC * NT, dr = step / nl, dk = step % nl;
+    const int r0 = (VEC * tid) / nl, k0 = (VEC * tid) % nl;
+    VT buf[PER];
+    auto gload = [&](long long t) {
+        const VT *src = reinterpret_cast<const VT *>(X + (size_t)t * TR * nl);
+        const long long rem = rows - t * TR;
+        const int limv = (int)((rem < TR ? rem : TR) * nl / VEC);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int ev = tid + j * NT;
+            buf[j] = (ev < limv) ? src[ev] : VT{};
+        }
+    };
+    const bool active = wave * 16 < nout;
+    long long t = blockIdx.x;
+    if (t < ntiles) gload(t);
+    for (; t < ntiles; t += gridDim.x) {
+        __syncthreads();
+        {
+            int r = r0, k = k0;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int ev = tid + j * NT;
+                if (ev < tilev) {
+                    __bf16 h[VEC], m[VEC], lo[VEC];
+                    nw_split3(buf[j].x, h[0], m[0], lo[0]);
+                    nw_split3(buf[j].y, h[1], m[1], lo[1]);
+                    if constexpr (VEC == 4) {
+                        nw_split3(buf[j].z, h[2], m[2], lo[2]);
+                        nw_split3(buf[j].w, h[3], m[3], lo[3]);
+                    }
+                    __bf16 *d = xb + r * LDB + k;
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) {
+                        d[v] = h[v];
+                        d[PLANE + v] = m[v];
+                        d[2 * PLANE + v] = lo[v];
+                    }
+                }
+                r += dr;
+                k += dk;
+                if (k >= nl) {
+                    k -= nl;
+                    ++r;
+                }
+            }
+        }
+        __syncthreads();
+        const long long tn = t + gridDim.x;
+        if (tn < ntiles) gload(tn);
+        if (!active) continue;
+        const long long m0 = t * TR;
+        const int mcount = (int)((rows - m0) < TR ? (rows - m0) : TR);
+        nw_f4 acc[TR / 16];
+#pragma unroll
+        for (int ns = 0; ns < TR / 16; ++ns) acc[ns] = nw_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (int ns = 0; ns < TR / 16; ++ns) {
+                const __bf16 *bp = xb + (ns * 16 + lr) * LDB + ks * 32 + lk * 8;
+                const nw_bf8 bh = *reinterpret_cast<const nw_bf8 *>(bp);
+                const nw_bf8 bm = *reinterpret_cast<const nw_bf8 *>(bp + PLANE);
+                const nw_bf8 bl = *reinterpret_cast<const nw_bf8 *>(bp + 2 * PLANE);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ks], bh, acc[ns], 0, 0, 0);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bl, acc[ns], 0, 0, 0);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[ks], bm, acc[ns], 0, 0, 0);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[ks], bh, acc[ns], 0, 0, 0);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bm, acc[ns], 0, 0, 0);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bh, acc[ns], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int ns = 0; ns < TR / 16; ++ns) {
+            const int mm = ns * 16 + lr;
+            if (mm >= mcount) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = wave * 16 + NW_F32_ROW(lk, r);
+                if (row < nout) Y[(size_t)row * rows + m0 + mm] = (OutT)acc[ns][r];
+            }
+        }
+    }
+}
+
+template <typename OutT>
+static bool launch_qtransform_bf3_t(hipStream_t st, const float *const *X, const float *const *c, OutT *const *Y,
+                                    int count, long long rows, int nl, int nout)
+{
+    const int ks = (nl + 31) / 32;
+    if (nl % 2 != 0 || ks > 8 || (nout + 15) / 16 > 7) return false;
+    const bool v4 = nl % 4 == 0;
+    const long long ntiles = (rows + 63) / 64;
+    const size_t lds = (size_t)3 * 64 * (ks * 32 + 8) * sizeof(__bf16);
+    const dim3 b(448);
+    for (int b0 = 0; b0 < count; b0 += QMAX) {
+        const int nb = count - b0 < QMAX ? count - b0 : QMAX;
+        QPtrs q{};
+        for (int k = 0; k < nb; ++k) { q.X[k] = X[b0 + k]; q.c[k] = c[b0 + k]; q.Y[k] = Y[b0 + k]; }
+#define NW_QB(K)                                                                                      \
+    case K: {                                                                                         \
+        void (*kern)(const QPtrs, long long, int, int, long long) =                                   \
+            v4 ? k_qtransform_bf3<K, OutT, 4> : k_qtransform_bf3<K, OutT, 2>;                         \
+        const dim3 g(grid_x(ntiles, nb, resident_grid(kern, 448, lds)), (unsigned)nb);                \
+        hipLaunchKernelGGL(kern, g, b, lds, st, q, rows, nl, nout, ntiles);                           \
+        break;                                                                                        \
+    }
+        switch (ks) {
+            NW_QB(1) NW_QB(2) NW_QB(3) NW_QB(4) NW_QB(5) NW_QB(6) NW_QB(7) NW_QB(8)
+        default: return false;
+        }
+#undef NW_QB
+    }
+    return true;
+}
+
+bool launch_qtransform_bf3_many(hipStream_t st, const float *const *X, const float *const *c, float *const *Y,
+                                int count, long long rows, int nl, int nout)
+{
+    return launch_qtransform_bf3_t<float>(st, X, c, Y, count, rows, nl, nout);
+}
+
+bool launch_qtransform_bf3_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
+                                      int count, long long rows, int nl, int nout)
+{
+    return launch_qtransform_bf3_t<double>(st, X, c, Y, count, rows, nl, nout);
+}
+
 void launch_qtransform_many(hipStream_t st, const double *const *X, const double *const *c, double *const *Y, int count,
                             long long rows, int nl, int nout)
 {
     const int ks = (nl + 3) / 4;
     const int nw = (nout + 15) / 16;
-    const long long ntiles = (rows + 63) / 64;
+    constexpr int TRD = NW_TR_F64;
+    const long long ntiles = (rows + TRD - 1) / TRD;
     for (int b0 = 0; b0 < count; b0 += QMAX) {
         const int nb = count - b0 < QMAX ? count - b0 : QMAX;
         QPtrs q{};
         for (int k = 0; k < nb; ++k) { q.X[k] = X[b0 + k]; q.c[k] = c[b0 + k]; q.Y[k] = Y[b0 + k]; }
         bool done = false;
         if (nl % 2 == 0 && ks <= 31 && nw <= 7) {
-            const size_t lds = (size_t)64 * (ks * 4 + 1) * sizeof(double);
-            dim3 g(grid_x(ntiles, nb, 512), (unsigned)nb), b(448);
+            const size_t lds = (size_t)TRD * (ks * 4 + 1) * sizeof(double);
+            const dim3 b(448);
             done = true;
 #define NW_QP(K)                                                                                        \
-    case K:                                                                                             \
-        if (nl % 4) hipLaunchKernelGGL((k_qtransform_mfma_pipe<K, true>), g, b, lds, st, q, rows, nout, ntiles); \
-        else hipLaunchKernelGGL((k_qtransform_mfma_pipe<K, false>), g, b, lds, st, q, rows, nout, ntiles);      \
-        break;
+    case K: {                                                                                           \
+        void (*kern)(const QPtrs, long long, int, long long) =                                          \
+            (nl % 4) ? k_qtransform_mfma_pipe<K, true, TRD> : k_qtransform_mfma_pipe<K, false, TRD>;    \
+        const dim3 g(grid_x(ntiles, nb, resident_grid(kern, 448, lds)), (unsigned)nb);                  \
+        hipLaunchKernelGGL(kern, g, b, lds, st, q, rows, nout, ntiles);                                 \
+        break;                                                                                          \
+    }
             switch (ks) {
                 NW_QP(1) NW_QP(2) NW_QP(3) NW_QP(4) NW_QP(5) NW_QP(6) NW_QP(7) NW_QP(8) NW_QP(9) NW_QP(10)
                 NW_QP(11) NW_QP(12) NW_QP(13) NW_QP(14) NW_QP(15) NW_QP(16) NW_QP(17) NW_QP(18) NW_QP(19)
@@ -464,11 +694,12 @@ void launch_qtransform_many(hipStream_t st, const double *const *X, const double
         }
         if (!done && ks <= 31 && nw <= 8) {  // LDS tile <= 64 KB
             const size_t lds = (size_t)64 * (ks * 4 + 1) * sizeof(double);
-            dim3 g(grid_x(ntiles, nb, 1024), (unsigned)nb), b(64 * nw);
+            const long long ntiles64 = (rows + 63) / 64;
+            dim3 g(grid_x(ntiles64, nb, 1024), (unsigned)nb), b(64 * nw);
             done = true;
 #define NW_QT(K)                                                                                        \
     case K:                                                                                             \
-        hipLaunchKernelGGL((k_qtransform_mfma<K>), g, b, lds, st, q, rows, nl, nout, ntiles);          \
+        hipLaunchKernelGGL((k_qtransform_mfma<K>), g, b, lds, st, q, rows, nl, nout, ntiles64);        \
         break;
             switch (ks) {
                 NW_QT(1) NW_QT(2) NW_QT(3) NW_QT(4) NW_QT(5) NW_QT(6) NW_QT(7) NW_QT(8) NW_QT(9) NW_QT(10)

@@ -41,6 +41,11 @@ bool launch_qtransform_f32_many(hipStream_t st, const float *const *X, const flo
 bool launch_qtransform_f32_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
                                       int count, long long rows, int nl, int nout);
 // FP32 variants (mixed-precision build); return false when the shape is unsupported.
+// The same with BF16 MFMA on three-way split operands (FP32-level accuracy).
+bool launch_qtransform_bf3_many(hipStream_t st, const float *const *X, const float *const *c, float *const *Y,
+                                int count, long long rows, int nl, int nout);
+bool launch_qtransform_bf3_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
+                                      int count, long long rows, int nl, int nout);
 bool launch_qtransform_f32(hipStream_t st, const float *X, const float *c, float *Y, long long rows, int nl, int nout);
 bool launch_qtransform_f32_out64(hipStream_t st, const float *X, const float *c, double *Y, long long rows, int nl,
                                  int nout);

@@ -39,6 +39,7 @@ EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matri
            "nw_factor_info", "nw_factor_ids", "nw_factor_table", "nw_factor_last_ms", "nw_factor_destroy"]
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_BF16X3 = 2
 BATCH_TUNE = 1
 BATCH_GRAPH = 2
 GROUP_CODE, GROUP_CLASS, GROUP_ALL = 0, 1, 2

@@ -113,15 +113,17 @@ def test_pair_mask_and_unbuilt_pair_error():
         eng.hs_batched(rec, np.array([[len(dets) - 1, 0]], dtype=np.uint64))
 
 
-def test_mixed_precision_build_close_to_fp64():
-    """Mixed build (FP32 MFMA transforms): elements within 1e-6 of FP64 and the
-    lowest NOCI energy within 1 uHa; FP64 quantities (S, m) unchanged."""
+@pytest.mark.parametrize("prec", [nwick.PREC_MIXED, nwick.PREC_BF16X3])
+def test_mixed_precision_build_close_to_fp64(prec):
+    """Mixed build (FP32 MFMA, or BF16 MFMA on three-way split operands):
+    elements within 1e-6 of FP64 and the lowest NOCI energy within 1 uHa;
+    FP64 quantities (S, m) unchanged."""
     sysm, dets = _case(16, 3, 3, 3, seed=51, orth_refs=(2,), orth_count=1, orthonormal=False, per_ref=40)
     rec = basis.to_records(dets)
     eng = nwick.Engine.from_system(sysm)
     eng.build()
     H64, S64 = eng.hs_matrix(rec, mode=1)
-    eng.set_precision(nwick.PREC_MIXED)
+    eng.set_precision(prec)
     meta = eng.build()
     assert meta[(0, 2)].ma == 1
     Hmx, Smx = eng.hs_matrix(rec, mode=1)
@@ -130,7 +132,7 @@ def test_mixed_precision_build_close_to_fp64():
     import scipy.linalg
     e64 = scipy.linalg.eigh(H64, S64, eigvals_only=True)[0]
     emx = scipy.linalg.eigh(Hmx, Smx, eigvals_only=True)[0]
-    print(f"mixed: max|dH|={np.max(np.abs(Hmx - H64)):.3e} dE={emx - e64:.3e}", flush=True)
+    print(f"precision {prec}: max|dH|={np.max(np.abs(Hmx - H64)):.3e} dE={emx - e64:.3e}", flush=True)
     assert abs(emx - e64) < 1e-6
 
 
@@ -187,11 +189,16 @@ def test_transform_shapes_odd_and_two_mod_four(n):
         eng = nwick.Engine.from_system(sysm)
         eng.build()
         H, S = eng.hs_matrix(rec, mode=1)
-        eng.set_precision(nwick.PREC_MIXED)
         eng.set_mixed_floor(0.0)
-        meta = eng.build()
-        assert all(m.fp32 == 1 for m in meta.values())
-        Hm, Sm = eng.hs_matrix(rec, mode=1)
-        print(f"n={n} mixed: max|dH|={np.max(np.abs(Hm - H)):.3e}", flush=True)
-        assert np.array_equal(Sm, S)
-        assert np.max(np.abs(Hm - H)) < 1e-5 * max(1.0, np.max(np.abs(H)))
+        err = {}
+        for prec in (nwick.PREC_MIXED, nwick.PREC_BF16X3):
+            eng.set_precision(prec)
+            meta = eng.build()
+            assert all(m.fp32 == 1 for m in meta.values())
+            Hm, Sm = eng.hs_matrix(rec, mode=1)
+            err[prec] = float(np.max(np.abs(Hm - H)))
+            print(f"n={n} precision {prec}: max|dH|={err[prec]:.3e}", flush=True)
+            assert np.array_equal(Sm, S)
+            assert err[prec] < 1e-5 * max(1.0, np.max(np.abs(H)))
+        # the split BF16 products carry FP32-level error
+        assert err[nwick.PREC_BF16X3] < 8.0 * err[nwick.PREC_MIXED] + 1e-12
