@@ -110,14 +110,22 @@ __global__ __launch_bounds__(NH) void k_mlp_fwd(const float *__restrict__ X, con
 }
 
 // Backward of the weighted mean squared error L = sum_r w_r (y_r - t_r)^2 / (2 n)
-// for the rows of one forward pass (H1, H2, y stored by k_mlp_fwd).  Gradients
-// are accumulated into g (FP32, zeroed beforehand) with atomics; loss into *loss.
+// for the rows of one forward pass (H1, H2, y stored by k_mlp_fwd).  Each
+// workgroup sums the gradients of its tiles in a fixed order into its own
+// partial vector (gpart + blockIdx.x * stride; the loss in the last slot);
+// every parameter belongs to one thread, so no atomics are needed and
+// k_mlp_reduce adds the partials in workgroup order: the result is the same
+// bit for bit on every run.
 __global__ __launch_bounds__(NH) void k_mlp_bwd(const float *__restrict__ X, const int *__restrict__ idx, long long n,
                                                  double inv_n, Net net, const float *__restrict__ H1,
                                                  const float *__restrict__ H2, const float *__restrict__ y,
                                                  const float *__restrict__ tgt, const float *__restrict__ wt,
-                                                 float *__restrict__ g, float *__restrict__ loss)
+                                                 float *__restrict__ gpart, long long stride)
 {
+    float *g = gpart + (size_t)blockIdx.x * stride;
+    float *loss = g + stride - 1;
+    bool first = true;  // the block's first tile stores, later tiles add
+    auto acc = [&](float *p, float v) { *p = first ? v : *p + v; };
     __shared__ float xs[TSB][MAXIN + 1];
     __shared__ float h1s[TSB][NH + 1];
     __shared__ float dh2s[TSB][NH + 1];
@@ -143,7 +151,7 @@ __global__ __launch_bounds__(NH) void k_mlp_bwd(const float *__restrict__ X, con
             float ls = l;
 #pragma unroll
             for (int o = 16; o >= 1; o >>= 1) ls += __shfl_xor(ls, o, 32);
-            if (j == 0) unsafeAtomicAdd(loss, ls);
+            if (j == 0) acc(loss, ls);
         }
         for (int e = j; e < TSB * MAXIN; e += NH) {
             const int s = e / MAXIN, i = e % MAXIN;
@@ -166,16 +174,16 @@ __global__ __launch_bounds__(NH) void k_mlp_bwd(const float *__restrict__ X, con
             dh2s[s][j] = d2;
             gb2j += d2;
         }
-        unsafeAtomicAdd(gW3 + j, gw3);
-        unsafeAtomicAdd(gb2 + j, gb2j);
-        if (j == 0) unsafeAtomicAdd(gb3, gb3s);
+        acc(gW3 + j, gw3);
+        acc(gb2 + j, gb2j);
+        if (j == 0) acc(gb3, gb3s);
         __syncthreads();
         // gW2[r][k] for column k = j (coalesced over threads per row r)
         for (int r = 0; r < NH; ++r) {
             float a = 0.f;
 #pragma unroll 8
             for (int s = 0; s < TSB; ++s) a += dh2s[s][r] * h1s[s][j];
-            unsafeAtomicAdd(gW2 + r * NH + j, a);
+            acc(gW2 + r * NH + j, a);
         }
         // hidden-1 error for unit k = j, then gW1 row j and gb1
         float d1[TSB];
@@ -192,14 +200,28 @@ __global__ __launch_bounds__(NH) void k_mlp_bwd(const float *__restrict__ X, con
             d1[s] = h1s[s][j] > 0.f ? d1[s] : 0.f;
             gb1j += d1[s];
         }
-        unsafeAtomicAdd(gb1 + j, gb1j);
+        acc(gb1 + j, gb1j);
         for (int i = 0; i < nin; ++i) {
             float a = 0.f;
 #pragma unroll
             for (int s = 0; s < TSB; ++s) a += d1[s] * xs[s][i];
-            unsafeAtomicAdd(gW1 + j * nin + i, a);
+            acc(gW1 + j * nin + i, a);
         }
+        first = false;
     }
+}
+
+// g[p] = sum over workgroups b (in order) of gpart[b][p]; the loss slot is
+// added to *loss.
+__global__ void k_mlp_reduce(const float *__restrict__ gpart, long long stride, int nblk, float *__restrict__ g,
+                             float *__restrict__ loss)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= stride) return;
+    float s = 0.f;
+    for (int b = 0; b < nblk; ++b) s += gpart[(size_t)b * stride + p];
+    if (p < stride - 1) g[p] = s;
+    else *loss += s;
 }
 
 __global__ void k_fill_f32(float *p, long long n, float v)
@@ -248,7 +270,7 @@ struct nw_mlp {
     int *bfidx = nullptr;
     long long adam_t = 0;
     float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
-    DBuf X, T, Wt, H1, H2, Y, perm;
+    DBuf X, T, Wt, H1, H2, Y, perm, gpart;
     ~nw_mlp()
     {
         (void)hipSetDevice(device);
@@ -318,9 +340,13 @@ void fwd_bwd(nw_mlp *p, const float *dX, const int *didx, long long n, const flo
 {
     float *H1 = p->H1.get<float>((size_t)n * NH), *H2 = p->H2.get<float>((size_t)n * NH), *Y = p->Y.get<float>(n);
     forward(p, dX, didx, n, Y, H1, H2);
-    hipLaunchKernelGGL(k_fill_f32, dim3(64), dim3(256), 0, p->st, p->g, p->np, 0.f);
-    hipLaunchKernelGGL(k_mlp_bwd, dim3(grid_for((n + TSB - 1) / TSB)), dim3(NH), 0, p->st, dX, didx, n, 1.0 / (double)n,
-                       p->net(), H1, H2, Y, dT, dW, p->g, dloss);
+    const int nblk = (int)std::max<long long>(1, std::min<long long>((n + TSB - 1) / TSB, 256));
+    const long long stride = p->np + 1;
+    float *gp = p->gpart.get<float>((size_t)nblk * stride);
+    hipLaunchKernelGGL(k_mlp_bwd, dim3(nblk), dim3(NH), 0, p->st, dX, didx, n, 1.0 / (double)n, p->net(), H1, H2, Y, dT,
+                       dW, gp, stride);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((unsigned)((stride + 255) / 256)), dim3(256), 0, p->st, gp, stride, nblk, p->g,
+                       dloss);
 }
 
 void adam(nw_mlp *p, float lr)

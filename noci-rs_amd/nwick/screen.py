@@ -328,8 +328,8 @@ class SelectionResult:
 
 
 def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_iters=2, top_frac=0.05,
-           audit_frac=0.01, chunk_frac=0.01, chunk_ratio=0.1, max_frac=0.5, epochs=8, batch=4096, lr=2e-3, seed=0,
-           mode=0, measure_recall=False, log=None):
+           audit_frac=0.01, chunk_frac=0.01, chunk_ratio=0.1, chunk_patience=2, max_frac=0.5, epochs=8, batch=4096,
+           lr=2e-3, seed=0, mode=0, measure_recall=False, log=None):
     """Selected NOCI from the references: each iteration adds the k_add
     best-scoring candidates.  learned=True trains the network on the exact
     scores of the first learn_iters iterations and screens the later ones.
@@ -392,19 +392,21 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
             order = torch.argsort(pred, descending=True).cpu().numpy()[:N - int(taken.sum())]
             sc = np.full(N, -1.0)
             # exact evaluation in order of predicted score: the top fraction,
-            # then further chunks while a chunk still holds a candidate within
-            # chunk_ratio of the current k_add-th best exact score
+            # then further chunks until chunk_patience chunks in a row hold no
+            # candidate within chunk_ratio of the current k_add-th best exact score
             chunk = max(k_add, int(chunk_frac * N))
             done = min(len(order), max(k_add, int(top_frac * N)))
             first = np.sort(order[:done])
             sc[first] = scorer.scores(cur_rec, c, e, first)
+            quiet = 0
             while done < len(order) and done < max_frac * N:
                 valid = np.sort(sc[sc >= 0])
                 kth = valid[-k_add] if len(valid) >= k_add else 0.0
                 nxt = np.sort(order[done:done + chunk])
                 sc[nxt] = scorer.scores(cur_rec, c, e, nxt)
                 done += len(nxt)
-                if sc[nxt].max() < chunk_ratio * kth:
+                quiet = quiet + 1 if sc[nxt].max() < chunk_ratio * kth else 0
+                if quiet >= chunk_patience:
                     break
             top = order[:done]
             rest = order[done:]
