@@ -509,11 +509,26 @@ def main():
     else:
         kname, peak = "k_qtransform_mfma_pipe", PEAK_FP64_TFLOPS
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"{'qtransform_f32' if use_mixed else 'qtransform'}_n100_hbm_traffic.txt")
+    pname = ("qtransform_bf3" if best == nwick.PREC_BF16X3 else "qtransform_f32") if use_mixed else "qtransform"
+    tfile = os.path.join(ROOT, "profiles", f"{pname}_n100_hbm_traffic.txt")
     if os.path.exists(tfile):
         for line in open(tfile):
             if line.startswith("TRAFFIC_BYTES"):
                 traffic = float(line.split()[1])
+    # algorithmic bytes per dispatch: FP64 passes read and write n^4 doubles; a
+    # reduced-precision tensor reads 4 n^4 floats and writes 3 n^4 floats + n^4
+    # doubles over its four passes, g tensors per dispatch
+    alg_bytes = g * 36.0 * n ** 4 / 4 if use_mixed else 16.0 * n ** 4
+    flops_disp = g * flops_tensor / 4
+    ridge = peak / (PEAK_HBM_GBS / 1e3)  # flop per byte where the MFMA and HBM roofs meet
+    hbm_bound = flops_disp / alg_bytes < ridge
+    if hbm_bound:
+        roof = {"bound": "hbm", "achieved": alg_bytes / (avg_ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": alg_bytes / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "mfma_tflops": achieved,
+                "mfma_peak_tflops": peak}
+    else:
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                "hbm_gbs": alg_bytes / (avg_ms * 1e-3) / 1e9}
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: oracle on pair (0,1) ...")
@@ -552,12 +567,11 @@ def main():
         "build_ms": float(np.mean(builds)),
         "transform_ms": float(np.mean(transforms)),
         "eval_ms": float(np.mean(evals)),
-        "roofline": {"bound": "mfma", "kernel": kname + " (rotating quarter transform)",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic,
+        "roofline": dict(roof, **{"kernel": kname + " (rotating quarter transform)", "traffic": traffic,
                      "traffic_source": os.path.relpath(tfile, ROOT) + " (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
+                     "algorithmic_bytes_per_dispatch": alg_bytes,
                      "dispatches_per_step": dispatches, "tensors_per_dispatch": g, "avg_launch_ms": avg_ms,
-                     "flops_per_dispatch": g * flops_tensor / 4},
+                     "flops_per_dispatch": flops_disp}),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
