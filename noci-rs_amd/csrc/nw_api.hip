@@ -125,25 +125,40 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         Uploader up(c, 64u << 20);
         // ---- scratch per (pair, spin) -------------------------------------
         int nomax = std::max(c->nocc[0], c->nocc[1]);
-        std::vector<double *> frees;
-        auto A = [&](size_t cnt) { double *p = dalloc<double>(cnt); frees.push_back(p); return p; };
-        struct Cleanup { std::vector<double *> &f; ~Cleanup() { for (double *p : f) (void)hipFree(p); } } cleanup{frees};
-        double *Tocc = A((size_t)2 * P * n * nomax), *Socc = A((size_t)2 * P * nomax * nomax);
-        double *U = A((size_t)2 * P * nomax * nomax), *V = A((size_t)2 * P * nomax * nomax);
-        double *sig = A((size_t)2 * P * nomax), *phase = A(2 * P), *tsp = A(2 * P);
-        int *mdev = (int *)A(P);
-        double *Lt = A((size_t)2 * P * n * nomax), *Gt = A((size_t)2 * P * n * nomax);
+        // The build's temporaries live in one grow-only arena of the context:
+        // in steady state a build allocates and frees nothing (hipFree waits
+        // for the whole device), so builds of different contexts overlap.
+        const size_t Pz = (size_t)P, nz = (size_t)n, noz = (size_t)nomax;
+        enum { T_TOCC, T_SOCC, T_U, T_V, T_SIG, T_PHASE, T_TSP, T_M, T_LT, T_GT, T_ROWC, T_COLC, T_MS, T_MTS, T_Q, T_R,
+               T_Z, T_JK, T_JC, T_HXC, T_JKXC, T_META, T_COUNT };
+        const size_t req[T_COUNT] = {2 * Pz * nz * noz, 2 * Pz * noz * noz, 2 * Pz * noz * noz, 2 * Pz * noz * noz,
+                                     2 * Pz * noz, 2 * Pz, 2 * Pz, Pz, 2 * Pz * nz * noz, 2 * Pz * nz * noz,
+                                     2 * Pz * nnm, 2 * Pz * nnm, 4 * Pz * nn, 4 * Pz * nn, 4 * Pz * nnm, 4 * Pz * nnm,
+                                     2 * Pz * nm2, 4 * Pz * nn, 4 * Pz * nn, 4 * Pz * nnm, 8 * Pz * nnm, Pz * M_LEN};
+        size_t need = 0;
+        for (size_t r : req) need += (r + 31) & ~size_t(31);
+        double *arena = c->buildscratch.get<double>(need);
+        double *tp[T_COUNT];
+        for (size_t k = 0, cur = 0; k < T_COUNT; ++k) {
+            tp[k] = arena + cur;
+            cur += (req[k] + 31) & ~size_t(31);
+        }
+        double *Tocc = tp[T_TOCC], *Socc = tp[T_SOCC];
+        double *U = tp[T_U], *V = tp[T_V];
+        double *sig = tp[T_SIG], *phase = tp[T_PHASE], *tsp = tp[T_TSP];
+        int *mdev = (int *)tp[T_M];
+        double *Lt = tp[T_LT], *Gt = tp[T_GT];
         // co-densities and the cx/xc contractions stay resident for nw_update_fock
         double *M = c->keepM.get<double>((size_t)4 * P * nn);  // [(p*2+sp)*2+mi][n][n]
-        double *rowc = A((size_t)2 * P * nnm), *colc = A((size_t)2 * P * nnm);
+        double *rowc = tp[T_ROWC], *colc = tp[T_COLC];
         double *cx = c->keepCX.get<double>((size_t)4 * P * nnm), *xc = c->keepXC.get<double>((size_t)4 * P * nnm);
-        double *MS = A((size_t)4 * P * nn), *MtS = A((size_t)4 * P * nn);
-        double *Q = A((size_t)4 * P * nnm), *R = A((size_t)4 * P * nnm);
-        double *Z = A((size_t)2 * P * nm2);
-        double *JK = A((size_t)4 * P * nn), *Jc = A((size_t)4 * P * nn);
-        double *hxc = A((size_t)4 * P * nnm);
-        double *JKxc = A((size_t)8 * P * nnm);
-        double *dmeta = A((size_t)P * M_LEN);
+        double *MS = tp[T_MS], *MtS = tp[T_MTS];
+        double *Q = tp[T_Q], *R = tp[T_R];
+        double *Z = tp[T_Z];
+        double *JK = tp[T_JK], *Jc = tp[T_JC];
+        double *hxc = tp[T_HXC];
+        double *JKxc = tp[T_JKXC];
+        double *dmeta = tp[T_META];
         HIPCHK(hipMemsetAsync(dmeta, 0, (size_t)P * M_LEN * 8, c->st));
         auto Cref = [&](int sp, int r) { return c->d_c[sp] + (size_t)r * nnm; };
         auto prob = [&](int p, int sp) { return p * 2 + sp; };
@@ -526,9 +541,13 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                 o.sigma_min = pi.smin; o.fp32 = pi.fp32; o.pad = 0;
             }
         }
-        if (c->d_pinfo) HIPCHK(hipFree(c->d_pinfo));
-        c->d_pinfo = nullptr;
-        c->d_pinfo = (PairInfo *)dalloc<char>(sizeof(PairInfo) * c->pinfo.size());
+        if (c->pinfo.size() > c->pinfo_cap) {
+            if (c->d_pinfo) HIPCHK(hipFree(c->d_pinfo));
+            c->d_pinfo = nullptr;
+            c->pinfo_cap = 0;
+            c->d_pinfo = (PairInfo *)dalloc<char>(sizeof(PairInfo) * c->pinfo.size());
+            c->pinfo_cap = c->pinfo.size();
+        }
         HIPCHK(hipMemcpyAsync(c->d_pinfo, c->pinfo.data(), sizeof(PairInfo) * c->pinfo.size(), hipMemcpyHostToDevice, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
         c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);

@@ -246,7 +246,9 @@ int nw_load_wicks(nw_ctx *c, const char *dir, double tol)
         c->pinfo = pinfo;
         if (c->d_pinfo) HIPCHK(hipFree(c->d_pinfo));
         c->d_pinfo = nullptr;
+        c->pinfo_cap = 0;
         c->d_pinfo = (PairInfo *)dalloc<char>(sizeof(PairInfo) * pinfo.size());
+        c->pinfo_cap = pinfo.size();
         HIPCHK(hipMemcpy(c->d_pinfo, pinfo.data(), sizeof(PairInfo) * pinfo.size(), hipMemcpyHostToDevice));
         c->tol = tol;
         c->have_codensity = false;  // nw_update_fock needs the co-densities of a build

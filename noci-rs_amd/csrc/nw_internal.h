@@ -100,9 +100,10 @@ struct nw_ctx {
     size_t slab_cap = 0;
     std::vector<nw::PairInfo> pinfo;
     nw::PairInfo *d_pinfo = nullptr;
+    size_t pinfo_cap = 0;
     double t_ms[3] = {0, 0, 0};
     // scratch
-    nwi::DBuf arena_desc, scratchA, scratchB, cx32, xc32, keepM, keepCX, keepXC, fockbuf, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
+    nwi::DBuf buildscratch, arena_desc, scratchA, scratchB, cx32, xc32, keepM, keepCX, keepXC, fockbuf, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
     std::deque<std::vector<char>> keep;
 
     ~nw_ctx()
