@@ -722,10 +722,106 @@ void launch_qtransform(hipStream_t st, const double *X, const double *c, double 
     launch_qtransform_many(st, &X, &c, &Y, 1, rows, nl, nout);
 }
 
+// C = A B^T on FP64 MFMA (A: M x K, B: N x K, both K-contiguous), the shape
+// of the J - K and J contractions of every co-density with the n^2 x n^2
+// integral matrices.  A workgroup owns a 48 x 64 tile of C (4 waves, each 16
+// columns x 3 row blocks of 16), K staged through LDS 32 at a time.
+__global__ __launch_bounds__(256) void k_gemm_nt_mfma(const GemmDesc *__restrict__ descs, int M, int N, int K, int lda,
+                                                      int ldb, int ldc)
+{
+    constexpr int MT = 48, NTL = 64, BK = 32, LD = BK + 2;
+    constexpr int PA = MT * BK / 2 / 256, PB = NTL * BK / 2 / 256;  // double2 per thread per chunk
+    __shared__ double As[MT][LD];
+    __shared__ double Bs[NTL][LD];
+    const GemmDesc g = descs[blockIdx.z];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+    // row blocks of M vary fastest over the grid: the workgroups sharing a B tile run together
+    const int m0 = blockIdx.x * MT, n0 = blockIdx.y * NTL;
+    const bool vec = (K % 2 == 0) && (lda % 2 == 0) && (ldb % 2 == 0);
+    double2 ra[PA], rb[PB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < PA; ++j) {
+            const int e = tid + j * 256, r = e / (BK / 2), k = 2 * (e % (BK / 2)), gi = m0 + r, gk = k0 + k;
+            double2 v{0.0, 0.0};
+            if (gi < M) {
+                const double *src = g.A + (size_t)gi * lda + gk;
+                if (vec && gk + 1 < K) v = *reinterpret_cast<const double2 *>(src);
+                else {
+                    if (gk < K) v.x = src[0];
+                    if (gk + 1 < K) v.y = src[1];
+                }
+            }
+            ra[j] = v;
+        }
+#pragma unroll
+        for (int j = 0; j < PB; ++j) {
+            const int e = tid + j * 256, r = e / (BK / 2), k = 2 * (e % (BK / 2)), gj = n0 + r, gk = k0 + k;
+            double2 v{0.0, 0.0};
+            if (gj < N) {
+                const double *src = g.B + (size_t)gj * ldb + gk;
+                if (vec && gk + 1 < K) v = *reinterpret_cast<const double2 *>(src);
+                else {
+                    if (gk < K) v.x = src[0];
+                    if (gk + 1 < K) v.y = src[1];
+                }
+            }
+            rb[j] = v;
+        }
+    };
+    nw_d4 acc[3];
+#pragma unroll
+    for (int ms = 0; ms < 3; ++ms) acc[ms] = nw_d4{0.0, 0.0, 0.0, 0.0};
+    fetch(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PA; ++j) {
+            const int e = tid + j * 256, r = e / (BK / 2), k = 2 * (e % (BK / 2));
+            As[r][k] = ra[j].x;
+            As[r][k + 1] = ra[j].y;
+        }
+#pragma unroll
+        for (int j = 0; j < PB; ++j) {
+            const int e = tid + j * 256, r = e / (BK / 2), k = 2 * (e % (BK / 2));
+            Bs[r][k] = rb[j].x;
+            Bs[r][k + 1] = rb[j].y;
+        }
+        __syncthreads();
+        if (k0 + BK < K) fetch(k0 + BK);  // next chunk in flight during the products
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+            const double b = Bs[wave * 16 + lr][kk * 4 + lk];
+#pragma unroll
+            for (int ms = 0; ms < 3; ++ms)
+                acc[ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[ms * 16 + lr][kk * 4 + lk], b, acc[ms], 0, 0, 0);
+        }
+    }
+    const int col = n0 + wave * 16 + lr;
+    if (col >= N) return;
+#pragma unroll
+    for (int ms = 0; ms < 3; ++ms)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = m0 + ms * 16 + 4 * r + lk;
+            if (row < M) g.C[(size_t)row * ldc + col] = acc[ms][r];
+        }
+}
+
 void launch_gemm(hipStream_t st, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc,
                  double alpha, double beta, const GemmDesc *d_descs, int nbatch)
 {
     if (nbatch <= 0 || M <= 0 || N <= 0) return;
+    if (ta == 0 && tb == 1 && alpha == 1.0 && beta == 0.0 && K >= 256 && (long long)N * K >= (1ll << 20) &&
+        (N + 63) / 64 <= 65535) {
+        // the large NT products (co-densities x integral matrices) on MFMA
+        for (int z0 = 0; z0 < nbatch; z0 += 65535) {
+            const int chunk = nbatch - z0 > 65535 ? 65535 : nbatch - z0;
+            dim3 grid((M + 47) / 48, (N + 63) / 64, chunk);
+            hipLaunchKernelGGL(k_gemm_nt_mfma, grid, dim3(256), 0, st, d_descs + z0, M, N, K, lda, ldb, ldc);
+        }
+        return;
+    }
     int nz = nbatch;
     while (nz > 0) {
         int chunk = nz > 65535 ? 65535 : nz;

@@ -202,3 +202,20 @@ def test_transform_shapes_odd_and_two_mod_four(n):
             assert err[prec] < 1e-5 * max(1.0, np.max(np.abs(H)))
         # the split BF16 products carry FP32-level error
         assert err[nwick.PREC_BF16X3] < 8.0 * err[nwick.PREC_MIXED] + 1e-12
+
+
+def test_large_basis_mfma_contractions():
+    """n_bas = 36: the co-density x integral-matrix products (J - K, J) take
+    the FP64 MFMA path (K = n^2 >= 256, N K >= 2^20); H, S against the oracle,
+    including an m = 1 reference pair."""
+    sysm, dets = _case(36, 3, 2, 2, seed=36, orth_refs=(1,), orth_count=1, per_ref=12)
+    print(f"large basis: n=36 ndet={len(dets)}", flush=True)
+    rec = basis.to_records(dets)
+    eng = nwick.Engine.from_system(sysm)
+    meta = eng.build(tol=1e-8)
+    assert meta[(0, 1)].ma == 1
+    H, S = eng.hs_matrix(rec, mode=1)
+    Ho, So = O.hs_matrix(O.Slab(sysm, tol=1e-8), dets, mode=1)
+    print(f"large basis: max|dH|={np.max(np.abs(H - Ho)):.3e} max|dS|={np.max(np.abs(S - So)):.3e}", flush=True)
+    assert np.max(np.abs(S - So)) < TOL
+    assert np.max(np.abs(H - Ho)) < TOL * max(1.0, np.max(np.abs(Ho)))
