@@ -455,6 +455,13 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
             net = MLP(N_FEATURES, seed=seed)
             t1 = time.perf_counter()
             el = net.train(Xn, yd, wd, epochs=epochs, batch=batch, lr=lr, seed=seed)
+            # first use of the inference path (network forward, full-size sort,
+            # masked assignment) once here, with the training time, so the
+            # screened iterations carry only their own work
+            probe = torch.empty(N, dtype=torch.float32, device=dev)
+            net.forward(Xn[:min(N, len(Xn))].contiguous(), out=probe[:min(N, len(Xn))])
+            probe[torch.from_numpy(taken).to(dev)] = -float("inf")
+            torch.argsort(probe, descending=True).cpu()
             res.train.update({"samples": int(len(y)), "epochs": epochs, "loss": [float(v) for v in el],
                               "seconds": time.perf_counter() - t1})
             del Xs, X, Xn
