@@ -536,7 +536,7 @@ void make_plans(nw_factor *f)
     }
     f->d_items.upload(items);
     f->d_rows.upload(rows);
-    (void)mat_need;
+    if (mat_need > f->d_mat.n) throw NwError(NW_E_STATE, "factor plan exceeds its materialisation buffer");
 }
 
 size_t mat_elems_needed(const nw_factor *f)
