@@ -181,8 +181,9 @@ __global__ __launch_bounds__(512) void k_qtransform_mfma(const QPtrs q, long lon
 // multiplied out of LDS, so HBM latency hides behind the MFMA work.  Rows of
 // X are nl long; the LDS tile is padded to NL = 4 KS columns with zeros so
 // the K = 4 MFMA steps need no guards.
-template <int KS, bool PAD2, int TRD>
-__global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, long long rows, int nout, long long ntiles)
+template <int KS, bool PAD2, int TRD, int NW>
+__global__ __launch_bounds__(64 * NW) void k_qtransform_mfma_pipe(const QPtrs q, long long rows, int nout,
+                                                                  long long ntiles)
 {
     constexpr int nl = KS * 4 - (PAD2 ? 2 : 0);  // row length of X (even)
     const double *__restrict__ X = (const double *)q.X[blockIdx.y];
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(448) void k_qtransform_mfma_pipe(const QPtrs q, lon
     double *__restrict__ Y = (double *)q.Y[blockIdx.y];
     constexpr int NL = KS * 4;
     constexpr int LDX = NL + 1;
-    constexpr int NT = 448;
+    constexpr int NT = 64 * NW;
     constexpr int PER = (TRD * NL / 2 + NT - 1) / NT;  // double2 per thread (upper bound)
     extern __shared__ double xs[];
     const int tid = threadIdx.x;
@@ -277,8 +278,9 @@ typedef float nw_f4 __attribute__((ext_vector_type(4)));
 #define NW_TR_F32 64
 #endif
 
-template <int KS, typename OutT, int VEC, int TR>
-__global__ __launch_bounds__(448) void k_qtransform_f32_pipe(const QPtrs q, long long rows, int nout, long long ntiles)
+template <int KS, typename OutT, int VEC, int TR, int NW>
+__global__ __launch_bounds__(64 * NW) void k_qtransform_f32_pipe(const QPtrs q, long long rows, int nout,
+                                                                 long long ntiles)
 {
     constexpr int nl = KS * 4 - (VEC == 4 ? 0 : 2);  // row length of X: float4 rows when nl % 4 == 0
     const float *__restrict__ X = (const float *)q.X[blockIdx.y];
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(448) void k_qtransform_f32_pipe(const QPtrs q, long
     using VT = typename std::conditional<VEC == 4, float4, float2>::type;
     constexpr int NL = KS * 4;
     constexpr int LDX = NL + 1;
-    constexpr int NT = 448;
+    constexpr int NT = 64 * NW;
     constexpr int PER = (TR * NL / VEC + NT - 1) / NT;     // vectors per thread (upper bound)
     extern __shared__ float xsf[];
     const int tid = threadIdx.x;
@@ -409,6 +411,38 @@ static unsigned grid_x(long long ntiles, int count, long long target)
     return (unsigned)g;
 }
 
+// Waves per workgroup: one per 16 output rows.  Narrow transforms (nout <= 64
+// with rows of at most 64) get 1- to 4-wave workgroups, so no wave idles.
+inline int waves_for(int nout) { return nout <= 64 ? (nout + 15) / 16 : 7; }
+
+template <int K, typename OutT, int TR>
+static void (*f32_kernel(bool v4, int nout, int &nw))(const QPtrs, long long, int, long long)
+{
+    nw = 7;
+    if constexpr (K <= 16) {
+        nw = waves_for(nout);
+        if (nw == 1) return v4 ? k_qtransform_f32_pipe<K, OutT, 4, TR, 1> : k_qtransform_f32_pipe<K, OutT, 2, TR, 1>;
+        if (nw == 2) return v4 ? k_qtransform_f32_pipe<K, OutT, 4, TR, 2> : k_qtransform_f32_pipe<K, OutT, 2, TR, 2>;
+        if (nw == 3) return v4 ? k_qtransform_f32_pipe<K, OutT, 4, TR, 3> : k_qtransform_f32_pipe<K, OutT, 2, TR, 3>;
+        if (nw == 4) return v4 ? k_qtransform_f32_pipe<K, OutT, 4, TR, 4> : k_qtransform_f32_pipe<K, OutT, 2, TR, 4>;
+    }
+    return v4 ? k_qtransform_f32_pipe<K, OutT, 4, TR, 7> : k_qtransform_f32_pipe<K, OutT, 2, TR, 7>;
+}
+
+template <int K, int TRD>
+static void (*f64_kernel(bool pad2, int nout, int &nw))(const QPtrs, long long, int, long long)
+{
+    nw = 7;
+    if constexpr (K <= 16) {
+        nw = waves_for(nout);
+        if (nw == 1) return pad2 ? k_qtransform_mfma_pipe<K, true, TRD, 1> : k_qtransform_mfma_pipe<K, false, TRD, 1>;
+        if (nw == 2) return pad2 ? k_qtransform_mfma_pipe<K, true, TRD, 2> : k_qtransform_mfma_pipe<K, false, TRD, 2>;
+        if (nw == 3) return pad2 ? k_qtransform_mfma_pipe<K, true, TRD, 3> : k_qtransform_mfma_pipe<K, false, TRD, 3>;
+        if (nw == 4) return pad2 ? k_qtransform_mfma_pipe<K, true, TRD, 4> : k_qtransform_mfma_pipe<K, false, TRD, 4>;
+    }
+    return pad2 ? k_qtransform_mfma_pipe<K, true, TRD, 7> : k_qtransform_mfma_pipe<K, false, TRD, 7>;
+}
+
 template <typename OutT>
 static bool launch_qtransform_f32_t(hipStream_t st, const float *const *X, const float *const *c, OutT *const *Y,
                                     int count, long long rows, int nl, int nout)
@@ -423,13 +457,12 @@ static bool launch_qtransform_f32_t(hipStream_t st, const float *const *X, const
         const int nb = count - b0 < QMAX ? count - b0 : QMAX;
         QPtrs q{};
         for (int k = 0; k < nb; ++k) { q.X[k] = X[b0 + k]; q.c[k] = c[b0 + k]; q.Y[k] = Y[b0 + k]; }
-        const dim3 b(448);
 #define NW_QF(K)                                                                                               \
     case K: {                                                                                                  \
-        void (*kern)(const QPtrs, long long, int, long long) =                                                 \
-            v4 ? k_qtransform_f32_pipe<K, OutT, 4, TR> : k_qtransform_f32_pipe<K, OutT, 2, TR>;                \
-        const dim3 g(grid_x(ntiles, nb, resident_grid(kern, 448, lds)), (unsigned)nb);                        \
-        hipLaunchKernelGGL(kern, g, b, lds, st, q, rows, nout, ntiles);                                        \
+        int nw = 7;                                                                                            \
+        void (*kern)(const QPtrs, long long, int, long long) = f32_kernel<K, OutT, TR>(v4, nout, nw);          \
+        const dim3 g(grid_x(ntiles, nb, resident_grid(kern, 64 * nw, lds)), (unsigned)nb);                    \
+        hipLaunchKernelGGL(kern, g, dim3(64 * nw), lds, st, q, rows, nout, ntiles);                           \
         break;                                                                                                 \
     }
         switch (ks) {
@@ -673,14 +706,13 @@ void launch_qtransform_many(hipStream_t st, const double *const *X, const double
         bool done = false;
         if (nl % 2 == 0 && ks <= 31 && nw <= 7) {
             const size_t lds = (size_t)TRD * (ks * 4 + 1) * sizeof(double);
-            const dim3 b(448);
             done = true;
 #define NW_QP(K)                                                                                        \
     case K: {                                                                                           \
-        void (*kern)(const QPtrs, long long, int, long long) =                                          \
-            (nl % 4) ? k_qtransform_mfma_pipe<K, true, TRD> : k_qtransform_mfma_pipe<K, false, TRD>;    \
-        const dim3 g(grid_x(ntiles, nb, resident_grid(kern, 448, lds)), (unsigned)nb);                  \
-        hipLaunchKernelGGL(kern, g, b, lds, st, q, rows, nout, ntiles);                                 \
+        int nwv = 7;                                                                                    \
+        void (*kern)(const QPtrs, long long, int, long long) = f64_kernel<K, TRD>(nl % 4 != 0, nout, nwv); \
+        const dim3 g(grid_x(ntiles, nb, resident_grid(kern, 64 * nwv, lds)), (unsigned)nb);            \
+        hipLaunchKernelGGL(kern, g, dim3(64 * nwv), lds, st, q, rows, nout, ntiles);                   \
         break;                                                                                          \
     }
             switch (ks) {
