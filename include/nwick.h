@@ -233,6 +233,15 @@ int nw_factor_diagonals(nw_factor *f, double lambda, double *mdiag, double *sdia
  * (OneBodyFactorisation::storage_bytes, onebody.rs:231-281, counted over
  * every ordered pair). */
 int nw_factor_info(const nw_factor *f, int parent, uint64_t *ncomp_a, uint64_t *ncomp_b, uint64_t *table_bytes);
+/* Tables that nw_factor_apply reads: NW_PREC_FP64 (default) or NW_PREC_MIXED,
+ * FP32 tables and FP32 materialised rows with FP64 sums (rounded copies of
+ * the resident FP64 tables, made on first use and kept current by
+ * nw_factor_refresh; regenerated tables are written in FP32 directly).
+ * Diagonals always come from FP64 tables; nw_factor_info and
+ * nw_factor_table report the selected tables.  No counterpart in the
+ * reference, which applies the tables in FP64: this is the low-precision
+ * operator of an iteratively refined PT2 solve, whose residuals use FP64. */
+int nw_factor_set_precision(nw_factor *f, int precision);
 /* Per-determinant component ids (SpinFactorisation::aid/bid, mod.rs:106-130). */
 int nw_factor_ids(const nw_factor *f, uint32_t *aids, uint32_t *bids);
 /* Copy one table: ncomp(source) x ncomp(target) interleaved {S, F} doubles,

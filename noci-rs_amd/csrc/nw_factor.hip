@@ -57,17 +57,26 @@ struct SpinComp {
 static_assert(sizeof(SpinComp) == 12, "SpinComp layout");
 
 // Stage-1 work item: out[t] = sum_{k in [e0, e1)} x[mdet[k]] tab[midx[k] nt + t], t < nt.
+// Tables and materialised rows hold {S, F} pairs as double2 (FP64 tables) or
+// float2 (FP32 tables); the kernels take the element type as a template.
 struct S1Item {
-    const double2 *tab;
-    double2 *out;
+    const void *tab;
+    void *out;
     int nt, e0, e1, pad;
 };
 // Stage-2 row: alpha-side row u (over target alpha components), beta-side row
 // v (over target beta components), weight x[wdet] (wdet >= 0) or 1.
 struct S2Row {
-    const double2 *u, *v;
+    const void *u, *v;
     long long wdet;
 };
+
+template <class T2>
+__device__ inline T2 pair2(double a, double b);
+template <>
+__device__ inline double2 pair2<double2>(double a, double b) { return make_double2(a, b); }
+template <>
+__device__ inline float2 pair2<float2>(double a, double b) { return make_float2((float)a, (float)b); }
 
 constexpr int S1_CHUNK = 128;  // entries per stage-1 item (pieces of a large group)
 constexpr int S2_BATCH = 256;  // rows staged in LDS at a time
@@ -91,17 +100,18 @@ __device__ inline void spin_fs(const double *slab, const long long *o, const Spi
 // with the bra = target when target_left, else the source; 16 x 16 tiles.
 // The table of the reverse pair is the transpose (the same Wick element with
 // the same bra), so when tabT is not null the tile is also written there,
-// tabT[t][s], through LDS.
+// tabT[t][s], through LDS.  Elements are evaluated in FP64 and stored as T2
+// (float2: rounded to nearest FP32).
+template <class T2>
 __global__ __launch_bounds__(256) void k_fac_table(const PairInfo *__restrict__ pairs, const double *__restrict__ slab0,
                                                    int pair, int spin, const SpinComp *__restrict__ tc, int nt,
                                                    const SpinComp *__restrict__ scomp, int ns, int target_left, int nmo,
-                                                   double tol, int mode, double2 *__restrict__ tab,
-                                                   double2 *__restrict__ tabT)
+                                                   double tol, int mode, T2 *__restrict__ tab, T2 *__restrict__ tabT)
 {
-    __shared__ double2 sh[16][17];
+    __shared__ T2 sh[16][17];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int t = blockIdx.x * 16 + tx, s = blockIdx.y * 16 + ty;
-    double2 out = make_double2(0.0, 0.0);
+    T2 out = pair2<T2>(0.0, 0.0);
     if (t < nt && s < ns) {
         const PairInfo &P = pairs[pair];
         const double *slab = slab0 + P.base;
@@ -120,7 +130,7 @@ __global__ __launch_bounds__(256) void k_fac_table(const PairInfo *__restrict__ 
             default: break;  // excluded on the host
         }
         const double ph = (double)(T.ph * S.ph) * sc.pref;
-        out = make_double2(ph * vs, ph * vf);
+        out = pair2<T2>(ph * vs, ph * vf);
         tab[(size_t)s * nt + t] = out;
     }
     if (!tabT) return;  // uniform over the launch
@@ -135,6 +145,8 @@ __global__ __launch_bounds__(256) void k_fac_table(const PairInfo *__restrict__ 
 // The host pads grid.x to a multiple of 8, so a column block always runs on
 // the same XCD (workgroups are dealt round-robin over the 8 XCDs) and the
 // table rows that several groups share are re-read from that XCD's L2.
+// Sums run in FP64 whatever the table type.
+template <class T2>
 __global__ __launch_bounds__(256) void k_fac_mat(const S1Item *__restrict__ items, const uint32_t *__restrict__ midx,
                                                  const uint32_t *__restrict__ mdet, const double *__restrict__ x)
 {
@@ -149,25 +161,27 @@ __global__ __launch_bounds__(256) void k_fac_mat(const S1Item *__restrict__ item
     __syncthreads();
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= it.nt) return;
-    const double2 *col = it.tab + t;
+    const T2 *col = (const T2 *)it.tab + t;
     double s = 0.0, f = 0.0;
 #pragma unroll 8
     for (int k = 0; k < ne; ++k) {
-        const double2 v = col[(size_t)s_r[k] * it.nt];
-        s = fma(s_x[k], v.x, s);
-        f = fma(s_x[k], v.y, f);
+        const T2 v = col[(size_t)s_r[k] * it.nt];
+        s = fma(s_x[k], (double)v.x, s);
+        f = fma(s_x[k], (double)v.y, f);
     }
-    it.out[t] = make_double2(s, f);
+    ((T2 *)it.out)[t] = pair2<T2>(s, f);
 }
 
-// Stage 2 over the rows [k chunk, (k+1) chunk): one lane per target entry.
+// Stage 2 over the rows [k chunk, (k+1) chunk): one lane per target entry;
+// FP64 sums whatever the table type.
+template <class T2>
 __global__ __launch_bounds__(256) void k_fac_rows(const uint32_t *__restrict__ ta, const uint32_t *__restrict__ tb,
                                                   int nt, const S2Row *__restrict__ rows, int nrows, int chunk,
                                                   const double *__restrict__ x, double lambda,
                                                   double2 *__restrict__ part2)
 {
-    __shared__ const double2 *s_u[S2_BATCH];
-    __shared__ const double2 *s_v[S2_BATCH];
+    __shared__ const T2 *s_u[S2_BATCH];
+    __shared__ const T2 *s_v[S2_BATCH];
     __shared__ double s_w[S2_BATCH];
     const int I = blockIdx.x * 256 + threadIdx.x;
     const bool live = I < nt;
@@ -179,21 +193,21 @@ __global__ __launch_bounds__(256) void k_fac_rows(const uint32_t *__restrict__ t
         __syncthreads();
         if ((int)threadIdx.x < nb) {
             const S2Row r = rows[base + threadIdx.x];
-            s_u[threadIdx.x] = r.u;
-            s_v[threadIdx.x] = r.v;
+            s_u[threadIdx.x] = (const T2 *)r.u;
+            s_v[threadIdx.x] = (const T2 *)r.v;
             s_w[threadIdx.x] = r.wdet >= 0 ? x[r.wdet] : 1.0;
         }
         __syncthreads();
         if (live) {
 #pragma unroll 8
             for (int k = 0; k < nb; ++k) {
-                const double2 u = s_u[k][a];  // alpha side {S, F}
-                const double2 v = s_v[k][b];  // beta side {S, F}
+                const T2 u = s_u[k][a];  // alpha side {S, F}
+                const T2 v = s_v[k][b];  // beta side {S, F}
                 const double w = s_w[k];
-                const double us = w * u.x, uf = w * u.y;
-                m = fma(uf, v.x, m);
-                m = fma(us, fma(lambda, v.x, v.y), m);
-                s = fma(us, v.x, s);
+                const double us = w * (double)u.x, uf = w * (double)u.y, vs = (double)v.x;
+                m = fma(uf, vs, m);
+                m = fma(us, fma(lambda, vs, (double)v.y), m);
+                s = fma(us, vs, s);
             }
         }
     }
@@ -236,6 +250,22 @@ __global__ void k_fac_zero(double *__restrict__ p, long long n)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
+}
+
+// FP64 tables rounded to nearest FP32 (the FP32 table copies).
+__global__ void k_fac_lower(const double2 *__restrict__ in, float2 *__restrict__ out, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double2 v = in[i];
+        out[i] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+__global__ void k_fac_widen(const float2 *__restrict__ in, double2 *__restrict__ out, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_double2((double)in[i].x, (double)in[i].y);
 }
 
 unsigned blocks_of(long long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -399,16 +429,19 @@ struct nw_factor {
     std::vector<ParentSpace> par;
     std::vector<uint32_t> aid, bid;  // per determinant
     DVec<SpinComp> d_comp[2];
-    // tables: per (Q * nref + P) * 2 + spin, offset into d_tab (-1 = none)
+    // tables: per (Q * nref + P) * 2 + spin, offset in elements into d_tab / d_tab32 (-1 = none)
     std::vector<long long> tab_off;
-    DVec<double2> d_tab;
+    DVec<double2> d_tab;    // FP64 tables (resident), or one pair's tables (regenerated)
+    DVec<float2> d_tab32;   // FP32 copies of the resident tables, made on first FP32 use
+    int prec = NW_PREC_FP64;  // tables nw_factor_apply reads: NW_PREC_FP64 or NW_PREC_MIXED (FP32)
     size_t tab_elems = 0;
-    // plan
+    // plan: stage-1 items and stage-2 rows per table type (0 FP64, 1 FP32)
     DVec<uint32_t> d_midx, d_mdet;
-    DVec<S1Item> d_items;
-    DVec<S2Row> d_rows;
+    DVec<S1Item> d_items[2];
+    DVec<S2Row> d_rows[2];
+    bool planned[2] = {false, false};
     std::vector<PairPlan> plans;  // ordered by target, then source
-    DVec<double2> d_mat, d_part2;
+    DVec<double2> d_mat, d_part2;  // d_mat holds double2 or float2 rows
     int maxnt = 0;
     double t_ms[2] = {0, 0};  // last apply: total, table regeneration
 };
@@ -423,9 +456,12 @@ size_t table_elems(const nw_factor *f, int q, int p, int sp)
     return (size_t)comps_of(f, p, sp) * (size_t)comps_of(f, q, sp);
 }
 
-// Table (q target, p source, spin sp) into out; when outT is not null also
-// its transpose, the table (p target, q source).
-void launch_table(nw_factor *f, int q, int p, int sp, double2 *out, double2 *outT = nullptr)
+size_t elem_bytes(int lo) { return lo ? sizeof(float2) : sizeof(double2); }
+
+// Table (q target, p source, spin sp) into out (float2 when lo, else
+// double2); when outT is not null also its transpose, the table (p target,
+// q source).
+void launch_table(nw_factor *f, int q, int p, int sp, void *out, void *outT = nullptr, int lo = 0)
 {
     nw_ctx *c = f->c;
     const ParentSpace &Q = f->par[q], &P = f->par[p];
@@ -434,9 +470,23 @@ void launch_table(nw_factor *f, int q, int p, int sp, double2 *out, double2 *out
     const int nt = Q.ncomp[sp], ns = P.ncomp[sp];
     if (nt == 0 || ns == 0) return;
     dim3 grid(blocks_of(nt, 16), blocks_of(ns, 16));
-    hipLaunchKernelGGL(k_fac_table, grid, dim3(256), 0, c->st, c->d_pinfo, c->d_slab, pair, sp,
-                       f->d_comp[sp].p + Q.comp_off[sp], nt, f->d_comp[sp].p + P.comp_off[sp], ns,
-                       target_left ? 1 : 0, c->nmo, c->tol, f->mode, out, outT);
+    if (lo)
+        hipLaunchKernelGGL(k_fac_table<float2>, grid, dim3(256), 0, c->st, c->d_pinfo, c->d_slab, pair, sp,
+                           f->d_comp[sp].p + Q.comp_off[sp], nt, f->d_comp[sp].p + P.comp_off[sp], ns,
+                           target_left ? 1 : 0, c->nmo, c->tol, f->mode, (float2 *)out, (float2 *)outT);
+    else
+        hipLaunchKernelGGL(k_fac_table<double2>, grid, dim3(256), 0, c->st, c->d_pinfo, c->d_slab, pair, sp,
+                           f->d_comp[sp].p + Q.comp_off[sp], nt, f->d_comp[sp].p + P.comp_off[sp], ns,
+                           target_left ? 1 : 0, c->nmo, c->tol, f->mode, (double2 *)out, (double2 *)outT);
+}
+
+// FP32 copies of the resident FP64 tables.
+void lower_tables(nw_factor *f)
+{
+    if (f->tab_elems == 0) return;
+    const long long n = (long long)f->tab_elems;
+    const unsigned g = (unsigned)std::min<long long>(8192, blocks_of(n, 256));
+    hipLaunchKernelGGL(k_fac_lower, dim3(g), dim3(256), 0, f->c->st, f->d_tab.p, f->d_tab32.p, n);
 }
 
 void build_tables(nw_factor *f)
@@ -453,6 +503,7 @@ void build_tables(nw_factor *f)
                 else if (f->par[q].first_det < f->par[p].first_det)
                     launch_table(f, q, p, sp, f->d_tab.p + off, f->d_tab.p + f->tab_off[(p * nref + q) * 2 + sp]);
             }
+    if (f->d_tab32.p) lower_tables(f);
     HIPCHK(hipGetLastError());
     f->build_gen = f->c->build_gen;
     f->fock_gen = f->c->fock_gen;
@@ -468,7 +519,7 @@ void check_current(const nw_factor *f)
         throw NwError(NW_E_STATE, "the context was rebuilt since nw_factor_create");
 }
 
-// Pointer of the (q, p, sp) table: resident, or regenerated into the scratch slot `slot`.
+// Pointer of the FP64 (q, p, sp) table: resident, or regenerated into `scratch`.
 double2 *table_ptr(nw_factor *f, int q, int p, int sp, double2 *scratch)
 {
     if (f->storage == NW_FACTOR_DEVICE) return f->d_tab.p + f->tab_off[(q * f->nref + p) * 2 + sp];
@@ -479,14 +530,19 @@ double2 *table_ptr(nw_factor *f, int q, int p, int sp, double2 *scratch)
 // Stage-1 items and stage-2 rows of every ordered pair.  Resident tables:
 // the materialisation buffer holds one target's pairs at a time (pairs of a
 // target are contiguous); regenerated tables: one pair at a time, both
-// tables in the scratch table slot.
-void make_plans(nw_factor *f)
+// tables in the scratch table slot.  lo: pointers into the FP32 tables and
+// float2 materialised rows.
+void make_plans(nw_factor *f, int lo)
 {
     const int nref = f->nref;
     std::vector<S1Item> items;
     std::vector<S2Row> rows;
     size_t mat_need = 0;
     const bool dev = f->storage == NW_FACTOR_DEVICE;
+    const size_t es = elem_bytes(lo);
+    const char *tbase = dev && lo ? (const char *)f->d_tab32.p : (const char *)f->d_tab.p;
+    char *mbase = (char *)f->d_mat.p;
+    f->plans.clear();
     for (int q = 0; q < nref; ++q) {
         const ParentSpace &Q = f->par[q];
         if (Q.nt == 0) continue;
@@ -498,30 +554,30 @@ void make_plans(nw_factor *f)
             pl.q = q;
             pl.p = p;
             const int nta = Q.ncomp[0], ntb = Q.ncomp[1];
-            const double2 *taba, *tabb;
+            const char *taba, *tabb;
             if (dev) {
-                taba = f->d_tab.p + f->tab_off[(q * nref + p) * 2 + 0];
-                tabb = f->d_tab.p + f->tab_off[(q * nref + p) * 2 + 1];
+                taba = tbase + es * (size_t)f->tab_off[(q * nref + p) * 2 + 0];
+                tabb = tbase + es * (size_t)f->tab_off[(q * nref + p) * 2 + 1];
             } else {
-                taba = f->d_tab.p;
-                tabb = f->d_tab.p + table_elems(f, q, p, 0);
+                taba = tbase;
+                tabb = tbase + es * table_elems(f, q, p, 0);
             }
             size_t mo = dev ? mat_q : 0;
             pl.item0 = (long long)items.size();
             pl.row0 = (long long)rows.size();
             for (const Group &g : P.groups) {
                 if (g.m0 < 0) {
-                    rows.push_back({taba + (size_t)g.single_a * nta, tabb + (size_t)g.single_b * ntb,
+                    rows.push_back({taba + es * (size_t)g.single_a * nta, tabb + es * (size_t)g.single_b * ntb,
                                     (long long)g.single_det});
                     continue;
                 }
                 const int len = g.side == 1 ? nta : ntb;
                 for (int e = g.m0; e < g.m1; e += S1_CHUNK) {
-                    double2 *out = f->d_mat.p + mo;
+                    char *out = mbase + es * mo;
                     items.push_back({g.side == 1 ? taba : tabb, out, len, (int)(P.m_off + e),
                                      (int)(P.m_off + std::min(g.m1, e + S1_CHUNK)), 0});
-                    if (g.side == 1) rows.push_back({out, tabb + (size_t)g.key * ntb, -1});
-                    else rows.push_back({taba + (size_t)g.key * nta, out, -1});
+                    if (g.side == 1) rows.push_back({out, tabb + es * (size_t)g.key * ntb, -1});
+                    else rows.push_back({taba + es * (size_t)g.key * nta, out, -1});
                     mo += (size_t)len;
                 }
             }
@@ -534,9 +590,11 @@ void make_plans(nw_factor *f)
         }
         if (dev) mat_need = std::max(mat_need, mat_q);
     }
-    f->d_items.upload(items);
-    f->d_rows.upload(rows);
+    // the float2 rows of an FP32 plan take half of the double2 buffer
     if (mat_need > f->d_mat.n) throw NwError(NW_E_STATE, "factor plan exceeds its materialisation buffer");
+    f->d_items[lo].upload(items);
+    f->d_rows[lo].upload(rows);
+    f->planned[lo] = true;
 }
 
 size_t mat_elems_needed(const nw_factor *f)
@@ -670,7 +728,7 @@ extern "C" int nw_factor_create(nw_ctx *c, const nw_det *dets, uint64_t ndet, in
         f->d_tab.alloc(std::max<size_t>(1, storage == NW_FACTOR_DEVICE ? off : maxpair));
         f->d_mat.alloc(std::max<size_t>(1, mat_elems_needed(f.get())));
         f->d_part2.alloc((size_t)64 * (size_t)std::max(1, f->maxnt));
-        make_plans(f.get());
+        make_plans(f.get(), 0);
         if (storage == NW_FACTOR_DEVICE) build_tables(f.get());
         else {
             f->build_gen = c->build_gen;
@@ -698,8 +756,10 @@ extern "C" int nw_factor_refresh(nw_factor *f)
 
 namespace {
 
-// Stage 1 over items [i0, i1), stage 2 over rows [r0, r1) for the targets of parent q.
-void run_range(nw_factor *f, int q, long long i0, long long i1, long long r0, long long r1, const double *dx,
+// Stage 1 over items [i0, i1), stage 2 over rows [r0, r1) for the targets of
+// parent q, on tables of type T2 (plan lo).
+template <class T2>
+void run_range(nw_factor *f, int lo, int q, long long i0, long long i1, long long r0, long long r1, const double *dx,
                double lambda, double *dy, double *dys)
 {
     nw_ctx *c = f->c;
@@ -708,8 +768,8 @@ void run_range(nw_factor *f, int q, long long i0, long long i1, long long r0, lo
     const unsigned gx = (blocks_of(maxlen, 256) + 7u) & ~7u;
     for (long long b = i0; b < i1; b += 65535) {
         const unsigned ni = (unsigned)std::min<long long>(65535, i1 - b);
-        hipLaunchKernelGGL(k_fac_mat, dim3(gx, ni), dim3(256), 0, c->st, f->d_items.p + b, f->d_midx.p, f->d_mdet.p,
-                           dx);
+        hipLaunchKernelGGL(k_fac_mat<T2>, dim3(gx, ni), dim3(256), 0, c->st, f->d_items[lo].p + b, f->d_midx.p,
+                           f->d_mdet.p, dx);
     }
     const int nr = (int)(r1 - r0);
     if (nr == 0) return;
@@ -718,10 +778,17 @@ void run_range(nw_factor *f, int q, long long i0, long long i1, long long r0, lo
     nk = std::min(nk, std::max(1, nr / 16));
     const int chunk = (nr + nk - 1) / nk;
     nk = (nr + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_fac_rows, dim3(blocks_of(Q.nt, 256), (unsigned)nk), dim3(256), 0, c->st, Q.d_ta.p, Q.d_tb.p,
-                       Q.nt, f->d_rows.p + r0, nr, chunk, dx, lambda, f->d_part2.p);
+    hipLaunchKernelGGL(k_fac_rows<T2>, dim3(blocks_of(Q.nt, 256), (unsigned)nk), dim3(256), 0, c->st, Q.d_ta.p,
+                       Q.d_tb.p, Q.nt, f->d_rows[lo].p + r0, nr, chunk, dx, lambda, f->d_part2.p);
     hipLaunchKernelGGL(k_fac_scatter, dim3(blocks_of(Q.nt, 256)), dim3(256), 0, c->st, Q.d_tdet.p, Q.nt, nk,
                        f->d_part2.p, dy, dys);
+}
+
+void run_range_any(nw_factor *f, int lo, int q, long long i0, long long i1, long long r0, long long r1,
+                   const double *dx, double lambda, double *dy, double *dys)
+{
+    if (lo) run_range<float2>(f, 1, q, i0, i1, r0, r1, dx, lambda, dy, dys);
+    else run_range<double2>(f, 0, q, i0, i1, r0, r1, dx, lambda, dy, dys);
 }
 
 }  // namespace
@@ -742,22 +809,24 @@ extern "C" int nw_factor_apply(nw_factor *f, double lambda, const double *x, dou
         hipLaunchKernelGGL(k_fac_zero, dim3(blocks_of(n, 256)), dim3(256), 0, c->st, dy, n);
         if (dys) hipLaunchKernelGGL(k_fac_zero, dim3(blocks_of(n, 256)), dim3(256), 0, c->st, dys, n);
         float regen = 0.f;
+        const int lo = f->prec == NW_PREC_MIXED ? 1 : 0;
         if (f->storage == NW_FACTOR_DEVICE) {
             // the pairs of one target are contiguous in the plan
             for (size_t k = 0; k < f->plans.size();) {
                 size_t j = k;
                 while (j < f->plans.size() && f->plans[j].q == f->plans[k].q) ++j;
-                run_range(f, f->plans[k].q, f->plans[k].item0, f->plans[j - 1].item1, f->plans[k].row0,
-                          f->plans[j - 1].row1, dx, lambda, dy, dys);
+                run_range_any(f, lo, f->plans[k].q, f->plans[k].item0, f->plans[j - 1].item1, f->plans[k].row0,
+                              f->plans[j - 1].row1, dx, lambda, dy, dys);
                 k = j;
             }
         } else {
             for (const PairPlan &pl : f->plans) {
                 HIPCHK(hipEventRecord(c->ev[2], c->st));
-                launch_table(f, pl.q, pl.p, 0, f->d_tab.p);
-                launch_table(f, pl.q, pl.p, 1, f->d_tab.p + table_elems(f, pl.q, pl.p, 0));
+                char *slot = (char *)f->d_tab.p;
+                launch_table(f, pl.q, pl.p, 0, slot, nullptr, lo);
+                launch_table(f, pl.q, pl.p, 1, slot + elem_bytes(lo) * table_elems(f, pl.q, pl.p, 0), nullptr, lo);
                 HIPCHK(hipEventRecord(c->ev[3], c->st));
-                run_range(f, pl.q, pl.item0, pl.item1, pl.row0, pl.row1, dx, lambda, dy, dys);
+                run_range_any(f, lo, pl.q, pl.item0, pl.item1, pl.row0, pl.row1, dx, lambda, dy, dys);
                 HIPCHK(hipStreamSynchronize(c->st));
                 regen += elapsed(c->ev[2], c->ev[3]);
             }
@@ -809,8 +878,30 @@ extern "C" int nw_factor_info(const nw_factor *f, int parent, uint64_t *ncomp_a,
         if (ncomp_a) *ncomp_a = (uint64_t)f->par[parent].ncomp[0];
         if (ncomp_b) *ncomp_b = (uint64_t)f->par[parent].ncomp[1];
     }
-    if (table_bytes) *table_bytes = (uint64_t)f->tab_elems * sizeof(double2);
+    if (table_bytes) *table_bytes = (uint64_t)f->tab_elems * elem_bytes(f->prec == NW_PREC_MIXED);
     return NW_OK;
+}
+
+extern "C" int nw_factor_set_precision(nw_factor *f, int precision)
+{
+    if (!f) return NW_E_ARG;
+    nw_ctx *c = f->c;
+    return guarded(c, [&]() -> int {
+        if (precision != NW_PREC_FP64 && precision != NW_PREC_MIXED)
+            return fail(c, NW_E_ARG, "factor precision must be NW_PREC_FP64 or NW_PREC_MIXED");
+        if (precision == NW_PREC_MIXED) {
+            check_current(f);
+            if (f->storage == NW_FACTOR_DEVICE && !f->d_tab32.p) {
+                f->d_tab32.alloc(std::max<size_t>(1, f->tab_elems));
+                lower_tables(f);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipStreamSynchronize(c->st));
+            }
+            if (!f->planned[1]) make_plans(f, 1);
+        }
+        f->prec = precision;
+        return NW_OK;
+    });
 }
 
 extern "C" int nw_factor_ids(const nw_factor *f, uint32_t *aids, uint32_t *bids)
@@ -831,6 +922,23 @@ extern "C" int nw_factor_table(nw_factor *f, int target, int source, int spin, d
         check_current(f);
         const size_t ne = table_elems(f, target, source, spin);
         if (ne == 0) return NW_OK;
+        if (f->prec == NW_PREC_MIXED) {
+            // the FP32 values apply reads, widened to FP64
+            const float2 *src32;
+            if (f->storage == NW_FACTOR_DEVICE) src32 = f->d_tab32.p + f->tab_off[(target * f->nref + source) * 2 + spin];
+            else {
+                launch_table(f, target, source, spin, f->d_tab.p, nullptr, 1);
+                src32 = (const float2 *)f->d_tab.p;
+            }
+            DVec<double2> wide;
+            wide.alloc(ne);
+            hipLaunchKernelGGL(k_fac_widen, dim3(blocks_of((long long)ne, 256)), dim3(256), 0, c->st, src32, wide.p,
+                               (long long)ne);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(out, wide.p, ne * sizeof(double2), hipMemcpyDefault, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            return NW_OK;
+        }
         const double2 *src = table_ptr(f, target, source, spin, f->d_tab.p);
         HIPCHK(hipMemcpyAsync(out, src, ne * sizeof(double2), hipMemcpyDefault, c->st));
         HIPCHK(hipStreamSynchronize(c->st));

@@ -36,7 +36,8 @@ EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matri
            "nw_batch_last_error", "nw_batch_destroy", "nw_mlp_create", "nw_mlp_num_params", "nw_mlp_set_params",
            "nw_mlp_get_params", "nw_mlp_forward", "nw_mlp_grad", "nw_mlp_step", "nw_mlp_train", "nw_mlp_last_error",
            "nw_mlp_destroy", "nw_factor_create", "nw_factor_refresh", "nw_factor_apply", "nw_factor_diagonals",
-           "nw_factor_info", "nw_factor_ids", "nw_factor_table", "nw_factor_last_ms", "nw_factor_destroy"]
+           "nw_factor_info", "nw_factor_ids", "nw_factor_table", "nw_factor_last_ms", "nw_factor_destroy",
+           "nw_factor_set_precision"]
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_BF16X3 = 2
@@ -111,6 +112,7 @@ def load():
     L.nw_factor_last_ms.argtypes = [vp, dp]
     L.nw_factor_destroy.argtypes = [vp]
     L.nw_factor_destroy.restype = None
+    L.nw_factor_set_precision.argtypes = [vp, ct.c_int]
     L.nw_batch_create.argtypes = [ct.POINTER(vp), ct.c_int]
     L.nw_batch_add.argtypes = [vp, vp, vp, ct.c_uint64, ct.c_int, ct.POINTER(ct.c_int)]
     L.nw_batch_plan.argtypes = [vp, ct.c_int]
@@ -337,12 +339,18 @@ class Factor:
         sync_torch(dets)
         eng._check(self.L.nw_factor_create(eng.ctx, _ptr(dets), self.n, int(mode), int(storage), ct.byref(h)))
         self.h = h
+        self.precision = PREC_FP64
 
     def _check(self, rc):
         self.eng._check(rc)
 
     def refresh(self):
         self._check(self.L.nw_factor_refresh(self.h))
+
+    def set_precision(self, precision):
+        """Tables apply reads: PREC_FP64 or PREC_MIXED (FP32 tables, FP64 sums)."""
+        self._check(self.L.nw_factor_set_precision(self.h, int(precision)))
+        self.precision = int(precision)
 
     def apply(self, lam, x, y=None, ys=None):
         """y = (F + lam S) x, and ys = S x if ys is given; arrays may be torch device tensors."""

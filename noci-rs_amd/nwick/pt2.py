@@ -94,22 +94,53 @@ def fock(h, eri, Da, Db):
     return np.tril(Fa) + np.tril(Fa, -1).T, np.tril(Fb) + np.tril(Fb, -1).T
 
 
-class Preconditioner:
-    """Diagonal or rank-2 Woodbury preconditioner (src/snoci/types.rs:161-279), real case."""
+def omega_m_diag(m_diag, proj):
+    """Diagonal of M^Omega from the unprojected diagonal of M
+    (build_omega_m_diag, src/snoci/operators.rs:1032-1041)."""
+    return (m_diag - proj["f_a0"] * proj["s_0a"] - proj["s_a0"] * proj["f_0a"]
+            + 2.0 * proj["e0"] * proj["s_a0"] * proj["s_0a"])
 
-    def __init__(self, torch, m_diag, proj, kind="diag"):
+
+def omega_s_diag(s_diag, proj):
+    """Diagonal of Q = S - s_a0 s_0a^T (build_omega_s_diag, operators.rs:1049-1054)."""
+    return s_diag - proj["s_a0"] * proj["s_0a"]
+
+
+class Preconditioner:
+    """build_preconditioner (src/snoci/operators.rs:954-985) with
+    Preconditioner::new / apply (src/snoci/types.rs:161-279).
+
+    m_diag, s_diag: unprojected diagonals of M and S.  kind "diag": the
+    inverse of the diagonal of M^Omega + i eps Q.  Other kinds: the rank-2
+    Woodbury inverse of diag(M + i eps S) plus the projection's two rank-1
+    terms.  Diagonal entries below 1e-12 of the largest are regularised as
+    conj(d) / floor^2.  Complex when imag_shift != 0."""
+
+    def __init__(self, torch, m_diag, proj, kind="diag", imag_shift=0.0, s_diag=None):
         self.t = torch
-        dmax = float(m_diag.abs().max())
+        iq = 0.0
+        if imag_shift != 0.0:
+            if s_diag is None:
+                raise ValueError("an imaginary shift needs the S diagonal")
+            iq = 1j * imag_shift
+            diag = (omega_m_diag(m_diag, proj) + iq * omega_s_diag(s_diag, proj) if kind == "diag"
+                    else m_diag + iq * s_diag)
+        else:
+            diag = omega_m_diag(m_diag, proj) if kind == "diag" else m_diag
+        dmax = float(diag.abs().max())
         dfloor = max(1e-12 * dmax, 1e-14)
-        self.dinv = torch.where(m_diag.abs() > dfloor, 1.0 / m_diag, m_diag / (dfloor * dfloor))
+        self.dinv = torch.where(diag.abs() > dfloor, 1.0 / diag, diag.conj() / (dfloor * dfloor))
         self.active = False
         if kind != "diag":
-            u0 = -proj["f_a0"] + 2.0 * proj["e0"] * proj["s_a0"]
-            u1 = -proj["s_a0"]
-            self.v0, self.v1 = proj["s_0a"], proj["f_0a"]
+            dt = diag.dtype
+            s_a0, f_a0 = proj["s_a0"].to(dt), proj["f_a0"].to(dt)
+            u0 = -f_a0 + (2.0 * proj["e0"] - iq) * s_a0
+            u1 = -s_a0
+            self.v0, self.v1 = proj["s_0a"].to(dt), proj["f_0a"].to(dt)
             self.z0, self.z1 = self.dinv * u0, self.dinv * u1
-            c00 = 1.0 + float(self.v0 @ self.z0); c01 = float(self.v0 @ self.z1)
-            c10 = float(self.v1 @ self.z0); c11 = 1.0 + float(self.v1 @ self.z1)
+            # unconjugated contractions (bilinear_dot)
+            c00 = 1.0 + (self.v0 @ self.z0).item(); c01 = (self.v0 @ self.z1).item()
+            c10 = (self.v1 @ self.z0).item(); c11 = 1.0 + (self.v1 @ self.z1).item()
             det = c00 * c11 - c01 * c10
             self.active = abs(det) > 1e-14
             if self.active:
@@ -119,19 +150,36 @@ class Preconditioner:
         y = v * self.dinv
         if not self.active:
             return y
-        t0, t1 = float(self.v0 @ y), float(self.v1 @ y)
+        t0, t1 = (self.v0 @ y).item(), (self.v1 @ y).item()
         w00, w01, w10, w11 = self.w
         return y - self.z0 * (w00 * t0 + w01 * t1) - self.z1 * (w10 * t0 + w11 * t1)
 
 
-def gmres(torch, apply, precondition, b, res_tol=1e-8, restart=30, max_iter=300, log=None):
+def gmres(torch, apply, precondition, b, res_tol=1e-8, restart=30, max_iter=300, log=None, apply_true=None):
     """Right-preconditioned restarted GMRES (src/snoci/gmres.rs:406-544):
     modified Gram-Schmidt with two passes, Givens rotations, convergence on
-    the RMS of the true residual ||b - A x|| / sqrt(n) recomputed at restarts."""
+    the RMS of the true residual ||b - A x|| / sqrt(n) recomputed at restarts.
+
+    apply_true: the operator of the true residual when it differs from the
+    Arnoldi operator `apply` (a lower-precision copy of it): every restart is
+    then an iterative-refinement step, b - A x in the higher precision and a
+    Krylov correction built with the lower one.
+
+    Real or complex b: inner products conjugate their first argument and the
+    Givens rotations are the complex ones of the reference (cs real, sn =
+    cs conj(y) / conj(x)); both reduce to the real forms for real b."""
+    if apply_true is None:
+        apply_true = apply
     n = b.shape[0]
     rms = math.sqrt(n)
+    cplx = b.is_complex()
+    dt = complex if cplx else float
+    if cplx:
+        dot = lambda u, v: torch.vdot(u, v).item()
+    else:
+        dot = lambda u, v: float(u @ v)
     x = torch.zeros_like(b)
-    rtrue = b - apply(x)
+    rtrue = b.clone()  # b - A x for the zero start
     res = float(torch.linalg.vector_norm(rtrue)) / rms
     total = 0
     if res <= res_tol:
@@ -142,15 +190,15 @@ def gmres(torch, apply, precondition, b, res_tol=1e-8, restart=30, max_iter=300,
             return x, beta / rms, total, beta / rms <= res_tol
         inner = min(restart, max_iter - total)
         q = [rtrue / beta]
-        Hm = np.zeros((inner + 1, inner))
-        cs = np.zeros(inner); sn = np.zeros(inner)
-        g = np.zeros(inner + 1); g[0] = beta
+        Hm = np.zeros((inner + 1, inner), dtype=dt)
+        cs = np.zeros(inner); sn = np.zeros(inner, dtype=dt)
+        g = np.zeros(inner + 1, dtype=dt); g[0] = beta
         kfinal = 0
         for k in range(inner):
             w = apply(precondition(q[k]))
             for _ in range(2):
                 for j in range(k + 1):
-                    hjk = float(q[j] @ w)
+                    hjk = dot(q[j], w)
                     Hm[j, k] += hjk
                     w = w - hjk * q[j]
             hn = float(torch.linalg.vector_norm(w))
@@ -159,36 +207,39 @@ def gmres(torch, apply, precondition, b, res_tol=1e-8, restart=30, max_iter=300,
                 q.append(w / hn)
             for j in range(k):
                 tmp = cs[j] * Hm[j, k] + sn[j] * Hm[j + 1, k]
-                Hm[j + 1, k] = -sn[j] * Hm[j, k] + cs[j] * Hm[j + 1, k]
+                Hm[j + 1, k] = -np.conj(sn[j]) * Hm[j, k] + cs[j] * Hm[j + 1, k]
                 Hm[j, k] = tmp
             xk, yk = Hm[k, k], Hm[k + 1, k]
-            den = math.hypot(xk, yk)
+            ax, ay = abs(xk), abs(yk)
+            den = math.sqrt(ax * ax + ay * ay)
             if den > SMALL:
-                cs[k] = abs(xk) / den
-                sn[k] = cs[k] * yk / xk if abs(xk) > SMALL else 1.0
+                cs[k] = ax / den
+                sn[k] = cs[k] * np.conj(yk) / np.conj(xk) if ax > SMALL else 1.0
             else:
                 cs[k], sn[k] = 1.0, 0.0
             h0, h1 = Hm[k, k], Hm[k + 1, k]
             Hm[k, k] = cs[k] * h0 + sn[k] * h1
-            Hm[k + 1, k] = -sn[k] * h0 + cs[k] * h1
+            Hm[k + 1, k] = -np.conj(sn[k]) * h0 + cs[k] * h1
             g0, g1 = g[k], g[k + 1]
             g[k] = cs[k] * g0 + sn[k] * g1
-            g[k + 1] = -sn[k] * g0 + cs[k] * g1
+            g[k + 1] = -np.conj(sn[k]) * g0 + cs[k] * g1
             kfinal = k + 1
             est = abs(g[k + 1]) / rms
             if log:
                 log(f"  gmres iter {total + k + 1}: residual estimate {est:.3e}")
             if est <= res_tol or hn <= SMALL:
                 break
-        y = np.zeros(kfinal)
+        y = np.zeros(kfinal, dtype=dt)
         for i in reversed(range(kfinal)):
             rhs = g[i] - sum(Hm[i, j] * y[j] for j in range(i + 1, kfinal))
             y[i] = rhs / Hm[i, i] if abs(Hm[i, i]) > SMALL else 0.0
         for j in range(kfinal):
-            x = x + y[j] * precondition(q[j])
+            x = x + dt(y[j]) * precondition(q[j])
         total += kfinal
-        rtrue = b - apply(x)
+        rtrue = b - apply_true(x)
         res = float(torch.linalg.vector_norm(rtrue)) / rms
+        if log:
+            log(f"  gmres restart after {total} iterations: true residual {res:.3e}")
         if res <= res_tol:
             return x, res, total, True
     return x, res, total, res <= res_tol
@@ -200,13 +251,20 @@ class PT2:
     eng must hold a build covering every ordered reference pair the
     candidate-current and candidate-candidate pairs touch."""
 
-    def __init__(self, eng, sysm, current, candidates, tol=1e-8, mode=0, device="cuda:0", factor_tables="device"):
+    def __init__(self, eng, sysm, current, candidates, tol=1e-8, mode=0, device="cuda:0", factor_tables="device",
+                 precision="fp64"):
         """factor_tables: "device" (tables resident in device memory, the
         reference's default snoci.gmres.factor_tables = RAM with full_m = None,
         src/input/snoci.rs:25-33), "transient" (regenerated per matvec) or None
         (pairwise on-the-fly matvec, nw_matvec_m).  Candidate lists whose
         per-spin ranks exceed the factor tables' limit, or whose tables do not
-        fit in device memory, use the pairwise matvec."""
+        fit in device memory, use the pairwise matvec.
+
+        precision: "fp64" (the reference's arithmetic) or "mixed": GMRES runs
+        on the factorised operator with FP32 tables (FP64 sums, half the table
+        bytes per matvec) inside FP64 iterative refinement, whose residuals
+        and E_PT2 use the FP64 tables (see solve).  "mixed" needs the
+        factorised operator."""
         import torch
         self.torch, self.eng, self.sysm = torch, eng, sysm
         self.current, self.candidates = list(current), list(candidates)
@@ -214,7 +272,11 @@ class PT2:
         self.dev = torch.device(device)
         self.timings = {}
         self.factor_tables = factor_tables
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
         self.fac = None
+        self.nmatvec = self.nmatvec_lo = 0
 
     def setup(self):
         torch, eng = self.torch, self.eng
@@ -261,39 +323,106 @@ class PT2:
         if self.fac is not None:
             # build_factorised_candidate_diags (src/snoci/operators.rs:354-365)
             md = torch.empty(nc, dtype=torch.float64, device=self.dev)
-            self.fac.diagonals(-self.e0, mdiag=md, sdiag=torch.empty_like(md))
-            self.m_diag = md
+            sd = torch.empty_like(md)
+            self.fac.diagonals(-self.e0, mdiag=md, sdiag=sd)
+            self.m_diag, self.s_diag = md, sd
         else:
-            self.m_diag = T(f_aa - self.e0 * s_aa)
+            self.m_diag, self.s_diag = T(f_aa - self.e0 * s_aa), T(s_aa)
         self.operator = "factorised" if self.fac is not None else "pairwise"
+        if self.precision == "mixed":
+            if self.fac is None:
+                raise RuntimeError("mixed-precision PT2 needs the factorised operator")
+            from . import PREC_FP64, PREC_MIXED
+            self.fac.set_precision(PREC_MIXED)  # FP32 table copies
+            self.fac.set_precision(PREC_FP64)
         self.timings["setup_s"] = time.perf_counter() - t0
 
-    def apply(self, x):
-        """M^Omega x (apply_omega_m with the projection correction)."""
-        x = x.contiguous()
-        p = self.proj
-        sx, fx = float(p["s_0a"] @ x), float(p["f_0a"] @ x)
+    def _m(self, x, s_out=None):
+        """(F - E0 S) x for a real vector, and S x into s_out if given."""
         y = self.torch.empty_like(self.y)
         if self.fac is not None:
-            # apply_factorised_shifted_omega_m (src/snoci/operators.rs:762-778)
-            self.fac.apply(-self.e0, x, y=y)
+            self.fac.apply(-self.e0, x, y=y, ys=s_out)
         else:
-            self.eng.matvec_m(self.cand_rec, self.e0, x, y=y, mode=self.mode)
-        self.nmatvec += 1
-        return y - p["f_a0"] * sx - p["s_a0"] * fx + 2.0 * self.e0 * p["s_a0"] * sx
+            self.eng.matvec_m(self.cand_rec, self.e0, x, y=y, ys=s_out, mode=self.mode)
+        return y
 
-    def solve(self, res_tol=1e-8, restart=30, max_iter=300, preconditioner="diag", log=None):
+    def apply(self, x, imag_shift=0.0):
+        """(M^Omega + i eps Q) x: apply_factorised_shifted_omega_m
+        (src/snoci/operators.rs:759-776) or, pairwise, apply_shifted_omega_m
+        (:856-887), with the projection correction (:938-947).  A complex x
+        takes one real application per part; eps != 0 also uses S x."""
         torch = self.torch
-        self.nmatvec = 0
-        prec = Preconditioner(torch, self.m_diag, self.proj, preconditioner)
+        p = self.proj
+        shift = imag_shift != 0.0
+        if x.is_complex():
+            parts = []
+            for comp in (x.real, x.imag):
+                sv = torch.empty_like(self.y) if shift else None
+                parts.append((self._m(comp.contiguous(), sv), sv))
+            y = torch.complex(parts[0][0], parts[1][0])
+            svec = torch.complex(parts[0][1], parts[1][1]) if shift else None
+        else:
+            x = x.contiguous()
+            svec = torch.empty_like(self.y) if shift else None
+            y = self._m(x, svec)
+        self.nmatvec += 1
+        if self.fac is not None and self.fac.precision != 0:
+            self.nmatvec_lo += 1
+        if not x.is_complex() and not shift:
+            sx, fx = float(p["s_0a"] @ x), float(p["f_0a"] @ x)
+            return y - p["f_a0"] * sx - p["s_a0"] * fx + 2.0 * self.e0 * p["s_a0"] * sx
+        dt = torch.complex128
+        if getattr(self, "_proj_c", None) is None:
+            self._proj_c = {k: p[k].to(dt) for k in ("s_a0", "f_a0", "s_0a", "f_0a")}
+        pc = self._proj_c
+        xc, y = x.to(dt), y.to(dt)
+        sx, fx = (pc["s_0a"] @ xc).item(), (pc["f_0a"] @ xc).item()
+        y = y - pc["f_a0"] * sx - pc["s_a0"] * fx + (2.0 * self.e0 - 1j * imag_shift) * pc["s_a0"] * sx
+        if shift:
+            y = y + (1j * imag_shift) * svec.to(dt)
+        return y
+
+    def solve(self, res_tol=1e-8, restart=30, max_iter=300, preconditioner="diag", log=None, imag_shift=0.0):
+        """Solve (M^Omega + i eps Q) a = -V^Omega to an RMS residual of
+        res_tol with restarted GMRES (gmres.rs:406-544) and return E_PT2 =
+        Re(a^H (M^Omega + i eps Q) a + a^H V + V^H a) (step.rs:366-429); eps
+        = imag_shift (snoci imag_shifts), complex a when eps != 0.
+        precision "mixed": the Arnoldi matvecs read the FP32 tables, the true
+        residual at each restart (the convergence test) and E_PT2 use the
+        FP64 tables."""
+        torch = self.torch
+        self.nmatvec = self.nmatvec_lo = 0
+        prec = Preconditioner(torch, self.m_diag, self.proj, preconditioner, imag_shift, self.s_diag)
+        b = -self.v_omega
+        if imag_shift != 0.0:
+            b = b.to(torch.complex128)
+        op = lambda v: self.apply(v, imag_shift)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        a, res, iters, conv = gmres(torch, self.apply, prec, -self.v_omega, res_tol, restart, max_iter, log)
-        ma = self.apply(a)
+        if self.precision == "fp64":
+            a, res, iters, conv = gmres(torch, op, prec, b, res_tol, restart, max_iter, log)
+        else:
+            from . import PREC_FP64, PREC_MIXED
+
+            def apply_fp32(v):
+                self.fac.set_precision(PREC_MIXED)
+                try:
+                    return self.apply(v, imag_shift)
+                finally:
+                    self.fac.set_precision(PREC_FP64)
+
+            a, res, iters, conv = gmres(torch, apply_fp32, prec, b, res_tol, restart, max_iter, log,
+                                        apply_true=op)
+        ma = op(a)
         torch.cuda.synchronize()
-        ept2 = float(a @ ma) + 2.0 * float(a @ self.v_omega)
+        if a.is_complex():
+            v = self.v_omega.to(a.dtype)
+            ept2 = float((torch.vdot(a, ma) + torch.vdot(a, v) + torch.vdot(v, a)).real)
+        else:
+            ept2 = float(a @ ma) + 2.0 * float(a @ self.v_omega)
         self.timings["gmres_s"] = time.perf_counter() - t0
         return {"e_noci": self.ecurrent, "e0": self.e0, "ept2": ept2, "e_total": self.ecurrent + ept2,
                 "residual_rms": res, "iterations": iters, "converged": conv, "matvecs": self.nmatvec,
+                "matvecs_fp32": self.nmatvec_lo, "precision": self.precision, "imag_shift": imag_shift,
                 "a": a}
 

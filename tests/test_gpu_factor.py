@@ -2,8 +2,10 @@
 against the oracle restatement (oracle/spin_factor.py) and the pairwise
 matvec (nw_matvec_m): component ids, per-spin tables for m = 0 and m > 0
 reference pairs in both arithmetic modes, y = (F + lambda S) x and S x with
-resident and regenerated tables, diagonals, the stale-table guard, split
-source groups, and NOCI-PT2 energies through the factorised operator."""
+resident and regenerated tables, FP32 tables and their error bound,
+diagonals, the stale-table guard, split source groups, and NOCI-PT2
+energies through the factorised operator, in FP64 and with FP32 tables
+inside FP64 iterative refinement."""
 import numpy as np
 import pytest
 
@@ -79,6 +81,69 @@ def test_apply_vs_oracle_and_pairwise(fcase, mode, storage):
     yp, ysp = eng.matvec_m(rec, -lam, x, ys=np.empty(len(dets)), mode=mode)
     assert np.max(np.abs(y - yp)) < 1e-9 * scale
     assert np.max(np.abs(ys - ysp)) < 1e-9 * scale
+    f.close()
+
+
+@pytest.mark.parametrize("storage", [nwick.FACTOR_DEVICE, nwick.FACTOR_TRANSIENT])
+def test_fp32_tables_and_apply(fcase, storage):
+    """FP32 tables are the FP64 tables rounded to nearest (bit-exact copies
+    for resident tables; regenerated ones are evaluated in FP64 and rounded,
+    so within one FP32 ulp).  The FP32 apply differs from the FP64 apply by
+    at most 8 * 2^-24 of the absolute sum of the terms, |Fa Sb| + |Sa Fb| +
+    |lam Sa Sb| over |x| (table rounding on both factors plus the rounding
+    of materialised rows), computed through the oracle on absolute tables."""
+    sysm, dets, eng, slab, fac, fa, fb = fcase
+    rec = basis.to_records(dets)
+    f = eng.factor(rec, mode=1, storage=storage)
+    keys = [(q, p, sp) for q in range(sysm.nref) for p in range(sysm.nref) for sp in (0, 1)]
+    t64 = {k: f.table(*k) for k in keys}
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=len(dets))
+    lam = -1.9
+    y64, ys64 = f.apply(lam, x, ys=np.empty(len(dets)))
+    bytes64 = f.info()["table_bytes"]
+    f.set_precision(nwick.PREC_MIXED)
+    assert 2 * f.info()["table_bytes"] == bytes64
+    for k in keys:
+        got, want = f.table(*k), t64[k].astype(np.float32).astype(np.float64)
+        if storage == nwick.FACTOR_DEVICE:
+            assert np.array_equal(got, want), k
+        else:
+            assert np.all(np.abs(got - want) <= 2.0 ** -23 * np.abs(want)), k
+    y32, ys32 = f.apply(lam, x, ys=np.empty(len(dets)))
+    tabs = OF.tables(slab, dets, fac, 1)
+    yabs, ysabs = OF.apply(fac, {k: np.abs(v) for k, v in tabs.items()}, abs(lam), np.abs(x))
+    ey, es = np.abs(y32 - y64) / yabs, np.abs(ys32 - ys64) / ysabs
+    print(f"storage {storage}: FP32 apply error / absolute term sum: y {ey.max():.2e}, ys {es.max():.2e} "
+          f"(2^-24 = {2.0 ** -24:.2e})", flush=True)
+    assert np.all(np.abs(y32 - y64) <= 8 * 2.0 ** -24 * yabs + 1e-300)
+    assert np.all(np.abs(ys32 - ys64) <= 8 * 2.0 ** -24 * ysabs + 1e-300)
+    assert ey.max() > 0  # the FP32 tables were read
+    # back to FP64: the FP64 result again, bit for bit
+    f.set_precision(nwick.PREC_FP64)
+    y2 = f.apply(lam, x)
+    assert np.array_equal(y2, y64)
+    with pytest.raises(nwick.NwickError):
+        f.set_precision(nwick.PREC_BF16X3)
+    f.close()
+
+
+def test_fp32_tables_follow_refresh(fcase):
+    sysm, dets, eng, slab, fac, fa, fb = fcase
+    rec = basis.to_records(dets)
+    f = eng.factor(rec)
+    f.set_precision(nwick.PREC_MIXED)
+    x = np.random.default_rng(3).normal(size=len(dets))
+    eng.update_fock(fa, 3.0 * fb)
+    with pytest.raises(nwick.NwickError):
+        f.apply(0.5, x)
+    f.refresh()
+    y32 = f.apply(0.5, x)
+    f.set_precision(nwick.PREC_FP64)
+    y64 = f.apply(0.5, x)
+    scale = float(np.max(np.abs(y64)))
+    assert np.max(np.abs(y32 - y64)) < 1e-5 * scale
+    eng.update_fock(fa, fb)
     f.close()
 
 
@@ -159,3 +224,32 @@ def test_pt2_factorised_equals_pairwise():
         out[ft] = r["ept2"]
     assert abs(out["device"] - out[None]) < 1e-9
     assert abs(out["transient"] - out[None]) < 1e-9
+
+
+@pytest.mark.parametrize("storage", ["device", "transient"])
+def test_pt2_mixed_refinement_equals_fp64(storage):
+    """E_PT2 from GMRES whose Arnoldi matvecs read the FP32 tables and whose
+    restart residuals use the FP64 tables equals the FP64 E_PT2, and its
+    FP64 residual meets the tolerance.  This system is indefinite and
+    ill-conditioned, so each FP64 restart residual gains only a few digits
+    over the previous one; the iteration limit allows for that."""
+    from nwick.pt2 import PT2
+    sysm = synth.make_system(10, 2, 2, 3, seed=71, orthonormal_ao=False, n_aux=20)
+    refs = basis.excited_basis(3, 10, 2, 2, per_ref=0, include_refs=True, counts={})
+    allx = basis.excited_basis(3, 10, 2, 2, per_ref=None, include_refs=False)
+    cands = [d for d in allx if d.parent == 0 or (d.parent == 1 and d.rank_a + d.rank_b == 1)]
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    res = {}
+    for prec in ("fp64", "mixed"):
+        pt = PT2(eng, sysm, refs, cands, mode=1, factor_tables=storage, precision=prec)
+        pt.setup()
+        restarts = []
+        r = pt.solve(res_tol=1e-10, restart=400, max_iter=4000,
+                     log=lambda m: restarts.append(m.strip()) if "restart" in m else None)
+        print(f"{storage} {prec}: E_PT2 {r['ept2']:.12f} residual {r['residual_rms']:.2e} iters {r['iterations']} "
+              f"FP32 matvecs {r['matvecs_fp32']} of {r['matvecs']}; " + "; ".join(restarts), flush=True)
+        assert r["converged"] and r["residual_rms"] <= 1e-10
+        res[prec] = r
+    assert res["mixed"]["matvecs_fp32"] > 0 and res["fp64"]["matvecs_fp32"] == 0
+    assert abs(res["mixed"]["ept2"] - res["fp64"]["ept2"]) < 1e-9

@@ -1,5 +1,7 @@
 """GPU parity of the generalised-Fock / overlap path (NOCI-PT2 operator M = F - E0 S):
 Fock-intermediate refresh, F,S elements and the matrix-free matvec, against the oracle."""
+import functools
+
 import numpy as np
 import pytest
 
@@ -74,26 +76,17 @@ def test_fock_refresh_replaces_previous(pt2_case):
     assert np.max(np.abs(F2 - F1)) > 1e-3
 
 
-def test_pt2_gmres_vs_dense_solve():
-    """GPU PT2 (matrix-free matvec + GPU GMRES) against a dense solve built
-    from the oracle's F, S, H blocks with the same Fock matrices."""
-    from nwick.pt2 import PT2, fock, general_evp, noci_density
+@functools.lru_cache(maxsize=1)
+def _pt2_dense_case():
+    """A small PT2 problem on the GPU and its dense operators from the
+    oracle's F, S, H blocks with the same Fock matrices."""
+    from nwick.pt2 import fock, general_evp, noci_density
     sysm = synth.make_system(10, 2, 2, 3, seed=71, orthonormal_ao=False, n_aux=20)
     refs = basis.excited_basis(3, 10, 2, 2, per_ref=0, include_refs=True, counts={})
     allx = basis.excited_basis(3, 10, 2, 2, per_ref=None, include_refs=False)
     # ref-0 singles and doubles plus ref-1 singles: a linearly independent candidate space
     cands = [d for d in allx if d.parent == 0 or (d.parent == 1 and d.rank_a + d.rank_b == 1)]
     print(f"pt2 dense check: ncur={len(refs)} ncand={len(cands)}", flush=True)
-    eng = nwick.Engine.from_system(sysm)
-    eng.build()
-    pt = PT2(eng, sysm, refs, cands, mode=1)
-    pt.setup()
-    # the synthetic Fock operator makes M^Omega indefinite (cond ~2e4): restarted
-    # GMRES(30) stagnates, so the test runs unrestarted GMRES (restart >= n)
-    r = pt.solve(res_tol=1e-10, restart=400, max_iter=800)
-    print(f"pt2 gmres: converged={r['converged']} residual={r['residual_rms']:.3e} iters={r['iterations']}", flush=True)
-    assert r["converged"]
-    # dense reference
     slab = O.Slab(sysm)
     Hc, Sc = O.hs_matrix(slab, refs, mode=1)
     e, C = general_evp(Hc, Sc)
@@ -103,19 +96,66 @@ def test_pt2_gmres_vs_dense_solve():
     Fc, _ = O.fs_matrix(slab, refs, mode=1)
     e0 = c @ Fc @ c
     allm = refs + cands
-    nc, ncur = len(cands), len(refs)
+    ncur = len(refs)
     Fall, Sall = O.fs_matrix(slab, allm, mode=1)
     Hall, _ = O.hs_matrix(slab, allm, mode=1)
     F_ai, S_ai, H_ai = Fall[ncur:, :ncur], Sall[ncur:, :ncur], Hall[ncur:, :ncur]
     Faa, Saa = Fall[ncur:, ncur:], Sall[ncur:, ncur:]
     s0, f0 = S_ai @ c, F_ai @ c
     M = Faa - e0 * Saa - np.outer(f0, s0) - np.outer(s0, f0) + 2 * e0 * np.outer(s0, s0)
+    Q = Saa - np.outer(s0, s0)
     v = H_ai @ c - e[0] * s0
+    return sysm, refs, cands, {"e_noci": e[0], "e0": e0, "M": M, "Q": Q, "v": v}
+
+
+def test_pt2_gmres_vs_dense_solve():
+    """GPU PT2 (matrix-free matvec + GPU GMRES) against a dense solve."""
+    from nwick.pt2 import PT2
+    sysm, refs, cands, dn = _pt2_dense_case()
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    pt = PT2(eng, sysm, refs, cands, mode=1)
+    pt.setup()
+    # the synthetic Fock operator makes M^Omega indefinite (cond ~2e4): restarted
+    # GMRES(30) stagnates, so the test runs unrestarted GMRES (restart >= n)
+    r = pt.solve(res_tol=1e-10, restart=400, max_iter=800)
+    print(f"pt2 gmres: converged={r['converged']} residual={r['residual_rms']:.3e} iters={r['iterations']}", flush=True)
+    assert r["converged"]
+    M, v = dn["M"], dn["v"]
     a = np.linalg.solve(M, -v)
     ept2 = a @ M @ a + 2 * a @ v
     print(f"pt2: gpu {r['ept2']:.12f} dense {ept2:.12f} iters {r['iterations']}", flush=True)
-    assert abs(r["e0"] - e0) < 1e-10 and abs(r["e_noci"] - e[0]) < 1e-10
+    assert abs(r["e0"] - dn["e0"]) < 1e-10 and abs(r["e_noci"] - dn["e_noci"]) < 1e-10
     assert abs(r["ept2"] - ept2) < 1e-8
+
+
+@pytest.mark.parametrize("operator", ["factorised", "pairwise"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kind", ["diag", "woodbury"])
+def test_pt2_imag_shift_vs_dense_solve(operator, precision, kind):
+    """(M^Omega + i eps Q) a = -V with complex GMRES (imag_shifts) against a
+    dense complex solve: E_PT2 = Re(a^H (M + i eps Q) a + 2 a^H V)."""
+    from nwick.pt2 import PT2
+    if operator == "pairwise" and precision == "mixed":
+        pytest.skip("FP32 tables belong to the factorised operator")
+    sysm, refs, cands, dn = _pt2_dense_case()
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    pt = PT2(eng, sysm, refs, cands, mode=1, factor_tables="device" if operator == "factorised" else None,
+             precision=precision)
+    pt.setup()
+    eps = 0.5
+    r = pt.solve(res_tol=1e-10, restart=60, max_iter=3000, preconditioner=kind, imag_shift=eps)
+    Cm = dn["M"] + 1j * eps * dn["Q"]
+    v = dn["v"]
+    a = np.linalg.solve(Cm, -v)
+    ept2 = float(np.real(np.conj(a) @ Cm @ a + 2 * np.conj(a) @ v))
+    print(f"{operator} {precision} {kind} eps={eps}: gpu {r['ept2']:.12f} dense {ept2:.12f} residual "
+          f"{r['residual_rms']:.2e} iters {r['iterations']} matvecs {r['matvecs']} (FP32 {r['matvecs_fp32']})",
+          flush=True)
+    assert r["converged"] and r["a"].is_complex()
+    assert np.max(np.abs(r["a"].cpu().numpy() - a)) < 1e-7 * max(1.0, float(np.max(np.abs(a))))
+    assert abs(r["ept2"] - ept2) < 1e-9
 
 
 def test_noci_fock_matches_host_restatement():
