@@ -225,6 +225,13 @@ int nw_factor_refresh(nw_factor *f);
  * lambda = -E0 gives the PT2 operator of apply_factorised_shifted_omega_m
  * (src/snoci/operators.rs:762-778) before the projection correction. */
 int nw_factor_apply(nw_factor *f, double lambda, const double *x, double *y, double *ys);
+/* nw_factor_apply on two vectors at once, interleaved as x2[2i + v] (the
+ * real and imaginary parts of a complex vector, v = 0, 1), likewise y2 and
+ * ys2: each table row is read once for both.  This is the real-table action
+ * on a complex vector that apply_factorised_shifted_omega_m
+ * (src/snoci/operators.rs:759-776) takes with T = Complex64 for imaginary
+ * shifts; the i eps S x part is formed by the caller from ys2. */
+int nw_factor_apply_pair(nw_factor *f, double lambda, const double *x2, double *y2, double *ys2);
 /* Diagonals of F + lambda S and of S (one_body_diagonals, onebody.rs:358-405);
  * either output may be NULL. */
 int nw_factor_diagonals(nw_factor *f, double lambda, double *mdiag, double *sdiag);

@@ -59,17 +59,24 @@ static_assert(sizeof(SpinComp) == 12, "SpinComp layout");
 // Stage-1 work item: out[t] = sum_{k in [e0, e1)} x[mdet[k]] tab[midx[k] nt + t], t < nt.
 // Tables and materialised rows hold {S, F} pairs as double2 (FP64 tables) or
 // float2 (FP32 tables); the kernels take the element type as a template.
+// NV input vectors are applied together (x[i NV + v], v < NV; NV = 2 for the
+// real and imaginary parts of a complex vector): each table row is read once
+// for all of them, and a materialised row holds NV interleaved entries per
+// target component (out[t NV + v]).
 struct S1Item {
     const void *tab;
     void *out;
     int nt, e0, e1, pad;
 };
 // Stage-2 row: alpha-side row u (over target alpha components), beta-side row
-// v (over target beta components), weight x[wdet] (wdet >= 0) or 1.
+// v (over target beta components), weight x[wdet] (wdet >= 0) or 1; umat /
+// vmat mark a materialised row (NV entries per component) on that side.
 struct S2Row {
     const void *u, *v;
-    long long wdet;
+    int wdet;
+    uint8_t umat, vmat, pad[2];
 };
+static_assert(sizeof(S2Row) == 24, "S2Row layout");
 
 template <class T2>
 __device__ inline T2 pair2(double a, double b);
@@ -146,35 +153,44 @@ __global__ __launch_bounds__(256) void k_fac_table(const PairInfo *__restrict__ 
 // the same XCD (workgroups are dealt round-robin over the 8 XCDs) and the
 // table rows that several groups share are re-read from that XCD's L2.
 // Sums run in FP64 whatever the table type.
-template <class T2>
+template <class T2, int NV>
 __global__ __launch_bounds__(256) void k_fac_mat(const S1Item *__restrict__ items, const uint32_t *__restrict__ midx,
                                                  const uint32_t *__restrict__ mdet, const double *__restrict__ x)
 {
     __shared__ int s_r[S1_CHUNK];
-    __shared__ double s_x[S1_CHUNK];
+    __shared__ double s_x[NV][S1_CHUNK];
     const S1Item it = items[blockIdx.y];
     const int ne = it.e1 - it.e0;
     if ((int)threadIdx.x < ne) {
         s_r[threadIdx.x] = (int)midx[it.e0 + threadIdx.x];
-        s_x[threadIdx.x] = x[mdet[it.e0 + threadIdx.x]];
+        const size_t d = mdet[it.e0 + threadIdx.x];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s_x[v][threadIdx.x] = x[d * NV + v];
     }
     __syncthreads();
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= it.nt) return;
     const T2 *col = (const T2 *)it.tab + t;
-    double s = 0.0, f = 0.0;
+    double s[NV], f[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s[v] = f[v] = 0.0;
 #pragma unroll 8
     for (int k = 0; k < ne; ++k) {
-        const T2 v = col[(size_t)s_r[k] * it.nt];
-        s = fma(s_x[k], (double)v.x, s);
-        f = fma(s_x[k], (double)v.y, f);
+        const T2 e = col[(size_t)s_r[k] * it.nt];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            s[v] = fma(s_x[v][k], (double)e.x, s[v]);
+            f[v] = fma(s_x[v][k], (double)e.y, f[v]);
+        }
     }
-    ((T2 *)it.out)[t] = pair2<T2>(s, f);
+    T2 *o = (T2 *)it.out + (size_t)t * NV;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) o[v] = pair2<T2>(s[v], f[v]);
 }
 
 // Stage 2 over the rows [k chunk, (k+1) chunk): one lane per target entry;
 // FP64 sums whatever the table type.
-template <class T2>
+template <class T2, int NV>
 __global__ __launch_bounds__(256) void k_fac_rows(const uint32_t *__restrict__ ta, const uint32_t *__restrict__ tb,
                                                   int nt, const S2Row *__restrict__ rows, int nrows, int chunk,
                                                   const double *__restrict__ x, double lambda,
@@ -182,12 +198,15 @@ __global__ __launch_bounds__(256) void k_fac_rows(const uint32_t *__restrict__ t
 {
     __shared__ const T2 *s_u[S2_BATCH];
     __shared__ const T2 *s_v[S2_BATCH];
-    __shared__ double s_w[S2_BATCH];
+    __shared__ double s_w[NV][S2_BATCH];
+    __shared__ int s_f[S2_BATCH];  // bit 0: u materialised, bit 1: v materialised
     const int I = blockIdx.x * 256 + threadIdx.x;
     const bool live = I < nt;
     const int r0 = blockIdx.y * chunk, r1 = min(nrows, r0 + chunk);
     const uint32_t a = live ? ta[I] : 0, b = live ? tb[I] : 0;
-    double m = 0.0, s = 0.0;
+    double m[NV], s[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) m[v] = s[v] = 0.0;
     for (int base = r0; base < r1; base += S2_BATCH) {
         const int nb = min(S2_BATCH, r1 - base);
         __syncthreads();
@@ -195,39 +214,55 @@ __global__ __launch_bounds__(256) void k_fac_rows(const uint32_t *__restrict__ t
             const S2Row r = rows[base + threadIdx.x];
             s_u[threadIdx.x] = (const T2 *)r.u;
             s_v[threadIdx.x] = (const T2 *)r.v;
-            s_w[threadIdx.x] = r.wdet >= 0 ? x[r.wdet] : 1.0;
+            s_f[threadIdx.x] = (r.umat ? 1 : 0) | (r.vmat ? 2 : 0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) s_w[v][threadIdx.x] = r.wdet >= 0 ? x[(size_t)r.wdet * NV + v] : 1.0;
         }
         __syncthreads();
         if (live) {
 #pragma unroll 8
             for (int k = 0; k < nb; ++k) {
-                const T2 u = s_u[k][a];  // alpha side {S, F}
-                const T2 v = s_v[k][b];  // beta side {S, F}
-                const double w = s_w[k];
-                const double us = w * (double)u.x, uf = w * (double)u.y, vs = (double)v.x;
-                m = fma(uf, vs, m);
-                m = fma(us, fma(lambda, vs, (double)v.y), m);
-                s = fma(us, vs, s);
+                const int fl = NV > 1 ? s_f[k] : 0;
+                const T2 *U = s_u[k] + ((fl & 1) ? (size_t)a * NV : (size_t)a);  // alpha side {S, F}
+                const T2 *V = s_v[k] + ((fl & 2) ? (size_t)b * NV : (size_t)b);  // beta side {S, F}
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const T2 u = U[(fl & 1) ? v : 0];
+                    const T2 e = V[(fl & 2) ? v : 0];
+                    const double w = s_w[v][k];
+                    const double us = w * (double)u.x, uf = w * (double)u.y, vs = (double)e.x;
+                    m[v] = fma(uf, vs, m[v]);
+                    m[v] = fma(us, fma(lambda, vs, (double)e.y), m[v]);
+                    s[v] = fma(us, vs, s[v]);
+                }
             }
         }
     }
-    if (live) part2[(size_t)blockIdx.y * nt + I] = make_double2(m, s);
+    if (live) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) part2[((size_t)blockIdx.y * nt + I) * NV + v] = make_double2(m[v], s[v]);
+    }
 }
 
-// y[det_I] += sum_k part2[k][I] (chunks in order); the same for ys.
+// y[det_I NV + v] += sum_k part2[k][I][v] (chunks in order); the same for ys.
+template <int NV>
 __global__ void k_fac_scatter(const uint32_t *__restrict__ tdet, int nt, int nk, const double2 *__restrict__ part2,
                               double *__restrict__ y, double *__restrict__ ys)
 {
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= nt) return;
-    double m = 0.0, s = 0.0;
-    for (int k = 0; k < nk; ++k) {
-        const double2 v = part2[(size_t)k * nt + I];
-        m += v.x;
-        s += v.y;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double m = 0.0, s = 0.0;
+        for (int k = 0; k < nk; ++k) {
+            const double2 e = part2[((size_t)k * nt + I) * NV + v];
+            m += e.x;
+            s += e.y;
+        }
+        const size_t d = (size_t)tdet[I] * NV + v;
+        y[d] += m;
+        if (ys) ys[d] += s;
     }
-    y[tdet[I]] += m;
-    if (ys) ys[tdet[I]] += s;
 }
 
 // Diagonals from the same-parent tables (one_body_diagonals, onebody.rs:358-405;
@@ -436,11 +471,13 @@ struct nw_factor {
     int prec = NW_PREC_FP64;  // tables nw_factor_apply reads: NW_PREC_FP64 or NW_PREC_MIXED (FP32)
     size_t tab_elems = 0;
     // plan: stage-1 items and stage-2 rows per table type (0 FP64, 1 FP32)
+    // and vector count (1, 2)
     DVec<uint32_t> d_midx, d_mdet;
-    DVec<S1Item> d_items[2];
-    DVec<S2Row> d_rows[2];
-    bool planned[2] = {false, false};
+    DVec<S1Item> d_items[2][2];
+    DVec<S2Row> d_rows[2][2];
+    bool planned[2][2] = {{false, false}, {false, false}};
     std::vector<PairPlan> plans;  // ordered by target, then source
+    size_t mat_elems = 0;          // materialisation entries of one vector
     DVec<double2> d_mat, d_part2;  // d_mat holds double2 or float2 rows
     int maxnt = 0;
     double t_ms[2] = {0, 0};  // last apply: total, table regeneration
@@ -531,8 +568,9 @@ double2 *table_ptr(nw_factor *f, int q, int p, int sp, double2 *scratch)
 // the materialisation buffer holds one target's pairs at a time (pairs of a
 // target are contiguous); regenerated tables: one pair at a time, both
 // tables in the scratch table slot.  lo: pointers into the FP32 tables and
-// float2 materialised rows.
-void make_plans(nw_factor *f, int lo)
+// float2 materialised rows.  nv: vectors per apply (materialised rows of nv
+// interleaved entries).
+void make_plans(nw_factor *f, int lo, int nv)
 {
     const int nref = f->nref;
     std::vector<S1Item> items;
@@ -568,7 +606,7 @@ void make_plans(nw_factor *f, int lo)
             for (const Group &g : P.groups) {
                 if (g.m0 < 0) {
                     rows.push_back({taba + es * (size_t)g.single_a * nta, tabb + es * (size_t)g.single_b * ntb,
-                                    (long long)g.single_det});
+                                    (int)g.single_det, 0, 0, {0, 0}});
                     continue;
                 }
                 const int len = g.side == 1 ? nta : ntb;
@@ -576,9 +614,9 @@ void make_plans(nw_factor *f, int lo)
                     char *out = mbase + es * mo;
                     items.push_back({g.side == 1 ? taba : tabb, out, len, (int)(P.m_off + e),
                                      (int)(P.m_off + std::min(g.m1, e + S1_CHUNK)), 0});
-                    if (g.side == 1) rows.push_back({out, tabb + es * (size_t)g.key * ntb, -1});
-                    else rows.push_back({taba + es * (size_t)g.key * nta, out, -1});
-                    mo += (size_t)len;
+                    if (g.side == 1) rows.push_back({out, tabb + es * (size_t)g.key * ntb, -1, 1, 0, {0, 0}});
+                    else rows.push_back({taba + es * (size_t)g.key * nta, out, -1, 0, 1, {0, 0}});
+                    mo += (size_t)len * nv;
                 }
             }
             pl.item1 = (long long)items.size();
@@ -592,9 +630,23 @@ void make_plans(nw_factor *f, int lo)
     }
     // the float2 rows of an FP32 plan take half of the double2 buffer
     if (mat_need > f->d_mat.n) throw NwError(NW_E_STATE, "factor plan exceeds its materialisation buffer");
-    f->d_items[lo].upload(items);
-    f->d_rows[lo].upload(rows);
-    f->planned[lo] = true;
+    f->d_items[lo][nv - 1].upload(items);
+    f->d_rows[lo][nv - 1].upload(rows);
+    f->planned[lo][nv - 1] = true;
+}
+
+// The plan of (lo, nv), made on first use; a larger materialisation buffer
+// for more vectors invalidates the plans made so far.
+void ensure_plan(nw_factor *f, int lo, int nv)
+{
+    if (f->planned[lo][nv - 1]) return;
+    const size_t need = std::max<size_t>(1, f->mat_elems * (size_t)nv);
+    if (need > f->d_mat.n) {
+        HIPCHK(hipStreamSynchronize(f->c->st));
+        f->d_mat.alloc(need);
+        for (auto &pl : f->planned) pl[0] = pl[1] = false;
+    }
+    make_plans(f, lo, nv);
 }
 
 size_t mat_elems_needed(const nw_factor *f)
@@ -726,9 +778,10 @@ extern "C" int nw_factor_create(nw_ctx *c, const nw_det *dets, uint64_t ndet, in
             }
         f->tab_elems = off;
         f->d_tab.alloc(std::max<size_t>(1, storage == NW_FACTOR_DEVICE ? off : maxpair));
-        f->d_mat.alloc(std::max<size_t>(1, mat_elems_needed(f.get())));
-        f->d_part2.alloc((size_t)64 * (size_t)std::max(1, f->maxnt));
-        make_plans(f.get(), 0);
+        f->mat_elems = mat_elems_needed(f.get());
+        f->d_mat.alloc(std::max<size_t>(1, f->mat_elems));
+        f->d_part2.alloc((size_t)64 * 2 * (size_t)std::max(1, f->maxnt));
+        make_plans(f.get(), 0, 1);
         if (storage == NW_FACTOR_DEVICE) build_tables(f.get());
         else {
             f->build_gen = c->build_gen;
@@ -757,8 +810,8 @@ extern "C" int nw_factor_refresh(nw_factor *f)
 namespace {
 
 // Stage 1 over items [i0, i1), stage 2 over rows [r0, r1) for the targets of
-// parent q, on tables of type T2 (plan lo).
-template <class T2>
+// parent q, on tables of type T2 (plan lo) and NV vectors.
+template <class T2, int NV>
 void run_range(nw_factor *f, int lo, int q, long long i0, long long i1, long long r0, long long r1, const double *dx,
                double lambda, double *dy, double *dys)
 {
@@ -768,8 +821,8 @@ void run_range(nw_factor *f, int lo, int q, long long i0, long long i1, long lon
     const unsigned gx = (blocks_of(maxlen, 256) + 7u) & ~7u;
     for (long long b = i0; b < i1; b += 65535) {
         const unsigned ni = (unsigned)std::min<long long>(65535, i1 - b);
-        hipLaunchKernelGGL(k_fac_mat<T2>, dim3(gx, ni), dim3(256), 0, c->st, f->d_items[lo].p + b, f->d_midx.p,
-                           f->d_mdet.p, dx);
+        hipLaunchKernelGGL((k_fac_mat<T2, NV>), dim3(gx, ni), dim3(256), 0, c->st, f->d_items[lo][NV - 1].p + b,
+                           f->d_midx.p, f->d_mdet.p, dx);
     }
     const int nr = (int)(r1 - r0);
     if (nr == 0) return;
@@ -778,29 +831,35 @@ void run_range(nw_factor *f, int lo, int q, long long i0, long long i1, long lon
     nk = std::min(nk, std::max(1, nr / 16));
     const int chunk = (nr + nk - 1) / nk;
     nk = (nr + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_fac_rows<T2>, dim3(blocks_of(Q.nt, 256), (unsigned)nk), dim3(256), 0, c->st, Q.d_ta.p,
-                       Q.d_tb.p, Q.nt, f->d_rows[lo].p + r0, nr, chunk, dx, lambda, f->d_part2.p);
-    hipLaunchKernelGGL(k_fac_scatter, dim3(blocks_of(Q.nt, 256)), dim3(256), 0, c->st, Q.d_tdet.p, Q.nt, nk,
+    hipLaunchKernelGGL((k_fac_rows<T2, NV>), dim3(blocks_of(Q.nt, 256), (unsigned)nk), dim3(256), 0, c->st,
+                       Q.d_ta.p, Q.d_tb.p, Q.nt, f->d_rows[lo][NV - 1].p + r0, nr, chunk, dx, lambda, f->d_part2.p);
+    hipLaunchKernelGGL(k_fac_scatter<NV>, dim3(blocks_of(Q.nt, 256)), dim3(256), 0, c->st, Q.d_tdet.p, Q.nt, nk,
                        f->d_part2.p, dy, dys);
 }
 
-void run_range_any(nw_factor *f, int lo, int q, long long i0, long long i1, long long r0, long long r1,
+void run_range_any(nw_factor *f, int lo, int nv, int q, long long i0, long long i1, long long r0, long long r1,
                    const double *dx, double lambda, double *dy, double *dys)
 {
-    if (lo) run_range<float2>(f, 1, q, i0, i1, r0, r1, dx, lambda, dy, dys);
-    else run_range<double2>(f, 0, q, i0, i1, r0, r1, dx, lambda, dy, dys);
+    if (lo && nv == 2) run_range<float2, 2>(f, 1, q, i0, i1, r0, r1, dx, lambda, dy, dys);
+    else if (lo) run_range<float2, 1>(f, 1, q, i0, i1, r0, r1, dx, lambda, dy, dys);
+    else if (nv == 2) run_range<double2, 2>(f, 0, q, i0, i1, r0, r1, dx, lambda, dy, dys);
+    else run_range<double2, 1>(f, 0, q, i0, i1, r0, r1, dx, lambda, dy, dys);
 }
 
 }  // namespace
 
-extern "C" int nw_factor_apply(nw_factor *f, double lambda, const double *x, double *y, double *ys)
+namespace {
+
+// y = (F + lambda S) x for nv interleaved vectors (x[i nv + v]); ys = S x if not null.
+int apply_nv(nw_factor *f, int nv, double lambda, const double *x, double *y, double *ys)
 {
-    if (!f) return NW_E_ARG;
     nw_ctx *c = f->c;
     return guarded(c, [&]() -> int {
         if (!x || !y) return fail(c, NW_E_ARG, "null arguments");
         check_current(f);
-        const long long n = (long long)f->ndet;
+        const int lo = f->prec == NW_PREC_MIXED ? 1 : 0;
+        ensure_plan(f, lo, nv);
+        const long long n = (long long)f->ndet * nv;
         HIPCHK(hipEventRecord(c->ev[0], c->st));
         const double *dx = (const double *)to_device(c, c->outH, x, n * 8);
         const bool ydev = is_device_ptr(y), ysdev = ys && is_device_ptr(ys);
@@ -809,13 +868,12 @@ extern "C" int nw_factor_apply(nw_factor *f, double lambda, const double *x, dou
         hipLaunchKernelGGL(k_fac_zero, dim3(blocks_of(n, 256)), dim3(256), 0, c->st, dy, n);
         if (dys) hipLaunchKernelGGL(k_fac_zero, dim3(blocks_of(n, 256)), dim3(256), 0, c->st, dys, n);
         float regen = 0.f;
-        const int lo = f->prec == NW_PREC_MIXED ? 1 : 0;
         if (f->storage == NW_FACTOR_DEVICE) {
             // the pairs of one target are contiguous in the plan
             for (size_t k = 0; k < f->plans.size();) {
                 size_t j = k;
                 while (j < f->plans.size() && f->plans[j].q == f->plans[k].q) ++j;
-                run_range_any(f, lo, f->plans[k].q, f->plans[k].item0, f->plans[j - 1].item1, f->plans[k].row0,
+                run_range_any(f, lo, nv, f->plans[k].q, f->plans[k].item0, f->plans[j - 1].item1, f->plans[k].row0,
                               f->plans[j - 1].row1, dx, lambda, dy, dys);
                 k = j;
             }
@@ -826,7 +884,7 @@ extern "C" int nw_factor_apply(nw_factor *f, double lambda, const double *x, dou
                 launch_table(f, pl.q, pl.p, 0, slot, nullptr, lo);
                 launch_table(f, pl.q, pl.p, 1, slot + elem_bytes(lo) * table_elems(f, pl.q, pl.p, 0), nullptr, lo);
                 HIPCHK(hipEventRecord(c->ev[3], c->st));
-                run_range_any(f, lo, pl.q, pl.item0, pl.item1, pl.row0, pl.row1, dx, lambda, dy, dys);
+                run_range_any(f, lo, nv, pl.q, pl.item0, pl.item1, pl.row0, pl.row1, dx, lambda, dy, dys);
                 HIPCHK(hipStreamSynchronize(c->st));
                 regen += elapsed(c->ev[2], c->ev[3]);
             }
@@ -840,6 +898,20 @@ extern "C" int nw_factor_apply(nw_factor *f, double lambda, const double *x, dou
         f->t_ms[1] = regen;
         return NW_OK;
     });
+}
+
+}  // namespace
+
+extern "C" int nw_factor_apply(nw_factor *f, double lambda, const double *x, double *y, double *ys)
+{
+    if (!f) return NW_E_ARG;
+    return apply_nv(f, 1, lambda, x, y, ys);
+}
+
+extern "C" int nw_factor_apply_pair(nw_factor *f, double lambda, const double *x2, double *y2, double *ys2)
+{
+    if (!f) return NW_E_ARG;
+    return apply_nv(f, 2, lambda, x2, y2, ys2);
 }
 
 extern "C" int nw_factor_diagonals(nw_factor *f, double lambda, double *mdiag, double *sdiag)
@@ -897,7 +969,7 @@ extern "C" int nw_factor_set_precision(nw_factor *f, int precision)
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipStreamSynchronize(c->st));
             }
-            if (!f->planned[1]) make_plans(f, 1);
+            ensure_plan(f, 1, 1);
         }
         f->prec = precision;
         return NW_OK;

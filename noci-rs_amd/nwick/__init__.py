@@ -37,7 +37,7 @@ EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matri
            "nw_mlp_get_params", "nw_mlp_forward", "nw_mlp_grad", "nw_mlp_step", "nw_mlp_train", "nw_mlp_last_error",
            "nw_mlp_destroy", "nw_factor_create", "nw_factor_refresh", "nw_factor_apply", "nw_factor_diagonals",
            "nw_factor_info", "nw_factor_ids", "nw_factor_table", "nw_factor_last_ms", "nw_factor_destroy",
-           "nw_factor_set_precision"]
+           "nw_factor_set_precision", "nw_factor_apply_pair"]
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_BF16X3 = 2
@@ -113,6 +113,7 @@ def load():
     L.nw_factor_destroy.argtypes = [vp]
     L.nw_factor_destroy.restype = None
     L.nw_factor_set_precision.argtypes = [vp, ct.c_int]
+    L.nw_factor_apply_pair.argtypes = [vp, ct.c_double, vp, vp, vp]
     L.nw_batch_create.argtypes = [ct.POINTER(vp), ct.c_int]
     L.nw_batch_add.argtypes = [vp, vp, vp, ct.c_uint64, ct.c_int, ct.POINTER(ct.c_int)]
     L.nw_batch_plan.argtypes = [vp, ct.c_int]
@@ -358,6 +359,30 @@ class Factor:
             y = np.empty(self.n)
         sync_torch(x, y, ys)
         self._check(self.L.nw_factor_apply(self.h, float(lam), _ptr(x), _ptr(y), _ptr(ys)))
+        return y if ys is None else (y, ys)
+
+    def apply_pair(self, lam, x, y=None, ys=None):
+        """The same operator on two vectors in one pass over the tables: x of
+        complex dtype (real and imaginary parts) or shape [n, 2], contiguous;
+        y, ys likewise (ys = S x if given)."""
+        def flat(a):
+            if a is None:
+                return None
+            if hasattr(a, "is_complex") and a.is_complex():
+                import torch
+                a = torch.view_as_real(a)
+            elif isinstance(a, np.ndarray) and np.iscomplexobj(a):
+                a = a.view(np.float64).reshape(-1, 2)
+            if tuple(a.shape) != (self.n, 2):
+                raise ValueError("apply_pair takes n complex values or an [n, 2] array")
+            if not (a.is_contiguous() if hasattr(a, "is_contiguous") else a.flags.c_contiguous):
+                raise ValueError("apply_pair needs contiguous arrays")
+            return a
+        if y is None:
+            y = np.empty((self.n, 2))
+        xf, yf, ysf = flat(x), flat(y), flat(ys)
+        sync_torch(xf, yf, ysf)
+        self._check(self.L.nw_factor_apply_pair(self.h, float(lam), _ptr(xf), _ptr(yf), _ptr(ysf)))
         return y if ys is None else (y, ys)
 
     def diagonals(self, lam, mdiag=None, sdiag=None):

@@ -350,11 +350,19 @@ class PT2:
         """(M^Omega + i eps Q) x: apply_factorised_shifted_omega_m
         (src/snoci/operators.rs:759-776) or, pairwise, apply_shifted_omega_m
         (:856-887), with the projection correction (:938-947).  A complex x
-        takes one real application per part; eps != 0 also uses S x."""
+        takes both parts through one pass over the factor tables
+        (nw_factor_apply_pair), or one pairwise matvec per part; eps != 0
+        also uses S x."""
         torch = self.torch
         p = self.proj
         shift = imag_shift != 0.0
-        if x.is_complex():
+        if x.is_complex() and self.fac is not None:
+            # both parts in one pass over the tables
+            x = x.contiguous()
+            y = torch.empty_like(x)
+            svec = torch.empty_like(x) if shift else None
+            self.fac.apply_pair(-self.e0, x, y=y, ys=svec)
+        elif x.is_complex():
             parts = []
             for comp in (x.real, x.imag):
                 sv = torch.empty_like(self.y) if shift else None

@@ -128,6 +128,35 @@ def test_fp32_tables_and_apply(fcase, storage):
     f.close()
 
 
+@pytest.mark.parametrize("storage", [nwick.FACTOR_DEVICE, nwick.FACTOR_TRANSIENT])
+@pytest.mark.parametrize("prec", [nwick.PREC_FP64, nwick.PREC_MIXED])
+def test_apply_pair_equals_two_applies(fcase, storage, prec):
+    """Two interleaved vectors in one pass give, bit for bit, what two
+    separate applies give (same summation order per vector), for FP64 and
+    FP32 tables; before and after a single apply shares the buffers."""
+    import torch
+    sysm, dets, eng, slab, fac, fa, fb = fcase
+    rec = basis.to_records(dets)
+    f = eng.factor(rec, mode=1, storage=storage)
+    f.set_precision(prec)
+    n = len(dets)
+    rng = np.random.default_rng(21)
+    xr, xi = rng.normal(size=n), rng.normal(size=n)
+    lam = 0.7
+    yr, ysr = f.apply(lam, xr, ys=np.empty(n))
+    yi, ysi = f.apply(lam, xi, ys=np.empty(n))
+    y2, ys2 = f.apply_pair(lam, xr + 1j * xi, y=np.empty(n, dtype=complex), ys=np.empty(n, dtype=complex))
+    assert np.array_equal(y2.real, yr) and np.array_equal(y2.imag, yi)
+    assert np.array_equal(ys2.real, ysr) and np.array_equal(ys2.imag, ysi)
+    # device tensors, then a single apply again (plans of both vector counts coexist)
+    xt = torch.tensor(xr + 1j * xi, device="cuda:0")
+    yt = torch.empty_like(xt)
+    f.apply_pair(lam, xt, y=yt)
+    assert np.array_equal(yt.cpu().numpy(), y2)
+    assert np.array_equal(f.apply(lam, xr), yr)
+    f.close()
+
+
 def test_fp32_tables_follow_refresh(fcase):
     sysm, dets, eng, slab, fac, fa, fb = fcase
     rec = basis.to_records(dets)
