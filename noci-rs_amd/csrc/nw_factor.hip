@@ -33,7 +33,9 @@
 // Stage 1 streams each table row about once; stage 2 is one lane per target
 // entry over the group rows.
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <vector>
@@ -84,6 +86,22 @@ template <>
 __device__ inline double2 pair2<double2>(double a, double b) { return make_double2(a, b); }
 template <>
 __device__ inline float2 pair2<float2>(double a, double b) { return make_float2((float)a, (float)b); }
+
+// Dual stage-1 item: one pass over the table Tab(q, p) of one spin (ns rows =
+// source components of p, nt columns = components of q) gives both
+//   forward  out_f[t] = sum_s x[fdet[s]] Tab[s][t]   (a group of p, target q) and
+//   reverse  out_r[s] = sum_t Tab[s][t] x[rdet[t]]   (a group of q, target p),
+// since Tab(p, q) = Tab(q, p)^T.  Partial sums per row chunk (forward) and per
+// column strip (reverse) are reduced in a fixed order by k_fac_dual_reduce.
+struct DualItem {
+    const void *tab;
+    void *out_f, *out_r;
+    const int *fdet, *rdet;  // determinant per row / column, -1 = no weight
+    double2 *fpart, *rpart;  // [nchunk][nt][NV], [nstrip][ns][NV]
+    int ns, nt, nchunk, nstrip;
+};
+constexpr int DUAL_ROWS = 256;   // rows per dual block
+constexpr int DUAL_COLS = 1024;  // columns per dual block: 4 waves x 64 lanes x 4
 
 constexpr int S1_CHUNK = 128;  // entries per stage-1 item (pieces of a large group)
 constexpr int S2_BATCH = 256;  // rows staged in LDS at a time
@@ -262,6 +280,163 @@ __global__ void k_fac_scatter(const uint32_t *__restrict__ tdet, int nt, int nk,
         const size_t d = (size_t)tdet[I] * NV + v;
         y[d] += m;
         if (ys) ys[d] += s;
+    }
+}
+
+// Sums a[k] over the 64 lanes of the wave for every k < V (V a power of two,
+// 2 <= V <= 64) by recursive halving: at offset o a lane keeps the upper half
+// of its values if lane & o, else the lower half, and adds its partner's copy
+// of that half; then the remaining offsets sum the one value left.  On return
+// a[0] holds the total of value rs_index<V>(lane), the same in every lane with
+// equal bits 5 .. 6 - log2(V).
+template <int V, int L = V, int O = 32>
+__device__ inline void wave_reduce_scatter(double (&a)[V], int lane)
+{
+    if constexpr (L > 1) {
+        const bool up = (lane & O) != 0;
+#pragma unroll
+        for (int k = 0; k < L / 2; ++k) {
+            const double send = up ? a[k] : a[k + L / 2];
+            const double keep = up ? a[k + L / 2] : a[k];
+            a[k] = keep + __shfl_xor(send, O);
+        }
+        wave_reduce_scatter<V, L / 2, O / 2>(a, lane);
+    } else {
+#pragma unroll
+        for (int o = O; o > 0; o >>= 1) a[0] += __shfl_xor(a[0], o);
+    }
+}
+
+template <int V>
+__device__ inline int rs_index(int lane)
+{
+    int k = 0;
+#pragma unroll
+    for (int L = V, o = 32; L > 1; L >>= 1, o >>= 1)
+        if (lane & o) k += L / 2;
+    return k;
+}
+
+// Dual stage 1: workgroup = DUAL_ROWS rows x DUAL_COLS columns of one table;
+// each lane owns 4 consecutive columns (forward sums over the rows in
+// registers), each wave reduces a row's reverse products over its 256
+// columns, and the 4 waves' row partials are summed in order through LDS.
+template <class T2, int NV>
+__global__ __launch_bounds__(256) void k_fac_dual(const DualItem *__restrict__ items, const double *__restrict__ x)
+{
+    __shared__ double s_wf[NV][DUAL_ROWS];
+    __shared__ double s_rev[4][DUAL_ROWS][2 * NV];
+    const DualItem it = items[blockIdx.y];
+    if ((int)blockIdx.x >= it.nchunk * it.nstrip) return;  // uniform over the workgroup
+    const int chunk = blockIdx.x / it.nstrip, strip = blockIdx.x % it.nstrip;
+    const int r0 = chunk * DUAL_ROWS, nr = min(it.ns - r0, DUAL_ROWS);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c0 = strip * DUAL_COLS + wave * 256 + lane * 4;
+    for (int i = threadIdx.x; i < nr; i += 256) {
+        const int d = it.fdet[r0 + i];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s_wf[v][i] = d >= 0 ? x[(size_t)d * NV + v] : 0.0;
+    }
+    double wr[4][NV];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int d = c0 + j < it.nt ? it.rdet[c0 + j] : -1;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) wr[j][v] = d >= 0 ? x[(size_t)d * NV + v] : 0.0;
+    }
+    __syncthreads();
+    double fS[4][NV], fF[4][NV];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fS[j][v] = fF[j][v] = 0.0;
+    const T2 *tab = (const T2 *)it.tab;
+    // rows in batches of DUAL_RB: reverse products a[(r NV + v) 2 + {S, F}]
+    // reduced over the wave together
+    constexpr int DUAL_RB = 4, V = 2 * NV * DUAL_RB;
+    const int k_mine = rs_index<V>(lane);
+    const bool writer = (lane & (64 / V - 1)) == 0;
+    for (int i0 = 0; i0 < nr; i0 += DUAL_RB) {
+        double a[V];
+#pragma unroll
+        for (int r = 0; r < DUAL_RB; ++r) {
+            const int i = i0 + r;
+            T2 e[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                e[j] = (i < nr && c0 + j < it.nt) ? tab[(size_t)(r0 + i) * it.nt + c0 + j] : pair2<T2>(0.0, 0.0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double w = i < nr ? s_wf[v][i] : 0.0;
+                double pS = 0.0, pF = 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const double es = (double)e[j].x, ef = (double)e[j].y;
+                    fS[j][v] = fma(w, es, fS[j][v]);
+                    fF[j][v] = fma(w, ef, fF[j][v]);
+                    pS = fma(es, wr[j][v], pS);
+                    pF = fma(ef, wr[j][v], pF);
+                }
+                a[(r * NV + v) * 2] = pS;
+                a[(r * NV + v) * 2 + 1] = pF;
+            }
+        }
+        wave_reduce_scatter<V>(a, lane);
+        const int r = k_mine / (2 * NV);
+        if (writer && i0 + r < nr) s_rev[wave][i0 + r][k_mine % (2 * NV)] = a[0];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (c0 + j < it.nt) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                it.fpart[((size_t)chunk * it.nt + c0 + j) * NV + v] = make_double2(fS[j][v], fF[j][v]);
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nr; i += 256) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double S = 0.0, F = 0.0;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                S += s_rev[w][i][2 * v];
+                F += s_rev[w][i][2 * v + 1];
+            }
+            it.rpart[((size_t)strip * it.ns + r0 + i) * NV + v] = make_double2(S, F);
+        }
+    }
+}
+
+// Materialised rows of the dual items: forward over nt, reverse over ns,
+// partials summed in chunk / strip order.
+template <class T2, int NV>
+__global__ void k_fac_dual_reduce(const DualItem *__restrict__ items)
+{
+    const DualItem it = items[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < it.nt) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double S = 0.0, F = 0.0;
+            for (int k = 0; k < it.nchunk; ++k) {
+                const double2 e = it.fpart[((size_t)k * it.nt + i) * NV + v];
+                S += e.x;
+                F += e.y;
+            }
+            ((T2 *)it.out_f)[(size_t)i * NV + v] = pair2<T2>(S, F);
+        }
+    } else if (i < it.nt + it.ns) {
+        const int r = i - it.nt;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double S = 0.0, F = 0.0;
+            for (int k = 0; k < it.nstrip; ++k) {
+                const double2 e = it.rpart[((size_t)k * it.ns + r) * NV + v];
+                S += e.x;
+                F += e.y;
+            }
+            ((T2 *)it.out_r)[(size_t)r * NV + v] = pair2<T2>(S, F);
+        }
     }
 }
 
@@ -446,6 +621,15 @@ SpinComp spin_comp(const PackedDet &d, int sp)
     return c;
 }
 
+// A dual pass chosen for one unordered parent pair and spin: the table
+// Tab(q, p) (rows = components of p), the forward group gp of p (target q)
+// and the reverse group gq of q (target p); det_off: its fdet (ns entries)
+// then rdet (nt entries) in the uploaded determinant-index array.
+struct DualSel {
+    int q, p, sp, gp, gq;
+    long long det_off;
+};
+
 // Per ordered parent pair: its stage-1 items and stage-2 rows (ranges into the
 // uploaded arrays) and its materialisation footprint.
 struct PairPlan {
@@ -479,6 +663,14 @@ struct nw_factor {
     std::vector<PairPlan> plans;  // ordered by target, then source
     size_t mat_elems = 0;          // materialisation entries of one vector
     DVec<double2> d_mat, d_part2;  // d_mat holds double2 or float2 rows
+    // dual passes (resident tables): selections, their row / column
+    // determinants, per-plan items and the partial-sum buffer
+    std::vector<DualSel> duals;
+    std::map<long long, std::pair<int, int>> dual_of;  // (target, source, group) -> (dual, 0 fwd / 1 rev)
+    DVec<int> d_dualdet;
+    DVec<DualItem> d_dual[2][2];
+    DVec<double2> d_dpart;
+    size_t dual_mat = 0, dual_part = 0;  // per vector: materialised entries, partial entries
     int maxnt = 0;
     double t_ms[2] = {0, 0};  // last apply: total, table regeneration
 };
@@ -494,6 +686,11 @@ size_t table_elems(const nw_factor *f, int q, int p, int sp)
 }
 
 size_t elem_bytes(int lo) { return lo ? sizeof(float2) : sizeof(double2); }
+
+long long dual_key(const nw_factor *f, int target, int source, int group)
+{
+    return ((long long)(target * f->nref + source) << 32) | (long long)group;
+}
 
 // Table (q target, p source, spin sp) into out (float2 when lo, else
 // double2); when outT is not null also its transpose, the table (p target,
@@ -581,10 +778,38 @@ void make_plans(nw_factor *f, int lo, int nv)
     const char *tbase = dev && lo ? (const char *)f->d_tab32.p : (const char *)f->d_tab.p;
     char *mbase = (char *)f->d_mat.p;
     f->plans.clear();
+    // dual passes: their materialised rows open the buffer, then the rows of
+    // one target at a time
+    std::vector<DualItem> dual_items;
+    std::vector<std::array<char *, 2>> dual_out(f->duals.size());
+    size_t mo_d = 0, po = 0;
+    for (size_t d = 0; d < f->duals.size(); ++d) {
+        const DualSel &D = f->duals[d];
+        const int ns = f->par[D.p].ncomp[D.sp], nt = f->par[D.q].ncomp[D.sp];
+        DualItem it{};
+        it.tab = tbase + es * (size_t)f->tab_off[(D.q * nref + D.p) * 2 + D.sp];
+        it.out_f = mbase + es * mo_d;
+        mo_d += (size_t)nt * nv;
+        it.out_r = mbase + es * mo_d;
+        mo_d += (size_t)ns * nv;
+        it.fdet = f->d_dualdet.p + D.det_off;
+        it.rdet = it.fdet + ns;
+        it.ns = ns;
+        it.nt = nt;
+        it.nchunk = (ns + DUAL_ROWS - 1) / DUAL_ROWS;
+        it.nstrip = (nt + DUAL_COLS - 1) / DUAL_COLS;
+        it.fpart = f->d_dpart.p + po;
+        po += (size_t)it.nchunk * nt * nv;
+        it.rpart = f->d_dpart.p + po;
+        po += (size_t)it.nstrip * ns * nv;
+        dual_out[d] = {(char *)it.out_f, (char *)it.out_r};
+        dual_items.push_back(it);
+    }
+    if (po > f->d_dpart.n) throw NwError(NW_E_STATE, "factor plan exceeds its partial-sum buffer");
     for (int q = 0; q < nref; ++q) {
         const ParentSpace &Q = f->par[q];
         if (Q.nt == 0) continue;
-        size_t mat_q = 0;
+        size_t mat_q = mo_d;
         for (int p = 0; p < nref; ++p) {
             const ParentSpace &P = f->par[p];
             if (P.nt == 0) continue;
@@ -603,7 +828,15 @@ void make_plans(nw_factor *f, int lo, int nv)
             size_t mo = dev ? mat_q : 0;
             pl.item0 = (long long)items.size();
             pl.row0 = (long long)rows.size();
-            for (const Group &g : P.groups) {
+            for (int gi = 0; gi < (int)P.groups.size(); ++gi) {
+                const Group &g = P.groups[gi];
+                const auto du = dev ? f->dual_of.find(dual_key(f, q, p, gi)) : f->dual_of.end();
+                if (du != f->dual_of.end()) {
+                    char *out = dual_out[du->second.first][du->second.second];
+                    if (g.side == 1) rows.push_back({out, tabb + es * (size_t)g.key * ntb, -1, 1, 0, {0, 0}});
+                    else rows.push_back({taba + es * (size_t)g.key * nta, out, -1, 0, 1, {0, 0}});
+                    continue;
+                }
                 if (g.m0 < 0) {
                     rows.push_back({taba + es * (size_t)g.single_a * nta, tabb + es * (size_t)g.single_b * ntb,
                                     (int)g.single_det, 0, 0, {0, 0}});
@@ -632,6 +865,7 @@ void make_plans(nw_factor *f, int lo, int nv)
     if (mat_need > f->d_mat.n) throw NwError(NW_E_STATE, "factor plan exceeds its materialisation buffer");
     f->d_items[lo][nv - 1].upload(items);
     f->d_rows[lo][nv - 1].upload(rows);
+    f->d_dual[lo][nv - 1].upload(dual_items);
     f->planned[lo][nv - 1] = true;
 }
 
@@ -641,9 +875,11 @@ void ensure_plan(nw_factor *f, int lo, int nv)
 {
     if (f->planned[lo][nv - 1]) return;
     const size_t need = std::max<size_t>(1, f->mat_elems * (size_t)nv);
-    if (need > f->d_mat.n) {
+    const size_t need_part = std::max<size_t>(1, f->dual_part * (size_t)nv);
+    if (need > f->d_mat.n || need_part > f->d_dpart.n) {
         HIPCHK(hipStreamSynchronize(f->c->st));
-        f->d_mat.alloc(need);
+        if (need > f->d_mat.n) f->d_mat.alloc(need);
+        if (need_part > f->d_dpart.n) f->d_dpart.alloc(need_part);
         for (auto &pl : f->planned) pl[0] = pl[1] = false;
     }
     make_plans(f, lo, nv);
@@ -651,27 +887,84 @@ void ensure_plan(nw_factor *f, int lo, int nv)
 
 size_t mat_elems_needed(const nw_factor *f)
 {
-    // the same walk as make_plans without pointers
-    size_t need = 0;
+    // the same walk as make_plans without pointers (one vector)
+    size_t dual = 0;
+    for (const DualSel &D : f->duals) dual += (size_t)f->par[D.p].ncomp[D.sp] + (size_t)f->par[D.q].ncomp[D.sp];
+    size_t need = dual;
     for (int q = 0; q < f->nref; ++q) {
         const ParentSpace &Q = f->par[q];
         if (Q.nt == 0) continue;
-        size_t mq = 0;
+        size_t mq = dual;
         for (int p = 0; p < f->nref; ++p) {
             const ParentSpace &P = f->par[p];
             if (P.nt == 0) continue;
             size_t mp = 0;
-            for (const Group &g : P.groups)
-                if (g.m0 >= 0) {
-                    const size_t len = g.side == 1 ? Q.ncomp[0] : Q.ncomp[1];
-                    mp += len * (size_t)((g.m1 - g.m0 + S1_CHUNK - 1) / S1_CHUNK);
-                }
+            for (int gi = 0; gi < (int)P.groups.size(); ++gi) {
+                const Group &g = P.groups[gi];
+                if (g.m0 < 0 || f->dual_of.count(dual_key(f, q, p, gi))) continue;
+                const size_t len = g.side == 1 ? Q.ncomp[0] : Q.ncomp[1];
+                mp += len * (size_t)((g.m1 - g.m0 + S1_CHUNK - 1) / S1_CHUNK);
+            }
             if (f->storage == NW_FACTOR_DEVICE) mq += mp;
             else need = std::max(need, mp);
         }
         need = std::max(need, mq);
     }
     return need;
+}
+
+// Dual passes over the resident tables: for each unordered pair of distinct
+// parents and each spin, the largest group of either parent that sums rows of
+// that spin's tables, when each covers at least a quarter of the components
+// it indexes and repeats none.
+void select_duals(nw_factor *f, const std::vector<uint32_t> &midx, const std::vector<uint32_t> &mdet,
+                  std::vector<int> &dets)
+{
+    auto largest = [](const ParentSpace &P, int side) {
+        int best = -1, bn = 0;
+        for (int g = 0; g < (int)P.groups.size(); ++g) {
+            const Group &G = P.groups[g];
+            if (G.m0 >= 0 && G.side == side && G.m1 - G.m0 > bn) {
+                bn = G.m1 - G.m0;
+                best = g;
+            }
+        }
+        return best;
+    };
+    for (int q = 0; q < f->nref; ++q)
+        for (int p = q + 1; p < f->nref; ++p) {
+            const ParentSpace &Q = f->par[q], &P = f->par[p];
+            if (Q.nt == 0 || P.nt == 0) continue;
+            for (int sp = 0; sp < 2; ++sp) {
+                const int side = sp == 0 ? 1 : 0;  // alpha rows are summed by groups keyed by a beta component
+                const int gp = largest(P, side), gq = largest(Q, side);
+                if (gp < 0 || gq < 0) continue;
+                const Group &GP = P.groups[gp], &GQ = Q.groups[gq];
+                const int ns = P.ncomp[sp], nt = Q.ncomp[sp];
+                if (4 * (GP.m1 - GP.m0) < ns || 4 * (GQ.m1 - GQ.m0) < nt) continue;
+                std::vector<int> fd(ns, -1), rd(nt, -1);
+                bool ok = true;
+                for (int k = GP.m0; k < GP.m1 && ok; ++k) {
+                    int &slot = fd[midx[P.m_off + k]];
+                    ok = slot < 0;
+                    slot = (int)mdet[P.m_off + k];
+                }
+                for (int k = GQ.m0; k < GQ.m1 && ok; ++k) {
+                    int &slot = rd[midx[Q.m_off + k]];
+                    ok = slot < 0;
+                    slot = (int)mdet[Q.m_off + k];
+                }
+                if (!ok) continue;
+                const int idx = (int)f->duals.size();
+                f->duals.push_back({q, p, sp, gp, gq, (long long)dets.size()});
+                dets.insert(dets.end(), fd.begin(), fd.end());
+                dets.insert(dets.end(), rd.begin(), rd.end());
+                f->dual_of[dual_key(f, q, p, gp)] = {idx, 0};
+                f->dual_of[dual_key(f, p, q, gq)] = {idx, 1};
+                f->dual_part += (size_t)((ns + DUAL_ROWS - 1) / DUAL_ROWS) * nt +
+                                (size_t)((nt + DUAL_COLS - 1) / DUAL_COLS) * ns;
+            }
+        }
 }
 
 }  // namespace
@@ -761,6 +1054,11 @@ extern "C" int nw_factor_create(nw_ctx *c, const nw_det *dets, uint64_t ndet, in
         if (midx.size() >= (1ull << 31)) return fail(c, NW_E_SHAPE, "too many source entries");
         f->d_midx.upload(midx);
         f->d_mdet.upload(mdet);
+        if (storage == NW_FACTOR_DEVICE) {
+            std::vector<int> dd;
+            select_duals(f.get(), midx, mdet, dd);
+            f->d_dualdet.upload(dd);
+        }
         // ---- tables
         const int nref = c->nref;
         f->tab_off.assign((size_t)nref * nref * 2, -1);
@@ -780,6 +1078,7 @@ extern "C" int nw_factor_create(nw_ctx *c, const nw_det *dets, uint64_t ndet, in
         f->d_tab.alloc(std::max<size_t>(1, storage == NW_FACTOR_DEVICE ? off : maxpair));
         f->mat_elems = mat_elems_needed(f.get());
         f->d_mat.alloc(std::max<size_t>(1, f->mat_elems));
+        f->d_dpart.alloc(std::max<size_t>(1, f->dual_part));
         f->d_part2.alloc((size_t)64 * 2 * (size_t)std::max(1, f->maxnt));
         make_plans(f.get(), 0, 1);
         if (storage == NW_FACTOR_DEVICE) build_tables(f.get());
@@ -837,6 +1136,32 @@ void run_range(nw_factor *f, int lo, int q, long long i0, long long i1, long lon
                        f->d_part2.p, dy, dys);
 }
 
+template <class T2, int NV>
+void run_duals(nw_factor *f, int lo, const double *dx)
+{
+    nw_ctx *c = f->c;
+    const int nd = (int)f->duals.size();
+    if (nd == 0) return;
+    int maxblk = 0, maxlen = 0;
+    for (const DualSel &D : f->duals) {
+        const int ns = f->par[D.p].ncomp[D.sp], nt = f->par[D.q].ncomp[D.sp];
+        maxblk = std::max(maxblk, ((ns + DUAL_ROWS - 1) / DUAL_ROWS) * ((nt + DUAL_COLS - 1) / DUAL_COLS));
+        maxlen = std::max(maxlen, ns + nt);
+    }
+    const DualItem *items = f->d_dual[lo][NV - 1].p;
+    hipLaunchKernelGGL((k_fac_dual<T2, NV>), dim3((unsigned)maxblk, (unsigned)nd), dim3(256), 0, c->st, items, dx);
+    hipLaunchKernelGGL((k_fac_dual_reduce<T2, NV>), dim3(blocks_of(maxlen, 256), (unsigned)nd), dim3(256), 0, c->st,
+                       items);
+}
+
+void run_duals_any(nw_factor *f, int lo, int nv, const double *dx)
+{
+    if (lo && nv == 2) run_duals<float2, 2>(f, 1, dx);
+    else if (lo) run_duals<float2, 1>(f, 1, dx);
+    else if (nv == 2) run_duals<double2, 2>(f, 0, dx);
+    else run_duals<double2, 1>(f, 0, dx);
+}
+
 void run_range_any(nw_factor *f, int lo, int nv, int q, long long i0, long long i1, long long r0, long long r1,
                    const double *dx, double lambda, double *dy, double *dys)
 {
@@ -869,6 +1194,7 @@ int apply_nv(nw_factor *f, int nv, double lambda, const double *x, double *y, do
         if (dys) hipLaunchKernelGGL(k_fac_zero, dim3(blocks_of(n, 256)), dim3(256), 0, c->st, dys, n);
         float regen = 0.f;
         if (f->storage == NW_FACTOR_DEVICE) {
+            run_duals_any(f, lo, nv, dx);
             // the pairs of one target are contiguous in the plan
             for (size_t k = 0; k < f->plans.size();) {
                 size_t j = k;
