@@ -249,6 +249,11 @@ int nw_factor_info(const nw_factor *f, int parent, uint64_t *ncomp_a, uint64_t *
  * reference, which applies the tables in FP64: this is the low-precision
  * operator of an iteratively refined PT2 solve, whose residuals use FP64. */
 int nw_factor_set_precision(nw_factor *f, int precision);
+/* Table bytes one nw_factor_apply reads with the selected tables: every
+ * table row it sums or gathers from, once, and each table that a pass reads
+ * for both directions of a parent pair (Tab(p, q) = Tab(q, p)^T) once.  No
+ * counterpart in the reference; the algorithmic traffic of the apply. */
+int nw_factor_apply_bytes(nw_factor *f, uint64_t *bytes);
 /* Per-determinant component ids (SpinFactorisation::aid/bid, mod.rs:106-130). */
 int nw_factor_ids(const nw_factor *f, uint32_t *aids, uint32_t *bids);
 /* Copy one table: ncomp(source) x ncomp(target) interleaved {S, F} doubles,

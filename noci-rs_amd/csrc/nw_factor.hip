@@ -80,6 +80,22 @@ struct S2Row {
 };
 static_assert(sizeof(S2Row) == 24, "S2Row layout");
 
+// Pointers read from item records are generic; loads through these
+// global-address-space views compile to global (not flat) loads.
+template <class T>
+using gptr = const T __attribute__((address_space(1))) *;
+// Native two-element vectors with the layout of double2 / float2.
+template <class T2>
+struct raw2;
+template <>
+struct raw2<double2> {
+    typedef double __attribute__((ext_vector_type(2))) type;
+};
+template <>
+struct raw2<float2> {
+    typedef float __attribute__((ext_vector_type(2))) type;
+};
+
 template <class T2>
 __device__ inline T2 pair2(double a, double b);
 template <>
@@ -100,7 +116,13 @@ struct DualItem {
     double2 *fpart, *rpart;  // [nchunk][nt][NV], [nstrip][ns][NV]
     int ns, nt, nchunk, nstrip;
 };
-constexpr int DUAL_ROWS = 256;   // rows per dual block
+#ifndef NW_DUAL_ROWS
+#define NW_DUAL_ROWS 256
+#endif
+#ifndef NW_DUAL_RB
+#define NW_DUAL_RB 4
+#endif
+constexpr int DUAL_ROWS = NW_DUAL_ROWS;  // rows per dual block
 constexpr int DUAL_COLS = 1024;  // columns per dual block: 4 waves x 64 lanes x 4
 
 constexpr int S1_CHUNK = 128;  // entries per stage-1 item (pieces of a large group)
@@ -188,13 +210,14 @@ __global__ __launch_bounds__(256) void k_fac_mat(const S1Item *__restrict__ item
     __syncthreads();
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= it.nt) return;
-    const T2 *col = (const T2 *)it.tab + t;
+    typedef typename raw2<T2>::type R2;
+    const gptr<R2> col = (gptr<R2>)it.tab + t;
     double s[NV], f[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) s[v] = f[v] = 0.0;
 #pragma unroll 8
     for (int k = 0; k < ne; ++k) {
-        const T2 e = col[(size_t)s_r[k] * it.nt];
+        const R2 e = col[(size_t)s_r[k] * it.nt];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             s[v] = fma(s_x[v][k], (double)e.x, s[v]);
@@ -241,12 +264,13 @@ __global__ __launch_bounds__(256) void k_fac_rows(const uint32_t *__restrict__ t
 #pragma unroll 8
             for (int k = 0; k < nb; ++k) {
                 const int fl = NV > 1 ? s_f[k] : 0;
-                const T2 *U = s_u[k] + ((fl & 1) ? (size_t)a * NV : (size_t)a);  // alpha side {S, F}
-                const T2 *V = s_v[k] + ((fl & 2) ? (size_t)b * NV : (size_t)b);  // beta side {S, F}
+                typedef typename raw2<T2>::type R2;
+                const gptr<R2> U = (gptr<R2>)s_u[k] + ((fl & 1) ? (size_t)a * NV : (size_t)a);  // alpha side {S, F}
+                const gptr<R2> V = (gptr<R2>)s_v[k] + ((fl & 2) ? (size_t)b * NV : (size_t)b);  // beta side {S, F}
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    const T2 u = U[(fl & 1) ? v : 0];
-                    const T2 e = V[(fl & 2) ? v : 0];
+                    const R2 u = U[(fl & 1) ? v : 0];
+                    const R2 e = V[(fl & 2) ? v : 0];
                     const double w = s_w[v][k];
                     const double us = w * (double)u.x, uf = w * (double)u.y, vs = (double)e.x;
                     m[v] = fma(uf, vs, m[v]);
@@ -318,9 +342,12 @@ __device__ inline int rs_index(int lane)
 }
 
 // Dual stage 1: workgroup = DUAL_ROWS rows x DUAL_COLS columns of one table;
-// each lane owns 4 consecutive columns (forward sums over the rows in
-// registers), each wave reduces a row's reverse products over its 256
-// columns, and the 4 waves' row partials are summed in order through LDS.
+// lane l of wave w owns 4 columns, VEC adjacent ones per 16-byte load (VEC =
+// 1 for double2, 2 for float2) with the lanes contiguous, so that each load
+// instruction reads 1 KB in one piece (forward sums over the rows in
+// registers); each wave reduces a row's reverse
+// products over its 256 columns, and the 4 waves' row partials are summed in
+// order through LDS.
 template <class T2, int NV>
 __global__ __launch_bounds__(256) void k_fac_dual(const DualItem *__restrict__ items, const double *__restrict__ x)
 {
@@ -331,7 +358,9 @@ __global__ __launch_bounds__(256) void k_fac_dual(const DualItem *__restrict__ i
     const int chunk = blockIdx.x / it.nstrip, strip = blockIdx.x % it.nstrip;
     const int r0 = chunk * DUAL_ROWS, nr = min(it.ns - r0, DUAL_ROWS);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int c0 = strip * DUAL_COLS + wave * 256 + lane * 4;
+    constexpr int VEC = sizeof(T2) == 8 ? 2 : 1;
+    const int cbase = strip * DUAL_COLS + wave * 256 + lane * VEC;
+    auto col = [&](int j) { return cbase + (j / VEC) * 64 * VEC + j % VEC; };
     for (int i = threadIdx.x; i < nr; i += 256) {
         const int d = it.fdet[r0 + i];
 #pragma unroll
@@ -340,7 +369,7 @@ __global__ __launch_bounds__(256) void k_fac_dual(const DualItem *__restrict__ i
     double wr[4][NV];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int d = c0 + j < it.nt ? it.rdet[c0 + j] : -1;
+        const int d = col(j) < it.nt ? it.rdet[col(j)] : -1;
 #pragma unroll
         for (int v = 0; v < NV; ++v) wr[j][v] = d >= 0 ? x[(size_t)d * NV + v] : 0.0;
     }
@@ -350,10 +379,18 @@ __global__ __launch_bounds__(256) void k_fac_dual(const DualItem *__restrict__ i
     for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int v = 0; v < NV; ++v) fS[j][v] = fF[j][v] = 0.0;
-    const T2 *tab = (const T2 *)it.tab;
+    typedef typename raw2<T2>::type R2;
+    const gptr<R2> tab = (gptr<R2>)it.tab;
+    // in-range columns and rows for every load (no branches around the loads):
+    // a clamped column has reverse weight 0 and its forward sums are not
+    // stored, a clamped row has forward weight 0 and its reverse sum is not
+    // stored
+    int colc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) colc[j] = min(col(j), it.nt - 1);
     // rows in batches of DUAL_RB: reverse products a[(r NV + v) 2 + {S, F}]
     // reduced over the wave together
-    constexpr int DUAL_RB = 4, V = 2 * NV * DUAL_RB;
+    constexpr int DUAL_RB = NW_DUAL_RB, V = 2 * NV * DUAL_RB;
     const int k_mine = rs_index<V>(lane);
     const bool writer = (lane & (64 / V - 1)) == 0;
     for (int i0 = 0; i0 < nr; i0 += DUAL_RB) {
@@ -361,10 +398,10 @@ __global__ __launch_bounds__(256) void k_fac_dual(const DualItem *__restrict__ i
 #pragma unroll
         for (int r = 0; r < DUAL_RB; ++r) {
             const int i = i0 + r;
-            T2 e[4];
+            const gptr<R2> row = tab + (size_t)(r0 + min(i, nr - 1)) * it.nt;
+            R2 e[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                e[j] = (i < nr && c0 + j < it.nt) ? tab[(size_t)(r0 + i) * it.nt + c0 + j] : pair2<T2>(0.0, 0.0);
+            for (int j = 0; j < 4; ++j) e[j] = row[colc[j]];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const double w = i < nr ? s_wf[v][i] : 0.0;
@@ -387,10 +424,10 @@ __global__ __launch_bounds__(256) void k_fac_dual(const DualItem *__restrict__ i
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-        if (c0 + j < it.nt) {
+        if (col(j) < it.nt) {
 #pragma unroll
             for (int v = 0; v < NV; ++v)
-                it.fpart[((size_t)chunk * it.nt + c0 + j) * NV + v] = make_double2(fS[j][v], fF[j][v]);
+                it.fpart[((size_t)chunk * it.nt + col(j)) * NV + v] = make_double2(fS[j][v], fF[j][v]);
         }
     __syncthreads();
     for (int i = threadIdx.x; i < nr; i += 256) {
@@ -661,6 +698,7 @@ struct nw_factor {
     DVec<S2Row> d_rows[2][2];
     bool planned[2][2] = {{false, false}, {false, false}};
     std::vector<PairPlan> plans;  // ordered by target, then source
+    std::vector<uint32_t> h_midx;  // host copy of the summed-component indices
     size_t mat_elems = 0;          // materialisation entries of one vector
     DVec<double2> d_mat, d_part2;  // d_mat holds double2 or float2 rows
     // dual passes (resident tables): selections, their row / column
@@ -671,6 +709,7 @@ struct nw_factor {
     DVec<DualItem> d_dual[2][2];
     DVec<double2> d_dpart;
     size_t dual_mat = 0, dual_part = 0;  // per vector: materialised entries, partial entries
+    uint64_t apply_bytes[2] = {0, 0};    // table bytes one apply reads (FP64, FP32 tables)
     int maxnt = 0;
     double t_ms[2] = {0, 0};  // last apply: total, table regeneration
 };
@@ -806,6 +845,20 @@ void make_plans(nw_factor *f, int lo, int nv)
         dual_items.push_back(it);
     }
     if (po > f->d_dpart.n) throw NwError(NW_E_STATE, "factor plan exceeds its partial-sum buffer");
+    // table rows an apply touches: whole dual tables, and every other row that
+    // stage 1 sums or stage 2 gathers from (each counted once per table)
+    std::vector<std::vector<char>> touched((size_t)nref * nref * 2);
+    std::vector<char> whole((size_t)nref * nref * 2, 0);  // tables a dual pass reads whole
+    for (const DualSel &D : f->duals) whole[((size_t)D.q * nref + D.p) * 2 + D.sp] = 1;
+    auto touch = [&](int tq, int tp, int sp, int row) {
+        if (whole[((size_t)tq * nref + tp) * 2 + sp]) return;
+        auto &t = touched[((size_t)tq * nref + tp) * 2 + sp];
+        if (t.empty()) t.assign(f->par[tp].ncomp[sp], 0);
+        t[row] = 1;
+    };
+    uint64_t bytes = 0;
+    for (const DualSel &D : f->duals)
+        bytes += (uint64_t)f->par[D.p].ncomp[D.sp] * (uint64_t)f->par[D.q].ncomp[D.sp] * es;
     for (int q = 0; q < nref; ++q) {
         const ParentSpace &Q = f->par[q];
         if (Q.nt == 0) continue;
@@ -831,6 +884,14 @@ void make_plans(nw_factor *f, int lo, int nv)
             for (int gi = 0; gi < (int)P.groups.size(); ++gi) {
                 const Group &g = P.groups[gi];
                 const auto du = dev ? f->dual_of.find(dual_key(f, q, p, gi)) : f->dual_of.end();
+                if (g.m0 < 0) {
+                    touch(q, p, 0, g.single_a);
+                    touch(q, p, 1, g.single_b);
+                } else {
+                    touch(q, p, g.side == 1 ? 1 : 0, g.key);
+                    if (du == f->dual_of.end())
+                        for (int e = g.m0; e < g.m1; ++e) touch(q, p, g.side == 1 ? 0 : 1, (int)f->h_midx[P.m_off + e]);
+                }
                 if (du != f->dual_of.end()) {
                     char *out = dual_out[du->second.first][du->second.second];
                     if (g.side == 1) rows.push_back({out, tabb + es * (size_t)g.key * ntb, -1, 1, 0, {0, 0}});
@@ -863,6 +924,13 @@ void make_plans(nw_factor *f, int lo, int nv)
     }
     // the float2 rows of an FP32 plan take half of the double2 buffer
     if (mat_need > f->d_mat.n) throw NwError(NW_E_STATE, "factor plan exceeds its materialisation buffer");
+    for (int q = 0; q < nref; ++q)
+        for (int p = 0; p < nref; ++p)
+            for (int sp = 0; sp < 2; ++sp) {
+                const auto &t = touched[((size_t)q * nref + p) * 2 + sp];
+                bytes += (uint64_t)std::count(t.begin(), t.end(), 1) * (uint64_t)f->par[q].ncomp[sp] * es;
+            }
+    f->apply_bytes[lo] = bytes;
     f->d_items[lo][nv - 1].upload(items);
     f->d_rows[lo][nv - 1].upload(rows);
     f->d_dual[lo][nv - 1].upload(dual_items);
@@ -1054,6 +1122,7 @@ extern "C" int nw_factor_create(nw_ctx *c, const nw_det *dets, uint64_t ndet, in
         if (midx.size() >= (1ull << 31)) return fail(c, NW_E_SHAPE, "too many source entries");
         f->d_midx.upload(midx);
         f->d_mdet.upload(mdet);
+        f->h_midx = midx;
         if (storage == NW_FACTOR_DEVICE) {
             std::vector<int> dd;
             select_duals(f.get(), midx, mdet, dd);
@@ -1278,6 +1347,17 @@ extern "C" int nw_factor_info(const nw_factor *f, int parent, uint64_t *ncomp_a,
     }
     if (table_bytes) *table_bytes = (uint64_t)f->tab_elems * elem_bytes(f->prec == NW_PREC_MIXED);
     return NW_OK;
+}
+
+extern "C" int nw_factor_apply_bytes(nw_factor *f, uint64_t *bytes)
+{
+    if (!f || !bytes) return NW_E_ARG;
+    return guarded(f->c, [&]() -> int {
+        const int lo = f->prec == NW_PREC_MIXED ? 1 : 0;
+        ensure_plan(f, lo, 1);
+        *bytes = f->apply_bytes[lo];
+        return NW_OK;
+    });
 }
 
 extern "C" int nw_factor_set_precision(nw_factor *f, int precision)

@@ -101,9 +101,10 @@ def test_fp32_tables_and_apply(fcase, storage):
     x = rng.normal(size=len(dets))
     lam = -1.9
     y64, ys64 = f.apply(lam, x, ys=np.empty(len(dets)))
-    bytes64 = f.info()["table_bytes"]
+    bytes64, abytes64 = f.info()["table_bytes"], f.info()["apply_bytes"]
+    assert 0 < abytes64 <= bytes64
     f.set_precision(nwick.PREC_MIXED)
-    assert 2 * f.info()["table_bytes"] == bytes64
+    assert 2 * f.info()["table_bytes"] == bytes64 and 2 * f.info()["apply_bytes"] == abytes64
     for k in keys:
         got, want = f.table(*k), t64[k].astype(np.float32).astype(np.float64)
         if storage == nwick.FACTOR_DEVICE:
