@@ -109,12 +109,15 @@ __device__ inline float2 pair2<float2>(double a, double b) { return make_float2(
 //   reverse  out_r[s] = sum_t Tab[s][t] x[rdet[t]]   (a group of q, target p),
 // since Tab(p, q) = Tab(q, p)^T.  Partial sums per row chunk (forward) and per
 // column strip (reverse) are reduced in a fixed order by k_fac_dual_reduce.
+// sym: a same-parent table (symmetric, fdet = rdet, one output): only its
+// upper triangle is read, s <= t forward and s < t reverse, both summed into
+// out_f.
 struct DualItem {
     const void *tab;
     void *out_f, *out_r;
     const int *fdet, *rdet;  // determinant per row / column, -1 = no weight
     double2 *fpart, *rpart;  // [nchunk][nt][NV], [nstrip][ns][NV]
-    int ns, nt, nchunk, nstrip;
+    int ns, nt, nchunk, nstrip, sym, pad;
 };
 #ifndef NW_DUAL_ROWS
 #define NW_DUAL_ROWS 256
@@ -361,6 +364,22 @@ __global__ __launch_bounds__(256) void k_fac_dual(const DualItem *__restrict__ i
     constexpr int VEC = sizeof(T2) == 8 ? 2 : 1;
     const int cbase = strip * DUAL_COLS + wave * 256 + lane * VEC;
     auto col = [&](int j) { return cbase + (j / VEC) * 64 * VEC + j % VEC; };
+    if (it.sym && r0 > strip * DUAL_COLS + DUAL_COLS - 1) {
+        // wholly below the diagonal of a symmetric table: zero partials
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (col(j) < it.nt) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) it.fpart[((size_t)chunk * it.nt + col(j)) * NV + v] = make_double2(0.0, 0.0);
+            }
+        for (int i = threadIdx.x; i < nr; i += 256) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) it.rpart[((size_t)strip * it.ns + r0 + i) * NV + v] = make_double2(0.0, 0.0);
+        }
+        return;  // uniform over the workgroup
+    }
+    // a symmetric table's elements count forward for s <= t, reverse for s < t
+    const bool sym = it.sym != 0;
     for (int i = threadIdx.x; i < nr; i += 256) {
         const int d = it.fdet[r0 + i];
 #pragma unroll
@@ -409,10 +428,12 @@ __global__ __launch_bounds__(256) void k_fac_dual(const DualItem *__restrict__ i
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const double es = (double)e[j].x, ef = (double)e[j].y;
-                    fS[j][v] = fma(w, es, fS[j][v]);
-                    fF[j][v] = fma(w, ef, fF[j][v]);
-                    pS = fma(es, wr[j][v], pS);
-                    pF = fma(ef, wr[j][v], pF);
+                    const double wf = (sym && r0 + i > col(j)) ? 0.0 : w;
+                    const double wrj = (sym && r0 + i >= col(j)) ? 0.0 : wr[j][v];
+                    fS[j][v] = fma(wf, es, fS[j][v]);
+                    fF[j][v] = fma(wf, ef, fF[j][v]);
+                    pS = fma(es, wrj, pS);
+                    pF = fma(ef, wrj, pF);
                 }
                 a[(r * NV + v) * 2] = pS;
                 a[(r * NV + v) * 2 + 1] = pF;
@@ -460,9 +481,15 @@ __global__ void k_fac_dual_reduce(const DualItem *__restrict__ items)
                 S += e.x;
                 F += e.y;
             }
+            if (it.sym)  // the lower triangle's share, from the transposed reads
+                for (int k = 0; k < it.nstrip; ++k) {
+                    const double2 e = it.rpart[((size_t)k * it.ns + i) * NV + v];
+                    S += e.x;
+                    F += e.y;
+                }
             ((T2 *)it.out_f)[(size_t)i * NV + v] = pair2<T2>(S, F);
         }
-    } else if (i < it.nt + it.ns) {
+    } else if (!it.sym && i < it.nt + it.ns) {
         const int r = i - it.nt;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -663,7 +690,7 @@ SpinComp spin_comp(const PackedDet &d, int sp)
 // and the reverse group gq of q (target p); det_off: its fdet (ns entries)
 // then rdet (nt entries) in the uploaded determinant-index array.
 struct DualSel {
-    int q, p, sp, gp, gq;
+    int q, p, sp, gp, gq;  // q == p: a same-parent (symmetric) table, gq = gp
     long long det_off;
 };
 
@@ -826,13 +853,14 @@ void make_plans(nw_factor *f, int lo, int nv)
         const DualSel &D = f->duals[d];
         const int ns = f->par[D.p].ncomp[D.sp], nt = f->par[D.q].ncomp[D.sp];
         DualItem it{};
+        it.sym = D.q == D.p ? 1 : 0;
         it.tab = tbase + es * (size_t)f->tab_off[(D.q * nref + D.p) * 2 + D.sp];
         it.out_f = mbase + es * mo_d;
         mo_d += (size_t)nt * nv;
-        it.out_r = mbase + es * mo_d;
-        mo_d += (size_t)ns * nv;
+        it.out_r = it.sym ? it.out_f : mbase + es * mo_d;
+        if (!it.sym) mo_d += (size_t)ns * nv;
         it.fdet = f->d_dualdet.p + D.det_off;
-        it.rdet = it.fdet + ns;
+        it.rdet = it.sym ? it.fdet : it.fdet + ns;
         it.ns = ns;
         it.nt = nt;
         it.nchunk = (ns + DUAL_ROWS - 1) / DUAL_ROWS;
@@ -858,7 +886,8 @@ void make_plans(nw_factor *f, int lo, int nv)
     };
     uint64_t bytes = 0;
     for (const DualSel &D : f->duals)
-        bytes += (uint64_t)f->par[D.p].ncomp[D.sp] * (uint64_t)f->par[D.q].ncomp[D.sp] * es;
+        bytes += (D.q == D.p ? (uint64_t)f->par[D.p].ncomp[D.sp] * (f->par[D.p].ncomp[D.sp] + 1ull) / 2
+                             : (uint64_t)f->par[D.p].ncomp[D.sp] * (uint64_t)f->par[D.q].ncomp[D.sp]) * es;
     for (int q = 0; q < nref; ++q) {
         const ParentSpace &Q = f->par[q];
         if (Q.nt == 0) continue;
@@ -957,7 +986,8 @@ size_t mat_elems_needed(const nw_factor *f)
 {
     // the same walk as make_plans without pointers (one vector)
     size_t dual = 0;
-    for (const DualSel &D : f->duals) dual += (size_t)f->par[D.p].ncomp[D.sp] + (size_t)f->par[D.q].ncomp[D.sp];
+    for (const DualSel &D : f->duals)
+        dual += (size_t)f->par[D.q].ncomp[D.sp] + (D.q == D.p ? 0 : (size_t)f->par[D.p].ncomp[D.sp]);
     size_t need = dual;
     for (int q = 0; q < f->nref; ++q) {
         const ParentSpace &Q = f->par[q];
@@ -999,6 +1029,33 @@ void select_duals(nw_factor *f, const std::vector<uint32_t> &midx, const std::ve
         }
         return best;
     };
+    // same-parent tables: the dominant group reads the symmetric table once,
+    // its upper triangle only
+    for (int p = 0; p < f->nref; ++p) {
+        const ParentSpace &P = f->par[p];
+        if (P.nt == 0) continue;
+        for (int sp = 0; sp < 2; ++sp) {
+            const int gp = largest(P, sp == 0 ? 1 : 0);
+            if (gp < 0) continue;
+            const Group &GP = P.groups[gp];
+            const int ns = P.ncomp[sp];
+            if (4 * (GP.m1 - GP.m0) < ns) continue;
+            std::vector<int> fd(ns, -1);
+            bool ok = true;
+            for (int k = GP.m0; k < GP.m1 && ok; ++k) {
+                int &slot = fd[midx[P.m_off + k]];
+                ok = slot < 0;
+                slot = (int)mdet[P.m_off + k];
+            }
+            if (!ok) continue;
+            const int idx = (int)f->duals.size();
+            f->duals.push_back({p, p, sp, gp, gp, (long long)dets.size()});
+            dets.insert(dets.end(), fd.begin(), fd.end());
+            f->dual_of[dual_key(f, p, p, gp)] = {idx, 0};
+            f->dual_part += (size_t)((ns + DUAL_ROWS - 1) / DUAL_ROWS) * ns +
+                            (size_t)((ns + DUAL_COLS - 1) / DUAL_COLS) * ns;
+        }
+    }
     for (int q = 0; q < f->nref; ++q)
         for (int p = q + 1; p < f->nref; ++p) {
             const ParentSpace &Q = f->par[q], &P = f->par[p];
