@@ -109,6 +109,141 @@ __global__ __launch_bounds__(NH) void k_mlp_fwd(const float *__restrict__ X, con
     }
 }
 
+typedef __bf16 mlp_bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short mlp_u8 __attribute__((ext_vector_type(8)));
+typedef float mlp_f4 __attribute__((ext_vector_type(4)));
+
+// v = h + m + l in BF16, each round to nearest (24 significant bits in all).
+__device__ inline void mlp_split3(float v, __bf16 &h, __bf16 &m, __bf16 &l)
+{
+    h = (__bf16)v;
+    const float r = v - (float)h;
+    m = (__bf16)r;
+    l = (__bf16)(r - (float)m);
+}
+
+constexpr int IW = 4;  // waves per inference workgroup, 16 samples per wave tile
+
+// Inference y = W3 relu(W2 relu(W1 x + b1) + b2) + b3 (the forward of
+// k_mlp_fwd without stored activations) on BF16 MFMA,
+// v_mfma_f32_16x16x32_bf16 with D[sample][unit] += A[sample][k] B[k][unit]:
+// the BF16 weights are exact operands, x and the layer-1 activations are
+// split into three BF16 parts (hi + mid + lo), so every product carries
+// FP32 accuracy and sums run in FP32.  Lane l holds A[l % 16][8 (l / 16) +
+// 0..7], B[8 (l / 16) + 0..7][l % 16] and D[4 (l / 16) + 0..3][l % 16];
+// layer-1 results pass to layer 2 through LDS, W2 stays in LDS.  All waves
+// of a workgroup run the same number of tile rounds (inactive ones idle),
+// so the barriers are uniform.
+__global__ __launch_bounds__(64 * IW) void k_mlp_infer(const float *__restrict__ X, long long n, Net net,
+                                                       float *__restrict__ y)
+{
+    constexpr int LW = NH + 8;       // bf16 per LDS row of W2 (16-byte pad)
+    constexpr int LH = NH + 4;       // floats per LDS row of the activations
+    __shared__ uint16_t w2s[NH * LW];  // W2[out][in]
+    __shared__ float hs[IW][16 * LH];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+    const int nin = net.nin;
+    for (int e = tid; e < NH * NH; e += 64 * IW) w2s[(e / NH) * LW + e % NH] = net.W2[e];
+    mlp_bf8 b1f[NH / 16];
+    float bias1[NH / 16], bias2[NH / 16], w3v[NH / 16];
+#pragma unroll
+    for (int ob = 0; ob < NH / 16; ++ob) {
+        const int j = 16 * ob + lr;
+        mlp_u8 u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = 8 * lk + q;
+            u[q] = k < nin ? net.W1[j * nin + k] : (unsigned short)0;
+        }
+        b1f[ob] = __builtin_bit_cast(mlp_bf8, u);
+        bias1[ob] = net.b1[j];
+        bias2[ob] = net.b2[j];
+        w3v[ob] = bf2f(net.W3[j]);
+    }
+    const float b3 = net.b3[0];
+    float *hw = hs[wave];
+    const long long ntile = (n + 15) / 16;
+    for (long long tb = blockIdx.x; tb * IW < ntile; tb += gridDim.x) {
+        const long long t = tb * IW + wave;
+        const bool active = t < ntile;
+        const long long r0 = t * 16;
+        __syncthreads();  // W2 in place (first round); this wave's previous tile read
+        if (active) {
+            // layer 1
+            mlp_bf8 xh, xm, xl;
+            const long long r = r0 + lr;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int k = 8 * lk + q;
+                const float v = (r < n && k < nin) ? X[(size_t)r * nin + k] : 0.f;
+                __bf16 h, m, l;
+                mlp_split3(v, h, m, l);
+                xh[q] = h;
+                xm[q] = m;
+                xl[q] = l;
+            }
+#pragma unroll
+            for (int ob = 0; ob < NH / 16; ++ob) {
+                mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, b1f[ob], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xm, b1f[ob], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, b1f[ob], acc, 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float z = acc[i] + bias1[ob];
+                    hw[(4 * lk + i) * LH + 16 * ob + lr] = z > 0.f ? z : 0.f;
+                }
+            }
+        }
+        __syncthreads();
+        if (active) {
+            // layer 2
+            mlp_f4 acc2[NH / 16];
+#pragma unroll
+            for (int ob = 0; ob < NH / 16; ++ob) acc2[ob] = mlp_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kc = 0; kc < NH / 32; ++kc) {
+                mlp_bf8 ah, am, al;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    __bf16 h, m, l;
+                    mlp_split3(hw[lr * LH + 32 * kc + 8 * lk + q], h, m, l);
+                    ah[q] = h;
+                    am[q] = m;
+                    al[q] = l;
+                }
+#pragma unroll
+                for (int ob = 0; ob < NH / 16; ++ob) {
+                    const mlp_bf8 bw = *reinterpret_cast<const mlp_bf8 *>(&w2s[(16 * ob + lr) * LW + 32 * kc + 8 * lk]);
+                    acc2[ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bw, acc2[ob], 0, 0, 0);
+                    acc2[ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bw, acc2[ob], 0, 0, 0);
+                    acc2[ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bw, acc2[ob], 0, 0, 0);
+                }
+            }
+            // layer 3: each lane's 8 units for its 4 samples, then the 16 lanes of a sample group
+            float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ob = 0; ob < NH / 16; ++ob)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float z = acc2[ob][i] + bias2[ob];
+                    part[i] += w3v[ob] * (z > 0.f ? z : 0.f);
+                }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) part[i] += __shfl_xor(part[i], o, 16);
+            if (lr == 0) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const long long row = r0 + 4 * lk + i;
+                    if (row < n) y[row] = b3 + part[i];
+                }
+            }
+        }
+    }
+}
+
 // Backward of the weighted mean squared error L = sum_r w_r (y_r - t_r)^2 / (2 n)
 // for the rows of one forward pass (H1, H2, y stored by k_mlp_fwd).  Each
 // workgroup sums the gradients of its tiles in a fixed order into its own
@@ -331,6 +466,12 @@ int grid_for(long long ntile) { return (int)std::max<long long>(1, std::min<long
 
 void forward(nw_mlp *p, const float *dX, const int *didx, long long n, float *dy, float *dH1, float *dH2)
 {
+    if (!didx && !dH1 && !dH2 && p->nin <= 32) {  // inference: BF16 MFMA, no stored activations
+        const long long tiles = (n + 15) / 16;
+        hipLaunchKernelGGL(k_mlp_infer, dim3(grid_for((tiles + IW - 1) / IW)), dim3(64 * IW), 0, p->st, dX, n,
+                           p->net(), dy);
+        return;
+    }
     hipLaunchKernelGGL(k_mlp_fwd, dim3(grid_for((n + TSF - 1) / TSF)), dim3(NH), 0, p->st, dX, didx, n, p->net(), dy,
                        dH1, dH2);
 }

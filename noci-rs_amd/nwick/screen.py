@@ -234,6 +234,18 @@ def dynamic_features(xp, Fs, par, pf, hpar, hdom, hdiag, energy):
     return F
 
 
+def dynamic_features_into(X, mu, sd, par, pf, hpar, hdom, hdiag, energy):
+    """dynamic_features written normalised, (F - mu) / sd, into the
+    state-dependent columns of the device matrix X in place (the static
+    columns of X already hold their normalised values)."""
+    t = pf[par]
+    X[:, 12:20] = (t[:, :8] - mu[12:20]) / sd[12:20]
+    X[:, 23] = (t[:, 8] - mu[23]) / sd[23]
+    X[:, 20] = (hdom.abs().add(1e-14).log10() - mu[20]) / sd[20]
+    den = (hdiag - energy).abs().clamp(min=1e-3)
+    X[:, 21] = ((hpar.abs() / den + 1e-14).log10() - mu[21]) / sd[21]
+
+
 def features(sysm, cand: Candidates, fmo, state, couplings):
     """All 24 pair descriptors on the host (numpy): static_features then
     dynamic_features.  couplings = (H_Dp, H_Dd, H_DD)."""
@@ -260,6 +272,7 @@ class ExactScorer:
         n = len(cand)
         req = np.stack([np.arange(n), np.arange(n)], 1).astype(np.uint64)
         self.hdiag, self.sdiag = eng.hs_batched(rec, req, mode=mode)
+        self.hdiag_d = torch.from_numpy(np.ascontiguousarray(self.hdiag)).to(self.dev)
         self.n_exact = 0
 
     def coupling(self, ket_rec, ket_of):
@@ -283,27 +296,59 @@ class ExactScorer:
         self.eng.project_hs(allrec, 1, one, v=v, s=s, mode=self.mode)
         return v
 
-    def scores(self, cur_rec, coef, energy, idx=None):
-        """Exact scores of the candidates idx (all when None), in idx order."""
+    def current(self, cur_rec, coef):
+        """The current space and its coefficients on the device, for repeated
+        scores_d calls within one selection iteration."""
         torch = self.torch
         cur = torch.from_numpy(cur_rec.view(np.uint8).copy()).to(self.dev).view(-1, DET_DTYPE.itemsize)
+        c = torch.as_tensor(np.ascontiguousarray(coef), dtype=torch.float64, device=self.dev)
+        return cur, c
+
+    def scores_d(self, cur_rec, coef, energy, idx=None, cur=None):
+        """Exact scores |<D|H - E S|Psi>|^2 / max(|E - H_DD|, 1e-3) of the
+        candidates idx (a device index tensor or array; all when None), in idx
+        order, as a device tensor.  cur: current() of (cur_rec, coef)."""
+        torch = self.torch
+        cur, c = cur if cur is not None else self.current(cur_rec, coef)
         if idx is None:
             sub = self.rec_d
-            n = sub.shape[0]
-            hd = self.hdiag
+            hd = self.hdiag_d
         else:
-            it = torch.as_tensor(np.asarray(idx, dtype=np.int64), device=self.dev)
+            it = idx if torch.is_tensor(idx) else torch.as_tensor(np.asarray(idx, dtype=np.int64), device=self.dev)
             sub = self.rec_d.index_select(0, it)
-            n = len(idx)
-            hd = self.hdiag[idx]
+            hd = self.hdiag_d.index_select(0, it)
+        n = sub.shape[0]
         allrec = torch.cat([cur, sub]).contiguous()
         v = torch.empty(n, dtype=torch.float64, device=self.dev)
         s = torch.empty(n, dtype=torch.float64, device=self.dev)
-        c = torch.as_tensor(np.ascontiguousarray(coef), dtype=torch.float64, device=self.dev)
         self.eng.project_hs(allrec, cur.shape[0], c, v=v, s=s, mode=self.mode)
-        num = (v - energy * s).cpu().numpy()
+        num = v - energy * s
         self.n_exact += n
-        return num * num / np.maximum(np.abs(energy - hd), 1e-3)
+        return num * num / (energy - hd).abs().clamp(min=1e-3)
+
+    def scores(self, cur_rec, coef, energy, idx=None):
+        """scores_d as a host array."""
+        return self.scores_d(cur_rec, coef, energy, idx).cpu().numpy()
+
+
+def pick_best(sc_d, taken_d, k):
+    """select_candidates (src/snoci/operators.rs:1064-1085) with threshold 0:
+    the k best scores above zero among the candidates not yet taken, best
+    first, equal scores in index order (a descending stable sort), on the
+    scores' device; returns host indices."""
+    import torch
+    vals, idx = torch.sort(sc_d.masked_fill(taken_d, -1.0), descending=True, stable=True)
+    return idx[:k][vals[:k] > 0].cpu().numpy()
+
+
+def kth_best_d(sc_d, k):
+    """The k-th largest exact score of a device score vector whose entries
+    without a score are -1; 0 when fewer than k are scored."""
+    import torch
+    if k > sc_d.shape[0]:
+        return 0.0
+    v = float(torch.topk(sc_d, k).values[-1])
+    return v if v >= 0.0 else 0.0
 
 
 def solve_current(eng, cur_rec, mode=0, tol=1e-8):
@@ -346,6 +391,67 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
     fmo = reference_fock(sysm)
     rng = np.random.default_rng(seed)
     taken = np.zeros(len(cand), dtype=bool)
+    taken_d = torch.zeros(len(cand), dtype=torch.bool, device=dev)
+    hdom_of = {}
+    Xcur = None
+
+    def screened(cur_rec, c, e, pf, hdom):
+        """One screened iteration: network ranking, exact scores of the top
+        fraction and of further chunks until chunk_patience chunks in a row
+        hold no candidate within chunk_ratio of the k_add-th best exact score,
+        a random audit of the rest, and the exhaustive fallback on an audit
+        miss.  Returns (scores, exact evaluations, misses, evaluated top, 0/1)."""
+        cur = scorer.current(cur_rec, c)
+        # the normalised descriptors: static columns kept from training,
+        # the state-dependent ones rewritten
+        dynamic_features_into(Xcur, mu, sd, par_d, pf, hpar_d, hdom, hdiag_d, e)
+        pred = torch.empty(N, dtype=torch.float32, device=dev)
+        net.forward(Xcur, out=pred)
+        pred.masked_fill_(taken_d, -float("inf"))
+        nfree = N - int(taken.sum())
+        order = torch.argsort(pred, descending=True)[:nfree]
+        sc_d = torch.full((N,), -1.0, dtype=torch.float64, device=dev)
+        chunk = max(k_add, int(chunk_frac * N))
+        done = min(nfree, max(k_add, int(top_frac * N)))
+        first = torch.sort(order[:done]).values
+        s_f = scorer.scores_d(cur_rec, c, e, first, cur=cur)
+        sc_d[first] = s_f
+        best = torch.topk(s_f, min(k_add, len(s_f))).values  # running k_add best exact scores
+        quiet = 0
+        while done < nfree and done < max_frac * N:
+            kth = float(best[-1]) if len(best) >= k_add else 0.0
+            nxt = torch.sort(order[done:done + chunk]).values
+            s_n = scorer.scores_d(cur_rec, c, e, nxt, cur=cur)
+            sc_d[nxt] = s_n
+            best = torch.topk(torch.cat([best, s_n]), min(k_add, len(best) + len(s_n))).values
+            done += len(nxt)
+            quiet = quiet + 1 if float(s_n.max()) < chunk_ratio * kth else 0
+            if quiet >= chunk_patience:
+                break
+        top = order[:done]
+        rest = order[done:]
+        m = min(len(rest), max(1, int(audit_frac * N)))
+        audit = torch.unique(rest[torch.randint(len(rest), (m,), generator=gen, device=dev)]) \
+            if len(rest) else rest
+        n_eval = done
+        misses = 0
+        if len(audit):
+            s_a = scorer.scores_d(cur_rec, c, e, audit, cur=cur)
+            sc_d[audit] = s_a
+            n_eval += len(audit)
+            # the exact evaluation judges the network: an audited candidate it
+            # ranked out that beats the k_add-th best is a miss
+            allbest = torch.topk(torch.cat([best, s_a]), min(k_add, len(best) + len(s_a))).values
+            kth = float(allbest[-1]) if len(allbest) >= k_add else 0.0
+            misses = int((s_a > kth).sum())
+        fell = 0
+        if misses > 0 or done >= max_frac * N:
+            fell = 1
+            sc_d = scorer.scores_d(cur_rec, c, e, cur=cur)
+            n_eval += N
+        return sc_d, n_eval, misses, top, fell
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
     res = SelectionResult()
     Xs, ys = [], []
     net, mu, sd = None, None, None
@@ -376,67 +482,33 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
         if learned:
             dom = int(np.argmax(wpar))
             pf = torch.from_numpy(parent_table(sysm, state).astype(np.float32)).to(dev)
-            hdom = scorer.project_single(ref_rec[dom:dom + 1])
-            F = dynamic_features(torch, Fs, par_d, pf, hpar_d, hdom, hdiag_d, e)
+            if dom not in hdom_of:  # couplings with the dominant reference, once per reference
+                hdom_of[dom] = scorer.project_single(ref_rec[dom:dom + 1])
+            hdom = hdom_of[dom]
+            if not use_net:
+                F = dynamic_features(torch, Fs, par_d, pf, hpar_d, hdom, hdiag_d, e)
         if not use_net:
-            sc = scorer.scores(cur_rec, c, e)
+            sc_d = scorer.scores_d(cur_rec, c, e)
             n_eval = N
             if learned:
                 Xs.append(F)
-                ys.append(np.log10(sc + 1e-20).astype(np.float32))
+                ys.append(torch.log10(sc_d + 1e-20).float())
         else:
-            Xn = ((F - mu) / sd).contiguous()
-            pred = torch.empty(N, dtype=torch.float32, device=dev)
-            net.forward(Xn, out=pred)
-            pred[torch.from_numpy(taken).to(dev)] = -float("inf")
-            order = torch.argsort(pred, descending=True).cpu().numpy()[:N - int(taken.sum())]
-            sc = np.full(N, -1.0)
-            # exact evaluation in order of predicted score: the top fraction,
-            # then further chunks until chunk_patience chunks in a row hold no
-            # candidate within chunk_ratio of the current k_add-th best exact score
-            chunk = max(k_add, int(chunk_frac * N))
-            done = min(len(order), max(k_add, int(top_frac * N)))
-            first = np.sort(order[:done])
-            sc[first] = scorer.scores(cur_rec, c, e, first)
-            quiet = 0
-            while done < len(order) and done < max_frac * N:
-                valid = np.sort(sc[sc >= 0])
-                kth = valid[-k_add] if len(valid) >= k_add else 0.0
-                nxt = np.sort(order[done:done + chunk])
-                sc[nxt] = scorer.scores(cur_rec, c, e, nxt)
-                done += len(nxt)
-                quiet = quiet + 1 if sc[nxt].max() < chunk_ratio * kth else 0
-                if quiet >= chunk_patience:
-                    break
-            top = order[:done]
-            rest = order[done:]
-            audit = np.sort(rng.choice(rest, min(len(rest), max(1, int(audit_frac * N))), replace=False)) \
-                if len(rest) else np.zeros(0, dtype=np.int64)
-            if len(audit):
-                sc[audit] = scorer.scores(cur_rec, c, e, audit)
-            n_eval = done + len(audit)
-            # the exact evaluation judges the network: an audited candidate
-            # it ranked out that beats the k_add-th pick is a miss
-            valid = np.sort(sc[(sc >= 0) & ~taken])
-            kth = valid[-k_add] if len(valid) >= k_add else 0.0
-            misses = int((sc[audit] > kth).sum()) if len(audit) else 0
+            sc_d, n_eval, misses, top, fell = screened(cur_rec, c, e, pf, hdom)
             res.audit_misses.append(misses)
-            if misses > 0 or done >= max_frac * N:
-                res.fallbacks += 1
-                sc = scorer.scores(cur_rec, c, e)
-                n_eval += N
-        sc = np.where(taken, -1.0, sc)
-        pick = np.argsort(-sc, kind="stable")[:k_add]
-        pick = pick[sc[pick] > 0]
+            res.fallbacks += fell
+        pick = pick_best(sc_d, taken_d, k_add)
         res.screen_s.append(time.perf_counter() - t0)
         res.exact_evals.append(int(n_eval))
         if use_net and measure_recall:
             full = scorer.scores(cur_rec, c, e)
             scorer.n_exact -= N
             best = np.argsort(-np.where(taken, -1.0, full))[:k_add]
-            res.recall.append(float(np.isin(best, top).mean()))
+            res.recall.append(float(np.isin(best, top.cpu().numpy()).mean()))
             res.captured.append(float(full[pick].sum() / full[best].sum()))
+        cur_it, c_it = cur_rec, c  # this iteration's current space, for the warm-up below
         taken[pick] = True
+        taken_d[torch.from_numpy(pick).to(dev)] = True
         res.selected.append(pick)
         cur_rec = np.concatenate([cur_rec, scorer.rec[np.sort(pick)]])
         cur_rec = cur_rec[np.argsort(cur_rec["parent"], kind="stable")]  # runs of one parent together
@@ -445,7 +517,7 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
                 f"screen {res.screen_s[-1]:.3f} s" + (f" misses={res.audit_misses[-1]}" if use_net else ""))
         if learned and it == learn_iters - 1:
             X = torch.cat(Xs)
-            y = np.concatenate(ys)
+            y = torch.cat(ys).cpu().numpy()
             mu, sd = X.mean(0), X.std(0) + 1e-6
             Xn = ((X - mu) / sd).contiguous()
             # emphasise the top of the distribution: the selection only needs its order
@@ -458,10 +530,15 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
             # first use of the inference path (network forward, full-size sort,
             # masked assignment) once here, with the training time, so the
             # screened iterations carry only their own work
-            probe = torch.empty(N, dtype=torch.float32, device=dev)
-            net.forward(Xn[:min(N, len(Xn))].contiguous(), out=probe[:min(N, len(Xn))])
-            probe[torch.from_numpy(taken).to(dev)] = -float("inf")
-            torch.argsort(probe, descending=True).cpu()
+            Xcur = ((Fs - mu) / sd).contiguous()
+            # first use of the screened path (allocations and kernels at their
+            # sizes) once here, with the training time, on the state of this
+            # iteration; its result is discarded
+            n_before = scorer.n_exact
+            screened(cur_it, c_it, e, pf, hdom)
+            pick_best(torch.zeros(N, dtype=torch.float64, device=dev), taken_d, k_add)
+            scorer.n_exact = n_before
+            torch.cuda.synchronize()
             res.train.update({"samples": int(len(y)), "epochs": epochs, "loss": [float(v) for v in el],
                               "seconds": time.perf_counter() - t1})
             del Xs, X, Xn

@@ -1,5 +1,6 @@
 """Host logic of the learned screener: vectorised candidate records against
-the per-determinant generator (masks, occupations, excitation phases)."""
+the per-determinant generator (masks, occupations, excitation phases), and
+the candidate pick and k-th best score against numpy's stable sort."""
 import numpy as np
 
 from nwick import screen
@@ -39,3 +40,31 @@ def test_candidates_unique_and_grouped():
     keys = np.concatenate([c.parent[:, None], c.ah, c.ap, c.bh, c.bp], 1)
     assert len(np.unique(keys, axis=0)) == len(c)
     assert np.all(np.diff(c.parent.astype(int)) >= 0)
+
+
+def test_pick_best_and_kth_match_stable_sort():
+    """The device pick equals numpy's descending stable sort of the untaken
+    scores (ties in index order), cut at k and at zero; kth_best_d is the
+    k-th largest scored entry (unscored entries are -1), or 0."""
+    import torch
+    from nwick.screen import kth_best_d, pick_best
+    rng = np.random.default_rng(9)
+    for n, k in [(1000, 64), (5000, 1), (300, 299), (300, 300), (300, 500), (10, 3)]:
+        for kind in ("ties", "unique", "sparse"):
+            if kind == "ties":
+                sc = rng.integers(-1, 5, size=n).astype(float)
+            elif kind == "unique":
+                sc = rng.random(n)
+            else:
+                sc = np.full(n, -1.0)
+                m = max(1, n // 10)
+                sc[rng.choice(n, m, replace=False)] = rng.integers(0, 3, size=m)
+            taken = rng.random(n) < 0.2
+            masked = np.where(taken, -1.0, sc)
+            want = np.argsort(-masked, kind="stable")[:k]
+            want = want[masked[want] > 0]
+            got = pick_best(torch.tensor(sc), torch.tensor(taken), k)
+            assert np.array_equal(got, want), (n, k, kind)
+            valid = np.sort(sc[sc >= 0])
+            kth = valid[-k] if len(valid) >= k else 0.0
+            assert kth_best_d(torch.tensor(sc), k) == kth, (n, k, kind)
