@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_eval_blocks(const EvalBlock *__restrict
         if (!valid) return;
     }
     const PairInfo &P = a.pairs[B.pair];
-    const double *slab = a.slab + P.base;
+    const slab_t slab = (slab_t)(a.slab + P.base);
     double v0 = 0.0, v1 = 0.0;
     if (valid) {
         const PackedDet bra = a.dets[i], ket = a.dets[j];
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_eval_reqs(const EvalReq *__restrict__ r
     const EvalReq r = reqs[q];
     const PackedDet bra = a.dets[r.a], ket = a.dets[r.b];
     const PairInfo &P = a.pairs[(int)bra.parent * a.nref + ket.parent];
-    const double *slab = a.slab + P.base;
+    const slab_t slab = (slab_t)(a.slab + P.base);
     double v0, v1;
     if constexpr (KIND == EVAL_HS) {
         if constexpr (M0) hs_m0<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);

@@ -8,6 +8,11 @@
 
 namespace nw {
 
+// Slab data through a global-address-space pointer: the evaluators read it
+// with global (not flat) loads also where the slab base was read from
+// memory (multi-system batches).
+typedef const double __attribute__((address_space(1))) *slab_t;
+
 // ---------------------------------------------------------------------------
 // Small dense helpers, fully unrolled for compile-time sizes.
 // ---------------------------------------------------------------------------
@@ -132,7 +137,7 @@ __device__ inline void labels(const uint8_t *bi, int brank, const uint8_t *ki, i
 
 // D_{ij} = X[r_i, c_j] (i >= j), Y[r_i, c_j] (i < j)   (build_d, src/maths/wick.rs:59)
 template <int L>
-__device__ inline void build_d(const double *X, const double *Y, int n, const int *rows, const int *cols, double *d)
+__device__ inline void build_d(slab_t X, slab_t Y, int n, const int *rows, const int *cols, double *d)
 {
 #pragma unroll
     for (int i = 0; i < L; ++i)
@@ -144,12 +149,12 @@ __device__ inline void build_d(const double *X, const double *Y, int n, const in
 // m_alpha = m_beta = 0 element.
 // ---------------------------------------------------------------------------
 template <int L>
-__device__ inline double same_m0(const double *slab, const long long *o, const SpinScal &sc, int n, const int *rows,
+__device__ inline double same_m0(slab_t slab, const long long *o, const SpinScal &sc, int n, const int *rows,
                                  const int *cols, const double *d, const double *cof, double det)
 {
     double acc = (sc.f0h[0] + 0.5 * sc.v0[0]) * det;
     if constexpr (L > 0) {
-        const double *hcol0 = slab + o[O_HCOL0];
+        slab_t hcol0 = slab + o[O_HCOL0];
         double repl = 0.0;
 #pragma unroll
         for (int z = 0; z < L; ++z)
@@ -158,7 +163,7 @@ __device__ inline double same_m0(const double *slab, const long long *o, const S
         acc -= repl;
     }
     if constexpr (L >= 2) {
-        const double *J = slab + o[O_J + 0];
+        slab_t J = slab + o[O_J + 0];
 #pragma unroll
         for (int eta = 0; eta < L; ++eta)
 #pragma unroll
@@ -189,7 +194,7 @@ __device__ inline double same_m0(const double *slab, const long long *o, const S
 }
 
 template <int LA, int LB>
-__device__ inline void hs_m0(const PairInfo &P, const double *slab, const PackedDet &bra, const PackedDet &ket,
+__device__ inline void hs_m0(const PairInfo &P, slab_t slab, const PackedDet &bra, const PackedDet &ket,
                              int nmo, double enuc, double &H, double &S)
 {
     constexpr int LA1 = LA > 0 ? LA : 1, LB1 = LB > 0 ? LB : 1;
@@ -212,7 +217,7 @@ __device__ inline void hs_m0(const PairInfo &P, const double *slab, const Packed
     double same_b = same_m0<LB>(slab, ob, sb, nmo, rb, cb, db, cofb, detb);
     double h2ab = P.meta[M_VAB0] * deta * detb;
     if constexpr (LA > 0 && LB > 0) {
-        const double *II = slab + P.off[O_II];
+        slab_t II = slab + P.off[O_II];
 #pragma unroll
         for (int z = 0; z < LA; ++z)
 #pragma unroll
@@ -247,7 +252,7 @@ __device__ inline int jslot(int mi, int mj, int mk, int ml, int &swap)
 __device__ inline int bitk(unsigned b, int k) { return (b >> k) & 1; }
 
 template <int L>
-__device__ inline void same_gen(const double *slab, const long long *o, const SpinScal &sc, int n, const int *rows,
+__device__ inline void same_gen(slab_t slab, const long long *o, const SpinScal &sc, int n, const int *rows,
                                 const int *cols, const double *d0, const double *d1, double tol, int mode, double &s_out,
                                 double &h1_out, double &h2_out)
 {
@@ -279,7 +284,7 @@ __device__ inline void same_gen(const double *slab, const long long *o, const Sp
 #pragma unroll
             for (int k = 0; k < L; ++k) {
                 const int mk = bitk(cbits, k);
-                const double *fsl = slab + o[O_FH + m1 * 2 + mk];
+                slab_t fsl = slab + o[O_FH + m1 * 2 + mk];
                 double corr = 0.0;
 #pragma unroll
                 for (int r = 0; r < L; ++r) corr += (fsl[cols[k] * n + rows[r]] - mix[r * L + k]) * cof[r * L + k];
@@ -294,7 +299,7 @@ __device__ inline void same_gen(const double *slab, const long long *o, const Sp
         for (int k = 0; k < L; ++k) {
             const int mk = bitk(cbits, k);
             const int vi = mode == 0 ? (m1 * 2 + m2) * 2 + mk : (m1 * 2 + mk) * 2 + m2;
-            const double *vsl = slab + o[O_V + vi];
+            slab_t vsl = slab + o[O_V + vi];
             double corr = 0.0;
 #pragma unroll
             for (int r = 0; r < L; ++r) corr += (vsl[cols[k] * n + rows[r]] - mix[r * L + k]) * cof[r * L + k];
@@ -319,7 +324,7 @@ __device__ inline void same_gen(const double *slab, const long long *o, const Sp
                         const int mk = bitk(cbits, kf);
                         int swap;
                         const int slot = mode == 0 ? jslot(m1, m2, mk, mj, swap) : jslot(m1, mk, m2, mj, swap);
-                        const double *J = slab + o[O_J + slot];
+                        slab_t J = slab + o[O_J + slot];
                         double v = 0.0;
 #pragma unroll
                         for (int r = 0; r < LM; ++r) {
@@ -340,7 +345,7 @@ __device__ inline void same_gen(const double *slab, const long long *o, const Sp
 }
 
 template <int LA, int LB>
-__device__ inline void hs_gen(const PairInfo &P, const double *slab, const PackedDet &bra, const PackedDet &ket,
+__device__ inline void hs_gen(const PairInfo &P, slab_t slab, const PackedDet &bra, const PackedDet &ket,
                               int nmo, double enuc, double tol, int mode, double &H, double &S)
 {
     constexpr int LA1 = LA > 0 ? LA : 1, LB1 = LB > 0 ? LB : 1;
@@ -398,7 +403,7 @@ __device__ inline void hs_gen(const PairInfo &P, const double *slab, const Packe
 #pragma unroll
             for (int k = 0; k < LA; ++k) {
                 const int mak = bitk(ba, k + 1);
-                const double *vsl = slab + P.off[O_VAB + (ma0 * 2 + mb0) * 2 + mak];
+                slab_t vsl = slab + P.off[O_VAB + (ma0 * 2 + mb0) * 2 + mak];
                 double v = 0.0;
 #pragma unroll
                 for (int r = 0; r < LA; ++r) v += vsl[ca[k] * nmo + ra[r]] * cofa[r * LA + k];
@@ -407,7 +412,7 @@ __device__ inline void hs_gen(const PairInfo &P, const double *slab, const Packe
 #pragma unroll
             for (int k = 0; k < LB; ++k) {
                 const int mbk = bitk(bb, k + 1);
-                const double *vsl = slab + P.off[O_VBA + (mb0 * 2 + ma0) * 2 + mbk];
+                slab_t vsl = slab + P.off[O_VBA + (mb0 * 2 + ma0) * 2 + mbk];
                 double v = 0.0;
 #pragma unroll
                 for (int r = 0; r < LB; ++r) v += vsl[cb[k] * nmo + rb[r]] * cofb[r * LB + k];
@@ -422,7 +427,7 @@ __device__ inline void hs_gen(const PairInfo &P, const double *slab, const Packe
 #pragma unroll
                     for (int k = 0; k < LB; ++k) {
                         const int mbk = bitk(bb, k + 1);
-                        const double *II = slab + P.off[O_II + ((ma0 * 2 + ma1) * 2 + mb0) * 2 + mbk];
+                        slab_t II = slab + P.off[O_II + ((ma0 * 2 + ma1) * 2 + mb0) * 2 + mbk];
                         double v = 0.0;
 #pragma unroll
                         for (int r = 0; r < LB; ++r) v += II[idx4(nmo, ra[i], ca[j], rb[r], cb[k])] * cofb[r * LB + k];
@@ -446,7 +451,7 @@ __device__ inline void hs_gen(const PairInfo &P, const double *slab, const Packe
 // the phase * tilde_s_prod factor.
 // ---------------------------------------------------------------------------
 template <int L>
-__device__ inline void fs_spin(const double *slab, const long long *o, const SpinScal &sc, int n, const int *rows,
+__device__ inline void fs_spin(slab_t slab, const long long *o, const SpinScal &sc, int n, const int *rows,
                                const int *cols, const double *d0, const double *d1, double tol, int mode, double &s_out,
                                double &f_out)
 {
@@ -457,7 +462,7 @@ __device__ inline void fs_spin(const double *slab, const long long *o, const Spi
         if constexpr (L > 0) dd = cofactors<L>(d0, cof);
         double f = dd * sc.f0f[0];
         if constexpr (L > 0) {
-            const double *ff = slab + o[O_FF];
+            slab_t ff = slab + o[O_FF];
 #pragma unroll
             for (int z = 0; z < L; ++z)
 #pragma unroll
@@ -496,7 +501,7 @@ __device__ inline void fs_spin(const double *slab, const long long *o, const Spi
 #pragma unroll
         for (int b = 0; b < L; ++b) {
             const int mj = bitk(bits, b + 1);
-            const double *fsl = slab + o[O_FF + mi * 2 + mj];
+            slab_t fsl = slab + o[O_FF + mi * 2 + mj];
             double corr = 0.0;
 #pragma unroll
             for (int r = 0; r < L; ++r) corr += (fsl[cols[b] * n + rows[r]] - mix[r * L + b]) * cof[r * L + b];
@@ -511,7 +516,7 @@ __device__ inline void fs_spin(const double *slab, const long long *o, const Spi
 // calculate_f_pair_wicks (src/noci/fock.rs:175-216) and the overlap
 // calculate_s_pair_wicks (src/noci/overlap.rs:106-141) of one pair.
 template <int LA, int LB>
-__device__ inline void fs_elem(const PairInfo &P, const double *slab, const PackedDet &bra, const PackedDet &ket,
+__device__ inline void fs_elem(const PairInfo &P, slab_t slab, const PackedDet &bra, const PackedDet &ket,
                                int nmo, double tol, int mode, double &F, double &S)
 {
     constexpr int LA1 = LA > 0 ? LA : 1, LB1 = LB > 0 ? LB : 1;

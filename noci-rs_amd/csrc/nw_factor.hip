@@ -132,7 +132,7 @@ constexpr int S1_CHUNK = 128;  // entries per stage-1 item (pieces of a large gr
 constexpr int S2_BATCH = 256;  // rows staged in LDS at a time
 
 template <int L>
-__device__ inline void spin_fs(const double *slab, const long long *o, const SpinScal &sc, int nmo,
+__device__ inline void spin_fs(slab_t slab, const long long *o, const SpinScal &sc, int nmo,
                                const SpinComp &bra, const SpinComp &ket, double tol, int mode, double &s, double &f)
 {
     constexpr int L1 = L > 0 ? L : 1;
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_fac_table(const PairInfo *__restrict__ 
     T2 out = pair2<T2>(0.0, 0.0);
     if (t < nt && s < ns) {
         const PairInfo &P = pairs[pair];
-        const double *slab = slab0 + P.base;
+        const slab_t slab = (slab_t)(slab0 + P.base);
         const SpinScal sc = load_spin(P, spin);
         const long long *o = P.off + (spin ? O_BB : O_AA);
         const SpinComp T = tc[t], S = scomp[s];

@@ -19,7 +19,7 @@ __host__ __device__ constexpr bool sel_has()
 }
 
 template <int SEL>
-__device__ inline void eval_code(int code, const PairInfo &P, const double *slab, const PackedDet &bra,
+__device__ inline void eval_code(int code, const PairInfo &P, slab_t slab, const PackedDet &bra,
                                  const PackedDet &ket, const BatchSys &S, double &v0, double &v1)
 {
     switch (code) {
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_eval_fused(const BatchTile *__restrict_
             const BatchSys &S = sys[T.sys];
             const PairInfo &P = S.pairs[T.pair];
             double v0 = 0.0, v1 = 0.0;
-            eval_code<SEL>(T.code, P, S.slab + P.base, S.dets[i], S.dets[j], S, v0, v1);
+            eval_code<SEL>(T.code, P, (slab_t)(S.slab + P.base), S.dets[i], S.dets[j], S, v0, v1);
             S.H[i * S.ldo + j] = v0;
             S.S[i * S.ldo + j] = v1;
             if (i != j) {

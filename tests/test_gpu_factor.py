@@ -220,6 +220,12 @@ def test_split_groups_vs_pairwise(storage):
     f = eng.factor(rec, storage=storage)
     counts = [f.info(p) for p in range(2)]
     assert all(c["ncomp_a"] > 64 for c in counts)
+    # the reference row and column of each parent are summed by dual passes
+    # (each table once for both directions, same-parent tables over their
+    # upper triangle): an apply reads well under the resident table bytes
+    info = f.info()
+    if storage == nwick.FACTOR_DEVICE:
+        assert info["apply_bytes"] < 0.8 * info["table_bytes"], info
     rng = np.random.default_rng(4)
     x = rng.normal(size=len(dets))
     lam = -0.8
