@@ -28,6 +28,19 @@ void gemm(nw_ctx *c, Uploader &up, int ta, int tb, int M, int N, int K, int lda,
     launch_gemm(c->st, ta, tb, M, N, K, lda, ldb, ldc, alpha, beta, d, (int)descs.size());
 }
 
+// NW_TRANSFORM_PASSES=3 runs the bf16x3 transform as three launches (half
+// transform + two leading passes; even n_bas <= 112, nmo <= n_bas).  It
+// moves 3/4 of the four-pass traffic but measured slower at n_bas = 100
+// (900 vs 753 us per tensor: the half transform is MFMA/LDS-bound at one
+// workgroup per CU, the leading passes read 256-byte row pieces), so the
+// four quarter passes are the default.
+bool three_launch_transform(int n, int nm)
+{
+    const char *e = std::getenv("NW_TRANSFORM_PASSES");
+    if (!e || std::strcmp(e, "3") != 0) return false;
+    return n % 2 == 0 && n <= 112 && nm <= n;
+}
+
 
 }  // namespace
 
@@ -492,10 +505,20 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                     std::vector<const float *> TAc(TA.begin(), TA.end()), TBc(TB.begin(), TB.end());
                     auto pass = bf3 ? launch_qtransform_bf3_many : launch_qtransform_f32_many;
                     auto last = bf3 ? launch_qtransform_bf3_out64_many : launch_qtransform_f32_out64_many;
-                    bool ok = pass(c->st, X.data(), C4.data(), TA.data(), nb, (long long)n * n * n, n, nm) &&
-                              pass(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm) &&
-                              pass(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm) &&
-                              last(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
+                    bool ok;
+                    if (bf3 && three_launch_transform(n, nm)) {
+                        // half transform of the (p,q) slices, then p and q as leading indices
+                        const long long nm2 = (long long)nm * nm;
+                        ok = launch_qhalf_bf3_many(c->st, X.data(), C3.data(), C4.data(), TA.data(), nb, n, nm) &&
+                             launch_qlead_bf3_many(c->st, TAc.data(), C1.data(), TB.data(), nb, n, nm, n * nm2, 1, 0, 0) &&
+                             launch_qlead_bf3_out64_many(c->st, TBc.data(), C2.data(), OUT.data(), nb, n, nm, nm2, nm,
+                                                         n * nm2, nm * nm2);
+                    } else {
+                        ok = pass(c->st, X.data(), C4.data(), TA.data(), nb, (long long)n * n * n, n, nm) &&
+                             pass(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm) &&
+                             pass(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm) &&
+                             last(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
+                    }
                     if (!ok) throw NwError(NW_E_SHAPE, "mixed-precision transform needs an even n_bas and nmo <= 112");
                 } else {
                     std::vector<const double *> X(nb), C1(nb), C2(nb), C3(nb), C4(nb);

@@ -692,6 +692,379 @@ bool launch_qtransform_bf3_out64_many(hipStream_t st, const float *const *X, con
     return launch_qtransform_bf3_t<double>(st, X, c, Y, count, rows, nl, nout);
 }
 
+// ---------------------------------------------------------------------------
+// Three-launch form of the bf16x3 four-index transform, which moves three
+// tensors' worth of HBM traffic instead of four:
+//   half:  Y2[p q][o3 o4] = sum_{r,s} c3[r][o3] c4[s][o4] X[p q][r s]
+//   lead:  Z[o1][q o3 o4]  = sum_p c1[p][o1] Y2[p][q o3 o4]
+//   lead:  Y[o1][o2 o3 o4] = sum_q c2[q][o2] Z[o1][q][o3 o4]   (batch over o1)
+// which is the same Y[o1 o2 o3 o4] as the four quarter passes.  The half
+// transform keeps a whole n x n slice and its first product in LDS; the
+// leading passes stream 64-column tiles of a row-major matrix.
+
+// Half transform of one (p,q) slice per step.  NS = ceil(n/16) tiles of 16
+// slice rows; seven waves own 16 output rows each in both products.  The
+// first product T[o4][r] = sum_s c4[s][o4] X[r][s] goes back to the same LDS
+// planes as rows o4 with K = r (zero for r >= n), the B operand of the
+// second product Y2[o3][o4] = sum_r c3[r][o3] T[o4][r].
+template <int NS, int VEC>
+__global__ __launch_bounds__(448) void k_qhalf_bf3(const QPtrs2 q, int n, int nout, long long nslice)
+{
+    constexpr int NT = 448;
+    constexpr int TR = NS * 16;        // slice rows (and T rows) held in LDS
+    constexpr int KS = (NS + 1) / 2;   // K steps of 32: KP = 32 KS >= n
+    constexpr int KP = KS * 32;
+    constexpr int LDB = KP + 8;
+    constexpr int PLANE = TR * LDB;
+    const float *__restrict__ X = (const float *)q.X[blockIdx.y];
+    const float *__restrict__ c3 = (const float *)q.c3[blockIdx.y];
+    const float *__restrict__ c4 = (const float *)q.c4[blockIdx.y];
+    float *__restrict__ Y = (float *)q.Y[blockIdx.y];
+    using VT = typename std::conditional<VEC == 4, float4, float2>::type;
+    extern __shared__ __bf16 xb[];  // planes h, m, l
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int lp = wave * 16 + lr;
+    // entries that neither the slice nor T ever writes (K columns >= TR, and
+    // the K padding before the first T) must read as zero
+    for (int e = tid; e < 3 * PLANE; e += NT) xb[e] = (__bf16)0.0f;
+    nw_bf8 ah[KS], am[KS], al[KS], bh3[KS], bm3[KS], bl3[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int l = ks * 32 + lk * 8 + j;
+            const bool ok = lp < nout && l < n;
+            __bf16 h, m, lo;
+            nw_split3(ok ? c4[(size_t)l * nout + lp] : 0.0f, h, m, lo);
+            ah[ks][j] = h;
+            am[ks][j] = m;
+            al[ks][j] = lo;
+            nw_split3(ok ? c3[(size_t)l * nout + lp] : 0.0f, h, m, lo);
+            bh3[ks][j] = h;
+            bm3[ks][j] = m;
+            bl3[ks][j] = lo;
+        }
+    const int tilev = n * n / VEC;
+    constexpr int PER = (TR * KP / VEC + NT - 1) / NT;  // n <= TR and n <= KP
+    const int step = VEC * NT, dr = step / n, dk = step % n;
+    const int r0 = (VEC * tid) / n, k0 = (VEC * tid) % n;
+    VT buf[PER];
+    auto gload = [&](long long t) {
+        const VT *src = reinterpret_cast<const VT *>(X + (size_t)t * n * n);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int ev = tid + j * NT;
+            buf[j] = src[ev < tilev ? ev : tilev - 1];  // clamped: no branch per load; staging skips ev >= tilev
+        }
+    };
+    const bool active = wave * 16 < nout;
+    long long t = blockIdx.x;
+    if (t < nslice) gload(t);
+    for (; t < nslice; t += gridDim.x) {
+        __syncthreads();  // the previous slice's second product has read the planes
+        {
+            int r = r0, k = k0;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int ev = tid + j * NT;
+                if (ev < tilev) {
+                    __bf16 h[VEC], m[VEC], lo[VEC];
+                    nw_split3(buf[j].x, h[0], m[0], lo[0]);
+                    nw_split3(buf[j].y, h[1], m[1], lo[1]);
+                    if constexpr (VEC == 4) {
+                        nw_split3(buf[j].z, h[2], m[2], lo[2]);
+                        nw_split3(buf[j].w, h[3], m[3], lo[3]);
+                    }
+                    __bf16 *d = xb + r * LDB + k;
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) {
+                        d[v] = h[v];
+                        d[PLANE + v] = m[v];
+                        d[2 * PLANE + v] = lo[v];
+                    }
+                }
+                r += dr;
+                k += dk;
+                if (k >= n) {
+                    k -= n;
+                    ++r;
+                }
+            }
+        }
+        __syncthreads();
+        const long long tn = t + gridDim.x;
+        if (tn < nslice) gload(tn);
+        nw_f4 acc[NS];
+#pragma unroll
+        for (int ns = 0; ns < NS; ++ns) acc[ns] = nw_f4{0.f, 0.f, 0.f, 0.f};
+        if (active) {
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+                for (int ns = 0; ns < NS; ++ns) {
+                    const __bf16 *bp = xb + (ns * 16 + lr) * LDB + ks * 32 + lk * 8;
+                    const nw_bf8 bh = *reinterpret_cast<const nw_bf8 *>(bp);
+                    const nw_bf8 bm = *reinterpret_cast<const nw_bf8 *>(bp + PLANE);
+                    const nw_bf8 bl = *reinterpret_cast<const nw_bf8 *>(bp + 2 * PLANE);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ks], bh, acc[ns], 0, 0, 0);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bl, acc[ns], 0, 0, 0);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[ks], bm, acc[ns], 0, 0, 0);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[ks], bh, acc[ns], 0, 0, 0);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bm, acc[ns], 0, 0, 0);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bh, acc[ns], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();  // every wave has read the slice
+        // T rows o4 < TR, K columns r < min(TR, KP); r >= n written as zero
+#pragma unroll
+        for (int ns = 0; ns < NS; ++ns) {
+            const int r = ns * 16 + lr;
+            if (r >= KP) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int o4 = wave * 16 + NW_F32_ROW(lk, i);
+                if (o4 >= TR) continue;
+                __bf16 h, m, lo;
+                nw_split3(r < n ? acc[ns][i] : 0.0f, h, m, lo);
+                __bf16 *d = xb + o4 * LDB + r;
+                d[0] = h;
+                d[PLANE] = m;
+                d[2 * PLANE] = lo;
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int ns = 0; ns < NS; ++ns) acc[ns] = nw_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+                for (int ns = 0; ns < NS; ++ns) {
+                    const __bf16 *bp = xb + (ns * 16 + lr) * LDB + ks * 32 + lk * 8;
+                    const nw_bf8 bh = *reinterpret_cast<const nw_bf8 *>(bp);
+                    const nw_bf8 bm = *reinterpret_cast<const nw_bf8 *>(bp + PLANE);
+                    const nw_bf8 bl = *reinterpret_cast<const nw_bf8 *>(bp + 2 * PLANE);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl3[ks], bh, acc[ns], 0, 0, 0);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh3[ks], bl, acc[ns], 0, 0, 0);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm3[ks], bm, acc[ns], 0, 0, 0);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bm3[ks], bh, acc[ns], 0, 0, 0);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh3[ks], bm, acc[ns], 0, 0, 0);
+                    acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh3[ks], bh, acc[ns], 0, 0, 0);
+                }
+            }
+            float *yo = Y + (size_t)t * nout * nout;
+#pragma unroll
+            for (int ns = 0; ns < NS; ++ns) {
+                const int o4 = ns * 16 + lr;
+                if (o4 >= nout) continue;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int o3 = wave * 16 + NW_F32_ROW(lk, i);
+                    if (o3 < nout) yo[(size_t)o3 * nout + o4] = acc[ns][i];
+                }
+            }
+        }
+    }
+}
+
+// Leading-index quarter transform of a batch of row-major matrices:
+// Y[b][o][j] = sum_p c[p][o] X[b][p][j] (p < nl, o < nout, j < ncols).  A
+// step is one 64-column tile; wave w stages the load groups w, w+7, ... of
+// eight consecutive p for the column of its lane (each load instruction
+// reads 64 consecutive floats of one row), so the planes hold the tile as
+// [column][p], K-contiguous for the MFMA B operand.
+template <int KS, typename OutT>
+__global__ __launch_bounds__(448) void k_qlead_bf3(const QPtrs q, int nl, int nout, long long ncols, int nbatch,
+                                                   long long xstride, long long ystride)
+{
+    constexpr int NT = 448, TC = 64;
+    constexpr int KP = KS * 32;
+    constexpr int LDB = KP + 8;
+    constexpr int PLANE = TC * LDB;
+    constexpr int NG = KP / 8;              // load groups of eight p
+    constexpr int PER = (NG + 6) / 7;       // groups per wave
+    const float *__restrict__ X = (const float *)q.X[blockIdx.y];
+    const float *__restrict__ c = (const float *)q.c[blockIdx.y];
+    OutT *__restrict__ Y = (OutT *)q.Y[blockIdx.y];
+    extern __shared__ __bf16 xb[];
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int lp = wave * 16 + lr;
+    nw_bf8 ah[KS], am[KS], al[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int l = ks * 32 + lk * 8 + j;
+            const float v = (lp < nout && l < nl) ? c[(size_t)l * nout + lp] : 0.0f;
+            __bf16 h, m, lo;
+            nw_split3(v, h, m, lo);
+            ah[ks][j] = h;
+            am[ks][j] = m;
+            al[ks][j] = lo;
+        }
+    const long long tpb = (ncols + TC - 1) / TC, ntiles = tpb * nbatch;
+    float buf[PER][8];
+    // loads from clamped (always valid) addresses, zeroed when staged: no
+    // branch or wait per load; the group test is uniform over the wave
+    auto gload = [&](long long t) {
+        const long long b = t / tpb, j = (t - b * tpb) * TC + lane;
+        const float *src = X + b * xstride + (j < ncols ? j : ncols - 1);
+#pragma unroll
+        for (int g = 0; g < PER; ++g) {
+            const int grp = wave + 7 * g;
+            if (grp < NG) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int p = grp * 8 + i;
+                    buf[g][i] = src[(size_t)(p < nl ? p : nl - 1) * ncols];
+                }
+            }
+        }
+    };
+    const bool active = wave * 16 < nout;
+    long long t = blockIdx.x;
+    if (t < ntiles) gload(t);
+    for (; t < ntiles; t += gridDim.x) {
+        __syncthreads();
+        const bool jok = (t % tpb) * TC + lane < ncols;
+#pragma unroll
+        for (int g = 0; g < PER; ++g) {
+            const int grp = wave + 7 * g;
+            if (grp >= NG) continue;
+            nw_bf8 h8, m8, l8;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                __bf16 h, m, lo;
+                nw_split3((jok && grp * 8 + i < nl) ? buf[g][i] : 0.0f, h, m, lo);
+                h8[i] = h;
+                m8[i] = m;
+                l8[i] = lo;
+            }
+            __bf16 *d = xb + lane * LDB + grp * 8;
+            *reinterpret_cast<nw_bf8 *>(d) = h8;
+            *reinterpret_cast<nw_bf8 *>(d + PLANE) = m8;
+            *reinterpret_cast<nw_bf8 *>(d + 2 * PLANE) = l8;
+        }
+        __syncthreads();
+        const long long tn = t + gridDim.x;
+        if (tn < ntiles) gload(tn);
+        if (!active) continue;
+        const long long b = t / tpb, j0 = (t - b * tpb) * TC;
+        nw_f4 acc[TC / 16];
+#pragma unroll
+        for (int ns = 0; ns < TC / 16; ++ns) acc[ns] = nw_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (int ns = 0; ns < TC / 16; ++ns) {
+                const __bf16 *bp = xb + (ns * 16 + lr) * LDB + ks * 32 + lk * 8;
+                const nw_bf8 bh = *reinterpret_cast<const nw_bf8 *>(bp);
+                const nw_bf8 bm = *reinterpret_cast<const nw_bf8 *>(bp + PLANE);
+                const nw_bf8 bl = *reinterpret_cast<const nw_bf8 *>(bp + 2 * PLANE);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ks], bh, acc[ns], 0, 0, 0);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bl, acc[ns], 0, 0, 0);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[ks], bm, acc[ns], 0, 0, 0);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[ks], bh, acc[ns], 0, 0, 0);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bm, acc[ns], 0, 0, 0);
+                acc[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bh, acc[ns], 0, 0, 0);
+            }
+        }
+        OutT *yb = Y + b * ystride;
+#pragma unroll
+        for (int ns = 0; ns < TC / 16; ++ns) {
+            const long long j = j0 + ns * 16 + lr;
+            if (j >= ncols) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = wave * 16 + NW_F32_ROW(lk, r);
+                if (row < nout) yb[(size_t)row * ncols + j] = (OutT)acc[ns][r];
+            }
+        }
+    }
+}
+
+bool launch_qhalf_bf3_many(hipStream_t st, const float *const *X, const float *const *c3, const float *const *c4,
+                           float *const *Y, int count, int n, int nout)
+{
+    if (n < 2 || n % 2 != 0 || n > 112 || nout < 1 || nout > n) return false;
+    const int ns = (n + 15) / 16, ks = (ns + 1) / 2;
+    const bool v4 = n % 4 == 0;
+    const size_t lds = (size_t)3 * (16 * ns) * (32 * ks + 8) * sizeof(__bf16);
+    const long long nslice = (long long)n * n;
+    const dim3 b(448);
+    for (int b0 = 0; b0 < count; b0 += QMAX) {
+        const int nb = count - b0 < QMAX ? count - b0 : QMAX;
+        QPtrs2 q{};
+        for (int k = 0; k < nb; ++k) {
+            q.X[k] = X[b0 + k];
+            q.c3[k] = c3[b0 + k];
+            q.c4[k] = c4[b0 + k];
+            q.Y[k] = Y[b0 + k];
+        }
+#define NW_QH(NS)                                                                                     \
+    case NS: {                                                                                        \
+        void (*kern)(const QPtrs2, int, int, long long) = v4 ? k_qhalf_bf3<NS, 4> : k_qhalf_bf3<NS, 2>; \
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        const dim3 g(grid_x(nslice, nb, resident_grid(kern, 448, lds)), (unsigned)nb);                \
+        hipLaunchKernelGGL(kern, g, b, lds, st, q, n, nout, nslice);                                  \
+        break;                                                                                        \
+    }
+        switch (ns) {
+            NW_QH(1) NW_QH(2) NW_QH(3) NW_QH(4) NW_QH(5) NW_QH(6) NW_QH(7)
+        default: return false;
+        }
+#undef NW_QH
+    }
+    return true;
+}
+
+template <typename OutT>
+static bool launch_qlead_bf3_t(hipStream_t st, const float *const *X, const float *const *c, OutT *const *Y, int count,
+                               int nl, int nout, long long ncols, int nbatch, long long xstride, long long ystride)
+{
+    if (nl < 1 || nl > 128 || nout < 1 || nout > 112 || ncols < 1 || nbatch < 1) return false;
+    const int ks = (nl + 31) / 32;
+    const size_t lds = (size_t)3 * 64 * (ks * 32 + 8) * sizeof(__bf16);
+    const long long ntiles = (ncols + 63) / 64 * nbatch;
+    const dim3 b(448);
+    for (int b0 = 0; b0 < count; b0 += QMAX) {
+        const int nb = count - b0 < QMAX ? count - b0 : QMAX;
+        QPtrs q{};
+        for (int k = 0; k < nb; ++k) { q.X[k] = X[b0 + k]; q.c[k] = c[b0 + k]; q.Y[k] = Y[b0 + k]; }
+#define NW_QL(K)                                                                                      \
+    case K: {                                                                                         \
+        void (*kern)(const QPtrs, int, int, long long, int, long long, long long) = k_qlead_bf3<K, OutT>; \
+        const dim3 g(grid_x(ntiles, nb, resident_grid(kern, 448, lds)), (unsigned)nb);                \
+        hipLaunchKernelGGL(kern, g, b, lds, st, q, nl, nout, ncols, nbatch, xstride, ystride);        \
+        break;                                                                                        \
+    }
+        switch (ks) {
+            NW_QL(1) NW_QL(2) NW_QL(3) NW_QL(4)
+        default: return false;
+        }
+#undef NW_QL
+    }
+    return true;
+}
+
+bool launch_qlead_bf3_many(hipStream_t st, const float *const *X, const float *const *c, float *const *Y, int count,
+                           int nl, int nout, long long ncols, int nbatch, long long xstride, long long ystride)
+{
+    return launch_qlead_bf3_t<float>(st, X, c, Y, count, nl, nout, ncols, nbatch, xstride, ystride);
+}
+
+bool launch_qlead_bf3_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
+                                 int count, int nl, int nout, long long ncols, int nbatch, long long xstride,
+                                 long long ystride)
+{
+    return launch_qlead_bf3_t<double>(st, X, c, Y, count, nl, nout, ncols, nbatch, xstride, ystride);
+}
+
 void launch_qtransform_many(hipStream_t st, const double *const *X, const double *const *c, double *const *Y, int count,
                             long long rows, int nl, int nout)
 {

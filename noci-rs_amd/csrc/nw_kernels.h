@@ -46,6 +46,21 @@ bool launch_qtransform_bf3_many(hipStream_t st, const float *const *X, const flo
                                 int count, long long rows, int nl, int nout);
 bool launch_qtransform_bf3_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
                                       int count, long long rows, int nl, int nout);
+// Three-launch bf16x3 four-index transform (see nw_build.hip): the half
+// transform of every (p,q) slice, then two leading-index passes.
+struct QPtrs2 {
+    const void *X[QMAX];
+    const void *c3[QMAX];
+    const void *c4[QMAX];
+    void *Y[QMAX];
+};
+bool launch_qhalf_bf3_many(hipStream_t st, const float *const *X, const float *const *c3, const float *const *c4,
+                           float *const *Y, int count, int n, int nout);
+bool launch_qlead_bf3_many(hipStream_t st, const float *const *X, const float *const *c, float *const *Y, int count,
+                           int nl, int nout, long long ncols, int nbatch, long long xstride, long long ystride);
+bool launch_qlead_bf3_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
+                                 int count, int nl, int nout, long long ncols, int nbatch, long long xstride,
+                                 long long ystride);
 bool launch_qtransform_f32(hipStream_t st, const float *X, const float *c, float *Y, long long rows, int nl, int nout);
 bool launch_qtransform_f32_out64(hipStream_t st, const float *X, const float *c, double *Y, long long rows, int nl,
                                  int nout);

@@ -204,6 +204,40 @@ def test_transform_shapes_odd_and_two_mod_four(n):
         assert err[nwick.PREC_BF16X3] < 8.0 * err[nwick.PREC_MIXED] + 1e-12
 
 
+@pytest.mark.parametrize("n", [16, 50, 100])
+def test_bf16x3_three_launch_transform(n, monkeypatch):
+    """The bf16x3 transform as three launches (NW_TRANSFORM_PASSES=3: half
+    transform of each (p,q) slice, two leading-index passes) against the
+    default four quarter passes and against FP64: n = 16 (one slice tile),
+    50 (n = 2 mod 4: float2 slice loads, ragged K and column tiles), 100
+    (seven slice tiles, the n_bas = 100 shape)."""
+    sysm = synth.make_system(n, 3, 2, 2, seed=300 + n, kappa_sigma=0.05, n_aux=2 * n)
+    dets = basis.excited_basis(2, n, 3, 2, per_ref=10, seed=300 + n, include_refs=True)
+    rec = basis.to_records(dets)
+    print(f"three-launch transform: n={n} ndet={len(dets)}", flush=True)
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    H64, S64 = eng.hs_matrix(rec, mode=1)
+    eng.set_mixed_floor(0.0)
+    eng.set_precision(nwick.PREC_BF16X3)
+    monkeypatch.delenv("NW_TRANSFORM_PASSES", raising=False)
+    meta = eng.build()
+    assert all(m.fp32 == 1 for m in meta.values())
+    H4, S4 = eng.hs_matrix(rec, mode=1)
+    monkeypatch.setenv("NW_TRANSFORM_PASSES", "3")
+    eng.build()
+    H3, S3 = eng.hs_matrix(rec, mode=1)
+    scale = max(1.0, float(np.max(np.abs(H64))))
+    e4 = float(np.max(np.abs(H4 - H64)))
+    e3 = float(np.max(np.abs(H3 - H64)))
+    d = float(np.max(np.abs(H3 - H4)))
+    print(f"n={n}: max|H4-H64|={e4:.3e} max|H3-H64|={e3:.3e} max|H3-H4|={d:.3e}", flush=True)
+    assert np.array_equal(S3, S64) and np.array_equal(S4, S64)
+    assert e3 < 1e-5 * scale
+    assert e3 < 4.0 * e4 + 1e-12
+    assert d < 4.0 * e4 + 1e-12
+
+
 def test_large_basis_mfma_contractions():
     """n_bas = 36: the co-density x integral-matrix products (J - K, J) take
     the FP64 MFMA path (K = n^2 >= 256, N K >= 2^20); H, S against the oracle,
