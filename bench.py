@@ -602,9 +602,9 @@ def main():
             if line.startswith("TRAFFIC_BYTES"):
                 traffic = float(line.split()[1])
     # algorithmic bytes per dispatch: FP64 passes read and write n^4 doubles; a
-    # reduced-precision tensor reads 4 n^4 floats and writes 3 n^4 floats + n^4
-    # doubles over its four passes, g tensors per dispatch
-    alg_bytes = g * 36.0 * n ** 4 / 4 if use_mixed else 16.0 * n ** 4
+    # reduced-precision pass reads and writes n^4 floats (the last pass too:
+    # its rank-4 slab blocks are stored in FP32), g tensors per dispatch
+    alg_bytes = g * 8.0 * n ** 4 if use_mixed else 16.0 * n ** 4
     flops_disp = g * flops_tensor / 4
     ridge = peak / (PEAK_HBM_GBS / 1e3)  # flop per byte where the MFMA and HBM roofs meet
     hbm_bound = flops_disp / alg_bytes < ridge

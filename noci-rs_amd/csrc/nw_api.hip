@@ -461,8 +461,8 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         bool any32 = false;
         for (auto &jb : jobs) any32 |= jb.f32;
         if (any32) {
-            // FP32 operands, intermediates and MFMA accumulation; the slab blocks
-            // they produce are stored in FP64.
+            // FP32 operands, intermediates and MFMA accumulation; the rank-4 slab
+            // blocks they produce hold FP32 values (PairInfo::fp32).
             const size_t n4 = (size_t)n * n * n * n;
             if (!c->d_eri32) {
                 c->d_eri32 = (float *)dalloc<char>(n4 * sizeof(float));
@@ -491,8 +491,7 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                 const int nb = (int)std::min<size_t>(g, todo.size() - j0);
                 if (f32) {
                     std::vector<const float *> X(nb), C1(nb), C2(nb), C3(nb), C4(nb);
-                    std::vector<float *> TA(nb), TB(nb);
-                    std::vector<double *> OUT(nb);
+                    std::vector<float *> TA(nb), TB(nb), OUT(nb);
                     auto cm = [&](int idx) { return (idx & 1 ? xc32 : cx32) + (size_t)(idx >> 1) * nnm; };
                     for (int k = 0; k < nb; ++k) {
                         const TJob &jb = *todo[j0 + k];
@@ -500,24 +499,23 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                         C1[k] = cm(jb.c1); C2[k] = cm(jb.c2); C3[k] = cm(jb.c3); C4[k] = cm(jb.c4);
                         TA[k] = (float *)(A + k * slot);
                         TB[k] = (float *)(B + k * slot);
-                        OUT[k] = jb.out;
+                        OUT[k] = (float *)jb.out;  // FP32 values at the start of the FP64-sized slab block
                     }
                     std::vector<const float *> TAc(TA.begin(), TA.end()), TBc(TB.begin(), TB.end());
                     auto pass = bf3 ? launch_qtransform_bf3_many : launch_qtransform_f32_many;
-                    auto last = bf3 ? launch_qtransform_bf3_out64_many : launch_qtransform_f32_out64_many;
                     bool ok;
                     if (bf3 && three_launch_transform(n, nm)) {
                         // half transform of the (p,q) slices, then p and q as leading indices
                         const long long nm2 = (long long)nm * nm;
                         ok = launch_qhalf_bf3_many(c->st, X.data(), C3.data(), C4.data(), TA.data(), nb, n, nm) &&
                              launch_qlead_bf3_many(c->st, TAc.data(), C1.data(), TB.data(), nb, n, nm, n * nm2, 1, 0, 0) &&
-                             launch_qlead_bf3_out64_many(c->st, TBc.data(), C2.data(), OUT.data(), nb, n, nm, nm2, nm,
-                                                         n * nm2, nm * nm2);
+                             launch_qlead_bf3_many(c->st, TBc.data(), C2.data(), OUT.data(), nb, n, nm, nm2, nm,
+                                                   n * nm2, nm * nm2);
                     } else {
                         ok = pass(c->st, X.data(), C4.data(), TA.data(), nb, (long long)n * n * n, n, nm) &&
                              pass(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm) &&
                              pass(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm) &&
-                             last(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
+                             pass(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
                     }
                     if (!ok) throw NwError(NW_E_SHAPE, "mixed-precision transform needs an even n_bas and nmo <= 112");
                 } else {
@@ -1059,6 +1057,7 @@ int nw_pair_slab(nw_ctx *c, int x, int w, double *out, int64_t *offsets, double 
         if (out) {
             HIPCHK(hipMemcpyAsync(out, c->d_slab + pi.base, (size_t)pi.off[O_TOTAL] * 8, hipMemcpyDeviceToHost, c->st));
             HIPCHK(hipStreamSynchronize(c->st));
+            widen_rank4(pi, c->nmo, out);
         }
         return NW_OK;
     });

@@ -483,22 +483,6 @@ bool launch_qtransform_f32_many(hipStream_t st, const float *const *X, const flo
     return launch_qtransform_f32_t<float>(st, X, c, Y, count, rows, nl, nout);
 }
 
-bool launch_qtransform_f32_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
-                                      int count, long long rows, int nl, int nout)
-{
-    return launch_qtransform_f32_t<double>(st, X, c, Y, count, rows, nl, nout);
-}
-
-bool launch_qtransform_f32(hipStream_t st, const float *X, const float *c, float *Y, long long rows, int nl, int nout)
-{
-    return launch_qtransform_f32_many(st, &X, &c, &Y, 1, rows, nl, nout);
-}
-
-bool launch_qtransform_f32_out64(hipStream_t st, const float *X, const float *c, double *Y, long long rows, int nl,
-                                 int nout)
-{
-    return launch_qtransform_f32_out64_many(st, &X, &c, &Y, 1, rows, nl, nout);
-}
 
 // ---------------------------------------------------------------------------
 // The same quarter transform with BF16 MFMA on split operands ("bf16x3"):
@@ -686,11 +670,6 @@ bool launch_qtransform_bf3_many(hipStream_t st, const float *const *X, const flo
     return launch_qtransform_bf3_t<float>(st, X, c, Y, count, rows, nl, nout);
 }
 
-bool launch_qtransform_bf3_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
-                                      int count, long long rows, int nl, int nout)
-{
-    return launch_qtransform_bf3_t<double>(st, X, c, Y, count, rows, nl, nout);
-}
 
 // ---------------------------------------------------------------------------
 // Three-launch form of the bf16x3 four-index transform, which moves three
@@ -1058,12 +1037,6 @@ bool launch_qlead_bf3_many(hipStream_t st, const float *const *X, const float *c
     return launch_qlead_bf3_t<float>(st, X, c, Y, count, nl, nout, ncols, nbatch, xstride, ystride);
 }
 
-bool launch_qlead_bf3_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
-                                 int count, int nl, int nout, long long ncols, int nbatch, long long xstride,
-                                 long long ystride)
-{
-    return launch_qlead_bf3_t<double>(st, X, c, Y, count, nl, nout, ncols, nbatch, xstride, ystride);
-}
 
 void launch_qtransform_many(hipStream_t st, const double *const *X, const double *const *c, double *const *Y, int count,
                             long long rows, int nl, int nout)

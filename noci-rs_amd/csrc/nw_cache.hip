@@ -154,6 +154,7 @@ int nw_save_wicks(nw_ctx *c, const char *dir)
             const size_t len = (size_t)pi.off[O_TOTAL];
             buf.resize(len);
             HIPCHK(hipMemcpy(buf.data(), c->d_slab + pi.base, len * 8, hipMemcpyDeviceToHost));
+            widen_rank4(pi, c->nmo, buf.data());
             f.write((const char *)buf.data(), (std::streamsize)(len * 8));
             if (!f) return fail(c, NW_E_ARG, "write failed: " + d + "/wicks.bin");
         }

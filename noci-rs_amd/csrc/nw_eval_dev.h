@@ -96,7 +96,16 @@ struct SpinScal {
     double f0h[2], f0f[2], v0[3];
     double pref;  // phase * tilde_s_prod
     int m, nocc;
+    int r4f;      // rank-4 blocks (J slots, II) stored in FP32
 };
+
+// A pair whose four-index transforms ran in reduced precision (FP32
+// accumulation) keeps its rank-4 blocks as FP32 values at the start of each
+// FP64-sized block: the rounding to FP32 is the one the accumulator already
+// had, so reading them back widened gives the values an FP64 store of the
+// same accumulator would hold.
+typedef const float __attribute__((address_space(1))) *slab32_t;
+__device__ inline double r4(slab_t B, long long i, int f32) { return f32 ? (double)((slab32_t)B)[i] : B[i]; }
 
 __device__ inline SpinScal load_spin(const PairInfo &P, int sp)
 {
@@ -112,6 +121,7 @@ __device__ inline SpinScal load_spin(const PairInfo &P, int sp)
     s.pref = mt[M_PH] * mt[M_TSP];
     s.m = (int)mt[M_M];
     s.nocc = (int)mt[M_NOCC];
+    s.r4f = P.fp32;
     return s;
 }
 
@@ -184,8 +194,8 @@ __device__ inline double same_m0(slab_t slab, const long long *o, const SpinScal
                             }
                         }
                         double second = det_n<L - 2>(mnr);
-                        double direct = J[idx4(n, rows[eta], cols[z], rows[xi], cols[y])];
-                        double exch = J[idx4(n, rows[eta], cols[y], rows[xi], cols[z])];
+                        double direct = r4(J, idx4(n, rows[eta], cols[z], rows[xi], cols[y]), sc.r4f);
+                        double exch = r4(J, idx4(n, rows[eta], cols[y], rows[xi], cols[z]), sc.r4f);
                         double term = second * (direct - exch);
                         acc += ((eta + xi + z + y) & 1) ? -term : term;
                     }
@@ -226,7 +236,7 @@ __device__ inline void hs_m0(const PairInfo &P, slab_t slab, const PackedDet &br
 #pragma unroll
                 for (int y = 0; y < LB; ++y)
 #pragma unroll
-                    for (int x = 0; x < LB; ++x) inner += cofb[x * LB + y] * II[idx4(nmo, ra[e], ca[z], rb[x], cb[y])];
+                    for (int x = 0; x < LB; ++x) inner += cofb[x * LB + y] * r4(II, idx4(nmo, ra[e], ca[z], rb[x], cb[y]), P.fp32);
                 h2ab += cofa[e * LA + z] * inner;
             }
     }
@@ -330,7 +340,7 @@ __device__ inline void same_gen(slab_t slab, const long long *o, const SpinScal 
                         for (int r = 0; r < LM; ++r) {
                             const int rf = r < i ? r : r + 1;
                             const int rr = rows[rf], cc = cols[kf];
-                            const double jv = swap ? J[idx4(n, rr, cc, ri, cj)] : J[idx4(n, ri, cj, rr, cc)];
+                            const double jv = r4(J, swap ? idx4(n, rr, cc, ri, cj) : idx4(n, ri, cj, rr, cc), sc.r4f);
                             v += jv * mcof[r * LM + k2];
                         }
                         contrib += ph * v;
@@ -430,7 +440,7 @@ __device__ inline void hs_gen(const PairInfo &P, slab_t slab, const PackedDet &b
                         slab_t II = slab + P.off[O_II + ((ma0 * 2 + ma1) * 2 + mb0) * 2 + mbk];
                         double v = 0.0;
 #pragma unroll
-                        for (int r = 0; r < LB; ++r) v += II[idx4(nmo, ra[i], ca[j], rb[r], cb[k])] * cofb[r * LB + k];
+                        for (int r = 0; r < LB; ++r) v += r4(II, idx4(nmo, ra[i], ca[j], rb[r], cb[k]), P.fp32) * cofb[r * LB + k];
                         contrib += cij * v;
                     }
                 }

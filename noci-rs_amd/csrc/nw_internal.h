@@ -3,6 +3,7 @@
 // packing / pair-block planning used by every evaluation entry point.
 #pragma once
 #include <algorithm>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <mutex>
@@ -185,6 +186,25 @@ inline const void *to_device(nw_ctx *c, DBuf &buf, const void *p, size_t bytes)
     char *d = buf.get<char>(bytes);
     HIPCHK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, c->st));
     return d;
+}
+
+// A pair built with reduced-precision transforms (PairInfo::fp32) keeps each
+// rank-4 block (the J slots of both spins, the II blocks; nmo^4 values) as
+// FP32 values at the start of its FP64-sized block.  Widen them in a host
+// copy of the pair's slab block, so exported and saved slabs are all FP64.
+inline void widen_rank4(const PairInfo &pi, int nmo, double *blk)
+{
+    if (!pi.fp32) return;
+    const size_t nn4 = (size_t)nmo * nmo * nmo * nmo;
+    std::vector<float> tmp(nn4);
+    auto widen = [&](long long off) {
+        if (off < 0) return;
+        std::memcpy(tmp.data(), blk + off, nn4 * sizeof(float));
+        for (size_t i = 0; i < nn4; ++i) blk[off + i] = (double)tmp[i];
+    };
+    for (int sp = 0; sp < 2; ++sp)
+        for (int s = 0; s < 10; ++s) widen(pi.off[(sp ? O_BB : O_AA) + O_J + s]);
+    for (int k = 0; k < 16; ++k) widen(pi.off[O_II + k]);
 }
 
 inline float elapsed(hipEvent_t a, hipEvent_t b)

@@ -36,16 +36,13 @@ struct QPtrs {
 };
 void launch_qtransform_many(hipStream_t st, const double *const *X, const double *const *c, double *const *Y, int count,
                             long long rows, int nl, int nout);
+// FP32 variants (mixed-precision build; every pass, the last included, writes
+// FP32); return false when the shape is unsupported.
 bool launch_qtransform_f32_many(hipStream_t st, const float *const *X, const float *const *c, float *const *Y,
                                 int count, long long rows, int nl, int nout);
-bool launch_qtransform_f32_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
-                                      int count, long long rows, int nl, int nout);
-// FP32 variants (mixed-precision build); return false when the shape is unsupported.
 // The same with BF16 MFMA on three-way split operands (FP32-level accuracy).
 bool launch_qtransform_bf3_many(hipStream_t st, const float *const *X, const float *const *c, float *const *Y,
                                 int count, long long rows, int nl, int nout);
-bool launch_qtransform_bf3_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
-                                      int count, long long rows, int nl, int nout);
 // Three-launch bf16x3 four-index transform (see nw_build.hip): the half
 // transform of every (p,q) slice, then two leading-index passes.
 struct QPtrs2 {
@@ -58,12 +55,6 @@ bool launch_qhalf_bf3_many(hipStream_t st, const float *const *X, const float *c
                            float *const *Y, int count, int n, int nout);
 bool launch_qlead_bf3_many(hipStream_t st, const float *const *X, const float *const *c, float *const *Y, int count,
                            int nl, int nout, long long ncols, int nbatch, long long xstride, long long ystride);
-bool launch_qlead_bf3_out64_many(hipStream_t st, const float *const *X, const float *const *c, double *const *Y,
-                                 int count, int nl, int nout, long long ncols, int nbatch, long long xstride,
-                                 long long ystride);
-bool launch_qtransform_f32(hipStream_t st, const float *X, const float *c, float *Y, long long rows, int nl, int nout);
-bool launch_qtransform_f32_out64(hipStream_t st, const float *X, const float *c, double *Y, long long rows, int nl,
-                                 int nout);
 void launch_to_f32(hipStream_t st, const double *in, float *out, long long n);
 
 // Evaluation.  Work is described by blocks: a rectangle of determinant pairs

@@ -218,12 +218,19 @@ def test_bf16x3_three_launch_transform(n, monkeypatch):
     eng = nwick.Engine.from_system(sysm)
     eng.build()
     H64, S64 = eng.hs_matrix(rec, mode=1)
+    g64, off64, _ = eng.pair_slab(0, 1)
     eng.set_mixed_floor(0.0)
     eng.set_precision(nwick.PREC_BF16X3)
     monkeypatch.delenv("NW_TRANSFORM_PASSES", raising=False)
     meta = eng.build()
     assert all(m.fp32 == 1 for m in meta.values())
     H4, S4 = eng.hs_matrix(rec, mode=1)
+    # the rank-4 blocks are kept in FP32 on the device; the exported slab is
+    # widened back to FP64 and agrees with the FP64 build's block
+    g4, off4, _ = eng.pair_slab(0, 1)
+    assert np.array_equal(off4, off64)
+    size = int(off64[94])
+    assert np.max(np.abs(g4[:size] - g64[:size])) < 1e-5 * max(1.0, float(np.max(np.abs(g64[:size]))))
     monkeypatch.setenv("NW_TRANSFORM_PASSES", "3")
     eng.build()
     H3, S3 = eng.hs_matrix(rec, mode=1)
