@@ -32,10 +32,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sysm, dets, tol, mode):
+def cpu_baseline(sysm, dets, tol, mode, with_build=True):
     """Time the CPU oracle on a bounded sample of the same workload: the
     build of one ordered reference pair (0, 1) plus the evaluation of all
-    512 x 512 determinant pairs of that block."""
+    determinant pairs of that block (with_build=False: the evaluation only,
+    the rate of requests against a resident slab)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ctypes as ct
 
@@ -73,6 +74,11 @@ def cpu_baseline(sysm, dets, tol, mode):
     L.ow_hs_many(O._d(sub), off.ctypes.data_as(O.I64), O._d(meta), sysm.nmo, npair, O._u(xex), O._u(wex),
                  O._d(exph), sysm.enuc, tol, mode, O._d(H), O._d(S))
     t3 = time.perf_counter()
+    if not with_build:
+        return {"value": npair / (t3 - t2), "unit": "pairs/s", "cores": threads, "kind": "port",
+                "sample": f"oracle evaluation of the {len(bra)}x{len(ket)} determinant pairs of ordered reference "
+                          f"pair (0,1) on its resident slab block ({t3 - t2:.2f} s; build {t1 - t0:.1f} s not "
+                          f"counted), OpenMP threads={threads}"}
     secs = (t1 - t0) + (t3 - t2)
     return {"value": npair / secs, "unit": "pairs/s", "cores": threads, "kind": "port",
             "sample": f"oracle build of ordered reference pair (0,1) ({t1 - t0:.1f} s) + evaluation of its "
@@ -429,6 +435,60 @@ def screen_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def spawn_workload(args):
+    """QMC spawning elements (find_hs_batched, src/stochastic/common.rs:88-112,
+    over calculate_hs_pairs_wicks_batched, src/noci/hs.rs:72-130): H and S of
+    uniformly random (bra, ket) determinant requests against a resident slab.
+    System: n_bas = 60, 8 + 8 electrons, 8 references (all 64 ordered
+    reference pairs built), 512 singles/doubles per reference.  One step is one
+    nw_hs_batched call of 2^21 requests with host arrays in and out, as the
+    caller passes them; the engine bins requests by contraction rank and
+    reference pair, uploads them and evaluates every bin on the GPU."""
+    import numpy as np
+
+    import nwick
+    from nwick import basis, synth
+
+    tol = 1e-8
+    sysm = synth.make_system(60, 8, 8, 8, seed=args.seed)
+    dets = basis.excited_basis(8, 60, 8, 8, per_ref=512, seed=args.seed, include_refs=True)
+    rec = basis.to_records(dets)
+    eng = nwick.Engine.from_system(sysm, device=0)
+    eng.build(tol=tol)
+    log(f"spawn setup: {len(dets)} determinants, 64 ordered reference pairs built in "
+        f"{eng.timings()['total_ms']:.1f} ms")
+    nreq = 1 << 21
+    rng = np.random.default_rng(args.seed)
+    req = rng.integers(0, len(dets), size=(nreq, 2)).astype(np.uint64)
+    for _ in range(args.warmup):
+        eng.hs_batched(rec, req, mode=args.mode)
+    calls, evals = [], []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        h, s = eng.hs_batched(rec, req, mode=args.mode)
+        calls.append(time.perf_counter() - t0)
+        evals.append(eng.timings()["eval_ms"])
+        log(f"spawn step {k}: call {1e3 * calls[-1]:.2f} ms, GPU evaluation {evals[-1]:.2f} ms")
+    t_call = float(np.mean(calls))
+    t_eval = float(np.mean(evals)) * 1e-3
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sysm, dets, tol, args.mode, with_build=False)
+        log(f"cpu baseline: {cpu['value']:.1f} pairs/s ({cpu['sample']})")
+    out = {"metric": "QMC spawning elements: H,S of random determinant-pair requests per second (resident slab)",
+           "value": nreq / t_call, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * t_call, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "random H,S requests: n_bas=60, nocc=8+8, 8 refs (64 ordered pairs built), "
+                                  "512 singles/doubles per ref, 2^21 requests per call", "determinants": len(dets),
+                      "requests": nreq},
+           "gpu_evaluation_pairs_per_s": nreq / t_eval, "gpu_evaluation_ms": 1e3 * t_eval,
+           "host_share_of_call": 1.0 - t_eval / t_call,
+           "nonzero_fraction": float(np.mean(s != 0.0)),
+           "roofline": None, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def copy_workload(args):
     """Calibration for HBM traffic counters: a plain device-to-device copy of
     256 MiB (torch's elementwise copy kernel), repeated steps times.  The
@@ -468,7 +528,7 @@ def main():
                     help="pt2 workload: imaginary shift of the shifted PT2 solves")
     ap.add_argument("--no-energy", action="store_true",
                     help="skip the NOCI energies (eigensolver) -- for profiler runs; then |dE| is not checked")
-    ap.add_argument("--workload", choices=["wick", "pt2", "batch", "screen", "copy"], default="wick",
+    ap.add_argument("--workload", choices=["wick", "pt2", "batch", "screen", "spawn", "copy"], default="wick",
                     help="wick: batched H,S elements (headline); pt2: NOCI-PT2 matvec; batch: uneven "
                          "multi-system batch; screen: learned SNOCI screening; copy: counter calibration")
     args = ap.parse_args()
@@ -482,6 +542,8 @@ def main():
         return batch_workload(args)
     if args.workload == "screen":
         return screen_workload(args)
+    if args.workload == "spawn":
+        return spawn_workload(args)
     if args.workload == "copy":
         return copy_workload(args)
 

@@ -32,7 +32,8 @@ enum nw_status {
     NW_E_SHAPE = 2,     /* dimensions inconsistent or beyond supported limits */
     NW_E_HIP = 3,       /* HIP runtime failure */
     NW_E_MEM = 4,       /* device allocation failed */
-    NW_E_RANK = 5,      /* excitation rank per spin beyond 4 */
+    NW_E_RANK = 5,      /* excitation rank per spin beyond 4, or a pair's contraction rank
+                           (bra plus ket excitations) per spin beyond 4 */
     NW_E_STATE = 6,     /* call order (e.g. evaluation before nw_build) */
     NW_E_NONFINITE = 7, /* non-finite input */
     NW_E_ORDER = 8      /* references not occupied-first */
@@ -102,7 +103,10 @@ int nw_build(nw_ctx *ctx, double tol, const uint8_t *pair_mask, nw_pair_meta *me
 /* Batched H,S for arbitrary determinant pairs: replaces
  * calculate_hs_pairs_wicks_batched (src/noci/hs.rs:72-130) and
  * xw_hamiltonian_overlap_prepared_batched (preparehamiltonianoverlap.rs:121).
- * req: nreq x 2 uint64 {bra index, ket index} into dets.  h_out, s_out: nreq. */
+ * req: nreq x 2 uint64 {bra index, ket index} into dets (host or device
+ * memory).  h_out, s_out: nreq (host or device).  The requests are binned by
+ * contraction rank on the GPU; an index past ndet is NW_E_ARG, an unbuilt
+ * reference pair NW_E_STATE, a contraction rank beyond 4 NW_E_RANK. */
 int nw_hs_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t *req,
                   uint64_t nreq, double *h_out, double *s_out, int mode);
 

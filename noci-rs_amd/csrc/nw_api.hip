@@ -621,9 +621,14 @@ long long pair_offsets(int ma, int mb, int nmo, long long *off)
 
 
 // Pack determinants on the device and return the host copy of the packed records.
-std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked)
+// The evaluation kernels cover contraction ranks (bra + ket excitations per
+// spin) up to four, the fixed-rank kernels of the reference.
+const char *const RANK_MSG = "contraction rank per spin (bra plus ket excitations) must be <= 4";
+
+std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked, const void **draw_out)
 {
     const void *draw = to_device(c, c->dets_raw, dets, ndet * sizeof(nw_det));
+    if (draw_out) *draw_out = draw;
     dpacked = c->dets_packed.get<PackedDet>(ndet);
     int *dflag = c->flag.get<int>(1);
     HIPCHK(hipMemsetAsync(dflag, 0, sizeof(int), c->st));
@@ -662,6 +667,7 @@ std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vect
                 throw NwError(NW_E_STATE, "reference pair (" + std::to_string(A.parent) + "," + std::to_string(B.parent) +
                                               ") was not built");
             const int la = A.ra + B.ra, lb = A.rb + B.rb;
+            if (la > 4 || lb > 4) throw NwError(NW_E_RANK, RANK_MSG);
             const bool m0 = need_hcol && pi.ma == 0 && pi.mb == 0;
             auto &v = bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)];
             v.push_back({A.start, B.start, (int)A.len, (int)B.len, pair, r1 == r2 ? 1 : 0, 0});
@@ -703,6 +709,7 @@ std::map<int, std::vector<EvalBlock>> rect_blocks(nw_ctx *c, const std::vector<P
                 throw NwError(NW_E_STATE, "reference pair (" + std::to_string(A.parent) + "," + std::to_string(B.parent) +
                                               ") was not built");
             const int la = A.ra + B.ra, lb = A.rb + B.rb;
+            if (la > 4 || lb > 4) throw NwError(NW_E_RANK, RANK_MSG);
             const bool m0 = need_hcol && pi.ma == 0 && pi.mb == 0;
             bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)].push_back({A.start, B.start, (int)A.len, (int)B.len, pair, 0, 0});
         }
@@ -781,63 +788,58 @@ int eval_batched(nw_ctx *c, int kind, const nw_det *dets, uint64_t ndet, const u
     if (nreq > (1ull << 31)) return fail(c, NW_E_SHAPE, "too many requests");
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     PackedDet *dp = nullptr;
-    std::vector<PackedDet> h = pack(c, dets, ndet, dp);
-    std::vector<uint64_t> rq(nreq * 2);
-    if (is_device_ptr(req)) {
-        HIPCHK(hipMemcpyAsync(rq.data(), req, nreq * 16, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipStreamSynchronize(c->st));
-    } else {
-        std::memcpy(rq.data(), req, nreq * 16);
-    }
-    std::vector<nw_det> hd;
-    const nw_det *hdp = dets;
-    if (is_device_ptr(dets)) {
-        hd.resize(ndet);
-        HIPCHK(hipMemcpyAsync(hd.data(), dets, ndet * sizeof(nw_det), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipStreamSynchronize(c->st));
-        hdp = hd.data();
-    }
-    // calculate_hs_pairs_wicks_batched (src/noci/hs.rs:86-109): same-parent pairs
-    // differing in more than four spin orbitals are exactly zero; the rest are
-    // grouped by contraction-rank bin and reference-pair zero class.
-    std::map<int, std::vector<EvalReq>> bins;
-    std::vector<long long> zeros;
-    for (uint64_t q = 0; q < nreq; ++q) {
-        const uint64_t a = rq[2 * q], b = rq[2 * q + 1];
-        if (a >= ndet || b >= ndet) throw NwError(NW_E_ARG, "request index out of range");
-        const nw_det &Ad = hdp[a], &Bd = hdp[b];
-        if (kind == EVAL_HS && Ad.parent == Bd.parent) {
-            int diff = __builtin_popcountll(Ad.oa[0] ^ Bd.oa[0]) + __builtin_popcountll(Ad.oa[1] ^ Bd.oa[1]) +
-                       __builtin_popcountll(Ad.ob[0] ^ Bd.ob[0]) + __builtin_popcountll(Ad.ob[1] ^ Bd.ob[1]);
-            if (diff > 4) { zeros.push_back((long long)q); continue; }
-        }
-        const int pair = (int)Ad.parent * c->nref + (int)Bd.parent;
-        const PairInfo &pi = c->pinfo[pair];
-        if (!pi.built) throw NwError(NW_E_STATE, "reference pair not built");
-        const int la = h[a].ra + h[b].ra, lb = h[a].rb + h[b].rb;
-        const bool m0 = kind == EVAL_HS && pi.ma == 0 && pi.mb == 0;
-        bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)].push_back({(int)a, (int)b, (long long)q});
-    }
+    const void *draw = nullptr;
+    pack(c, dets, ndet, dp, &draw);
+    const uint64_t *dreq = (const uint64_t *)to_device(c, c->req_in, req, nreq * 16);
     const bool adev = is_device_ptr(A), bdev = is_device_ptr(B);
     double *dA = adev ? A : c->outH.get<double>(nreq);
     double *dB = bdev ? B : c->outS.get<double>(nreq);
-    Uploader up(c, std::max<size_t>(64u << 20, nreq * 40 + (1u << 20)));
-    if (!zeros.empty()) {
-        std::vector<LinDesc> lz;
-        for (long long q : zeros) {
-            lz.push_back({nullptr, nullptr, nullptr, dA + q, 0, 0, 0});
-            lz.push_back({nullptr, nullptr, nullptr, dB + q, 0, 0, 0});
-        }
-        launch_lincomb(c->st, up.put(lz), (int)lz.size(), 1);
+    // calculate_hs_pairs_wicks_batched (src/noci/hs.rs:86-109): exact zeros and
+    // contraction-rank bins, on the GPU (k_req_key / k_req_scatter)
+    unsigned *bins = c->req_bins.get<unsigned>(2 * REQ_NKEY + 1);
+    ReqBin rb{};
+    rb.req = dreq;
+    rb.nreq = (long long)nreq;
+    rb.ndet = (long long)ndet;
+    rb.raw = draw;
+    rb.dp = dp;
+    rb.pairs = c->d_pinfo;
+    rb.nref = c->nref;
+    rb.hs = kind == EVAL_HS ? 1 : 0;
+    rb.key = c->req_key.get<int>(nreq);
+    rb.hist = bins;
+    rb.cursor = bins + REQ_NKEY;
+    rb.err = (int *)(bins + 2 * REQ_NKEY);
+    rb.A = dA;
+    rb.B = dB;
+    rb.out = c->req_sorted.get<EvalReq>(nreq);
+    HIPCHK(hipMemsetAsync(bins, 0, (2 * REQ_NKEY + 1) * sizeof(unsigned), c->st));
+    launch_req_bin(c->st, rb, false);
+    HIPCHK(hipGetLastError());
+    unsigned hb[REQ_NKEY + 1];
+    HIPCHK(hipMemcpyAsync(hb, bins, REQ_NKEY * sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(hb + REQ_NKEY, rb.err, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (hb[REQ_NKEY] == REQ_ERR_INDEX) throw NwError(NW_E_ARG, "request index out of range");
+    if (hb[REQ_NKEY] == REQ_ERR_UNBUILT) throw NwError(NW_E_STATE, "reference pair not built");
+    if (hb[REQ_NKEY] == REQ_ERR_RANK) throw NwError(NW_E_RANK, RANK_MSG);
+    std::vector<unsigned> start(REQ_NKEY);
+    unsigned tot = 0;
+    for (int k = 0; k < REQ_NKEY; ++k) {
+        start[k] = tot;
+        tot += hb[k];
     }
+    Uploader up(c, 4096);
+    rb.start = up.put(start);
+    launch_req_bin(c->st, rb, true);
     EvalArgs args = base_args(c, dp, mode);
     args.A = dA;
     args.B = dB;
     HIPCHK(hipEventRecord(c->ev[1], c->st));
-    for (auto &kv : bins) {
-        const int key = kv.first, lab = key >> 1;
-        EvalReq *dr = up.put(kv.second);
-        launch_eval_reqs(c->st, kind, lab / 5, lab % 5, key & 1, dr, (int)kv.second.size(), args);
+    for (int k = 0; k < REQ_NKEY; ++k) {
+        if (!hb[k]) continue;
+        const int lab = k >> 1;
+        launch_eval_reqs(c->st, kind, lab / 5, lab % 5, k & 1, rb.out + start[k], (int)hb[k], args);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[2], c->st));

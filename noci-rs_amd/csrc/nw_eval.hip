@@ -201,6 +201,83 @@ void launch_eval_blocks(hipStream_t st, int kind, int la, int lb, bool m0, const
 #undef NW_T
 }
 
+// ---------------------------------------------------------------------------
+// Request binning for the batched entry points on the GPU
+// (calculate_hs_pairs_wicks_batched, src/noci/hs.rs:86-109): same-parent
+// pairs that differ in more than four spin orbitals are exactly zero (written
+// here); every other request gets the key ((L_a * 5 + L_b) << 1) | m0 of its
+// contraction-rank bin.  Counting and placement go through LDS first, so a
+// workgroup issues one global atomic per key.  The order of requests inside a
+// bin is not fixed; each request's value does not depend on it.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_req_key(ReqBin b)
+{
+    __shared__ unsigned s_hist[REQ_NKEY];
+    for (int k = threadIdx.x; k < REQ_NKEY; k += blockDim.x) s_hist[k] = 0;
+    __syncthreads();
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < b.nreq) {
+        const uint64_t ia = b.req[2 * q], ib = b.req[2 * q + 1];
+        int key = -1;
+        if (ia >= (uint64_t)b.ndet || ib >= (uint64_t)b.ndet) {
+            atomicMax(b.err, REQ_ERR_INDEX);
+        } else {
+            const PackedDet da = b.dp[ia], db = b.dp[ib];
+            bool zero = false;
+            if (b.hs && da.parent == db.parent) {
+                const RawDet *ra = reinterpret_cast<const RawDet *>(b.raw) + ia;
+                const RawDet *rb = reinterpret_cast<const RawDet *>(b.raw) + ib;
+                const int diff = __popcll(ra->oa[0] ^ rb->oa[0]) + __popcll(ra->oa[1] ^ rb->oa[1]) +
+                                 __popcll(ra->ob[0] ^ rb->ob[0]) + __popcll(ra->ob[1] ^ rb->ob[1]);
+                zero = diff > 4;
+            }
+            if (zero) {
+                b.A[q] = 0.0;
+                b.B[q] = 0.0;
+            } else {
+                const PairInfo &pi = b.pairs[(int)da.parent * b.nref + (int)db.parent];
+                const int la = da.ra + db.ra, lb = da.rb + db.rb;
+                if (!pi.built) atomicMax(b.err, REQ_ERR_UNBUILT);
+                else if (la > 4 || lb > 4) atomicMax(b.err, REQ_ERR_RANK);
+                else key = ((la * 5 + lb) << 1) | ((b.hs && pi.ma == 0 && pi.mb == 0) ? 1 : 0);
+            }
+        }
+        b.key[q] = key;
+        if (key >= 0) atomicAdd(&s_hist[key], 1u);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < REQ_NKEY; k += blockDim.x)
+        if (s_hist[k]) atomicAdd(&b.hist[k], s_hist[k]);
+}
+
+__global__ __launch_bounds__(256) void k_req_scatter(ReqBin b)
+{
+    __shared__ unsigned s_cnt[REQ_NKEY], s_base[REQ_NKEY];
+    for (int k = threadIdx.x; k < REQ_NKEY; k += blockDim.x) s_cnt[k] = 0;
+    __syncthreads();
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
+    unsigned local = 0;
+    if (q < b.nreq) {
+        key = b.key[q];
+        if (key >= 0) local = atomicAdd(&s_cnt[key], 1u);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < REQ_NKEY; k += blockDim.x)
+        s_base[k] = s_cnt[k] ? atomicAdd(&b.cursor[k], s_cnt[k]) : 0u;
+    __syncthreads();
+    if (key >= 0)
+        b.out[(size_t)b.start[key] + s_base[key] + local] = EvalReq{(int)b.req[2 * q], (int)b.req[2 * q + 1], q};
+}
+
+void launch_req_bin(hipStream_t st, const ReqBin &b, bool scatter)
+{
+    if (b.nreq <= 0) return;
+    const dim3 grid((unsigned)((b.nreq + 255) / 256)), block(256);
+    if (scatter) hipLaunchKernelGGL(k_req_scatter, grid, block, 0, st, b);
+    else hipLaunchKernelGGL(k_req_key, grid, block, 0, st, b);
+}
+
 void launch_eval_reqs(hipStream_t st, int kind, int la, int lb, bool m0, const EvalReq *reqs, int nreq,
                       const EvalArgs &args)
 {

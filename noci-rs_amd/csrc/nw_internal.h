@@ -105,6 +105,7 @@ struct nw_ctx {
     double t_ms[3] = {0, 0, 0};
     // scratch
     nwi::DBuf buildscratch, arena_desc, scratchA, scratchB, cx32, xc32, keepM, keepCX, keepXC, fockbuf, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
+    nwi::DBuf req_in, req_key, req_bins, req_sorted;  // batched requests binned on the GPU
     std::deque<std::vector<char>> keep;
 
     ~nw_ctx()
@@ -220,7 +221,8 @@ inline float elapsed(hipEvent_t a, hipEvent_t b)
 
 struct Run { long long start, len; int parent, ra, rb; };
 long long pair_offsets(int ma, int mb, int nmo, long long *off);
-std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked);
+std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked,
+                            const void **draw = nullptr);
 std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vector<PackedDet> &h, bool need_hcol);
 long long tiles_of(const std::vector<EvalBlock> &v);
 std::map<int, std::vector<EvalBlock>> rect_blocks(nw_ctx *c, const std::vector<PackedDet> &h, long long nrow0,

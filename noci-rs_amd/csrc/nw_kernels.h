@@ -85,6 +85,27 @@ struct EvalArgs {
 };
 
 void launch_pack_dets(hipStream_t st, const void *dets, long long ndet, PackedDet *out, int *err);
+
+// GPU binning of batched requests (nw_hs_batched / nw_fs_batched): k_req_key
+// writes each request's bin key (-1: exact zero or error) and the per-key
+// counts; k_req_scatter places the requests of every key at start[key].
+constexpr int REQ_NKEY = 50;  // ((L_a * 5 + L_b) << 1) | m0 with L_a, L_b <= 4
+enum { REQ_ERR_INDEX = 1, REQ_ERR_UNBUILT = 2, REQ_ERR_RANK = 3 };
+struct ReqBin {
+    const uint64_t *req;  // nreq (a, b) index pairs
+    long long nreq, ndet;
+    const void *raw;      // nw_det records on the device
+    const PackedDet *dp;
+    const PairInfo *pairs;
+    int nref, hs;         // hs: H,S requests (exact zeros and m0 bins apply)
+    int *key;
+    unsigned *hist, *cursor;  // REQ_NKEY counts each (zeroed by the caller)
+    const unsigned *start;    // REQ_NKEY offsets into out
+    int *err;
+    double *A, *B;
+    EvalReq *out;
+};
+void launch_req_bin(hipStream_t st, const ReqBin &b, bool scatter);
 void launch_eval_blocks(hipStream_t st, int kind, int la, int lb, bool m0, const EvalBlock *blocks, int nblocks,
                         long long ntiles, const EvalArgs &args);
 void launch_eval_reqs(hipStream_t st, int kind, int la, int lb, bool m0, const EvalReq *reqs, int nreq,

@@ -113,6 +113,32 @@ def test_pair_mask_and_unbuilt_pair_error():
         eng.hs_batched(rec, np.array([[len(dets) - 1, 0]], dtype=np.uint64))
 
 
+def test_batched_request_errors_and_rank_limit():
+    """Batched requests are binned on the GPU: an index past the determinant
+    list, and a pair whose contraction rank on one spin exceeds four (an alpha
+    triple against an alpha double), are errors rather than unwritten outputs;
+    the full-matrix path refuses the second too.  Valid requests after an
+    error still evaluate, and match the full matrix."""
+    sysm, dets = _case(10, 3, 3, 2, seed=61, per_ref=6)
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    rec = basis.to_records(dets)
+    with pytest.raises(nwick.NwickError):
+        eng.hs_batched(rec, np.array([[0, len(dets)]], dtype=np.uint64))
+    triple = basis.Det(1, (1 << 3) - 1, (1 << 3) - 1, (0, 1, 2), (3, 4, 5))
+    double = basis.Det(0, (1 << 3) - 1, (1 << 3) - 1, (0, 1), (6, 7))
+    rec3 = basis.to_records(dets + [triple, double])
+    n = len(dets)
+    with pytest.raises(nwick.NwickError):
+        eng.hs_batched(rec3, np.array([[n + 1, n]], dtype=np.uint64))
+    with pytest.raises(nwick.NwickError):
+        eng.hs_matrix(rec3)
+    req = np.array([[a, b] for a in range(n) for b in range(n)], dtype=np.uint64)
+    h, s = eng.hs_batched(rec, req)
+    H, S = eng.hs_matrix(rec)
+    assert np.max(np.abs(h.reshape(n, n) - H)) < 1e-10 and np.max(np.abs(s.reshape(n, n) - S)) < 1e-12
+
+
 @pytest.mark.parametrize("prec", [nwick.PREC_MIXED, nwick.PREC_BF16X3])
 def test_mixed_precision_build_close_to_fp64(prec):
     """Mixed build (FP32 MFMA, or BF16 MFMA on three-way split operands):
