@@ -168,7 +168,10 @@ int nw_set_mixed_floor(nw_ctx *ctx, double sigma_floor);
 /* Copy one ordered pair's slab block (layout of layout.rs:21-189) and its
  * offsets (95 int64, relative to the block, -1 = absent, last = length) and
  * scalar meta (32 doubles) out of device memory; out may be NULL to query
- * the length through offsets[94].  For tests and disk interop. */
+ * the length through offsets[94].  For tests and disk interop.  A pair built
+ * with reduced-precision transforms (nw_pair_meta.fp32) keeps its rank-4
+ * blocks as FP32 on the device; they are copied out widened to FP64, as
+ * nw_save_wicks writes them. */
 int nw_pair_slab(nw_ctx *ctx, int x, int w, double *out, int64_t *offsets, double *meta);
 
 /* Wick disk cache in the reference's format (src/nonorthogonalwicks/storage.rs:
