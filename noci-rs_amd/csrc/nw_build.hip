@@ -265,14 +265,17 @@ __global__ __launch_bounds__(64 * NW) void k_qtransform_mfma_pipe(const QPtrs q,
 }
 
 // FP32 form used by the mixed-precision build: FP32 operands and
-// intermediates, FP32 MFMA accumulation (v_mfma_f32_16x16x4f32), output
-// either FP32 (intermediate passes) or FP64 (the final pass, into the slab).
+// intermediates, FP32 MFMA accumulation (v_mfma_f32_16x16x4f32), FP32 output
+// (the final pass writes the pair's FP32 rank-4 slab block).
 // 128-row tiles of X, staged in LDS, next tile prefetched into registers.
 typedef float nw_f4 __attribute__((ext_vector_type(4)));
 
 // Rows of X per LDS tile of the pipelined FP64 / FP32 transforms.
 #ifndef NW_TR_F64
 #define NW_TR_F64 32
+#endif
+#ifndef NW_TR_F64_SMALL
+#define NW_TR_F64_SMALL 64
 #endif
 #ifndef NW_TR_F32
 #define NW_TR_F32 64
@@ -1043,20 +1046,22 @@ void launch_qtransform_many(hipStream_t st, const double *const *X, const double
 {
     const int ks = (nl + 3) / 4;
     const int nw = (nout + 15) / 16;
-    constexpr int TRD = NW_TR_F64;
-    const long long ntiles = (rows + TRD - 1) / TRD;
     for (int b0 = 0; b0 < count; b0 += QMAX) {
         const int nb = count - b0 < QMAX ? count - b0 : QMAX;
         QPtrs q{};
         for (int k = 0; k < nb; ++k) { q.X[k] = X[b0 + k]; q.c[k] = c[b0 + k]; q.Y[k] = Y[b0 + k]; }
         bool done = false;
         if (nl % 2 == 0 && ks <= 31 && nw <= 7) {
-            const size_t lds = (size_t)TRD * (ks * 4 + 1) * sizeof(double);
             done = true;
+            // small n (K <= 64): tiles of NW_TR_F64_SMALL rows, so a tile's
+            // staging and barriers are spread over more products
 #define NW_QP(K)                                                                                        \
     case K: {                                                                                           \
+        constexpr int TRK = K <= 16 ? NW_TR_F64_SMALL : NW_TR_F64;                                      \
+        const size_t lds = (size_t)TRK * (ks * 4 + 1) * sizeof(double);                                 \
+        const long long ntiles = (rows + TRK - 1) / TRK;                                                \
         int nwv = 7;                                                                                    \
-        void (*kern)(const QPtrs, long long, int, long long) = f64_kernel<K, TRD>(nl % 4 != 0, nout, nwv); \
+        void (*kern)(const QPtrs, long long, int, long long) = f64_kernel<K, TRK>(nl % 4 != 0, nout, nwv); \
         const dim3 g(grid_x(ntiles, nb, resident_grid(kern, 64 * nwv, lds)), (unsigned)nb);            \
         hipLaunchKernelGGL(kern, g, dim3(64 * nwv), lds, st, q, rows, nout, ntiles);                   \
         break;                                                                                          \
