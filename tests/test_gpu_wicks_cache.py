@@ -95,3 +95,30 @@ def test_cache_round_trip(tmp_path):
     assert np.array_equal(H2, H) and np.array_equal(S2, S)
     with pytest.raises(nwick.NwickError):
         eng2.update_fock(sysm.h, sysm.h)
+
+
+def test_cache_of_mixed_build_round_trip(tmp_path):
+    """A reduced-precision build keeps its rank-4 blocks as FP32 on the device;
+    nw_save_wicks writes them widened to FP64 (the reference's f64 file), so a
+    fresh context that loads the cache reproduces the mixed build's elements
+    bit for bit (m > 0 pairs included: several J slots and II blocks)."""
+    sysm = synth.make_system(10, 3, 2, 3, seed=45, orthonormal_ao=False, n_aux=20, orth_refs=(2,))
+    dets = basis.excited_basis(3, 10, 3, 2, per_ref=25, seed=45, include_refs=True)
+    rec = basis.to_records(dets)
+    eng = nwick.Engine.from_system(sysm)
+    eng.set_precision(nwick.PREC_BF16X3)
+    eng.set_mixed_floor(0.0)
+    meta = eng.build(tol=1e-8)
+    assert all(m.fp32 == 1 for m in meta.values()) and any(m.ma > 0 for m in meta.values())
+    H, S = eng.hs_matrix(rec, mode=1)
+    eng.save_wicks(str(tmp_path))
+    slab = np.fromfile(os.path.join(tmp_path, "wicks.bin"), dtype="<f8")
+    base = 0
+    for p in range(9):
+        blk, off, _ = eng.pair_slab(*divmod(p, 3))
+        assert np.array_equal(slab[base:base + off[94]], blk)
+        base += off[94]
+    eng2 = nwick.Engine.from_system(sysm)
+    eng2.load_wicks(str(tmp_path), tol=1e-8)
+    H2, S2 = eng2.hs_matrix(rec, mode=1)
+    assert np.array_equal(H2, H) and np.array_equal(S2, S)
