@@ -632,15 +632,14 @@ std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, Packed
     dpacked = c->dets_packed.get<PackedDet>(ndet);
     int *dflag = c->flag.get<int>(1);
     HIPCHK(hipMemsetAsync(dflag, 0, sizeof(int), c->st));
-    launch_pack_dets(c->st, draw, (long long)ndet, dpacked, dflag);
+    launch_pack_dets(c->st, draw, (long long)ndet, c->nref, dpacked, dflag);
     std::vector<PackedDet> h(ndet);
     int flag = 0;
     HIPCHK(hipMemcpyAsync(h.data(), dpacked, ndet * sizeof(PackedDet), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    if (flag) throw NwError(NW_E_RANK, "excitation rank per spin must be <= 4 with equal holes and particles");
-    for (uint64_t i = 0; i < ndet; ++i)
-        if ((int)h[i].parent >= c->nref) throw NwError(NW_E_ARG, "determinant parent out of range");
+    if (flag & 2) throw NwError(NW_E_ARG, "determinant parent out of range");
+    if (flag & 1) throw NwError(NW_E_RANK, "excitation rank per spin must be <= 4 with equal holes and particles");
     return h;
 }
 

@@ -7,6 +7,9 @@
 // src/nonorthogonalwicks/eval/preparehamiltonianoverlap.rs (m = 0: the fixed
 // kernels :380-19544 / generic m0 path :20304-20533; m > 0: :20550-20966) and
 // eval/helpers.rs.
+#include <algorithm>
+#include <cstdlib>
+
 #include "nw_eval_dev.h"
 
 namespace nw {
@@ -36,7 +39,11 @@ __device__ inline int cache_spin(const uint64_t *holes, const uint64_t *parts, u
     return nh;
 }
 
-__global__ void k_pack_dets(const RawDet *__restrict__ in, long long n, PackedDet *__restrict__ out, int *err)
+// err bit 0: an excitation rank per spin above 4 or unequal holes and
+// particles; bit 1: a parent index at or past nref (checked on the full
+// 32-bit value before it is narrowed to the packed 16-bit field).
+__global__ void k_pack_dets(const RawDet *__restrict__ in, long long n, int nref, PackedDet *__restrict__ out,
+                            int *err)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -45,6 +52,10 @@ __global__ void k_pack_dets(const RawDet *__restrict__ in, long long n, PackedDe
     int ra = cache_spin(d.ha, d.pa, p.ia);
     int rb = cache_spin(d.hb, d.pb, p.ib);
     if (ra < 0 || rb < 0 || ra > 4 || rb > 4) atomicOr(err, 1);
+    if (d.parent >= (uint32_t)nref) {
+        atomicOr(err, 2);
+        d.parent = 0;
+    }
     p.ra = (uint8_t)(ra < 0 ? 0 : ra);
     p.rb = (uint8_t)(rb < 0 ? 0 : rb);
     p.parent = (uint16_t)d.parent;
@@ -54,11 +65,11 @@ __global__ void k_pack_dets(const RawDet *__restrict__ in, long long n, PackedDe
     out[i] = p;
 }
 
-void launch_pack_dets(hipStream_t st, const void *dets, long long ndet, PackedDet *out, int *err)
+void launch_pack_dets(hipStream_t st, const void *dets, long long ndet, int nref, PackedDet *out, int *err)
 {
     if (ndet <= 0) return;
     hipLaunchKernelGGL(k_pack_dets, dim3((unsigned)((ndet + 255) / 256)), dim3(256), 0, st,
-                       (const RawDet *)dets, ndet, out, err);
+                       (const RawDet *)dets, ndet, nref, out, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -74,10 +85,12 @@ __device__ inline int find_block(const EvalBlock *__restrict__ blocks, int nb, l
     return lo;
 }
 
+// One 16 x 16 tile t of the bin's blocks.  The tile loop of k_eval_blocks is
+// workgroup-uniform, so the barrier of the matvec branch is reached by every
+// lane of the workgroup on every tile.
 template <int KIND, int LA, int LB, bool M0>
-__global__ __launch_bounds__(256) void k_eval_blocks(const EvalBlock *__restrict__ blocks, int nblocks, EvalArgs a)
+__device__ inline void eval_tile(const EvalBlock *__restrict__ blocks, int nblocks, const EvalArgs &a, long long t)
 {
-    const long long t = blockIdx.x;
     const EvalBlock B = blocks[find_block(blocks, nblocks, t)];
     const int ntj = (B.nj + 15) >> 4;
     const long long loc = t - B.tstart;
@@ -152,7 +165,17 @@ __global__ __launch_bounds__(256) void k_eval_blocks(const EvalBlock *__restrict
             unsafeAtomicAdd(a.y + B.j0 + tj * 16 + threadIdx.x, cm);
             if (a.ys) unsafeAtomicAdd(a.ys + B.j0 + tj * 16 + threadIdx.x, cs);
         }
+        __syncthreads();  // colm / cols_ are rewritten by the next tile
     }
+}
+
+// Tiles are visited grid-stride, so a bin of any size fits one launch whose
+// grid stays within the runtime's limit of 2^32 - 1 work-items.
+template <int KIND, int LA, int LB, bool M0>
+__global__ __launch_bounds__(256) void k_eval_blocks(const EvalBlock *__restrict__ blocks, int nblocks, long long ntiles,
+                                                     EvalArgs a)
+{
+    for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) eval_tile<KIND, LA, LB, M0>(blocks, nblocks, a, t);
 }
 
 template <int KIND, int LA, int LB, bool M0>
@@ -177,23 +200,31 @@ __global__ __launch_bounds__(256) void k_eval_reqs(const EvalReq *__restrict__ r
 
 
 
+long long eval_grid_cap()
+{
+    // NW_EVAL_GRID_CAP lowers the cap (tests of the grid-stride tile loop)
+    const char *e = std::getenv("NW_EVAL_GRID_CAP");
+    const long long v = e ? std::atoll(e) : 0;
+    return v > 0 ? std::min(v, EVAL_GRID_MAX) : EVAL_GRID_MAX;
+}
+
 void launch_eval_blocks(hipStream_t st, int kind, int la, int lb, bool m0, const EvalBlock *blocks, int nblocks,
                         long long ntiles, const EvalArgs &args)
 {
     if (ntiles <= 0) return;
-    dim3 grid((unsigned)ntiles), block(256);
+    dim3 grid((unsigned)std::min(ntiles, eval_grid_cap())), block(256);
 #define NW_T(A, B)                                                                                             \
     if (la == A && lb == B) {                                                                                  \
         if (kind == EVAL_HS) {                                                                                 \
-            if (m0) hipLaunchKernelGGL((k_eval_blocks<EVAL_HS, A, B, true>), grid, block, 0, st, blocks, nblocks, args); \
-            else hipLaunchKernelGGL((k_eval_blocks<EVAL_HS, A, B, false>), grid, block, 0, st, blocks, nblocks, args); \
+            if (m0) hipLaunchKernelGGL((k_eval_blocks<EVAL_HS, A, B, true>), grid, block, 0, st, blocks, nblocks, ntiles, args); \
+            else hipLaunchKernelGGL((k_eval_blocks<EVAL_HS, A, B, false>), grid, block, 0, st, blocks, nblocks, ntiles, args); \
         } else if (kind == EVAL_FS) {                                                                          \
-            hipLaunchKernelGGL((k_eval_blocks<EVAL_FS, A, B, false>), grid, block, 0, st, blocks, nblocks, args);  \
+            hipLaunchKernelGGL((k_eval_blocks<EVAL_FS, A, B, false>), grid, block, 0, st, blocks, nblocks, ntiles, args);  \
         } else if (kind == EVAL_PROJECT) {                                                                     \
-            if (m0) hipLaunchKernelGGL((k_eval_blocks<EVAL_PROJECT, A, B, true>), grid, block, 0, st, blocks, nblocks, args); \
-            else hipLaunchKernelGGL((k_eval_blocks<EVAL_PROJECT, A, B, false>), grid, block, 0, st, blocks, nblocks, args); \
+            if (m0) hipLaunchKernelGGL((k_eval_blocks<EVAL_PROJECT, A, B, true>), grid, block, 0, st, blocks, nblocks, ntiles, args); \
+            else hipLaunchKernelGGL((k_eval_blocks<EVAL_PROJECT, A, B, false>), grid, block, 0, st, blocks, nblocks, ntiles, args); \
         } else {                                                                                               \
-            hipLaunchKernelGGL((k_eval_blocks<EVAL_MATVEC, A, B, false>), grid, block, 0, st, blocks, nblocks, args); \
+            hipLaunchKernelGGL((k_eval_blocks<EVAL_MATVEC, A, B, false>), grid, block, 0, st, blocks, nblocks, ntiles, args); \
         }                                                                                                      \
         return;                                                                                                \
     }

@@ -84,7 +84,7 @@ struct EvalArgs {
     double *y, *ys;       // matvec outputs (ys may be null)
 };
 
-void launch_pack_dets(hipStream_t st, const void *dets, long long ndet, PackedDet *out, int *err);
+void launch_pack_dets(hipStream_t st, const void *dets, long long ndet, int nref, PackedDet *out, int *err);
 
 // GPU binning of batched requests (nw_hs_batched / nw_fs_batched): k_req_key
 // writes each request's bin key (-1: exact zero or error) and the per-key
@@ -106,6 +106,10 @@ struct ReqBin {
     EvalReq *out;
 };
 void launch_req_bin(hipStream_t st, const ReqBin &b, bool scatter);
+// Workgroups per k_eval_blocks launch: tiles beyond it are visited
+// grid-stride (256 lanes x 2^22 workgroups stays below 2^32 work-items).
+constexpr long long EVAL_GRID_MAX = 1ll << 22;
+long long eval_grid_cap();
 void launch_eval_blocks(hipStream_t st, int kind, int la, int lb, bool m0, const EvalBlock *blocks, int nblocks,
                         long long ntiles, const EvalArgs &args);
 void launch_eval_reqs(hipStream_t st, int kind, int la, int lb, bool m0, const EvalReq *reqs, int nreq,

@@ -250,61 +250,84 @@ int ow_svd(int n, const double *A, double *U, double *s, double *V)
 /* AO -> MO four-index transform (src/maths/eri.rs:258-313):                 */
 /* out[p,q,r,s] = sum c1[mu,p] c2[nu,q] c3[lam,r] c4[sig,s] (mu nu|lam sig)  */
 /* ------------------------------------------------------------------------- */
+/* One quarter of the transform as a cache-blocked product:
+ * Y[s, row] = sum_l X[row, l] c[l, s] for X rows x nb, c nb x nm, Y nm x rows.
+ * Rows go in blocks of QP_RB through a 4 x 8 register tile (the inner loop
+ * of 8 columns vectorises), the block's result is written transposed, so the
+ * next quarter again contracts the last index. */
+#define QP_RB 64
+typedef double v4d __attribute__((vector_size(32)));
+static void qpass(long long rows, int nb, int nm, const double *X, const double *c, double *Y)
+{
+    const int np = (nm + 7) & ~7;
+    double *cp = (double *)calloc((size_t)nb * np, sizeof(double));
+    for (int l = 0; l < nb; ++l)
+        for (int t = 0; t < nm; ++t) cp[(size_t)l * np + t] = c[(size_t)l * nm + t];
+#pragma omp parallel
+    {
+        double *buf = (double *)malloc(sizeof(double) * QP_RB * np);
+#pragma omp for schedule(static)
+        for (long long r0 = 0; r0 < rows; r0 += QP_RB) {
+            const int rn = (int)(rows - r0 < QP_RB ? rows - r0 : QP_RB);
+            int rr = 0;
+            for (; rr + 4 <= rn; rr += 4) {
+                const double *x0 = X + (size_t)(r0 + rr) * nb;
+                for (int s0 = 0; s0 < np; s0 += 8) {
+                    v4d a00 = {0, 0, 0, 0}, a01 = a00, a10 = a00, a11 = a00, a20 = a00, a21 = a00, a30 = a00, a31 = a00;
+                    for (int l = 0; l < nb; ++l) {
+                        const double *bl = cp + (size_t)l * np + s0;
+                        v4d b0, b1;
+                        memcpy(&b0, bl, 32);
+                        memcpy(&b1, bl + 4, 32);
+                        const double x0l = x0[l], x1l = x0[nb + l], x2l = x0[2 * nb + l], x3l = x0[3 * nb + l];
+                        a00 += x0l * b0; a01 += x0l * b1;
+                        a10 += x1l * b0; a11 += x1l * b1;
+                        a20 += x2l * b0; a21 += x2l * b1;
+                        a30 += x3l * b0; a31 += x3l * b1;
+                    }
+                    double *o = buf + (size_t)rr * np + s0;
+                    memcpy(o, &a00, 32); memcpy(o + 4, &a01, 32);
+                    memcpy(o + np, &a10, 32); memcpy(o + np + 4, &a11, 32);
+                    memcpy(o + 2 * np, &a20, 32); memcpy(o + 2 * np + 4, &a21, 32);
+                    memcpy(o + 3 * np, &a30, 32); memcpy(o + 3 * np + 4, &a31, 32);
+                }
+            }
+            for (; rr < rn; ++rr) {
+                const double *x0 = X + (size_t)(r0 + rr) * nb;
+                for (int t = 0; t < np; ++t) buf[(size_t)rr * np + t] = 0.0;
+                for (int l = 0; l < nb; ++l)
+                    for (int t = 0; t < np; ++t) buf[(size_t)rr * np + t] += x0[l] * cp[(size_t)l * np + t];
+            }
+            for (int t = 0; t < nm; ++t) {
+                double *dst = Y + (size_t)t * rows + r0;
+                for (int i = 0; i < rn; ++i) dst[i] = buf[(size_t)i * np + t];
+            }
+        }
+        free(buf);
+    }
+    free(cp);
+}
+
+/* ------------------------------------------------------------------------- */
+/* AO -> MO four-index transform (src/maths/eri.rs:258-313):                 */
+/* out[p,q,r,s] = sum c1[mu,p] c2[nu,q] c3[lam,r] c4[sig,s] (mu nu|lam sig)  */
+/* as four quarter passes, each contracting the last index and moving the    */
+/* new index to the front: [mu nu lam sig] -> [s mu nu lam] -> [r s mu nu]   */
+/* -> [q r s mu] -> [p q r s].                                               */
+/* ------------------------------------------------------------------------- */
 static void ao2mo(int nb, int nm, const double *eri, const double *c1, const double *c2,
                   const double *c3, const double *c4, double *out)
 {
-    size_t nb2 = (size_t)nb * nb, nb3 = nb2 * nb, nm2 = (size_t)nm * nm, nm3 = nm2 * nm;
-    double *c4t = (double *)malloc(sizeof(double) * nb * nm);
-    for (int sg = 0; sg < nb; ++sg)
-        for (int s = 0; s < nm; ++s) c4t[(size_t)s * nb + sg] = c4[(size_t)sg * nm + s];
-    double *t1 = (double *)calloc(nb3 * nm, sizeof(double));
-    /* t1[mu,nu,lam,s] = sum_sig eri[mu,nu,lam,sig] c4[sig,s] */
-#pragma omp parallel for schedule(static)
-    for (long long a = 0; a < (long long)nb3; ++a)
-        for (int s = 0; s < nm; ++s) {
-            double acc = 0;
-            const double *e = eri + (size_t)a * nb, *c = c4t + (size_t)s * nb;
-            for (int sg = 0; sg < nb; ++sg) acc += e[sg] * c[sg];
-            t1[(size_t)a * nm + s] = acc;
-        }
-    free(c4t);
-    double *t2 = (double *)calloc(nb2 * nm2, sizeof(double));
-    /* t2[mu,nu,r,s] = sum_lam c3[lam,r] t1[mu,nu,lam,s] */
-#pragma omp parallel for schedule(static)
-    for (long long mn = 0; mn < (long long)nb2; ++mn)
-        for (int l = 0; l < nb; ++l) {
-            const double *src = t1 + ((size_t)mn * nb + l) * nm;
-            for (int r = 0; r < nm; ++r) {
-                double cr = c3[(size_t)l * nm + r];
-                double *dst = t2 + ((size_t)mn * nm + r) * nm;
-                for (int s = 0; s < nm; ++s) dst[s] += cr * src[s];
-            }
-        }
+    size_t nb2 = (size_t)nb * nb, nm2 = (size_t)nm * nm;
+    double *t1 = (double *)malloc(sizeof(double) * nb2 * nb * nm);
+    qpass((long long)(nb2 * nb), nb, nm, eri, c4, t1);
+    double *t2 = (double *)malloc(sizeof(double) * nb2 * nm2);
+    qpass((long long)(nm * nb2), nb, nm, t1, c3, t2);
     free(t1);
-    double *t3 = (double *)calloc((size_t)nb * nm3, sizeof(double));
-    /* t3[mu,q,r,s] = sum_nu c2[nu,q] t2[mu,nu,r,s] */
-#pragma omp parallel for schedule(static)
-    for (int mu = 0; mu < nb; ++mu)
-        for (int nu = 0; nu < nb; ++nu) {
-            const double *src = t2 + ((size_t)mu * nb + nu) * nm2;
-            for (int q = 0; q < nm; ++q) {
-                double cq = c2[(size_t)nu * nm + q];
-                double *dst = t3 + ((size_t)mu * nm + q) * nm2;
-                for (size_t rs = 0; rs < nm2; ++rs) dst[rs] += cq * src[rs];
-            }
-        }
+    double *t3 = (double *)malloc(sizeof(double) * nb * nm2 * nm);
+    qpass((long long)(nm2 * nb), nb, nm, t2, c2, t3);
     free(t2);
-    /* out[p,q,r,s] = sum_mu c1[mu,p] t3[mu,q,r,s] */
-#pragma omp parallel for schedule(static)
-    for (int p = 0; p < nm; ++p) {
-        double *dst = out + (size_t)p * nm3;
-        for (size_t k = 0; k < nm3; ++k) dst[k] = 0.0;
-        for (int mu = 0; mu < nb; ++mu) {
-            double cp = c1[(size_t)mu * nm + p];
-            const double *src = t3 + (size_t)mu * nm3;
-            for (size_t k = 0; k < nm3; ++k) dst[k] += cp * src[k];
-        }
-    }
+    qpass((long long)(nm2 * nm), nb, nm, t3, c1, out);
     free(t3);
 }
 
@@ -470,14 +493,16 @@ static void coulomb(int n, const double *eri, const double *M, double *J)
 static void exchange(int n, const double *eri, const double *M, double *K)
 {
 #pragma omp parallel for schedule(static)
-    for (int s = 0; s < n; ++s)
-        for (int t = 0; t < n; ++t) {
-            double acc = 0;
-            for (int mu = 0; mu < n; ++mu)
-                for (int nu = 0; nu < n; ++nu)
-                    acc += eri[(((size_t)s * n + mu) * n + nu) * n + t] * M[mu * n + nu];
-            K[s * n + t] = acc;
-        }
+    for (int s = 0; s < n; ++s) {
+        double *k = K + (size_t)s * n;
+        for (int t = 0; t < n; ++t) k[t] = 0.0;
+        for (int mu = 0; mu < n; ++mu)
+            for (int nu = 0; nu < n; ++nu) {
+                const double m = M[mu * n + nu];
+                const double *e = eri + (((size_t)s * n + mu) * n + nu) * n;
+                for (int t = 0; t < n; ++t) k[t] += e[t] * m;
+            }
+    }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -628,12 +653,14 @@ int ow_build_pair(int nb, int nm, const double *S, const double *h, const double
             int i = JBR[s][0], j = JBR[s][1], k = JBR[s][2], l = JBR[s][3];
             ao2mo(nb, nm, eri, c->cx[i], c->xc[j], c->cx[k], c->xc[l], blk);
             ao2mo(nb, nm, eri, c->cx[i], c->cx[k], c->xc[j], c->xc[l], exch);
+#pragma omp parallel for schedule(static)
             for (int p = 0; p < nm; ++p)
                 for (int q = 0; q < nm; ++q)
                     for (int r = 0; r < nm; ++r)
                         for (int t = 0; t < nm; ++t)
                             blk[((p * (size_t)nm + q) * nm + r) * nm + t] -= exch[((p * (size_t)nm + r) * nm + q) * nm + t];
             double *dst = slab + o[O_J + s];
+#pragma omp parallel for schedule(static)
             for (int r = 0; r < nm; ++r)
                 for (int cc = 0; cc < nm; ++cc)
                     for (int ii = 0; ii < nm; ++ii)
@@ -896,23 +923,38 @@ static double col_repl_det(int n, const double *cof, int col, const double *newc
     return v;
 }
 
-/* Same-spin sum over distributions of an L+2 bit assignment (pho.rs:20550-20760). */
-/* mode 0 reproduces the reference arithmetic literally; mode 1 is the
- * corrected form (agrees with the generalised Slater-Condon route and with the
+/* Same-spin sum over distributions of an L+2 bit assignment (pho.rs:20550-20760).
+ * `rd` selects, one bit per reading, which literal readings of the
+ * reference's m>0 arithmetic apply (RD_ALL = the reference, 0 = the corrected
+ * form, which agrees with the generalised Slater-Condon route and with the
  * brute-force determinant expansion):
- *   (a) v0[1] is stored as the sum of the (0,1) and (1,0) assignments
- *       (build.rs:229-232) while the (m1,m2) enumeration visits both, so the
- *       corrected form uses v0[1]/2 per visit;
- *   (b) the one-column term reads v[m1][m2][mk] (pho.rs:20648-20651); the
- *       stored order is v[left][right][JK] (build.rs:242-259), so the
- *       replaced column's bit belongs on the right: v[m1][mk][m2];
- *   (c) the two-column term uses jslot(m1, m2, mk, mj) (pho.rs:20690); the
- *       column bit belongs with the xc index: jslot(m1, mk, m2, mj);
- *   (d) first minors with |det| <= tol are skipped (helpers.rs:636-660) and
- *       distributions with |det D_mix| <= tol are dropped from H
- *       (helpers.rs:737-748); cofactors exist for singular matrices, so the
- *       corrected form keeps both. */
-static void same_gen(const same_view *w, const prep *p, double tol, int mode, double *s_out, double *h1_out, double *h2_out)
+ *   RD_V0    v0[1] is stored as the sum of the (0,1) and (1,0) assignments
+ *            (build.rs:229-232) while the (m1,m2) enumeration visits both and
+ *            adds v0[m1+m2] each time (pho.rs:20637); corrected: v0[1]/2 per visit;
+ *   RD_VIDX  the one-column term reads v[m1][m2][mk] (pho.rs:20648-20651); the
+ *            stored order is v[left][right][JK] (build.rs:242-259), so the
+ *            replaced column's bit belongs on the right: v[m1][mk][m2];
+ *   RD_JSLOT the two-column term uses jslot(m1, m2, mk, mj) (pho.rs:20690);
+ *            corrected: jslot(m1, mk, m2, mj);
+ *   RD_DIST  distributions with |det D_mix| <= tol are dropped from H
+ *            (adjugate_transpose_generic, helpers.rs:737-748; pho.rs:20577-20586,
+ *            and the different-spin sum), their overlap kept (:20735-20739);
+ *   RD_MINOR first minors with |det| <= tol are skipped (minor_adjt,
+ *            helpers.rs:636-660);
+ *   RD_ENUC  E_nuc S is not added (the `_enuc` argument, pho.rs:20555).
+ * Cofactors exist for singular matrices, so the corrected form keeps every
+ * distribution and minor.
+ * RD_VIDX and RD_JSLOT leave the sum unchanged: exchanging the operator
+ * assignment m2 with the replaced column's assignment mk is a bijection of
+ * the distributions, and the cofactors of column k do not involve column k.
+ * Two planted mistakes (test use only, never part of mode 0) show that the
+ * checks of those readings would see a wrong index: RD_PLANT_VT reads the
+ * one-column matrix untransposed, RD_PLANT_JCOL exchanges the two column
+ * labels of the J entry. */
+enum { RD_V0 = 1, RD_VIDX = 2, RD_JSLOT = 4, RD_DIST = 8, RD_MINOR = 16, RD_ENUC = 32, RD_ALL = 63,
+       RD_PLANT_VT = 256, RD_PLANT_JCOL = 512 };
+
+static void same_gen(const same_view *w, const prep *p, double tol, int rd, double *s_out, double *h1_out, double *h2_out)
 {
     int l = p->l, n = w->n;
     uint64_t combos[4096];
@@ -926,7 +968,7 @@ static void same_gen(const same_view *w, const prep *p, double tol, int mode, do
         double dd = 1.0;
         int ok = 1;
         if (l > 0) {
-            if (mode == 0) ok = adjugate_transpose_generic(cof, mix, l, tol, &dd);
+            if (rd & RD_DIST) ok = adjugate_transpose_generic(cof, mix, l, tol, &dd);
             else dd = adjugate_transpose(cof, mix, l);
         }
         if (ok) {
@@ -944,14 +986,17 @@ static void same_gen(const same_view *w, const prep *p, double tol, int mode, do
                 h1 += contrib;
             }
             double v0 = w->v0[m1 + m2];
-            if (mode == 1 && m1 + m2 == 1) v0 *= 0.5;
+            if (!(rd & RD_V0) && m1 + m2 == 1) v0 *= 0.5;
             double contrib = v0 * dd;
             for (int k = 0; k < l; ++k) {
                 int mk = bit(bits, k + 2);
-                const double *vsl = mode == 0 ? w->v[m1][m2][mk] : w->v[m1][mk][m2];
+                const double *vsl = (rd & RD_VIDX) ? w->v[m1][m2][mk] : w->v[m1][mk][m2];
                 int ck = p->cols[k];
                 double corr = 0;
-                for (int r = 0; r < l; ++r) corr += (vsl[ck * n + p->rows[r]] - mix[r * l + k]) * cof[r * l + k];
+                for (int r = 0; r < l; ++r) {
+                    double vv = (rd & RD_PLANT_VT) ? vsl[p->rows[r] * n + ck] : vsl[ck * n + p->rows[r]];
+                    corr += (vv - mix[r * l + k]) * cof[r * l + k];
+                }
                 contrib -= 2.0 * (dd + corr);
             }
             if (l >= 2) {
@@ -965,7 +1010,7 @@ static void same_gen(const same_view *w, const prep *p, double tol, int mode, do
                         int lm1 = l - 1;
                         build_minor(minor, mix, l, i, j);
                         double dm;
-                        if (mode == 1) {
+                        if (!(rd & RD_MINOR)) {
                             dm = adjugate_transpose(mcof, minor, lm1);
                         } else if (l <= 4) {
                             dm = adjugate_transpose(mcof, minor, lm1);
@@ -977,12 +1022,14 @@ static void same_gen(const same_view *w, const prep *p, double tol, int mode, do
                             int kf = k2 < j ? k2 : k2 + 1;
                             int mk = bit(bits, kf + 2);
                             int swap;
-                            int slot = mode == 0 ? jslot(m1, m2, mk, mj, &swap) : jslot(m1, mk, m2, mj, &swap);
+                            int slot = (rd & RD_JSLOT) ? jslot(m1, m2, mk, mj, &swap) : jslot(m1, mk, m2, mj, &swap);
                             const double *jsl = w->j[slot];
                             for (int r = 0; r < lm1; ++r) {
                                 int rf = r < i ? r : r + 1;
                                 int rr = p->rows[rf], cc = p->cols[kf];
+                                if (rd & RD_PLANT_JCOL) { int t = cc; cc = cj; cj = t; }
                                 newc[r] = swap ? jsl[idx4(n, rr, cc, ri, cj)] : jsl[idx4(n, ri, cj, rr, cc)];
+                                if (rd & RD_PLANT_JCOL) cj = p->cols[j];
                             }
                             contrib += ph * col_repl_det(lm1, mcof, k2, newc);
                         }
@@ -998,12 +1045,12 @@ static void same_gen(const same_view *w, const prep *p, double tol, int mode, do
 
 /* xw_hamiltonian_overlap_gen_prepared (pho.rs:20550-20966); omits E_nuc as the reference does. */
 static void hs_gen(const same_view *wa, const same_view *wb, const diff_view *ab, const prep *pa,
-                   const prep *pb, double exph, double tol, int mode, double *H, double *S)
+                   const prep *pb, double exph, double tol, int rd, double *H, double *S)
 {
     int la = pa->l, lb = pb->l, n = ab->n;
     double sa, h1a, h2aa, sb, h1b, h2bb;
-    same_gen(wa, pa, tol, mode, &sa, &h1a, &h2aa);
-    same_gen(wb, pb, tol, mode, &sb, &h1b, &h2bb);
+    same_gen(wa, pa, tol, rd, &sa, &h1a, &h2aa);
+    same_gen(wb, pb, tol, rd, &sb, &h1b, &h2bb);
     double h2ab = 0.0;
     uint64_t ca[4096], cb[4096];
     int nca = m_combinations(la + 1, wa->m, ca);
@@ -1014,7 +1061,7 @@ static void hs_gen(const same_view *wa, const same_view *wb, const diff_view *ab
         mix_columns(mixa, pa->det0, pa->det1, la, ba >> 1);
         double da = 1.0;
         if (la > 0) {
-            if (mode == 0) { if (!adjugate_transpose_generic(cofa, mixa, la, tol, &da)) continue; }
+            if (rd & RD_DIST) { if (!adjugate_transpose_generic(cofa, mixa, la, tol, &da)) continue; }
             else da = adjugate_transpose(cofa, mixa, la);
         }
         for (int ib = 0; ib < ncb; ++ib) {
@@ -1022,7 +1069,7 @@ static void hs_gen(const same_view *wa, const same_view *wb, const diff_view *ab
             mix_columns(mixb, pb->det0, pb->det1, lb, bb >> 1);
             double db = 1.0;
             if (lb > 0) {
-                if (mode == 0) { if (!adjugate_transpose_generic(cofb, mixb, lb, tol, &db)) continue; }
+                if (rd & RD_DIST) { if (!adjugate_transpose_generic(cofb, mixb, lb, tol, &db)) continue; }
                 else db = adjugate_transpose(cofb, mixb, lb);
             }
             int ma0 = bit(ba, 0), mb0 = bit(bb, 0);
@@ -1067,10 +1114,11 @@ static void hs_gen(const same_view *wa, const same_view *wb, const diff_view *ab
 static u128 mk128(const uint64_t *p) { return ((u128)p[1] << 64) | p[0]; }
 
 /* xw_hamiltonian_overlap_prepared (pho.rs:53): one determinant pair on one
- * ordered reference pair (slab at the pair base). */
-int ow_hs(const double *slab, const long long *off, const double *meta, int nm,
-          const uint64_t *xex, const uint64_t *wex, double exph, double enuc, double tol,
-          int mode, double *H, double *S)
+ * ordered reference pair (slab at the pair base), with the m>0 readings `rd`
+ * (RD_* above; m = 0 pairs do not depend on them). */
+int ow_hs_readings(const double *slab, const long long *off, const double *meta, int nm,
+                   const uint64_t *xex, const uint64_t *wex, double exph, double enuc, double tol,
+                   int rd, double *H, double *S)
 {
     same_view sv[2];
     diff_view dv;
@@ -1080,11 +1128,18 @@ int ow_hs(const double *slab, const long long *off, const double *meta, int nm,
     prepare_same(&sv[1], mk128(xex + 4), mk128(xex + 6), mk128(wex + 4), mk128(wex + 6), &pb);
     if (sv[0].m == 0 && sv[1].m == 0) hs_m0(&sv[0], &sv[1], &dv, &pa, &pb, exph, enuc, H, S);
     else {
-        hs_gen(&sv[0], &sv[1], &dv, &pa, &pb, exph, tol, mode, H, S);
-        /* the reference's m>0 path omits E_nuc S (pho.rs:20555,20963) */
-        if (mode == 1) *H += enuc * *S;
+        hs_gen(&sv[0], &sv[1], &dv, &pa, &pb, exph, tol, rd, H, S);
+        if (!(rd & RD_ENUC)) *H += enuc * *S;
     }
     return 0;
+}
+
+/* mode 0: the reference arithmetic (every reading); mode 1: corrected. */
+int ow_hs(const double *slab, const long long *off, const double *meta, int nm,
+          const uint64_t *xex, const uint64_t *wex, double exph, double enuc, double tol,
+          int mode, double *H, double *S)
+{
+    return ow_hs_readings(slab, off, meta, nm, xex, wex, exph, enuc, tol, mode == 0 ? RD_ALL : 0, H, S);
 }
 
 /* construct_f_scalar (build.rs:564-617) as applied by update_wicks_fock

@@ -37,6 +37,7 @@ def lib():
         L.ow_build_pair.argtypes = [ct.c_int, ct.c_int, D, D, D, D, D, U64, U64, D, D, U64, U64,
                                     ct.c_double, I64, D, D]
         L.ow_hs.argtypes = [D, I64, D, ct.c_int, U64, U64, ct.c_double, ct.c_double, ct.c_double, ct.c_int, D, D]
+        L.ow_hs_readings.argtypes = L.ow_hs.argtypes
         L.ow_excitation_phase.argtypes = [U64, U64, U64]
         L.ow_excitation_phase.restype = ct.c_double
         L.ow_hs_naive.argtypes = [ct.c_int, ct.c_int, D, D, D, D, D, D, U64, U64, D, D, U64, U64,
@@ -50,6 +51,13 @@ def lib():
         L.ow_offsets_len.restype = ct.c_int
         _lib = L
     return _lib
+
+
+# Literal readings of the reference's m>0 arithmetic (oracle/nwick_oracle.c RD_*).
+RD_V0, RD_VIDX, RD_JSLOT, RD_DIST, RD_MINOR, RD_ENUC = 1, 2, 4, 8, 16, 32
+RD_ALL = 63
+# planted index mistakes, for showing that the checks of RD_VIDX / RD_JSLOT see one
+RD_PLANT_VT, RD_PLANT_JCOL = 256, 512
 
 
 def _d(a):
@@ -167,17 +175,25 @@ class Slab:
         b = self.base[(x, w)]
         return self.slab[b:], self.off[(x, w)], self.meta[(x, w)]
 
-    def hs(self, bra, ket, mode=0):
-        """xw_hamiltonian_overlap_prepared for one determinant pair (bra = x side).
-        mode 0: the reference arithmetic; mode 1: corrected m>0 path."""
+    def _hs_call(self, fn, bra, ket, arg):
         sub, off, meta = self.pair_view(bra.parent, ket.parent)
         xex = np.concatenate([mask2(bra.ha), mask2(bra.pa), mask2(bra.hb), mask2(bra.pb)])
         wex = np.concatenate([mask2(ket.ha), mask2(ket.pa), mask2(ket.hb), mask2(ket.pb)])
         H = ct.c_double(); S = ct.c_double()
         exph = (bra.pha * ket.pha) * (bra.phb * ket.phb)
-        lib().ow_hs(_d(sub), off.ctypes.data_as(I64), _d(meta), self.nmo, _u(xex), _u(wex),
-                    exph, self.enuc, self.tol, mode, ct.byref(H), ct.byref(S))
+        fn(_d(sub), off.ctypes.data_as(I64), _d(meta), self.nmo, _u(xex), _u(wex),
+           exph, self.enuc, self.tol, int(arg), ct.byref(H), ct.byref(S))
         return H.value, S.value
+
+    def hs(self, bra, ket, mode=0):
+        """xw_hamiltonian_overlap_prepared for one determinant pair (bra = x side).
+        mode 0: the reference arithmetic; mode 1: corrected m>0 path."""
+        return self._hs_call(lib().ow_hs, bra, ket, mode)
+
+    def hs_readings(self, bra, ket, rd):
+        """One determinant pair with a chosen subset `rd` of the reference's
+        literal m>0 readings (RD_* flags; RD_ALL = mode 0, 0 = mode 1)."""
+        return self._hs_call(lib().ow_hs_readings, bra, ket, rd)
 
 
 def det_coeffs(sysm, det):
