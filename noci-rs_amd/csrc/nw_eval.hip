@@ -169,13 +169,25 @@ __device__ inline void eval_tile(const EvalBlock *__restrict__ blocks, int nbloc
     }
 }
 
-// Tiles are visited grid-stride, so a bin of any size fits one launch whose
-// grid stays within the runtime's limit of 2^32 - 1 work-items.
+// XCD-aware tile order: workgroups are dealt round-robin over the 8 XCDs, so
+// workgroup b runs on XCD b % 8.  The tile list is cut into 8 contiguous
+// chunks and XCD x takes chunk x, in order: neighbouring tiles (one bra strip
+// against consecutive ket strips, which gather from the same rank-4 slices
+// J[a, i, :, :] of the strip's bra labels) then meet in one XCD's L2 instead
+// of being spread over all eight.  Within a chunk the tiles are visited
+// grid-stride, so a bin of any size fits one launch whose grid stays within
+// the runtime's limit of 2^32 - 1 work-items.
+constexpr int EVAL_XCD = 8;
+
 template <int KIND, int LA, int LB, bool M0>
 __global__ __launch_bounds__(256) void k_eval_blocks(const EvalBlock *__restrict__ blocks, int nblocks, long long ntiles,
                                                      EvalArgs a)
 {
-    for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) eval_tile<KIND, LA, LB, M0>(blocks, nblocks, a, t);
+    const long long chunk = (ntiles + EVAL_XCD - 1) / EVAL_XCD;
+    const int xcd = blockIdx.x % EVAL_XCD;
+    const long long lo = xcd * chunk, hi = min(ntiles, lo + chunk);
+    const long long step = gridDim.x / EVAL_XCD;
+    for (long long t = lo + blockIdx.x / EVAL_XCD; t < hi; t += step) eval_tile<KIND, LA, LB, M0>(blocks, nblocks, a, t);
 }
 
 template <int KIND, int LA, int LB, bool M0>
@@ -212,7 +224,10 @@ void launch_eval_blocks(hipStream_t st, int kind, int la, int lb, bool m0, const
                         long long ntiles, const EvalArgs &args)
 {
     if (ntiles <= 0) return;
-    dim3 grid((unsigned)std::min(ntiles, eval_grid_cap())), block(256);
+    // a multiple of the XCD count: every XCD gets the same number of workgroups
+    const long long chunk = (ntiles + EVAL_XCD - 1) / EVAL_XCD;
+    const long long per_xcd = std::max(1ll, std::min(chunk, eval_grid_cap() / EVAL_XCD));
+    dim3 grid((unsigned)(per_xcd * EVAL_XCD)), block(256);
 #define NW_T(A, B)                                                                                             \
     if (la == A && lb == B) {                                                                                  \
         if (kind == EVAL_HS) {                                                                                 \
