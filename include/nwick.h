@@ -186,6 +186,20 @@ int nw_pair_slab(nw_ctx *ctx, int x, int w, double *out, int64_t *offsets, doubl
 int nw_save_wicks(nw_ctx *ctx, const char *dir);
 int nw_load_wicks(nw_ctx *ctx, const char *dir, double tol);
 
+/* nw_hs_batched for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * No counterpart in the reference.  Same requests, outputs and errors as
+ * nw_hs_batched, but dets, req, h_out and s_out must be device memory of the
+ * context's GPU (NW_E_ARG otherwise), ndet <= 2^18 and nreq <= 2^20
+ * (NW_E_SHAPE otherwise), and nothing is allocated: the temporaries live in
+ * one fixed device array of about 26 MiB per GPU, which that GPU holds from
+ * the first call on it until the process ends.  Calls on one GPU share the
+ * array and are serialised within the process (contexts included); the call
+ * synchronises with the device (it reads the per-bin counts), so it is not
+ * itself graph-capturable. */
+int nw_hs_batched_noalloc(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t *req, uint64_t nreq,
+                          double *h_out, double *s_out, int mode);
+
 /* Device time in milliseconds of the kernels in the last nw_build /
  * nw_hs_matrix / nw_hs_batched call, split as {total, transform, evaluate}. */
 int nw_last_timings(nw_ctx *ctx, double *ms3);

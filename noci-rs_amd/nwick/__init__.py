@@ -37,7 +37,7 @@ EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matri
            "nw_mlp_get_params", "nw_mlp_forward", "nw_mlp_grad", "nw_mlp_step", "nw_mlp_train", "nw_mlp_last_error",
            "nw_mlp_destroy", "nw_factor_create", "nw_factor_refresh", "nw_factor_apply", "nw_factor_diagonals",
            "nw_factor_info", "nw_factor_ids", "nw_factor_table", "nw_factor_last_ms", "nw_factor_destroy",
-           "nw_factor_set_precision", "nw_factor_apply_pair", "nw_factor_apply_bytes"]
+           "nw_factor_set_precision", "nw_factor_apply_pair", "nw_factor_apply_bytes", "nw_hs_batched_noalloc"]
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_BF16X3 = 2
@@ -85,6 +85,7 @@ def load():
     L.nw_set_refs.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
     L.nw_build.argtypes = [vp, ct.c_double, vp, vp]
     L.nw_hs_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, vp, vp, ct.c_int]
+    L.nw_hs_batched_noalloc.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, vp, vp, ct.c_int]
     L.nw_hs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
     L.nw_pair_slab.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
     L.nw_set_precision.argtypes = [vp, ct.c_int]
@@ -232,6 +233,16 @@ class Engine:
         sync_torch(dets)
         self._check(self.L.nw_hs_batched(self.ctx, _ptr(dets), len(dets), _ptr(req), len(req), _ptr(h), _ptr(s),
                                          int(mode)))
+        return h, s
+
+    def hs_batched_device(self, dets, req, h, s, mode=MODE_REFERENCE, noalloc=False):
+        """nw_hs_batched (or nw_hs_batched_noalloc) on device tensors: dets a
+        uint8 tensor of records, req an int64 tensor (n, 2), h and s float64."""
+        ndet = dets.numel() * dets.element_size() // DET_DTYPE.itemsize
+        nreq = req.numel() // 2
+        sync_torch(dets, req, h, s)
+        fn = self.L.nw_hs_batched_noalloc if noalloc else self.L.nw_hs_batched
+        self._check(fn(self.ctx, _ptr(dets), ndet, _ptr(req), nreq, _ptr(h), _ptr(s), int(mode)))
         return h, s
 
     def update_fock(self, fa, fb):
