@@ -32,6 +32,69 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_info():
+    """CPU model, logical CPUs of the machine and of this process."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": aff}
+
+
+def openblas_quarter_pass(n, rows=None):
+    """GFLOP/s of numpy's bundled OpenBLAS (the GEMM library the reference
+    links through ndarray-linalg / openblas-src) on one quarter of the n^4
+    transform, rows x n times n x n: the rate a BLAS-based CPU build reaches."""
+    import numpy as np
+    rows = rows or n ** 3
+    X = np.random.default_rng(0).random((rows, n))
+    c = np.random.default_rng(1).random((n, n))
+    X @ c
+    t = time.perf_counter()
+    X @ c
+    return 2.0 * rows * n * n / (time.perf_counter() - t) / 1e9
+
+
+def transform_flops(nb, ma, mb):
+    """Rank-4 transforms of one ordered pair's CPU build in the restatement:
+    two per same-spin J slot (direct and exchange, as build.rs:267-313) and
+    one per II block, 8 nb^5 flop each."""
+    slots = lambda m: sum(1 for b in [(0, 0, 0, 0), (0, 0, 0, 1), (0, 0, 1, 0), (0, 0, 1, 1), (0, 1, 0, 1),
+                                      (0, 1, 1, 0), (0, 1, 1, 1), (1, 0, 1, 0), (1, 0, 1, 1), (1, 1, 1, 1)]
+                          if sum(b) <= m)
+    nii = sum(1 for a0 in (0, 1) for aj in (0, 1) for b0 in (0, 1) for bj in (0, 1) if a0 + aj <= ma and b0 + bj <= mb)
+    return (2 * (slots(ma) + slots(mb)) + nii) * 8.0 * nb ** 5
+
+
+def tensors_of(ma, mb):
+    """Rank-4 tensors one ordered pair's GPU build transforms: one per
+    same-spin J slot (allocated when its branch sum is at most m,
+    build.rs:19-30) and one per II block (a0 + aj <= m_a, b0 + bj <= m_b)."""
+    slots = lambda m: sum(1 for b in [(0, 0, 0, 0), (0, 0, 0, 1), (0, 0, 1, 0), (0, 0, 1, 1), (0, 1, 0, 1),
+                                      (0, 1, 1, 0), (0, 1, 1, 1), (1, 0, 1, 0), (1, 0, 1, 1), (1, 1, 1, 1)]
+                          if sum(b) <= m)
+    nii = sum(1 for a0 in (0, 1) for aj in (0, 1) for b0 in (0, 1) for bj in (0, 1) if a0 + aj <= ma and b0 + bj <= mb)
+    return slots(ma) + slots(mb) + nii
+
+
+def eval_bytes(la, lb):
+    """Algorithmic bytes of one determinant-pair element of contraction ranks
+    (la, lb) on an m = 0 reference pair: the X/Y entries of both spins, the
+    one-column hcol0 entries, the J and II gathers, the two outputs and the
+    two packed determinants (the per-element formula of the evaluation
+    roofline).  Numpy arrays or ints."""
+    c2 = lambda l: l * (l - 1) // 2
+    return 8.0 * (2 * (la * la + lb * lb) + 2 * c2(la) ** 2 + 2 * c2(lb) ** 2 + la * la * lb * lb) + 60.0
+
+
 def cpu_baseline(sysm, dets, tol, mode, with_build=True):
     """Time the CPU oracle on a bounded sample of the same workload: the
     build of one ordered reference pair (0, 1) plus the evaluation of all
@@ -74,15 +137,20 @@ def cpu_baseline(sysm, dets, tol, mode, with_build=True):
     L.ow_hs_many(O._d(sub), off.ctypes.data_as(O.I64), O._d(meta), sysm.nmo, npair, O._u(xex), O._u(wex),
                  O._d(exph), sysm.enuc, tol, mode, O._d(H), O._d(S))
     t3 = time.perf_counter()
+    host = host_info()
     if not with_build:
-        return {"value": npair / (t3 - t2), "unit": "pairs/s", "cores": threads, "kind": "port",
-                "sample": f"oracle evaluation of the {len(bra)}x{len(ket)} determinant pairs of ordered reference "
-                          f"pair (0,1) on its resident slab block ({t3 - t2:.2f} s; build {t1 - t0:.1f} s not "
-                          f"counted), OpenMP threads={threads}"}
+        return dict({"value": npair / (t3 - t2), "unit": "pairs/s", "cores": threads, "kind": "port",
+                     "sample": f"oracle evaluation of the {len(bra)}x{len(ket)} determinant pairs of ordered "
+                               f"reference pair (0,1) on its resident slab block ({t3 - t2:.2f} s; build "
+                               f"{t1 - t0:.1f} s not counted), OpenMP threads={threads}"}, **host)
     secs = (t1 - t0) + (t3 - t2)
-    return {"value": npair / secs, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"oracle build of ordered reference pair (0,1) ({t1 - t0:.1f} s) + evaluation of its "
-                      f"{len(bra)}x{len(ket)} determinant pairs ({t3 - t2:.2f} s), OpenMP threads={threads}"}
+    ma, mb = slab.m[(0, 1)]
+    fl = transform_flops(sysm.n_bas, ma, mb)
+    return dict({"value": npair / secs, "unit": "pairs/s", "cores": threads, "kind": "port",
+                 "sample": f"oracle build of ordered reference pair (0,1) ({t1 - t0:.1f} s) + evaluation of its "
+                           f"{len(bra)}x{len(ket)} determinant pairs ({t3 - t2:.2f} s), OpenMP threads={threads}",
+                 "build_transform_gflops": fl / (t1 - t0) / 1e9,
+                 "openblas_quarter_pass_gflops": openblas_quarter_pass(sysm.n_bas)}, **host)
 
 
 def pt2_workload(args):
@@ -189,7 +257,8 @@ def pt2_workload(args):
         f"FP32 vs FP64 tables {rel32:.2e}")
     # PT2 solves: FP64, and FP32 Arnoldi matvecs with FP64 restart residuals
     sol = {}
-    for eps in (0.0, args.pt2_shift):
+    shifts = (args.pt2_shift,) if getattr(args, "compact", False) else (0.0, args.pt2_shift)
+    for eps in shifts:
         key = f"imag_shift_{eps:g}"
         sol[key] = {}
         for prec in ("fp64", "mixed"):
@@ -208,30 +277,64 @@ def pt2_workload(args):
     pairs = n * (n + 1) // 2
     use32 = mixed_ok and el32 < el
     el_v, kern_v, bytes_v = (el32, kern32, bytes32) if use32 else (el, kern, bytes64)
-    out = {"metric": "NOCI-PT2 matrix-free M = F - E0 S matvec, candidate pairs/s", "value": pairs * args.steps / el_v,
-           "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = pt2_cpu_baseline(info, pt)
+    out = {"metric": "NOCI-PT2 matrix-free M = F - E0 S matvecs per second (spin-factorised operator)",
+           "value": args.steps / el_v, "unit": "matvecs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": 1e3 * el_v / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32-tables+f64" if use32 else "f64", "data": "synthetic",
            "config": {"workload": "NOCI-PT2 matvec, n_bas=28, nocc=9+9, 6 refs, all singles + 20000 doubles "
                                   "per ref", "candidates": n, "mode": args.mode, "operator": "spin-factorised"},
+           "ms_per_matvec": 1e3 * el_v / args.steps,
            "roofline": {"bound": "hbm", "kernel": "k_fac_dual + k_fac_mat + k_fac_rows (factorised matvec)",
                         "achieved": bytes_v / (float(np.mean(kern_v)) * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
                         "unit": "GB/s",
                         "frac": bytes_v / (float(np.mean(kern_v)) * 1e-3) / 1e9 / PEAK_HBM_GBS,
                         "traffic": None, "traffic_source": "profiles/factor_pt2_c4_hbm_traffic.txt",
-                        "algorithmic_bytes": bytes_v},
+                        "algorithmic_bytes": bytes_v,
+                        "bytes_meaning": "table bytes one apply reads (each table row once; dual passes read a "
+                                         "parent pair's table once for both directions)"},
            "kernel_ms": float(np.mean(kern_v)), "kernel_ms_fp64_tables": float(np.mean(kern)),
-           "kernel_ms_fp32_tables": float(np.mean(kern32)), "fp64_tables_pairs_per_s": pairs * args.steps / el,
-           "fp32_tables_pairs_per_s": pairs * args.steps / el32, "fp32_vs_fp64_tables_max_rel": rel32,
+           "kernel_ms_fp32_tables": float(np.mean(kern32)), "fp32_vs_fp64_tables_max_rel": rel32,
            "kernel_ms_two_vectors_fp64_tables": pair_ms["fp64"], "kernel_ms_two_vectors_fp32_tables": pair_ms["fp32"],
            "table_build_s": pt.timings["factor_s"], "table_bytes": info["table_bytes"],
            "table_bytes_fp32": info["table_bytes"] // 2, "apply_bytes_fp64": bytes64, "apply_bytes_fp32": bytes32,
-           "pairwise_kernel_ms": float(np.mean(pk)),
-           "pairwise_pairs_per_s": pairs / el_p, "factorised_vs_pairwise_max_rel": rel,
+           "pairwise_kernel_ms": float(np.mean(pk)), "factorised_vs_pairwise_max_rel": rel,
            "pt2_solve": sol, "dE_pt2_mixed_minus_fp64": shifted["mixed"]["ept2"] - shifted["fp64"]["ept2"],
            "pt2_solve_speedup_mixed": shifted["fp64"]["solve_s"] / shifted["mixed"]["solve_s"],
-           "e_noci": pt.ecurrent, "e0": pt.e0}
-    print(json.dumps(out), flush=True)
+           "e_noci": pt.ecurrent, "e0": pt.e0, "cpu_baseline": cpu}
+    pt.fac.close()
+    eng.close()
+    return out
+
+
+def pt2_cpu_baseline(info, pt):
+    """CPU rate of the reference's default PT2 operator, the factorised apply
+    over RAM tables (apply_one_body, src/noci/factorise/onebody.rs:290-345):
+    its cost is one pass over the {S, F} tables.  Bounded sample: numpy
+    (OpenBLAS) applies one parent pair's table of the workload's component
+    counts (random values of the same shape) to a vector; the rate, scaled
+    to the resident table bytes of one apply (the GPU's apply_bytes), gives
+    matvecs per second.  The table build on the CPU is not counted."""
+    import numpy as np
+    i0 = pt.fac.info(0)
+    nc = max(i0["ncomp_a"], i0["ncomp_b"])
+    tab = np.random.default_rng(0).random((nc, 2 * nc))
+    x = np.random.default_rng(1).random(nc)
+    x @ tab
+    reps, t = 0, time.perf_counter()
+    while time.perf_counter() - t < 2.0:
+        x @ tab
+        reps += 1
+    dt = (time.perf_counter() - t) / reps
+    gbs = tab.nbytes / dt / 1e9
+    apply_bytes = info["apply_bytes"]
+    return dict({"value": gbs * 1e9 / apply_bytes, "unit": "matvecs/s", "cores": host_info()["affinity_cpus"],
+                 "kind": "port",
+                 "sample": f"numpy (OpenBLAS) apply of one {nc}x{nc} {{S,F}} float64 table ({tab.nbytes / 1e6:.0f} MB) "
+                           f"at {gbs:.1f} GB/s, scaled to the {apply_bytes / 1e9:.2f} GB of FP64 tables one apply reads"},
+                **host_info())
 
 
 def batch_workload(args):
@@ -266,10 +369,15 @@ def batch_workload(args):
             masks.append(np.triu(np.ones((sysm.nref, sysm.nref), dtype=np.uint8)))
             engs.append(eng)
         serial = 0.0
-        for eng, mask in zip(engs, masks):
+        tstat["transform_ms"] = tstat["flops"] = 0.0
+        for eng, mask, (sysm, _) in zip(engs, masks, systems):
             for w in range(args.warmup + 1):
-                eng.build(tol=tol, pair_mask=mask)
-            serial += eng.timings()["total_ms"]
+                meta = eng.build(tol=tol, pair_mask=mask)
+            tm = eng.timings()
+            serial += tm["total_ms"]
+            # FP64 transforms only (the mixed build keeps a varying share in FP64)
+            tstat["transform_ms"] += tm["transform_ms"]
+            tstat["flops"] += sum(tensors_of(pm.ma, pm.mb) for pm in meta.values()) * 8.0 * sysm.n_bas ** 5
         pool = ThreadPoolExecutor(max_workers=len(engs))
         walls = []
         for k in range(args.warmup + args.steps):
@@ -298,8 +406,11 @@ def batch_workload(args):
         return out
 
     res = {}
+    tstat = {}
+    tstats = {}
     for prec, name in ((nwick.PREC_FP64, "fp64"), (nwick.PREC_MIXED, "mixed")):
         engs, build_ms, build_serial = build_all(prec)
+        tstats[name] = dict(tstat)
         b = nwick.Batch()
         for eng, (sysm, dets) in zip(engs, systems):
             b.add(eng, basis.to_records(dets), mode=args.mode)
@@ -358,6 +469,7 @@ def batch_workload(args):
                 B = 8.0 * (2 * (la * la + lb * lb) + 2 * c2(la) ** 2 + 2 * c2(lb) ** 2 + la * la * lb * lb) + 60.0
                 total_bytes += B.sum() * (0.5 if x == y else 1.0)
     achieved = total_bytes / (r["eval_ms"] * 1e-3) / 1e9
+    t64_tflops = tstats["fp64"]["flops"] / (tstats["fp64"]["transform_ms"] * 1e-3) / 1e12
     cpu = None
     if not args.no_cpu_baseline:
         sysm, dets = systems[6]
@@ -383,11 +495,17 @@ def batch_workload(args):
            "schedule": {k: r["schedule"][k] for k in ("grouping", "dynamic", "grid_per_cu")},
            "tuned_buckets": [[e["nocc"], e["la"], e["lb"], e["m0"], e["shape"], round(e["ms_default"] / max(e["ms_best"], 1e-9), 3)]
                              for e in r["schedule"]["buckets"]],
-           "roofline": {"bound": "hbm", "kernel": "k_eval_fused (batched determinant-pair evaluation)",
-                        "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                        "traffic": None, "algorithmic_bytes": total_bytes},
+           "roofline": {"bound": "mfma", "kernel": "k_qtransform_mfma_pipe (FP64 four-index transforms of the "
+                                                 "builds, the step's largest device time)",
+                        "achieved": t64_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": t64_tflops / PEAK_FP64_TFLOPS, "traffic": None,
+                        "flops_per_step": tstats["fp64"]["flops"], "transform_ms_per_step": tstats["fp64"]["transform_ms"],
+                        "counters": "profiles/qtransform_mfma_counters.txt"},
+           "roofline_eval": {"bound": "hbm", "kernel": "k_eval_fused (batched determinant-pair evaluation)",
+                             "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                             "traffic": None, "algorithmic_bytes": total_bytes},
            "cpu_baseline": cpu}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def screen_workload(args):
@@ -432,7 +550,79 @@ def screen_workload(args):
            "screen_s_exhaustive": ex.screen_s, "screen_s_learned": ln.screen_s,
            "recall_top64": ln.recall, "captured_score_fraction": ln.captured, "audit_misses": ln.audit_misses,
            "fallbacks": ln.fallbacks, "training": ln.train}
-    print(json.dumps(out), flush=True)
+    # algorithmic bytes of the exact scores of the screened iterations: every
+    # exactly scored candidate against every current determinant (references
+    # and the candidates selected so far) plus its diagonal element
+    ra, rb = cand.ranks()
+    cur_a, cur_b = [np.zeros(16, int)], [np.zeros(16, int)]
+    for it, pick in enumerate(ln.selected):
+        cur_a.append(ra[np.asarray(pick)]); cur_b.append(rb[np.asarray(pick)])
+    mean_b = []
+    for it in range(2, len(ln.selected)):
+        ca, cb = np.concatenate(cur_a[:it + 1]), np.concatenate(cur_b[:it + 1])
+        sub = np.random.default_rng(it).choice(n, size=4096, replace=False)
+        B = eval_bytes(ra[sub][:, None] + ca[None, :], rb[sub][:, None] + cb[None, :]).sum(1) + \
+            eval_bytes(2 * ra[sub], 2 * rb[sub])
+        mean_b.append(float(B.mean()) * ln.exact_evals[it])
+    alg = float(np.mean(mean_b))
+    out["roofline"] = {"bound": "hbm", "kernel": "k_eval_blocks<PROJECT> + diagonal k_eval_reqs (exact scores; "
+                                                 "the screened iteration's time includes ranking and picking)",
+                       "achieved": alg / t_ln / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "frac": alg / t_ln / 1e9 / PEAK_HBM_GBS, "traffic": None, "algorithmic_bytes": alg,
+                       "bytes_meaning": "per exactly scored candidate: the element bytes against each current "
+                                        "determinant and of its diagonal (m = 0 formula; sampled over 4096 "
+                                        "candidates per iteration)"}
+    out["cpu_baseline"] = None if args.no_cpu_baseline else screen_cpu_baseline(sysm, cand, args.mode)
+    eng.close()
+    return out
+
+
+def screen_cpu_baseline(sysm, cand, mode):
+    """Exact candidate scores on the CPU oracle for a bounded sample: the
+    first 20,000 candidates of reference 0 against the 16 references (the
+    first selection iteration's current space) plus their diagonal elements,
+    after the oracle builds the 16 ordered reference pairs (0, r) (counted)."""
+    import ctypes as ct
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    L = O.lib()
+    L.ow_hs_many.argtypes = [O.D, O.I64, O.D, ct.c_int, ct.c_longlong, O.U64, O.U64, O.D, ct.c_double,
+                             ct.c_double, ct.c_int, O.D, O.D]
+    L.ow_num_threads.restype = ct.c_int
+    t0 = time.perf_counter()
+    slab = O.Slab(sysm, tol=1e-8, pairs=[(0, r) for r in range(sysm.nref)])
+    t1 = time.perf_counter()
+    idx = np.nonzero(cand.parent == 0)[0][:20000]
+    m = len(idx)
+
+    def masks(h):
+        """(lo, hi) 64-bit words of the orbital sets (n_bas = 60: lo only)."""
+        out = np.zeros((m, 2), dtype=np.uint64)
+        for k in range(h.shape[1]):
+            ok = h[idx, k] >= 0
+            out[ok, 0] |= np.uint64(1) << h[idx, k][ok].astype(np.uint64)
+        return out
+    xex = np.ascontiguousarray(np.concatenate([masks(cand.ah), masks(cand.ap), masks(cand.bh), masks(cand.bp)], 1))
+    ph = np.ones(m)
+    wex0 = np.zeros((m, 8), dtype=np.uint64)
+    H = np.zeros(m); S = np.zeros(m)
+    t2 = time.perf_counter()
+    for r in range(sysm.nref):
+        sub, off, meta = slab.pair_view(0, r)
+        L.ow_hs_many(O._d(sub), off.ctypes.data_as(O.I64), O._d(meta), sysm.nmo, m, O._u(xex), O._u(wex0),
+                     O._d(ph), sysm.enuc, 1e-8, mode, O._d(H), O._d(S))
+    sub, off, meta = slab.pair_view(0, 0)
+    L.ow_hs_many(O._d(sub), off.ctypes.data_as(O.I64), O._d(meta), sysm.nmo, m, O._u(xex), O._u(xex),
+                 O._d(ph), sysm.enuc, 1e-8, mode, O._d(H), O._d(S))
+    t3 = time.perf_counter()
+    threads = L.ow_num_threads()
+    return dict({"value": m / ((t1 - t0) + (t3 - t2)), "unit": "candidates/s", "cores": threads, "kind": "port",
+                 "sample": f"oracle build of the 16 ordered pairs (0, r) ({t1 - t0:.1f} s) + exact elements of "
+                           f"{m} candidates of reference 0 against the 16 references and their diagonals "
+                           f"({t3 - t2:.2f} s), OpenMP threads={threads}"}, **host_info())
 
 
 def spawn_workload(args):
@@ -471,22 +661,57 @@ def spawn_workload(args):
         log(f"spawn step {k}: call {1e3 * calls[-1]:.2f} ms, GPU evaluation {evals[-1]:.2f} ms")
     t_call = float(np.mean(calls))
     t_eval = float(np.mean(evals)) * 1e-3
+    # the same requests with every array already in device memory (the rate
+    # reported as value; the host-array call above adds the PCIe copies)
+    import torch
+    rec_d = torch.from_numpy(rec.view(np.uint8).copy()).to("cuda:0")
+    req_d = torch.from_numpy(req.view(np.int64).copy()).to("cuda:0")
+    h_d = torch.empty(nreq, dtype=torch.float64, device="cuda:0")
+    s_d = torch.empty_like(h_d)
+    eng.hs_batched_device(rec_d, req_d, h_d, s_d, mode=args.mode)
+    torch.cuda.synchronize()
+    dcalls = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        eng.hs_batched_device(rec_d, req_d, h_d, s_d, mode=args.mode)
+        dcalls.append(time.perf_counter() - t0)
+    t_dev = float(np.mean(dcalls))
+    assert np.array_equal(h_d.cpu().numpy(), h) and np.array_equal(s_d.cpu().numpy(), s)
+    log(f"spawn device-array call {1e3 * t_dev:.2f} ms")
+    # algorithmic bytes of the requests (evaluation kernels k_eval_reqs)
+    ra = np.array([d.rank_a for d in dets]); rb = np.array([d.rank_b for d in dets])
+    par = np.array([d.parent for d in dets])
+    oa = np.array([d.oa for d in dets], dtype=object); ob = np.array([d.ob for d in dets], dtype=object)
+    a, b = req[:, 0].astype(np.int64), req[:, 1].astype(np.int64)
+    same = par[a] == par[b]
+    zero = np.zeros(nreq, dtype=bool)
+    si = np.nonzero(same)[0]
+    diff = np.array([bin(oa[a[q]] ^ oa[b[q]]).count("1") + bin(ob[a[q]] ^ ob[b[q]]).count("1") for q in si])
+    zero[si[diff > 4]] = True
+    B = np.where(zero, 80.0, eval_bytes(ra[a] + ra[b], rb[a] + rb[b]) + 16.0)
+    alg = float(B.sum())
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sysm, dets, tol, args.mode, with_build=False)
         log(f"cpu baseline: {cpu['value']:.1f} pairs/s ({cpu['sample']})")
     out = {"metric": "QMC spawning elements: H,S of random determinant-pair requests per second (resident slab)",
-           "value": nreq / t_call, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": 1e3 * t_call, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "value": nreq / t_dev, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * t_dev, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic",
            "config": {"workload": "random H,S requests: n_bas=60, nocc=8+8, 8 refs (64 ordered pairs built), "
                                   "512 singles/doubles per ref, 2^21 requests per call", "determinants": len(dets),
                       "requests": nreq},
            "gpu_evaluation_pairs_per_s": nreq / t_eval, "gpu_evaluation_ms": 1e3 * t_eval,
-           "host_share_of_call": 1.0 - t_eval / t_call,
-           "nonzero_fraction": float(np.mean(s != 0.0)),
-           "roofline": None, "cpu_baseline": cpu}
-    print(json.dumps(out), flush=True)
+           "host_array_call_pairs_per_s": nreq / t_call, "host_array_call_ms": 1e3 * t_call,
+           "nonzero_fraction": float(np.mean(s != 0.0)), "exact_zero_fraction": float(zero.mean()),
+           "roofline": {"bound": "hbm", "kernel": "k_eval_reqs (per contraction-rank bin)",
+                        "achieved": alg / t_eval / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": alg / t_eval / 1e9 / PEAK_HBM_GBS, "traffic": None, "algorithmic_bytes": alg,
+                        "bytes_meaning": "per request: the element bytes of its contraction-rank bin plus the "
+                                         "16-byte request; 80 bytes for exact zeros"},
+           "cpu_baseline": cpu}
+    eng.close()
+    return out
 
 
 def copy_workload(args):
@@ -509,44 +734,12 @@ def copy_workload(args):
            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": 1e3 * el / args.steps,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "copy of 256 MiB", "bytes_each_way": n * 8}}
-    print(json.dumps(out), flush=True)
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", type=int, default=0, help="0: reference arithmetic, 1: corrected m>0 path")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
-                    help="mixed: also run the mixed-precision build and report it when |dE| < 1 uHa")
-    ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--pt2-max-iter", type=int, default=300,
-                    help="pt2 workload: GMRES iteration limit of each PT2 solve")
-    ap.add_argument("--pt2-shift", type=float, default=2.0,
-                    help="pt2 workload: imaginary shift of the shifted PT2 solves")
-    ap.add_argument("--no-energy", action="store_true",
-                    help="skip the NOCI energies (eigensolver) -- for profiler runs; then |dE| is not checked")
-    ap.add_argument("--workload", choices=["wick", "pt2", "batch", "screen", "spawn", "copy"], default="wick",
-                    help="wick: batched H,S elements (headline); pt2: NOCI-PT2 matvec; batch: uneven "
-                         "multi-system batch; screen: learned SNOCI screening; copy: counter calibration")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU only: the engine has no multi-GPU path in this build "
-              "(sharding is described in DESIGN.md).", file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "pt2":
-        return pt2_workload(args)
-    if args.workload == "batch":
-        return batch_workload(args)
-    if args.workload == "screen":
-        return screen_workload(args)
-    if args.workload == "spawn":
-        return spawn_workload(args)
-    if args.workload == "copy":
-        return copy_workload(args)
-
+def wick_workload(args):
+    """The headline: batched Wick elements at n_bas = 100 (module docstring),
+    FP64 then the mixed forms; returns the result dict."""
     import numpy as np
     import torch
 
@@ -677,13 +870,51 @@ def main():
     else:
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                 "hbm_gbs": alg_bytes / (avg_ms * 1e-3) / 1e9}
+    # evaluation roofline: algorithmic bytes of every upper-triangle element by
+    # contraction-rank bin (runs of equal parent and ranks), over the measured
+    # evaluation time of the step
+    runs = []
+    for d in dets:
+        key = (d.parent, d.rank_a, d.rank_b)
+        if runs and runs[-1][0] == key:
+            runs[-1][1] += 1
+        else:
+            runs.append([key, 1])
+    eval_alg = eval_alg_m0 = 0.0
+    n_m0 = 0
+    for i, (ka, na) in enumerate(runs):
+        for j in range(i, len(runs)):
+            kb, nb = runs[j]
+            cnt = na * (na + 1) // 2 if i == j else na * nb
+            bts = cnt * eval_bytes(ka[1] + kb[1], ka[2] + kb[2])
+            eval_alg += bts
+            pm = res64["meta"][(ka[0], kb[0])]
+            if pm.ma == 0 and pm.mb == 0:
+                eval_alg_m0 += bts
+                n_m0 += cnt
+    eval_s = float(np.mean(evals)) * 1e-3
+    roof_eval = {"bound": "hbm", "kernel": "k_eval_blocks<HS, L_a, L_b, m0> (every contraction-rank bin)",
+                 "achieved": eval_alg / eval_s / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                 "frac": eval_alg / eval_s / 1e9 / PEAK_HBM_GBS, "traffic": None,
+                 "traffic_source": "profiles/eval_blocks_n100_hbm_traffic.txt",
+                 "algorithmic_bytes": eval_alg, "algorithmic_bytes_m0_pairs": eval_alg_m0, "pairs_m0": n_m0,
+                 "bytes_meaning": "per element: X/Y entries of both spins, hcol0, J and II gathers, H and S out, two "
+                                  "packed determinants (m = 0 formula; a lower bound for the m = 1 pairs, which "
+                                  "read the D1 blocks and more J slots as well)"}
+    tfile_e = os.path.join(ROOT, "profiles", "eval_blocks_n100_hbm_traffic.txt")
+    if os.path.exists(tfile_e):
+        for line in open(tfile_e):
+            if line.startswith("TRAFFIC_BYTES"):
+                roof_eval["traffic"] = float(line.split()[1])
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: oracle on pair (0,1) ...")
         cpu = cpu_baseline(sysm, dets, tol, args.mode)
         log(f"cpu baseline: {cpu['value']:.1f} pairs/s ({cpu['sample']})")
     out = {
-        "metric": "Wick H,S pair elements/s (end-to-end: per-reference-pair build + all upper-triangle elements)",
+        "metric": "Wick H,S pair elements/s: mixed vs FP64 at |\u0394E|<1\u00b5Ha; % roofline",
+        "metric_detail": "end to end: the per-reference-pair build of the 36 upper-triangle pairs + all "
+                         "upper-triangle H,S elements; value = the fastest mixed form within 1 uHa of FP64",
         "value": value,
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -720,10 +951,63 @@ def main():
                      "algorithmic_bytes_per_dispatch": alg_bytes,
                      "dispatches_per_step": dispatches, "tensors_per_dispatch": g, "avg_launch_ms": avg_ms,
                      "flops_per_dispatch": flops_disp}),
+        "roofline_eval": roof_eval,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     eng.close()
+    del Hd, Sd
+    return out
+
+
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", type=int, default=0, help="0: reference arithmetic, 1: corrected m>0 path")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
+                    help="mixed: also run the mixed-precision build and report it when |dE| < 1 uHa")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--pt2-max-iter", type=int, default=300,
+                    help="pt2 workload: GMRES iteration limit of each PT2 solve")
+    ap.add_argument("--pt2-shift", type=float, default=2.0,
+                    help="pt2 workload: imaginary shift of the shifted PT2 solves")
+    ap.add_argument("--no-energy", action="store_true",
+                    help="skip the NOCI energies (eigensolver) -- for profiler runs; then |dE| is not checked")
+    ap.add_argument("--workload", choices=["all", "wick", "pt2", "batch", "screen", "spawn", "copy"], default="all",
+                    help="all (default): the headline wick line carrying compact runs of the other workloads; "
+                         "wick: batched H,S elements (headline); pt2: NOCI-PT2 matvec; batch: uneven "
+                         "multi-system batch; screen: learned SNOCI screening; spawn: QMC spawning requests; "
+                         "copy: counter calibration")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the engine has no multi-GPU path in this build "
+              "(sharding is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    args.compact = False
+    one = {"wick": wick_workload, "pt2": pt2_workload, "batch": batch_workload, "screen": screen_workload,
+           "spawn": spawn_workload, "copy": copy_workload}
+    if args.workload != "all":
+        print(json.dumps(one[args.workload](args)), flush=True)
+        return
+    import torch
+    out = wick_workload(args)
+    torch.cuda.empty_cache()
+    # the other workloads, compact (steps and warm-ups as given, PT2 shifted solves only), each with its own
+    # roofline and CPU baseline, inside the same driver-timed run
+    args.compact = True
+    out["workloads"] = {}
+    for name in ("batch", "pt2", "screen", "spawn"):
+        t0 = time.perf_counter()
+        log(f"--- workload {name}")
+        r = one[name](args)
+        r["wall_s"] = time.perf_counter() - t0
+        out["workloads"][name] = r
+        torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
