@@ -131,6 +131,22 @@ int nw_fs_matrix(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double *F, doub
 int nw_fs_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t *req,
                   uint64_t nreq, double *f_out, double *s_out, int mode);
 
+/* Per-spin overlap factors of arbitrary determinant pairs (req as in
+ * nw_hs_batched): s_out[q] = the spin-`spin` (0 alpha, 1 beta) overlap of
+ * bra req[2q] and ket req[2q+1] with both excitation phases applied,
+ * ph_bra ph_ket phase tilde_s sum det D_mix -- xw_overlap_same_f64_batched
+ * (src/nonorthogonalwicks/eval/overlap.rs:136-352) over xw_overlap (:42-66)
+ * and prepare_same (eval/prepare.rs:27).  The full overlap of the pair is the
+ * product of the two spins' factors. */
+int nw_s_same_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t *req, uint64_t nreq, int spin,
+                      double *s_out);
+
+/* Shifted generalised-Fock elements of arbitrary determinant pairs:
+ * m_out[q] = F - e0 S of (req[2q], req[2q+1]), calculate_m_pair_wicks
+ * (src/noci/m.rs:160-197). */
+int nw_m_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t *req, uint64_t nreq, double e0,
+                 double *m_out, int mode);
+
 /* Matrix-free y = (F - e0 S) x over all pairs a <= b with the mirror
  * contribution (apply_candidate_m, src/snoci/operators.rs:375-488); if ys is
  * not NULL also ys = S x (apply_candidate_s).  x, y, ys: ndet doubles. */

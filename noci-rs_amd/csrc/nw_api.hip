@@ -623,7 +623,7 @@ long long pair_offsets(int ma, int mb, int nmo, long long *off)
 // Pack determinants on the device and return the host copy of the packed records.
 // The evaluation kernels cover contraction ranks (bra + ket excitations per
 // spin) up to four, the fixed-rank kernels of the reference.
-const char *const RANK_MSG = "contraction rank per spin (bra plus ket excitations) must be <= 4";
+const char *const RANK_MSG = "contraction rank per spin (bra plus ket excitations) must be <= 8";
 
 std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked, const void **draw_out)
 {
@@ -641,6 +641,14 @@ std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, Packed
     if (flag & 2) throw NwError(NW_E_ARG, "determinant parent out of range");
     if (flag & 1) throw NwError(NW_E_RANK, "excitation rank per spin must be <= 4 with equal holes and particles");
     return h;
+}
+
+// Bin key of a block: ((L_a * 5 + L_b) << 1) | m0 for contraction ranks up
+// to four per spin (the fixed-rank kernels), REQ_KEY_GEN + m0 above (the
+// run-time-rank kernels).
+int bin_key(int la, int lb, bool m0)
+{
+    return (la > 4 || lb > 4) ? REQ_KEY_GEN + (m0 ? 1 : 0) : ((la * 5 + lb) << 1) | (m0 ? 1 : 0);
 }
 
 // Upper-triangle pairs (i <= j) of a determinant list grouped into blocks of
@@ -666,9 +674,9 @@ std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vect
                 throw NwError(NW_E_STATE, "reference pair (" + std::to_string(A.parent) + "," + std::to_string(B.parent) +
                                               ") was not built");
             const int la = A.ra + B.ra, lb = A.rb + B.rb;
-            if (la > 4 || lb > 4) throw NwError(NW_E_RANK, RANK_MSG);
+            if (la > 8 || lb > 8) throw NwError(NW_E_RANK, RANK_MSG);
             const bool m0 = need_hcol && pi.ma == 0 && pi.mb == 0;
-            auto &v = bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)];
+            auto &v = bins[bin_key(la, lb, m0)];
             v.push_back({A.start, B.start, (int)A.len, (int)B.len, pair, r1 == r2 ? 1 : 0, 0});
         }
     for (auto &kv : bins) {
@@ -708,9 +716,9 @@ std::map<int, std::vector<EvalBlock>> rect_blocks(nw_ctx *c, const std::vector<P
                 throw NwError(NW_E_STATE, "reference pair (" + std::to_string(A.parent) + "," + std::to_string(B.parent) +
                                               ") was not built");
             const int la = A.ra + B.ra, lb = A.rb + B.rb;
-            if (la > 4 || lb > 4) throw NwError(NW_E_RANK, RANK_MSG);
+            if (la > 8 || lb > 8) throw NwError(NW_E_RANK, RANK_MSG);
             const bool m0 = need_hcol && pi.ma == 0 && pi.mb == 0;
-            bins[((la * 5 + lb) << 1) | (m0 ? 1 : 0)].push_back({A.start, B.start, (int)A.len, (int)B.len, pair, 0, 0});
+            bins[bin_key(la, lb, m0)].push_back({A.start, B.start, (int)A.len, (int)B.len, pair, 0, 0});
         }
     for (auto &kv : bins) {
         long long t = 0;
@@ -762,9 +770,11 @@ int eval_matrix(nw_ctx *c, int kind, const nw_det *dets, uint64_t ndet, double *
     Uploader up(c, 64u << 20);
     HIPCHK(hipEventRecord(c->ev[1], c->st));
     for (auto &kv : bins) {
-        const int key = kv.first, lab = key >> 1;
+        const int key = kv.first;
+        int la, lb;
+        key_ranks(key, la, lb);
         EvalBlock *db = up.put(kv.second);
-        launch_eval_blocks(c->st, kind, lab / 5, lab % 5, key & 1, db, (int)kv.second.size(), tiles_of(kv.second), args);
+        launch_eval_blocks(c->st, kind, la, lb, key & 1, db, (int)kv.second.size(), tiles_of(kv.second), args);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[2], c->st));
@@ -837,8 +847,9 @@ int eval_batched(nw_ctx *c, int kind, const nw_det *dets, uint64_t ndet, const u
     HIPCHK(hipEventRecord(c->ev[1], c->st));
     for (int k = 0; k < REQ_NKEY; ++k) {
         if (!hb[k]) continue;
-        const int lab = k >> 1;
-        launch_eval_reqs(c->st, kind, lab / 5, lab % 5, k & 1, rb.out + start[k], (int)hb[k], args);
+        int la, lb;
+        key_ranks(k, la, lb);
+        launch_eval_reqs(c->st, kind, la, lb, k & 1, rb.out + start[k], (int)hb[k], args);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[2], c->st));
@@ -875,6 +886,63 @@ int nw_fs_batched(nw_ctx *c, const nw_det *dets, uint64_t ndet, const uint64_t *
                   double *s_out, int mode)
 {
     return guarded(c, [&]() -> int { return eval_batched(c, EVAL_FS, dets, ndet, req, nreq, f_out, s_out, mode); });
+}
+
+int nw_s_same_batched(nw_ctx *c, const nw_det *dets, uint64_t ndet, const uint64_t *req, uint64_t nreq, int spin,
+                      double *s_out)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->built) return fail(c, NW_E_STATE, "nw_build first");
+        if (!dets || !req || !s_out || (spin != 0 && spin != 1)) return fail(c, NW_E_ARG, "bad arguments");
+        if (nreq == 0) return NW_OK;
+        if (nreq > (1ull << 31)) return fail(c, NW_E_SHAPE, "too many requests");
+        HIPCHK(hipEventRecord(c->ev[0], c->st));
+        PackedDet *dp = nullptr;
+        pack(c, dets, ndet, dp);
+        const uint64_t *dreq = (const uint64_t *)to_device(c, c->req_in, req, nreq * 16);
+        const bool odev = is_device_ptr(s_out);
+        double *dout = odev ? s_out : c->outS.get<double>(nreq);
+        int *derr = c->flag.get<int>(1);
+        HIPCHK(hipMemsetAsync(derr, 0, sizeof(int), c->st));
+        EvalArgs args = base_args(c, dp, 0);
+        HIPCHK(hipEventRecord(c->ev[1], c->st));
+        launch_s_same(c->st, dreq, (long long)nreq, (long long)ndet, spin, args, dout, derr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev[2], c->st));
+        int err = 0;
+        HIPCHK(hipMemcpyAsync(&err, derr, sizeof(int), hipMemcpyDeviceToHost, c->st));
+        if (!odev) HIPCHK(hipMemcpyAsync(s_out, dout, nreq * 8, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (err == REQ_ERR_INDEX) return fail(c, NW_E_ARG, "request index out of range");
+        if (err == REQ_ERR_UNBUILT) return fail(c, NW_E_STATE, "reference pair not built");
+        if (err == REQ_ERR_RANK) return fail(c, NW_E_RANK, RANK_MSG);
+        c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
+        c->t_ms[1] = 0.0;
+        c->t_ms[2] = elapsed(c->ev[1], c->ev[2]);
+        return NW_OK;
+    });
+}
+
+int nw_m_batched(nw_ctx *c, const nw_det *dets, uint64_t ndet, const uint64_t *req, uint64_t nreq, double e0,
+                 double *m_out, int mode)
+{
+    return guarded(c, [&]() -> int {
+        if (!m_out) return fail(c, NW_E_ARG, "null output");
+        if (nreq == 0) return NW_OK;
+        // F and S of every request into device temporaries, then M = F - e0 S
+        double *f = c->scratchA.get<double>(2 * nreq), *s = f + nreq;
+        int rc = eval_batched(c, EVAL_FS, dets, ndet, req, nreq, f, s, mode);
+        if (rc != NW_OK) return rc;
+        const bool mdev = is_device_ptr(m_out);
+        double *dm = mdev ? m_out : c->outH.get<double>(nreq);
+        std::vector<LinDesc> lc = {{f, s, nullptr, dm, 1.0, -e0, 0.0}};
+        Uploader up(c, 4096);
+        launch_lincomb(c->st, up.put(lc), 1, (long long)nreq);
+        HIPCHK(hipGetLastError());
+        if (!mdev) HIPCHK(hipMemcpyAsync(m_out, dm, nreq * 8, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        return NW_OK;
+    });
 }
 
 int nw_update_fock(nw_ctx *c, const double *fa, const double *fb)
@@ -963,9 +1031,10 @@ int nw_matvec_m(nw_ctx *c, const nw_det *dets, uint64_t ndet, double e0, const d
         args.ys = dys;
         HIPCHK(hipEventRecord(c->ev[1], c->st));
         for (auto &kv : bins) {
-            const int lab = kv.first >> 1;
+            int la, lb;
+            key_ranks(kv.first, la, lb);
             EvalBlock *db = up.put(kv.second);
-            launch_eval_blocks(c->st, EVAL_MATVEC, lab / 5, lab % 5, false, db, (int)kv.second.size(),
+            launch_eval_blocks(c->st, EVAL_MATVEC, la, lb, false, db, (int)kv.second.size(),
                                tiles_of(kv.second), args);
         }
         HIPCHK(hipGetLastError());
@@ -1006,9 +1075,11 @@ int nw_project_hs(nw_ctx *c, const nw_det *dets, uint64_t ndet, uint64_t ncur, c
         args.ldo = (long long)ncur;  // row index offset of the outputs
         HIPCHK(hipEventRecord(c->ev[1], c->st));
         for (auto &kv : bins) {
-            const int key = kv.first, lab = key >> 1;
+            const int key = kv.first;
+            int la, lb;
+            key_ranks(key, la, lb);
             EvalBlock *db = up.put(kv.second);
-            launch_eval_blocks(c->st, EVAL_PROJECT, lab / 5, lab % 5, key & 1, db, (int)kv.second.size(),
+            launch_eval_blocks(c->st, EVAL_PROJECT, la, lb, key & 1, db, (int)kv.second.size(),
                                tiles_of(kv.second), args);
         }
         HIPCHK(hipGetLastError());

@@ -398,6 +398,9 @@ int nw_batch_add(nw_batch *b, nw_ctx *ctx, const nw_det *dets, uint64_t ndet, in
         PackedDet *dp = nullptr;
         std::vector<PackedDet> h = pack(ctx, dets, ndet, dp);
         S.bins = triangle_blocks(ctx, h, true);
+        for (auto &kv : S.bins)
+            if (kv.first >= REQ_KEY_GEN)
+                throw NwError(NW_E_RANK, "multi-system batches take contraction ranks up to 4 per spin");
         S.dets = dalloc<PackedDet>(ndet);
         HIPCHK(hipMemcpyAsync(S.dets, dp, ndet * sizeof(PackedDet), hipMemcpyDeviceToDevice, ctx->st));
         HIPCHK(hipStreamSynchronize(ctx->st));

@@ -10,7 +10,7 @@
 #include <algorithm>
 #include <cstdlib>
 
-#include "nw_eval_dev.h"
+#include "nw_eval_gen.h"
 
 namespace nw {
 
@@ -88,7 +88,7 @@ __device__ inline int find_block(const EvalBlock *__restrict__ blocks, int nb, l
 // One 16 x 16 tile t of the bin's blocks.  The tile loop of k_eval_blocks is
 // workgroup-uniform, so the barrier of the matvec branch is reached by every
 // lane of the workgroup on every tile.
-template <int KIND, int LA, int LB, bool M0>
+template <int KIND, int LA, int LB, bool M0, bool GEN>
 __device__ inline void eval_tile(const EvalBlock *__restrict__ blocks, int nblocks, const EvalArgs &a, long long t)
 {
     const EvalBlock B = blocks[find_block(blocks, nblocks, t)];
@@ -106,7 +106,15 @@ __device__ inline void eval_tile(const EvalBlock *__restrict__ blocks, int nbloc
     double v0 = 0.0, v1 = 0.0;
     if (valid) {
         const PackedDet bra = a.dets[i], ket = a.dets[j];
-        if constexpr (KIND == EVAL_HS || KIND == EVAL_PROJECT) {
+        if constexpr (GEN) {
+            // contraction rank above four on a spin: the rank is the lane's own
+            if constexpr (KIND == EVAL_HS || KIND == EVAL_PROJECT) {
+                if constexpr (M0) hs_m0_rt(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);
+                else hs_gen_rt(P, slab, bra, ket, a.nmo, a.enuc, a.tol, a.mode, v0, v1);
+            } else {
+                fs_elem_rt(P, slab, bra, ket, a.nmo, a.tol, a.mode, v0, v1);
+            }
+        } else if constexpr (KIND == EVAL_HS || KIND == EVAL_PROJECT) {
             if constexpr (M0) hs_m0<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);
             else hs_gen<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, a.tol, a.mode, v0, v1);
         } else {
@@ -179,7 +187,7 @@ __device__ inline void eval_tile(const EvalBlock *__restrict__ blocks, int nbloc
 // the runtime's limit of 2^32 - 1 work-items.
 constexpr int EVAL_XCD = 8;
 
-template <int KIND, int LA, int LB, bool M0>
+template <int KIND, int LA, int LB, bool M0, bool GEN = false>
 __global__ __launch_bounds__(256) void k_eval_blocks(const EvalBlock *__restrict__ blocks, int nblocks, long long ntiles,
                                                      EvalArgs a)
 {
@@ -187,10 +195,10 @@ __global__ __launch_bounds__(256) void k_eval_blocks(const EvalBlock *__restrict
     const int xcd = blockIdx.x % EVAL_XCD;
     const long long lo = xcd * chunk, hi = min(ntiles, lo + chunk);
     const long long step = gridDim.x / EVAL_XCD;
-    for (long long t = lo + blockIdx.x / EVAL_XCD; t < hi; t += step) eval_tile<KIND, LA, LB, M0>(blocks, nblocks, a, t);
+    for (long long t = lo + blockIdx.x / EVAL_XCD; t < hi; t += step) eval_tile<KIND, LA, LB, M0, GEN>(blocks, nblocks, a, t);
 }
 
-template <int KIND, int LA, int LB, bool M0>
+template <int KIND, int LA, int LB, bool M0, bool GEN = false>
 __global__ __launch_bounds__(256) void k_eval_reqs(const EvalReq *__restrict__ reqs, int nreq, EvalArgs a)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,7 +208,14 @@ __global__ __launch_bounds__(256) void k_eval_reqs(const EvalReq *__restrict__ r
     const PairInfo &P = a.pairs[(int)bra.parent * a.nref + ket.parent];
     const slab_t slab = (slab_t)(a.slab + P.base);
     double v0, v1;
-    if constexpr (KIND == EVAL_HS) {
+    if constexpr (GEN) {
+        if constexpr (KIND == EVAL_HS) {
+            if constexpr (M0) hs_m0_rt(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);
+            else hs_gen_rt(P, slab, bra, ket, a.nmo, a.enuc, a.tol, a.mode, v0, v1);
+        } else {
+            fs_elem_rt(P, slab, bra, ket, a.nmo, a.tol, a.mode, v0, v1);
+        }
+    } else if constexpr (KIND == EVAL_HS) {
         if constexpr (M0) hs_m0<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);
         else hs_gen<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, a.tol, a.mode, v0, v1);
     } else {
@@ -224,6 +239,24 @@ void launch_eval_blocks(hipStream_t st, int kind, int la, int lb, bool m0, const
                         long long ntiles, const EvalArgs &args)
 {
     if (ntiles <= 0) return;
+    if (la > 4 || lb > 4) {
+        // generic bin: every pair with a contraction rank above four on a spin
+        const long long chunk = (ntiles + EVAL_XCD - 1) / EVAL_XCD;
+        const long long per_xcd = std::max(1ll, std::min(chunk, eval_grid_cap() / EVAL_XCD));
+        dim3 grid((unsigned)(per_xcd * EVAL_XCD)), block(256);
+        if (kind == EVAL_HS) {
+            if (m0) hipLaunchKernelGGL((k_eval_blocks<EVAL_HS, 0, 0, true, true>), grid, block, 0, st, blocks, nblocks, ntiles, args);
+            else hipLaunchKernelGGL((k_eval_blocks<EVAL_HS, 0, 0, false, true>), grid, block, 0, st, blocks, nblocks, ntiles, args);
+        } else if (kind == EVAL_FS) {
+            hipLaunchKernelGGL((k_eval_blocks<EVAL_FS, 0, 0, false, true>), grid, block, 0, st, blocks, nblocks, ntiles, args);
+        } else if (kind == EVAL_PROJECT) {
+            if (m0) hipLaunchKernelGGL((k_eval_blocks<EVAL_PROJECT, 0, 0, true, true>), grid, block, 0, st, blocks, nblocks, ntiles, args);
+            else hipLaunchKernelGGL((k_eval_blocks<EVAL_PROJECT, 0, 0, false, true>), grid, block, 0, st, blocks, nblocks, ntiles, args);
+        } else {
+            hipLaunchKernelGGL((k_eval_blocks<EVAL_MATVEC, 0, 0, false, true>), grid, block, 0, st, blocks, nblocks, ntiles, args);
+        }
+        return;
+    }
     // a multiple of the XCD count: every XCD gets the same number of workgroups
     const long long chunk = (ntiles + EVAL_XCD - 1) / EVAL_XCD;
     const long long per_xcd = std::max(1ll, std::min(chunk, eval_grid_cap() / EVAL_XCD));
@@ -283,9 +316,10 @@ __global__ __launch_bounds__(256) void k_req_key(ReqBin b)
             } else {
                 const PairInfo &pi = b.pairs[(int)da.parent * b.nref + (int)db.parent];
                 const int la = da.ra + db.ra, lb = da.rb + db.rb;
+                const int m0 = (b.hs && pi.ma == 0 && pi.mb == 0) ? 1 : 0;
                 if (!pi.built) atomicMax(b.err, REQ_ERR_UNBUILT);
-                else if (la > 4 || lb > 4) atomicMax(b.err, REQ_ERR_RANK);
-                else key = ((la * 5 + lb) << 1) | ((b.hs && pi.ma == 0 && pi.mb == 0) ? 1 : 0);
+                else if (la > 8 || lb > 8) atomicMax(b.err, REQ_ERR_RANK);
+                else key = (la > 4 || lb > 4) ? REQ_KEY_GEN + m0 : ((la * 5 + lb) << 1) | m0;
             }
         }
         b.key[q] = key;
@@ -324,11 +358,95 @@ void launch_req_bin(hipStream_t st, const ReqBin &b, bool scatter)
     else hipLaunchKernelGGL(k_req_key, grid, block, 0, st, b);
 }
 
+// ---------------------------------------------------------------------------
+// Per-spin overlap factors of requested determinant pairs
+// (xw_overlap_same_f64_batched, eval/overlap.rs:136-352, over xw_overlap
+// :42-66): ph_bra ph_ket phase tilde_s sum over the m-bit column
+// distributions of det D_mix, zero when m > L.
+// ---------------------------------------------------------------------------
+template <int L>
+__device__ inline double s_spin(slab_t slab, const long long *o, const SpinScal &sc, int n, const uint8_t *bi,
+                                int brank, const uint8_t *ki)
+{
+    constexpr int L1 = L > 0 ? L : 1;
+    if (sc.m > L) return 0.0;
+    int rows[L1], cols[L1];
+    double d0[L1 * L1], d1[L1 * L1];
+    if constexpr (L > 0) {
+        labels<L>(bi, brank, ki, sc.nocc, n - sc.nocc, rows, cols);
+        build_d<L>(slab + o[O_X], slab + o[O_Y], n, rows, cols, d0);
+        if (sc.m > 0) build_d<L>(slab + o[O_X + 1], slab + o[O_Y + 1], n, rows, cols, d1);
+    }
+    if (sc.m == 0) return det_n<L>(d0);
+    double acc = 0.0;
+    for (unsigned bits = 0; bits < (1u << L); ++bits) {
+        if (__popc(bits) != sc.m) continue;
+        double mix[L1 * L1];
+#pragma unroll
+        for (int r = 0; r < L; ++r)
+#pragma unroll
+            for (int c = 0; c < L; ++c) mix[r * L + c] = bitk(bits, c) ? d1[r * L + c] : d0[r * L + c];
+        acc += det_n<L>(mix);
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void k_s_same(const uint64_t *__restrict__ req, long long nreq, long long ndet,
+                                                int spin, EvalArgs a, double *__restrict__ out, int *err)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nreq) return;
+    const uint64_t ia = req[2 * q], ib = req[2 * q + 1];
+    if (ia >= (uint64_t)ndet || ib >= (uint64_t)ndet) {
+        atomicMax(err, REQ_ERR_INDEX);
+        return;
+    }
+    const PackedDet bra = a.dets[ia], ket = a.dets[ib];
+    const PairInfo &P = a.pairs[(int)bra.parent * a.nref + ket.parent];
+    if (!P.built) {
+        atomicMax(err, REQ_ERR_UNBUILT);
+        return;
+    }
+    const int rb = spin ? bra.rb : bra.ra, rk = spin ? ket.rb : ket.ra;
+    const uint8_t *bi = spin ? bra.ib : bra.ia, *ki = spin ? ket.ib : ket.ia;
+    const SpinScal sc = load_spin(P, spin);
+    const long long *o = P.off + (spin ? O_BB : O_AA);
+    const slab_t slab = (slab_t)(a.slab + P.base);
+    double v = 0.0;
+    switch (rb + rk) {
+    case 0: v = s_spin<0>(slab, o, sc, a.nmo, bi, rb, ki); break;
+    case 1: v = s_spin<1>(slab, o, sc, a.nmo, bi, rb, ki); break;
+    case 2: v = s_spin<2>(slab, o, sc, a.nmo, bi, rb, ki); break;
+    case 3: v = s_spin<3>(slab, o, sc, a.nmo, bi, rb, ki); break;
+    case 4: v = s_spin<4>(slab, o, sc, a.nmo, bi, rb, ki); break;
+    default: v = s_spin_rt(slab, o, sc, a.nmo, bi, rb, ki, rk); break;
+    }
+    const double ph = spin ? (double)(bra.phb * ket.phb) : (double)(bra.pha * ket.pha);
+    out[q] = ph * sc.pref * v;
+}
+
+void launch_s_same(hipStream_t st, const uint64_t *req, long long nreq, long long ndet, int spin,
+                   const EvalArgs &args, double *out, int *err)
+{
+    if (nreq <= 0) return;
+    hipLaunchKernelGGL(k_s_same, dim3((unsigned)((nreq + 255) / 256)), dim3(256), 0, st, req, nreq, ndet, spin, args,
+                       out, err);
+}
+
 void launch_eval_reqs(hipStream_t st, int kind, int la, int lb, bool m0, const EvalReq *reqs, int nreq,
                       const EvalArgs &args)
 {
     if (nreq <= 0) return;
     dim3 grid((nreq + 255) / 256), block(256);
+    if (la > 4 || lb > 4) {
+        if (kind == EVAL_HS) {
+            if (m0) hipLaunchKernelGGL((k_eval_reqs<EVAL_HS, 0, 0, true, true>), grid, block, 0, st, reqs, nreq, args);
+            else hipLaunchKernelGGL((k_eval_reqs<EVAL_HS, 0, 0, false, true>), grid, block, 0, st, reqs, nreq, args);
+        } else {
+            hipLaunchKernelGGL((k_eval_reqs<EVAL_FS, 0, 0, false, true>), grid, block, 0, st, reqs, nreq, args);
+        }
+        return;
+    }
 #define NW_R(A, B)                                                                                             \
     if (la == A && lb == B) {                                                                                  \
         if (kind == EVAL_HS) {                                                                                 \

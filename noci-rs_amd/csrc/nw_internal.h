@@ -223,6 +223,7 @@ struct Run { long long start, len; int parent, ra, rb; };
 long long pair_offsets(int ma, int mb, int nmo, long long *off);
 std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked,
                             const void **draw = nullptr);
+int bin_key(int la, int lb, bool m0);
 std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vector<PackedDet> &h, bool need_hcol);
 long long tiles_of(const std::vector<EvalBlock> &v);
 std::map<int, std::vector<EvalBlock>> rect_blocks(nw_ctx *c, const std::vector<PackedDet> &h, long long nrow0,

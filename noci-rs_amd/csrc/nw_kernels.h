@@ -89,7 +89,18 @@ void launch_pack_dets(hipStream_t st, const void *dets, long long ndet, int nref
 // GPU binning of batched requests (nw_hs_batched / nw_fs_batched): k_req_key
 // writes each request's bin key (-1: exact zero or error) and the per-key
 // counts; k_req_scatter places the requests of every key at start[key].
-constexpr int REQ_NKEY = 50;  // ((L_a * 5 + L_b) << 1) | m0 with L_a, L_b <= 4
+// ((L_a * 5 + L_b) << 1) | m0 with L_a, L_b <= 4; REQ_KEY_GEN + m0 for the
+// pairs with a contraction rank above four (up to eight) on a spin
+constexpr int REQ_KEY_GEN = 50;
+constexpr int REQ_NKEY = 52;
+// the same bin of a rank-4-or-less key, or the generic bin's (L_a, L_b)
+// stand-in (5, 0) that routes launch_eval_reqs / launch_eval_blocks to the
+// run-time-rank kernels
+__host__ __device__ inline void key_ranks(int key, int &la, int &lb)
+{
+    if (key >= REQ_KEY_GEN) { la = 5; lb = 0; }
+    else { la = (key >> 1) / 5; lb = (key >> 1) % 5; }
+}
 enum { REQ_ERR_INDEX = 1, REQ_ERR_UNBUILT = 2, REQ_ERR_RANK = 3 };
 struct ReqBin {
     const uint64_t *req;  // nreq (a, b) index pairs
@@ -114,6 +125,10 @@ void launch_eval_blocks(hipStream_t st, int kind, int la, int lb, bool m0, const
                         long long ntiles, const EvalArgs &args);
 void launch_eval_reqs(hipStream_t st, int kind, int la, int lb, bool m0, const EvalReq *reqs, int nreq,
                       const EvalArgs &args);
+// Per-spin overlap factor of each request (nw_s_same_batched); err takes the
+// REQ_ERR_* codes (atomicMax).
+void launch_s_same(hipStream_t st, const uint64_t *req, long long nreq, long long ndet, int spin,
+                   const EvalArgs &args, double *out, int *err);
 
 // Multi-system fused evaluation (nw_batch_*).  Per-system addressing and one
 // work item per tile; see k_eval_fused.

@@ -113,7 +113,7 @@ extern "C" int nw_hs_batched_noalloc(nw_ctx *c, const nw_det *dets, uint64_t nde
         if (hb[REQ_NKEY] == REQ_ERR_INDEX) throw NwError(NW_E_ARG, "request index out of range");
         if (hb[REQ_NKEY] == REQ_ERR_UNBUILT) throw NwError(NW_E_STATE, "reference pair not built");
         if (hb[REQ_NKEY] == REQ_ERR_RANK)
-            throw NwError(NW_E_RANK, "contraction rank per spin (bra plus ket excitations) must be <= 4");
+            throw NwError(NW_E_RANK, "contraction rank per spin (bra plus ket excitations) must be <= 8");
         unsigned start[REQ_NKEY];
         unsigned tot = 0;
         for (int k = 0; k < REQ_NKEY; ++k) {
@@ -129,8 +129,9 @@ extern "C" int nw_hs_batched_noalloc(nw_ctx *c, const nw_det *dets, uint64_t nde
         HIPCHK(hipEventRecord(c->ev[1], c->st));
         for (int k = 0; k < REQ_NKEY; ++k) {
             if (!hb[k]) continue;
-            const int lab = k >> 1;
-            launch_eval_reqs(c->st, EVAL_HS, lab / 5, lab % 5, k & 1, rb.out + start[k], (int)hb[k], args);
+            int la, lb;
+            key_ranks(k, la, lb);
+            launch_eval_reqs(c->st, EVAL_HS, la, lb, k & 1, rb.out + start[k], (int)hb[k], args);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c->ev[2], c->st));

@@ -37,7 +37,7 @@ EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matri
            "nw_mlp_get_params", "nw_mlp_forward", "nw_mlp_grad", "nw_mlp_step", "nw_mlp_train", "nw_mlp_last_error",
            "nw_mlp_destroy", "nw_factor_create", "nw_factor_refresh", "nw_factor_apply", "nw_factor_diagonals",
            "nw_factor_info", "nw_factor_ids", "nw_factor_table", "nw_factor_last_ms", "nw_factor_destroy",
-           "nw_factor_set_precision", "nw_factor_apply_pair", "nw_factor_apply_bytes", "nw_hs_batched_noalloc"]
+           "nw_factor_set_precision", "nw_factor_apply_pair", "nw_factor_apply_bytes", "nw_hs_batched_noalloc", "nw_s_same_batched", "nw_m_batched"]
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_BF16X3 = 2
@@ -86,6 +86,8 @@ def load():
     L.nw_build.argtypes = [vp, ct.c_double, vp, vp]
     L.nw_hs_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, vp, vp, ct.c_int]
     L.nw_hs_batched_noalloc.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, vp, vp, ct.c_int]
+    L.nw_s_same_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, ct.c_int, vp]
+    L.nw_m_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, ct.c_double, vp, ct.c_int]
     L.nw_hs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
     L.nw_pair_slab.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
     L.nw_set_precision.argtypes = [vp, ct.c_int]
@@ -244,6 +246,24 @@ class Engine:
         fn = self.L.nw_hs_batched_noalloc if noalloc else self.L.nw_hs_batched
         self._check(fn(self.ctx, _ptr(dets), ndet, _ptr(req), nreq, _ptr(h), _ptr(s), int(mode)))
         return h, s
+
+    def s_same_batched(self, dets, req, spin):
+        """Per-spin overlap factors (excitation phases applied) of request pairs."""
+        req = np.ascontiguousarray(req, dtype=np.uint64).reshape(-1, 2)
+        s = np.empty(len(req))
+        sync_torch(dets)
+        self._check(self.L.nw_s_same_batched(self.ctx, _ptr(dets), len(dets), _ptr(req), len(req), int(spin),
+                                             _ptr(s)))
+        return s
+
+    def m_batched(self, dets, req, e0, mode=MODE_REFERENCE):
+        """M = F - e0 S of request pairs (calculate_m_pair_wicks)."""
+        req = np.ascontiguousarray(req, dtype=np.uint64).reshape(-1, 2)
+        m = np.empty(len(req))
+        sync_torch(dets)
+        self._check(self.L.nw_m_batched(self.ctx, _ptr(dets), len(dets), _ptr(req), len(req), float(e0), _ptr(m),
+                                        int(mode)))
+        return m
 
     def update_fock(self, fa, fb):
         """update_wicks_fock: generalised-Fock intermediates from spin Fock matrices."""

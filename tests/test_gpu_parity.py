@@ -115,24 +115,35 @@ def test_pair_mask_and_unbuilt_pair_error():
 
 def test_batched_request_errors_and_rank_limit():
     """Batched requests are binned on the GPU: an index past the determinant
-    list, and a pair whose contraction rank on one spin exceeds four (an alpha
-    triple against an alpha double), are errors rather than unwritten outputs;
-    the full-matrix path refuses the second too.  Valid requests after an
-    error still evaluate, and match the full matrix."""
-    sysm, dets = _case(10, 3, 3, 2, seed=61, per_ref=6)
+    list and a determinant with an excitation rank above four on a spin (a
+    quintuple) are errors rather than unwritten outputs, in the batched and
+    full-matrix paths; a pair of contraction rank five on one spin (an alpha
+    triple against an alpha double) is evaluated by the run-time-rank
+    kernels and matches the oracle.  Valid requests after an error still
+    evaluate, and match the full matrix."""
+    sysm, dets = _case(12, 5, 3, 2, seed=61, per_ref=6)
     eng = nwick.Engine.from_system(sysm)
     eng.build()
     rec = basis.to_records(dets)
     with pytest.raises(nwick.NwickError):
         eng.hs_batched(rec, np.array([[0, len(dets)]], dtype=np.uint64))
-    triple = basis.Det(1, (1 << 3) - 1, (1 << 3) - 1, (0, 1, 2), (3, 4, 5))
-    double = basis.Det(0, (1 << 3) - 1, (1 << 3) - 1, (0, 1), (6, 7))
-    rec3 = basis.to_records(dets + [triple, double])
+    oa, ob = (1 << 5) - 1, (1 << 3) - 1
+    triple = basis.Det(1, oa, ob, (0, 1, 2), (5, 6, 7))
+    double = basis.Det(0, oa, ob, (0, 1), (8, 9))
+    quint = basis.Det(0, oa, ob, (0, 1, 2, 3, 4), (5, 6, 7, 8, 9))
     n = len(dets)
+    rec3 = basis.to_records(dets + [triple, double])
+    h, s = eng.hs_batched(rec3, np.array([[n + 1, n]], dtype=np.uint64))
+    slab = O.Slab(sysm)
+    for mode in (0, 1):
+        h, s = eng.hs_batched(rec3, np.array([[n + 1, n]], dtype=np.uint64), mode=mode)
+        ho, so = slab.hs(double, triple, mode)
+        assert abs(h[0] - ho) < TOL and abs(s[0] - so) < TOL
+    rec5 = basis.to_records(dets + [quint])
     with pytest.raises(nwick.NwickError):
-        eng.hs_batched(rec3, np.array([[n + 1, n]], dtype=np.uint64))
+        eng.hs_batched(rec5, np.array([[n, 0]], dtype=np.uint64))
     with pytest.raises(nwick.NwickError):
-        eng.hs_matrix(rec3)
+        eng.hs_matrix(rec5)
     req = np.array([[a, b] for a in range(n) for b in range(n)], dtype=np.uint64)
     h, s = eng.hs_batched(rec, req)
     H, S = eng.hs_matrix(rec)

@@ -50,6 +50,32 @@ def test_fs_batched_vs_oracle(pt2_case):
         assert abs(f[q] - fo) < TOL and abs(s[q] - so) < TOL
 
 
+def test_s_same_and_m_batched_vs_oracle(pt2_case):
+    """Per-spin overlap factors (excitation phases applied) against the
+    oracle's xw_overlap factor of each spin, their product against the pair
+    overlap, and M = F - e0 S per request against the oracle's F and S, in
+    both modes (m = 1 reference pairs included)."""
+    sysm, dets, eng, slab, fa, fb = pt2_case
+    rec = basis.to_records(dets)
+    rng = np.random.default_rng(11)
+    req = rng.integers(0, len(dets), size=(1500, 2)).astype(np.uint64)
+    sa = eng.s_same_batched(rec, req, 0)
+    sb = eng.s_same_batched(rec, req, 1)
+    e0 = -2.75
+    for mode in (0, 1):
+        m = eng.m_batched(rec, req, e0, mode=mode)
+        for q in range(0, len(req), 3):
+            a, b = dets[int(req[q, 0])], dets[int(req[q, 1])]
+            soa, _ = slab.fs_spin(a, b, 0, mode)
+            sob, _ = slab.fs_spin(a, b, 1, mode)
+            fo, so = slab.fs(a, b, mode)
+            assert abs(sa[q] - soa) < TOL and abs(sb[q] - sob) < TOL
+            assert abs(sa[q] * sb[q] - so) < TOL
+            assert abs(m[q] - (fo - e0 * so)) < TOL
+    with pytest.raises(nwick.NwickError):
+        eng.s_same_batched(rec, np.array([[0, len(dets)]], dtype=np.uint64), 0)
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_matvec_vs_dense(pt2_case, mode):
     sysm, dets, eng, slab, fa, fb = pt2_case

@@ -177,3 +177,41 @@ def test_fock_elements_vs_generalised_slater_condon():
                 assert abs(f - fn) < 1e-10 and abs(s - sn) < 1e-10
                 if slab.m[(dets[i].parent, dets[j].parent)] == (0, 0):
                     assert slab.fs(dets[i], dets[j], 0) == (f, s)
+
+
+@pytest.mark.parametrize("orth_count", [0, 1])
+def test_high_rank_vs_bruteforce(orth_count):
+    """Triple excitations (contraction ranks up to 6 per spin): the corrected
+    element against brute force and the generalised Slater-Condon route on
+    m = 0 and m = 1 reference pairs, and the reference-arithmetic element
+    against the numpy restatement of the Rust text, which handles any rank."""
+    import m_readings as R
+    sysm = synth.make_system(7, 3, 3, 2, seed=17 + orth_count, orth_refs=(1,) if orth_count else (),
+                             orth_count=max(orth_count, 1), orthonormal_ao=False, n_aux=14, enuc=0.8)
+    bf = BruteForce(sysm.S, sysm.h, sysm.eri, 3, 3)
+    slab = O.Slab(sysm, tol=1e-8)
+    assert slab.m[(0, 1)] == (orth_count, 0)
+    dets = basis.excited_basis(2, 7, 3, 3, per_ref=0, seed=4, include_refs=True,
+                               counts={"aaa": 3, "aab": 3, "abb": 2, "bbb": 2, "aa": 2, "b": 1})
+    ea = sysm.eri_asym()
+    vec = {}
+    seen_l = set()
+    for i in range(len(dets)):
+        for j in range(len(dets)):
+            a, b = dets[i], dets[j]
+            if a.parent != 0 or b.parent != 1:
+                continue
+            for k, d in ((i, a), (j, b)):
+                if k not in vec:
+                    vec[k] = bf.vector(*sysm.refs[d.parent], _cols(d.oa), _cols(d.ob))
+            hb, sb = bf.hs(vec[i], vec[j])
+            h1, s1 = slab.hs(a, b, 1)
+            assert abs(h1 - (hb + sysm.enuc * sb)) < 1e-10 and abs(s1 - sb) < 1e-10
+            hn, sn = O.hs_naive(sysm, a, b, eri_asym=ea)
+            assert abs(hn - (hb + sysm.enuc * sb)) < 1e-10
+            seen_l.add(max(a.rank_a + b.rank_a, a.rank_b + b.rank_b))
+            if orth_count:
+                h0, s0 = slab.hs(a, b, 0)
+                hr, sr = R.element(slab.pair_view(0, 1), 7, a, b, O.RD_ALL, sysm.enuc, slab.tol)
+                assert abs(h0 - hr) < 1e-10 * max(1.0, abs(hr)) and abs(s0 - sr) < 1e-12
+    assert max(seen_l) == 6
