@@ -35,6 +35,10 @@ BOHR = 0.52917721092  # Angstrom per Bohr (PySCF's value)
 BASIS_SETS = {
     "sto-3g": {
         "H": [(0, [(3.42525091, 0.15432897), (0.62391373, 0.53532814), (0.16885540, 0.44463454)])],
+        # Hehre, Stewart, Pople, J. Chem. Phys. 51, 2657 (1969): 1s and the shared-exponent 2sp shell
+        "F": [(0, [(166.6791300, 0.15432897), (30.3608120, 0.53532814), (8.2168207, 0.44463454)]),
+              (0, [(6.4648032, -0.09996723), (1.5022812, 0.39951283), (0.4885885, 0.70011547)]),
+              (1, [(6.4648032, 0.15591627), (1.5022812, 0.60768372), (0.4885885, 0.39195739)])],
     },
     "3-21g": {
         "H": [(0, [(5.4471780, 0.1562850), (0.8245470, 0.9046910)]),
@@ -43,6 +47,13 @@ BASIS_SETS = {
     "6-31g": {
         "H": [(0, [(18.7311370, 0.03349460), (2.8253937, 0.23472695), (0.6401217, 0.81375733)]),
               (0, [(0.1612778, 1.0)])],
+        # Dill and Pople, J. Chem. Phys. 62, 2921 (1975): 1s, 2sp (shared exponents), 3sp
+        "Li": [(0, [(642.4189150, 0.0021426), (96.7985150, 0.0162089), (22.0911210, 0.0773156),
+                    (6.2010703, 0.2457860), (1.9351177, 0.4701890), (0.6367358, 0.3454708)]),
+               (0, [(2.3249184, -0.0350917), (0.6324306, -0.1912328), (0.0790534, 1.0839878)]),
+               (1, [(2.3249184, 0.0089415), (0.6324306, 0.1410095), (0.0790534, 0.9453637)]),
+               (0, [(0.0359620, 1.0)]),
+               (1, [(0.0359620, 1.0)])],
     },
     "cc-pvdz": {
         "H": [(0, [(13.01, 0.019685), (1.962, 0.137977), (0.4446, 0.478148), (0.122, 0.50124)]),

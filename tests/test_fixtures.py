@@ -16,7 +16,7 @@ from nwick.molecule import Molecule, integrals
 from pt2_dense import DensePT2Backend, dense_pt2
 
 NOCI_CASES = ["REF_NOCI_H2_STO-3G_1_5", "PT2_H2_3-21G_1_5_WICKS", "SNOCI_H4_STO-3G_1_5_WICKS",
-              "REF_NOCI_H4_STO-3G_1_75"]
+              "REF_NOCI_H4_STO-3G_1_75", "REF_NOCI_F2_STO-3G_1_75_WICKS", "REF_NOCI_LiH_6-31G_2_8_WICKS"]
 
 
 def test_h2_sto3g_integrals_textbook():
@@ -95,3 +95,30 @@ def test_snoci_energy(name):
     res = SN.snoci(DensePT2Backend(sysm, SCF_TOL), snoci_refs(st, sysm), sysm.nmo, snoci_options(fx))
     print(f"{name}: E_SNOCI {res.ecurrent:.12f} (reference {fx['snoci_energy']:.12f}), ncur={len(res.current)}")
     assert close(res.ecurrent, fx["snoci_energy"])
+
+
+@pytest.mark.parametrize("name", ["DET_H2_STO-3G_1_5_WICKS", "DET_F2_sto-3g_1_75_WICKS"])
+def test_deterministic_energy(name):
+    """Deterministic NOCI-QMC (tests/deterministic.rs:91-160): propagation in
+    the references' singles/doubles space converges to the lowest root of the
+    (H, S) pencil over that space, which the general eigensolver gives
+    directly.  F2 / STO-3G: 300 determinants (every reference's singles and
+    doubles fill the 100-determinant space), at the reference's 1e-7."""
+    fx, ao, st, sysm = system(name)
+    for g, w in zip(sorted(s.e for s in st), sorted(fx["scf_energies"])):
+        assert close(g, w), (g, w)
+    dets = candidates(fx, sysm)
+    dets = [d for r in range(sysm.nref) for d in ([ref_dets(sysm)[r]] + [x for x in dets if x.parent == r])]
+    slab = O.Slab(sysm, SCF_TOL)
+    H, S = O.hs_matrix(slab, dets, mode=0)
+    e = noci_energy(H, S)
+    print(f"{name}: {len(dets)} dets, E_det {e:.12f} (reference {fx['deterministic_noci_energy']:.12f})")
+    assert abs(e - fx["deterministic_noci_energy"]) < fx["tolerance"]
+
+
+def test_h2_321g_scan_last_geometry_scf():
+    """REF_NOCI_H2_3-21G_1_0_WICKS at its last geometry (1.00 Angstrom): the
+    RHF and both spin-biased UHF states collapse to one energy."""
+    fx, ao, st, sysm = system("REF_NOCI_H2_3-21G_1_0_WICKS")
+    for g, w in zip(sorted(s.e for s in st), sorted(fx["final_scf_energies"])):
+        assert close(g, w), (g, w)

@@ -28,7 +28,8 @@ def _engine(sysm):
 
 
 @pytest.mark.parametrize("name", ["REF_NOCI_H2_STO-3G_1_5", "PT2_H2_3-21G_1_5_WICKS", "SNOCI_H4_STO-3G_1_5_WICKS",
-                                  "REF_NOCI_H4_STO-3G_1_75"])
+                                  "REF_NOCI_H4_STO-3G_1_75", "REF_NOCI_F2_STO-3G_1_75_WICKS",
+                                  "REF_NOCI_LiH_6-31G_2_8_WICKS"])
 def test_gpu_reference_noci_energy(name):
     fx, ao, st, sysm = system(name)
     print(f"fixture {name}: n_bas={sysm.n_bas} nref={sysm.nref}", flush=True)
@@ -84,3 +85,23 @@ def test_gpu_h2_ccpvdz_noci_matches_oracle():
     print(f"h2 cc-pvdz: E_NOCI gpu {noci_energy(H, S):.12f} oracle {noci_energy(Ho, So):.12f}", flush=True)
     assert np.max(np.abs(H - Ho)) < 1e-10 and np.max(np.abs(S - So)) < 1e-10
     assert abs(noci_energy(H, S) - noci_energy(Ho, So)) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["DET_H2_STO-3G_1_5_WICKS", "DET_F2_sto-3g_1_75_WICKS"])
+def test_gpu_deterministic_energy(name):
+    """The deterministic NOCI-QMC fixtures through the GPU: H, S over the
+    references and their singles and doubles (300 determinants for F2 /
+    STO-3G, every element class with contraction ranks up to 4 per spin on
+    9+9-electron references), element-wise against the oracle, and the
+    lowest root of the pencil against the reference's energy."""
+    fx, ao, st, sysm = system(name)
+    dets = candidates(fx, sysm)
+    dets = [d for r in range(sysm.nref) for d in ([ref_dets(sysm)[r]] + [x for x in dets if x.parent == r])]
+    print(f"fixture {name}: {len(dets)} determinants", flush=True)
+    eng = _engine(sysm)
+    H, S = eng.hs_matrix(to_records(dets), mode=0)
+    Ho, So = O.hs_matrix(O.Slab(sysm, SCF_TOL), dets, mode=0)
+    assert np.max(np.abs(H - Ho)) < 1e-10 * max(1.0, np.max(np.abs(Ho))) and np.max(np.abs(S - So)) < 1e-10
+    e = noci_energy(H, S)
+    print(f"{name}: GPU E_det {e:.12f} (reference {fx['deterministic_noci_energy']:.12f})", flush=True)
+    assert abs(e - fx["deterministic_noci_energy"]) < fx["tolerance"]
