@@ -150,15 +150,23 @@ class Preconditioner:
         y = v * self.dinv
         if not self.active:
             return y
-        t0, t1 = (self.v0 @ y).item(), (self.v1 @ y).item()
+        # device-side contractions: no host synchronisation per application
+        t0, t1 = self.v0 @ y, self.v1 @ y
         w00, w01, w10, w11 = self.w
         return y - self.z0 * (w00 * t0 + w01 * t1) - self.z1 * (w10 * t0 + w11 * t1)
 
 
 def gmres(torch, apply, precondition, b, res_tol=1e-8, restart=30, max_iter=300, log=None, apply_true=None):
     """Right-preconditioned restarted GMRES (src/snoci/gmres.rs:406-544):
-    modified Gram-Schmidt with two passes, Givens rotations, convergence on
-    the RMS of the true residual ||b - A x|| / sqrt(n) recomputed at restarts.
+    Arnoldi with two orthogonalisation passes, Givens rotations, convergence
+    on the RMS of the true residual ||b - A x|| / sqrt(n) recomputed at
+    restarts.
+
+    The Krylov basis stays in device memory as the rows of one matrix, and
+    each orthogonalisation pass is classical Gram-Schmidt against all of it
+    at once (two matrix-vector products; the second pass re-orthogonalises,
+    as the reference's second modified pass does), so an Arnoldi step
+    synchronises with the host once, to read its new Hessenberg column.
 
     apply_true: the operator of the true residual when it differs from the
     Arnoldi operator `apply` (a lower-precision copy of it): every restart is
@@ -174,10 +182,6 @@ def gmres(torch, apply, precondition, b, res_tol=1e-8, restart=30, max_iter=300,
     rms = math.sqrt(n)
     cplx = b.is_complex()
     dt = complex if cplx else float
-    if cplx:
-        dot = lambda u, v: torch.vdot(u, v).item()
-    else:
-        dot = lambda u, v: float(u @ v)
     x = torch.zeros_like(b)
     rtrue = b.clone()  # b - A x for the zero start
     res = float(torch.linalg.vector_norm(rtrue)) / rms
@@ -189,22 +193,27 @@ def gmres(torch, apply, precondition, b, res_tol=1e-8, restart=30, max_iter=300,
         if beta <= SMALL:
             return x, beta / rms, total, beta / rms <= res_tol
         inner = min(restart, max_iter - total)
-        q = [rtrue / beta]
+        Q = torch.empty((inner + 1, n), dtype=b.dtype, device=b.device)
+        Q[0] = rtrue / beta
         Hm = np.zeros((inner + 1, inner), dtype=dt)
         cs = np.zeros(inner); sn = np.zeros(inner, dtype=dt)
         g = np.zeros(inner + 1, dtype=dt); g[0] = beta
         kfinal = 0
         for k in range(inner):
-            w = apply(precondition(q[k]))
-            for _ in range(2):
-                for j in range(k + 1):
-                    hjk = dot(q[j], w)
-                    Hm[j, k] += hjk
-                    w = w - hjk * q[j]
-            hn = float(torch.linalg.vector_norm(w))
+            w = apply(precondition(Q[k]))
+            Qk = Q[:k + 1]
+            Qh = Qk.conj() if cplx else Qk
+            h = torch.mv(Qh, w)
+            w = w - torch.mv(Qk.T, h)
+            h2 = torch.mv(Qh, w)
+            w = w - torch.mv(Qk.T, h2)
+            hnorm = torch.linalg.vector_norm(w)
+            col = torch.cat([(h + h2), hnorm.reshape(1).to(h.dtype)]).cpu().numpy()  # the step's one sync
+            hn = float(abs(col[k + 1]))
+            Hm[:k + 1, k] = col[:k + 1]
             Hm[k + 1, k] = hn
             if hn > SMALL:
-                q.append(w / hn)
+                Q[k + 1] = w / hn
             for j in range(k):
                 tmp = cs[j] * Hm[j, k] + sn[j] * Hm[j + 1, k]
                 Hm[j + 1, k] = -np.conj(sn[j]) * Hm[j, k] + cs[j] * Hm[j + 1, k]
@@ -233,8 +242,9 @@ def gmres(torch, apply, precondition, b, res_tol=1e-8, restart=30, max_iter=300,
         for i in reversed(range(kfinal)):
             rhs = g[i] - sum(Hm[i, j] * y[j] for j in range(i + 1, kfinal))
             y[i] = rhs / Hm[i, i] if abs(Hm[i, i]) > SMALL else 0.0
-        for j in range(kfinal):
-            x = x + dt(y[j]) * precondition(q[j])
+        # x += M^-1 (Q y): the preconditioner is linear, so it is applied once
+        yd = torch.as_tensor(y, dtype=b.dtype, device=b.device)
+        x = x + precondition(torch.mv(Q[:kfinal].T, yd))
         total += kfinal
         rtrue = b - apply_true(x)
         res = float(torch.linalg.vector_norm(rtrue)) / rms

@@ -41,7 +41,7 @@ def small_selection():
 def test_learned_screening_matches_exhaustive(small_selection):
     """The trained network keeps the exact best candidates inside its top 10%
     (random screening of the same size would keep about 10%), the audit finds
-    no miss, and the selected energy equals exhaustive screening."""
+    no miss, and the selected energy is within 1 uHa of exhaustive screening."""
     sysm, cand, eng = small_selection
     kw = dict(iters=5, k_add=16, seed=3)
     ex = screen.select(eng, sysm, cand, learned=False, **kw)
@@ -52,5 +52,5 @@ def test_learned_screening_matches_exhaustive(small_selection):
           f"misses={ln.audit_misses} fallbacks={ln.fallbacks} train loss {ln.train['loss'][0]:.3f}->"
           f"{ln.train['loss'][-1]:.3f} evals {ex.exact_evals} vs {ln.exact_evals}", flush=True)
     assert min(ln.recall) >= 0.6
-    assert abs(ln.energies[-1] - ex.energies[-1]) < 1e-5
+    assert abs(ln.energies[-1] - ex.energies[-1]) < 1e-6  # the screened energy within 1 uHa
     assert sum(ln.exact_evals[2:]) < 0.5 * sum(ex.exact_evals[2:])
