@@ -545,3 +545,154 @@ def select(eng, sysm, cand: Candidates, iters=4, k_add=64, learned=False, learn_
     e, c, H = solve_current(eng, cur_rec, mode)
     res.energies.append(e)
     return res
+
+
+# ---------------------------------------------------------------------------
+# PT2 candidates ranked by the screener
+# ---------------------------------------------------------------------------
+def candidate_dets(cand: Candidates, idx):
+    """nwick.basis.Det objects of the candidates idx (for nwick.pt2.PT2)."""
+    from .basis import Det
+    ra, rb = (1 << cand.nocc_a) - 1, (1 << cand.nocc_b) - 1
+    out = []
+    for i in np.asarray(idx, dtype=np.int64):
+        f = lambda a: [int(v) for v in a[i] if v >= 0]
+        out.append(Det(int(cand.parent[i]), ra, rb, f(cand.ah), f(cand.ap), f(cand.bh), f(cand.bp)))
+    return out
+
+
+@dataclass
+class ScreenedPT2:
+    ept2: float
+    e_current: float
+    kept: np.ndarray            # candidate indices in the PT2 space
+    exact_evals: int            # candidates scored exactly (training, ranking, audit)
+    tail_estimate: float        # estimated first-order weight of the dropped candidates
+    audit_tail: float           # the audit's own estimate of it
+    fallback: bool
+    solve: dict = field(default_factory=dict)
+    seconds: dict = field(default_factory=dict)
+
+
+def screened_pt2(eng, sysm, cand: Candidates, tau=1e-6, train_frac=0.1, top_frac=0.05, chunk_frac=0.02,
+                 chunk_patience=2, audit_frac=0.02, epochs=30, batch=1024, lr=2e-3, seed=0, mode=0, solve_kw=None,
+                 factor_tables="device", log=None):
+    """NOCI-PT2 of the reference space over the candidates the learned
+    screener ranks as contributing.  The reference solves the PT2 equations
+    over every candidate (src/snoci/step.rs:198-429); here the network,
+    trained on the exact first-order weights |<D|H - E S|Psi>|^2 / |E - H_DD|
+    of a random train_frac of the candidates, orders the rest; they are
+    scored exactly in that order, a chunk at a time, until chunk_patience
+    chunks in a row each hold less than tau / 4 of weight and the estimated
+    weight of everything unscored (the last chunk's mean weight times the
+    unscored count) is below tau / 2.  A random audit of the unscored rest
+    judges the network: when the audit's own estimate of the dropped weight
+    exceeds tau, every candidate is scored exactly and the PT2 space is cut
+    by weight instead (the exact fallback).  The PT2 equations are then
+    solved over the kept candidates with nwick.pt2.PT2."""
+    import torch
+
+    from .basis import Det
+    from .mlp import MLP
+    from .pt2 import PT2
+    solve_kw = dict(solve_kw or {})
+    t0 = time.perf_counter()
+    refs = [Det(i, (1 << sysm.nocc_a) - 1, (1 << sysm.nocc_b) - 1) for i in range(sysm.nref)]
+    ref_rec = to_records(refs)
+    scorer = ExactScorer(eng, cand, mode=mode)
+    dev = scorer.dev
+    N = len(cand)
+    e, c, H = solve_current(eng, ref_rec, mode)
+    wpar = np.bincount(np.arange(sysm.nref), weights=c * c, minlength=sysm.nref)
+    state = {"parent_weight": wpar, "parent_energy": np.diag(H).copy(), "energy": e, "reference_energy": e}
+    fmo = reference_fock(sysm)
+    hpar = scorer.coupling(ref_rec, cand.parent.astype(np.int64))
+    dom = int(np.argmax(wpar))
+    hdom = scorer.project_single(ref_rec[dom:dom + 1])
+    Fs = torch.from_numpy(static_features(sysm, cand, fmo, hpar)).to(dev)
+    par_d = torch.from_numpy(cand.parent.astype(np.int64)).to(dev)
+    pf = torch.from_numpy(parent_table(sysm, state).astype(np.float32)).to(dev)
+    F = dynamic_features(torch, Fs, par_d, pf, torch.from_numpy(hpar).to(dev), hdom, scorer.hdiag_d, e)
+    cur = scorer.current(ref_rec, c)
+    sc_d = torch.full((N,), -1.0, dtype=torch.float64, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    # training on exact weights of a random subset
+    tr = torch.randperm(N, generator=gen, device=dev)[:max(64, int(train_frac * N))]
+    tr = torch.sort(tr).values
+    sc_d[tr] = scorer.scores_d(ref_rec, c, e, tr, cur=cur)
+    mu, sd = F.mean(0), F.std(0) + 1e-6
+    X = ((F - mu) / sd).contiguous()
+    y = torch.log10(sc_d[tr] + 1e-20).float()
+    net = MLP(N_FEATURES, seed=seed)
+    net.train(X[tr].contiguous(), (y - y.mean()).contiguous(), None, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    t1 = time.perf_counter()
+    pred = torch.empty(N, dtype=torch.float32, device=dev)
+    net.forward(X, out=pred)
+    net.close()
+    pred[tr] = -float("inf")  # already scored
+    order = torch.argsort(pred, descending=True)[:N - len(tr)]
+    done, quiet = 0, 0
+    chunk = max(64, int(chunk_frac * N))
+    first = max(chunk, int(top_frac * N))
+    tail = float("inf")
+    while done < len(order):
+        step = first if done == 0 else chunk
+        nxt = torch.sort(order[done:done + step]).values
+        s_n = scorer.scores_d(ref_rec, c, e, nxt, cur=cur)
+        sc_d[nxt] = s_n
+        done += len(nxt)
+        tail = float(s_n.mean()) * (len(order) - done)
+        quiet = quiet + 1 if float(s_n.sum()) < 0.25 * tau else 0
+        if quiet >= chunk_patience and tail < 0.5 * tau:
+            break
+    rest = order[done:]
+    audit_tail, fallback = 0.0, False
+    if len(rest):
+        m = min(len(rest), max(16, int(audit_frac * N)))
+        aud = torch.unique(rest[torch.randint(len(rest), (m,), generator=gen, device=dev)])
+        s_a = scorer.scores_d(ref_rec, c, e, aud, cur=cur)
+        sc_d[aud] = s_a
+        audit_tail = float(s_a.mean()) * len(rest)
+        if audit_tail > tau:
+            fallback = True
+            sc_d = scorer.scores_d(ref_rec, c, e, cur=cur)
+    scored = (sc_d >= 0).cpu().numpy()
+    w = sc_d.cpu().numpy()
+    if fallback:
+        # keep the largest weights until what is left sums below tau / 2
+        srt = np.argsort(-w)
+        cum = np.cumsum(w[srt][::-1])[::-1]
+        keep_n = int(np.searchsorted(-cum, -0.5 * tau))
+        kept = np.sort(srt[:max(keep_n, 1)])
+        tail = float(w[srt[keep_n:]].sum())
+    else:
+        kept = np.nonzero(scored)[0]
+    t2 = time.perf_counter()
+    if log:
+        log(f"screened PT2: kept {len(kept)} of {N} candidates, exact evaluations {scorer.n_exact}, "
+            f"tail {tail:.2e} (audit {audit_tail:.2e}), fallback {fallback}")
+    pt = PT2(eng, sysm, refs, candidate_dets(cand, kept), tol=1e-8, mode=mode, factor_tables=factor_tables)
+    pt.setup()
+    r = pt.solve(**solve_kw)
+    if pt.fac is not None:
+        pt.fac.close()
+    t3 = time.perf_counter()
+    return ScreenedPT2(ept2=r["ept2"], e_current=pt.ecurrent, kept=kept, exact_evals=int(scorer.n_exact),
+                       tail_estimate=tail, audit_tail=audit_tail, fallback=fallback, solve=r,
+                       seconds={"train": t1 - t0, "rank_and_score": t2 - t1, "pt2": t3 - t2})
+
+
+def candidates_from_dets(dets, nocc_a, nocc_b) -> Candidates:
+    """The vectorised candidate arrays of a list of singles and doubles
+    (nwick.basis.Det, occupied-first references)."""
+    n = len(dets)
+    arr = {k: np.full((n, 2), -1, dtype=np.int64) for k in ("ah", "ap", "bh", "bp")}
+    for i, d in enumerate(dets):
+        for k in arr:
+            bits = [b for b in range(128) if getattr(d, {"ah": "ha", "ap": "pa", "bh": "hb", "bp": "pb"}[k]) >> b & 1]
+            if len(bits) > 2:
+                raise ValueError("candidate arrays hold singles and doubles")
+            arr[k][i, :len(bits)] = bits
+    return Candidates(np.array([d.parent for d in dets], dtype=np.uint32), arr["ah"], arr["ap"], arr["bh"],
+                      arr["bp"], nocc_a, nocc_b)
