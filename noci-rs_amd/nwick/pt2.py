@@ -267,8 +267,9 @@ class PT2:
         reference's default snoci.gmres.factor_tables = RAM with full_m = None,
         src/input/snoci.rs:25-33), "transient" (regenerated per matvec) or None
         (pairwise on-the-fly matvec, nw_matvec_m).  Candidate lists whose
-        tables do not fit in device memory use the pairwise matvec; per-spin
-        contraction ranks above 4 are refused by every path (NW_E_RANK).
+        tables do not fit in device memory (NW_E_MEM), or whose tables would
+        need per-spin contraction ranks above 4 (NW_E_RANK: triples against
+        doubles), use the pairwise matvec, which covers ranks up to 8.
 
         precision: "fp64" (the reference's arithmetic) or "mixed": GMRES runs
         on the factorised operator with FP32 tables (FP64 sums, half the table
@@ -326,7 +327,7 @@ class PT2:
             try:
                 self.fac = eng.factor(self.cand_rec, mode=self.mode, storage=storage)
             except NwickError as e:
-                if e.code != 4:  # NW_E_MEM: pairwise matvec instead
+                if e.code not in (4, 5):  # NW_E_MEM, NW_E_RANK: pairwise matvec instead
                     raise
                 self.fac = None
             self.timings["factor_s"] = time.perf_counter() - t1

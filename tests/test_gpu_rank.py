@@ -83,3 +83,30 @@ def test_batched_and_spin_overlaps_high_rank(rank_case):
             if mode == 0:
                 assert abs(sa[q] - slab.fs_spin(a, b, 0, 0)[0]) < TOL
                 assert abs(sb[q] - slab.fs_spin(a, b, 1, 0)[0]) < TOL
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_pt2_with_triples_uses_pairwise_matvec(mode):
+    """A PT2 candidate space with triple excitations: the factor tables are
+    refused (a triple against a double is contraction rank 5 on one spin),
+    the driver takes the pairwise matvec, and E_PT2 equals a dense solve
+    formed from the oracle's elements."""
+    from nwick.pt2 import PT2
+    from pt2_dense import dense_pt2
+    n, na, nb = 8, 3, 3
+    sysm = synth.make_system(n, na, nb, 2, seed=41, orth_refs=(1,), orth_count=1, orthonormal_ao=False, n_aux=16)
+    refs = basis.excited_basis(2, n, na, nb, per_ref=0, include_refs=True, counts={})
+    counts = {"a": 5, "b": 5, "aa": 5, "ab": 8, "bb": 5, "aaa": 3, "aab": 3, "abb": 2}
+    cands = basis.excited_basis(2, n, na, nb, per_ref=0, seed=41, include_refs=False, counts=counts)
+    print(f"pt2 triples: mode {mode} ncur={len(refs)} ncand={len(cands)}", flush=True)
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    pt = PT2(eng, sysm, refs, cands, mode=mode, factor_tables="device")
+    pt.setup()
+    assert pt.operator == "pairwise"
+    r = pt.solve(res_tol=1e-11, restart=len(cands) + 10, max_iter=4 * len(cands))
+    dn = dense_pt2(sysm, refs, cands, mode=mode)
+    print(f"pt2 triples: gpu {r['ept2']:.12f} dense {dn['ept2']:.12f} converged {r['converged']}", flush=True)
+    assert r["converged"]
+    assert abs(r["e_noci"] - dn["e_noci"]) < 1e-10
+    assert abs(r["ept2"] - dn["ept2"]) < 1e-8
