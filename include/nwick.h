@@ -33,7 +33,8 @@ enum nw_status {
     NW_E_HIP = 3,       /* HIP runtime failure */
     NW_E_MEM = 4,       /* device allocation failed */
     NW_E_RANK = 5,      /* excitation rank per spin beyond 4, or a pair's contraction rank
-                           (bra plus ket excitations) per spin beyond 4 */
+                           (bra plus ket excitations) per spin beyond 8 (beyond 4 for
+                           factor tables and multi-system batches) */
     NW_E_STATE = 6,     /* call order (e.g. evaluation before nw_build) */
     NW_E_NONFINITE = 7, /* non-finite input */
     NW_E_ORDER = 8      /* references not occupied-first */
@@ -106,7 +107,7 @@ int nw_build(nw_ctx *ctx, double tol, const uint8_t *pair_mask, nw_pair_meta *me
  * req: nreq x 2 uint64 {bra index, ket index} into dets (host or device
  * memory).  h_out, s_out: nreq (host or device).  The requests are binned by
  * contraction rank on the GPU; an index past ndet is NW_E_ARG, an unbuilt
- * reference pair NW_E_STATE, a contraction rank beyond 4 NW_E_RANK. */
+ * reference pair NW_E_STATE, a contraction rank beyond 8 NW_E_RANK. */
 int nw_hs_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t *req,
                   uint64_t nreq, double *h_out, double *s_out, int mode);
 
@@ -115,6 +116,21 @@ int nw_hs_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t
  * build_noci_hs (:214).  Evaluates pairs i <= j and mirrors them; H, S are
  * ndet x ndet row-major. */
 int nw_hs_matrix(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double *H, double *S, int mode);
+
+/* nw_build + nw_hs_matrix for slabs larger than device memory: replaces
+ * build_wicks_shared with a disk-backed slab (src/noci/wicks.rs:213-278,
+ * src/nonorthogonalwicks/storage.rs:189-255) followed by
+ * calculate_matrix_elements.  The ordered reference pairs the upper triangle
+ * of dets touches are sized from their zero-overlap counts, dealt in (x, w)
+ * order into groups whose slab blocks fit in slab_bytes, and each group is
+ * built, its elements evaluated into H, S, and its slab reused by the next.
+ * slab_bytes = 0: free device memory less room for the build's temporaries.
+ * A pair whose block alone exceeds the budget is NW_E_MEM.  H, S as in
+ * nw_hs_matrix; ngroups (may be NULL) receives the number of groups.  The
+ * context then holds the last group's build (as nw_build with that mask).
+ * nw_last_timings: total, builds, evaluations. */
+int nw_hs_matrix_streamed(nw_ctx *ctx, double tol, const nw_det *dets, uint64_t ndet, double *H,
+                          double *S, int mode, uint64_t slab_bytes, int *ngroups);
 
 /* Rebuild the generalised-Fock intermediates of every built pair from the
  * spin Fock matrices fa, fb (n_bas x n_bas): replaces update_wicks_fock
@@ -186,8 +202,9 @@ int nw_set_mixed_floor(nw_ctx *ctx, double sigma_floor);
  * scalar meta (32 doubles) out of device memory; out may be NULL to query
  * the length through offsets[94].  For tests and disk interop.  A pair built
  * with reduced-precision transforms (nw_pair_meta.fp32) keeps its rank-4
- * blocks as FP32 on the device; they are copied out widened to FP64, as
- * nw_save_wicks writes them. */
+ * blocks as half-length FP32 blocks on the device; the copy (and its offsets)
+ * is in the reference layout, those blocks widened to FP64, as nw_save_wicks
+ * writes them. */
 int nw_pair_slab(nw_ctx *ctx, int x, int w, double *out, int64_t *offsets, double *meta);
 
 /* Wick disk cache in the reference's format (src/nonorthogonalwicks/storage.rs:

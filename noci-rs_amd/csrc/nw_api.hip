@@ -1,6 +1,7 @@
 // C ABI of the engine (include/nwick.h): context, per-reference-pair build
 // orchestration and the evaluation entry points.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -121,7 +122,19 @@ int nw_set_refs(nw_ctx *c, int nref, int nmo, const double *ca, const double *cb
 
 int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta_out)
 {
-    return guarded(c, [&]() -> int {
+    return guarded(c, [&]() -> int { return build_impl(c, tol, pair_mask, meta_out, nullptr); });
+}
+
+}  // extern "C"
+
+namespace nwi {
+
+// Body of nw_build.  With `sizes` set, only the occupied-overlap SVDs run:
+// sizes receives, per masked pair in (x, w) order, the slab length (in
+// doubles) its block would take, and no slab is written.
+int build_impl(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta_out, std::vector<long long> *sizes)
+{
+    {
         if (c->nref == 0) return fail(c, NW_E_STATE, "nw_set_refs first");
         const int n = c->n, nm = c->nmo, nref = c->nref;
         const size_t nn = (size_t)n * n, nnm = (size_t)n * nm, nm2 = (size_t)nm * nm;
@@ -226,17 +239,23 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                     if (sv > tol) smin[p] = std::min(smin[p], sv);
                 }
         // ---- slab layout ------------------------------------------------------
+        // A pair whose smallest nonzero singular value clears the floor runs its
+        // four-index transforms in reduced precision and keeps its rank-4 blocks
+        // as FP32 (half-length blocks).
+        const bool mixed = c->precision != NW_PREC_FP64 && n % 2 == 0;
         long long total = 0;
         for (int p = 0; p < P; ++p) {
             PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
             pi.smin = smin[p];
-            pi.fp32 = 0;
-            long long sz = pair_offsets(pi.ma, pi.mb, nm, pi.off);
+            pi.fp32 = mixed && smin[p] >= c->mixed_floor ? 1 : 0;
+            long long sz = pair_offsets(pi.ma, pi.mb, nm, pi.off, pi.fp32 != 0);
             pi.base = total;
             pi.built = 1;
             pi.bidx = p;
             total += sz;
+            if (sizes) sizes->push_back(sz);
         }
+        if (sizes) return NW_OK;
         if ((size_t)total > c->slab_cap) {
             if (c->d_slab) HIPCHK(hipFree(c->d_slab));
             c->d_slab = nullptr;
@@ -425,7 +444,6 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         HIPCHK(hipGetLastError());
         // ---- four-index transforms (build.rs:261-313, 877-917) ----------------
         HIPCHK(hipEventRecord(c->ev[1], c->st));
-        const bool mixed = c->precision != NW_PREC_FP64 && n % 2 == 0;
         const bool bf3 = c->precision == NW_PREC_BF16X3;
         // One job per rank-4 tensor: the J slots per spin and the II blocks.
         // J slab block [a,b,c,d] = blk[c,d,a,b] with blk = (cx_i xc_j || cx_k xc_l);
@@ -436,8 +454,7 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         auto XC = [&](int pr, int mi) { return 2 * (pr * 2 + mi) + 1; };
         for (int p = 0; p < P; ++p) {
             PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
-            const bool f32 = mixed && smin[p] >= c->mixed_floor;
-            pi.fp32 = f32 ? 1 : 0;
+            const bool f32 = pi.fp32 != 0;
             for (int sp = 0; sp < 2; ++sp) {
                 const long long *o = pi.off + (sp ? O_BB : O_AA);
                 const int pr = prob(p, sp);
@@ -499,7 +516,7 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
                         C1[k] = cm(jb.c1); C2[k] = cm(jb.c2); C3[k] = cm(jb.c3); C4[k] = cm(jb.c4);
                         TA[k] = (float *)(A + k * slot);
                         TB[k] = (float *)(B + k * slot);
-                        OUT[k] = (float *)jb.out;  // FP32 values at the start of the FP64-sized slab block
+                        OUT[k] = (float *)jb.out;  // the pair's half-length FP32 rank-4 block
                     }
                     std::vector<const float *> TAc(TA.begin(), TA.end()), TBc(TB.begin(), TB.end());
                     auto pass = bf3 ? launch_qtransform_bf3_many : launch_qtransform_f32_many;
@@ -577,17 +594,16 @@ int nw_build(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta
         c->built = true;
         c->have_codensity = true;
         return NW_OK;
-    });
+    }
 }
 
-}  // extern "C"
-
-namespace nwi {
-
 // assign_offsets (src/nonorthogonalwicks/layout.rs:21-189) for one pair.
-long long pair_offsets(int ma, int mb, int nmo, long long *off)
+// With r4_f32 the rank-4 blocks (J slots, II) hold nmo^4 FP32 values in
+// half-length blocks; every other block and the block order are the
+// reference's.
+long long pair_offsets(int ma, int mb, int nmo, long long *off, bool r4_f32)
 {
-    const long long nn2 = (long long)nmo * nmo, nn4 = nn2 * nn2;
+    const long long nn2 = (long long)nmo * nmo, nn4 = r4_f32 ? (nn2 * nn2 + 1) / 2 : nn2 * nn2;
     long long i = 0;
     for (int k = 0; k < O_TOTAL; ++k) off[k] = -1;
     for (int sp = 0; sp < 2; ++sp) {
@@ -617,6 +633,34 @@ long long pair_offsets(int ma, int mb, int nmo, long long *off)
                     }
     off[O_TOTAL] = i;
     return i;
+}
+
+// One pair's block in the reference layout (FP64 throughout, the offsets of
+// pair_offsets(ma, mb, nmo)) from a host copy `blk` of its device block, whose
+// rank-4 blocks are half-length FP32 blocks when the pair was built in
+// reduced precision.
+void export_pair(const PairInfo &pi, int nmo, const double *blk, double *out)
+{
+    long long ref[O_LEN];
+    pair_offsets(pi.ma, pi.mb, nmo, ref);
+    const size_t nn2 = (size_t)nmo * nmo, nn4 = nn2 * nn2;
+    auto rank4 = [](int k) {
+        return (k >= O_AA + O_J && k < O_AA + O_J + 10) || (k >= O_BB + O_J && k < O_BB + O_J + 10) ||
+               (k >= O_II && k < O_II + 16);
+    };
+    for (int k = 0; k < O_TOTAL; ++k) {
+        if (pi.off[k] < 0) continue;
+        if (rank4(k) && pi.fp32) {
+            float f;
+            const char *src = (const char *)(blk + pi.off[k]);
+            for (size_t i = 0; i < nn4; ++i) {
+                std::memcpy(&f, src + i * sizeof(float), sizeof(float));
+                out[ref[k] + i] = (double)f;
+            }
+        } else {
+            std::memcpy(out + ref[k], blk + pi.off[k], (rank4(k) ? nn4 : nn2) * sizeof(double));
+        }
+    }
 }
 
 
@@ -654,7 +698,8 @@ int bin_key(int la, int lb, bool m0)
 // Upper-triangle pairs (i <= j) of a determinant list grouped into blocks of
 // equal (parent, rank_a, rank_b) runs: calculate_matrix_elements' pair list
 // (src/noci/matrix.rs:55-59), partitioned by contraction-rank bin.
-std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vector<PackedDet> &h, bool need_hcol)
+std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vector<PackedDet> &h, bool need_hcol,
+                                                      bool built_only)
 {
     std::vector<Run> runs;
     for (long long i = 0; i < (long long)h.size(); ++i) {
@@ -670,11 +715,13 @@ std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vect
             const Run &A = runs[r1], &B = runs[r2];
             const int pair = A.parent * c->nref + B.parent;
             const PairInfo &pi = c->pinfo[pair];
-            if (!pi.built)
-                throw NwError(NW_E_STATE, "reference pair (" + std::to_string(A.parent) + "," + std::to_string(B.parent) +
-                                              ") was not built");
             const int la = A.ra + B.ra, lb = A.rb + B.rb;
             if (la > 8 || lb > 8) throw NwError(NW_E_RANK, RANK_MSG);
+            if (!pi.built) {
+                if (built_only) continue;
+                throw NwError(NW_E_STATE, "reference pair (" + std::to_string(A.parent) + "," + std::to_string(B.parent) +
+                                              ") was not built");
+            }
             const bool m0 = need_hcol && pi.ma == 0 && pi.mb == 0;
             auto &v = bins[bin_key(la, lb, m0)];
             v.push_back({A.start, B.start, (int)A.len, (int)B.len, pair, r1 == r2 ? 1 : 0, 0});
@@ -869,6 +916,106 @@ extern "C" {
 int nw_hs_matrix(nw_ctx *c, const nw_det *dets, uint64_t ndet, double *H, double *S, int mode)
 {
     return guarded(c, [&]() -> int { return eval_matrix(c, EVAL_HS, dets, ndet, H, S, mode); });
+}
+
+int nw_hs_matrix_streamed(nw_ctx *c, double tol, const nw_det *dets, uint64_t ndet, double *H, double *S, int mode,
+                          uint64_t slab_bytes, int *ngroups)
+{
+    return guarded(c, [&]() -> int {
+        if (c->nref == 0) return fail(c, NW_E_STATE, "nw_set_refs first");
+        if (!dets || !H || !S || ndet == 0) return fail(c, NW_E_ARG, "null arguments");
+        if (ndet > (1ull << 31)) return fail(c, NW_E_SHAPE, "too many determinants");
+        const int nref = c->nref;
+        const auto t0 = std::chrono::steady_clock::now();
+        PackedDet *dp = nullptr;
+        std::vector<PackedDet> h = pack(c, dets, ndet, dp);
+        // the ordered reference pairs of the upper triangle's run pairs (triangle_blocks)
+        std::vector<int> runp;
+        for (size_t i = 0; i < h.size(); ++i)
+            if (i == 0 || h[i].parent != h[i - 1].parent || h[i].ra != h[i - 1].ra || h[i].rb != h[i - 1].rb)
+                runp.push_back(h[i].parent);
+        std::vector<uint8_t> need((size_t)nref * nref, 0);
+        for (size_t r1 = 0; r1 < runp.size(); ++r1)
+            for (size_t r2 = r1; r2 < runp.size(); ++r2) need[(size_t)runp[r1] * nref + runp[r2]] = 1;
+        // the slab length of every needed pair, from its zero-overlap counts
+        std::vector<long long> sizes;
+        int rc = build_impl(c, tol, need.data(), nullptr, &sizes);
+        if (rc != NW_OK) return rc;
+        std::vector<int> pid;
+        for (int p = 0; p < nref * nref; ++p)
+            if (need[p]) pid.push_back(p);
+        // outputs first, then the budget: the slab of an earlier build is released
+        const bool adev = is_device_ptr(H), bdev = is_device_ptr(S);
+        double *dA = adev ? H : c->outH.get<double>(ndet * ndet);
+        double *dB = bdev ? S : c->outS.get<double>(ndet * ndet);
+        if (c->d_slab) {
+            HIPCHK(hipFree(c->d_slab));
+            c->d_slab = nullptr;
+            c->slab_cap = 0;
+        }
+        uint64_t budget = slab_bytes;
+        if (budget == 0) {
+            size_t fr = 0, tot = 0;
+            HIPCHK(hipMemGetInfo(&fr, &tot));
+            // room for the build's temporaries: two transform scratch buffers of up to
+            // 1 GiB each, the per-pair arenas and the FP32 integral copies
+            const uint64_t n4 = (uint64_t)c->n * c->n * c->n * c->n;
+            const uint64_t reserve = (6ull << 30) + n4 * 8;
+            budget = fr > reserve ? (uint64_t)((double)(fr - reserve) * 0.9) : 0;
+        }
+        // pairs in (x, w) order, greedily into groups whose blocks fit the budget
+        std::vector<std::vector<int>> groups;
+        uint64_t cur = 0;
+        for (size_t k = 0; k < pid.size(); ++k) {
+            const uint64_t b = (uint64_t)sizes[k] * 8;
+            if (b > budget)
+                return fail(c, NW_E_MEM, "the slab block of reference pair (" + std::to_string(pid[k] / nref) + "," +
+                                             std::to_string(pid[k] % nref) + ") takes " + std::to_string(b) +
+                                             " bytes, more than the slab budget of " + std::to_string(budget));
+            if (groups.empty() || cur + b > budget) {
+                groups.emplace_back();
+                cur = 0;
+            }
+            groups.back().push_back(pid[k]);
+            cur += b;
+        }
+        double build_ms = 0.0, eval_ms = 0.0;
+        std::vector<uint8_t> mask((size_t)nref * nref);
+        for (const auto &g : groups) {
+            std::fill(mask.begin(), mask.end(), 0);
+            for (int p : g) mask[p] = 1;
+            rc = build_impl(c, tol, mask.data(), nullptr, nullptr);
+            if (rc != NW_OK) return rc;
+            build_ms += c->t_ms[0];
+            // every element of the group's pairs; each pair is in exactly one group
+            auto bins = triangle_blocks(c, h, true, true);
+            EvalArgs args = base_args(c, dp, mode);
+            args.A = dA;
+            args.B = dB;
+            args.ldo = (long long)ndet;
+            Uploader up(c, 64u << 20);
+            HIPCHK(hipEventRecord(c->ev[1], c->st));
+            for (auto &kv : bins) {
+                int la, lb;
+                key_ranks(kv.first, la, lb);
+                EvalBlock *db = up.put(kv.second);
+                launch_eval_blocks(c->st, EVAL_HS, la, lb, kv.first & 1, db, (int)kv.second.size(), tiles_of(kv.second),
+                                   args);
+            }
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(c->ev[2], c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            eval_ms += elapsed(c->ev[1], c->ev[2]);
+        }
+        if (!adev) HIPCHK(hipMemcpyAsync(H, dA, ndet * ndet * 8, hipMemcpyDeviceToHost, c->st));
+        if (!bdev) HIPCHK(hipMemcpyAsync(S, dB, ndet * ndet * 8, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (ngroups) *ngroups = (int)groups.size();
+        c->t_ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        c->t_ms[1] = build_ms;
+        c->t_ms[2] = eval_ms;
+        return NW_OK;
+    });
 }
 
 int nw_fs_matrix(nw_ctx *c, const nw_det *dets, uint64_t ndet, double *F, double *S, int mode)
@@ -1122,14 +1269,18 @@ int nw_pair_slab(nw_ctx *c, int x, int w, double *out, int64_t *offsets, double 
         if (x < 0 || w < 0 || x >= c->nref || w >= c->nref) return fail(c, NW_E_ARG, "pair out of range");
         const PairInfo &pi = c->pinfo[(size_t)x * c->nref + w];
         if (!pi.built) return fail(c, NW_E_STATE, "pair not built");
+        // the reference layout, whatever precision the pair was built in
+        long long ref[O_LEN];
+        pair_offsets(pi.ma, pi.mb, c->nmo, ref);
         if (offsets)
-            for (int k = 0; k < O_LEN; ++k) offsets[k] = pi.off[k];
+            for (int k = 0; k < O_LEN; ++k) offsets[k] = ref[k];
         if (meta)
             for (int k = 0; k < M_LEN; ++k) meta[k] = pi.meta[k];
         if (out) {
-            HIPCHK(hipMemcpyAsync(out, c->d_slab + pi.base, (size_t)pi.off[O_TOTAL] * 8, hipMemcpyDeviceToHost, c->st));
+            std::vector<double> blk((size_t)pi.off[O_TOTAL]);
+            HIPCHK(hipMemcpyAsync(blk.data(), c->d_slab + pi.base, blk.size() * 8, hipMemcpyDeviceToHost, c->st));
             HIPCHK(hipStreamSynchronize(c->st));
-            widen_rank4(pi, c->nmo, out);
+            export_pair(pi, c->nmo, blk.data(), out);
         }
         return NW_OK;
     });

@@ -110,12 +110,13 @@ int nw_save_wicks(nw_ctx *c, const char *dir)
         const int nref = c->nref, nm = c->nmo;
         for (auto &pi : c->pinfo)
             if (!pi.built) return fail(c, NW_E_STATE, "the disk cache holds every ordered pair: build without a mask");
-        // slab in pair order x * nref + w; offsets absolute
-        std::vector<long long> base(nref * nref);
+        // slab in pair order x * nref + w, in the reference layout; offsets absolute
+        std::vector<long long> base(nref * nref), refoff((size_t)nref * nref * O_LEN);
         long long total = 0;
         for (int p = 0; p < nref * nref; ++p) {
+            const PairInfo &pi = c->pinfo[p];
             base[p] = total;
-            total += c->pinfo[p].off[O_TOTAL];
+            total += pair_offsets(pi.ma, pi.mb, nm, &refoff[(size_t)p * O_LEN]);
         }
         Writer w;
         w.u32(WICKS_DISK_CACHE_VERSION);
@@ -123,12 +124,12 @@ int nw_save_wicks(nw_ctx *c, const char *dir)
         w.u64((uint64_t)total);
         w.u64((uint64_t)nref * nref);
         for (int p = 0; p < nref * nref; ++p) {
-            const PairInfo &pi = c->pinfo[p];
-            put_same_off(w, pi.off + O_AA, base[p]);
-            put_same_off(w, pi.off + O_BB, base[p]);
-            for (int k = 0; k < 8; ++k) w.u64((uint64_t)(base[p] + pi.off[O_VAB + k]));
-            for (int k = 0; k < 8; ++k) w.u64((uint64_t)(base[p] + pi.off[O_VBA + k]));
-            for (int k = 0; k < 16; ++k) w.u64(pi.off[O_II + k] < 0 ? NONE : (uint64_t)(base[p] + pi.off[O_II + k]));
+            const long long *off = &refoff[(size_t)p * O_LEN];
+            put_same_off(w, off + O_AA, base[p]);
+            put_same_off(w, off + O_BB, base[p]);
+            for (int k = 0; k < 8; ++k) w.u64((uint64_t)(base[p] + off[O_VAB + k]));
+            for (int k = 0; k < 8; ++k) w.u64((uint64_t)(base[p] + off[O_VBA + k]));
+            for (int k = 0; k < 16; ++k) w.u64(off[O_II + k] < 0 ? NONE : (uint64_t)(base[p] + off[O_II + k]));
         }
         w.u64((uint64_t)nref * nref);
         for (int p = 0; p < nref * nref; ++p) {
@@ -148,14 +149,15 @@ int nw_save_wicks(nw_ctx *c, const char *dir)
         }
         std::ofstream f(d + "/wicks.bin", std::ios::binary | std::ios::trunc);
         if (!f) return fail(c, NW_E_ARG, "cannot write " + d + "/wicks.bin");
-        std::vector<double> buf;
+        std::vector<double> buf, out;
         for (int p = 0; p < nref * nref; ++p) {
             const PairInfo &pi = c->pinfo[p];
-            const size_t len = (size_t)pi.off[O_TOTAL];
-            buf.resize(len);
-            HIPCHK(hipMemcpy(buf.data(), c->d_slab + pi.base, len * 8, hipMemcpyDeviceToHost));
-            widen_rank4(pi, c->nmo, buf.data());
-            f.write((const char *)buf.data(), (std::streamsize)(len * 8));
+            buf.resize((size_t)pi.off[O_TOTAL]);
+            HIPCHK(hipMemcpy(buf.data(), c->d_slab + pi.base, buf.size() * 8, hipMemcpyDeviceToHost));
+            const size_t len = (size_t)refoff[(size_t)p * O_LEN + O_TOTAL];
+            out.resize(len);
+            export_pair(pi, nm, buf.data(), out.data());
+            f.write((const char *)out.data(), (std::streamsize)(len * 8));
             if (!f) return fail(c, NW_E_ARG, "write failed: " + d + "/wicks.bin");
         }
         return NW_OK;

@@ -189,25 +189,6 @@ inline const void *to_device(nw_ctx *c, DBuf &buf, const void *p, size_t bytes)
     return d;
 }
 
-// A pair built with reduced-precision transforms (PairInfo::fp32) keeps each
-// rank-4 block (the J slots of both spins, the II blocks; nmo^4 values) as
-// FP32 values at the start of its FP64-sized block.  Widen them in a host
-// copy of the pair's slab block, so exported and saved slabs are all FP64.
-inline void widen_rank4(const PairInfo &pi, int nmo, double *blk)
-{
-    if (!pi.fp32) return;
-    const size_t nn4 = (size_t)nmo * nmo * nmo * nmo;
-    std::vector<float> tmp(nn4);
-    auto widen = [&](long long off) {
-        if (off < 0) return;
-        std::memcpy(tmp.data(), blk + off, nn4 * sizeof(float));
-        for (size_t i = 0; i < nn4; ++i) blk[off + i] = (double)tmp[i];
-    };
-    for (int sp = 0; sp < 2; ++sp)
-        for (int s = 0; s < 10; ++s) widen(pi.off[(sp ? O_BB : O_AA) + O_J + s]);
-    for (int k = 0; k < 16; ++k) widen(pi.off[O_II + k]);
-}
-
 inline float elapsed(hipEvent_t a, hipEvent_t b)
 {
     float ms = 0.f;
@@ -220,11 +201,14 @@ inline float elapsed(hipEvent_t a, hipEvent_t b)
 
 
 struct Run { long long start, len; int parent, ra, rb; };
-long long pair_offsets(int ma, int mb, int nmo, long long *off);
+long long pair_offsets(int ma, int mb, int nmo, long long *off, bool r4_f32 = false);
+int build_impl(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *meta_out, std::vector<long long> *sizes);
+void export_pair(const PairInfo &pi, int nmo, const double *blk, double *out);
 std::vector<PackedDet> pack(nw_ctx *c, const nw_det *dets, uint64_t ndet, PackedDet *&dpacked,
                             const void **draw = nullptr);
 int bin_key(int la, int lb, bool m0);
-std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vector<PackedDet> &h, bool need_hcol);
+std::map<int, std::vector<EvalBlock>> triangle_blocks(nw_ctx *c, const std::vector<PackedDet> &h, bool need_hcol,
+                                                      bool built_only = false);
 long long tiles_of(const std::vector<EvalBlock> &v);
 std::map<int, std::vector<EvalBlock>> rect_blocks(nw_ctx *c, const std::vector<PackedDet> &h, long long nrow0,
                                                   long long ncol, bool need_hcol);

@@ -30,7 +30,7 @@ STATUS = {0: "NW_OK", 1: "NW_E_ARG", 2: "NW_E_SHAPE", 3: "NW_E_HIP", 4: "NW_E_ME
 
 # Every symbol the header declares.
 EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matrix", "nw_update_fock",
-           "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_project_hs", "nw_noci_fock", "nw_set_precision", "nw_set_mixed_floor", "nw_save_wicks", "nw_load_wicks", "nw_pair_slab", "nw_last_timings",
+           "nw_fs_matrix", "nw_fs_batched", "nw_matvec_m", "nw_project_hs", "nw_noci_fock", "nw_set_precision", "nw_set_mixed_floor", "nw_save_wicks", "nw_load_wicks", "nw_pair_slab", "nw_last_timings", "nw_hs_matrix_streamed",
            "nw_last_error", "nw_destroy", "nw_batch_create", "nw_batch_add", "nw_batch_plan", "nw_batch_run",
            "nw_batch_result", "nw_batch_schedule", "nw_batch_set_schedule", "nw_batch_last_ms",
            "nw_batch_last_error", "nw_batch_destroy", "nw_mlp_create", "nw_mlp_num_params", "nw_mlp_set_params",
@@ -89,6 +89,7 @@ def load():
     L.nw_s_same_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, ct.c_int, vp]
     L.nw_m_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, ct.c_double, vp, ct.c_int]
     L.nw_hs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
+    L.nw_hs_matrix_streamed.argtypes = [vp, ct.c_double, vp, ct.c_uint64, vp, vp, ct.c_int, ct.c_uint64, vp]
     L.nw_pair_slab.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
     L.nw_set_precision.argtypes = [vp, ct.c_int]
     L.nw_project_hs.argtypes = [vp, vp, ct.c_uint64, ct.c_uint64, vp, vp, vp, ct.c_int]
@@ -228,6 +229,21 @@ class Engine:
         sync_torch(dets, H, S)
         self._check(self.L.nw_hs_matrix(self.ctx, _ptr(dets), n, _ptr(H), _ptr(S), int(mode)))
         return H, S
+
+    def hs_matrix_streamed(self, dets, tol=1e-8, mode=MODE_REFERENCE, slab_bytes=0, out=None):
+        """Build and evaluate in groups of reference pairs whose slab blocks fit
+        in slab_bytes (0: what device memory allows); returns H, S and the
+        number of groups."""
+        n = len(dets) if not hasattr(dets, "data_ptr") else dets.numel() * dets.element_size() // DET_DTYPE.itemsize
+        if out is None:
+            H = np.empty((n, n)); S = np.empty((n, n))
+        else:
+            H, S = out
+        groups = ct.c_int()
+        sync_torch(dets, H, S)
+        self._check(self.L.nw_hs_matrix_streamed(self.ctx, float(tol), _ptr(dets), n, _ptr(H), _ptr(S), int(mode),
+                                                 int(slab_bytes), ct.byref(groups)))
+        return H, S, groups.value
 
     def hs_batched(self, dets, req, mode=MODE_REFERENCE):
         req = np.ascontiguousarray(req, dtype=np.uint64).reshape(-1, 2)

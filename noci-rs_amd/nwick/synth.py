@@ -71,7 +71,7 @@ def make_system(n_bas: int, nocc_a: int, nocc_b: int, nref: int, seed: int,
                 h_sigma: float = 0.1, chol_sigma: float = 0.05,
                 kappa_sigma: float = 0.3, orth_refs=(), orth_count: int = 1,
                 orth_spin: str = "alpha", dense_eri: bool = True,
-                enuc: float = 0.0) -> System:
+                enuc: float = 0.0, orth_against=None) -> System:
     """Build one seeded synthetic system.
 
     h: symmetric N(0, h_sigma) entries plus diag linspace(-2, 0).
@@ -80,6 +80,9 @@ def make_system(n_bas: int, nocc_a: int, nocc_b: int, nref: int, seed: int,
     C0: eigenvectors of h in the S metric (ascending), occupied first.
     Each reference: C = C0 expm(kappa), kappa occ-virt antisymmetric with
     N(0, kappa_sigma) entries, drawn independently per spin.
+    orth_refs: references whose first orth_count occupied orbitals (of
+    orth_spin) are made S-orthogonal to the occupied spaces of the references
+    in orth_against (default: every other reference).
     """
     rng = np.random.default_rng(seed)
     n = n_bas
@@ -119,7 +122,8 @@ def make_system(n_bas: int, nocc_a: int, nocc_b: int, nref: int, seed: int,
     spin_idx = 0 if orth_spin == "alpha" else 1
     nocc_s = (nocc_a, nocc_b)[spin_idx]
     for r in orth_refs:
-        others = [refs[x][spin_idx][:, :nocc_s] for x in range(nref) if x != r]
+        against = range(nref) if orth_against is None else orth_against
+        others = [refs[x][spin_idx][:, :nocc_s] for x in against if x != r]
         C = refs[r][spin_idx].copy()
         Q = _s_orthonormalise(np.concatenate(others, axis=1), S) if others else np.zeros((n, 0))
         occ = C[:, :nocc_s].copy()

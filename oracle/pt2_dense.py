@@ -13,6 +13,22 @@ import numpy as np
 import oracle as O  # oracle/oracle.py (the oracle directory is on sys.path in tests)
 
 
+def candidate_left(slab, cands, current, mode):
+    """F_ai, S_ai, H_ai with the candidate as bra: the reference evaluates the
+    candidate-current blocks candidate-left (build_noci_hs / build_noci_fock
+    with candidates first, src/snoci/operators.rs:128-159) and takes the
+    current-candidate blocks as their adjoints (:113, :138).  The orientation
+    matters for reference pairs with zero-overlap orbitals (m > 0) in the
+    reference arithmetic (mode 0), whose elements are not symmetric."""
+    na, ni = len(cands), len(current)
+    F, S, H = np.empty((na, ni)), np.empty((na, ni)), np.empty((na, ni))
+    for a, da in enumerate(cands):
+        for i, di in enumerate(current):
+            F[a, i], S[a, i] = slab.fs(da, di, mode)
+            H[a, i] = slab.hs(da, di, mode)[0]
+    return F, S, H
+
+
 def dense_pt2(sysm, current, cands, mode=0, tol=1e-12, evp_tol=1e-8):
     from nwick.pt2 import fock, general_evp, noci_density
     slab = O.Slab(sysm, tol)
@@ -23,12 +39,8 @@ def dense_pt2(sysm, current, cands, mode=0, tol=1e-12, evp_tol=1e-8):
     slab.update_fock(sysm, fa, fb)
     Fc, _ = O.fs_matrix(slab, current, mode=mode)
     e0 = float(c @ Fc @ c)
-    allm = list(current) + list(cands)
-    ncur = len(current)
-    Fall, Sall = O.fs_matrix(slab, allm, mode=mode)
-    Hall, _ = O.hs_matrix(slab, allm, mode=mode)
-    F_ai, S_ai, H_ai = Fall[ncur:, :ncur], Sall[ncur:, :ncur], Hall[ncur:, :ncur]
-    Faa, Saa = Fall[ncur:, ncur:], Sall[ncur:, ncur:]
+    F_ai, S_ai, H_ai = candidate_left(slab, cands, current, mode)
+    Faa, Saa = O.fs_matrix(slab, list(cands), mode=mode)
     s0, f0 = S_ai @ c, F_ai @ c
     M = Faa - e0 * Saa - np.outer(f0, s0) - np.outer(s0, f0) + 2 * e0 * np.outer(s0, s0)
     v = H_ai @ c - e[0] * s0
@@ -61,12 +73,8 @@ class DensePT2Backend:
         slab.update_fock(sysm, fa, fb)
         Fc, _ = O.fs_matrix(slab, current, mode=self.mode)
         e0 = float(c @ Fc @ c)
-        allm = list(current) + list(cands)
-        ncur = len(current)
-        Fall, Sall = O.fs_matrix(slab, allm, mode=self.mode)
-        Hall, _ = O.hs_matrix(slab, allm, mode=self.mode)
-        F_ai, S_ai, H_ai = Fall[ncur:, :ncur], Sall[ncur:, :ncur], Hall[ncur:, :ncur]
-        Faa, Saa = Fall[ncur:, ncur:], Sall[ncur:, ncur:]
+        F_ai, S_ai, H_ai = candidate_left(slab, cands, current, self.mode)
+        Faa, Saa = O.fs_matrix(slab, list(cands), mode=self.mode)
         s0, f0 = S_ai @ c, F_ai @ c
         Mraw = Faa - e0 * Saa
         M = Mraw - np.outer(f0, s0) - np.outer(s0, f0) + 2 * e0 * np.outer(s0, s0)
