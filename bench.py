@@ -961,6 +961,103 @@ def wick_workload(args):
 
 
 
+def stream_workload(args):
+    """The headline recipe with 16 references (136 upper-triangle pairs, about
+    340 GB of FP64 slab: more than one device holds) through the streamed
+    build -> evaluate -> reuse schedule, FP64 and bf16x3; returns the result dict."""
+    import numpy as np
+    import torch
+
+    import nwick
+    from nwick import basis, synth
+
+    tol, nref = 1e-8, 16
+    t = time.perf_counter()
+    # reference 15 has an alpha orbital orthogonal to the occupied spaces of references 0-6
+    sysm = synth.make_system(100, 10, 10, nref, args.seed, n_aux=300, orth_refs=(nref - 1,), orth_count=1,
+                             orth_against=tuple(range(7)))
+    dets = basis.excited_basis(nref, 100, 10, 10, per_ref=512, seed=args.seed)
+    rec = basis.to_records(dets)
+    ndet = len(dets)
+    npairs = ndet * (ndet + 1) // 2
+    log(f"stream setup: {nref} refs, {ndet} determinants in {time.perf_counter() - t:.1f} s")
+    eng = nwick.Engine.from_system(sysm, device=0)
+    dev = torch.device("cuda:0")
+    Hd = torch.empty((ndet, ndet), dtype=torch.float64, device=dev)
+    Sd = torch.empty((ndet, ndet), dtype=torch.float64, device=dev)
+    rec_d = torch.from_numpy(rec.view(np.uint8).copy()).to(dev)
+    torch.cuda.synchronize()
+
+    def lowest_energy():
+        L = torch.linalg.cholesky(Sd)
+        A = torch.linalg.solve_triangular(L, Hd, upper=False)
+        A = torch.linalg.solve_triangular(L, A.T.contiguous(), upper=False)
+        return float(torch.linalg.eigvalsh(0.5 * (A + A.T))[0])
+
+    runs = {}
+    for name, prec in (("fp64", nwick.PREC_FP64), ("bf16x3", nwick.PREC_BF16X3)):
+        eng.set_precision(prec)
+        meta = {}
+        for w in range(args.warmup):
+            _, _, groups = eng.hs_matrix_streamed(rec_d, tol=tol, mode=args.mode, out=(Hd, Sd), meta=meta)
+        torch.cuda.synchronize()
+        totals, builds, evals = [], [], []
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            _, _, groups = eng.hs_matrix_streamed(rec_d, tol=tol, mode=args.mode, out=(Hd, Sd), meta=meta)
+            tm = eng.timings()
+            totals.append(tm["total_ms"]); builds.append(tm["transform_ms"]); evals.append(tm["eval_ms"])
+            log(f"[stream {name}] step {k}: {groups} groups, {tm['total_ms']:.0f} ms (builds {tm['transform_ms']:.0f}"
+                f" ms, evaluations {tm['eval_ms']:.1f} ms)")
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        energy = float("nan") if args.no_energy else lowest_energy()
+        ntensor = n32 = 0
+        for pm in meta.values():
+            k = tensors_of(pm.ma, pm.mb)
+            ntensor += k
+            n32 += k if pm.fp32 else 0
+        runs[name] = {"rate": npairs * args.steps / elapsed, "ms_per_step": 1e3 * elapsed / args.steps,
+                      "groups": groups, "energy": energy, "build_ms": float(np.mean(builds)),
+                      "eval_ms": float(np.mean(evals)), "pairs_built": len(meta), "rank4_tensors": ntensor,
+                      "fp32_tensors": n32,
+                      "slab_bytes_fp64_estimate": sum(8 * (58 * 100 ** 2 + tensors_of(pm.ma, pm.mb) * 100 ** 4)
+                                                    for pm in meta.values())}
+        log(f"[stream {name}] {runs[name]['rate']:.4e} pairs/s, {groups} groups, energy {energy:.12f}")
+    eng.close()
+    del Hd, Sd
+    r64, rbf = runs["fp64"], runs["bf16x3"]
+    dE = None if args.no_energy else rbf["energy"] - r64["energy"]
+    use_bf = args.no_energy or abs(dE) < 1e-6
+    res = rbf if use_bf else r64
+    # dominant kernel: the four-index transforms inside the builds; the build time
+    # also holds the SVDs, co-densities and small GEMMs, so this is a lower bound
+    n = 100
+    bytes_tensor = 32.0 * n ** 4  # four passes, each reading and writing n^4 floats
+    alg = res["fp32_tensors"] * bytes_tensor + (res["rank4_tensors"] - res["fp32_tensors"]) * 2 * bytes_tensor
+    achieved = alg / (res["build_ms"] * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sysm, dets, tol, args.mode)
+    return {
+        "metric": "Wick H,S pair elements/s with the slab streamed through device memory (build -> evaluate -> "
+                  "reuse per group of reference pairs)",
+        "value": res["rate"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3+f64" if use_bf else "f64", "data": "synthetic",
+        "config": {"workload": "batched Wick elements, n_bas=nmo=100, nocc=10+10, 16 refs (1 with an alpha orbital "
+                               "orthogonal to 7 others), 512 singles/doubles per ref, 8192 dets, "
+                               f"{npairs} pairs, 136 upper-triangle reference pairs", "mode": args.mode},
+        "fp64": r64, "bf16x3": rbf, "dE_bf16x3_minus_fp64": dE,
+        "roofline": {"bound": "hbm", "kernel": "k_qtransform_bf3 / k_qtransform_mfma_pipe (whole builds: a lower "
+                     "bound for the transform kernels)", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "algorithmic_bytes": alg,
+                     "bytes_meaning": "32 n^4 bytes per reduced-precision rank-4 tensor (four passes over FP32), "
+                                      "64 n^4 per FP64 tensor"},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -977,11 +1074,12 @@ def main():
                     help="pt2 workload: imaginary shift of the shifted PT2 solves")
     ap.add_argument("--no-energy", action="store_true",
                     help="skip the NOCI energies (eigensolver) -- for profiler runs; then |dE| is not checked")
-    ap.add_argument("--workload", choices=["all", "wick", "pt2", "batch", "screen", "spawn", "copy"], default="all",
+    ap.add_argument("--workload", choices=["all", "wick", "pt2", "batch", "screen", "spawn", "stream", "copy"],
+                    default="all",
                     help="all (default): the headline wick line carrying compact runs of the other workloads; "
                          "wick: batched H,S elements (headline); pt2: NOCI-PT2 matvec; batch: uneven "
                          "multi-system batch; screen: learned SNOCI screening; spawn: QMC spawning requests; "
-                         "copy: counter calibration")
+                         "stream: 16-reference slab streamed through device memory; copy: counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the engine has no multi-GPU path in this build "
@@ -989,7 +1087,7 @@ def main():
         sys.exit(2)
     args.compact = False
     one = {"wick": wick_workload, "pt2": pt2_workload, "batch": batch_workload, "screen": screen_workload,
-           "spawn": spawn_workload, "copy": copy_workload}
+           "spawn": spawn_workload, "stream": stream_workload, "copy": copy_workload}
     if args.workload != "all":
         print(json.dumps(one[args.workload](args)), flush=True)
         return
@@ -1000,7 +1098,7 @@ def main():
     # roofline and CPU baseline, inside the same driver-timed run
     args.compact = True
     out["workloads"] = {}
-    for name in ("batch", "pt2", "screen", "spawn"):
+    for name in ("batch", "pt2", "screen", "spawn", "stream"):
         t0 = time.perf_counter()
         log(f"--- workload {name}")
         r = one[name](args)

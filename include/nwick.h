@@ -124,13 +124,17 @@ int nw_hs_matrix(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double *H, doub
  * of dets touches are sized from their zero-overlap counts, dealt in (x, w)
  * order into groups whose slab blocks fit in slab_bytes, and each group is
  * built, its elements evaluated into H, S, and its slab reused by the next.
- * slab_bytes = 0: free device memory less room for the build's temporaries.
+ * slab_bytes = 0: the context's current slab allocation plus free device
+ * memory, less room for the build's temporaries.  A slab allocation that
+ * fits the budget is kept and reused (as by nw_build); a larger one is freed.
  * A pair whose block alone exceeds the budget is NW_E_MEM.  H, S as in
- * nw_hs_matrix; ngroups (may be NULL) receives the number of groups.  The
- * context then holds the last group's build (as nw_build with that mask).
- * nw_last_timings: total, builds, evaluations. */
+ * nw_hs_matrix; meta_out (nref*nref entries or NULL) receives the meta of
+ * every pair built, as from nw_build; ngroups (may be NULL) the number of
+ * groups.  The context then holds the last group's build (as nw_build with
+ * that mask).  nw_last_timings: total, builds, evaluations. */
 int nw_hs_matrix_streamed(nw_ctx *ctx, double tol, const nw_det *dets, uint64_t ndet, double *H,
-                          double *S, int mode, uint64_t slab_bytes, int *ngroups);
+                          double *S, int mode, uint64_t slab_bytes, nw_pair_meta *meta_out,
+                          int *ngroups);
 
 /* Rebuild the generalised-Fock intermediates of every built pair from the
  * spin Fock matrices fa, fb (n_bas x n_bas): replaces update_wicks_fock

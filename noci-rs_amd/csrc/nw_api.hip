@@ -919,7 +919,7 @@ int nw_hs_matrix(nw_ctx *c, const nw_det *dets, uint64_t ndet, double *H, double
 }
 
 int nw_hs_matrix_streamed(nw_ctx *c, double tol, const nw_det *dets, uint64_t ndet, double *H, double *S, int mode,
-                          uint64_t slab_bytes, int *ngroups)
+                          uint64_t slab_bytes, nw_pair_meta *meta_out, int *ngroups)
 {
     return guarded(c, [&]() -> int {
         if (c->nref == 0) return fail(c, NW_E_STATE, "nw_set_refs first");
@@ -944,15 +944,13 @@ int nw_hs_matrix_streamed(nw_ctx *c, double tol, const nw_det *dets, uint64_t nd
         std::vector<int> pid;
         for (int p = 0; p < nref * nref; ++p)
             if (need[p]) pid.push_back(p);
-        // outputs first, then the budget: the slab of an earlier build is released
+        // outputs first, then the budget.  The slab allocation of an earlier build
+        // is kept when it fits the budget: the groups reuse it, and so do later
+        // calls (freeing and re-allocating hundreds of GB costs seconds, while the
+        // driver clears the freed memory).
         const bool adev = is_device_ptr(H), bdev = is_device_ptr(S);
         double *dA = adev ? H : c->outH.get<double>(ndet * ndet);
         double *dB = bdev ? S : c->outS.get<double>(ndet * ndet);
-        if (c->d_slab) {
-            HIPCHK(hipFree(c->d_slab));
-            c->d_slab = nullptr;
-            c->slab_cap = 0;
-        }
         uint64_t budget = slab_bytes;
         if (budget == 0) {
             size_t fr = 0, tot = 0;
@@ -960,8 +958,16 @@ int nw_hs_matrix_streamed(nw_ctx *c, double tol, const nw_det *dets, uint64_t nd
             // room for the build's temporaries: two transform scratch buffers of up to
             // 1 GiB each, the per-pair arenas and the FP32 integral copies
             const uint64_t n4 = (uint64_t)c->n * c->n * c->n * c->n;
-            const uint64_t reserve = (6ull << 30) + n4 * 8;
-            budget = fr > reserve ? (uint64_t)((double)(fr - reserve) * 0.9) : 0;
+            const uint64_t reserve = (6ull << 30) + n4 * 8, cap = (uint64_t)c->slab_cap * 8;
+            const double avail = (double)cap + (double)fr - (double)reserve;
+            // keep a slab that already holds most of what is available: growing it
+            // by a few percent would cost a free and a re-allocation
+            budget = (double)cap >= 0.85 * avail ? cap : (uint64_t)std::max(0.0, 0.9 * avail);
+        }
+        if (c->d_slab && (uint64_t)c->slab_cap * 8 > budget) {
+            HIPCHK(hipFree(c->d_slab));
+            c->d_slab = nullptr;
+            c->slab_cap = 0;
         }
         // pairs in (x, w) order, greedily into groups whose blocks fit the budget
         std::vector<std::vector<int>> groups;
@@ -984,7 +990,7 @@ int nw_hs_matrix_streamed(nw_ctx *c, double tol, const nw_det *dets, uint64_t nd
         for (const auto &g : groups) {
             std::fill(mask.begin(), mask.end(), 0);
             for (int p : g) mask[p] = 1;
-            rc = build_impl(c, tol, mask.data(), nullptr, nullptr);
+            rc = build_impl(c, tol, mask.data(), meta_out, nullptr);
             if (rc != NW_OK) return rc;
             build_ms += c->t_ms[0];
             // every element of the group's pairs; each pair is in exactly one group

@@ -89,7 +89,7 @@ def load():
     L.nw_s_same_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, ct.c_int, vp]
     L.nw_m_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, ct.c_double, vp, ct.c_int]
     L.nw_hs_matrix.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int]
-    L.nw_hs_matrix_streamed.argtypes = [vp, ct.c_double, vp, ct.c_uint64, vp, vp, ct.c_int, ct.c_uint64, vp]
+    L.nw_hs_matrix_streamed.argtypes = [vp, ct.c_double, vp, ct.c_uint64, vp, vp, ct.c_int, ct.c_uint64, vp, vp]
     L.nw_pair_slab.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
     L.nw_set_precision.argtypes = [vp, ct.c_int]
     L.nw_project_hs.argtypes = [vp, vp, ct.c_uint64, ct.c_uint64, vp, vp, vp, ct.c_int]
@@ -230,19 +230,26 @@ class Engine:
         self._check(self.L.nw_hs_matrix(self.ctx, _ptr(dets), n, _ptr(H), _ptr(S), int(mode)))
         return H, S
 
-    def hs_matrix_streamed(self, dets, tol=1e-8, mode=MODE_REFERENCE, slab_bytes=0, out=None):
+    def hs_matrix_streamed(self, dets, tol=1e-8, mode=MODE_REFERENCE, slab_bytes=0, out=None, meta=None):
         """Build and evaluate in groups of reference pairs whose slab blocks fit
         in slab_bytes (0: what device memory allows); returns H, S and the
-        number of groups."""
+        number of groups.  meta: a dict that receives {(x, w): PairMeta} of
+        every pair built."""
         n = len(dets) if not hasattr(dets, "data_ptr") else dets.numel() * dets.element_size() // DET_DTYPE.itemsize
         if out is None:
             H = np.empty((n, n)); S = np.empty((n, n))
         else:
             H, S = out
         groups = ct.c_int()
+        pm = (PairMeta * (self.nref * self.nref))()
+        ct.memset(pm, 0xFF, ct.sizeof(pm))  # ma = -1 marks pairs not built
         sync_torch(dets, H, S)
         self._check(self.L.nw_hs_matrix_streamed(self.ctx, float(tol), _ptr(dets), n, _ptr(H), _ptr(S), int(mode),
-                                                 int(slab_bytes), ct.byref(groups)))
+                                                 int(slab_bytes), ct.cast(pm, ct.c_void_p), ct.byref(groups)))
+        if meta is not None:
+            for p in range(self.nref * self.nref):
+                if pm[p].ma >= 0:
+                    meta[divmod(p, self.nref)] = pm[p]
         return H, S, groups.value
 
     def hs_batched(self, dets, req, mode=MODE_REFERENCE):

@@ -77,9 +77,13 @@ def test_streamed_mixed_precision_equals_full_build(small):
     blk = [int(eng.pair_slab(x, w)[1][94]) * 8 for x in range(nref) for w in range(x, nref)]
     # FP32 pairs keep half-length rank-4 blocks: a budget of the largest FP64
     # block holds at least one pair
-    H, S, groups = eng.hs_matrix_streamed(rec, tol=1e-8, slab_bytes=max(blk))
+    smeta = {}
+    H, S, groups = eng.hs_matrix_streamed(rec, tol=1e-8, slab_bytes=max(blk), meta=smeta)
     print(f"bf16x3 streamed: {groups} groups, max|dH| {np.max(np.abs(H - H0)):.2e}", flush=True)
     assert groups > 1
+    assert sorted(smeta) == [(x, w) for x in range(nref) for w in range(x, nref)]
+    for k, m in smeta.items():
+        assert (m.ma, m.mb, m.fp32, m.sigma_min) == (meta[k].ma, meta[k].mb, meta[k].fp32, meta[k].sigma_min)
     assert np.max(np.abs(H - H0)) < 1e-12 * max(1.0, np.max(np.abs(H0))) and np.max(np.abs(S - S0)) < 1e-12
     eng.close()
 
