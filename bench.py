@@ -112,6 +112,8 @@ def cpu_baseline(sysm, dets, tol, mode, with_build=True):
                              ct.c_double, ct.c_int, O.D, O.D]
     L.ow_num_threads.restype = ct.c_int
     threads = L.ow_num_threads()
+    # the build's quarter passes on OpenBLAS dgemm, the reference's GEMM library
+    blas = O.use_openblas(True) if with_build else None
     t0 = time.perf_counter()
     slab = O.Slab(sysm, tol=tol, pairs=[(0, 1)])
     t1 = time.perf_counter()
@@ -147,8 +149,10 @@ def cpu_baseline(sysm, dets, tol, mode, with_build=True):
     ma, mb = slab.m[(0, 1)]
     fl = transform_flops(sysm.n_bas, ma, mb)
     return dict({"value": npair / secs, "unit": "pairs/s", "cores": threads, "kind": "port",
-                 "sample": f"oracle build of ordered reference pair (0,1) ({t1 - t0:.1f} s) + evaluation of its "
-                           f"{len(bra)}x{len(ket)} determinant pairs ({t3 - t2:.2f} s), OpenMP threads={threads}",
+                 "sample": f"oracle build of ordered reference pair (0,1) ({t1 - t0:.1f} s; four-index transforms "
+                           f"on {blas} dgemm) + evaluation of its {len(bra)}x{len(ket)} determinant pairs "
+                           f"({t3 - t2:.2f} s), OpenMP threads={threads}",
+                 "build_gemm": blas,
                  "build_transform_gflops": fl / (t1 - t0) / 1e9,
                  "openblas_quarter_pass_gflops": openblas_quarter_pass(sysm.n_bas)}, **host)
 
@@ -592,6 +596,7 @@ def screen_cpu_baseline(sysm, cand, mode):
     L.ow_hs_many.argtypes = [O.D, O.I64, O.D, ct.c_int, ct.c_longlong, O.U64, O.U64, O.D, ct.c_double,
                              ct.c_double, ct.c_int, O.D, O.D]
     L.ow_num_threads.restype = ct.c_int
+    blas = O.use_openblas(True)
     t0 = time.perf_counter()
     slab = O.Slab(sysm, tol=1e-8, pairs=[(0, r) for r in range(sysm.nref)])
     t1 = time.perf_counter()
@@ -620,9 +625,9 @@ def screen_cpu_baseline(sysm, cand, mode):
     t3 = time.perf_counter()
     threads = L.ow_num_threads()
     return dict({"value": m / ((t1 - t0) + (t3 - t2)), "unit": "candidates/s", "cores": threads, "kind": "port",
-                 "sample": f"oracle build of the 16 ordered pairs (0, r) ({t1 - t0:.1f} s) + exact elements of "
-                           f"{m} candidates of reference 0 against the 16 references and their diagonals "
-                           f"({t3 - t2:.2f} s), OpenMP threads={threads}"}, **host_info())
+                 "sample": f"oracle build of the 16 ordered pairs (0, r) ({t1 - t0:.1f} s; transforms on {blas} "
+                           f"dgemm) + exact elements of {m} candidates of reference 0 against the 16 references "
+                           f"and their diagonals ({t3 - t2:.2f} s), OpenMP threads={threads}"}, **host_info())
 
 
 def spawn_workload(args):

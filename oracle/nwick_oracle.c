@@ -257,8 +257,27 @@ int ow_svd(int n, const double *A, double *U, double *s, double *V)
  * next quarter again contracts the last index. */
 #define QP_RB 64
 typedef double v4d __attribute__((vector_size(32)));
+
+/* An external BLAS dgemm (Fortran calling convention) for the quarter passes:
+ * the reference computes its transforms with ndarray GEMMs on OpenBLAS
+ * (ndarray-linalg 0.15 with openblas-system), so the CPU baseline can time the
+ * build with that library instead of the loops below.  NULL: built-in loops. */
+typedef void (*ow_dgemm_fn)(const char *ta, const char *tb, const int *m, const int *n, const int *k,
+                            const double *alpha, const double *a, const int *lda, const double *b, const int *ldb,
+                            const double *beta, double *c, const int *ldc);
+static ow_dgemm_fn ext_dgemm = NULL;
+void ow_set_dgemm(void *fn) { ext_dgemm = (ow_dgemm_fn)fn; }
+
 static void qpass(long long rows, int nb, int nm, const double *X, const double *c, double *Y)
 {
+    if (ext_dgemm && rows < 0x7fffffffLL) {
+        /* column-major: Y^T (rows x nm, ld rows) = X^T^T c^T^T with X^T (nb x rows)
+         * and c^T (nm x nb) the row-major operands read column-major */
+        const int m = (int)rows, n = nm, k = nb, lda = nb, ldb = nm, ldc = (int)rows;
+        const double one = 1.0, zero = 0.0;
+        ext_dgemm("T", "T", &m, &n, &k, &one, X, &lda, c, &ldb, &zero, Y, &ldc);
+        return;
+    }
     const int np = (nm + 7) & ~7;
     double *cp = (double *)calloc((size_t)nb * np, sizeof(double));
     for (int l = 0; l < nb; ++l)

@@ -47,10 +47,37 @@ def lib():
         L.ow_fs.argtypes = [D, I64, D, ct.c_int, U64, U64, ct.c_double, ct.c_double, ct.c_double, ct.c_int, D, D]
         L.ow_fs_spin.argtypes = [D, I64, D, ct.c_int, ct.c_int, U64, U64, U64, U64, ct.c_double, ct.c_double,
                                  ct.c_int, D, D]
+        L.ow_set_dgemm.argtypes = [ct.c_void_p]
         L.ow_meta_len.restype = ct.c_int
         L.ow_offsets_len.restype = ct.c_int
         _lib = L
     return _lib
+
+
+def use_openblas(enable=True):
+    """Quarter passes of the oracle's four-index transforms through OpenBLAS
+    dgemm (scipy's bundled build, reached through scipy.linalg.cython_blas),
+    the GEMM library the reference links; enable=False restores the built-in
+    loops.  Returns a description of the library and its thread count."""
+    if not enable:
+        lib().ow_set_dgemm(None)
+        return None
+    from scipy.linalg import cython_blas
+    cap = cython_blas.__pyx_capi__["dgemm"]
+    get_name = ct.pythonapi.PyCapsule_GetName
+    get_name.restype, get_name.argtypes = ct.c_char_p, [ct.py_object]
+    get_ptr = ct.pythonapi.PyCapsule_GetPointer
+    get_ptr.restype, get_ptr.argtypes = ct.c_void_p, [ct.py_object, ct.c_char_p]
+    lib().ow_set_dgemm(get_ptr(cap, get_name(cap)))
+    info = "OpenBLAS (scipy)"
+    try:
+        from threadpoolctl import threadpool_info
+        for t in threadpool_info():
+            if t.get("internal_api") == "openblas" and "scipy" in t.get("filepath", ""):
+                info = f"OpenBLAS {t.get('version')} (scipy), {t.get('num_threads')} threads"
+    except Exception:
+        pass
+    return info
 
 
 # Literal readings of the reference's m>0 arithmetic (oracle/nwick_oracle.c RD_*).
