@@ -242,6 +242,8 @@ int nw_hs_batched_noalloc(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const 
 int nw_last_timings(nw_ctx *ctx, double *ms3);
 
 const char *nw_last_error(nw_ctx *ctx);
+/* Frees the context and every factor object created on it (nw_factor_create);
+ * such a factor must not be used or passed to nw_factor_destroy afterwards. */
 void nw_destroy(nw_ctx *ctx);
 
 /* ---------------------------------------------------------------------------
@@ -312,6 +314,11 @@ int nw_factor_set_precision(nw_factor *f, int precision);
  * for both directions of a parent pair (Tab(p, q) = Tab(q, p)^T) once.  No
  * counterpart in the reference; the algorithmic traffic of the apply. */
 int nw_factor_apply_bytes(nw_factor *f, uint64_t *bytes);
+/* Source groups whose stage-1 sums run as dense products on FP64 MFMA (the
+ * single x single block of each parent; environment NW_FAC_BLOCK=0 turns them
+ * off, NW_FAC_BLOCK_MIN_G sets the smallest block, default 16 groups), and the
+ * number of such blocks.  Diagnostics; no counterpart in the reference. */
+int nw_factor_block_groups(const nw_factor *f, uint64_t *groups, uint64_t *blocks);
 /* Per-determinant component ids (SpinFactorisation::aid/bid, mod.rs:106-130). */
 int nw_factor_ids(const nw_factor *f, uint32_t *aids, uint32_t *bids);
 /* Copy one table: ncomp(source) x ncomp(target) interleaved {S, F} doubles,

@@ -1301,6 +1301,11 @@ int nw_last_timings(nw_ctx *c, double *ms3)
 
 const char *nw_last_error(nw_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
-void nw_destroy(nw_ctx *c) { delete c; }
+void nw_destroy(nw_ctx *c)
+{
+    if (!c) return;
+    destroy_factors(c);
+    delete c;
+}
 
 }  // extern "C"

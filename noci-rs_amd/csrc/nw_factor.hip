@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -94,6 +95,17 @@ struct raw2<double2> {
 template <>
 struct raw2<float2> {
     typedef float __attribute__((ext_vector_type(2))) type;
+};
+
+template <class T2>
+struct elem_of;
+template <>
+struct elem_of<double2> {
+    typedef double type;
+};
+template <>
+struct elem_of<float2> {
+    typedef float type;
 };
 
 template <class T2>
@@ -286,6 +298,136 @@ __global__ __launch_bounds__(256) void k_fac_rows(const uint32_t *__restrict__ t
     if (live) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) part2[((size_t)blockIdx.y * nt + I) * NV + v] = make_double2(m[v], s[v]);
+    }
+}
+
+// Stage 1 for a block of groups of one source parent and side whose entries
+// index few components (K <= BLK_KMAX): U = W T on FP64 MFMA, with W the
+// groups x components matrix of the entries' x weights (zero where a group
+// has no entry) and T the components' table rows, read once per block
+// instead of once per entry.  Output rows as k_fac_mat's: group g's row at
+// out + g nt NV, {S, F} per target component and vector.
+struct BlkItem {
+    const void *tab;        // table [source component][target component]
+    const uint32_t *rows;   // the block's K source components (table rows)
+    const double *w;        // W [NV][G][Kp]
+    void *out;
+    int nt, G, K, Kp;
+};
+// Dense weights of one block, per apply: w[(v G + g) Kp + k] = x[d[g Kp + k] NV + v] (0 where d < 0).
+struct BlkDense {
+    const int *d;
+    double *w;
+    int G, Kp;
+};
+constexpr int BLK_GMAX = 128;  // groups per block (8 row blocks of 16)
+constexpr int BLK_KMAX = 256;  // components per block
+constexpr int BLK_KC = 16;     // components per LDS stage
+constexpr int BLK_NC = 128;    // real columns ({S, F} of 64 target components) per workgroup
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <int NV>
+__global__ void k_fac_densify(const BlkDense *__restrict__ blks, const double *__restrict__ x)
+{
+    const BlkDense b = blks[blockIdx.y];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= b.G * b.Kp) return;
+    const int d = b.d[e], g = e / b.Kp, k = e % b.Kp;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) b.w[((size_t)v * b.G + g) * b.Kp + k] = d >= 0 ? x[(size_t)d * NV + v] : 0.0;
+}
+
+// One workgroup: all G groups x BLK_NC real columns of one block; each of the
+// 4 waves owns 32 columns (two 16-column MFMA blocks) and every row block,
+// v_mfma_f64_16x16x4f64 over K in LDS stages of BLK_KC; the next stage is
+// loaded into registers while the current one is multiplied.  FP64 sums
+// whatever the table type.  Used by one-vector applies: with two vectors the
+// per-group sums read each row once for both and measured faster.
+template <class T2, int NV>
+__global__ __launch_bounds__(256, NV == 1 ? 2 : 1) void k_fac_block(const BlkItem *__restrict__ items)
+{
+    constexpr int NCOMP = BLK_NC / 2, TPT = BLK_KC * NCOMP / 256, WPT = BLK_GMAX * BLK_KC / 256;
+    __shared__ double Ws[NV][BLK_GMAX][BLK_KC + 1];
+    __shared__ double Ts[BLK_KC][BLK_NC + 2];
+    const BlkItem it = items[blockIdx.y];
+    const int ncol = 2 * it.nt, n0 = blockIdx.x * BLK_NC;
+    if (n0 >= ncol) return;  // uniform over the workgroup, before any barrier
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+    const int nms = (it.G + 15) >> 4;
+    typedef typename raw2<T2>::type R2;
+    const gptr<R2> tab = (gptr<R2>)it.tab;
+    R2 tr[TPT];
+    double wr[NV][WPT];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < TPT; ++j) {
+            const int e = tid + j * 256, k = e / NCOMP, t = n0 / 2 + e % NCOMP;
+            tr[j] = (k0 + k < it.K && t < it.nt) ? tab[(size_t)it.rows[k0 + k] * it.nt + t] : R2{0, 0};
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int j = 0; j < WPT; ++j) {
+                const int e = tid + j * 256, g = e / BLK_KC, k = e % BLK_KC;
+                wr[v][j] = (g < it.G && k0 + k < it.K) ? it.w[((size_t)v * it.G + g) * it.Kp + k0 + k] : 0.0;
+            }
+    };
+    f64x4 acc[NV][BLK_GMAX / 16][2];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int ms = 0; ms < BLK_GMAX / 16; ++ms) acc[v][ms][0] = acc[v][ms][1] = f64x4{0.0, 0.0, 0.0, 0.0};
+    fetch(0);
+    for (int k0 = 0; k0 < it.K; k0 += BLK_KC) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < TPT; ++j) {
+            const int e = tid + j * 256, k = e / NCOMP, cc = e % NCOMP;
+            Ts[k][2 * cc] = (double)tr[j].x;
+            Ts[k][2 * cc + 1] = (double)tr[j].y;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int j = 0; j < WPT; ++j) {
+                const int e = tid + j * 256;
+                Ws[v][e / BLK_KC][e % BLK_KC] = wr[v][j];
+            }
+        __syncthreads();
+        if (k0 + BLK_KC < it.K) fetch(k0 + BLK_KC);  // the next stage in flight during the products
+#pragma unroll
+        for (int kk = 0; kk < BLK_KC / 4; ++kk) {
+            const double b0 = Ts[kk * 4 + lk][wave * 32 + lr], b1 = Ts[kk * 4 + lk][wave * 32 + 16 + lr];
+#pragma unroll
+            for (int ms = 0; ms < BLK_GMAX / 16; ++ms)
+                if (ms < nms) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        const double a = Ws[v][ms * 16 + lr][kk * 4 + lk];
+                        acc[v][ms][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[v][ms][0], 0, 0, 0);
+                        acc[v][ms][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[v][ms][1], 0, 0, 0);
+                    }
+                }
+        }
+    }
+    typedef typename elem_of<T2>::type T;
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+        const int col = n0 + wave * 32 + nb * 16 + lr;
+        if (col >= ncol) continue;
+        const int t = col >> 1, part = col & 1;
+#pragma unroll
+        for (int ms = 0; ms < BLK_GMAX / 16; ++ms)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int g = ms * 16 + 4 * r + lk;
+                if (g < it.G)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        T *o = (T *)((T2 *)it.out + ((size_t)g * it.nt + t) * NV + v);
+                        o[part] = (T)acc[v][ms][nb][r];
+                    }
+            }
     }
 }
 
@@ -698,8 +840,18 @@ struct DualSel {
 // uploaded arrays) and its materialisation footprint.
 struct PairPlan {
     int q = -1, p = -1;
-    long long item0 = 0, item1 = 0, row0 = 0, row1 = 0;
+    long long item0 = 0, item1 = 0, row0 = 0, row1 = 0, blk0 = 0, blk1 = 0;
     size_t mat_elems = 0;
+};
+
+// A block of one source parent's groups (one side) whose stage 1 is a dense
+// product (k_fac_block): the groups (indices into P.groups), the sorted
+// components their entries index, and offsets of its component list, entry
+// determinant matrix (G x Kp, -1 = none) and dense weights (2 x G x Kp).
+struct BlkSrc {
+    int p = -1, side = 0, G = 0, K = 0, Kp = 0;
+    std::vector<int> groups;
+    long long row_off = 0, d_off = 0, w_off = 0;
 };
 
 }  // namespace
@@ -724,7 +876,7 @@ struct nw_factor {
     DVec<S1Item> d_items[2][2];
     DVec<S2Row> d_rows[2][2];
     bool planned[2][2] = {{false, false}, {false, false}};
-    std::vector<PairPlan> plans;  // ordered by target, then source
+    std::vector<PairPlan> plans[2][2];  // per plan (lo, nv): ordered by target, then source
     std::vector<uint32_t> h_midx;  // host copy of the summed-component indices
     size_t mat_elems = 0;          // materialisation entries of one vector
     DVec<double2> d_mat, d_part2;  // d_mat holds double2 or float2 rows
@@ -732,6 +884,15 @@ struct nw_factor {
     // determinants, per-plan items and the partial-sum buffer
     std::vector<DualSel> duals;
     std::map<long long, std::pair<int, int>> dual_of;  // (target, source, group) -> (dual, 0 fwd / 1 rev)
+    // dense-product blocks of source groups: per parent, group -> (block, position) or (-1, -1)
+    std::vector<BlkSrc> blks;
+    std::vector<std::vector<std::pair<int, int>>> blk_of;
+    DVec<uint32_t> d_brow;
+    DVec<int> d_bD;
+    DVec<double> d_bW;
+    DVec<BlkDense> d_bdense;
+    DVec<BlkItem> d_blk[2][2];
+    int blk_gmax = 0, blk_kmax = 0;
     DVec<int> d_dualdet;
     DVec<DualItem> d_dual[2][2];
     DVec<double2> d_dpart;
@@ -843,7 +1004,11 @@ void make_plans(nw_factor *f, int lo, int nv)
     const size_t es = elem_bytes(lo);
     const char *tbase = dev && lo ? (const char *)f->d_tab32.p : (const char *)f->d_tab.p;
     char *mbase = (char *)f->d_mat.p;
-    f->plans.clear();
+    std::vector<BlkItem> blk_items;
+    // dense-product blocks in one-vector plans; two-vector plans keep the per-group sums
+    const bool use_blk = nv == 1;
+    std::vector<PairPlan> &plans = f->plans[lo][nv - 1];
+    plans.clear();
     // dual passes: their materialised rows open the buffer, then the rows of
     // one target at a time
     std::vector<DualItem> dual_items;
@@ -910,6 +1075,22 @@ void make_plans(nw_factor *f, int lo, int nv)
             size_t mo = dev ? mat_q : 0;
             pl.item0 = (long long)items.size();
             pl.row0 = (long long)rows.size();
+            // dense-product blocks of p: their groups' rows, contiguous per block
+            pl.blk0 = (long long)blk_items.size();
+            std::vector<char *> blk_out;
+            for (const BlkSrc &b : f->blks) {
+                if (!use_blk || b.p != p) {
+                    blk_out.push_back(nullptr);
+                    continue;
+                }
+                const int len = b.side == 1 ? nta : ntb;
+                char *out = mbase + es * mo;
+                blk_out.push_back(out);
+                blk_items.push_back({b.side == 1 ? taba : tabb, f->d_brow.p + b.row_off, f->d_bW.p + b.w_off, out, len,
+                                     b.G, b.K, b.Kp});
+                mo += (size_t)b.G * len * nv;
+            }
+            pl.blk1 = (long long)blk_items.size();
             for (int gi = 0; gi < (int)P.groups.size(); ++gi) {
                 const Group &g = P.groups[gi];
                 const auto du = dev ? f->dual_of.find(dual_key(f, q, p, gi)) : f->dual_of.end();
@@ -933,6 +1114,13 @@ void make_plans(nw_factor *f, int lo, int nv)
                     continue;
                 }
                 const int len = g.side == 1 ? nta : ntb;
+                const auto bo = f->blk_of[p][gi];
+                if (use_blk && bo.first >= 0) {
+                    char *out = blk_out[bo.first] + es * ((size_t)bo.second * len * nv);
+                    if (g.side == 1) rows.push_back({out, tabb + es * (size_t)g.key * ntb, -1, 1, 0, {0, 0}});
+                    else rows.push_back({taba + es * (size_t)g.key * nta, out, -1, 0, 1, {0, 0}});
+                    continue;
+                }
                 for (int e = g.m0; e < g.m1; e += S1_CHUNK) {
                     char *out = mbase + es * mo;
                     items.push_back({g.side == 1 ? taba : tabb, out, len, (int)(P.m_off + e),
@@ -947,7 +1135,7 @@ void make_plans(nw_factor *f, int lo, int nv)
             pl.mat_elems = mo - (dev ? mat_q : 0);
             if (dev) mat_q = mo;
             else mat_need = std::max(mat_need, pl.mat_elems);
-            f->plans.push_back(pl);
+            plans.push_back(pl);
         }
         if (dev) mat_need = std::max(mat_need, mat_q);
     }
@@ -962,6 +1150,7 @@ void make_plans(nw_factor *f, int lo, int nv)
     f->apply_bytes[lo] = bytes;
     f->d_items[lo][nv - 1].upload(items);
     f->d_rows[lo][nv - 1].upload(rows);
+    f->d_blk[lo][nv - 1].upload(blk_items);
     f->d_dual[lo][nv - 1].upload(dual_items);
     f->planned[lo][nv - 1] = true;
 }
@@ -1092,6 +1281,86 @@ void select_duals(nw_factor *f, const std::vector<uint32_t> &midx, const std::ve
         }
 }
 
+// Dense-product blocks: per source parent and side, the multi-entry groups of
+// at most BLK_KMAX / 2 entries that no dual pass serves, in chunks of at most
+// BLK_GMAX groups, when a chunk indexes at most BLK_KMAX components, holds at
+// least NW_FAC_BLOCK_MIN_G groups (default 16) and fills at least 15% of its
+// groups x components matrix (the single x single block of an arrow-shaped
+// PT2 space).
+// NW_FAC_BLOCK=0 keeps every group on the per-group sums.
+void select_blocks(nw_factor *f, const std::vector<uint32_t> &midx, const std::vector<uint32_t> &mdet)
+{
+    f->blk_of.assign(f->nref, {});
+    for (int p = 0; p < f->nref; ++p) f->blk_of[p].assign(f->par[p].groups.size(), {-1, -1});
+    const char *on = std::getenv("NW_FAC_BLOCK");
+    if (on && std::atoi(on) == 0) return;
+    const char *mg = std::getenv("NW_FAC_BLOCK_MIN_G");
+    const int min_g = std::max(1, mg ? std::atoi(mg) : 16);
+    std::vector<uint32_t> rows;
+    std::vector<int> dmat;
+    for (int p = 0; p < f->nref; ++p) {
+        const ParentSpace &P = f->par[p];
+        if (P.nt == 0) continue;
+        for (int side = 0; side < 2; ++side) {
+            std::vector<int> cand;
+            for (int gi = 0; gi < (int)P.groups.size(); ++gi) {
+                const Group &g = P.groups[gi];
+                // large groups (a reference row or column) stay on the per-group sums
+                if (g.m0 < 0 || g.side != side || g.m1 - g.m0 > BLK_KMAX / 2) continue;
+                bool dual = false;
+                for (int q = 0; q < f->nref && !dual; ++q) dual = f->dual_of.count(dual_key(f, q, p, gi)) > 0;
+                if (!dual) cand.push_back(gi);
+            }
+            for (size_t c0 = 0; c0 < cand.size(); c0 += BLK_GMAX) {
+                BlkSrc b;
+                b.p = p;
+                b.side = side;
+                b.groups.assign(cand.begin() + c0, cand.begin() + std::min(cand.size(), c0 + BLK_GMAX));
+                std::vector<uint32_t> comp;
+                long long nnz = 0;
+                for (int gi : b.groups) {
+                    const Group &g = P.groups[gi];
+                    nnz += g.m1 - g.m0;
+                    for (int e = g.m0; e < g.m1; ++e) comp.push_back(midx[P.m_off + e]);
+                }
+                std::sort(comp.begin(), comp.end());
+                comp.erase(std::unique(comp.begin(), comp.end()), comp.end());
+                b.G = (int)b.groups.size();
+                b.K = (int)comp.size();
+                if (b.G < min_g || b.K > BLK_KMAX || 20 * nnz < 3ll * b.G * b.K) continue;
+                b.Kp = (b.K + 1) & ~1;
+                b.row_off = (long long)rows.size();
+                rows.insert(rows.end(), comp.begin(), comp.end());
+                b.d_off = (long long)dmat.size();
+                dmat.resize(dmat.size() + (size_t)b.G * b.Kp, -1);
+                for (int j = 0; j < b.G; ++j) {
+                    const Group &g = P.groups[b.groups[j]];
+                    for (int e = g.m0; e < g.m1; ++e) {
+                        const int k = (int)(std::lower_bound(comp.begin(), comp.end(), midx[P.m_off + e]) - comp.begin());
+                        dmat[b.d_off + (size_t)j * b.Kp + k] = (int)mdet[P.m_off + e];
+                    }
+                    f->blk_of[p][b.groups[j]] = {(int)f->blks.size(), j};
+                }
+                f->blk_gmax = std::max(f->blk_gmax, b.G);
+                f->blk_kmax = std::max(f->blk_kmax, b.Kp);
+                f->blks.push_back(std::move(b));
+            }
+        }
+    }
+    if (f->blks.empty()) return;
+    size_t wn = 0;
+    for (BlkSrc &b : f->blks) {
+        b.w_off = (long long)wn;
+        wn += 2 * (size_t)b.G * b.Kp;  // room for two vectors
+    }
+    f->d_brow.upload(rows);
+    f->d_bD.upload(dmat);
+    f->d_bW.alloc(wn);
+    std::vector<BlkDense> dense;
+    for (const BlkSrc &b : f->blks) dense.push_back({f->d_bD.p + b.d_off, f->d_bW.p + b.w_off, b.G, b.Kp});
+    f->d_bdense.upload(dense);
+}
+
 }  // namespace
 
 extern "C" int nw_factor_create(nw_ctx *c, const nw_det *dets, uint64_t ndet, int mode, int storage,
@@ -1185,6 +1454,7 @@ extern "C" int nw_factor_create(nw_ctx *c, const nw_det *dets, uint64_t ndet, in
             select_duals(f.get(), midx, mdet, dd);
             f->d_dualdet.upload(dd);
         }
+        select_blocks(f.get(), midx, mdet);
         // ---- tables
         const int nref = c->nref;
         f->tab_off.assign((size_t)nref * nref * 2, -1);
@@ -1214,6 +1484,7 @@ extern "C" int nw_factor_create(nw_ctx *c, const nw_det *dets, uint64_t ndet, in
         }
         HIPCHK(hipStreamSynchronize(c->st));
         *out = f.release();
+        c->factors.insert(*out);
         return NW_OK;
     });
 }
@@ -1237,8 +1508,8 @@ namespace {
 // Stage 1 over items [i0, i1), stage 2 over rows [r0, r1) for the targets of
 // parent q, on tables of type T2 (plan lo) and NV vectors.
 template <class T2, int NV>
-void run_range(nw_factor *f, int lo, int q, long long i0, long long i1, long long r0, long long r1, const double *dx,
-               double lambda, double *dy, double *dys)
+void run_range(nw_factor *f, int lo, int q, long long i0, long long i1, long long b0, long long b1, long long r0,
+               long long r1, const double *dx, double lambda, double *dy, double *dys)
 {
     nw_ctx *c = f->c;
     const ParentSpace &Q = f->par[q];
@@ -1249,6 +1520,12 @@ void run_range(nw_factor *f, int lo, int q, long long i0, long long i1, long lon
         hipLaunchKernelGGL((k_fac_mat<T2, NV>), dim3(gx, ni), dim3(256), 0, c->st, f->d_items[lo][NV - 1].p + b,
                            f->d_midx.p, f->d_mdet.p, dx);
     }
+    if constexpr (NV == 1)
+        for (long long b = b0; b < b1; b += 65535) {
+            const unsigned ni = (unsigned)std::min<long long>(65535, b1 - b);
+            hipLaunchKernelGGL((k_fac_block<T2, NV>), dim3(blocks_of(2ll * maxlen, BLK_NC), ni), dim3(256), 0, c->st,
+                               f->d_blk[lo][NV - 1].p + b);
+        }
     const int nr = (int)(r1 - r0);
     if (nr == 0) return;
     const long long waves = (Q.nt + 63) / 64;
@@ -1288,13 +1565,22 @@ void run_duals_any(nw_factor *f, int lo, int nv, const double *dx)
     else run_duals<double2, 1>(f, 0, dx);
 }
 
-void run_range_any(nw_factor *f, int lo, int nv, int q, long long i0, long long i1, long long r0, long long r1,
-                   const double *dx, double lambda, double *dy, double *dys)
+void run_range_any(nw_factor *f, int lo, int nv, int q, long long i0, long long i1, long long b0, long long b1,
+                   long long r0, long long r1, const double *dx, double lambda, double *dy, double *dys)
 {
-    if (lo && nv == 2) run_range<float2, 2>(f, 1, q, i0, i1, r0, r1, dx, lambda, dy, dys);
-    else if (lo) run_range<float2, 1>(f, 1, q, i0, i1, r0, r1, dx, lambda, dy, dys);
-    else if (nv == 2) run_range<double2, 2>(f, 0, q, i0, i1, r0, r1, dx, lambda, dy, dys);
-    else run_range<double2, 1>(f, 0, q, i0, i1, r0, r1, dx, lambda, dy, dys);
+    if (lo && nv == 2) run_range<float2, 2>(f, 1, q, i0, i1, b0, b1, r0, r1, dx, lambda, dy, dys);
+    else if (lo) run_range<float2, 1>(f, 1, q, i0, i1, b0, b1, r0, r1, dx, lambda, dy, dys);
+    else if (nv == 2) run_range<double2, 2>(f, 0, q, i0, i1, b0, b1, r0, r1, dx, lambda, dy, dys);
+    else run_range<double2, 1>(f, 0, q, i0, i1, b0, b1, r0, r1, dx, lambda, dy, dys);
+}
+
+// Dense block weights from x, once per apply.
+void run_densify(nw_factor *f, int nv, const double *dx)
+{
+    if (f->blks.empty() || nv != 1) return;
+    const dim3 grid(blocks_of((long long)f->blk_gmax * f->blk_kmax, 256), (unsigned)f->blks.size());
+    if (nv == 2) hipLaunchKernelGGL(k_fac_densify<2>, grid, dim3(256), 0, f->c->st, f->d_bdense.p, dx);
+    else hipLaunchKernelGGL(k_fac_densify<1>, grid, dim3(256), 0, f->c->st, f->d_bdense.p, dx);
 }
 
 }  // namespace
@@ -1319,24 +1605,27 @@ int apply_nv(nw_factor *f, int nv, double lambda, const double *x, double *y, do
         hipLaunchKernelGGL(k_fac_zero, dim3(blocks_of(n, 256)), dim3(256), 0, c->st, dy, n);
         if (dys) hipLaunchKernelGGL(k_fac_zero, dim3(blocks_of(n, 256)), dim3(256), 0, c->st, dys, n);
         float regen = 0.f;
+        run_densify(f, nv, dx);
         if (f->storage == NW_FACTOR_DEVICE) {
             run_duals_any(f, lo, nv, dx);
             // the pairs of one target are contiguous in the plan
-            for (size_t k = 0; k < f->plans.size();) {
+            const std::vector<PairPlan> &plans = f->plans[lo][nv - 1];
+            for (size_t k = 0; k < plans.size();) {
                 size_t j = k;
-                while (j < f->plans.size() && f->plans[j].q == f->plans[k].q) ++j;
-                run_range_any(f, lo, nv, f->plans[k].q, f->plans[k].item0, f->plans[j - 1].item1, f->plans[k].row0,
-                              f->plans[j - 1].row1, dx, lambda, dy, dys);
+                while (j < plans.size() && plans[j].q == plans[k].q) ++j;
+                run_range_any(f, lo, nv, plans[k].q, plans[k].item0, plans[j - 1].item1, plans[k].blk0,
+                              plans[j - 1].blk1, plans[k].row0, plans[j - 1].row1, dx, lambda, dy, dys);
                 k = j;
             }
         } else {
-            for (const PairPlan &pl : f->plans) {
+            for (const PairPlan &pl : f->plans[lo][nv - 1]) {
                 HIPCHK(hipEventRecord(c->ev[2], c->st));
                 char *slot = (char *)f->d_tab.p;
                 launch_table(f, pl.q, pl.p, 0, slot, nullptr, lo);
                 launch_table(f, pl.q, pl.p, 1, slot + elem_bytes(lo) * table_elems(f, pl.q, pl.p, 0), nullptr, lo);
                 HIPCHK(hipEventRecord(c->ev[3], c->st));
-                run_range_any(f, lo, nv, pl.q, pl.item0, pl.item1, pl.row0, pl.row1, dx, lambda, dy, dys);
+                run_range_any(f, lo, nv, pl.q, pl.item0, pl.item1, pl.blk0, pl.blk1, pl.row0, pl.row1, dx, lambda, dy,
+                              dys);
                 HIPCHK(hipStreamSynchronize(c->st));
                 regen += elapsed(c->ev[2], c->ev[3]);
             }
@@ -1403,6 +1692,16 @@ extern "C" int nw_factor_info(const nw_factor *f, int parent, uint64_t *ncomp_a,
         if (ncomp_b) *ncomp_b = (uint64_t)f->par[parent].ncomp[1];
     }
     if (table_bytes) *table_bytes = (uint64_t)f->tab_elems * elem_bytes(f->prec == NW_PREC_MIXED);
+    return NW_OK;
+}
+
+extern "C" int nw_factor_block_groups(const nw_factor *f, uint64_t *groups, uint64_t *blocks)
+{
+    if (!f) return NW_E_ARG;
+    uint64_t g = 0;
+    for (const BlkSrc &b : f->blks) g += (uint64_t)b.G;
+    if (groups) *groups = g;
+    if (blocks) *blocks = (uint64_t)f->blks.size();
     return NW_OK;
 }
 
@@ -1493,10 +1792,18 @@ extern "C" void nw_factor_destroy(nw_factor *f)
 {
     if (!f) return;
     nw_ctx *c = f->c;
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        (void)hipSetDevice(c->device);
-        (void)hipStreamSynchronize(c->st);
-    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->st);
+    c->factors.erase(f);
     delete f;
+}
+
+void nwi::destroy_factors(nw_ctx *c)
+{
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->st);
+    for (nw_factor *f : c->factors) delete f;
+    c->factors.clear();
 }

@@ -7,6 +7,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -77,6 +78,11 @@ inline bool is_device_ptr(const void *p)
 }  // namespace nwi
 
 
+namespace nwi {
+// Destroys the factor objects still alive on a context (nw_factor.hip).
+void destroy_factors(nw_ctx *c);
+}  // namespace nwi
+
 struct nw_ctx {
     int device = 0;
     hipStream_t st = nullptr;
@@ -99,6 +105,7 @@ struct nw_ctx {
     double tol = 1e-8;
     double *d_slab = nullptr;
     size_t slab_cap = 0;
+    std::set<nw_factor *> factors;  // live factor objects on this context (under mu)
     std::vector<nw::PairInfo> pinfo;
     nw::PairInfo *d_pinfo = nullptr;
     size_t pinfo_cap = 0;

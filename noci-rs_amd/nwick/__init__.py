@@ -37,7 +37,7 @@ EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matri
            "nw_mlp_get_params", "nw_mlp_forward", "nw_mlp_grad", "nw_mlp_step", "nw_mlp_train", "nw_mlp_last_error",
            "nw_mlp_destroy", "nw_factor_create", "nw_factor_refresh", "nw_factor_apply", "nw_factor_diagonals",
            "nw_factor_info", "nw_factor_ids", "nw_factor_table", "nw_factor_last_ms", "nw_factor_destroy",
-           "nw_factor_set_precision", "nw_factor_apply_pair", "nw_factor_apply_bytes", "nw_hs_batched_noalloc", "nw_s_same_batched", "nw_m_batched"]
+           "nw_factor_set_precision", "nw_factor_apply_pair", "nw_factor_apply_bytes", "nw_factor_block_groups", "nw_hs_batched_noalloc", "nw_s_same_batched", "nw_m_batched"]
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_BF16X3 = 2
@@ -119,6 +119,7 @@ def load():
     L.nw_factor_set_precision.argtypes = [vp, ct.c_int]
     L.nw_factor_apply_pair.argtypes = [vp, ct.c_double, vp, vp, vp]
     L.nw_factor_apply_bytes.argtypes = [vp, u64p]
+    L.nw_factor_block_groups.argtypes = [vp, u64p, u64p]
     L.nw_batch_create.argtypes = [ct.POINTER(vp), ct.c_int]
     L.nw_batch_add.argtypes = [vp, vp, vp, ct.c_uint64, ct.c_int, ct.POINTER(ct.c_int)]
     L.nw_batch_plan.argtypes = [vp, ct.c_int]
@@ -450,9 +451,11 @@ class Factor:
     def info(self, parent=-1):
         na, nb, tb = ct.c_uint64(), ct.c_uint64(), ct.c_uint64()
         self._check(self.L.nw_factor_info(self.h, int(parent), ct.byref(na), ct.byref(nb), ct.byref(tb)))
-        ab = ct.c_uint64()
+        ab, bg, bk = ct.c_uint64(), ct.c_uint64(), ct.c_uint64()
         self._check(self.L.nw_factor_apply_bytes(self.h, ct.byref(ab)))
-        return {"ncomp_a": na.value, "ncomp_b": nb.value, "table_bytes": tb.value, "apply_bytes": ab.value}
+        self._check(self.L.nw_factor_block_groups(self.h, ct.byref(bg), ct.byref(bk)))
+        return {"ncomp_a": na.value, "ncomp_b": nb.value, "table_bytes": tb.value, "apply_bytes": ab.value,
+                "block_groups": bg.value, "blocks": bk.value}
 
     def ids(self):
         a = np.empty(self.n, dtype=np.uint32); b = np.empty(self.n, dtype=np.uint32)
@@ -474,7 +477,9 @@ class Factor:
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.nw_factor_destroy(self.h)
+            # a closed engine has already freed its factors (nw_destroy)
+            if getattr(self.eng, "ctx", None):
+                self.L.nw_factor_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -289,3 +289,64 @@ def test_pt2_mixed_refinement_equals_fp64(storage):
         res[prec] = r
     assert res["mixed"]["matvecs_fp32"] > 0 and res["fp64"]["matvecs_fp32"] == 0
     assert abs(res["mixed"]["ept2"] - res["fp64"]["ept2"]) < 1e-9
+
+
+@pytest.mark.parametrize("storage", [nwick.FACTOR_DEVICE, nwick.FACTOR_TRANSIENT])
+def test_block_products_equal_per_group_sums(monkeypatch, storage):
+    """The single x single groups of each parent as one dense product on FP64
+    MFMA (k_fac_block, one-vector applies) against the per-group sums
+    (NW_FAC_BLOCK=0): FP64 and FP32 tables, and against the pairwise matvec;
+    two-vector applies are the per-group sums either way."""
+    sysm = synth.make_system(16, 4, 4, 2, seed=5, kappa_sigma=0.3)
+    dets = basis.excited_basis(2, 16, 4, 4, per_ref=None, include_refs=True)
+    rec = basis.to_records(dets)
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    rng = np.random.default_rng(21)
+    x = rng.normal(size=len(dets))
+    x2 = rng.normal(size=(len(dets), 2))
+    lam = -0.6
+    monkeypatch.setenv("NW_FAC_BLOCK", "0")
+    f0 = eng.factor(rec, storage=storage)
+    monkeypatch.delenv("NW_FAC_BLOCK")
+    f1 = eng.factor(rec, storage=storage)
+    i0, i1 = f0.info(), f1.info()
+    print(f"storage {storage}: {i1['block_groups']} groups in {i1['blocks']} dense blocks", flush=True)
+    assert i0["block_groups"] == 0 and i1["block_groups"] >= 32 and i1["blocks"] >= 2
+    yp, ysp = eng.matvec_m(rec, -lam, x, ys=np.empty(len(dets)))
+    scale = max(1.0, float(np.max(np.abs(yp))))
+    for prec in (nwick.PREC_FP64, nwick.PREC_MIXED):
+        f0.set_precision(prec)
+        f1.set_precision(prec)
+        y0, ys0 = f0.apply(lam, x, ys=np.empty(len(dets)))
+        y1, ys1 = f1.apply(lam, x, ys=np.empty(len(dets)))
+        d = max(float(np.max(np.abs(y1 - y0))), float(np.max(np.abs(ys1 - ys0))))
+        print(f"prec {prec}: max |block - per-group| {d:.2e} (scale {scale:.1f})", flush=True)
+        # FP64 sums in another order; with FP32 tables the materialised rows
+        # are rounded to FP32, so the two orders may round them differently
+        rel = 1e-12 if prec == nwick.PREC_FP64 else 2.0 ** -20
+        assert d < rel * scale
+        if prec == nwick.PREC_FP64:
+            assert np.max(np.abs(y1 - yp)) < 1e-9 * scale and np.max(np.abs(ys1 - ysp)) < 1e-9 * scale
+        # two-vector applies keep the per-group sums in both
+        z0 = f0.apply_pair(lam, x2)
+        z1 = f1.apply_pair(lam, x2)
+        assert np.array_equal(z1, z0)
+        assert np.max(np.abs(z1[:, 0] - f1.apply(lam, np.ascontiguousarray(x2[:, 0])))) < rel * scale
+    f0.close()
+    f1.close()
+
+
+def test_engine_closed_before_its_factor():
+    """Closing the engine frees its factor objects; closing such a factor
+    afterwards is a no-op (either order of destruction is safe)."""
+    sysm = synth.make_system(8, 2, 2, 2, seed=3, n_aux=16)
+    dets = basis.excited_basis(2, 8, 2, 2, per_ref=6, seed=3, include_refs=True)
+    eng = nwick.Engine.from_system(sysm)
+    eng.build()
+    f = eng.factor(basis.to_records(dets))
+    g = eng.factor(basis.to_records(dets))
+    g.close()
+    eng.close()
+    f.close()
+    assert f.h is None
