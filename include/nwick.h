@@ -85,6 +85,31 @@ typedef struct nw_pair_meta {
 int nw_create(nw_ctx **out, int device, int n_bas, const double *S, const double *h,
               const double *eri, double enuc);
 
+/* Cholesky-type factors of the context's two-electron integrals,
+ * (ab|cd) = sum_K L[K][a][b] L[K][c][d] with every L[K] symmetric
+ * (L: naux x n_bas x n_bas, host or device memory), the other form of the
+ * integrals that the AO problem can come in.  With factors installed,
+ * nw_build leaves the rank-4 blocks (same-spin J slots, different-spin II
+ * blocks) of every reference pair unbuilt, and each
+ * evaluation call (nw_hs_matrix, nw_hs_batched, nw_project_hs,
+ * nw_hs_matrix_streamed, nw_batch_add) first builds the (row, row) blocks
+ * T[p][:][r][:] that its determinants read, as products over K of
+ * MO-transformed factors on MFMA, in the pair's build precision.  The slab
+ * keeps the reference layout (layout.rs:21-189); nw_pair_slab and
+ * nw_save_wicks complete every block first (dense transforms).  The factors
+ * are checked against the dense eri of nw_create on a sample of entries and
+ * for the symmetry of each L[K]: the largest deviation is returned in
+ * max_dev (may be NULL), and one above check_tol is NW_E_ARG.  They take
+ * effect at the next nw_build; naux = 0 removes them.  No counterpart in the
+ * reference, which transforms the dense integrals (eri_ao2mo_into,
+ * src/maths/eri.rs:258-313, called from build.rs:261-313 and 877-917). */
+int nw_set_eri_factors(nw_ctx *ctx, const double *L, int naux, double check_tol, double *max_dev);
+
+/* Fraction of the (row, row) blocks of ordered pair (x, w)'s rank-4 tensors
+ * that are built: frac3 = {alpha J slots, beta J slots, II blocks}; 1 for a
+ * pair built without factors.  Diagnostics; no reference counterpart. */
+int nw_rank4_coverage(nw_ctx *ctx, int x, int w, double *frac3);
+
 /* Install the reference determinants (the `refs` of build_wicks_shared,
  * src/noci/wicks.rs:133).  ca, cb: nref x n_bas x nmo row-major MO
  * coefficients; occ: nref x 4 uint64 {oa.lo, oa.hi, ob.lo, ob.hi}.  Occupied

@@ -443,118 +443,21 @@ int build_impl(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *me
         }
         HIPCHK(hipGetLastError());
         // ---- four-index transforms (build.rs:261-313, 877-917) ----------------
+        // With integral factors installed, every pair leaves its rank-4 blocks
+        // to the evaluation calls (chol_ensure).
         HIPCHK(hipEventRecord(c->ev[1], c->st));
-        const bool bf3 = c->precision == NW_PREC_BF16X3;
-        // One job per rank-4 tensor: the J slots per spin and the II blocks.
-        // J slab block [a,b,c,d] = blk[c,d,a,b] with blk = (cx_i xc_j || cx_k xc_l);
-        // the pair symmetry of (ab||cd) lets one transform produce it directly.
-        struct TJob { bool f32; bool asym; int c1, c2, c3, c4; double *out; };  // cN: matrix index into cx/xc
-        std::vector<TJob> jobs;
-        auto CX = [&](int pr, int mi) { return 2 * (pr * 2 + mi); };       // even: cx, odd: xc
-        auto XC = [&](int pr, int mi) { return 2 * (pr * 2 + mi) + 1; };
-        for (int p = 0; p < P; ++p) {
-            PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
-            const bool f32 = pi.fp32 != 0;
-            for (int sp = 0; sp < 2; ++sp) {
-                const long long *o = pi.off + (sp ? O_BB : O_AA);
-                const int pr = prob(p, sp);
-                for (int s = 0; s < 10; ++s) {
-                    if (o[O_J + s] < 0) continue;
-                    const int i = jbranch(s, 0), j = jbranch(s, 1), k = jbranch(s, 2), l = jbranch(s, 3);
-                    jobs.push_back({f32, true, CX(pr, k), XC(pr, l), CX(pr, i), XC(pr, j), slab + pi.base + o[O_J + s]});
-                }
-            }
-            const int pa = prob(p, 0), pb = prob(p, 1);
-            for (int a0 = 0; a0 < 2; ++a0)
-                for (int aj = 0; aj < 2; ++aj)
-                    for (int b0 = 0; b0 < 2; ++b0)
-                        for (int bj = 0; bj < 2; ++bj) {
-                            long long oo = pi.off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj];
-                            if (oo < 0) continue;
-                            jobs.push_back({f32, false, CX(pa, a0), XC(pa, aj), CX(pb, b0), XC(pb, bj), slab + pi.base + oo});
-                        }
-        }
-        float *cx32 = nullptr, *xc32 = nullptr;
-        bool any32 = false;
-        for (auto &jb : jobs) any32 |= jb.f32;
-        if (any32) {
-            // FP32 operands, intermediates and MFMA accumulation; the rank-4 slab
-            // blocks they produce hold FP32 values (PairInfo::fp32).
-            const size_t n4 = (size_t)n * n * n * n;
-            if (!c->d_eri32) {
-                c->d_eri32 = (float *)dalloc<char>(n4 * sizeof(float));
-                c->d_easym32 = (float *)dalloc<char>(n4 * sizeof(float));
-                launch_to_f32(c->st, c->d_eri, c->d_eri32, (long long)n4);
-                launch_to_f32(c->st, c->d_easym, c->d_easym32, (long long)n4);
-            }
-            cx32 = c->cx32.get<float>((size_t)4 * P * nnm);
-            xc32 = c->xc32.get<float>((size_t)4 * P * nnm);
-            launch_to_f32(c->st, cx, cx32, (long long)4 * P * nnm);
-            launch_to_f32(c->st, xc, xc32, (long long)4 * P * nnm);
-        }
-        // Tensors in flight per launch group: each needs two scratch slots of
-        // max(n^4, nmo n^3) elements; the group is sized to 1 GiB per buffer.
-        const size_t big = std::max((size_t)n * n * n * n, (size_t)nm * n * n * n);
-        for (int f32 = 0; f32 < 2; ++f32) {
-            std::vector<const TJob *> todo;
-            for (auto &jb : jobs)
-                if (jb.f32 == (f32 == 1)) todo.push_back(&jb);
-            if (todo.empty()) continue;
-            const size_t slot = big * (f32 ? sizeof(float) : sizeof(double));
-            const int g = (int)std::max<size_t>(1, std::min<size_t>(QMAX, (size_t(1) << 30) / slot));
-            char *A = c->scratchA.get<char>(slot * std::min<size_t>(g, todo.size()));
-            char *B = c->scratchB.get<char>(slot * std::min<size_t>(g, todo.size()));
-            for (size_t j0 = 0; j0 < todo.size(); j0 += g) {
-                const int nb = (int)std::min<size_t>(g, todo.size() - j0);
-                if (f32) {
-                    std::vector<const float *> X(nb), C1(nb), C2(nb), C3(nb), C4(nb);
-                    std::vector<float *> TA(nb), TB(nb), OUT(nb);
-                    auto cm = [&](int idx) { return (idx & 1 ? xc32 : cx32) + (size_t)(idx >> 1) * nnm; };
-                    for (int k = 0; k < nb; ++k) {
-                        const TJob &jb = *todo[j0 + k];
-                        X[k] = jb.asym ? c->d_easym32 : c->d_eri32;
-                        C1[k] = cm(jb.c1); C2[k] = cm(jb.c2); C3[k] = cm(jb.c3); C4[k] = cm(jb.c4);
-                        TA[k] = (float *)(A + k * slot);
-                        TB[k] = (float *)(B + k * slot);
-                        OUT[k] = (float *)jb.out;  // the pair's half-length FP32 rank-4 block
-                    }
-                    std::vector<const float *> TAc(TA.begin(), TA.end()), TBc(TB.begin(), TB.end());
-                    auto pass = bf3 ? launch_qtransform_bf3_many : launch_qtransform_f32_many;
-                    bool ok;
-                    if (bf3 && three_launch_transform(n, nm)) {
-                        // half transform of the (p,q) slices, then p and q as leading indices
-                        const long long nm2 = (long long)nm * nm;
-                        ok = launch_qhalf_bf3_many(c->st, X.data(), C3.data(), C4.data(), TA.data(), nb, n, nm) &&
-                             launch_qlead_bf3_many(c->st, TAc.data(), C1.data(), TB.data(), nb, n, nm, n * nm2, 1, 0, 0) &&
-                             launch_qlead_bf3_many(c->st, TBc.data(), C2.data(), OUT.data(), nb, n, nm, nm2, nm,
-                                                   n * nm2, nm * nm2);
-                    } else {
-                        ok = pass(c->st, X.data(), C4.data(), TA.data(), nb, (long long)n * n * n, n, nm) &&
-                             pass(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm) &&
-                             pass(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm) &&
-                             pass(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
-                    }
-                    if (!ok) throw NwError(NW_E_SHAPE, "mixed-precision transform needs an even n_bas and nmo <= 112");
-                } else {
-                    std::vector<const double *> X(nb), C1(nb), C2(nb), C3(nb), C4(nb);
-                    std::vector<double *> TA(nb), TB(nb), OUT(nb);
-                    auto cm = [&](int idx) { return (idx & 1 ? xc : cx) + (size_t)(idx >> 1) * nnm; };
-                    for (int k = 0; k < nb; ++k) {
-                        const TJob &jb = *todo[j0 + k];
-                        X[k] = jb.asym ? c->d_easym : c->d_eri;
-                        C1[k] = cm(jb.c1); C2[k] = cm(jb.c2); C3[k] = cm(jb.c3); C4[k] = cm(jb.c4);
-                        TA[k] = (double *)(A + k * slot);
-                        TB[k] = (double *)(B + k * slot);
-                        OUT[k] = jb.out;
-                    }
-                    std::vector<const double *> TAc(TA.begin(), TA.end()), TBc(TB.begin(), TB.end());
-                    launch_qtransform_many(c->st, X.data(), C4.data(), TA.data(), nb, (long long)n * n * n, n, nm);
-                    launch_qtransform_many(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm);
-                    launch_qtransform_many(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm);
-                    launch_qtransform_many(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
-                }
+        std::vector<uint8_t> lazy(P, 0);
+        std::vector<int> keys_all, keys_dense;
+        {
+            const bool route = chol_route(c);
+            for (int p = 0; p < P; ++p) {
+                const int key = px[p] * nref + pw[p];
+                keys_all.push_back(key);
+                if (route) lazy[p] = 1;
+                else keys_dense.push_back(key);
             }
         }
+        run_rank4_transforms(c, keys_dense);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c->ev[2], c->st));
         // ---- meta to host, pair table to device ------------------------------
@@ -593,7 +496,136 @@ int build_impl(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *me
         c->t_ms[2] = 0.0;
         c->built = true;
         c->have_codensity = true;
+        chol_on_build(c, keys_all, lazy);
         return NW_OK;
+    }
+}
+
+// The four-index transforms of every rank-4 tensor of the given built pairs
+// (build.rs:261-313, 877-917), from the cx/xc contractions the last build
+// kept: one job per tensor, the J slots per spin and the II blocks.
+void run_rank4_transforms(nw_ctx *c, const std::vector<int> &keys)
+{
+    if (keys.empty()) return;
+    const int n = c->n, nm = c->nmo;
+    const size_t nnm = (size_t)n * nm;
+    int P = 0;
+    for (const PairInfo &q : c->pinfo)
+        if (q.built) P = std::max(P, q.bidx + 1);
+    double *slab = c->d_slab;
+    double *cx = (double *)c->keepCX.p, *xc = (double *)c->keepXC.p;
+    auto prob = [&](int p, int sp) { return p * 2 + sp; };
+    const bool bf3 = c->precision == NW_PREC_BF16X3;
+    // One job per rank-4 tensor: the J slots per spin and the II blocks.
+    // J slab block [a,b,c,d] = blk[c,d,a,b] with blk = (cx_i xc_j || cx_k xc_l);
+    // the pair symmetry of (ab||cd) lets one transform produce it directly.
+    struct TJob { bool f32; bool asym; int c1, c2, c3, c4; double *out; };  // cN: matrix index into cx/xc
+    std::vector<TJob> jobs;
+    auto CX = [&](int pr, int mi) { return 2 * (pr * 2 + mi); };       // even: cx, odd: xc
+    auto XC = [&](int pr, int mi) { return 2 * (pr * 2 + mi) + 1; };
+    for (int key : keys) {
+        PairInfo &pi = c->pinfo[key];
+        const int p = pi.bidx;
+        const bool f32 = pi.fp32 != 0;
+        for (int sp = 0; sp < 2; ++sp) {
+            const long long *o = pi.off + (sp ? O_BB : O_AA);
+            const int pr = prob(p, sp);
+            for (int s = 0; s < 10; ++s) {
+                if (o[O_J + s] < 0) continue;
+                const int i = jbranch(s, 0), j = jbranch(s, 1), k = jbranch(s, 2), l = jbranch(s, 3);
+                jobs.push_back({f32, true, CX(pr, k), XC(pr, l), CX(pr, i), XC(pr, j), slab + pi.base + o[O_J + s]});
+            }
+        }
+        const int pa = prob(p, 0), pb = prob(p, 1);
+        for (int a0 = 0; a0 < 2; ++a0)
+            for (int aj = 0; aj < 2; ++aj)
+                for (int b0 = 0; b0 < 2; ++b0)
+                    for (int bj = 0; bj < 2; ++bj) {
+                        long long oo = pi.off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj];
+                        if (oo < 0) continue;
+                        jobs.push_back({f32, false, CX(pa, a0), XC(pa, aj), CX(pb, b0), XC(pb, bj), slab + pi.base + oo});
+                    }
+    }
+    float *cx32 = nullptr, *xc32 = nullptr;
+    bool any32 = false;
+    for (auto &jb : jobs) any32 |= jb.f32;
+    if (any32) {
+        // FP32 operands, intermediates and MFMA accumulation; the rank-4 slab
+        // blocks they produce hold FP32 values (PairInfo::fp32).
+        const size_t n4 = (size_t)n * n * n * n;
+        if (!c->d_eri32) {
+            c->d_eri32 = (float *)dalloc<char>(n4 * sizeof(float));
+            c->d_easym32 = (float *)dalloc<char>(n4 * sizeof(float));
+            launch_to_f32(c->st, c->d_eri, c->d_eri32, (long long)n4);
+            launch_to_f32(c->st, c->d_easym, c->d_easym32, (long long)n4);
+        }
+        cx32 = c->cx32.get<float>((size_t)4 * P * nnm);
+        xc32 = c->xc32.get<float>((size_t)4 * P * nnm);
+        launch_to_f32(c->st, cx, cx32, (long long)4 * P * nnm);
+        launch_to_f32(c->st, xc, xc32, (long long)4 * P * nnm);
+    }
+    // Tensors in flight per launch group: each needs two scratch slots of
+    // max(n^4, nmo n^3) elements; the group is sized to 1 GiB per buffer.
+    const size_t big = std::max((size_t)n * n * n * n, (size_t)nm * n * n * n);
+    for (int f32 = 0; f32 < 2; ++f32) {
+        std::vector<const TJob *> todo;
+        for (auto &jb : jobs)
+            if (jb.f32 == (f32 == 1)) todo.push_back(&jb);
+        if (todo.empty()) continue;
+        const size_t slot = big * (f32 ? sizeof(float) : sizeof(double));
+        const int g = (int)std::max<size_t>(1, std::min<size_t>(QMAX, (size_t(1) << 30) / slot));
+        char *A = c->scratchA.get<char>(slot * std::min<size_t>(g, todo.size()));
+        char *B = c->scratchB.get<char>(slot * std::min<size_t>(g, todo.size()));
+        for (size_t j0 = 0; j0 < todo.size(); j0 += g) {
+            const int nb = (int)std::min<size_t>(g, todo.size() - j0);
+            if (f32) {
+                std::vector<const float *> X(nb), C1(nb), C2(nb), C3(nb), C4(nb);
+                std::vector<float *> TA(nb), TB(nb), OUT(nb);
+                auto cm = [&](int idx) { return (idx & 1 ? xc32 : cx32) + (size_t)(idx >> 1) * nnm; };
+                for (int k = 0; k < nb; ++k) {
+                    const TJob &jb = *todo[j0 + k];
+                    X[k] = jb.asym ? c->d_easym32 : c->d_eri32;
+                    C1[k] = cm(jb.c1); C2[k] = cm(jb.c2); C3[k] = cm(jb.c3); C4[k] = cm(jb.c4);
+                    TA[k] = (float *)(A + k * slot);
+                    TB[k] = (float *)(B + k * slot);
+                    OUT[k] = (float *)jb.out;  // the pair's half-length FP32 rank-4 block
+                }
+                std::vector<const float *> TAc(TA.begin(), TA.end()), TBc(TB.begin(), TB.end());
+                auto pass = bf3 ? launch_qtransform_bf3_many : launch_qtransform_f32_many;
+                bool ok;
+                if (bf3 && three_launch_transform(n, nm)) {
+                    // half transform of the (p,q) slices, then p and q as leading indices
+                    const long long nm2 = (long long)nm * nm;
+                    ok = launch_qhalf_bf3_many(c->st, X.data(), C3.data(), C4.data(), TA.data(), nb, n, nm) &&
+                         launch_qlead_bf3_many(c->st, TAc.data(), C1.data(), TB.data(), nb, n, nm, n * nm2, 1, 0, 0) &&
+                         launch_qlead_bf3_many(c->st, TBc.data(), C2.data(), OUT.data(), nb, n, nm, nm2, nm,
+                                               n * nm2, nm * nm2);
+                } else {
+                    ok = pass(c->st, X.data(), C4.data(), TA.data(), nb, (long long)n * n * n, n, nm) &&
+                         pass(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm) &&
+                         pass(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm) &&
+                         pass(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
+                }
+                if (!ok) throw NwError(NW_E_SHAPE, "mixed-precision transform needs an even n_bas and nmo <= 112");
+            } else {
+                std::vector<const double *> X(nb), C1(nb), C2(nb), C3(nb), C4(nb);
+                std::vector<double *> TA(nb), TB(nb), OUT(nb);
+                auto cm = [&](int idx) { return (idx & 1 ? xc : cx) + (size_t)(idx >> 1) * nnm; };
+                for (int k = 0; k < nb; ++k) {
+                    const TJob &jb = *todo[j0 + k];
+                    X[k] = jb.asym ? c->d_easym : c->d_eri;
+                    C1[k] = cm(jb.c1); C2[k] = cm(jb.c2); C3[k] = cm(jb.c3); C4[k] = cm(jb.c4);
+                    TA[k] = (double *)(A + k * slot);
+                    TB[k] = (double *)(B + k * slot);
+                    OUT[k] = jb.out;
+                }
+                std::vector<const double *> TAc(TA.begin(), TA.end()), TBc(TB.begin(), TB.end());
+                launch_qtransform_many(c->st, X.data(), C4.data(), TA.data(), nb, (long long)n * n * n, n, nm);
+                launch_qtransform_many(c->st, TAc.data(), C3.data(), TB.data(), nb, (long long)nm * n * n, n, nm);
+                launch_qtransform_many(c->st, TBc.data(), C2.data(), TA.data(), nb, (long long)nm * nm * n, n, nm);
+                launch_qtransform_many(c->st, TAc.data(), C1.data(), OUT.data(), nb, (long long)nm * nm * nm, n, nm);
+            }
+        }
     }
 }
 
@@ -806,6 +838,7 @@ int eval_matrix(nw_ctx *c, int kind, const nw_det *dets, uint64_t ndet, double *
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     PackedDet *dp = nullptr;
     std::vector<PackedDet> h = pack(c, dets, ndet, dp);
+    if (kind == EVAL_HS) chol_ensure(c, h);
     auto bins = triangle_blocks(c, h, kind == EVAL_HS);
     const bool adev = is_device_ptr(A), bdev = is_device_ptr(B);
     double *dA = adev ? A : c->outH.get<double>(ndet * ndet);
@@ -829,7 +862,7 @@ int eval_matrix(nw_ctx *c, int kind, const nw_det *dets, uint64_t ndet, double *
     if (!bdev) HIPCHK(hipMemcpyAsync(B, dB, ndet * ndet * 8, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
-    c->t_ms[1] = 0.0;
+    c->t_ms[1] = kind == EVAL_HS ? chol_last_ms(c) : 0.0;
     c->t_ms[2] = elapsed(c->ev[1], c->ev[2]);
     return NW_OK;
 }
@@ -845,7 +878,10 @@ int eval_batched(nw_ctx *c, int kind, const nw_det *dets, uint64_t ndet, const u
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     PackedDet *dp = nullptr;
     const void *draw = nullptr;
-    pack(c, dets, ndet, dp, &draw);
+    {
+        std::vector<PackedDet> h = pack(c, dets, ndet, dp, &draw);
+        if (kind == EVAL_HS) chol_ensure(c, h);
+    }
     const uint64_t *dreq = (const uint64_t *)to_device(c, c->req_in, req, nreq * 16);
     const bool adev = is_device_ptr(A), bdev = is_device_ptr(B);
     double *dA = adev ? A : c->outH.get<double>(nreq);
@@ -904,7 +940,7 @@ int eval_batched(nw_ctx *c, int kind, const nw_det *dets, uint64_t ndet, const u
     if (!bdev) HIPCHK(hipMemcpyAsync(B, dB, nreq * 8, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     c->t_ms[0] = elapsed(c->ev[0], c->ev[2]);
-    c->t_ms[1] = 0.0;
+    c->t_ms[1] = kind == EVAL_HS ? chol_last_ms(c) : 0.0;
     c->t_ms[2] = elapsed(c->ev[1], c->ev[2]);
     return NW_OK;
 }
@@ -993,6 +1029,8 @@ int nw_hs_matrix_streamed(nw_ctx *c, double tol, const nw_det *dets, uint64_t nd
             rc = build_impl(c, tol, mask.data(), meta_out, nullptr);
             if (rc != NW_OK) return rc;
             build_ms += c->t_ms[0];
+            chol_ensure(c, h);  // on-demand rank-4 blocks of the group (integral factors)
+            build_ms += chol_last_ms(c);
             // every element of the group's pairs; each pair is in exactly one group
             auto bins = triangle_blocks(c, h, true, true);
             EvalArgs args = base_args(c, dp, mode);
@@ -1213,6 +1251,7 @@ int nw_project_hs(nw_ctx *c, const nw_det *dets, uint64_t ndet, uint64_t ncur, c
         HIPCHK(hipEventRecord(c->ev[0], c->st));
         PackedDet *dp = nullptr;
         std::vector<PackedDet> h = pack(c, dets, ndet, dp);
+        chol_ensure(c, h);
         auto bins = rect_blocks(c, h, (long long)ncur, (long long)ncur, true);
         const double *dx = (const double *)to_device(c, c->outH, coef, ncur * 8);
         const bool vdev = is_device_ptr(v), sdev = is_device_ptr(s);
@@ -1275,6 +1314,7 @@ int nw_pair_slab(nw_ctx *c, int x, int w, double *out, int64_t *offsets, double 
         if (x < 0 || w < 0 || x >= c->nref || w >= c->nref) return fail(c, NW_E_ARG, "pair out of range");
         const PairInfo &pi = c->pinfo[(size_t)x * c->nref + w];
         if (!pi.built) return fail(c, NW_E_STATE, "pair not built");
+        if (out) chol_complete(c);  // every rank-4 block of on-demand pairs
         // the reference layout, whatever precision the pair was built in
         long long ref[O_LEN];
         pair_offsets(pi.ma, pi.mb, c->nmo, ref);

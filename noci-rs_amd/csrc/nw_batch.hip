@@ -397,6 +397,7 @@ int nw_batch_add(nw_batch *b, nw_ctx *ctx, const nw_det *dets, uint64_t ndet, in
         S.nocc = std::max(ctx->nocc[0], ctx->nocc[1]);
         PackedDet *dp = nullptr;
         std::vector<PackedDet> h = pack(ctx, dets, ndet, dp);
+        chol_ensure(ctx, h);  // on-demand rank-4 blocks (integral factors)
         S.bins = triangle_blocks(ctx, h, true);
         for (auto &kv : S.bins)
             if (kv.first >= REQ_KEY_GEN)

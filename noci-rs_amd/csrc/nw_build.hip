@@ -1191,6 +1191,17 @@ __global__ __launch_bounds__(256) void k_gemm_nt_mfma(const GemmDesc *__restrict
         }
 }
 
+void launch_gemm_nt_mfma(hipStream_t st, int M, int N, int K, int lda, int ldb, int ldc, const GemmDesc *d_descs,
+                         int nbatch)
+{
+    if (nbatch <= 0 || M <= 0 || N <= 0) return;
+    for (int z0 = 0; z0 < nbatch; z0 += 65535) {
+        const int chunk = nbatch - z0 > 65535 ? 65535 : nbatch - z0;
+        dim3 grid((M + 47) / 48, (N + 63) / 64, chunk);
+        hipLaunchKernelGGL(k_gemm_nt_mfma, grid, dim3(256), 0, st, d_descs + z0, M, N, K, lda, ldb, ldc);
+    }
+}
+
 void launch_gemm(hipStream_t st, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc,
                  double alpha, double beta, const GemmDesc *d_descs, int nbatch)
 {

@@ -110,6 +110,7 @@ int nw_save_wicks(nw_ctx *c, const char *dir)
         const int nref = c->nref, nm = c->nmo;
         for (auto &pi : c->pinfo)
             if (!pi.built) return fail(c, NW_E_STATE, "the disk cache holds every ordered pair: build without a mask");
+        chol_complete(c);  // every rank-4 block of on-demand pairs
         // slab in pair order x * nref + w, in the reference layout; offsets absolute
         std::vector<long long> base(nref * nref), refoff((size_t)nref * nref * O_LEN);
         long long total = 0;
@@ -232,6 +233,7 @@ int nw_load_wicks(nw_ctx *c, const char *dir, double tol)
         c->built = false;
         c->build_gen++;
         c->fock_gen++;
+        chol_forget(c);  // a loaded slab is complete
         if (slab_len > c->slab_cap) {
             if (c->d_slab) HIPCHK(hipFree(c->d_slab));
             c->d_slab = nullptr;

@@ -81,6 +81,9 @@ inline bool is_device_ptr(const void *p)
 namespace nwi {
 // Destroys the factor objects still alive on a context (nw_factor.hip).
 void destroy_factors(nw_ctx *c);
+// Integral factors and on-demand rank-4 blocks of a context (nw_chol.hip).
+struct CholState;
+void destroy_chol(nw_ctx *c);
 }  // namespace nwi
 
 struct nw_ctx {
@@ -114,11 +117,13 @@ struct nw_ctx {
     nwi::DBuf buildscratch, arena_desc, scratchA, scratchB, cx32, xc32, keepM, keepCX, keepXC, fockbuf, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
     nwi::DBuf req_in, req_key, req_bins, req_sorted;  // batched requests binned on the GPU
     std::deque<std::vector<char>> keep;
+    nwi::CholState *chol = nullptr;  // integral factors (nw_set_eri_factors) and block coverage
 
     ~nw_ctx()
     {
         (void)hipSetDevice(device);
         if (st) (void)hipStreamSynchronize(st);
+        nwi::destroy_chol(this);
         for (double *p : {d_S, d_h, d_eri, d_easym, d_c[0], d_c[1], d_slab})
             if (p) (void)hipFree(p);
         for (float *p : {d_eri32, d_easym32})
@@ -220,5 +225,28 @@ long long tiles_of(const std::vector<EvalBlock> &v);
 std::map<int, std::vector<EvalBlock>> rect_blocks(nw_ctx *c, const std::vector<PackedDet> &h, long long nrow0,
                                                   long long ncol, bool need_hcol);
 EvalArgs base_args(nw_ctx *c, const PackedDet *dp, int mode);
+
+// The four-index transforms of every rank-4 tensor of the given built pairs
+// (keys x*nref + w) from the co-densities of the context's last build.
+void run_rank4_transforms(nw_ctx *c, const std::vector<int> &keys);
+
+// On-demand rank-4 blocks (nw_chol.hip).  chol_route: pairs of the build
+// about to run may leave their rank-4 blocks to chol_ensure.  chol_on_build:
+// record which pairs of the new build did (lazy[p] for build index p).
+// chol_ensure: build every block the packed determinants h read on any
+// pair (superset: every determinant of a pair's parents as bra and ket).
+// chol_missing: true if chol_ensure would have work for h (no device
+// allocation).  chol_complete: every rank-4 block of every built pair.
+bool chol_route(nw_ctx *c);
+void chol_on_build(nw_ctx *c, const std::vector<int> &keys, const std::vector<uint8_t> &lazy);
+void chol_ensure(nw_ctx *c, const std::vector<PackedDet> &h);
+bool chol_missing(nw_ctx *c, const std::vector<PackedDet> &h);
+void chol_complete(nw_ctx *c);
+void chol_forget(nw_ctx *c);  // a loaded slab: every pair complete
+double chol_last_ms(nw_ctx *c);  // device time of the last chol_ensure's blocks
+bool chol_any_lazy(nw_ctx *c);
+// Packed records of host determinants (ranks and ascending holes / particles
+// per spin; ranks above 4 are clipped), for host-side planning.
+std::vector<PackedDet> host_pack(const nw_det *dets, uint64_t ndet);
 
 }  // namespace nwi

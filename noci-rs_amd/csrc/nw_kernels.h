@@ -10,6 +10,9 @@ struct TraceDesc { const double *g; const double *h; double *out; double scale; 
 
 void launch_gemm(hipStream_t st, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc,
                  double alpha, double beta, const GemmDesc *d_descs, int nbatch);
+// C = A B^T on FP64 MFMA for any K (A: M x K, B: N x K, both K-contiguous).
+void launch_gemm_nt_mfma(hipStream_t st, int M, int N, int K, int lda, int ldb, int ldc, const GemmDesc *d_descs,
+                         int nbatch);
 void launch_svd(hipStream_t st, const double *A, int n, int nprob, double *U, double *V, double *sig,
                 double *phase, int *m, double *tsp, double tol);
 // Strides (in doubles) between consecutive problems of one launch.

@@ -73,6 +73,16 @@ extern "C" int nw_hs_batched_noalloc(nw_ctx *c, const nw_det *dets, uint64_t nde
         if (nreq == 0) return NW_OK;
         if (!is_device_ptr(dets) || !is_device_ptr(req) || !is_device_ptr(h_out) || !is_device_ptr(s_out))
             return fail(c, NW_E_ARG, "nw_hs_batched_noalloc needs device arrays for dets, req, h_out and s_out");
+        if (chol_any_lazy(c)) {
+            // pairs built from integral factors: the blocks these determinants read
+            // must exist already (no device allocation here to build them)
+            std::vector<nw_det> hd(ndet);
+            HIPCHK(hipMemcpyAsync(hd.data(), dets, ndet * sizeof(nw_det), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            if (chol_missing(c, host_pack(hd.data(), ndet)))
+                return fail(c, NW_E_STATE, "rank-4 blocks these determinants read are not built yet (pairs built "
+                                           "from integral factors): evaluate them once with nw_hs_batched first");
+        }
         std::lock_guard<std::mutex> lk(na_mu[c->device]);
         unsigned char *base = na_scratch(c->device);
         PackedDet *dp = (PackedDet *)(base + NA_PACKED);

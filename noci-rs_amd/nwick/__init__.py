@@ -37,7 +37,8 @@ EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matri
            "nw_mlp_get_params", "nw_mlp_forward", "nw_mlp_grad", "nw_mlp_step", "nw_mlp_train", "nw_mlp_last_error",
            "nw_mlp_destroy", "nw_factor_create", "nw_factor_refresh", "nw_factor_apply", "nw_factor_diagonals",
            "nw_factor_info", "nw_factor_ids", "nw_factor_table", "nw_factor_last_ms", "nw_factor_destroy",
-           "nw_factor_set_precision", "nw_factor_apply_pair", "nw_factor_apply_bytes", "nw_factor_block_groups", "nw_hs_batched_noalloc", "nw_s_same_batched", "nw_m_batched"]
+           "nw_factor_set_precision", "nw_factor_apply_pair", "nw_factor_apply_bytes", "nw_factor_block_groups", "nw_hs_batched_noalloc", "nw_s_same_batched", "nw_m_batched",
+           "nw_set_eri_factors", "nw_rank4_coverage"]
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_BF16X3 = 2
@@ -92,6 +93,8 @@ def load():
     L.nw_hs_matrix_streamed.argtypes = [vp, ct.c_double, vp, ct.c_uint64, vp, vp, ct.c_int, ct.c_uint64, vp, vp]
     L.nw_pair_slab.argtypes = [vp, ct.c_int, ct.c_int, vp, vp, vp]
     L.nw_set_precision.argtypes = [vp, ct.c_int]
+    L.nw_set_eri_factors.argtypes = [vp, vp, ct.c_int, ct.c_double, dp]
+    L.nw_rank4_coverage.argtypes = [vp, ct.c_int, ct.c_int, dp]
     L.nw_project_hs.argtypes = [vp, vp, ct.c_uint64, ct.c_uint64, vp, vp, vp, ct.c_int]
     L.nw_noci_fock.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_double, vp, vp, vp, vp]
     L.nw_save_wicks.argtypes = [vp, ct.c_char_p]
@@ -197,6 +200,34 @@ class Engine:
                 occ[i, 2 * k + 1] = v >> 64
         self._check(self.L.nw_set_refs(self.ctx, nref, nmo, _ptr(ca), _ptr(cb), _ptr(occ)))
         self.nref, self.nmo = nref, nmo
+
+    def set_eri_factors(self, L, check_tol=1e-10):
+        """Install integral factors (ab|cd) = sum_K L[K,a,b] L[K,c,d] (L: naux x n x n,
+        symmetric per K; None removes them).  The next build leaves the rank-4
+        blocks of m = 0 pairs to the evaluation calls, which build the (row, row)
+        blocks their determinants read.  Returns the largest sampled deviation
+        from the context's dense integrals."""
+        dev = ct.c_double(0.0)
+        if L is None:
+            self._check(self.L.nw_set_eri_factors(self.ctx, None, 0, float(check_tol), ct.byref(dev)))
+            self._fac = None
+            return 0.0
+        if hasattr(L, "data_ptr"):
+            naux = int(L.shape[0])
+            sync_torch(L)
+            self._fac = L
+        else:
+            L = np.ascontiguousarray(L, dtype=np.float64)
+            naux = L.shape[0]
+            self._fac = L
+        self._check(self.L.nw_set_eri_factors(self.ctx, _ptr(L), naux, float(check_tol), ct.byref(dev)))
+        return dev.value
+
+    def rank4_coverage(self, x, w):
+        """Fraction of the (row, row) blocks built of pair (x, w)'s alpha J, beta J and II tensors."""
+        out = (ct.c_double * 3)()
+        self._check(self.L.nw_rank4_coverage(self.ctx, int(x), int(w), out))
+        return tuple(out)
 
     def set_precision(self, prec):
         """PREC_FP64 or PREC_MIXED (FP32 MFMA four-index transforms)."""

@@ -9,7 +9,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_header_symbols_exported():
     hdr = open(os.path.join(ROOT, "include", "nwick.h")).read()
-    declared = set(re.findall(r"\b(nw_[a-z_]+)\s*\(", hdr))
+    declared = set(re.findall(r"\b(nw_[a-z0-9_]+)\s*\(", hdr))
     assert declared == set(nwick.EXPORTS)
     L = nwick.load()
     for name in declared:
