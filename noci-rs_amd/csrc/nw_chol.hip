@@ -6,21 +6,27 @@
 //   J[p,q,r,s]   = sum_K A_K[p,q] A'_K[r,s] - sum_K E_K[p,r] F_K[q,s]
 //                                                               (build.rs:261-313)
 // with A = cx^T L_K xc, E = cx^T L_K cx, F = xc^T L_K xc for the coefficient
-// matrices of the J slot or II block.  The evaluators of pairs with
-// m_alpha = m_beta = 0 read J (slot 0 only) as J[p,q,r,s] - J[p,s,r,q] (the
-// same-spin two-electron term of preparehamiltonianoverlap.rs, and every
-// fixed and run-time-rank kernel here), and that slot's F_K is symmetric, so
-// the exchange products cancel in what they read: their blocks hold the
-// direct products only.  Pairs with zero-overlap orbitals read single J
-// entries, and get the exchange products as well (k_r4_exch).
+// matrices of the J slot or II block, each held as [row][column][K] (K
+// contiguous).  The evaluators of pairs with m_alpha = m_beta = 0 read J
+// (slot 0 only) as J[p,q,r,s] - J[p,s,r,q] (the same-spin two-electron term
+// of preparehamiltonianoverlap.rs, and every fixed and run-time-rank kernel
+// here), and that slot's F_K is symmetric, so the exchange products cancel
+// in what they read: their blocks hold the direct products only.  Pairs
+// with zero-overlap orbitals read single J entries, and get the exchange
+// products as well.
 //
 // A determinant pair reads its tensors at (row, row) positions
 // (p, r) = (rows[eta], rows[xi]) of its contraction labels
-// (construct_determinant_indices, eval/prepare.rs:555-595), and every such
-// block T[p][:][r][:] is one nm x nm product over K.  Only the blocks the
-// determinants of an evaluation call read are built, each as a 16-row-tile
-// MFMA product, into the reference slab layout; a later call builds what it
-// adds.  Exports complete the tensors with the dense transforms.
+// (construct_determinant_indices, eval/prepare.rs:555-595) and, within the
+// block T[p][:][r][:], at columns (q, s) of a bra hole (q < nocc or
+// s < nocc) or two ket particles of one ket determinant.  The (p, r) blocks
+// an evaluation call reads form rectangles (every bra row of one parent x
+// every ket row of the other) plus a few single blocks (two rows of one
+// determinant).  Each rectangle's bra-hole columns and ket-particle pairs
+// are row-gathered A B^T products on MFMA (k_r4_gemm); the single blocks
+// take a per-block kernel (k_r4_block).  Only what a call reads is built,
+// into the reference slab layout; a later call builds what it adds.
+// Exports complete the tensors with the dense transforms.
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -32,8 +38,12 @@
 namespace nw {
 
 struct R4Tensor {
-    const void *P, *Q;  // [naux][nm][nm] factor tensors (double or float)
+    const void *P, *Q;  // factor tensors [nm][nm][naux] (double or float), K contiguous
     void *out;          // rank-4 slab block [nm][nm][nm][nm] (double or float)
+    const int *list;    // sparse (q, s) entries outside the dense tiles, q * nm + s
+    int nlist;
+    int qt, st;         // dense region: 16-row tiles q < 16 qt, or 16-column tiles s < 16 st
+    int pad;
 };
 struct R4Item {
     int tensor, p, r, pad;
@@ -64,145 +74,239 @@ struct R4Mfma<float> {
     static __device__ inline int row(int lk, int r) { return 4 * lk + r; }
 };
 
-// One (row, row) block per workgroup iteration: out[p][q][r][s] =
-// sum_K P[K][p][q] Q[K][r][s] for all q, s < nm.  NW waves; wave w owns the
-// 16 output rows q in [16w, 16w + 16) and all NW column tiles s.  K is staged
-// through LDS 32 at a time ([k][column] for both operands, so the A and B
-// fragments are lane-contiguous rows), the next chunk prefetched into
-// registers while the current one is multiplied.  Items are dealt to the
-// eight XCDs in contiguous chunks (workgroup b runs chunk b mod 8), so
-// neighbouring blocks of one tensor, which share factor rows, meet in one L2.
+// Single (row, row) blocks (the few a rectangle does not cover: both rows of
+// one determinant): out[p][q][r][s] = sum_K P[p][q][K] Q[r][s][K] over the
+// entries the evaluators read -- the columns of bra holes (every 16 x 16
+// tile with q < 16 qt or s < 16 st, on MFMA) and the listed ket-particle
+// pairs outside those tiles (VALU sums from the same staged chunks).  Four
+// waves; a pass takes 16 tiles (4 per wave) and 1,024 listed entries (4 per
+// thread); more passes restage K.  K is staged through LDS 128 bytes per row
+// at a time ([row][k]).  Items are dealt to the eight XCDs in contiguous
+// chunks (workgroup b runs chunk b mod 8).
 template <typename T, int NW>
-__global__ __launch_bounds__(64 * NW) void k_r4_block(const R4Tensor *__restrict__ ts,
-                                                      const R4Item *__restrict__ items, int nitems, int nm, int naux)
+__global__ __launch_bounds__(256) void k_r4_block(const R4Tensor *__restrict__ ts,
+                                                  const R4Item *__restrict__ items, int nitems, int nm, int naux)
 {
-    // K chunk: 32 (16 for FP64 at NW = 8, to stay within 64 KB of LDS)
-    constexpr int NP = 16 * NW, KC = (sizeof(T) == 8 && NW == 8) ? 16 : 32, NT = 64 * NW, LDW = NP + 2;
+    constexpr int NP = 16 * NW, KC = 128 / sizeof(T), NT = 256, LDK = KC + 1;
     constexpr int PER = 2 * KC * NP / NT;  // elements of the two operand chunks per thread
     static_assert(PER * NT == 2 * KC * NP, "operand chunks divide over the workgroup");
-    __shared__ T Ps[KC][LDW];
-    __shared__ T Qs[KC][LDW];
+    constexpr int TPW = 4, SPT = 4;  // tiles per wave, listed entries per thread, per pass
+    __shared__ T Ps[NP][LDK];
+    __shared__ T Qs[NP][LDK];
     using M = R4Mfma<T>;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
-    const size_t nm2 = (size_t)nm * nm;
     const int gper = gridDim.x / 8, chunk = blockIdx.x % 8, lidx = blockIdx.x / 8;
     const int per = (nitems + 7) / 8;
     const int i0 = chunk * per, i1 = min(nitems, i0 + per);
     for (int it = i0 + lidx; it < i1; it += gper) {
         const R4Item w = items[it];
         const R4Tensor tt = ts[w.tensor];
-        const T *__restrict__ P = (const T *)tt.P + (size_t)w.p * nm;
-        const T *__restrict__ Q = (const T *)tt.Q + (size_t)w.r * nm;
+        const T *__restrict__ P = (const T *)tt.P + (size_t)w.p * nm * naux;
+        const T *__restrict__ Q = (const T *)tt.Q + (size_t)w.r * nm * naux;
+        T *out = (T *)tt.out;
+        const int qt = min(tt.qt, NW), st = min(tt.st, NW);
+        const int ntile = qt * NW + (NW - qt) * st;
+        const int npass = max(1, max((ntile + 4 * TPW - 1) / (4 * TPW), (tt.nlist + NT * SPT - 1) / (NT * SPT)));
+        for (int pass = 0; pass < npass; ++pass) {
+            int tq[TPW], tsl[TPW], sq[SPT], ss[SPT];
+#pragma unroll
+            for (int j = 0; j < TPW; ++j) {
+                const int id = pass * 4 * TPW + wave + 4 * j;
+                if (id < qt * NW) {
+                    tq[j] = id / NW;
+                    tsl[j] = id % NW;
+                } else if (id < ntile) {
+                    tq[j] = qt + (id - qt * NW) / st;
+                    tsl[j] = (id - qt * NW) % st;
+                } else {
+                    tq[j] = -1;
+                    tsl[j] = 0;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < SPT; ++j) {
+                const int e = pass * NT * SPT + tid + NT * j;
+                const int v = e < tt.nlist ? tt.list[e] : -1;
+                sq[j] = v < 0 ? -1 : v / nm;
+                ss[j] = v < 0 ? 0 : v % nm;
+            }
+            typename M::acc acc[TPW];
+            T sacc[SPT];
+#pragma unroll
+            for (int j = 0; j < TPW; ++j) acc[j] = typename M::acc{0, 0, 0, 0};
+#pragma unroll
+            for (int j = 0; j < SPT; ++j) sacc[j] = T(0);
+            int ao[TPW], bo[TPW];
+#pragma unroll
+            for (int j = 0; j < TPW; ++j) {
+                ao[j] = (tq[j] < 0 ? 0 : tq[j]) * 16 + lr;
+                bo[j] = tsl[j] * 16 + lr;
+            }
+            for (int k0 = 0; k0 < naux; k0 += KC) {
+                __syncthreads();
+#pragma unroll 4
+                for (int j = 0; j < PER; ++j) {
+                    const int e = tid + j * NT;
+                    const bool second = e >= KC * NP;
+                    const int e2 = second ? e - KC * NP : e;
+                    const int q = e2 / KC, k = e2 % KC;
+                    const T *src = second ? Q : P;
+                    const T v = (q < nm && k0 + k < naux) ? src[(size_t)q * naux + k0 + k] : T(0);
+                    if (second) Qs[q][k] = v;
+                    else Ps[q][k] = v;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int kk = 0; kk < KC / 4; ++kk)
+#pragma unroll
+                    for (int j = 0; j < TPW; ++j)
+                        if (tq[j] >= 0) acc[j] = M::mma(Ps[ao[j]][kk * 4 + lk], Qs[bo[j]][kk * 4 + lk], acc[j]);
+#pragma unroll
+                for (int j = 0; j < SPT; ++j)
+                    if (sq[j] >= 0)
+#pragma unroll 4
+                        for (int k = 0; k < KC; ++k) sacc[j] += Ps[sq[j]][k] * Qs[ss[j]][k];
+            }
+#pragma unroll
+            for (int j = 0; j < TPW; ++j) {
+                if (tq[j] < 0) continue;
+                const int s = tsl[j] * 16 + lr;
+                if (s >= nm) continue;
+#pragma unroll
+                for (int r4 = 0; r4 < 4; ++r4) {
+                    const int q = tq[j] * 16 + M::row(lk, r4);
+                    if (q < nm) out[(((size_t)w.p * nm + q) * nm + w.r) * nm + s] = acc[j][r4];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < SPT; ++j)
+                if (sq[j] >= 0) out[(((size_t)w.p * nm + sq[j]) * nm + w.r) * nm + ss[j]] = sacc[j];
+        }
+    }
+}
+
+// Index pairs (u, v) of a product's rows or columns: element i is
+// (u[i], v[i]) when paired, else (u[i / nv], v[i % nv]) or (u[i / nv],
+// v0 + i % nv) without a v list.
+struct R4Axis {
+    const int *u, *v;
+    int nu, nv, v0, paired;
+};
+__device__ inline void axis_at(const R4Axis &a, int i, int &u, int &v)
+{
+    if (a.paired) {
+        u = a.u[i];
+        v = a.v[i];
+    } else {
+        u = a.u[i / a.nv];
+        v = a.v ? a.v[i % a.nv] : a.v0 + i % a.nv;
+    }
+}
+// One product C[i][j] = sum_K A[row_i][K] B[col_j][K] with A rows
+// (u, v) = A[u * nm + v] and B rows (u', v') = B[u' * nm + v'] (factor
+// tensors [nm][nm][naux]); mode 0 stores C at out[((u nm + v) nm + u') nm + v']
+// (direct products: u = p, v = q, u' = r, v' = s), mode 1 subtracts it at
+// out[((u nm + u') nm + v) nm + v'] (exchange products: rows (p, r) of E,
+// columns (q, s) of F).
+struct R4Gemm {
+    const void *A, *B;
+    void *out;
+    R4Axis row, col;
+    int M, N, mode, pad;
+};
+struct R4GTile {
+    int g, m0, n0, pad;
+};
+
+// Row-gathered A B^T products in 64 x 64 tiles: four waves of 16 rows x 4
+// column tiles each, K staged through LDS 32 at a time ([row][k], rows
+// gathered from the factor tensors, 256 contiguous bytes per row and chunk
+// in FP64), the next chunk prefetched into registers during the products.
+// Tiles are dealt to the eight XCDs in contiguous chunks, so the tiles of
+// one product, which share rows, meet in one L2.
+template <typename T>
+__global__ __launch_bounds__(256) void k_r4_gemm(const R4Gemm *__restrict__ gs, const R4GTile *__restrict__ tiles,
+                                                 int ntiles, int nm, int naux)
+{
+    constexpr int BM = 64, BN = 64, KC = 32, LDK = KC + 2, NT = 256, PER = (BM + BN) * KC / NT;
+    __shared__ T As[BM][LDK];
+    __shared__ T Bs[BN][LDK];
+    __shared__ long long ra[BM], rb[BN], ro[BM], co[BN];
+    using M = R4Mfma<T>;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+    const int gper = gridDim.x / 8, chunk = blockIdx.x % 8, lidx = blockIdx.x / 8;
+    const int per = (ntiles + 7) / 8;
+    const int t0 = chunk * per, t1 = min(ntiles, t0 + per);
+    const long long nm2 = (long long)nm * nm;
+    for (int it = t0 + lidx; it < t1; it += gper) {
+        const R4GTile tl = tiles[it];
+        const R4Gemm g = gs[tl.g];
+        const int mloc = min(BM, g.M - tl.m0), nloc = min(BN, g.N - tl.n0);
+        __syncthreads();  // the previous tile's indices are no longer read
+        if (tid < BM) {
+            int u = 0, v = 0;
+            if (tid < mloc) axis_at(g.row, tl.m0 + tid, u, v);
+            ra[tid] = ((long long)u * nm + v) * naux;
+            ro[tid] = g.mode == 0 ? ((long long)u * nm + v) * nm2 : (long long)u * nm * nm2 + (long long)v * nm;
+        } else if (tid < BM + BN) {
+            const int j = tid - BM;
+            int u = 0, v = 0;
+            if (j < nloc) axis_at(g.col, tl.n0 + j, u, v);
+            rb[j] = ((long long)u * nm + v) * naux;
+            co[j] = g.mode == 0 ? (long long)u * nm + v : (long long)u * nm2 + v;
+        }
+        __syncthreads();
+        const T *__restrict__ A = (const T *)g.A;
+        const T *__restrict__ B = (const T *)g.B;
         T buf[PER];
         auto gload = [&](int k0) {
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
                 const int e = tid + j * NT;
-                const int e2 = j < PER / 2 ? e : e - KC * NP;
-                const int k = e2 / NP, col = e2 % NP;
-                const T *src = j < PER / 2 ? P : Q;
-                buf[j] = (k0 + k < naux && col < nm) ? src[(size_t)(k0 + k) * nm2 + col] : T(0);
+                const bool second = e >= BM * KC;
+                const int e2 = second ? e - BM * KC : e;
+                const int r = e2 / KC, k = e2 % KC;
+                const bool ok = (second ? r < nloc : r < mloc) && k0 + k < naux;
+                buf[j] = ok ? (second ? B[rb[r] + k0 + k] : A[ra[r] + k0 + k]) : T(0);
             }
         };
-        typename M::acc acc[NW];
+        typename M::acc acc[4];
 #pragma unroll
-        for (int t = 0; t < NW; ++t) acc[t] = typename M::acc{0, 0, 0, 0};
+        for (int t = 0; t < 4; ++t) acc[t] = typename M::acc{0, 0, 0, 0};
         gload(0);
         for (int k0 = 0; k0 < naux; k0 += KC) {
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
                 const int e = tid + j * NT;
-                const int e2 = j < PER / 2 ? e : e - KC * NP;
-                const int k = e2 / NP, col = e2 % NP;
-                if (j < PER / 2) Ps[k][col] = buf[j];
-                else Qs[k][col] = buf[j];
+                const bool second = e >= BM * KC;
+                const int e2 = second ? e - BM * KC : e;
+                const int r = e2 / KC, k = e2 % KC;
+                if (second) Bs[r][k] = buf[j];
+                else As[r][k] = buf[j];
             }
             __syncthreads();
             if (k0 + KC < naux) gload(k0 + KC);
 #pragma unroll
             for (int kk = 0; kk < KC / 4; ++kk) {
-                const T a = Ps[kk * 4 + lk][wave * 16 + lr];
+                const T a = As[wave * 16 + lr][kk * 4 + lk];
 #pragma unroll
-                for (int t = 0; t < NW; ++t) acc[t] = M::mma(a, Qs[kk * 4 + lk][t * 16 + lr], acc[t]);
+                for (int t = 0; t < 4; ++t) acc[t] = M::mma(a, Bs[t * 16 + lr][kk * 4 + lk], acc[t]);
             }
         }
-        T *out = (T *)tt.out;
+        T *out = (T *)g.out;
 #pragma unroll
-        for (int t = 0; t < NW; ++t) {
-            const int s = t * 16 + lr;
-            if (s >= nm) continue;
+        for (int t = 0; t < 4; ++t) {
+            const int j = t * 16 + lr;
+            if (j >= nloc) continue;
 #pragma unroll
             for (int r4 = 0; r4 < 4; ++r4) {
-                const int q = wave * 16 + M::row(lk, r4);
-                if (q < nm) out[(((size_t)w.p * nm + q) * nm + w.r) * nm + s] = acc[t][r4];
+                const int i = wave * 16 + M::row(lk, r4);
+                if (i >= mloc) continue;
+                T *o = out + ro[i] + co[j];
+                if (g.mode == 0) *o = acc[t][r4];
+                else *o -= acc[t][r4];
             }
-        }
-    }
-}
-
-// Exchange products of the J slots of pairs with zero-overlap orbitals
-// (whose evaluators read single J entries): out[p][q][r][s] -=
-// sum_K E[K][p][r] F[K][q][s] for every listed block (p, r), as one product
-// per tensor with the blocks as rows: X[b][(q, s)] = sum_K Esel[b][K]
-// F[K][(q, s)], 64 blocks x 64 columns per workgroup (4 waves of 16 blocks x
-// 4 column tiles), K staged through LDS 32 at a time.  Runs after the direct
-// products of the same blocks (stream order) and subtracts from them.
-struct R4XTile {
-    int tensor, i0, n, pad;  // items i0 .. i0 + n - 1 (n <= 64) of one tensor
-};
-template <typename T>
-__global__ __launch_bounds__(256) void k_r4_exch(const R4Tensor *__restrict__ ts, const R4Item *__restrict__ items,
-                                                 const R4XTile *__restrict__ tiles, int nm, int naux)
-{
-    constexpr int KC = 32, BT = 64, CT = 64, LDW = 64 + 2;
-    __shared__ T Es[KC][LDW];
-    __shared__ T Fs[KC][LDW];
-    __shared__ int sp_[BT], sr_[BT];
-    using M = R4Mfma<T>;
-    const R4XTile tl = tiles[blockIdx.x];
-    const R4Tensor tt = ts[tl.tensor];  // P = E, Q = F
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
-    const size_t nm2 = (size_t)nm * nm;
-    const long long c0 = (long long)blockIdx.y * CT;
-    if (tid < BT) {
-        const R4Item w = tid < tl.n ? items[tl.i0 + tid] : R4Item{0, 0, 0, 0};
-        sp_[tid] = w.p;
-        sr_[tid] = w.r;
-    }
-    __syncthreads();
-    const T *__restrict__ E = (const T *)tt.P;
-    const T *__restrict__ F = (const T *)tt.Q;
-    typename M::acc acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = typename M::acc{0, 0, 0, 0};
-    for (int k0 = 0; k0 < naux; k0 += KC) {
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < KC * BT / 256; ++j) {
-            const int e = tid + j * 256, k = e / BT, b = e % BT;
-            Es[k][b] = (k0 + k < naux && b < tl.n) ? E[(size_t)(k0 + k) * nm2 + (size_t)sp_[b] * nm + sr_[b]] : T(0);
-            const long long col = c0 + b;
-            Fs[k][b] = (k0 + k < naux && col < (long long)nm2) ? F[(size_t)(k0 + k) * nm2 + col] : T(0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < KC / 4; ++kk) {
-            const T a = Es[kk * 4 + lk][wave * 16 + lr];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = M::mma(a, Fs[kk * 4 + lk][t * 16 + lr], acc[t]);
-        }
-    }
-    T *out = (T *)tt.out;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const long long col = c0 + t * 16 + lr;
-        if (col >= (long long)nm2) continue;
-        const int q = (int)(col / nm), s = (int)(col % nm);
-#pragma unroll
-        for (int r4 = 0; r4 < 4; ++r4) {
-            const int b = wave * 16 + M::row(lk, r4);
-            if (b < tl.n) out[(((size_t)sp_[b] * nm + q) * nm + sr_[b]) * nm + s] -= acc[t][r4];
         }
     }
 }
@@ -253,14 +357,15 @@ namespace {
 // slots, II blocks.
 enum { CL_JA = 0, CL_JB = 1, CL_II = 2 };
 
-// Factor tensors [naux][nm][nm] of one pair, keyed fkey(kind, spin, a, b):
+// Factor tensors [nm][nm][naux] of one pair, keyed fkey(kind, spin, a, b):
 // kind FA = cx_a^T L_K xc_b, FE = cx_a^T L_K cx_b, FF = xc_a^T L_K xc_b.
 enum { FA = 0, FE = 1, FF = 2 };
 inline int fkey(int kind, int sp, int a, int b) { return ((kind * 2 + sp) * 2 + a) * 2 + b; }
 
 struct CholPair {
     bool lazy = false;
-    std::vector<uint8_t> cov[3];  // nm x nm: block (p, r) built
+    std::vector<uint8_t> cov[3];    // nm x nm: block (p, r) built
+    std::vector<uint8_t> kkcov[3];  // nm x nm: ket-particle column pairs (q, s) the built blocks hold
     std::map<int, double *> fac;  // FP64 factor tensors
     std::map<int, float *> fac32; // their FP32 copies (pairs built in reduced precision)
 };
@@ -324,6 +429,7 @@ void chol_on_build(nw_ctx *c, const std::vector<int> &keys, const std::vector<ui
         CholPair &cp = s->pairs[keys[k]];
         cp.lazy = true;
         for (auto &v : cp.cov) v.assign(nm2, 0);
+        for (auto &v : cp.kkcov) v.assign(nm2, 0);
     }
 }
 
@@ -337,18 +443,18 @@ void chol_forget(nw_ctx *c)
 
 namespace {
 
-template <typename T, int NW>
-int r4_grid()
+// Workgroups of a block launch: up to eight per CU, in multiples of eight
+// (the XCD chunks); each runs its chunk's items in a grid-stride loop.
+int r4_grid(int nitems)
 {
-    static int g = 0;
-    if (!g) {
-        int dev = 0, ncu = 0, occ = 0;
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_r4_block<T, NW>, 64 * NW, 0));
-        g = std::max(8, (std::max(1, occ) * ncu) / 8 * 8);
+        ncu = std::max(1, ncu);
     }
-    return g;
+    return std::max(8, std::min((nitems + 7) / 8 * 8, 8 * ncu));
 }
 
 template <typename T>
@@ -357,8 +463,8 @@ void launch_r4(hipStream_t st, int nw, const R4Tensor *ts, const R4Item *items, 
     if (nitems <= 0) return;
 #define NW_R4_CASE(W)                                                                                     \
     case W: {                                                                                             \
-        const int g = std::min(r4_grid<T, W>(), ((nitems + 7) / 8) * 8);                                  \
-        hipLaunchKernelGGL((k_r4_block<T, W>), dim3(g), dim3(64 * W), 0, st, ts, items, nitems, nm, naux); \
+        const int g = r4_grid(nitems);                                                                    \
+        hipLaunchKernelGGL((k_r4_block<T, W>), dim3(g), dim3(256), 0, st, ts, items, nitems, nm, naux);    \
         break;                                                                                            \
     }
     switch (nw) {
@@ -369,6 +475,13 @@ void launch_r4(hipStream_t st, int nw, const R4Tensor *ts, const R4Item *items, 
 #undef NW_R4_CASE
 }
 
+template <typename T>
+void launch_r4g(hipStream_t st, const R4Gemm *gs, const R4GTile *tiles, int ntiles, int nm, int naux)
+{
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(k_r4_gemm<T>, dim3(r4_grid(ntiles)), dim3(256), 0, st, gs, tiles, ntiles, nm, naux);
+}
+
 // Row index sets of one parent's determinants per spin, as bra (rows
 // part - nocc) and as ket (rows nvirt + hole), and the row pairs that occur
 // within one determinant (both orders; the mixed-spin pairs as (alpha, beta)).
@@ -377,6 +490,7 @@ struct ParentRows {
     std::vector<uint8_t> bra[2], ket[2];            // nm
     std::vector<uint8_t> bra2[2], ket2[2];          // nm x nm, same-spin pairs of one determinant
     std::vector<uint8_t> braab, ketab;              // nm x nm, (alpha row, beta row) of one determinant
+    std::vector<uint8_t> kcol2[2], kcolab;          // nm x nm, ket particle (column label) pairs of one determinant
 };
 
 std::vector<ParentRows> parent_rows(nw_ctx *c, const std::vector<PackedDet> &h)
@@ -397,14 +511,18 @@ std::vector<ParentRows> parent_rows(nw_ctx *c, const std::vector<PackedDet> &h)
             }
             r.braab.assign(nm2, 0);
             r.ketab.assign(nm2, 0);
+            r.kcol2[0].assign(nm2, 0);
+            r.kcol2[1].assign(nm2, 0);
+            r.kcolab.assign(nm2, 0);
         }
-        int br[2][4], kr[2][4], rk[2] = {d.ra, d.rb};
+        int br[2][4], kr[2][4], kc[2][4], rk[2] = {d.ra, d.rb};
         for (int sp = 0; sp < 2; ++sp) {
             const uint8_t *ix = sp ? d.ib : d.ia;
             const int no = c->nocc[sp], nv = nm - no;
             for (int i = 0; i < rk[sp]; ++i) {
                 br[sp][i] = std::min(nm - 1, std::max(0, (int)ix[4 + i] - no));
                 kr[sp][i] = std::min(nm - 1, std::max(0, nv + (int)ix[i]));
+                kc[sp][i] = std::min(nm - 1, (int)ix[4 + i]);  // column label of a ket particle
                 r.bra[sp][br[sp][i]] = 1;
                 r.ket[sp][kr[sp][i]] = 1;
             }
@@ -413,53 +531,136 @@ std::vector<ParentRows> parent_rows(nw_ctx *c, const std::vector<PackedDet> &h)
                     if (i != j) {
                         r.bra2[sp][(size_t)br[sp][i] * nm + br[sp][j]] = 1;
                         r.ket2[sp][(size_t)kr[sp][i] * nm + kr[sp][j]] = 1;
+                        r.kcol2[sp][(size_t)kc[sp][i] * nm + kc[sp][j]] = 1;
                     }
         }
         for (int i = 0; i < rk[0]; ++i)
             for (int j = 0; j < rk[1]; ++j) {
                 r.braab[(size_t)br[0][i] * nm + br[1][j]] = 1;
                 r.ketab[(size_t)kr[0][i] * nm + kr[1][j]] = 1;
+                r.kcolab[(size_t)kc[0][i] * nm + kc[1][j]] = 1;
             }
     }
     return pr;
 }
 
-// Blocks (p, r) of pair (x, w) that its determinant pairs read, per class,
-// not yet built: J_sigma at (rows[eta], rows[xi]) -- both bra rows of one
-// determinant, a bra row then a ket row, or both ket rows of one
-// determinant; II at (alpha row, beta row) -- both of one bra determinant,
-// bra alpha x ket beta, ket alpha x bra beta, or both of one ket determinant.
-// The evaluators of pairs with zero-overlap orbitals (both = true) read J
-// rows in either order (the distribution sums and the slot swaps of
-// preparehamiltonianoverlap.rs:20550-20966).
-void missing_blocks(nw_ctx *c, const CholPair &cp, const ParentRows &X, const ParentRows &W, bool both,
-                    std::vector<int> miss[3])
+// Blocks (p, r) of pair (x, w) that its determinant pairs read, per class:
+// J_sigma at (rows[eta], rows[xi]) -- both bra rows of one determinant, a
+// bra row then a ket row, or both ket rows of one determinant; II at
+// (alpha row, beta row) -- both of one bra determinant, bra alpha x ket
+// beta, ket alpha x bra beta, or both of one ket determinant.  The
+// evaluators of pairs with zero-overlap orbitals (both = true) read J rows
+// in either order (the distribution sums and the slot swaps of
+// preparehamiltonianoverlap.rs:20550-20966).  Within a block they read the
+// columns (q, s) of the contraction labels: a bra hole (q < nocc or
+// s < nocc, dense) or two ket particles of one ket determinant (kk, sparse).
+bool block_needed(int cl, const ParentRows &X, const ParentRows &W, bool both, int p, int r, int nm)
 {
-    const int nm = c->nmo;
-    for (int sp = 0; sp < 2; ++sp) {
-        const std::vector<uint8_t> &cov = cp.cov[sp];
-        for (int p = 0; p < nm; ++p)
-            for (int r = 0; r < nm; ++r) {
-                const size_t k = (size_t)p * nm + r;
-                if (cov[k]) continue;
-                if (X.bra2[sp][k] || W.ket2[sp][k] || (X.bra[sp][p] && W.ket[sp][r]) ||
-                    (both && X.bra[sp][r] && W.ket[sp][p]))
-                    miss[sp].push_back((int)k);
-            }
-    }
-    const std::vector<uint8_t> &cov = cp.cov[CL_II];
-    for (int p = 0; p < nm; ++p)
-        for (int r = 0; r < nm; ++r) {
-            const size_t k = (size_t)p * nm + r;
-            if (cov[k]) continue;
-            if (X.braab[k] || W.ketab[k] || (X.bra[0][p] && W.ket[1][r]) || (W.ket[0][p] && X.bra[1][r]))
-                miss[CL_II].push_back((int)k);
-        }
+    const size_t k = (size_t)p * nm + r;
+    if (cl < 2)
+        return X.bra2[cl][k] || W.ket2[cl][k] || (X.bra[cl][p] && W.ket[cl][r]) || (both && X.bra[cl][r] && W.ket[cl][p]);
+    return X.braab[k] || W.ketab[k] || (X.bra[0][p] && W.ket[1][r]) || (W.ket[0][p] && X.bra[1][r]);
 }
 
-// A_K = X^T L_K Y for every job (X, Y: n x nm row-major), [naux][nm][nm]:
+// Work of one pair: per class, the blocks to (re)build and the union of the
+// ket-particle column pairs they must hold (a later call whose kets add
+// pairs rebuilds the class's built blocks with the union).
+struct ClassWork {
+    std::vector<int> blocks;   // p * nm + r: every block this call builds
+    std::vector<uint8_t> kk;   // nm x nm: column pairs (q, s) to hold
+    std::vector<std::pair<std::vector<int>, std::vector<int>>> rects;  // (p rows, r rows)
+    std::vector<int> single;   // blocks outside the rectangles
+};
+struct PairWork {
+    int key;
+    ClassWork cls[3];
+};
+
+// Lazy pairs touched by h and their work.
+bool plan_missing(nw_ctx *c, const std::vector<PackedDet> &h, std::vector<PairWork> &work)
+{
+    CholState *s = c->chol;
+    if (!s || s->pairs.empty()) return false;
+    bool any_lazy = false;
+    for (const CholPair &cp : s->pairs) any_lazy |= cp.lazy;
+    if (!any_lazy) return false;
+    const int nref = c->nref, nm = c->nmo;
+    const size_t nm2 = (size_t)nm * nm;
+    std::vector<ParentRows> pr = parent_rows(c, h);
+    for (int x = 0; x < nref; ++x) {
+        if (!pr[x].present) continue;
+        for (int w = 0; w < nref; ++w) {
+            if (!pr[w].present) continue;
+            const int key = x * nref + w;
+            const CholPair &cp = s->pairs[key];
+            const PairInfo &pi = c->pinfo[key];
+            if (!cp.lazy || !pi.built) continue;
+            const bool both = pi.ma + pi.mb > 0;
+            PairWork pw;
+            pw.key = key;
+            bool any = false;
+            for (int cl = 0; cl < 3; ++cl) {
+                const std::vector<uint8_t> &need_kk = cl < 2 ? pr[w].kcol2[cl] : pr[w].kcolab;
+                bool grow = false;
+                for (size_t k = 0; k < nm2 && !grow; ++k) grow = need_kk[k] && !cp.kkcov[cl][k];
+                ClassWork &cw = pw.cls[cl];
+                for (int p = 0; p < nm; ++p)
+                    for (int r = 0; r < nm; ++r) {
+                        const size_t k = (size_t)p * nm + r;
+                        const bool built = cp.cov[cl][k] != 0;
+                        if (built ? grow : block_needed(cl, pr[x], pr[w], both, p, r, nm)) cw.blocks.push_back((int)k);
+                    }
+                if (cw.blocks.empty()) continue;
+                cw.kk = cp.kkcov[cl];
+                for (size_t k = 0; k < nm2; ++k) cw.kk[k] |= need_kk[k];
+                // rectangles: bra rows of x against ket rows of w (both orders
+                // for J of pairs with zero-overlap orbitals; II: alpha bra x beta
+                // ket and alpha ket x beta bra), cut to the rows with work
+                std::vector<uint8_t> todo(nm2, 0), done(nm2, 0);
+                for (int k : cw.blocks) todo[k] = 1;
+                std::vector<std::pair<const std::vector<uint8_t> *, const std::vector<uint8_t> *>> cand;
+                if (cl < 2) {
+                    cand.push_back({&pr[x].bra[cl], &pr[w].ket[cl]});
+                    if (both) cand.push_back({&pr[w].ket[cl], &pr[x].bra[cl]});
+                } else {
+                    cand.push_back({&pr[x].bra[0], &pr[w].ket[1]});
+                    cand.push_back({&pr[w].ket[0], &pr[x].bra[1]});
+                }
+                for (auto &cd : cand) {
+                    std::vector<int> P, R;
+                    for (int p = 0; p < nm; ++p) {
+                        if (!(*cd.first)[p]) continue;
+                        bool any_r = false;
+                        for (int r = 0; r < nm && !any_r; ++r) any_r = (*cd.second)[r] && todo[(size_t)p * nm + r];
+                        if (any_r) P.push_back(p);
+                    }
+                    for (int r = 0; r < nm; ++r) {
+                        if (!(*cd.second)[r]) continue;
+                        bool any_p = false;
+                        for (int p : P) any_p = any_p || todo[(size_t)p * nm + r];
+                        if (any_p) R.push_back(r);
+                    }
+                    if (P.empty() || R.empty()) continue;
+                    for (int p : P)
+                        for (int r : R) done[(size_t)p * nm + r] = 1;
+                    cw.rects.push_back({std::move(P), std::move(R)});
+                }
+                for (int k : cw.blocks)
+                    if (!done[k]) cw.single.push_back(k);
+                cw.blocks.clear();
+                for (size_t k = 0; k < nm2; ++k)
+                    if (todo[k] || done[k]) cw.blocks.push_back((int)k);
+                any = true;
+            }
+            if (any) work.push_back(std::move(pw));
+        }
+    }
+    return !work.empty();
+}
+
+// A_K = X^T L_K Y for every job (X, Y: n x nm row-major), as [p][q][K]:
 // T1[q][K][a] = sum_b Y[b][q] L[K][a][b] (one product over the stacked
-// factors per distinct Y), then per K A_K[p][q] = sum_a X[a][p] T1[q][K][a];
+// factors per distinct Y), then out[p][(q, K)] = sum_a X[a][p] T1[(q, K)][a];
 // both as A B^T products on FP64 MFMA.
 struct FacJob {
     const double *X, *Y;
@@ -510,38 +711,44 @@ void factor_tensors(nw_ctx *c, const std::vector<FacJob> &jobs)
             const auto pos = std::find(ys.begin() + g0, ys.begin() + g0 + gn, yi[j]);
             if (pos == ys.begin() + g0 + gn) continue;
             const double *t1y = t1 + (size_t)(pos - (ys.begin() + g0)) * t1sz;
-            for (int K = 0; K < naux; ++K)
-                g2.push_back({trp[xi[j]], t1y + (size_t)K * n, jobs[j].out + (size_t)K * nm * nm});
+            g2.push_back({trp[xi[j]], t1y, jobs[j].out});
         }
-        launch_gemm_nt_mfma(c->st, nm, nm, n, n, naux * n, nm, up.put(g2), (int)g2.size());
+        launch_gemm_nt_mfma(c->st, nm, nm * naux, n, n, n, nm * naux, up.put(g2), (int)g2.size());
     }
     HIPCHK(hipGetLastError());
 }
 
-// Build the listed missing blocks of lazy pairs (per pair, per class: block
-// keys p * nm + r) and mark them built.  Per pair and class, every tensor of
-// the class (the spin's J slots, the II blocks) gets the same blocks:
+// Build the planned blocks and mark them built.  Per pair and class, every
+// tensor of the class (the spin's J slots, the II blocks) gets the same
+// blocks:
 //   J slot (i, j, k, l) = jbranch: sum_K A^{kl}[p,q] A^{ij}[r,s] (the slab block
 //     of the transform of (ab||cd) with cx_k, xc_l, cx_i, xc_j, build.rs:261-313),
 //     less sum_K E^{ki}[p,r] F^{lj}[q,s] for pairs with zero-overlap orbitals;
 //   II (a0, aj, b0, bj): sum_K A_alpha^{a0 aj}[p,q] A_beta^{b0 bj}[r,s].
-void build_blocks(nw_ctx *c, const std::vector<int> &pkeys, std::vector<std::vector<int>> miss[3])
+// Within a block, the columns of bra holes and the class's ket-particle
+// pairs.  Per rectangle P x R three product families: rows (p, q < oq) x
+// columns (r, every s); rows (p, q >= oq) x columns (r, s < os); and per ket
+// particle q its pairs S(q): rows (p, q) x columns (r, s in S(q)).  Exchange
+// products take every built block as a row and the read columns as columns.
+void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
 {
     CholState *s = c->chol;
     const int nm = c->nmo, naux = s->naux;
     const size_t nm2 = (size_t)nm * nm, nnm = (size_t)c->n * nm, ften = (size_t)naux * nm2;
     double *cx = (double *)c->keepCX.p, *xc = (double *)c->keepXC.p;
+    auto occ_q = [&](int cl) { return cl < 2 ? c->nocc[cl] : c->nocc[0]; };
+    auto occ_s = [&](int cl) { return cl < 2 ? c->nocc[cl] : c->nocc[1]; };
     struct TSpec {
-        int pair_i, cl;
+        int wi, cl;
         int P, Q, E, F;  // factor keys (E, F < 0: no exchange)
         long long off;
     };
     std::vector<TSpec> specs;
-    for (size_t i = 0; i < pkeys.size(); ++i) {
-        const PairInfo &pi = c->pinfo[pkeys[i]];
+    for (size_t i = 0; i < work.size(); ++i) {
+        const PairInfo &pi = c->pinfo[work[i].key];
         const bool exch = pi.ma + pi.mb > 0;
         for (int sp = 0; sp < 2; ++sp) {
-            if (miss[sp][i].empty()) continue;
+            if (work[i].cls[sp].blocks.empty()) continue;
             const long long *o = pi.off + (sp ? O_BB : O_AA);
             for (int sl = 0; sl < 10; ++sl) {
                 if (o[O_J + sl] < 0) continue;
@@ -550,7 +757,7 @@ void build_blocks(nw_ctx *c, const std::vector<int> &pkeys, std::vector<std::vec
                                  exch ? fkey(FE, sp, kk, ii) : -1, exch ? fkey(FF, sp, ll, jj) : -1, o[O_J + sl]});
             }
         }
-        if (!miss[CL_II][i].empty())
+        if (!work[i].cls[CL_II].blocks.empty())
             for (int a0 = 0; a0 < 2; ++a0)
                 for (int aj = 0; aj < 2; ++aj)
                     for (int b0 = 0; b0 < 2; ++b0)
@@ -564,9 +771,9 @@ void build_blocks(nw_ctx *c, const std::vector<int> &pkeys, std::vector<std::vec
     std::vector<FacJob> fj;
     auto need = [&](int i, int key) {
         if (key < 0) return;
-        CholPair &cp = s->pairs[pkeys[i]];
+        CholPair &cp = s->pairs[work[i].key];
         if (cp.fac.count(key)) return;
-        const PairInfo &pi = c->pinfo[pkeys[i]];
+        const PairInfo &pi = c->pinfo[work[i].key];
         const int kind = key >> 3, sp = (key >> 2) & 1, a = (key >> 1) & 1, b = key & 1;
         const int pr = pi.bidx * 2 + sp;
         const double *ca = cx + (size_t)(pr * 2 + a) * nnm, *cb = cx + (size_t)(pr * 2 + b) * nnm;
@@ -578,14 +785,14 @@ void build_blocks(nw_ctx *c, const std::vector<int> &pkeys, std::vector<std::vec
         else fj.push_back({xa, xb, t});
     };
     for (const TSpec &t : specs) {
-        need(t.pair_i, t.P);
-        need(t.pair_i, t.Q);
-        need(t.pair_i, t.E);
-        need(t.pair_i, t.F);
+        need(t.wi, t.P);
+        need(t.wi, t.Q);
+        need(t.wi, t.E);
+        need(t.wi, t.F);
     }
     if (!fj.empty()) factor_tensors(c, fj);
     auto f32of = [&](int i, int key) -> const void * {
-        CholPair &cp = s->pairs[pkeys[i]];
+        CholPair &cp = s->pairs[work[i].key];
         auto it = cp.fac32.find(key);
         if (it != cp.fac32.end()) return it->second;
         float *t = (float *)s->arena(ften * sizeof(float));
@@ -593,93 +800,174 @@ void build_blocks(nw_ctx *c, const std::vector<int> &pkeys, std::vector<std::vec
         cp.fac32[key] = t;
         return t;
     };
-    // direct products, then the exchange products of the same blocks, per precision
-    for (int f32 = 0; f32 < 2; ++f32) {
-        std::vector<R4Tensor> ts, tx;
-        std::vector<R4Item> its, itx;
-        std::vector<R4XTile> tiles;
-        for (const TSpec &t : specs) {
-            const PairInfo &pi = c->pinfo[pkeys[t.pair_i]];
-            if ((pi.fp32 != 0) != (f32 == 1)) continue;
-            CholPair &cp = s->pairs[pkeys[t.pair_i]];
-            auto ptr = [&](int key) -> const void * {
-                return f32 ? f32of(t.pair_i, key) : (const void *)cp.fac.at(key);
-            };
-            void *out = (void *)(c->d_slab + pi.base + t.off);
-            const std::vector<int> &blocks = miss[t.cl][t.pair_i];
-            const int ti = (int)ts.size();
-            ts.push_back({ptr(t.P), ptr(t.Q), out});
-            for (int k : blocks) its.push_back({ti, k / nm, k % nm, 0});
-            if (t.E >= 0) {
-                const int xi = (int)tx.size();
-                tx.push_back({ptr(t.E), ptr(t.F), out});
-                const int i0 = (int)itx.size();
-                for (int k : blocks) itx.push_back({xi, k / nm, k % nm, 0});
-                for (int b = 0; b < (int)blocks.size(); b += 64)
-                    tiles.push_back({xi, i0 + b, std::min(64, (int)blocks.size() - b), 0});
+    // index lists shared by the products of a (pair, class), in one upload:
+    // rectangle rows, per-q ket-particle partners, sparse lists of single
+    // blocks, exchange rows and columns.  Offsets are patched after upload.
+    std::vector<int> ints;
+    auto put = [&](const std::vector<int> &v) {
+        const size_t o = ints.size();
+        ints.insert(ints.end(), v.begin(), v.end());
+        return o;
+    };
+    struct ClassLists {
+        std::vector<std::pair<size_t, size_t>> rect;  // offsets of P and R lists
+        std::vector<std::pair<int, size_t>> kq;       // (q, offset of S(q)) with |S(q)| in kqn
+        std::vector<int> kqn;
+        size_t single_list = 0;
+        int nsingle_list = 0;
+        size_t xp = 0, xr = 0, xq = 0, xs = 0;        // exchange rows (p, r) and columns (q, s)
+        int nxrow = 0, nxcol = 0;
+    };
+    std::vector<ClassLists> cls(work.size() * 3);
+    for (size_t i = 0; i < work.size(); ++i)
+        for (int cl = 0; cl < 3; ++cl) {
+            const ClassWork &cw = work[i].cls[cl];
+            if (cw.blocks.empty()) continue;
+            ClassLists &L = cls[i * 3 + cl];
+            const int oq = occ_q(cl), os = occ_s(cl);
+            for (auto &rc : cw.rects) L.rect.push_back({put(rc.first), put(rc.second)});
+            for (int q = oq; q < nm; ++q) {
+                std::vector<int> S;
+                for (int t = os; t < nm; ++t)
+                    if (cw.kk[(size_t)q * nm + t]) S.push_back(t);
+                if (!S.empty()) {
+                    L.kq.push_back({q, put(S)});
+                    L.kqn.push_back((int)S.size());
+                }
+            }
+            const int qe = 16 * ((oq + 15) / 16), se = 16 * ((os + 15) / 16);
+            std::vector<int> dl;
+            for (int q = qe; q < nm; ++q)
+                for (int t = se; t < nm; ++t)
+                    if (cw.kk[(size_t)q * nm + t]) dl.push_back(q * nm + t);
+            L.single_list = put(dl);
+            L.nsingle_list = (int)dl.size();
+            if (c->pinfo[work[i].key].ma + c->pinfo[work[i].key].mb > 0 && cl < 2) {
+                std::vector<int> rp, rr, cq, cs;
+                for (int k : cw.blocks) {
+                    rp.push_back(k / nm);
+                    rr.push_back(k % nm);
+                }
+                for (int q = 0; q < nm; ++q)
+                    for (int t = 0; t < nm; ++t)
+                        if (q < oq || t < os || cw.kk[(size_t)q * nm + t]) {
+                            cq.push_back(q);
+                            cs.push_back(t);
+                        }
+                L.xp = put(rp);
+                L.xr = put(rr);
+                L.xq = put(cq);
+                L.xs = put(cs);
+                L.nxrow = (int)rp.size();
+                L.nxcol = (int)cq.size();
             }
         }
-        if (its.empty()) continue;
-        const size_t bytes = (its.size() + itx.size()) * sizeof(R4Item) + (ts.size() + tx.size()) * sizeof(R4Tensor) +
-                             tiles.size() * sizeof(R4XTile) + 8192;
-        Uploader up(c, std::max<size_t>(1u << 20, bytes));
-        const R4Tensor *dts = up.put(ts);
-        const R4Item *dit = up.put(its);
-        const int nw = (nm + 15) / 16;
-        if (f32) launch_r4<float>(c->st, nw, dts, dit, (int)its.size(), nm, naux);
-        else launch_r4<double>(c->st, nw, dts, dit, (int)its.size(), nm, naux);
-        if (!tiles.empty()) {
-            const R4Tensor *dtx = up.put(tx);
-            const R4Item *ditx = up.put(itx);
-            const R4XTile *dtl = up.put(tiles);
-            const unsigned ncol = (unsigned)((nm2 + 63) / 64);
-            for (size_t t0 = 0; t0 < tiles.size(); t0 += 65535) {
-                const unsigned nt = (unsigned)std::min<size_t>(65535, tiles.size() - t0);
-                if (f32)
-                    hipLaunchKernelGGL(k_r4_exch<float>, dim3(nt, ncol), dim3(256), 0, c->st, dtx, ditx, dtl + t0, nm, naux);
-                else
-                    hipLaunchKernelGGL(k_r4_exch<double>, dim3(nt, ncol), dim3(256), 0, c->st, dtx, ditx, dtl + t0, nm,
-                                       naux);
+    // List pointers hold (offset + 1) into `ints` until the upload; 0 = none.
+    const int *dints = nullptr;
+    auto at = [](size_t off) { return (const int *)(uintptr_t)(off + 1); };
+    std::vector<R4Gemm> gd[2], gx[2];
+    std::vector<R4GTile> td[2], tx[2];
+    std::vector<R4Tensor> ts[2];
+    std::vector<R4Item> its[2];
+    for (int f32 = 0; f32 < 2; ++f32) {
+        auto tiles_of = [&](std::vector<R4GTile> &tl, int g, int M, int N) {
+            for (int m0 = 0; m0 < M; m0 += 64)
+                for (int n0 = 0; n0 < N; n0 += 64) tl.push_back({g, m0, n0, 0});
+        };
+        for (const TSpec &t : specs) {
+            const PairWork &pw = work[t.wi];
+            const PairInfo &pi = c->pinfo[pw.key];
+            if ((pi.fp32 != 0) != (f32 == 1)) continue;
+            CholPair &cp = s->pairs[pw.key];
+            auto ptr = [&](int key) -> const void * { return f32 ? f32of(t.wi, key) : (const void *)cp.fac.at(key); };
+            void *out = (void *)(c->d_slab + pi.base + t.off);
+            const ClassWork &cw = pw.cls[t.cl];
+            const ClassLists &L = cls[t.wi * 3 + t.cl];
+            const int oq = occ_q(t.cl), os = occ_s(t.cl);
+            const void *PA = ptr(t.P), *QB = ptr(t.Q);
+            for (size_t ri = 0; ri < cw.rects.size(); ++ri) {
+                const int np = (int)cw.rects[ri].first.size(), nr = (int)cw.rects[ri].second.size();
+                const int *Pl = at(L.rect[ri].first), *Rl = at(L.rect[ri].second);
+                if (oq > 0) {  // rows (p, q < oq) x columns (r, every s)
+                    R4Gemm g{PA, QB, out, {Pl, nullptr, np, oq, 0, 0}, {Rl, nullptr, nr, nm, 0, 0}, np * oq, nr * nm, 0, 0};
+                    tiles_of(td[f32], (int)gd[f32].size(), g.M, g.N);
+                    gd[f32].push_back(g);
+                }
+                if (os > 0 && oq < nm) {  // rows (p, q >= oq) x columns (r, s < os)
+                    R4Gemm g{PA, QB, out, {Pl, nullptr, np, nm - oq, oq, 0}, {Rl, nullptr, nr, os, 0, 0},
+                             np * (nm - oq), nr * os, 0, 0};
+                    tiles_of(td[f32], (int)gd[f32].size(), g.M, g.N);
+                    gd[f32].push_back(g);
+                }
+                for (size_t k = 0; k < L.kq.size(); ++k) {  // rows (p, q) x columns (r, s in S(q))
+                    const int q = L.kq[k].first, ns = L.kqn[k];
+                    R4Gemm g{PA, QB, out, {Pl, nullptr, np, 1, q, 0}, {Rl, at(L.kq[k].second), nr, ns, 0, 0},
+                             np, nr * ns, 0, 0};
+                    tiles_of(td[f32], (int)gd[f32].size(), g.M, g.N);
+                    gd[f32].push_back(g);
+                }
             }
+            if (!cw.single.empty()) {
+                const int ti = (int)ts[f32].size();
+                ts[f32].push_back({PA, QB, out, at(L.single_list), L.nsingle_list, (oq + 15) / 16, (os + 15) / 16, 0});
+                for (int k : cw.single) its[f32].push_back({ti, k / nm, k % nm, 0});
+            }
+            if (t.E >= 0 && L.nxrow > 0 && L.nxcol > 0) {
+                R4Gemm g{ptr(t.E), ptr(t.F), out, {at(L.xp), at(L.xr), L.nxrow, 1, 0, 1},
+                         {at(L.xq), at(L.xs), L.nxcol, 1, 0, 1}, L.nxrow, L.nxcol, 1, 0};
+                tiles_of(tx[f32], (int)gx[f32].size(), g.M, g.N);
+                gx[f32].push_back(g);
+            }
+        }
+    }
+    // one upload of the lists and every descriptor, then the launches
+    size_t bytes = ints.size() * sizeof(int) + 8192;
+    for (int f = 0; f < 2; ++f)
+        bytes += (gd[f].size() + gx[f].size()) * sizeof(R4Gemm) + (td[f].size() + tx[f].size()) * sizeof(R4GTile) +
+                 ts[f].size() * sizeof(R4Tensor) + its[f].size() * sizeof(R4Item) + 2048;
+    Uploader up(c, bytes);
+    dints = up.put(ints);
+    auto fix = [&](const int *p) { return p ? dints + ((uintptr_t)p - 1) : nullptr; };
+    for (int f = 0; f < 2; ++f) {
+        for (auto *v : {&gd[f], &gx[f]})
+            for (R4Gemm &g : *v) {
+                g.row.u = fix(g.row.u);
+                g.row.v = fix(g.row.v);
+                g.col.u = fix(g.col.u);
+                g.col.v = fix(g.col.v);
+            }
+        for (R4Tensor &t : ts[f]) t.list = fix(t.list);
+    }
+    for (int f32 = 0; f32 < 2; ++f32) {
+        if (!td[f32].empty()) {
+            const R4Gemm *dg = up.put(gd[f32]);
+            const R4GTile *dt = up.put(td[f32]);
+            if (f32) launch_r4g<float>(c->st, dg, dt, (int)td[f32].size(), nm, naux);
+            else launch_r4g<double>(c->st, dg, dt, (int)td[f32].size(), nm, naux);
+        }
+        if (!its[f32].empty()) {
+            const R4Tensor *dts = up.put(ts[f32]);
+            const R4Item *dit = up.put(its[f32]);
+            const int nw = (nm + 15) / 16;
+            if (f32) launch_r4<float>(c->st, nw, dts, dit, (int)its[f32].size(), nm, naux);
+            else launch_r4<double>(c->st, nw, dts, dit, (int)its[f32].size(), nm, naux);
+        }
+        if (!tx[f32].empty()) {  // after the direct products of the same blocks (stream order)
+            const R4Gemm *dg = up.put(gx[f32]);
+            const R4GTile *dt = up.put(tx[f32]);
+            if (f32) launch_r4g<float>(c->st, dg, dt, (int)tx[f32].size(), nm, naux);
+            else launch_r4g<double>(c->st, dg, dt, (int)tx[f32].size(), nm, naux);
         }
         HIPCHK(hipGetLastError());
     }
-    for (size_t i = 0; i < pkeys.size(); ++i) {
-        CholPair &cp = s->pairs[pkeys[i]];
-        for (int cl = 0; cl < 3; ++cl)
-            for (int k : miss[cl][i]) cp.cov[cl][k] = 1;
-    }
-}
-
-// Lazy pairs touched by h and the blocks they still need.
-bool plan_missing(nw_ctx *c, const std::vector<PackedDet> &h, std::vector<int> &pkeys,
-                  std::vector<std::vector<int>> miss[3])
-{
-    CholState *s = c->chol;
-    if (!s || s->pairs.empty()) return false;
-    bool any_lazy = false;
-    for (const CholPair &cp : s->pairs) any_lazy |= cp.lazy;
-    if (!any_lazy) return false;
-    const int nref = c->nref;
-    std::vector<ParentRows> pr = parent_rows(c, h);
-    bool any = false;
-    for (int x = 0; x < nref; ++x) {
-        if (!pr[x].present) continue;
-        for (int w = 0; w < nref; ++w) {
-            if (!pr[w].present) continue;
-            const int key = x * nref + w;
-            const CholPair &cp = s->pairs[key];
-            if (!cp.lazy || !c->pinfo[key].built) continue;
-            std::vector<int> m[3];
-            const PairInfo &pi = c->pinfo[key];
-            missing_blocks(c, cp, pr[x], pr[w], pi.ma + pi.mb > 0, m);
-            if (m[0].empty() && m[1].empty() && m[2].empty()) continue;
-            pkeys.push_back(key);
-            for (int cl = 0; cl < 3; ++cl) miss[cl].push_back(std::move(m[cl]));
-            any = true;
+    for (PairWork &pw : work) {
+        CholPair &cp = s->pairs[pw.key];
+        for (int cl = 0; cl < 3; ++cl) {
+            if (pw.cls[cl].blocks.empty()) continue;
+            for (int k : pw.cls[cl].blocks) cp.cov[cl][k] = 1;
+            cp.kkcov[cl] = pw.cls[cl].kk;
         }
     }
-    return any;
 }
 
 }  // namespace
@@ -689,11 +977,10 @@ void chol_ensure(nw_ctx *c, const std::vector<PackedDet> &h)
     CholState *s = c->chol;
     if (!s) return;
     s->ms_blocks = 0.0;
-    std::vector<int> pkeys;
-    std::vector<std::vector<int>> miss[3];
-    if (!plan_missing(c, h, pkeys, miss)) return;
+    std::vector<PairWork> work;
+    if (!plan_missing(c, h, work)) return;
     HIPCHK(hipEventRecord(c->ev[3], c->st));
-    build_blocks(c, pkeys, miss);
+    build_blocks(c, work);
     hipEvent_t e1;
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e1, c->st));
@@ -704,9 +991,8 @@ void chol_ensure(nw_ctx *c, const std::vector<PackedDet> &h)
 
 bool chol_missing(nw_ctx *c, const std::vector<PackedDet> &h)
 {
-    std::vector<int> pkeys;
-    std::vector<std::vector<int>> miss[3];
-    return plan_missing(c, h, pkeys, miss);
+    std::vector<PairWork> work;
+    return plan_missing(c, h, work);
 }
 
 void chol_complete(nw_ctx *c)
@@ -723,6 +1009,7 @@ void chol_complete(nw_ctx *c)
         CholPair &cp = s->pairs[k];
         cp.lazy = false;
         for (auto &v : cp.cov) std::fill(v.begin(), v.end(), 1);
+        for (auto &v : cp.kkcov) std::fill(v.begin(), v.end(), 1);
     }
 }
 
