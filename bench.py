@@ -11,7 +11,12 @@ pairs), 512 random singles/doubles per reference: 4096 determinants and
 One step = the whole hot path on inputs resident in HBM: the per-reference-
 pair build for every ordered pair the upper triangle needs (x <= w: 36 pairs)
 followed by the evaluation of all 8,390,656 H and S elements into device
-matrices.  value = pairs / second of that end-to-end step.
+matrices.  value = pairs / second of that end-to-end step.  The step runs in
+dense forms (every rank-4 tensor transformed from the dense ERIs; FP64, FP32
+and bf16x3 transforms) and in factored forms (the workload's own Cholesky
+vectors installed with nw_set_eri_factors: the rank-4 blocks the
+determinants read built on demand, FP64 and FP32 products); value = the
+fastest mixed form whose NOCI energy is within 1 uHa of the dense FP64 one.
 """
 import argparse
 import json
@@ -780,26 +785,36 @@ def wick_workload(args):
         A = torch.linalg.solve_triangular(L, A.T.contiguous(), upper=False)
         return float(torch.linalg.eigvalsh(0.5 * (A + A.T))[0])
 
-    def phase(prec):
+    def phase(prec, factored=False):
+        """Steps of one form.  Dense forms: the build transforms every rank-4
+        tensor (transforms = its transform time).  Factored forms (integral
+        factors installed): the build leaves the rank-4 blocks to hs_matrix,
+        which builds the blocks the determinants read (transforms = that
+        time; stages from nw_rank4_stats)."""
         eng.set_precision(prec)
+        name = ("factored " if factored else "") + PREC_NAME[prec]
+        stages = []
         for w in range(args.warmup):
             meta, tb, te = step()
-            log(f"[{PREC_NAME[prec]}] warmup {w}: build {tb['total_ms']:.1f} ms "
+            log(f"[{name}] warmup {w}: build {tb['total_ms']:.1f} ms "
                 f"(transforms {tb['transform_ms']:.1f} ms), eval {te['eval_ms']:.2f} ms")
         torch.cuda.synchronize()
         builds, transforms, evals = [], [], []
         t0 = time.perf_counter()
         for k in range(args.steps):
             meta, tb, te = step()
-            builds.append(tb["total_ms"]); transforms.append(tb["transform_ms"]); evals.append(te["eval_ms"])
-            log(f"[{PREC_NAME[prec]}] step {k}: build {tb['total_ms']:.1f} ms "
-                f"(transforms {tb['transform_ms']:.1f} ms), eval {te['eval_ms']:.2f} ms")
+            r4 = te["transform_ms"] if factored else tb["transform_ms"]
+            builds.append(tb["total_ms"] + (r4 if factored else 0.0)); transforms.append(r4); evals.append(te["eval_ms"])
+            if factored:
+                stages.append(eng.rank4_stats())
+            log(f"[{name}] step {k}: build {tb['total_ms']:.1f} ms, rank-4 {r4:.1f} ms, eval {te['eval_ms']:.2f} ms")
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         energy = float("nan") if args.no_energy else lowest_energy()
-        log(f"[{PREC_NAME[prec]}] {npairs * args.steps / elapsed:.4e} pairs/s, lowest NOCI energy {energy:.12f}")
+        log(f"[{name}] {npairs * args.steps / elapsed:.4e} pairs/s, lowest NOCI energy {energy:.12f}")
+        st = {k: float(np.mean([x[k] for x in stages])) for k in stages[0]} if stages else None
         return {"elapsed": elapsed, "meta": meta, "builds": builds, "transforms": transforms, "evals": evals,
-                "energy": energy, "rate": npairs * args.steps / elapsed}
+                "energy": energy, "rate": npairs * args.steps / elapsed, "stages": st}
 
     PREC_NAME = {nwick.PREC_FP64: "fp64", nwick.PREC_MIXED: "mixed", nwick.PREC_BF16X3: "bf16x3"}
     res64 = phase(nwick.PREC_FP64)
@@ -809,7 +824,18 @@ def wick_workload(args):
             r = phase(prec)
             r["dE"] = None if args.no_energy else r["energy"] - res64["energy"]
             mixed_runs[prec] = r
-    # report the faster mixed form whose energy is within 1 uHa of FP64
+    # the same steps with the workload's integral factors (n_aux = 300): rank-4
+    # blocks built on demand, FP64 and FP32 products
+    dev_f = eng.set_eri_factors(sysm.chol)
+    resf64 = phase(nwick.PREC_FP64, factored=True)
+    resf64["dE"] = None if args.no_energy else resf64["energy"] - res64["energy"]
+    resfmx = None
+    if args.precision == "mixed":
+        resfmx = phase(nwick.PREC_MIXED, factored=True)
+        resfmx["dE"] = None if args.no_energy else resfmx["energy"] - res64["energy"]
+        mixed_runs["factored"] = resfmx
+    eng.set_eri_factors(None)
+    # report the fastest mixed form whose energy is within 1 uHa of the dense FP64 one
     ok = [p for p, r in mixed_runs.items() if args.no_energy or abs(r["dE"]) < 1e-6]
     best = max(ok, key=lambda p: mixed_runs[p]["rate"]) if ok else None
     resmx = mixed_runs.get(nwick.PREC_MIXED)
@@ -820,6 +846,12 @@ def wick_workload(args):
     ms_per_step = 1e3 * res["elapsed"] / args.steps
     value = res["rate"]
     transforms, evals, builds = res["transforms"], res["evals"], res["builds"]
+    # roofline of the dense four-index transforms: the fastest dense mixed
+    # form within 1 uHa (or the FP64 transforms)
+    dok = [p for p in ok if p != "factored"]
+    dbest = max(dok, key=lambda p: mixed_runs[p]["rate"]) if dok else None
+    dense_mixed = dbest is not None
+    dres = mixed_runs[dbest] if dense_mixed else res64
     # dominant kernel: the rotating quarter transform, 2 * n_bas * rows * nmo = 2 n^5 flop per tensor pass;
     # the build launches up to g tensors per dispatch (g tensors of scratch fit in 1 GiB)
     n = sysm.n_bas
@@ -832,30 +864,30 @@ def wick_workload(args):
                   if a0 + aj <= pm.ma and b0 + bj <= pm.mb)
         k = slots(pm.ma) + slots(pm.mb) + nii
         ntensor += k
-        if use_mixed and res["meta"][(x, w)].fp32:
+        if dense_mixed and dres["meta"][(x, w)].fp32:
             n32 += k
     flops_tensor = 4 * 2.0 * n ** 5
     t64 = float(np.mean(res64["transforms"])) / ntensor  # ms per FP64 tensor
     group = lambda elem: max(1, min(32, (1 << 30) // (n ** 4 * elem)))
-    if use_mixed:
-        t_k = float(np.mean(transforms)) - (ntensor - n32) * t64
+    if dense_mixed:
+        t_k = float(np.mean(dres["transforms"])) - (ntensor - n32) * t64
         nk, g = n32, group(4)
     else:
-        t_k = float(np.mean(transforms))
+        t_k = float(np.mean(res64["transforms"]))
         nk, g = ntensor, group(8)
     dispatches = 4 * ((nk + g - 1) // g)
     achieved = nk * flops_tensor / (t_k * 1e-3) / 1e12
     avg_ms = t_k / dispatches
     eval_rate = npairs / (float(np.mean(evals)) * 1e-3)
-    if use_mixed and best == nwick.PREC_BF16X3:
+    if dense_mixed and dbest == nwick.PREC_BF16X3:
         # six BF16 products per FP32-equivalent product: the emulated peak is dense BF16 MFMA / 6
         kname, peak = "k_qtransform_bf3", PEAK_BF16_TFLOPS / 6.0
-    elif use_mixed:
+    elif dense_mixed:
         kname, peak = "k_qtransform_f32_pipe", PEAK_FP32_TFLOPS
     else:
         kname, peak = "k_qtransform_mfma_pipe", PEAK_FP64_TFLOPS
     traffic = None
-    pname = ("qtransform_bf3" if best == nwick.PREC_BF16X3 else "qtransform_f32") if use_mixed else "qtransform"
+    pname = ("qtransform_bf3" if dbest == nwick.PREC_BF16X3 else "qtransform_f32") if dense_mixed else "qtransform"
     tfile = os.path.join(ROOT, "profiles", f"{pname}_n100_hbm_traffic.txt")
     if os.path.exists(tfile):
         for line in open(tfile):
@@ -864,7 +896,7 @@ def wick_workload(args):
     # algorithmic bytes per dispatch: FP64 passes read and write n^4 doubles; a
     # reduced-precision pass reads and writes n^4 floats (the last pass too:
     # its rank-4 slab blocks are stored in FP32), g tensors per dispatch
-    alg_bytes = g * 8.0 * n ** 4 if use_mixed else 16.0 * n ** 4
+    alg_bytes = g * 8.0 * n ** 4 if dense_mixed else 16.0 * n ** 4
     flops_disp = g * flops_tensor / 4
     ridge = peak / (PEAK_HBM_GBS / 1e3)  # flop per byte where the MFMA and HBM roofs meet
     hbm_bound = flops_disp / alg_bytes < ridge
@@ -911,6 +943,39 @@ def wick_workload(args):
         for line in open(tfile_e):
             if line.startswith("TRAFFIC_BYTES"):
                 roof_eval["traffic"] = float(line.split()[1])
+    roof_dense = dict(roof, **{"kernel": kname + " (rotating quarter transform)", "traffic": traffic,
+                      "traffic_source": os.path.relpath(tfile, ROOT) + " (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
+                      "algorithmic_bytes_per_dispatch": alg_bytes,
+                      "dispatches_per_step": dispatches, "tensors_per_dispatch": g, "avg_launch_ms": avg_ms,
+                      "flops_per_dispatch": flops_disp})
+
+    def stage_roofs(r, mixed):
+        """MFMA rooflines of the on-demand rank-4 stages of a factored run:
+        algorithmic flops (nw_rank4_stats) over the stage's device time.  The
+        factor products run on FP64 MFMA; the block products of a mixed run on
+        FP32 MFMA except those of pairs under the sigma floor (FP64), which
+        this peak leaves in, so their fraction is a lower bound."""
+        st = r["stages"]
+        peak_b = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
+        out_s = {}
+        for key, ms, fl, kern, peak in (
+                ("factor", "ms_factor", "flops_factor", "k_gemm_nt_mfma (factor products X^T L_K Y)", PEAK_FP64_TFLOPS),
+                ("rect", "ms_rect", "flops_rect", "k_r4_gemm (rectangle products)", peak_b),
+                ("single", "ms_single", "flops_single", "k_r4_block (single blocks)", peak_b),
+                ("exchange", "ms_exchange", "flops_exchange", "k_r4_gemm (exchange products)", peak_b)):
+            if st[ms] <= 0.0:
+                continue
+            ach = st[fl] / (st[ms] * 1e-3) / 1e12
+            out_s[key] = {"bound": "mfma", "kernel": kern, "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                          "frac": ach / peak, "ms_per_step": st[ms], "flops_per_step": st[fl]}
+        return out_s
+    stages_f64 = stage_roofs(resf64, False)
+    stages_fmx = stage_roofs(resfmx, True) if resfmx is not None else None
+    if best == "factored":
+        dom = max(stages_fmx.items(), key=lambda kv: kv[1]["ms_per_step"])
+        headline_roof = dict(dom[1], **{"traffic": None, "stage": dom[0]})
+    else:
+        headline_roof = roof_dense
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: oracle on pair (0,1) ...")
@@ -929,12 +994,14 @@ def wick_workload(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("bf16x3+f64" if best == nwick.PREC_BF16X3 else "f32+f64") if use_mixed else "f64",
+        "dtype": ({nwick.PREC_BF16X3: "bf16x3+f64", nwick.PREC_MIXED: "f32+f64", "factored": "f32+f64"}[best]
+                  if use_mixed else "f64"),
         "data": "synthetic",
         "config": {"workload": "batched Wick elements, n_bas=nmo=100, nocc=10+10, 8 refs (1 with an exactly "
                                "orthogonal alpha orbital), 512 singles/doubles per ref, 4096 dets, 8390656 pairs",
                    "mode": "reference" if args.mode == 0 else "corrected", "pairs_built": len(meta),
-                   "precision": PREC_NAME[best] if use_mixed else "fp64",
+                   "precision": ("factored mixed" if best == "factored" else PREC_NAME[best]) if use_mixed else "fp64",
+                   "integrals": "dense ERIs; factored forms also take the workload's 300 Cholesky vectors",
                    "rank4_tensors": ntensor, "fp32_tensors": n32},
         "fp64_pairs_per_s": res64["rate"],
         "mixed_pairs_per_s": None if resmx is None else resmx["rate"],
@@ -947,15 +1014,23 @@ def wick_workload(args):
         "dE_mixed_minus_fp64": dE,
         "dE_bf16x3_minus_fp64": mixed_runs[nwick.PREC_BF16X3]["dE"] if nwick.PREC_BF16X3 in mixed_runs else None,
         "value_over_fp64": value / res64["rate"],
+        "factored_fp64_pairs_per_s": resf64["rate"],
+        "factored_fp64_over_fp64": resf64["rate"] / res64["rate"],
+        "dE_factored_fp64_minus_fp64": resf64["dE"],
+        "factored_mixed_pairs_per_s": None if resfmx is None else resfmx["rate"],
+        "factored_mixed_over_fp64": None if resfmx is None else resfmx["rate"] / res64["rate"],
+        "factored_mixed_over_factored_fp64": None if resfmx is None else resfmx["rate"] / resf64["rate"],
+        "dE_factored_mixed_minus_fp64": None if resfmx is None else resfmx["dE"],
+        "factor_check_deviation": dev_f,
+        "factored_rank4_ms": {"fp64": float(np.mean(resf64["transforms"])),
+                              "mixed": None if resfmx is None else float(np.mean(resfmx["transforms"]))},
         "eval_only_pairs_per_s": eval_rate,
         "build_ms": float(np.mean(builds)),
         "transform_ms": float(np.mean(transforms)),
         "eval_ms": float(np.mean(evals)),
-        "roofline": dict(roof, **{"kernel": kname + " (rotating quarter transform)", "traffic": traffic,
-                     "traffic_source": os.path.relpath(tfile, ROOT) + " (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
-                     "algorithmic_bytes_per_dispatch": alg_bytes,
-                     "dispatches_per_step": dispatches, "tensors_per_dispatch": g, "avg_launch_ms": avg_ms,
-                     "flops_per_dispatch": flops_disp}),
+        "roofline": headline_roof,
+        "roofline_stages": {"factored_fp64": stages_f64, "factored_mixed": stages_fmx},
+        "roofline_dense_transform": roof_dense,
         "roofline_eval": roof_eval,
         "cpu_baseline": cpu,
     }

@@ -110,6 +110,14 @@ int nw_set_eri_factors(nw_ctx *ctx, const double *L, int naux, double check_tol,
  * pair built without factors.  Diagnostics; no reference counterpart. */
 int nw_rank4_coverage(nw_ctx *ctx, int x, int w, double *frac3);
 
+/* The on-demand rank-4 work of the last evaluation call that built blocks:
+ * out10 = {device ms in total, of the factor products, of the rectangle
+ * products, of the single blocks, of the exchange products; algorithmic
+ * flops of the rectangle products, single blocks, exchange products and
+ * factor products; single blocks built}.  Zeros when nothing was built.
+ * Diagnostics (measurement); no reference counterpart. */
+int nw_rank4_stats(nw_ctx *ctx, double *out10);
+
 /* Install the reference determinants (the `refs` of build_wicks_shared,
  * src/noci/wicks.rs:133).  ca, cb: nref x n_bas x nmo row-major MO
  * coefficients; occ: nref x 4 uint64 {oa.lo, oa.hi, ob.lo, ob.hi}.  Occupied

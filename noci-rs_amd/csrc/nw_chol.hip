@@ -22,9 +22,9 @@
 // s < nocc) or two ket particles of one ket determinant.  The (p, r) blocks
 // an evaluation call reads form rectangles (every bra row of one parent x
 // every ket row of the other) plus a few single blocks (two rows of one
-// determinant).  Each rectangle's bra-hole columns and ket-particle pairs
-// are row-gathered A B^T products on MFMA (k_r4_gemm); the single blocks
-// take a per-block kernel (k_r4_block).  Only what a call reads is built,
+// determinant).  Each rectangle's bra-hole columns and ket-particle pairs,
+// and the single blocks whole (grouped by row p), are row-gathered A B^T
+// products on MFMA (k_r4_gemm).  Only what a call reads is built,
 // into the reference slab layout; a later call builds what it adds.
 // Exports complete the tensors with the dense transforms.
 #include <algorithm>
@@ -36,18 +36,6 @@
 #include "nw_internal.h"
 
 namespace nw {
-
-struct R4Tensor {
-    const void *P, *Q;  // factor tensors [nm][nm][naux] (double or float), K contiguous
-    void *out;          // rank-4 slab block [nm][nm][nm][nm] (double or float)
-    const int *list;    // sparse (q, s) entries outside the dense tiles, q * nm + s
-    int nlist;
-    int qt, st;         // dense region: 16-row tiles q < 16 qt, or 16-column tiles s < 16 st
-    int pad;
-};
-struct R4Item {
-    int tensor, p, r, pad;
-};
 
 typedef double r4_d4 __attribute__((ext_vector_type(4)));
 typedef float r4_f4 __attribute__((ext_vector_type(4)));
@@ -73,117 +61,6 @@ struct R4Mfma<float> {
     }
     static __device__ inline int row(int lk, int r) { return 4 * lk + r; }
 };
-
-// Single (row, row) blocks (the few a rectangle does not cover: both rows of
-// one determinant): out[p][q][r][s] = sum_K P[p][q][K] Q[r][s][K] over the
-// entries the evaluators read -- the columns of bra holes (every 16 x 16
-// tile with q < 16 qt or s < 16 st, on MFMA) and the listed ket-particle
-// pairs outside those tiles (VALU sums from the same staged chunks).  Four
-// waves; a pass takes 16 tiles (4 per wave) and 1,024 listed entries (4 per
-// thread); more passes restage K.  K is staged through LDS 128 bytes per row
-// at a time ([row][k]).  Items are dealt to the eight XCDs in contiguous
-// chunks (workgroup b runs chunk b mod 8).
-template <typename T, int NW>
-__global__ __launch_bounds__(256) void k_r4_block(const R4Tensor *__restrict__ ts,
-                                                  const R4Item *__restrict__ items, int nitems, int nm, int naux)
-{
-    constexpr int NP = 16 * NW, KC = 128 / sizeof(T), NT = 256, LDK = KC + 1;
-    constexpr int PER = 2 * KC * NP / NT;  // elements of the two operand chunks per thread
-    static_assert(PER * NT == 2 * KC * NP, "operand chunks divide over the workgroup");
-    constexpr int TPW = 4, SPT = 4;  // tiles per wave, listed entries per thread, per pass
-    __shared__ T Ps[NP][LDK];
-    __shared__ T Qs[NP][LDK];
-    using M = R4Mfma<T>;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
-    const int gper = gridDim.x / 8, chunk = blockIdx.x % 8, lidx = blockIdx.x / 8;
-    const int per = (nitems + 7) / 8;
-    const int i0 = chunk * per, i1 = min(nitems, i0 + per);
-    for (int it = i0 + lidx; it < i1; it += gper) {
-        const R4Item w = items[it];
-        const R4Tensor tt = ts[w.tensor];
-        const T *__restrict__ P = (const T *)tt.P + (size_t)w.p * nm * naux;
-        const T *__restrict__ Q = (const T *)tt.Q + (size_t)w.r * nm * naux;
-        T *out = (T *)tt.out;
-        const int qt = min(tt.qt, NW), st = min(tt.st, NW);
-        const int ntile = qt * NW + (NW - qt) * st;
-        const int npass = max(1, max((ntile + 4 * TPW - 1) / (4 * TPW), (tt.nlist + NT * SPT - 1) / (NT * SPT)));
-        for (int pass = 0; pass < npass; ++pass) {
-            int tq[TPW], tsl[TPW], sq[SPT], ss[SPT];
-#pragma unroll
-            for (int j = 0; j < TPW; ++j) {
-                const int id = pass * 4 * TPW + wave + 4 * j;
-                if (id < qt * NW) {
-                    tq[j] = id / NW;
-                    tsl[j] = id % NW;
-                } else if (id < ntile) {
-                    tq[j] = qt + (id - qt * NW) / st;
-                    tsl[j] = (id - qt * NW) % st;
-                } else {
-                    tq[j] = -1;
-                    tsl[j] = 0;
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < SPT; ++j) {
-                const int e = pass * NT * SPT + tid + NT * j;
-                const int v = e < tt.nlist ? tt.list[e] : -1;
-                sq[j] = v < 0 ? -1 : v / nm;
-                ss[j] = v < 0 ? 0 : v % nm;
-            }
-            typename M::acc acc[TPW];
-            T sacc[SPT];
-#pragma unroll
-            for (int j = 0; j < TPW; ++j) acc[j] = typename M::acc{0, 0, 0, 0};
-#pragma unroll
-            for (int j = 0; j < SPT; ++j) sacc[j] = T(0);
-            int ao[TPW], bo[TPW];
-#pragma unroll
-            for (int j = 0; j < TPW; ++j) {
-                ao[j] = (tq[j] < 0 ? 0 : tq[j]) * 16 + lr;
-                bo[j] = tsl[j] * 16 + lr;
-            }
-            for (int k0 = 0; k0 < naux; k0 += KC) {
-                __syncthreads();
-#pragma unroll 4
-                for (int j = 0; j < PER; ++j) {
-                    const int e = tid + j * NT;
-                    const bool second = e >= KC * NP;
-                    const int e2 = second ? e - KC * NP : e;
-                    const int q = e2 / KC, k = e2 % KC;
-                    const T *src = second ? Q : P;
-                    const T v = (q < nm && k0 + k < naux) ? src[(size_t)q * naux + k0 + k] : T(0);
-                    if (second) Qs[q][k] = v;
-                    else Ps[q][k] = v;
-                }
-                __syncthreads();
-#pragma unroll
-                for (int kk = 0; kk < KC / 4; ++kk)
-#pragma unroll
-                    for (int j = 0; j < TPW; ++j)
-                        if (tq[j] >= 0) acc[j] = M::mma(Ps[ao[j]][kk * 4 + lk], Qs[bo[j]][kk * 4 + lk], acc[j]);
-#pragma unroll
-                for (int j = 0; j < SPT; ++j)
-                    if (sq[j] >= 0)
-#pragma unroll 4
-                        for (int k = 0; k < KC; ++k) sacc[j] += Ps[sq[j]][k] * Qs[ss[j]][k];
-            }
-#pragma unroll
-            for (int j = 0; j < TPW; ++j) {
-                if (tq[j] < 0) continue;
-                const int s = tsl[j] * 16 + lr;
-                if (s >= nm) continue;
-#pragma unroll
-                for (int r4 = 0; r4 < 4; ++r4) {
-                    const int q = tq[j] * 16 + M::row(lk, r4);
-                    if (q < nm) out[(((size_t)w.p * nm + q) * nm + w.r) * nm + s] = acc[j][r4];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < SPT; ++j)
-                if (sq[j] >= 0) out[(((size_t)w.p * nm + sq[j]) * nm + w.r) * nm + ss[j]] = sacc[j];
-        }
-    }
-}
 
 // Index pairs (u, v) of a product's rows or columns: element i is
 // (u[i], v[i]) when paired, else (u[i / nv], v[i % nv]) or (u[i / nv],
@@ -380,6 +257,21 @@ struct CholState {
     size_t ci = 0, used = 0;
     DBuf t1, trans, ptrs, items, tensors;
     double ms_blocks = 0.0;
+    // last chol_ensure: device ms of the factor products, rectangle products,
+    // single blocks and exchange products; their algorithmic flops; single blocks
+    double stats[10] = {};
+    std::vector<hipEvent_t> evs;  // stage boundaries of the last build_blocks
+    std::vector<int> ev_stage;
+    hipEvent_t mark(int stage)
+    {
+        if (evs.size() == ev_stage.size()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            evs.push_back(e);
+        }
+        ev_stage.push_back(stage);
+        return evs[ev_stage.size() - 1];
+    }
 
     char *arena(size_t bytes)
     {
@@ -404,6 +296,7 @@ struct CholState {
     }
     ~CholState()
     {
+        for (hipEvent_t e : evs) (void)hipEventDestroy(e);
         if (d_L) (void)hipFree(d_L);
         for (auto &ch : chunks) (void)hipFree(ch.first);
     }
@@ -443,8 +336,8 @@ void chol_forget(nw_ctx *c)
 
 namespace {
 
-// Workgroups of a block launch: up to eight per CU, in multiples of eight
-// (the XCD chunks); each runs its chunk's items in a grid-stride loop.
+// Workgroups of a product launch: up to eight per CU, in multiples of eight
+// (the XCD chunks); each runs its chunk's tiles in a grid-stride loop.
 int r4_grid(int nitems)
 {
     static int ncu = 0;
@@ -455,24 +348,6 @@ int r4_grid(int nitems)
         ncu = std::max(1, ncu);
     }
     return std::max(8, std::min((nitems + 7) / 8 * 8, 8 * ncu));
-}
-
-template <typename T>
-void launch_r4(hipStream_t st, int nw, const R4Tensor *ts, const R4Item *items, int nitems, int nm, int naux)
-{
-    if (nitems <= 0) return;
-#define NW_R4_CASE(W)                                                                                     \
-    case W: {                                                                                             \
-        const int g = r4_grid(nitems);                                                                    \
-        hipLaunchKernelGGL((k_r4_block<T, W>), dim3(g), dim3(256), 0, st, ts, items, nitems, nm, naux);    \
-        break;                                                                                            \
-    }
-    switch (nw) {
-        NW_R4_CASE(1) NW_R4_CASE(2) NW_R4_CASE(3) NW_R4_CASE(4) NW_R4_CASE(5) NW_R4_CASE(6) NW_R4_CASE(7)
-        NW_R4_CASE(8)
-        default: throw NwError(NW_E_SHAPE, "on-demand rank-4 blocks need nmo <= 128");
-    }
-#undef NW_R4_CASE
 }
 
 template <typename T>
@@ -728,8 +603,10 @@ void factor_tensors(nw_ctx *c, const std::vector<FacJob> &jobs)
 // Within a block, the columns of bra holes and the class's ket-particle
 // pairs.  Per rectangle P x R three product families: rows (p, q < oq) x
 // columns (r, every s); rows (p, q >= oq) x columns (r, s < os); and per ket
-// particle q its pairs S(q): rows (p, q) x columns (r, s in S(q)).  Exchange
-// products take every built block as a row and the read columns as columns.
+// particle q its pairs S(q): rows (p, q) x columns (r, s in S(q)).  Single
+// blocks are whole: rows (p, every q) x columns (r in R(p), every s).
+// Exchange products take every built block as a row and the read columns as
+// columns.
 void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
 {
     CholState *s = c->chol;
@@ -790,7 +667,14 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
         need(t.wi, t.E);
         need(t.wi, t.F);
     }
-    if (!fj.empty()) factor_tensors(c, fj);
+    HIPCHK(hipEventRecord(s->mark(1), c->st));
+    if (!fj.empty()) {
+        factor_tensors(c, fj);
+        std::vector<const double *> ys;
+        for (const FacJob &j : fj)
+            if (std::find(ys.begin(), ys.end(), j.Y) == ys.end()) ys.push_back(j.Y);
+        s->stats[8] += 2.0 * nm * (double)naux * c->n * c->n * (double)(ys.size() + fj.size());
+    }
     auto f32of = [&](int i, int key) -> const void * {
         CholPair &cp = s->pairs[work[i].key];
         auto it = cp.fac32.find(key);
@@ -813,8 +697,8 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
         std::vector<std::pair<size_t, size_t>> rect;  // offsets of P and R lists
         std::vector<std::pair<int, size_t>> kq;       // (q, offset of S(q)) with |S(q)| in kqn
         std::vector<int> kqn;
-        size_t single_list = 0;
-        int nsingle_list = 0;
+        std::vector<std::pair<size_t, size_t>> sgl;   // single blocks grouped by p: offsets of {p} and R(p)
+        std::vector<int> sgln;
         size_t xp = 0, xr = 0, xq = 0, xs = 0;        // exchange rows (p, r) and columns (q, s)
         int nxrow = 0, nxcol = 0;
     };
@@ -835,13 +719,14 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
                     L.kqn.push_back((int)S.size());
                 }
             }
-            const int qe = 16 * ((oq + 15) / 16), se = 16 * ((os + 15) / 16);
-            std::vector<int> dl;
-            for (int q = qe; q < nm; ++q)
-                for (int t = se; t < nm; ++t)
-                    if (cw.kk[(size_t)q * nm + t]) dl.push_back(q * nm + t);
-            L.single_list = put(dl);
-            L.nsingle_list = (int)dl.size();
+            for (size_t k = 0; k < cw.single.size();) {  // cw.single is ordered by p
+                const int p0 = cw.single[k] / nm;
+                std::vector<int> R;
+                for (; k < cw.single.size() && cw.single[k] / nm == p0; ++k) R.push_back(cw.single[k] % nm);
+                const size_t po = put(std::vector<int>{p0});
+                L.sgl.push_back({po, put(R)});
+                L.sgln.push_back((int)R.size());
+            }
             if (c->pinfo[work[i].key].ma + c->pinfo[work[i].key].mb > 0 && cl < 2) {
                 std::vector<int> rp, rr, cq, cs;
                 for (int k : cw.blocks) {
@@ -867,8 +752,9 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
     auto at = [](size_t off) { return (const int *)(uintptr_t)(off + 1); };
     std::vector<R4Gemm> gd[2], gx[2];
     std::vector<R4GTile> td[2], tx[2];
-    std::vector<R4Tensor> ts[2];
-    std::vector<R4Item> its[2];
+    std::vector<R4Gemm> gs1[2];  // single blocks, whole
+    std::vector<R4GTile> ts1[2];
+    double nsingle[2] = {0.0, 0.0};
     for (int f32 = 0; f32 < 2; ++f32) {
         auto tiles_of = [&](std::vector<R4GTile> &tl, int g, int M, int N) {
             for (int m0 = 0; m0 < M; m0 += 64)
@@ -907,10 +793,12 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
                     gd[f32].push_back(g);
                 }
             }
-            if (!cw.single.empty()) {
-                const int ti = (int)ts[f32].size();
-                ts[f32].push_back({PA, QB, out, at(L.single_list), L.nsingle_list, (oq + 15) / 16, (os + 15) / 16, 0});
-                for (int k : cw.single) its[f32].push_back({ti, k / nm, k % nm, 0});
+            for (size_t k = 0; k < L.sgl.size(); ++k) {  // whole blocks: rows (p, every q) x columns (r, every s)
+                R4Gemm g{PA, QB, out, {at(L.sgl[k].first), nullptr, 1, nm, 0, 0},
+                         {at(L.sgl[k].second), nullptr, L.sgln[k], nm, 0, 0}, nm, L.sgln[k] * nm, 0, 0};
+                tiles_of(ts1[f32], (int)gs1[f32].size(), g.M, g.N);
+                gs1[f32].push_back(g);
+                nsingle[f32] += L.sgln[k];
             }
             if (t.E >= 0 && L.nxrow > 0 && L.nxcol > 0) {
                 R4Gemm g{ptr(t.E), ptr(t.F), out, {at(L.xp), at(L.xr), L.nxrow, 1, 0, 1},
@@ -923,36 +811,43 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
     // one upload of the lists and every descriptor, then the launches
     size_t bytes = ints.size() * sizeof(int) + 8192;
     for (int f = 0; f < 2; ++f)
-        bytes += (gd[f].size() + gx[f].size()) * sizeof(R4Gemm) + (td[f].size() + tx[f].size()) * sizeof(R4GTile) +
-                 ts[f].size() * sizeof(R4Tensor) + its[f].size() * sizeof(R4Item) + 2048;
+        bytes += (gd[f].size() + gx[f].size() + gs1[f].size()) * sizeof(R4Gemm) +
+                 (td[f].size() + tx[f].size() + ts1[f].size()) * sizeof(R4GTile) + 3072;
     Uploader up(c, bytes);
     dints = up.put(ints);
     auto fix = [&](const int *p) { return p ? dints + ((uintptr_t)p - 1) : nullptr; };
     for (int f = 0; f < 2; ++f) {
-        for (auto *v : {&gd[f], &gx[f]})
+        for (auto *v : {&gd[f], &gx[f], &gs1[f]})
             for (R4Gemm &g : *v) {
                 g.row.u = fix(g.row.u);
                 g.row.v = fix(g.row.v);
                 g.col.u = fix(g.col.u);
                 g.col.v = fix(g.col.v);
             }
-        for (R4Tensor &t : ts[f]) t.list = fix(t.list);
+    }
+    for (int f = 0; f < 2; ++f) {
+        for (const R4Gemm &g : gd[f]) s->stats[5] += 2.0 * g.M * (double)g.N * naux;
+        for (const R4Gemm &g : gx[f]) s->stats[7] += 2.0 * g.M * (double)g.N * naux;
+        for (const R4Gemm &g : gs1[f]) s->stats[6] += 2.0 * g.M * (double)g.N * naux;
+        s->stats[9] += nsingle[f];
     }
     for (int f32 = 0; f32 < 2; ++f32) {
         if (!td[f32].empty()) {
+            HIPCHK(hipEventRecord(s->mark(2), c->st));
             const R4Gemm *dg = up.put(gd[f32]);
             const R4GTile *dt = up.put(td[f32]);
             if (f32) launch_r4g<float>(c->st, dg, dt, (int)td[f32].size(), nm, naux);
             else launch_r4g<double>(c->st, dg, dt, (int)td[f32].size(), nm, naux);
         }
-        if (!its[f32].empty()) {
-            const R4Tensor *dts = up.put(ts[f32]);
-            const R4Item *dit = up.put(its[f32]);
-            const int nw = (nm + 15) / 16;
-            if (f32) launch_r4<float>(c->st, nw, dts, dit, (int)its[f32].size(), nm, naux);
-            else launch_r4<double>(c->st, nw, dts, dit, (int)its[f32].size(), nm, naux);
+        if (!ts1[f32].empty()) {
+            HIPCHK(hipEventRecord(s->mark(3), c->st));
+            const R4Gemm *dg = up.put(gs1[f32]);
+            const R4GTile *dt = up.put(ts1[f32]);
+            if (f32) launch_r4g<float>(c->st, dg, dt, (int)ts1[f32].size(), nm, naux);
+            else launch_r4g<double>(c->st, dg, dt, (int)ts1[f32].size(), nm, naux);
         }
         if (!tx[f32].empty()) {  // after the direct products of the same blocks (stream order)
+            HIPCHK(hipEventRecord(s->mark(4), c->st));
             const R4Gemm *dg = up.put(gx[f32]);
             const R4GTile *dt = up.put(tx[f32]);
             if (f32) launch_r4g<float>(c->st, dg, dt, (int)tx[f32].size(), nm, naux);
@@ -960,6 +855,7 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
         }
         HIPCHK(hipGetLastError());
     }
+    HIPCHK(hipEventRecord(s->mark(0), c->st));
     for (PairWork &pw : work) {
         CholPair &cp = s->pairs[pw.key];
         for (int cl = 0; cl < 3; ++cl) {
@@ -977,16 +873,18 @@ void chol_ensure(nw_ctx *c, const std::vector<PackedDet> &h)
     CholState *s = c->chol;
     if (!s) return;
     s->ms_blocks = 0.0;
+    for (double &v : s->stats) v = 0.0;
+    s->ev_stage.clear();
     std::vector<PairWork> work;
     if (!plan_missing(c, h, work)) return;
-    HIPCHK(hipEventRecord(c->ev[3], c->st));
+    HIPCHK(hipEventRecord(s->mark(0), c->st));
     build_blocks(c, work);
-    hipEvent_t e1;
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e1, c->st));
-    HIPCHK(hipEventSynchronize(e1));
-    s->ms_blocks = elapsed(c->ev[3], e1);
-    (void)hipEventDestroy(e1);
+    HIPCHK(hipEventSynchronize(s->evs[s->ev_stage.size() - 1]));
+    // stage k runs from its mark to the next mark
+    for (size_t k = 0; k + 1 < s->ev_stage.size(); ++k)
+        s->stats[s->ev_stage[k]] += elapsed(s->evs[k], s->evs[k + 1]);
+    s->ms_blocks = elapsed(s->evs[0], s->evs[s->ev_stage.size() - 1]);
+    s->stats[0] = s->ms_blocks;
 }
 
 bool chol_missing(nw_ctx *c, const std::vector<PackedDet> &h)
@@ -1095,6 +993,15 @@ int nw_set_eri_factors(nw_ctx *c, const double *L, int naux, double check_tol, d
         }
         s->d_L = dL;
         s->naux = naux;
+        return NW_OK;
+    });
+}
+
+int nw_rank4_stats(nw_ctx *c, double *out10)
+{
+    return guarded(c, [&]() -> int {
+        if (!out10) return fail(c, NW_E_ARG, "null output");
+        for (int k = 0; k < 10; ++k) out10[k] = c->chol ? c->chol->stats[k] : 0.0;
         return NW_OK;
     });
 }
