@@ -838,7 +838,10 @@ int eval_matrix(nw_ctx *c, int kind, const nw_det *dets, uint64_t ndet, double *
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     PackedDet *dp = nullptr;
     std::vector<PackedDet> h = pack(c, dets, ndet, dp);
-    if (kind == EVAL_HS) chol_ensure(c, h);
+    if (kind == EVAL_HS) {
+        const std::vector<uint8_t> reads = triangle_pairs(c, h);
+        chol_ensure(c, h, false, &reads);
+    }
     auto bins = triangle_blocks(c, h, kind == EVAL_HS);
     const bool adev = is_device_ptr(A), bdev = is_device_ptr(B);
     double *dA = adev ? A : c->outH.get<double>(ndet * ndet);
@@ -1023,13 +1026,14 @@ int nw_hs_matrix_streamed(nw_ctx *c, double tol, const nw_det *dets, uint64_t nd
         }
         double build_ms = 0.0, eval_ms = 0.0;
         std::vector<uint8_t> mask((size_t)nref * nref);
-        for (const auto &g : groups) {
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            const auto &g = groups[gi];
             std::fill(mask.begin(), mask.end(), 0);
             for (int p : g) mask[p] = 1;
             rc = build_impl(c, tol, mask.data(), meta_out, nullptr);
             if (rc != NW_OK) return rc;
             build_ms += c->t_ms[0];
-            chol_ensure(c, h);  // on-demand rank-4 blocks of the group (integral factors)
+            chol_ensure(c, h, gi > 0);  // on-demand rank-4 blocks of the group (integral factors)
             build_ms += chol_last_ms(c);
             // every element of the group's pairs; each pair is in exactly one group
             auto bins = triangle_blocks(c, h, true, true);

@@ -84,7 +84,10 @@ __device__ inline void axis_at(const R4Axis &a, int i, int &u, int &v)
 // tensors [nm][nm][naux]); mode 0 stores C at out[((u nm + v) nm + u') nm + v']
 // (direct products: u = p, v = q, u' = r, v' = s), mode 1 subtracts it at
 // out[((u nm + u') nm + v) nm + v'] (exchange products: rows (p, r) of E,
-// columns (q, s) of F).
+// columns (q, s) of F), mode 2 stores it at out[((u' nm + v') nm + u) nm + v]
+// (direct products with rows (r, s) and columns (p, q)), mode 3 as mode 0
+// and also at out[((u' nm + v') nm + u) nm + v] (blocks (p, r) and (r, p) of
+// a product whose two factor tensors are one: J[r,s,p,q] = J[p,q,r,s]).
 struct R4Gemm {
     const void *A, *B;
     void *out;
@@ -95,22 +98,43 @@ struct R4GTile {
     int g, m0, n0, pad;
 };
 
-// Row-gathered A B^T products in 64 x 64 tiles: four waves of 16 rows x 4
-// column tiles each, K staged through LDS 32 at a time ([row][k], rows
-// gathered from the factor tensors, 256 contiguous bytes per row and chunk
-// in FP64), the next chunk prefetched into registers during the products.
-// Tiles are dealt to the eight XCDs in contiguous chunks, so the tiles of
-// one product, which share rows, meet in one L2.
+// Row-gathered A B^T products in BM x BN tiles: four waves in a 2 x 2
+// layout, each (BM/2) x (BN/2) of 16 x 16 MFMA tiles; K staged through LDS
+// KC at a time ([row][k], rows gathered from the factor tensors, 16-byte
+// loads), the next chunk held in registers during the products.  MFMA step
+// kk of lane group lk takes k = lk * (KC/4) + kk (the dot product does not
+// depend on the order of k), so each lane's fragments of a chunk are
+// contiguous in LDS and read 16 bytes at a time.  Factor-tensor rows are
+// naux values with naux a multiple of 4 (the factors are zero-padded), so
+// every row starts 16-byte aligned.  Tiles are dealt to the eight XCDs in
+// contiguous chunks, so the tiles of one product, which share rows, meet in
+// one L2.
 template <typename T>
-__global__ __launch_bounds__(256) void k_r4_gemm(const R4Gemm *__restrict__ gs, const R4GTile *__restrict__ tiles,
-                                                 int ntiles, int nm, int naux)
+struct R4Vec;
+template <>
+struct R4Vec<float> {
+    typedef float4 t;
+};
+template <>
+struct R4Vec<double> {
+    typedef double2 t;
+};
+
+template <typename T, int BM, int BN, int KC, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_r4_gemm(const R4Gemm *__restrict__ gs, const R4GTile *__restrict__ tiles,
+                                                      int ntiles, int nm, int naux)
 {
-    constexpr int BM = 64, BN = 64, KC = 32, LDK = KC + 2, NT = 256, PER = (BM + BN) * KC / NT;
-    __shared__ T As[BM][LDK];
-    __shared__ T Bs[BN][LDK];
-    __shared__ long long ra[BM], rb[BN], ro[BM], co[BN];
+    typedef typename R4Vec<T>::t VT;
+    constexpr int VE = 16 / sizeof(T), KV = KC / VE, LDK = KC + VE, NT = 256;
+    constexpr int PER = (BM + BN) * KV / NT;  // 16-byte vectors per thread and chunk
+    static_assert((BM + BN) * KV % NT == 0 && KC % (4 * VE) == 0, "tile shape");
+    constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16, KQ = KC / 4;
+    __shared__ __attribute__((aligned(16))) T As[BM][LDK];
+    __shared__ __attribute__((aligned(16))) T Bs[BN][LDK];
+    __shared__ long long ra[BM], rb[BN], ro[BM], co[BN], mro[BM], mco[BN];
     using M = R4Mfma<T>;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+    const int wm = wave >> 1, wn = wave & 1;
     const int gper = gridDim.x / 8, chunk = blockIdx.x % 8, lidx = blockIdx.x / 8;
     const int per = (ntiles + 7) / 8;
     const int t0 = chunk * per, t1 = min(ntiles, t0 + per);
@@ -120,83 +144,214 @@ __global__ __launch_bounds__(256) void k_r4_gemm(const R4Gemm *__restrict__ gs, 
         const R4Gemm g = gs[tl.g];
         const int mloc = min(BM, g.M - tl.m0), nloc = min(BN, g.N - tl.n0);
         __syncthreads();  // the previous tile's indices are no longer read
-        if (tid < BM) {
+        for (int t = tid; t < BM + BN; t += NT) {
             int u = 0, v = 0;
-            if (tid < mloc) axis_at(g.row, tl.m0 + tid, u, v);
-            ra[tid] = ((long long)u * nm + v) * naux;
-            ro[tid] = g.mode == 0 ? ((long long)u * nm + v) * nm2 : (long long)u * nm * nm2 + (long long)v * nm;
-        } else if (tid < BM + BN) {
-            const int j = tid - BM;
-            int u = 0, v = 0;
-            if (j < nloc) axis_at(g.col, tl.n0 + j, u, v);
-            rb[j] = ((long long)u * nm + v) * naux;
-            co[j] = g.mode == 0 ? (long long)u * nm + v : (long long)u * nm2 + v;
+            if (t < BM) {
+                const bool ok = t < mloc;
+                if (ok) axis_at(g.row, tl.m0 + t, u, v);
+                ra[t] = ok ? ((long long)u * nm + v) * naux : -1;
+                ro[t] = g.mode == 0 || g.mode == 3 ? ((long long)u * nm + v) * nm2
+                        : g.mode == 1              ? (long long)u * nm * nm2 + (long long)v * nm
+                                                   : (long long)u * nm + v;
+                mro[t] = (long long)u * nm + v;
+            } else {
+                const int j = t - BM;
+                const bool ok = j < nloc;
+                if (ok) axis_at(g.col, tl.n0 + j, u, v);
+                rb[j] = ok ? ((long long)u * nm + v) * naux : -1;
+                co[j] = g.mode == 0 || g.mode == 3 ? (long long)u * nm + v
+                        : g.mode == 1              ? (long long)u * nm2 + v
+                                                   : ((long long)u * nm + v) * nm2;
+                mco[j] = ((long long)u * nm + v) * nm2;
+            }
         }
         __syncthreads();
         const T *__restrict__ A = (const T *)g.A;
         const T *__restrict__ B = (const T *)g.B;
-        T buf[PER];
+        // each thread loads the same rows and k offsets in every chunk
+        const T *src[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = tid + j * NT, r = e / KV, kv = e % KV;
+            const long long b = r < BM ? ra[r] : rb[r - BM];
+            src[j] = b < 0 ? nullptr : (r < BM ? A : B) + b + kv * VE;
+        }
+        VT buf[PER];
         auto gload = [&](int k0) {
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
-                const int e = tid + j * NT;
-                const bool second = e >= BM * KC;
-                const int e2 = second ? e - BM * KC : e;
-                const int r = e2 / KC, k = e2 % KC;
-                const bool ok = (second ? r < nloc : r < mloc) && k0 + k < naux;
-                buf[j] = ok ? (second ? B[rb[r] + k0 + k] : A[ra[r] + k0 + k]) : T(0);
+                const int kv = (tid + j * NT) % KV;
+                buf[j] = src[j] && k0 + kv * VE < naux ? *reinterpret_cast<const VT *>(src[j] + k0) : VT{};
             }
         };
-        typename M::acc acc[4];
+        typename M::acc acc[TM][TN];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] = typename M::acc{0, 0, 0, 0};
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b) acc[a][b] = typename M::acc{0, 0, 0, 0};
         gload(0);
         for (int k0 = 0; k0 < naux; k0 += KC) {
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
-                const int e = tid + j * NT;
-                const bool second = e >= BM * KC;
-                const int e2 = second ? e - BM * KC : e;
-                const int r = e2 / KC, k = e2 % KC;
-                if (second) Bs[r][k] = buf[j];
-                else As[r][k] = buf[j];
+                const int e = tid + j * NT, r = e / KV, kv = e % KV;
+                T *dst = r < BM ? &As[r][kv * VE] : &Bs[r - BM][kv * VE];
+                *reinterpret_cast<VT *>(dst) = buf[j];
             }
             __syncthreads();
-            if (k0 + KC < naux) gload(k0 + KC);
+            if (k0 + KC < naux) gload(k0 + KC);  // next chunk in flight during the products
 #pragma unroll
-            for (int kk = 0; kk < KC / 4; ++kk) {
-                const T a = As[wave * 16 + lr][kk * 4 + lk];
+            for (int q0 = 0; q0 < KQ; q0 += VE) {
+                VT af[TM], bf[TN];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) acc[t] = M::mma(a, Bs[t * 16 + lr][kk * 4 + lk], acc[t]);
+                for (int a = 0; a < TM; ++a)
+                    af[a] = *reinterpret_cast<const VT *>(&As[wm * WM + a * 16 + lr][lk * KQ + q0]);
+#pragma unroll
+                for (int b = 0; b < TN; ++b)
+                    bf[b] = *reinterpret_cast<const VT *>(&Bs[wn * WN + b * 16 + lr][lk * KQ + q0]);
+#pragma unroll
+                for (int e = 0; e < VE; ++e)
+#pragma unroll
+                    for (int a = 0; a < TM; ++a)
+#pragma unroll
+                        for (int b = 0; b < TN; ++b)
+                            acc[a][b] = M::mma(reinterpret_cast<const T *>(&af[a])[e],
+                                               reinterpret_cast<const T *>(&bf[b])[e], acc[a][b]);
             }
         }
         T *out = (T *)g.out;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int j = t * 16 + lr;
+        for (int b = 0; b < TN; ++b) {
+            const int j = wn * WN + b * 16 + lr;
             if (j >= nloc) continue;
 #pragma unroll
-            for (int r4 = 0; r4 < 4; ++r4) {
-                const int i = wave * 16 + M::row(lk, r4);
-                if (i >= mloc) continue;
-                T *o = out + ro[i] + co[j];
-                if (g.mode == 0) *o = acc[t][r4];
-                else *o -= acc[t][r4];
-            }
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int r4 = 0; r4 < 4; ++r4) {
+                    const int i = wm * WM + a * 16 + M::row(lk, r4);
+                    if (i >= mloc) continue;
+                    T *o = out + ro[i] + co[j];
+                    if (g.mode == 1) {
+                        *o -= acc[a][b][r4];
+                    } else {
+                        *o = acc[a][b][r4];
+                        if (g.mode == 3) out[mco[j] + mro[i]] = acc[a][b][r4];
+                    }
+                }
         }
     }
 }
 
-// out[b][j][i] = in[b][i][j] for rows x cols matrices.
-__global__ void k_transpose_batched(const double *const *__restrict__ in, double *const *__restrict__ out, int rows,
+// out[b][j][i] = in[b][i][j] for rows x cols matrices (rounded to To).
+template <typename To>
+__global__ void k_transpose_batched(const double *const *__restrict__ in, To *const *__restrict__ out, int rows,
                                     int cols)
 {
     const double *a = in[blockIdx.y];
-    double *o = out[blockIdx.y];
+    To *o = out[blockIdx.y];
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < rows * cols; e += gridDim.x * blockDim.x) {
         const int i = e / cols, j = e % cols;
-        o[(size_t)j * rows + i] = a[e];
+        o[(size_t)j * rows + i] = (To)a[e];
+    }
+}
+
+// One product of the factor transforms: C[i][j] = sum_k A[i][k] B[j][k].
+struct FacGemm {
+    const void *A, *B;
+    void *C;
+};
+
+// Factor-transform products: few rows (M = nmo, one tile of 16 TM rows per
+// row block, blockIdx.z), few k (K = n_bas) and many columns (naux n_bas or
+// naux nmo), batched over blockIdx.y.  64 TN columns per workgroup (wave w:
+// 16 TN of them), K through LDS 16 at a time with 16-byte loads (lda and ldb
+// multiples of 4, 16-byte aligned rows), the next chunk held in registers;
+// lane group lk's MFMA steps take k = lk * 4 + kk, so its fragments are
+// contiguous in LDS.  Every B row is read once; the A tile is shared by the
+// workgroups of a product through L2.
+template <typename T, int TM, int TN>
+__global__ __launch_bounds__(256) void k_fac_gemm(const FacGemm *__restrict__ descs, int M, int N, int K, int lda,
+                                                  int ldb, int ldc)
+{
+    typedef typename R4Vec<T>::t VT;
+    constexpr int KC = 16, VE = 16 / sizeof(T), KV = KC / VE, LDK = KC + VE, BM = 16 * TM, BN = 64 * TN, KQ = KC / 4;
+    constexpr int NV = (BM + BN) * KV, PER = (NV + 255) / 256;
+    static_assert(KQ % VE == 0, "chunk");
+    __shared__ __attribute__((aligned(16))) T As[BM][LDK];
+    __shared__ __attribute__((aligned(16))) T Bs[BN][LDK];
+    using Mf = R4Mfma<T>;
+    const FacGemm g = descs[blockIdx.y];
+    const T *__restrict__ A = (const T *)g.A;
+    const T *__restrict__ B = (const T *)g.B;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+    const int n0 = blockIdx.x * BN, m0 = blockIdx.z * BM, mloc = min(BM, M - m0);
+    const T *src[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int e = tid + j * 256, r = e / KV, kv = e % KV;
+        const T *q = nullptr;
+        if (e < NV) {
+            if (r < BM) {
+                if (r < mloc) q = A + (size_t)(m0 + r) * lda + kv * VE;
+            } else if (n0 + r - BM < N) {
+                q = B + (size_t)(n0 + r - BM) * ldb + kv * VE;
+            }
+        }
+        src[j] = q;
+    }
+    VT buf[PER];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int kv = (tid + j * 256) % KV;
+            buf[j] = src[j] && k0 + kv * VE < K ? *reinterpret_cast<const VT *>(src[j] + k0) : VT{};
+        }
+    };
+    typename Mf::acc acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = typename Mf::acc{0, 0, 0, 0};
+    gload(0);
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = tid + j * 256, r = e / KV, kv = e % KV;
+            if (e < NV) *reinterpret_cast<VT *>(r < BM ? &As[r][kv * VE] : &Bs[r - BM][kv * VE]) = buf[j];
+        }
+        __syncthreads();
+        if (k0 + KC < K) gload(k0 + KC);  // next chunk in flight during the products
+#pragma unroll
+        for (int q0 = 0; q0 < KQ; q0 += VE) {
+            VT bf[TN];
+#pragma unroll
+            for (int b = 0; b < TN; ++b)
+                bf[b] = *reinterpret_cast<const VT *>(&Bs[wave * 16 * TN + b * 16 + lr][lk * KQ + q0]);
+#pragma unroll
+            for (int a = 0; a < TM; ++a) {
+                const VT af = *reinterpret_cast<const VT *>(&As[a * 16 + lr][lk * KQ + q0]);
+#pragma unroll
+                for (int e = 0; e < VE; ++e)
+#pragma unroll
+                    for (int b = 0; b < TN; ++b)
+                        acc[a][b] = Mf::mma(reinterpret_cast<const T *>(&af)[e], reinterpret_cast<const T *>(&bf[b])[e],
+                                            acc[a][b]);
+            }
+        }
+    }
+    T *C = (T *)g.C;
+#pragma unroll
+    for (int b = 0; b < TN; ++b) {
+        const int col = n0 + wave * 16 * TN + b * 16 + lr;
+        if (col >= N) continue;
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = a * 16 + Mf::row(lk, r);
+                if (row < mloc) C[(size_t)(m0 + row) * ldc + col] = acc[a][b][r];
+            }
     }
 }
 
@@ -251,17 +406,20 @@ struct CholPair {
 
 struct CholState {
     double *d_L = nullptr;  // [naux][n][n]
-    int naux = 0;
+    int naux = 0;           // factors held: the given ones, zero-padded to a multiple of 4
+    int naux_in = 0;        // factors given
     std::vector<CholPair> pairs;  // x * nref + w, for the context's current build
     std::vector<std::pair<char *, size_t>> chunks;  // factor-tensor arena, reset by every build
     size_t ci = 0, used = 0;
-    DBuf t1, trans, ptrs, items, tensors;
+    DBuf t1, trans, ptrs, items, tensors, t1f, trans32;
+    float *d_L32 = nullptr;  // FP32 copy of d_L, made on first FP32 use
     double ms_blocks = 0.0;
     // last chol_ensure: device ms of the factor products, rectangle products,
     // single blocks and exchange products; their algorithmic flops; single blocks
     double stats[10] = {};
     std::vector<hipEvent_t> evs;  // stage boundaries of the last build_blocks
     std::vector<int> ev_stage;
+    bool pending = false;  // stage times not yet read from the events
     hipEvent_t mark(int stage)
     {
         if (evs.size() == ev_stage.size()) {
@@ -298,6 +456,7 @@ struct CholState {
     {
         for (hipEvent_t e : evs) (void)hipEventDestroy(e);
         if (d_L) (void)hipFree(d_L);
+        if (d_L32) (void)hipFree(d_L32);
         for (auto &ch : chunks) (void)hipFree(ch.first);
     }
 };
@@ -336,9 +495,7 @@ void chol_forget(nw_ctx *c)
 
 namespace {
 
-// Workgroups of a product launch: up to eight per CU, in multiples of eight
-// (the XCD chunks); each runs its chunk's tiles in a grid-stride loop.
-int r4_grid(int nitems)
+int r4_ncu()
 {
     static int ncu = 0;
     if (!ncu) {
@@ -347,14 +504,60 @@ int r4_grid(int nitems)
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         ncu = std::max(1, ncu);
     }
-    return std::max(8, std::min((nitems + 7) / 8 * 8, 8 * ncu));
+    return ncu;
 }
 
-template <typename T>
-void launch_r4g(hipStream_t st, const R4Gemm *gs, const R4GTile *tiles, int ntiles, int nm, int naux)
+// Workgroups of a product launch: up to eight per CU, in multiples of eight
+// (the XCD chunks); each runs its chunk's tiles in a grid-stride loop.
+int r4_grid(int nitems) { return std::max(8, std::min((nitems + 7) / 8 * 8, 8 * r4_ncu())); }
+
+size_t r4_tiles(const std::vector<R4Gemm> &gs, int bm, int bn)
 {
-    if (ntiles <= 0) return;
-    hipLaunchKernelGGL(k_r4_gemm<T>, dim3(r4_grid(ntiles)), dim3(256), 0, st, gs, tiles, ntiles, nm, naux);
+    size_t n = 0;
+    for (const R4Gemm &g : gs) n += (size_t)((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn);
+    return n;
+}
+
+template <typename T, int BM, int BN, int KC, int OCC>
+void launch_r4_shape(nw_ctx *c, Uploader &up, const R4Gemm *dg, const std::vector<R4Gemm> &gs, int naux)
+{
+    std::vector<R4GTile> tl;
+    tl.reserve(r4_tiles(gs, BM, BN));
+    for (size_t i = 0; i < gs.size(); ++i)
+        for (int m0 = 0; m0 < gs[i].M; m0 += BM)
+            for (int n0 = 0; n0 < gs[i].N; n0 += BN) tl.push_back({(int)i, m0, n0, 0});
+    if (tl.empty()) return;
+    const R4GTile *dt = up.put(tl);
+    hipLaunchKernelGGL((k_r4_gemm<T, BM, BN, KC, OCC>), dim3(r4_grid((int)tl.size())), dim3(256), 0, c->st, dg, dt,
+                       (int)tl.size(), c->nmo, naux);
+}
+
+// One launch over a list of products: 128 x 128 output tiles in FP32 and
+// 128 x 64 in FP64 (measured 1.3-1.6x the rate of 64 x 64 tiles on the
+// n_bas = 100 products) when there are enough of them to fill the chip
+// (four per CU), else 64 x 64 tiles.
+template <typename T>
+void launch_r4g(nw_ctx *c, Uploader &up, const std::vector<R4Gemm> &gs, int naux)
+{
+    if (gs.empty()) return;
+    const R4Gemm *dg = up.put(gs);
+    constexpr bool f32 = sizeof(T) == 4;
+    if (r4_tiles(gs, 128, f32 ? 128 : 64) >= (size_t)4 * r4_ncu()) {
+        if constexpr (f32) launch_r4_shape<float, 128, 128, 32, 1>(c, up, dg, gs, naux);
+        else launch_r4_shape<double, 128, 64, 16, 2>(c, up, dg, gs, naux);
+    } else {
+        launch_r4_shape<T, 64, 64, 32, 2>(c, up, dg, gs, naux);
+    }
+}
+
+// Products with a narrow side (at most a few dozen columns: ket-particle
+// pairs) in 128 x 32 tiles.
+void launch_r4g_narrow(nw_ctx *c, Uploader &up, const std::vector<R4Gemm> &gs, int naux, bool f32)
+{
+    if (gs.empty()) return;
+    const R4Gemm *dg = up.put(gs);
+    if (f32) launch_r4_shape<float, 128, 32, 32, 2>(c, up, dg, gs, naux);
+    else launch_r4_shape<double, 128, 32, 16, 2>(c, up, dg, gs, naux);
 }
 
 // Row index sets of one parent's determinants per spin, as bra (rows
@@ -452,7 +655,8 @@ struct PairWork {
 };
 
 // Lazy pairs touched by h and their work.
-bool plan_missing(nw_ctx *c, const std::vector<PackedDet> &h, std::vector<PairWork> &work)
+bool plan_missing(nw_ctx *c, const std::vector<PackedDet> &h, std::vector<PairWork> &work,
+                  const std::vector<uint8_t> *reads)
 {
     CholState *s = c->chol;
     if (!s || s->pairs.empty()) return false;
@@ -469,7 +673,7 @@ bool plan_missing(nw_ctx *c, const std::vector<PackedDet> &h, std::vector<PairWo
             const int key = x * nref + w;
             const CholPair &cp = s->pairs[key];
             const PairInfo &pi = c->pinfo[key];
-            if (!cp.lazy || !pi.built) continue;
+            if (!cp.lazy || !pi.built || (reads && !(*reads)[key])) continue;
             const bool both = pi.ma + pi.mb > 0;
             PairWork pw;
             pw.key = key;
@@ -533,63 +737,125 @@ bool plan_missing(nw_ctx *c, const std::vector<PackedDet> &h, std::vector<PairWo
     return !work.empty();
 }
 
+template <typename T, int TN>
+void launch_fac_gemm(hipStream_t st, int M, int N, int K, int lda, int ldb, int ldc, const FacGemm *d, int nb)
+{
+    const int tm = std::min(8, (M + 15) / 16);
+    const unsigned mblk = (unsigned)((M + 16 * tm - 1) / (16 * tm));
+    for (int z0 = 0; z0 < nb; z0 += 65535) {
+        const dim3 grid((unsigned)((N + 64 * TN - 1) / (64 * TN)), (unsigned)std::min(65535, nb - z0), mblk);
+        const FacGemm *dz = d + z0;
+        switch (tm) {
+            case 1: hipLaunchKernelGGL((k_fac_gemm<T, 1, TN>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 2: hipLaunchKernelGGL((k_fac_gemm<T, 2, TN>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 3: hipLaunchKernelGGL((k_fac_gemm<T, 3, TN>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 4: hipLaunchKernelGGL((k_fac_gemm<T, 4, TN>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 5: hipLaunchKernelGGL((k_fac_gemm<T, 5, TN>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 6: hipLaunchKernelGGL((k_fac_gemm<T, 6, TN>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 7: hipLaunchKernelGGL((k_fac_gemm<T, 7, TN>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            default: hipLaunchKernelGGL((k_fac_gemm<T, 8, TN>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+        }
+    }
+}
+
 // A_K = X^T L_K Y for every job (X, Y: n x nm row-major), as [p][q][K]:
 // T1[q][K][a] = sum_b Y[b][q] L[K][a][b] (one product over the stacked
-// factors per distinct Y), then out[p][(q, K)] = sum_a X[a][p] T1[(q, K)][a];
-// both as A B^T products on FP64 MFMA.
+// factors per distinct Y), then out[p][(q, K)] = sum_a X[a][p] T1[(q, K)][a].
+// Jobs of FP32 pairs run in FP32 (FP32-rounded L, X and Y; FP32 MFMA),
+// the others in FP64.  n_bas a multiple of 4: k_fac_gemm (64 and 128
+// columns per workgroup in FP64 and FP32, measured best at n_bas = 100);
+// otherwise the generic FP64 product kernel, rounded to FP32 afterwards for
+// FP32 jobs.
 struct FacJob {
     const double *X, *Y;
-    double *out;
+    void *out;
+    bool f32;
 };
 
-void factor_tensors(nw_ctx *c, const std::vector<FacJob> &jobs)
+template <typename T>
+void factor_run(nw_ctx *c, const std::vector<FacJob> &jobs, Uploader &up)
 {
     CholState *s = c->chol;
     const int n = c->n, nm = c->nmo, naux = s->naux;
     const size_t nnm = (size_t)n * nm, t1sz = (size_t)nm * naux * n;
+    constexpr bool f32 = sizeof(T) == 4;
+    const bool fast = n % 4 == 0;
     // transposes of every distinct X and Y
     std::vector<const double *> mats;
+    std::vector<int> xi(jobs.size()), yi(jobs.size());
     auto idx_of = [&](const double *m) {
         for (size_t i = 0; i < mats.size(); ++i)
             if (mats[i] == m) return (int)i;
         mats.push_back(m);
         return (int)mats.size() - 1;
     };
-    std::vector<int> xi(jobs.size()), yi(jobs.size());
     for (size_t j = 0; j < jobs.size(); ++j) {
         xi[j] = idx_of(jobs[j].X);
         yi[j] = idx_of(jobs[j].Y);
     }
-    double *tr = s->trans.get<double>(mats.size() * nnm);
-    std::vector<double *> trp(mats.size());
+    T *tr = (f32 ? s->trans32 : s->trans).get<T>(mats.size() * nnm);
+    std::vector<T *> trp(mats.size());
     for (size_t i = 0; i < mats.size(); ++i) trp[i] = tr + i * nnm;
-    Uploader up(c, 16u << 20);
-    {
-        const double *const *din = up.put(mats);
-        double *const *dout = up.put(trp);
-        hipLaunchKernelGGL(k_transpose_batched, dim3((unsigned)std::min<size_t>(64, (nnm + 255) / 256), (unsigned)mats.size()),
-                           dim3(256), 0, c->st, din, dout, n, nm);
+    hipLaunchKernelGGL(k_transpose_batched<T>, dim3((unsigned)std::min<size_t>(64, (nnm + 255) / 256), (unsigned)mats.size()),
+                       dim3(256), 0, c->st, up.put(mats), up.put(trp), n, nm);
+    const T *L = (const T *)s->d_L;
+    if (f32) {
+        if (!s->d_L32) {
+            s->d_L32 = dalloc<float>((size_t)naux * n * n);
+            launch_to_f32(c->st, s->d_L, s->d_L32, (long long)naux * n * n);
+        }
+        L = (const T *)s->d_L32;
     }
     // distinct Y in groups whose T1 scratch stays within 1 GiB
     std::vector<int> ys;
     for (int y : yi)
         if (std::find(ys.begin(), ys.end(), y) == ys.end()) ys.push_back(y);
-    const size_t gmax = std::max<size_t>(1, (size_t(1) << 30) / (t1sz * sizeof(double)));
+    const size_t gmax = std::max<size_t>(1, (size_t(1) << 30) / (t1sz * sizeof(T)));
     for (size_t g0 = 0; g0 < ys.size(); g0 += gmax) {
         const size_t gn = std::min(gmax, ys.size() - g0);
-        double *t1 = s->t1.get<double>(gn * t1sz);
-        std::vector<GemmDesc> g1;
-        for (size_t k = 0; k < gn; ++k) g1.push_back({trp[ys[g0 + k]], s->d_L, t1 + k * t1sz});
-        launch_gemm_nt_mfma(c->st, nm, naux * n, n, n, n, naux * n, up.put(g1), (int)gn);
-        std::vector<GemmDesc> g2;
+        T *t1 = (f32 ? s->t1f : s->t1).get<T>(gn * t1sz);
+        std::vector<FacGemm> g1, g2;
+        for (size_t k = 0; k < gn; ++k) g1.push_back({trp[ys[g0 + k]], L, t1 + k * t1sz});
         for (size_t j = 0; j < jobs.size(); ++j) {
             const auto pos = std::find(ys.begin() + g0, ys.begin() + g0 + gn, yi[j]);
             if (pos == ys.begin() + g0 + gn) continue;
-            const double *t1y = t1 + (size_t)(pos - (ys.begin() + g0)) * t1sz;
-            g2.push_back({trp[xi[j]], t1y, jobs[j].out});
+            g2.push_back({trp[xi[j]], t1 + (size_t)(pos - (ys.begin() + g0)) * t1sz, jobs[j].out});
         }
-        launch_gemm_nt_mfma(c->st, nm, nm * naux, n, n, n, nm * naux, up.put(g2), (int)g2.size());
+        if (fast) {
+            launch_fac_gemm<T, f32 ? 2 : 1>(c->st, nm, naux * n, n, n, n, naux * n, up.put(g1), (int)g1.size());
+            launch_fac_gemm<T, f32 ? 2 : 1>(c->st, nm, nm * naux, n, n, n, nm * naux, up.put(g2), (int)g2.size());
+        } else if constexpr (!f32) {
+            auto gd = [](const std::vector<FacGemm> &v) {
+                std::vector<GemmDesc> o;
+                for (const FacGemm &f : v) o.push_back({(const double *)f.A, (const double *)f.B, (double *)f.C});
+                return o;
+            };
+            launch_gemm_nt_mfma(c->st, nm, naux * n, n, n, n, naux * n, up.put(gd(g1)), (int)g1.size());
+            launch_gemm_nt_mfma(c->st, nm, nm * naux, n, n, n, nm * naux, up.put(gd(g2)), (int)g2.size());
+        }
     }
+    HIPCHK(hipGetLastError());
+}
+
+void factor_tensors(nw_ctx *c, const std::vector<FacJob> &jobs)
+{
+    CholState *s = c->chol;
+    const size_t ften = (size_t)s->naux * c->nmo * c->nmo;
+    std::vector<FacJob> j64, j32;
+    std::vector<std::pair<const double *, float *>> lower;  // FP32 jobs without the fast kernel
+    for (const FacJob &j : jobs) {
+        if (!j.f32) j64.push_back(j);
+        else if (c->n % 4 == 0) j32.push_back(j);
+        else {
+            double *t = (double *)s->arena(ften * sizeof(double));
+            j64.push_back({j.X, j.Y, t, false});
+            lower.push_back({t, (float *)j.out});
+        }
+    }
+    Uploader up(c, (jobs.size() * 3 + 16) * (sizeof(FacGemm) + 2 * sizeof(void *)) + 16384);
+    if (!j64.empty()) factor_run<double>(c, j64, up);
+    if (!j32.empty()) factor_run<float>(c, j32, up);
+    for (auto &l : lower) launch_to_f32(c->st, l.first, l.second, (long long)ften);
     HIPCHK(hipGetLastError());
 }
 
@@ -644,22 +910,24 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
                             specs.push_back({(int)i, CL_II, fkey(FA, 0, a0, aj), fkey(FA, 1, b0, bj), -1, -1, oo});
                         }
     }
-    // factor tensors not yet made (FP64), then FP32 copies for reduced-precision pairs
+    // factor tensors not yet made, in the pair's precision
     std::vector<FacJob> fj;
     auto need = [&](int i, int key) {
         if (key < 0) return;
         CholPair &cp = s->pairs[work[i].key];
-        if (cp.fac.count(key)) return;
         const PairInfo &pi = c->pinfo[work[i].key];
+        if (pi.fp32 ? cp.fac32.count(key) : cp.fac.count(key)) return;
         const int kind = key >> 3, sp = (key >> 2) & 1, a = (key >> 1) & 1, b = key & 1;
         const int pr = pi.bidx * 2 + sp;
         const double *ca = cx + (size_t)(pr * 2 + a) * nnm, *cb = cx + (size_t)(pr * 2 + b) * nnm;
         const double *xa = xc + (size_t)(pr * 2 + a) * nnm, *xb = xc + (size_t)(pr * 2 + b) * nnm;
-        double *t = (double *)s->arena(ften * sizeof(double));
-        cp.fac[key] = t;
-        if (kind == FA) fj.push_back({ca, xb, t});
-        else if (kind == FE) fj.push_back({ca, cb, t});
-        else fj.push_back({xa, xb, t});
+        void *t;
+        if (pi.fp32) t = cp.fac32[key] = (float *)s->arena(ften * sizeof(float));
+        else t = cp.fac[key] = (double *)s->arena(ften * sizeof(double));
+        const bool f = pi.fp32 != 0;
+        if (kind == FA) fj.push_back({ca, xb, t, f});
+        else if (kind == FE) fj.push_back({ca, cb, t, f});
+        else fj.push_back({xa, xb, t, f});
     };
     for (const TSpec &t : specs) {
         need(t.wi, t.P);
@@ -670,20 +938,12 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
     HIPCHK(hipEventRecord(s->mark(1), c->st));
     if (!fj.empty()) {
         factor_tensors(c, fj);
-        std::vector<const double *> ys;
+        // per distinct (Y, precision) nm x (naux n) x n, per job nm x (nm naux) x n
+        std::vector<std::pair<const double *, bool>> ys;
         for (const FacJob &j : fj)
-            if (std::find(ys.begin(), ys.end(), j.Y) == ys.end()) ys.push_back(j.Y);
-        s->stats[8] += 2.0 * nm * (double)naux * c->n * c->n * (double)(ys.size() + fj.size());
+            if (std::find(ys.begin(), ys.end(), std::make_pair(j.Y, j.f32)) == ys.end()) ys.push_back({j.Y, j.f32});
+        s->stats[8] += 2.0 * nm * (double)s->naux_in * c->n * ((double)ys.size() * c->n + (double)fj.size() * nm);
     }
-    auto f32of = [&](int i, int key) -> const void * {
-        CholPair &cp = s->pairs[work[i].key];
-        auto it = cp.fac32.find(key);
-        if (it != cp.fac32.end()) return it->second;
-        float *t = (float *)s->arena(ften * sizeof(float));
-        launch_to_f32(c->st, cp.fac.at(key), t, (long long)ften);
-        cp.fac32[key] = t;
-        return t;
-    };
     // index lists shared by the products of a (pair, class), in one upload:
     // rectangle rows, per-q ket-particle partners, sparse lists of single
     // blocks, exchange rows and columns.  Offsets are patched after upload.
@@ -699,6 +959,10 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
         std::vector<int> kqn;
         std::vector<std::pair<size_t, size_t>> sgl;   // single blocks grouped by p: offsets of {p} and R(p)
         std::vector<int> sgln;
+        // the same blocks for products whose two factor tensors are one: (p, r)
+        // with p < r built with its mirror (r, p) (sglm), the unpaired ones (sglp)
+        std::vector<std::pair<size_t, size_t>> sglm, sglp;
+        std::vector<int> sglmn, sglpn;
         size_t xp = 0, xr = 0, xq = 0, xs = 0;        // exchange rows (p, r) and columns (q, s)
         int nxrow = 0, nxcol = 0;
     };
@@ -719,13 +983,31 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
                     L.kqn.push_back((int)S.size());
                 }
             }
+            std::vector<uint8_t> is_single(cl < 2 ? nm2 : 0, 0);
+            for (int k : cw.single)
+                if (cl < 2) is_single[k] = 1;
             for (size_t k = 0; k < cw.single.size();) {  // cw.single is ordered by p
                 const int p0 = cw.single[k] / nm;
-                std::vector<int> R;
-                for (; k < cw.single.size() && cw.single[k] / nm == p0; ++k) R.push_back(cw.single[k] % nm);
+                std::vector<int> R, Rm, Rp;
+                for (; k < cw.single.size() && cw.single[k] / nm == p0; ++k) {
+                    const int r = cw.single[k] % nm;
+                    R.push_back(r);
+                    if (cl >= 2) continue;
+                    const bool paired = r != p0 && is_single[(size_t)r * nm + p0];
+                    if (!paired) Rp.push_back(r);
+                    else if (p0 < r) Rm.push_back(r);  // (r, p0) comes as its mirror
+                }
                 const size_t po = put(std::vector<int>{p0});
                 L.sgl.push_back({po, put(R)});
                 L.sgln.push_back((int)R.size());
+                if (!Rm.empty()) {
+                    L.sglm.push_back({po, put(Rm)});
+                    L.sglmn.push_back((int)Rm.size());
+                }
+                if (!Rp.empty()) {
+                    L.sglp.push_back({po, put(Rp)});
+                    L.sglpn.push_back((int)Rp.size());
+                }
             }
             if (c->pinfo[work[i].key].ma + c->pinfo[work[i].key].mb > 0 && cl < 2) {
                 std::vector<int> rp, rr, cq, cs;
@@ -751,21 +1033,18 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
     const int *dints = nullptr;
     auto at = [](size_t off) { return (const int *)(uintptr_t)(off + 1); };
     std::vector<R4Gemm> gd[2], gx[2];
-    std::vector<R4GTile> td[2], tx[2];
+    std::vector<R4Gemm> gk[2];   // ket-particle pair products (narrow)
     std::vector<R4Gemm> gs1[2];  // single blocks, whole
-    std::vector<R4GTile> ts1[2];
     double nsingle[2] = {0.0, 0.0};
     for (int f32 = 0; f32 < 2; ++f32) {
-        auto tiles_of = [&](std::vector<R4GTile> &tl, int g, int M, int N) {
-            for (int m0 = 0; m0 < M; m0 += 64)
-                for (int n0 = 0; n0 < N; n0 += 64) tl.push_back({g, m0, n0, 0});
-        };
         for (const TSpec &t : specs) {
             const PairWork &pw = work[t.wi];
             const PairInfo &pi = c->pinfo[pw.key];
             if ((pi.fp32 != 0) != (f32 == 1)) continue;
             CholPair &cp = s->pairs[pw.key];
-            auto ptr = [&](int key) -> const void * { return f32 ? f32of(t.wi, key) : (const void *)cp.fac.at(key); };
+            auto ptr = [&](int key) -> const void * {
+                return f32 ? (const void *)cp.fac32.at(key) : (const void *)cp.fac.at(key);
+            };
             void *out = (void *)(c->d_slab + pi.base + t.off);
             const ClassWork &cw = pw.cls[t.cl];
             const ClassLists &L = cls[t.wi * 3 + t.cl];
@@ -776,48 +1055,55 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
                 const int *Pl = at(L.rect[ri].first), *Rl = at(L.rect[ri].second);
                 if (oq > 0) {  // rows (p, q < oq) x columns (r, every s)
                     R4Gemm g{PA, QB, out, {Pl, nullptr, np, oq, 0, 0}, {Rl, nullptr, nr, nm, 0, 0}, np * oq, nr * nm, 0, 0};
-                    tiles_of(td[f32], (int)gd[f32].size(), g.M, g.N);
                     gd[f32].push_back(g);
                 }
                 if (os > 0 && oq < nm) {  // rows (p, q >= oq) x columns (r, s < os)
                     R4Gemm g{PA, QB, out, {Pl, nullptr, np, nm - oq, oq, 0}, {Rl, nullptr, nr, os, 0, 0},
                              np * (nm - oq), nr * os, 0, 0};
-                    tiles_of(td[f32], (int)gd[f32].size(), g.M, g.N);
                     gd[f32].push_back(g);
                 }
-                for (size_t k = 0; k < L.kq.size(); ++k) {  // rows (p, q) x columns (r, s in S(q))
+                for (size_t k = 0; k < L.kq.size(); ++k) {  // rows (p, q) x columns (r, s in S(q)), or transposed
                     const int q = L.kq[k].first, ns = L.kqn[k];
-                    R4Gemm g{PA, QB, out, {Pl, nullptr, np, 1, q, 0}, {Rl, at(L.kq[k].second), nr, ns, 0, 0},
-                             np, nr * ns, 0, 0};
-                    tiles_of(td[f32], (int)gd[f32].size(), g.M, g.N);
-                    gd[f32].push_back(g);
+                    const R4Axis pq{Pl, nullptr, np, 1, q, 0}, rs{Rl, at(L.kq[k].second), nr, ns, 0, 0};
+                    if (np >= nr * ns) gk[f32].push_back(R4Gemm{PA, QB, out, pq, rs, np, nr * ns, 0, 0});
+                    else gk[f32].push_back(R4Gemm{QB, PA, out, rs, pq, nr * ns, np, 2, 0});
                 }
             }
-            for (size_t k = 0; k < L.sgl.size(); ++k) {  // whole blocks: rows (p, every q) x columns (r, every s)
-                R4Gemm g{PA, QB, out, {at(L.sgl[k].first), nullptr, 1, nm, 0, 0},
-                         {at(L.sgl[k].second), nullptr, L.sgln[k], nm, 0, 0}, nm, L.sgln[k] * nm, 0, 0};
-                tiles_of(ts1[f32], (int)gs1[f32].size(), g.M, g.N);
-                gs1[f32].push_back(g);
-                nsingle[f32] += L.sgln[k];
+            // whole blocks: rows (p, every q) x columns (r, every s); with one
+            // factor tensor (J slots (i, j, i, j)) a pair of blocks (p, r), (r, p)
+            // comes from one product
+            auto singles = [&](const std::vector<std::pair<size_t, size_t>> &lst, const std::vector<int> &cnt,
+                               int mode) {
+                for (size_t k = 0; k < lst.size(); ++k) {
+                    R4Gemm g{PA, QB, out, {at(lst[k].first), nullptr, 1, nm, 0, 0},
+                             {at(lst[k].second), nullptr, cnt[k], nm, 0, 0}, nm, cnt[k] * nm, mode, 0};
+                    gs1[f32].push_back(g);
+                    nsingle[f32] += mode == 3 ? 2 * cnt[k] : cnt[k];
+                }
+            };
+            if (t.cl < 2 && t.P == t.Q) {
+                singles(L.sglm, L.sglmn, 3);
+                singles(L.sglp, L.sglpn, 0);
+            } else {
+                singles(L.sgl, L.sgln, 0);
             }
             if (t.E >= 0 && L.nxrow > 0 && L.nxcol > 0) {
                 R4Gemm g{ptr(t.E), ptr(t.F), out, {at(L.xp), at(L.xr), L.nxrow, 1, 0, 1},
                          {at(L.xq), at(L.xs), L.nxcol, 1, 0, 1}, L.nxrow, L.nxcol, 1, 0};
-                tiles_of(tx[f32], (int)gx[f32].size(), g.M, g.N);
                 gx[f32].push_back(g);
             }
         }
     }
     // one upload of the lists and every descriptor, then the launches
     size_t bytes = ints.size() * sizeof(int) + 8192;
-    for (int f = 0; f < 2; ++f)
-        bytes += (gd[f].size() + gx[f].size() + gs1[f].size()) * sizeof(R4Gemm) +
-                 (td[f].size() + tx[f].size() + ts1[f].size()) * sizeof(R4GTile) + 3072;
+    for (int f = 0; f < 2; ++f)  // descriptors, and tiles of at most 64 x 64 elements
+        for (auto *v : {&gd[f], &gk[f], &gx[f], &gs1[f]})
+            bytes += v->size() * sizeof(R4Gemm) + r4_tiles(*v, 64, 32) * sizeof(R4GTile) + 1024;
     Uploader up(c, bytes);
     dints = up.put(ints);
     auto fix = [&](const int *p) { return p ? dints + ((uintptr_t)p - 1) : nullptr; };
     for (int f = 0; f < 2; ++f) {
-        for (auto *v : {&gd[f], &gx[f], &gs1[f]})
+        for (auto *v : {&gd[f], &gk[f], &gx[f], &gs1[f]})
             for (R4Gemm &g : *v) {
                 g.row.u = fix(g.row.u);
                 g.row.v = fix(g.row.v);
@@ -826,32 +1112,22 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
             }
     }
     for (int f = 0; f < 2; ++f) {
-        for (const R4Gemm &g : gd[f]) s->stats[5] += 2.0 * g.M * (double)g.N * naux;
-        for (const R4Gemm &g : gx[f]) s->stats[7] += 2.0 * g.M * (double)g.N * naux;
-        for (const R4Gemm &g : gs1[f]) s->stats[6] += 2.0 * g.M * (double)g.N * naux;
+        for (const R4Gemm &g : gd[f]) s->stats[5] += 2.0 * g.M * (double)g.N * s->naux_in;
+        for (const R4Gemm &g : gk[f]) s->stats[5] += 2.0 * g.M * (double)g.N * s->naux_in;
+        for (const R4Gemm &g : gx[f]) s->stats[7] += 2.0 * g.M * (double)g.N * s->naux_in;
+        for (const R4Gemm &g : gs1[f]) s->stats[6] += 2.0 * g.M * (double)g.N * s->naux_in;
         s->stats[9] += nsingle[f];
     }
     for (int f32 = 0; f32 < 2; ++f32) {
-        if (!td[f32].empty()) {
-            HIPCHK(hipEventRecord(s->mark(2), c->st));
-            const R4Gemm *dg = up.put(gd[f32]);
-            const R4GTile *dt = up.put(td[f32]);
-            if (f32) launch_r4g<float>(c->st, dg, dt, (int)td[f32].size(), nm, naux);
-            else launch_r4g<double>(c->st, dg, dt, (int)td[f32].size(), nm, naux);
-        }
-        if (!ts1[f32].empty()) {
-            HIPCHK(hipEventRecord(s->mark(3), c->st));
-            const R4Gemm *dg = up.put(gs1[f32]);
-            const R4GTile *dt = up.put(ts1[f32]);
-            if (f32) launch_r4g<float>(c->st, dg, dt, (int)ts1[f32].size(), nm, naux);
-            else launch_r4g<double>(c->st, dg, dt, (int)ts1[f32].size(), nm, naux);
-        }
-        if (!tx[f32].empty()) {  // after the direct products of the same blocks (stream order)
-            HIPCHK(hipEventRecord(s->mark(4), c->st));
-            const R4Gemm *dg = up.put(gx[f32]);
-            const R4GTile *dt = up.put(tx[f32]);
-            if (f32) launch_r4g<float>(c->st, dg, dt, (int)tx[f32].size(), nm, naux);
-            else launch_r4g<double>(c->st, dg, dt, (int)tx[f32].size(), nm, naux);
+        // the exchange products run after the direct products of the same blocks (stream order)
+        const int stage[4] = {2, 2, 3, 4};
+        std::vector<R4Gemm> *lists[4] = {&gd[f32], &gk[f32], &gs1[f32], &gx[f32]};
+        for (int k = 0; k < 4; ++k) {
+            if (lists[k]->empty()) continue;
+            if (k != 1) HIPCHK(hipEventRecord(s->mark(stage[k]), c->st));
+            if (k == 1) launch_r4g_narrow(c, up, *lists[k], naux, f32 != 0);
+            else if (f32) launch_r4g<float>(c, up, *lists[k], naux);
+            else launch_r4g<double>(c, up, *lists[k], naux);
         }
         HIPCHK(hipGetLastError());
     }
@@ -868,29 +1144,59 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
 
 }  // namespace
 
-void chol_ensure(nw_ctx *c, const std::vector<PackedDet> &h)
+void chol_read_stages(CholState *s);
+
+std::vector<uint8_t> triangle_pairs(const nw_ctx *c, const std::vector<PackedDet> &h)
+{
+    // runs of equal (parent, ranks) in list order; the upper triangle pairs
+    // runs r1 <= r2, so it reads the ordered reference pairs (parent(r1), parent(r2))
+    std::vector<int> runp;
+    for (size_t i = 0; i < h.size(); ++i)
+        if (i == 0 || h[i].parent != h[i - 1].parent || h[i].ra != h[i - 1].ra || h[i].rb != h[i - 1].rb)
+            runp.push_back(h[i].parent);
+    std::vector<uint8_t> seen((size_t)c->nref, 0), out((size_t)c->nref * c->nref, 0);
+    for (size_t r2 = 0; r2 < runp.size(); ++r2) {
+        seen[runp[r2]] = 1;
+        for (int x = 0; x < c->nref; ++x)
+            if (seen[x]) out[(size_t)x * c->nref + runp[r2]] = 1;
+    }
+    return out;
+}
+
+void chol_ensure(nw_ctx *c, const std::vector<PackedDet> &h, bool accumulate, const std::vector<uint8_t> *reads)
 {
     CholState *s = c->chol;
     if (!s) return;
+    if (accumulate) chol_read_stages(s);
+    else
+        for (double &v : s->stats) v = 0.0;
     s->ms_blocks = 0.0;
-    for (double &v : s->stats) v = 0.0;
     s->ev_stage.clear();
+    s->pending = false;
     std::vector<PairWork> work;
-    if (!plan_missing(c, h, work)) return;
+    if (!plan_missing(c, h, work, reads)) return;
     HIPCHK(hipEventRecord(s->mark(0), c->st));
     build_blocks(c, work);
+    s->pending = true;  // the caller's evaluation is planned while the blocks build
+}
+
+// Stage times of the last chol_ensure, once its last event has completed.
+void chol_read_stages(CholState *s)
+{
+    if (!s || !s->pending) return;
+    s->pending = false;
     HIPCHK(hipEventSynchronize(s->evs[s->ev_stage.size() - 1]));
     // stage k runs from its mark to the next mark
     for (size_t k = 0; k + 1 < s->ev_stage.size(); ++k)
         s->stats[s->ev_stage[k]] += elapsed(s->evs[k], s->evs[k + 1]);
     s->ms_blocks = elapsed(s->evs[0], s->evs[s->ev_stage.size() - 1]);
-    s->stats[0] = s->ms_blocks;
+    s->stats[0] += s->ms_blocks;
 }
 
 bool chol_missing(nw_ctx *c, const std::vector<PackedDet> &h)
 {
     std::vector<PairWork> work;
-    return plan_missing(c, h, work);
+    return plan_missing(c, h, work, nullptr);
 }
 
 void chol_complete(nw_ctx *c)
@@ -911,7 +1217,12 @@ void chol_complete(nw_ctx *c)
     }
 }
 
-double chol_last_ms(nw_ctx *c) { return c->chol ? c->chol->ms_blocks : 0.0; }
+double chol_last_ms(nw_ctx *c)
+{
+    if (!c->chol) return 0.0;
+    chol_read_stages(c->chol);
+    return c->chol->ms_blocks;
+}
 
 bool chol_any_lazy(nw_ctx *c)
 {
@@ -964,15 +1275,19 @@ int nw_set_eri_factors(nw_ctx *c, const double *L, int naux, double check_tol, d
         CholState *s = c->chol;
         HIPCHK(hipStreamSynchronize(c->st));
         if (s->d_L) HIPCHK(hipFree(s->d_L));
+        if (s->d_L32) HIPCHK(hipFree(s->d_L32));
         s->d_L = nullptr;
-        s->naux = 0;
+        s->d_L32 = nullptr;
+        s->naux = s->naux_in = 0;
         s->pairs.clear();
         c->built = false;  // pairs built before take effect only after the next nw_build
         if (naux == 0) return NW_OK;
         const int n = c->n;
-        const size_t cnt = (size_t)naux * n * n;
-        double *dL = dalloc<double>(cnt);
+        const int npad = (naux + 3) & ~3;  // zero factors keep every factor-tensor row 16-byte aligned
+        const size_t cnt = (size_t)naux * n * n, cpad = (size_t)npad * n * n;
+        double *dL = dalloc<double>(cpad);
         HIPCHK(hipMemcpyAsync(dL, L, cnt * sizeof(double), hipMemcpyDefault, c->st));
+        if (cpad > cnt) HIPCHK(hipMemsetAsync(dL + cnt, 0, (cpad - cnt) * sizeof(double), c->st));
         unsigned long long *dmax = c->flag.get<unsigned long long>(1);
         HIPCHK(hipMemsetAsync(dmax, 0, sizeof(unsigned long long), c->st));
         const int nsample = 16384;
@@ -992,7 +1307,8 @@ int nw_set_eri_factors(nw_ctx *c, const double *L, int naux, double check_tol, d
             return fail(c, NW_E_ARG, msg);
         }
         s->d_L = dL;
-        s->naux = naux;
+        s->naux = npad;
+        s->naux_in = naux;
         return NW_OK;
     });
 }
@@ -1001,6 +1317,7 @@ int nw_rank4_stats(nw_ctx *c, double *out10)
 {
     return guarded(c, [&]() -> int {
         if (!out10) return fail(c, NW_E_ARG, "null output");
+        if (c->chol) chol_read_stages(c->chol);
         for (int k = 0; k < 10; ++k) out10[k] = c->chol ? c->chol->stats[k] : 0.0;
         return NW_OK;
     });

@@ -234,12 +234,18 @@ void run_rank4_transforms(nw_ctx *c, const std::vector<int> &keys);
 // about to run may leave their rank-4 blocks to chol_ensure.  chol_on_build:
 // record which pairs of the new build did (lazy[p] for build index p).
 // chol_ensure: build every block the packed determinants h read on any
-// pair (superset: every determinant of a pair's parents as bra and ket).
+// pair (superset: every determinant of a pair's parents as bra and ket), or
+// on the pairs marked in reads (x * nref + w); the stage times and flops it
+// records add to the previous call's when accumulate is set (the groups of
+// one streamed call).  triangle_pairs: the ordered reference pairs an
+// upper-triangle evaluation of h reads.
 // chol_missing: true if chol_ensure would have work for h (no device
 // allocation).  chol_complete: every rank-4 block of every built pair.
 bool chol_route(nw_ctx *c);
 void chol_on_build(nw_ctx *c, const std::vector<int> &keys, const std::vector<uint8_t> &lazy);
-void chol_ensure(nw_ctx *c, const std::vector<PackedDet> &h);
+void chol_ensure(nw_ctx *c, const std::vector<PackedDet> &h, bool accumulate = false,
+                 const std::vector<uint8_t> *reads = nullptr);
+std::vector<uint8_t> triangle_pairs(const nw_ctx *c, const std::vector<PackedDet> &h);
 bool chol_missing(nw_ctx *c, const std::vector<PackedDet> &h);
 void chol_complete(nw_ctx *c);
 void chol_forget(nw_ctx *c);  // a loaded slab: every pair complete

@@ -105,10 +105,21 @@ def test_batched_project_and_streamed(small):
     assert np.max(np.abs(v - vd)) < TOL and np.max(np.abs(s - sd)) < TOL
     eng.close()
     eng = _pair(sysm)
+    eng.build()
+    eng.hs_matrix(rec)
+    whole = eng.rank4_stats()
+    eng.close()
+    eng = _pair(sysm)
     H, S, ng = eng.hs_matrix_streamed(rec, slab_bytes=5 << 19)
     assert ng > 1
     Hd, Sd = dense.hs_matrix(rec)
     assert np.max(np.abs(H - Hd)) < TOL and np.max(np.abs(S - Sd)) < TOL
+    # the stage records of a streamed call add up over its groups: the same
+    # blocks as one call over every pair
+    st = eng.rank4_stats()
+    for k in ("flops_rect", "flops_single", "flops_exchange", "flops_factor", "single_blocks"):
+        assert st[k] == pytest.approx(whole[k], rel=1e-12), k
+    assert st["ms_total"] > 0.0
     eng.close()
 
 
