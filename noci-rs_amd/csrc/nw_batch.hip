@@ -406,6 +406,7 @@ int nw_batch_add(nw_batch *b, nw_ctx *ctx, const nw_det *dets, uint64_t ndet, in
         HIPCHK(hipMemcpyAsync(S.dets, dp, ndet * sizeof(PackedDet), hipMemcpyDeviceToDevice, ctx->st));
         HIPCHK(hipStreamSynchronize(ctx->st));
         ctx->keep.clear();
+        ctx->pinned.reset();
         S.H = dalloc<double>(ndet * ndet);
         S.S = dalloc<double>(ndet * ndet);
         b->sys.push_back(std::move(S));

@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <thread>
 #include <cstdio>
 #include <vector>
 
@@ -569,6 +570,8 @@ struct ParentRows {
     std::vector<uint8_t> bra2[2], ket2[2];          // nm x nm, same-spin pairs of one determinant
     std::vector<uint8_t> braab, ketab;              // nm x nm, (alpha row, beta row) of one determinant
     std::vector<uint8_t> kcol2[2], kcolab;          // nm x nm, ket particle (column label) pairs of one determinant
+    // the same sets as lists (each entry once, in first-seen order)
+    std::vector<int> bral[2], ketl[2], bra2l[2], ket2l[2], braabl, ketabl, kcol2l[2], kcolabl;
 };
 
 std::vector<ParentRows> parent_rows(nw_ctx *c, const std::vector<PackedDet> &h)
@@ -593,6 +596,12 @@ std::vector<ParentRows> parent_rows(nw_ctx *c, const std::vector<PackedDet> &h)
             r.kcol2[1].assign(nm2, 0);
             r.kcolab.assign(nm2, 0);
         }
+        auto add = [](std::vector<uint8_t> &set, std::vector<int> &list, size_t k) {
+            if (!set[k]) {
+                set[k] = 1;
+                list.push_back((int)k);
+            }
+        };
         int br[2][4], kr[2][4], kc[2][4], rk[2] = {d.ra, d.rb};
         for (int sp = 0; sp < 2; ++sp) {
             const uint8_t *ix = sp ? d.ib : d.ia;
@@ -601,22 +610,22 @@ std::vector<ParentRows> parent_rows(nw_ctx *c, const std::vector<PackedDet> &h)
                 br[sp][i] = std::min(nm - 1, std::max(0, (int)ix[4 + i] - no));
                 kr[sp][i] = std::min(nm - 1, std::max(0, nv + (int)ix[i]));
                 kc[sp][i] = std::min(nm - 1, (int)ix[4 + i]);  // column label of a ket particle
-                r.bra[sp][br[sp][i]] = 1;
-                r.ket[sp][kr[sp][i]] = 1;
+                add(r.bra[sp], r.bral[sp], br[sp][i]);
+                add(r.ket[sp], r.ketl[sp], kr[sp][i]);
             }
             for (int i = 0; i < rk[sp]; ++i)
                 for (int j = 0; j < rk[sp]; ++j)
                     if (i != j) {
-                        r.bra2[sp][(size_t)br[sp][i] * nm + br[sp][j]] = 1;
-                        r.ket2[sp][(size_t)kr[sp][i] * nm + kr[sp][j]] = 1;
-                        r.kcol2[sp][(size_t)kc[sp][i] * nm + kc[sp][j]] = 1;
+                        add(r.bra2[sp], r.bra2l[sp], (size_t)br[sp][i] * nm + br[sp][j]);
+                        add(r.ket2[sp], r.ket2l[sp], (size_t)kr[sp][i] * nm + kr[sp][j]);
+                        add(r.kcol2[sp], r.kcol2l[sp], (size_t)kc[sp][i] * nm + kc[sp][j]);
                     }
         }
         for (int i = 0; i < rk[0]; ++i)
             for (int j = 0; j < rk[1]; ++j) {
-                r.braab[(size_t)br[0][i] * nm + br[1][j]] = 1;
-                r.ketab[(size_t)kr[0][i] * nm + kr[1][j]] = 1;
-                r.kcolab[(size_t)kc[0][i] * nm + kc[1][j]] = 1;
+                add(r.braab, r.braabl, (size_t)br[0][i] * nm + br[1][j]);
+                add(r.ketab, r.ketabl, (size_t)kr[0][i] * nm + kr[1][j]);
+                add(r.kcolab, r.kcolabl, (size_t)kc[0][i] * nm + kc[1][j]);
             }
     }
     return pr;
@@ -632,13 +641,6 @@ std::vector<ParentRows> parent_rows(nw_ctx *c, const std::vector<PackedDet> &h)
 // preparehamiltonianoverlap.rs:20550-20966).  Within a block they read the
 // columns (q, s) of the contraction labels: a bra hole (q < nocc or
 // s < nocc, dense) or two ket particles of one ket determinant (kk, sparse).
-bool block_needed(int cl, const ParentRows &X, const ParentRows &W, bool both, int p, int r, int nm)
-{
-    const size_t k = (size_t)p * nm + r;
-    if (cl < 2)
-        return X.bra2[cl][k] || W.ket2[cl][k] || (X.bra[cl][p] && W.ket[cl][r]) || (both && X.bra[cl][r] && W.ket[cl][p]);
-    return X.braab[k] || W.ketab[k] || (X.bra[0][p] && W.ket[1][r]) || (W.ket[0][p] && X.bra[1][r]);
-}
 
 // Work of one pair: per class, the blocks to (re)build and the union of the
 // ket-particle column pairs they must hold (a later call whose kets add
@@ -666,74 +668,138 @@ bool plan_missing(nw_ctx *c, const std::vector<PackedDet> &h, std::vector<PairWo
     const int nref = c->nref, nm = c->nmo;
     const size_t nm2 = (size_t)nm * nm;
     std::vector<ParentRows> pr = parent_rows(c, h);
-    for (int x = 0; x < nref; ++x) {
-        if (!pr[x].present) continue;
+    std::vector<int> keys;
+    for (int x = 0; x < nref; ++x)
         for (int w = 0; w < nref; ++w) {
-            if (!pr[w].present) continue;
             const int key = x * nref + w;
+            if (pr[x].present && pr[w].present && s->pairs[key].lazy && c->pinfo[key].built &&
+                (!reads || (*reads)[key]))
+                keys.push_back(key);
+        }
+    if (keys.empty()) return false;
+    // the pairs' plans are independent: a few host threads, each with its own scratch
+    std::vector<PairWork> res(keys.size());
+    std::vector<uint8_t> has(keys.size(), 0);
+    auto plan_range = [&](size_t t0, size_t step) {
+        std::vector<unsigned> stamp(nm2, 0);  // block k seen in the current class when stamp[k] == stamp_gen
+        unsigned stamp_gen = 0;
+        std::vector<uint8_t> todo(nm2, 0), done(nm2, 0);  // cleared entry by entry after each class
+        std::vector<int> pcount(nm + 1), sorted;
+        for (size_t ki = t0; ki < keys.size(); ki += step) {
+            const int key = keys[ki], x = key / nref, w = key % nref;
             const CholPair &cp = s->pairs[key];
             const PairInfo &pi = c->pinfo[key];
-            if (!cp.lazy || !pi.built || (reads && !(*reads)[key])) continue;
             const bool both = pi.ma + pi.mb > 0;
             PairWork pw;
             pw.key = key;
             bool any = false;
             for (int cl = 0; cl < 3; ++cl) {
-                const std::vector<uint8_t> &need_kk = cl < 2 ? pr[w].kcol2[cl] : pr[w].kcolab;
+                const std::vector<int> &need_kk = cl < 2 ? pr[w].kcol2l[cl] : pr[w].kcolabl;
+                const std::vector<uint8_t> &cov = cp.cov[cl];
                 bool grow = false;
-                for (size_t k = 0; k < nm2 && !grow; ++k) grow = need_kk[k] && !cp.kkcov[cl][k];
+                for (int k : need_kk) grow = grow || !cp.kkcov[cl][k];
                 ClassWork &cw = pw.cls[cl];
-                for (int p = 0; p < nm; ++p)
-                    for (int r = 0; r < nm; ++r) {
-                        const size_t k = (size_t)p * nm + r;
-                        const bool built = cp.cov[cl][k] != 0;
-                        if (built ? grow : block_needed(cl, pr[x], pr[w], both, p, r, nm)) cw.blocks.push_back((int)k);
-                    }
+                // candidates: the blocks the determinant pairs read (above), from the
+                // parents' row lists; with new ket-particle pairs every built block is
+                // rebuilt as well
+                ++stamp_gen;
+                auto see = [&](int p, int r) {
+                    const size_t k = (size_t)p * nm + r;
+                    if (stamp[k] == stamp_gen) return;
+                    stamp[k] = stamp_gen;
+                    if (!cov[k] || grow) cw.blocks.push_back((int)k);
+                };
+                const ParentRows &X = pr[x], &W = pr[w];
+                if (cl < 2) {
+                    for (int k : X.bra2l[cl]) see(k / nm, k % nm);
+                    for (int k : W.ket2l[cl]) see(k / nm, k % nm);
+                    for (int p : X.bral[cl])
+                        for (int r : W.ketl[cl]) see(p, r);
+                    if (both)
+                        for (int p : W.ketl[cl])
+                            for (int r : X.bral[cl]) see(p, r);
+                } else {
+                    for (int k : X.braabl) see(k / nm, k % nm);
+                    for (int k : W.ketabl) see(k / nm, k % nm);
+                    for (int p : X.bral[0])
+                        for (int r : W.ketl[1]) see(p, r);
+                    for (int p : W.ketl[0])
+                        for (int r : X.bral[1]) see(p, r);
+                }
+                if (grow)
+                    for (size_t k = 0; k < nm2; ++k)
+                        if (cov[k] && stamp[k] != stamp_gen) cw.blocks.push_back((int)k);
                 if (cw.blocks.empty()) continue;
+                {  // order by p (counting sort): single blocks are grouped by p
+                    std::fill(pcount.begin(), pcount.end(), 0);
+                    for (int k : cw.blocks) ++pcount[k / nm + 1];
+                    for (int p = 0; p < nm; ++p) pcount[p + 1] += pcount[p];
+                    sorted.resize(cw.blocks.size());
+                    for (int k : cw.blocks) sorted[pcount[k / nm]++] = k;
+                    cw.blocks.swap(sorted);
+                }
                 cw.kk = cp.kkcov[cl];
-                for (size_t k = 0; k < nm2; ++k) cw.kk[k] |= need_kk[k];
+                for (int k : need_kk) cw.kk[k] = 1;
                 // rectangles: bra rows of x against ket rows of w (both orders
                 // for J of pairs with zero-overlap orbitals; II: alpha bra x beta
                 // ket and alpha ket x beta bra), cut to the rows with work
-                std::vector<uint8_t> todo(nm2, 0), done(nm2, 0);
                 for (int k : cw.blocks) todo[k] = 1;
-                std::vector<std::pair<const std::vector<uint8_t> *, const std::vector<uint8_t> *>> cand;
+                std::vector<std::pair<const std::vector<int> *, const std::vector<int> *>> rc;
                 if (cl < 2) {
-                    cand.push_back({&pr[x].bra[cl], &pr[w].ket[cl]});
-                    if (both) cand.push_back({&pr[w].ket[cl], &pr[x].bra[cl]});
+                    rc.push_back({&X.bral[cl], &W.ketl[cl]});
+                    if (both) rc.push_back({&W.ketl[cl], &X.bral[cl]});
                 } else {
-                    cand.push_back({&pr[x].bra[0], &pr[w].ket[1]});
-                    cand.push_back({&pr[w].ket[0], &pr[x].bra[1]});
+                    rc.push_back({&X.bral[0], &W.ketl[1]});
+                    rc.push_back({&W.ketl[0], &X.bral[1]});
                 }
-                for (auto &cd : cand) {
+                std::vector<int> extra;  // rectangle blocks not in the list
+                for (auto &cd : rc) {
                     std::vector<int> P, R;
-                    for (int p = 0; p < nm; ++p) {
-                        if (!(*cd.first)[p]) continue;
+                    for (int p : *cd.first) {
                         bool any_r = false;
-                        for (int r = 0; r < nm && !any_r; ++r) any_r = (*cd.second)[r] && todo[(size_t)p * nm + r];
+                        for (int r : *cd.second) any_r = any_r || todo[(size_t)p * nm + r];
                         if (any_r) P.push_back(p);
                     }
-                    for (int r = 0; r < nm; ++r) {
-                        if (!(*cd.second)[r]) continue;
+                    for (int r : *cd.second) {
                         bool any_p = false;
                         for (int p : P) any_p = any_p || todo[(size_t)p * nm + r];
                         if (any_p) R.push_back(r);
                     }
                     if (P.empty() || R.empty()) continue;
+                    std::sort(P.begin(), P.end());
+                    std::sort(R.begin(), R.end());
                     for (int p : P)
-                        for (int r : R) done[(size_t)p * nm + r] = 1;
+                        for (int r : R) {
+                            const size_t k = (size_t)p * nm + r;
+                            if (!done[k] && !todo[k]) extra.push_back((int)k);
+                            done[k] = 1;
+                        }
                     cw.rects.push_back({std::move(P), std::move(R)});
                 }
                 for (int k : cw.blocks)
                     if (!done[k]) cw.single.push_back(k);
-                cw.blocks.clear();
-                for (size_t k = 0; k < nm2; ++k)
-                    if (todo[k] || done[k]) cw.blocks.push_back((int)k);
+                for (int k : cw.blocks) todo[k] = done[k] = 0;
+                for (int k : extra) done[k] = 0;
+                cw.blocks.insert(cw.blocks.end(), extra.begin(), extra.end());
                 any = true;
             }
-            if (any) work.push_back(std::move(pw));
+            if (any) {
+                res[ki] = std::move(pw);
+                has[ki] = 1;
+            }
         }
+    };
+    const size_t nt = std::min<size_t>({8, keys.size(), std::max(1u, std::thread::hardware_concurrency())});
+    if (nt <= 1) {
+        plan_range(0, 1);
+    } else {
+        std::vector<std::thread> th;
+        for (size_t t = 1; t < nt; ++t) th.emplace_back(plan_range, t, nt);
+        plan_range(0, nt);
+        for (auto &t : th) t.join();
     }
+    for (size_t ki = 0; ki < keys.size(); ++ki)
+        if (has[ki]) work.push_back(std::move(res[ki]));
     return !work.empty();
 }
 
@@ -873,49 +939,56 @@ void factor_tensors(nw_ctx *c, const std::vector<FacJob> &jobs)
 // blocks are whole: rows (p, every q) x columns (r in R(p), every s).
 // Exchange products take every built block as a row and the read columns as
 // columns.
-void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
+// The rank-4 tensors of a pair's classes (bit cl of clmask) and the factor
+// tensors of each: J slot (i, j, k, l) direct P = A^{kl}, Q = A^{ij} and,
+// for pairs with zero-overlap orbitals, exchange E = E^{ki}, F = F^{lj};
+// II (a0, aj, b0, bj) P = A_alpha^{a0 aj}, Q = A_beta^{b0 bj}.
+struct TSpec {
+    int wi, cl;
+    int P, Q, E, F;  // factor keys (E, F < 0: no exchange)
+    long long off;
+};
+void pair_specs(const nw_ctx *c, int wi, int key, unsigned clmask, std::vector<TSpec> &specs)
+{
+    const PairInfo &pi = c->pinfo[key];
+    const bool exch = pi.ma + pi.mb > 0;
+    for (int sp = 0; sp < 2; ++sp) {
+        if (!(clmask >> sp & 1)) continue;
+        const long long *o = pi.off + (sp ? O_BB : O_AA);
+        for (int sl = 0; sl < 10; ++sl) {
+            if (o[O_J + sl] < 0) continue;
+            const int ii = jbranch(sl, 0), jj = jbranch(sl, 1), kk = jbranch(sl, 2), ll = jbranch(sl, 3);
+            specs.push_back({wi, sp, fkey(FA, sp, kk, ll), fkey(FA, sp, ii, jj), exch ? fkey(FE, sp, kk, ii) : -1,
+                             exch ? fkey(FF, sp, ll, jj) : -1, o[O_J + sl]});
+        }
+    }
+    if (clmask >> CL_II & 1)
+        for (int a0 = 0; a0 < 2; ++a0)
+            for (int aj = 0; aj < 2; ++aj)
+                for (int b0 = 0; b0 < 2; ++b0)
+                    for (int bj = 0; bj < 2; ++bj) {
+                        const long long oo = pi.off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj];
+                        if (oo < 0) continue;
+                        specs.push_back({wi, CL_II, fkey(FA, 0, a0, aj), fkey(FA, 1, b0, bj), -1, -1, oo});
+                    }
+}
+
+// Launch the factor products of every factor tensor the given (pair,
+// class mask) entries read and the pairs do not hold yet, in each pair's
+// precision (FacJob), as stage 1.
+void make_factors(nw_ctx *c, const std::vector<std::pair<int, unsigned>> &pairs)
 {
     CholState *s = c->chol;
-    const int nm = c->nmo, naux = s->naux;
-    const size_t nm2 = (size_t)nm * nm, nnm = (size_t)c->n * nm, ften = (size_t)naux * nm2;
+    const int nm = c->nmo;
+    const size_t nnm = (size_t)c->n * nm, ften = (size_t)s->naux * nm * nm;
     double *cx = (double *)c->keepCX.p, *xc = (double *)c->keepXC.p;
-    auto occ_q = [&](int cl) { return cl < 2 ? c->nocc[cl] : c->nocc[0]; };
-    auto occ_s = [&](int cl) { return cl < 2 ? c->nocc[cl] : c->nocc[1]; };
-    struct TSpec {
-        int wi, cl;
-        int P, Q, E, F;  // factor keys (E, F < 0: no exchange)
-        long long off;
-    };
     std::vector<TSpec> specs;
-    for (size_t i = 0; i < work.size(); ++i) {
-        const PairInfo &pi = c->pinfo[work[i].key];
-        const bool exch = pi.ma + pi.mb > 0;
-        for (int sp = 0; sp < 2; ++sp) {
-            if (work[i].cls[sp].blocks.empty()) continue;
-            const long long *o = pi.off + (sp ? O_BB : O_AA);
-            for (int sl = 0; sl < 10; ++sl) {
-                if (o[O_J + sl] < 0) continue;
-                const int ii = jbranch(sl, 0), jj = jbranch(sl, 1), kk = jbranch(sl, 2), ll = jbranch(sl, 3);
-                specs.push_back({(int)i, sp, fkey(FA, sp, kk, ll), fkey(FA, sp, ii, jj),
-                                 exch ? fkey(FE, sp, kk, ii) : -1, exch ? fkey(FF, sp, ll, jj) : -1, o[O_J + sl]});
-            }
-        }
-        if (!work[i].cls[CL_II].blocks.empty())
-            for (int a0 = 0; a0 < 2; ++a0)
-                for (int aj = 0; aj < 2; ++aj)
-                    for (int b0 = 0; b0 < 2; ++b0)
-                        for (int bj = 0; bj < 2; ++bj) {
-                            const long long oo = pi.off[O_II + ((a0 * 2 + aj) * 2 + b0) * 2 + bj];
-                            if (oo < 0) continue;
-                            specs.push_back({(int)i, CL_II, fkey(FA, 0, a0, aj), fkey(FA, 1, b0, bj), -1, -1, oo});
-                        }
-    }
-    // factor tensors not yet made, in the pair's precision
+    for (size_t i = 0; i < pairs.size(); ++i) pair_specs(c, (int)i, pairs[i].first, pairs[i].second, specs);
     std::vector<FacJob> fj;
     auto need = [&](int i, int key) {
         if (key < 0) return;
-        CholPair &cp = s->pairs[work[i].key];
-        const PairInfo &pi = c->pinfo[work[i].key];
+        CholPair &cp = s->pairs[pairs[i].first];
+        const PairInfo &pi = c->pinfo[pairs[i].first];
         if (pi.fp32 ? cp.fac32.count(key) : cp.fac.count(key)) return;
         const int kind = key >> 3, sp = (key >> 2) & 1, a = (key >> 1) & 1, b = key & 1;
         const int pr = pi.bidx * 2 + sp;
@@ -935,15 +1008,35 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
         need(t.wi, t.E);
         need(t.wi, t.F);
     }
+    if (fj.empty()) return;
+    if (s->ev_stage.empty()) HIPCHK(hipEventRecord(s->mark(0), c->st));  // first launch of the call
     HIPCHK(hipEventRecord(s->mark(1), c->st));
-    if (!fj.empty()) {
-        factor_tensors(c, fj);
-        // per distinct (Y, precision) nm x (naux n) x n, per job nm x (nm naux) x n
-        std::vector<std::pair<const double *, bool>> ys;
-        for (const FacJob &j : fj)
-            if (std::find(ys.begin(), ys.end(), std::make_pair(j.Y, j.f32)) == ys.end()) ys.push_back({j.Y, j.f32});
-        s->stats[8] += 2.0 * nm * (double)s->naux_in * c->n * ((double)ys.size() * c->n + (double)fj.size() * nm);
+    factor_tensors(c, fj);
+    // per distinct (Y, precision) nm x (naux n) x n, per job nm x (nm naux) x n
+    std::vector<std::pair<const double *, bool>> ys;
+    for (const FacJob &j : fj)
+        if (std::find(ys.begin(), ys.end(), std::make_pair(j.Y, j.f32)) == ys.end()) ys.push_back({j.Y, j.f32});
+    s->stats[8] += 2.0 * nm * (double)s->naux_in * c->n * ((double)ys.size() * c->n + (double)fj.size() * nm);
+}
+
+void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
+{
+    CholState *s = c->chol;
+    const int nm = c->nmo, naux = s->naux;
+    const size_t nm2 = (size_t)nm * nm;
+    auto occ_q = [&](int cl) { return cl < 2 ? c->nocc[cl] : c->nocc[0]; };
+    auto occ_s = [&](int cl) { return cl < 2 ? c->nocc[cl] : c->nocc[1]; };
+    if (s->ev_stage.empty()) HIPCHK(hipEventRecord(s->mark(0), c->st));  // first launch of the call
+    std::vector<TSpec> specs;
+    std::vector<std::pair<int, unsigned>> pairs;
+    for (size_t i = 0; i < work.size(); ++i) {
+        unsigned mask = 0;
+        for (int cl = 0; cl < 3; ++cl)
+            if (!work[i].cls[cl].blocks.empty()) mask |= 1u << cl;
+        pair_specs(c, (int)i, work[i].key, mask, specs);
+        pairs.push_back({work[i].key, mask});
     }
+    make_factors(c, pairs);  // those not made before the planning
     // index lists shared by the products of a (pair, class), in one upload:
     // rectangle rows, per-q ket-particle partners, sparse lists of single
     // blocks, exchange rows and columns.  Offsets are patched after upload.
@@ -967,6 +1060,8 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
         int nxrow = 0, nxcol = 0;
     };
     std::vector<ClassLists> cls(work.size() * 3);
+    std::vector<uint8_t> is_single(nm2, 0);  // single blocks of the current J class (cleared after it)
+    ints.reserve(1 << 20);
     for (size_t i = 0; i < work.size(); ++i)
         for (int cl = 0; cl < 3; ++cl) {
             const ClassWork &cw = work[i].cls[cl];
@@ -974,53 +1069,74 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
             ClassLists &L = cls[i * 3 + cl];
             const int oq = occ_q(cl), os = occ_s(cl);
             for (auto &rc : cw.rects) L.rect.push_back({put(rc.first), put(rc.second)});
-            for (int q = oq; q < nm; ++q) {
-                std::vector<int> S;
-                for (int t = os; t < nm; ++t)
-                    if (cw.kk[(size_t)q * nm + t]) S.push_back(t);
-                if (!S.empty()) {
-                    L.kq.push_back({q, put(S)});
-                    L.kqn.push_back((int)S.size());
+            for (int q = oq; q < nm; ++q) {  // S(q), appended in place (sparse rows: 8 bytes at a time)
+                const size_t o = ints.size();
+                const uint8_t *row = cw.kk.data() + (size_t)q * nm;
+                int t = os;
+                for (; t + 8 <= nm; t += 8) {
+                    uint64_t w8;
+                    std::memcpy(&w8, row + t, 8);
+                    if (!w8) continue;
+                    for (int u = 0; u < 8; ++u)
+                        if (row[t + u]) ints.push_back(t + u);
+                }
+                for (; t < nm; ++t)
+                    if (row[t]) ints.push_back(t);
+                if (ints.size() > o) {
+                    L.kq.push_back({q, o});
+                    L.kqn.push_back((int)(ints.size() - o));
                 }
             }
-            std::vector<uint8_t> is_single(cl < 2 ? nm2 : 0, 0);
-            for (int k : cw.single)
-                if (cl < 2) is_single[k] = 1;
+            if (cl < 2)
+                for (int k : cw.single) is_single[k] = 1;
             for (size_t k = 0; k < cw.single.size();) {  // cw.single is ordered by p
                 const int p0 = cw.single[k] / nm;
-                std::vector<int> R, Rm, Rp;
+                const size_t k0 = k;
                 for (; k < cw.single.size() && cw.single[k] / nm == p0; ++k) {
-                    const int r = cw.single[k] % nm;
-                    R.push_back(r);
-                    if (cl >= 2) continue;
-                    const bool paired = r != p0 && is_single[(size_t)r * nm + p0];
-                    if (!paired) Rp.push_back(r);
-                    else if (p0 < r) Rm.push_back(r);  // (r, p0) comes as its mirror
                 }
-                const size_t po = put(std::vector<int>{p0});
-                L.sgl.push_back({po, put(R)});
-                L.sgln.push_back((int)R.size());
-                if (!Rm.empty()) {
-                    L.sglm.push_back({po, put(Rm)});
-                    L.sglmn.push_back((int)Rm.size());
+                const size_t po = ints.size();
+                ints.push_back(p0);
+                const size_t ro = ints.size();
+                for (size_t e = k0; e < k; ++e) ints.push_back(cw.single[e] % nm);
+                L.sgl.push_back({po, ro});
+                L.sgln.push_back((int)(k - k0));
+                if (cl >= 2) continue;
+                // (p0, r) with p0 < r and (r, p0) also single comes with its mirror
+                const size_t mo = ints.size();
+                for (size_t e = k0; e < k; ++e) {
+                    const int r = cw.single[e] % nm;
+                    if (r != p0 && is_single[(size_t)r * nm + p0] && p0 < r) ints.push_back(r);
                 }
-                if (!Rp.empty()) {
-                    L.sglp.push_back({po, put(Rp)});
-                    L.sglpn.push_back((int)Rp.size());
+                if (ints.size() > mo) {
+                    L.sglm.push_back({po, mo});
+                    L.sglmn.push_back((int)(ints.size() - mo));
+                }
+                const size_t uo = ints.size();
+                for (size_t e = k0; e < k; ++e) {
+                    const int r = cw.single[e] % nm;
+                    if (!(r != p0 && is_single[(size_t)r * nm + p0])) ints.push_back(r);
+                }
+                if (ints.size() > uo) {
+                    L.sglp.push_back({po, uo});
+                    L.sglpn.push_back((int)(ints.size() - uo));
                 }
             }
+            if (cl < 2)
+                for (int k : cw.single) is_single[k] = 0;
             if (c->pinfo[work[i].key].ma + c->pinfo[work[i].key].mb > 0 && cl < 2) {
                 std::vector<int> rp, rr, cq, cs;
                 for (int k : cw.blocks) {
                     rp.push_back(k / nm);
                     rr.push_back(k % nm);
                 }
-                for (int q = 0; q < nm; ++q)
+                for (int q = 0; q < nm; ++q) {  // q < oq: every s; else s < os and the row's kk pairs
+                    const uint8_t *row = cw.kk.data() + (size_t)q * nm;
                     for (int t = 0; t < nm; ++t)
-                        if (q < oq || t < os || cw.kk[(size_t)q * nm + t]) {
+                        if (q < oq || t < os || row[t]) {
                             cq.push_back(q);
                             cs.push_back(t);
                         }
+                }
                 L.xp = put(rp);
                 L.xr = put(rr);
                 L.xq = put(cq);
@@ -1173,11 +1289,25 @@ void chol_ensure(nw_ctx *c, const std::vector<PackedDet> &h, bool accumulate, co
     s->ms_blocks = 0.0;
     s->ev_stage.clear();
     s->pending = false;
+    // Factor products first, for every pair this call may build (lazy,
+    // built, read, both parents among h; every class), so that they run while
+    // the host plans the blocks; those of classes left without blocks are
+    // unused but kept with the pair.
+    std::vector<uint8_t> present((size_t)c->nref, 0);
+    for (const PackedDet &d : h)
+        if (d.parent < c->nref) present[d.parent] = 1;
+    std::vector<std::pair<int, unsigned>> pre;
+    for (size_t k = 0; k < s->pairs.size(); ++k) {
+        const int x = (int)k / c->nref, w = (int)k % c->nref;
+        if (s->pairs[k].lazy && c->pinfo[k].built && present[x] && present[w] && (!reads || (*reads)[k]))
+            pre.push_back({(int)k, 7u});
+    }
+    if (pre.empty()) return;
+    make_factors(c, pre);
     std::vector<PairWork> work;
-    if (!plan_missing(c, h, work, reads)) return;
-    HIPCHK(hipEventRecord(s->mark(0), c->st));
-    build_blocks(c, work);
-    s->pending = true;  // the caller's evaluation is planned while the blocks build
+    if (plan_missing(c, h, work, reads)) build_blocks(c, work);
+    else if (!s->ev_stage.empty()) HIPCHK(hipEventRecord(s->mark(0), c->st));
+    s->pending = !s->ev_stage.empty();  // the caller's evaluation is planned while the blocks build
 }
 
 // Stage times of the last chol_ensure, once its last event has completed.
@@ -1188,7 +1318,7 @@ void chol_read_stages(CholState *s)
     HIPCHK(hipEventSynchronize(s->evs[s->ev_stage.size() - 1]));
     // stage k runs from its mark to the next mark
     for (size_t k = 0; k + 1 < s->ev_stage.size(); ++k)
-        s->stats[s->ev_stage[k]] += elapsed(s->evs[k], s->evs[k + 1]);
+        if (s->ev_stage[k] != 0) s->stats[s->ev_stage[k]] += elapsed(s->evs[k], s->evs[k + 1]);
     s->ms_blocks = elapsed(s->evs[0], s->evs[s->ev_stage.size() - 1]);
     s->stats[0] += s->ms_blocks;
 }

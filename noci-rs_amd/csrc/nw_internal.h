@@ -63,6 +63,43 @@ struct DBuf {
     }
 };
 
+// Pinned host staging for descriptor uploads: a bump allocator over chunks of
+// hipHostMalloc memory, reset at the start of every API call (each call ends
+// with its stream drained, so no copy from an earlier call is pending).
+// Copies from pinned memory are asynchronous DMA; from pageable memory the
+// runtime stages them and the host can wait on work queued before.
+struct PinnedArena {
+    std::vector<std::pair<char *, size_t>> chunks;
+    size_t ci = 0, used = 0;
+    char *get(size_t bytes)
+    {
+        bytes = (bytes + 255) & ~size_t(255);
+        while (ci < chunks.size() && used + bytes > chunks[ci].second) {
+            ++ci;
+            used = 0;
+        }
+        if (ci == chunks.size()) {
+            const size_t sz = std::max<size_t>(bytes, size_t(32) << 20);
+            void *p = nullptr;
+            HIPCHK(hipHostMalloc(&p, sz, hipHostMallocDefault));
+            chunks.push_back({(char *)p, sz});
+            used = 0;
+        }
+        char *p = chunks[ci].first + used;
+        used += bytes;
+        return p;
+    }
+    void reset()
+    {
+        ci = 0;
+        used = 0;
+    }
+    ~PinnedArena()
+    {
+        for (auto &ch : chunks) (void)hipHostFree(ch.first);
+    }
+};
+
 inline bool is_device_ptr(const void *p)
 {
     if (!p) return false;
@@ -117,6 +154,7 @@ struct nw_ctx {
     nwi::DBuf buildscratch, arena_desc, scratchA, scratchB, cx32, xc32, keepM, keepCX, keepXC, fockbuf, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
     nwi::DBuf req_in, req_key, req_bins, req_sorted;  // batched requests binned on the GPU
     std::deque<std::vector<char>> keep;
+    nwi::PinnedArena pinned;  // staging of Uploader copies
     nwi::CholState *chol = nullptr;  // integral factors (nw_set_eri_factors) and block coverage
 
     ~nw_ctx()
@@ -156,8 +194,9 @@ struct Uploader {
         size_t off = (used + 255) & ~size_t(255);
         if (off + bytes > cap) throw NwError(NW_E_MEM, "descriptor arena exhausted");
         if (bytes) {
-            c->keep.emplace_back((const char *)v.data(), (const char *)v.data() + bytes);
-            HIPCHK(hipMemcpyAsync(base + off, c->keep.back().data(), bytes, hipMemcpyHostToDevice, c->st));
+            char *h = c->pinned.get(bytes);
+            std::memcpy(h, v.data(), bytes);
+            HIPCHK(hipMemcpyAsync(base + off, h, bytes, hipMemcpyHostToDevice, c->st));
         }
         used = off + bytes;
         return (T *)(base + off);
@@ -179,6 +218,8 @@ int guarded(nw_ctx *c, F &&f)
         hipError_t e = hipSetDevice(c->device);
         if (e != hipSuccess) return fail(c, NW_E_HIP, hipGetErrorString(e));
         c->keep.clear();
+        if (c->st) HIPCHK(hipStreamSynchronize(c->st));  // no copy from the staging is pending
+        c->pinned.reset();
         int rc = f();
         c->keep.clear();
         return rc;
