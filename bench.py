@@ -806,7 +806,7 @@ def wick_workload(args):
             r4 = te["transform_ms"] if factored else tb["transform_ms"]
             builds.append(tb["total_ms"] + (r4 if factored else 0.0)); transforms.append(r4); evals.append(te["eval_ms"])
             if factored:
-                stages.append(eng.rank4_stats())
+                stages.append({**eng.rank4_stats(), **eng.rank4_kernel_stats()})
             log(f"[{name}] step {k}: build {tb['total_ms']:.1f} ms, rank-4 {r4:.1f} ms, eval {te['eval_ms']:.2f} ms")
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
@@ -829,11 +829,15 @@ def wick_workload(args):
     dev_f = eng.set_eri_factors(sysm.chol)
     resf64 = phase(nwick.PREC_FP64, factored=True)
     resf64["dE"] = None if args.no_energy else resf64["energy"] - res64["energy"]
-    resfmx = None
+    resfmx = resfbf = None
     if args.precision == "mixed":
+        # FP32 pairs' block products on FP32 MFMA (mixed) and on BF16 MFMA with split operands (bf16x3)
         resfmx = phase(nwick.PREC_MIXED, factored=True)
         resfmx["dE"] = None if args.no_energy else resfmx["energy"] - res64["energy"]
         mixed_runs["factored"] = resfmx
+        resfbf = phase(nwick.PREC_BF16X3, factored=True)
+        resfbf["dE"] = None if args.no_energy else resfbf["energy"] - res64["energy"]
+        mixed_runs["factored_bf16x3"] = resfbf
     eng.set_eri_factors(None)
     # report the fastest mixed form whose energy is within 1 uHa of the dense FP64 one
     ok = [p for p, r in mixed_runs.items() if args.no_energy or abs(r["dE"]) < 1e-6]
@@ -848,7 +852,7 @@ def wick_workload(args):
     transforms, evals, builds = res["transforms"], res["evals"], res["builds"]
     # roofline of the dense four-index transforms: the fastest dense mixed
     # form within 1 uHa (or the FP64 transforms)
-    dok = [p for p in ok if p != "factored"]
+    dok = [p for p in ok if p not in ("factored", "factored_bf16x3")]
     dbest = max(dok, key=lambda p: mixed_runs[p]["rate"]) if dok else None
     dense_mixed = dbest is not None
     dres = mixed_runs[dbest] if dense_mixed else res64
@@ -949,31 +953,56 @@ def wick_workload(args):
                       "dispatches_per_step": dispatches, "tensors_per_dispatch": g, "avg_launch_ms": avg_ms,
                       "flops_per_dispatch": flops_disp})
 
-    def stage_roofs(r, mixed):
+    def stage_roofs(r, mixed, bf3=False):
         """MFMA rooflines of the on-demand rank-4 stages of a factored run:
         algorithmic flops (nw_rank4_stats) over the stage's device time.  A
         mixed run's products (factor and block) run on FP32 MFMA except those
         of pairs under the sigma floor (FP64), which the FP32 peak leaves in,
-        so its fractions are lower bounds."""
+        so its fractions are lower bounds.  With bf3 the FP32 pairs' block
+        products run on BF16 MFMA as six products of split operands: their
+        peak is the BF16 dense peak / 6 in FP32-equivalent flops."""
         st = r["stages"]
         peak_b = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
+        peak_k = PEAK_BF16_TFLOPS / 6.0 if bf3 else peak_b
+        kb = "k_r4_bf3" if bf3 else "k_r4_gemm"
         out_s = {}
         for key, ms, fl, kern, peak in (
                 ("factor", "ms_factor", "flops_factor", "k_fac_gemm (factor products X^T L_K Y)", peak_b),
-                ("rect", "ms_rect", "flops_rect", "k_r4_gemm (rectangle products)", peak_b),
-                ("single", "ms_single", "flops_single", "k_r4_gemm (single blocks)", peak_b),
-                ("exchange", "ms_exchange", "flops_exchange", "k_r4_gemm (exchange products)", peak_b)):
+                ("rect", "ms_rect", "flops_rect", kb + " (rectangle products)", peak_k),
+                ("single", "ms_single", "flops_single", kb + " (single blocks)", peak_k),
+                ("exchange", "ms_exchange", "flops_exchange", kb + " (exchange products)", peak_k)):
             if st[ms] <= 0.0:
                 continue
             ach = st[fl] / (st[ms] * 1e-3) / 1e12
             out_s[key] = {"bound": "mfma", "kernel": kern, "achieved": ach, "peak": peak, "unit": "TFLOP/s",
                           "frac": ach / peak, "ms_per_step": st[ms], "flops_per_step": st[fl]}
+        # the block products by kernel launch class (nw_rank4_kernel_stats): the
+        # FP32 pairs' large tiles are one kernel (k_r4_bf3 or k_r4_gemm<float>)
+        for key, kern, peak in (
+                ("f32_large", ("k_r4_bf3<128,128> / <64,64>" if bf3 else "k_r4_gemm<float> large tiles")
+                 + " (block products of the FP32 pairs)", peak_k),
+                ("f32_narrow", "k_r4_gemm<float,128,32> (ket-particle pair products of the FP32 pairs)",
+                 PEAK_FP32_TFLOPS),
+                ("f64_large", "k_r4_gemm<double> large tiles (block products of the FP64 pairs)", PEAK_FP64_TFLOPS),
+                ("f64_narrow", "k_r4_gemm<double,128,32> (ket-particle pair products of the FP64 pairs)",
+                 PEAK_FP64_TFLOPS)):
+            ms, fl = st["ms_" + key], st["flops_" + key]
+            if ms <= 0.0:
+                continue
+            ach = fl / (ms * 1e-3) / 1e12
+            out_s["kernel_" + key] = {"bound": "mfma", "kernel": kern, "achieved": ach, "peak": peak,
+                                      "unit": "TFLOP/s", "frac": ach / peak, "ms_per_step": ms, "flops_per_step": fl}
         return out_s
     stages_f64 = stage_roofs(resf64, False)
     stages_fmx = stage_roofs(resfmx, True) if resfmx is not None else None
-    if best == "factored":
-        dom = max(stages_fmx.items(), key=lambda kv: kv[1]["ms_per_step"])
-        headline_roof = dict(dom[1], **{"traffic": None, "stage": dom[0]})
+    stages_fbf = stage_roofs(resfbf, True, bf3=True) if resfbf is not None else None
+    if best in ("factored", "factored_bf16x3"):
+        # the dominant kernel: the FP32 pairs' large-tile block products
+        st_best = stages_fmx if best == "factored" else stages_fbf
+        kern_roofs = {k: v for k, v in st_best.items() if k.startswith("kernel_")}
+        dom = max(kern_roofs.items(), key=lambda kv: kv[1]["ms_per_step"])
+        headline_roof = dict(dom[1], **{"traffic": None, "stage": dom[0],
+                                        "profile": "profiles/r4_factored_c2_kernel_stats.txt"})
     else:
         headline_roof = roof_dense
     cpu = None
@@ -994,13 +1023,14 @@ def wick_workload(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ({nwick.PREC_BF16X3: "bf16x3+f64", nwick.PREC_MIXED: "f32+f64", "factored": "f32+f64"}[best]
+        "dtype": ({nwick.PREC_BF16X3: "bf16x3+f64", nwick.PREC_MIXED: "f32+f64", "factored": "f32+f64",
+                   "factored_bf16x3": "bf16x3+f32+f64"}[best]
                   if use_mixed else "f64"),
         "data": "synthetic",
         "config": {"workload": "batched Wick elements, n_bas=nmo=100, nocc=10+10, 8 refs (1 with an exactly "
                                "orthogonal alpha orbital), 512 singles/doubles per ref, 4096 dets, 8390656 pairs",
                    "mode": "reference" if args.mode == 0 else "corrected", "pairs_built": len(meta),
-                   "precision": ("factored mixed" if best == "factored" else PREC_NAME[best]) if use_mixed else "fp64",
+                   "precision": ({"factored": "factored mixed", "factored_bf16x3": "factored bf16x3"}.get(best) or PREC_NAME[best]) if use_mixed else "fp64",
                    "integrals": "dense ERIs; factored forms also take the workload's 300 Cholesky vectors",
                    "rank4_tensors": ntensor, "fp32_tensors": n32},
         "fp64_pairs_per_s": res64["rate"],
@@ -1021,15 +1051,20 @@ def wick_workload(args):
         "factored_mixed_over_fp64": None if resfmx is None else resfmx["rate"] / res64["rate"],
         "factored_mixed_over_factored_fp64": None if resfmx is None else resfmx["rate"] / resf64["rate"],
         "dE_factored_mixed_minus_fp64": None if resfmx is None else resfmx["dE"],
+        "factored_bf16x3_pairs_per_s": None if resfbf is None else resfbf["rate"],
+        "factored_bf16x3_over_fp64": None if resfbf is None else resfbf["rate"] / res64["rate"],
+        "dE_factored_bf16x3_minus_fp64": None if resfbf is None else resfbf["dE"],
         "factor_check_deviation": dev_f,
         "factored_rank4_ms": {"fp64": float(np.mean(resf64["transforms"])),
-                              "mixed": None if resfmx is None else float(np.mean(resfmx["transforms"]))},
+                              "mixed": None if resfmx is None else float(np.mean(resfmx["transforms"])),
+                              "bf16x3": None if resfbf is None else float(np.mean(resfbf["transforms"]))},
         "eval_only_pairs_per_s": eval_rate,
         "build_ms": float(np.mean(builds)),
         "transform_ms": float(np.mean(transforms)),
         "eval_ms": float(np.mean(evals)),
         "roofline": headline_roof,
-        "roofline_stages": {"factored_fp64": stages_f64, "factored_mixed": stages_fmx},
+        "roofline_stages": {"factored_fp64": stages_f64, "factored_mixed": stages_fmx,
+                            "factored_bf16x3": stages_fbf},
         "roofline_dense_transform": roof_dense,
         "roofline_eval": roof_eval,
         "cpu_baseline": cpu,

@@ -55,7 +55,10 @@ enum nw_mode { NW_MODE_REFERENCE = 0, NW_MODE_CORRECTED = 1 };
  * classification, co-densities, one-body and one-column intermediates and
  * every evaluation stay FP64.  NW_PREC_BF16X3: as NW_PREC_MIXED, with the
  * transforms on BF16 MFMA: every FP32 operand split into three BF16 parts,
- * six cross products accumulated in FP32 (FP32-level error). */
+ * six cross products accumulated in FP32 (FP32-level error).  With integral
+ * factors installed (nw_set_eri_factors) the same two settings choose how
+ * the rank-4 block products of pairs above the sigma floor run: FP32 MFMA
+ * (NW_PREC_MIXED) or BF16 MFMA on split FP32 factors (NW_PREC_BF16X3). */
 enum nw_precision { NW_PREC_FP64 = 0, NW_PREC_MIXED = 1, NW_PREC_BF16X3 = 2 };
 
 /* One determinant: the fields of DetState (src/scalar.rs:32-61) the Wick path
@@ -117,6 +120,15 @@ int nw_rank4_coverage(nw_ctx *ctx, int x, int w, double *frac3);
  * factor products; single blocks built}.  Zeros when nothing was built.
  * Diagnostics (measurement); no reference counterpart. */
 int nw_rank4_stats(nw_ctx *ctx, double *out10);
+
+/* The same call's block products by kernel launch class: out8 = {device ms
+ * of the FP64 pairs' large-tile products (k_r4_gemm, rectangles, single
+ * blocks and exchange products), of their narrow-tile products (ket-particle
+ * pairs), of the FP32 pairs' large-tile products (k_r4_bf3 under
+ * NW_PREC_BF16X3, else k_r4_gemm on FP32 MFMA), of their narrow-tile
+ * products; then the algorithmic flops of the same four}.  Diagnostics
+ * (measurement); no reference counterpart. */
+int nw_rank4_kernel_stats(nw_ctx *ctx, double *out8);
 
 /* Install the reference determinants (the `refs` of build_wicks_shared,
  * src/noci/wicks.rs:133).  ca, cb: nref x n_bas x nmo row-major MO

@@ -102,10 +102,11 @@ struct R4GTile {
 // Row-gathered A B^T products in BM x BN tiles: four waves in a 2 x 2
 // layout, each (BM/2) x (BN/2) of 16 x 16 MFMA tiles; K staged through LDS
 // KC at a time ([row][k], rows gathered from the factor tensors, 16-byte
-// loads), the next chunk held in registers during the products.  MFMA step
-// kk of lane group lk takes k = lk * (KC/4) + kk (the dot product does not
-// depend on the order of k), so each lane's fragments of a chunk are
-// contiguous in LDS and read 16 bytes at a time.  Factor-tensor rows are
+// loads), the next chunk held in registers during the products.  With GL
+// the loads go through global-address-space pointers, rows past the tile's
+// end and K past naux read the tensor's first entries and are replaced by
+// zeros (no branches around the loads); without, they are skipped.  Which
+// is faster depends on the tile shape (launch_r4g).  Factor-tensor rows are
 // naux values with naux a multiple of 4 (the factors are zero-padded), so
 // every row starts 16-byte aligned.  Tiles are dealt to the eight XCDs in
 // contiguous chunks, so the tiles of one product, which share rows, meet in
@@ -114,18 +115,83 @@ template <typename T>
 struct R4Vec;
 template <>
 struct R4Vec<float> {
-    typedef float4 t;
+    typedef float t __attribute__((ext_vector_type(4)));
 };
 template <>
 struct R4Vec<double> {
-    typedef double2 t;
+    typedef double t __attribute__((ext_vector_type(2)));
 };
 
-template <typename T, int BM, int BN, int KC, int OCC>
+// The tile's row and column offsets into the factor tensors (ra, rb; -1 past
+// the product's edge) and into the output (ro + co; mro + mco for the mirror
+// of mode 3), in LDS.
+template <int BM, int BN>
+__device__ inline void r4_tile_index(const R4Gemm &g, const R4GTile &tl, int nm, int naux, long long *ra,
+                                     long long *rb, long long *ro, long long *co, long long *mro, long long *mco)
+{
+    const long long nm2 = (long long)nm * nm;
+    const int mloc = min(BM, g.M - tl.m0), nloc = min(BN, g.N - tl.n0);
+    for (int t = threadIdx.x; t < BM + BN; t += blockDim.x) {
+        int u = 0, v = 0;
+        if (t < BM) {
+            const bool ok = t < mloc;
+            if (ok) axis_at(g.row, tl.m0 + t, u, v);
+            ra[t] = ok ? ((long long)u * nm + v) * naux : -1;
+            ro[t] = g.mode == 0 || g.mode == 3 ? ((long long)u * nm + v) * nm2
+                    : g.mode == 1              ? (long long)u * nm * nm2 + (long long)v * nm
+                                               : (long long)u * nm + v;
+            mro[t] = (long long)u * nm + v;
+        } else {
+            const int j = t - BM;
+            const bool ok = j < nloc;
+            if (ok) axis_at(g.col, tl.n0 + j, u, v);
+            rb[j] = ok ? ((long long)u * nm + v) * naux : -1;
+            co[j] = g.mode == 0 || g.mode == 3 ? (long long)u * nm + v
+                    : g.mode == 1              ? (long long)u * nm2 + v
+                                               : ((long long)u * nm + v) * nm2;
+            mco[j] = ((long long)u * nm + v) * nm2;
+        }
+    }
+}
+
+// Writes a wave's TM x TN 16 x 16 result tiles (register r4 of lane group lk
+// holds row L::row(lk, r4), lane & 15 the column) by the product's mode.
+template <typename T, typename L, typename Acc, int TM, int TN, int WM, int WN>
+__device__ inline void r4_store(const R4Gemm &g, const Acc (&acc)[TM][TN], int wm, int wn, int mloc, int nloc,
+                                const long long *ro, const long long *co, const long long *mro, const long long *mco)
+{
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    T *out = (T *)g.out;
+#pragma unroll
+    for (int b = 0; b < TN; ++b) {
+        const int j = wn * WN + b * 16 + lr;
+        if (j >= nloc) continue;
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+                const int i = wm * WM + a * 16 + L::row(lk, r4);
+                if (i >= mloc) continue;
+                T *o = out + ro[i] + co[j];
+                if (g.mode == 1) {
+                    *o -= acc[a][b][r4];
+                } else {
+                    *o = acc[a][b][r4];
+                    if (g.mode == 3) out[mco[j] + mro[i]] = acc[a][b][r4];
+                }
+            }
+    }
+}
+
+// MFMA step kk of lane group lk takes k = lk * (KC/4) + kk (the dot product
+// does not depend on the order of k), so each lane's fragments of a chunk
+// are contiguous in LDS and read 16 bytes at a time.
+template <typename T, int BM, int BN, int KC, int OCC, bool GL>
 __global__ __launch_bounds__(256, OCC) void k_r4_gemm(const R4Gemm *__restrict__ gs, const R4GTile *__restrict__ tiles,
                                                       int ntiles, int nm, int naux)
 {
     typedef typename R4Vec<T>::t VT;
+    typedef const T __attribute__((address_space(1))) *gptr;
     constexpr int VE = 16 / sizeof(T), KV = KC / VE, LDK = KC + VE, NT = 256;
     constexpr int PER = (BM + BN) * KV / NT;  // 16-byte vectors per thread and chunk
     static_assert((BM + BN) * KV % NT == 0 && KC % (4 * VE) == 0, "tile shape");
@@ -139,50 +205,37 @@ __global__ __launch_bounds__(256, OCC) void k_r4_gemm(const R4Gemm *__restrict__
     const int gper = gridDim.x / 8, chunk = blockIdx.x % 8, lidx = blockIdx.x / 8;
     const int per = (ntiles + 7) / 8;
     const int t0 = chunk * per, t1 = min(ntiles, t0 + per);
-    const long long nm2 = (long long)nm * nm;
     for (int it = t0 + lidx; it < t1; it += gper) {
         const R4GTile tl = tiles[it];
         const R4Gemm g = gs[tl.g];
         const int mloc = min(BM, g.M - tl.m0), nloc = min(BN, g.N - tl.n0);
         __syncthreads();  // the previous tile's indices are no longer read
-        for (int t = tid; t < BM + BN; t += NT) {
-            int u = 0, v = 0;
-            if (t < BM) {
-                const bool ok = t < mloc;
-                if (ok) axis_at(g.row, tl.m0 + t, u, v);
-                ra[t] = ok ? ((long long)u * nm + v) * naux : -1;
-                ro[t] = g.mode == 0 || g.mode == 3 ? ((long long)u * nm + v) * nm2
-                        : g.mode == 1              ? (long long)u * nm * nm2 + (long long)v * nm
-                                                   : (long long)u * nm + v;
-                mro[t] = (long long)u * nm + v;
-            } else {
-                const int j = t - BM;
-                const bool ok = j < nloc;
-                if (ok) axis_at(g.col, tl.n0 + j, u, v);
-                rb[j] = ok ? ((long long)u * nm + v) * naux : -1;
-                co[j] = g.mode == 0 || g.mode == 3 ? (long long)u * nm + v
-                        : g.mode == 1              ? (long long)u * nm2 + v
-                                                   : ((long long)u * nm + v) * nm2;
-                mco[j] = ((long long)u * nm + v) * nm2;
-            }
-        }
+        r4_tile_index<BM, BN>(g, tl, nm, naux, ra, rb, ro, co, mro, mco);
         __syncthreads();
-        const T *__restrict__ A = (const T *)g.A;
-        const T *__restrict__ B = (const T *)g.B;
+        const T *A = (const T *)g.A;
+        const T *B = (const T *)g.B;
         // each thread loads the same rows and k offsets in every chunk
-        const T *src[PER];
+        gptr src[PER];
+        bool okr[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const int e = tid + j * NT, r = e / KV, kv = e % KV;
             const long long b = r < BM ? ra[r] : rb[r - BM];
-            src[j] = b < 0 ? nullptr : (r < BM ? A : B) + b + kv * VE;
+            okr[j] = b >= 0;
+            src[j] = (gptr)((r < BM ? A : B) + (b < 0 ? 0 : b) + kv * VE);
         }
         VT buf[PER];
         auto gload = [&](int k0) {
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
                 const int kv = (tid + j * NT) % KV;
-                buf[j] = src[j] && k0 + kv * VE < naux ? *reinterpret_cast<const VT *>(src[j] + k0) : VT{};
+                const bool ok = okr[j] && k0 + kv * VE < naux;
+                if constexpr (GL) {
+                    const VT v = *reinterpret_cast<const VT __attribute__((address_space(1))) *>(src[j] + (ok ? k0 : 0));
+                    buf[j] = ok ? v : VT{};
+                } else {
+                    buf[j] = ok ? *reinterpret_cast<const VT *>((const T *)src[j] + k0) : VT{};
+                }
             }
         };
         typename M::acc acc[TM][TN];
@@ -215,31 +268,127 @@ __global__ __launch_bounds__(256, OCC) void k_r4_gemm(const R4Gemm *__restrict__
 #pragma unroll
                     for (int a = 0; a < TM; ++a)
 #pragma unroll
-                        for (int b = 0; b < TN; ++b)
-                            acc[a][b] = M::mma(reinterpret_cast<const T *>(&af[a])[e],
-                                               reinterpret_cast<const T *>(&bf[b])[e], acc[a][b]);
+                        for (int b = 0; b < TN; ++b) acc[a][b] = M::mma(af[a][e], bf[b][e], acc[a][b]);
             }
         }
-        T *out = (T *)g.out;
+        r4_store<T, M, typename M::acc, TM, TN, WM, WN>(g, acc, wm, wn, mloc, nloc, ro, co, mro, mco);
+    }
+}
+
+// The same FP32 products on BF16 MFMA: each FP32 operand v is split into
+// three BF16 parts h + m + l (h = bf16(v), m = bf16(v - h), l = bf16(v - h -
+// m), round to nearest even), which hold its 24-bit significand, as the
+// chunk is written to LDS (three planes [row][k]); a chunk of K = 32 is one
+// v_mfma_f32_16x16x32_bf16 per pair of parts, and the six pairs whose
+// product is above 2^-24 of h_a h_b (hh, hm, mh, hl, lh, mm) accumulate in
+// FP32, smallest first.  The dropped ml, lm and ll terms are below FP32
+// rounding, so the result carries FP32-level error.  Lane group lk holds
+// k = 8 lk .. 8 lk + 7 of the chunk (one 16-byte LDS read per plane).
+template <int BM, int BN, int OCC, bool GL>
+__global__ __launch_bounds__(256, OCC) void k_r4_bf3(const R4Gemm *__restrict__ gs, const R4GTile *__restrict__ tiles,
+                                                     int ntiles, int nm, int naux)
+{
+    typedef float gf4 __attribute__((ext_vector_type(4)));
+    typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+    typedef const float __attribute__((address_space(1))) *gptr;
+    constexpr int KC = 32, VE = 4, KV = KC / VE, LDK = KC + 8, NT = 256, ROWS = BM + BN;
+    constexpr int PER = ROWS * KV / NT;
+    static_assert(ROWS * KV % NT == 0, "tile shape");
+    constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+    __shared__ __attribute__((aligned(16))) __bf16 Xs[3][ROWS][LDK];  // planes h, m, l; rows A then B
+    __shared__ long long ra[BM], rb[BN], ro[BM], co[BN], mro[BM], mco[BN];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int gper = gridDim.x / 8, chunk = blockIdx.x % 8, lidx = blockIdx.x / 8;
+    const int per = (ntiles + 7) / 8;
+    const int t0 = chunk * per, t1 = min(ntiles, t0 + per);
+    for (int it = t0 + lidx; it < t1; it += gper) {
+        const R4GTile tl = tiles[it];
+        const R4Gemm g = gs[tl.g];
+        const int mloc = min(BM, g.M - tl.m0), nloc = min(BN, g.N - tl.n0);
+        __syncthreads();
+        r4_tile_index<BM, BN>(g, tl, nm, naux, ra, rb, ro, co, mro, mco);
+        __syncthreads();
+        const float *A = (const float *)g.A;
+        const float *B = (const float *)g.B;
+        gptr src[PER];
+        bool okr[PER];
 #pragma unroll
-        for (int b = 0; b < TN; ++b) {
-            const int j = wn * WN + b * 16 + lr;
-            if (j >= nloc) continue;
-#pragma unroll
-            for (int a = 0; a < TM; ++a)
-#pragma unroll
-                for (int r4 = 0; r4 < 4; ++r4) {
-                    const int i = wm * WM + a * 16 + M::row(lk, r4);
-                    if (i >= mloc) continue;
-                    T *o = out + ro[i] + co[j];
-                    if (g.mode == 1) {
-                        *o -= acc[a][b][r4];
-                    } else {
-                        *o = acc[a][b][r4];
-                        if (g.mode == 3) out[mco[j] + mro[i]] = acc[a][b][r4];
-                    }
-                }
+        for (int j = 0; j < PER; ++j) {
+            const int e = tid + j * NT, r = e / KV, kv = e % KV;
+            const long long b = r < BM ? ra[r] : rb[r - BM];
+            okr[j] = b >= 0;
+            src[j] = (gptr)((r < BM ? A : B) + (b < 0 ? 0 : b) + kv * VE);
         }
+        gf4 buf[PER];
+        auto gload = [&](int k0) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int kv = (tid + j * NT) % KV;
+                const bool ok = okr[j] && k0 + kv * VE < naux;
+                if constexpr (GL) {
+                    const gf4 v = *reinterpret_cast<const gf4 __attribute__((address_space(1))) *>(src[j] + (ok ? k0 : 0));
+                    buf[j] = ok ? v : gf4{0, 0, 0, 0};
+                } else {
+                    buf[j] = ok ? *reinterpret_cast<const gf4 *>((const float *)src[j] + k0) : gf4{0, 0, 0, 0};
+                }
+            }
+        };
+        r4_f4 acc[TM][TN];
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b) acc[a][b] = r4_f4{0, 0, 0, 0};
+        gload(0);
+        for (int k0 = 0; k0 < naux; k0 += KC) {
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int e = tid + j * NT, r = e / KV, kv = e % KV;
+                bf4 h, m, l;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float x = buf[j][i];
+                    const __bf16 hh = (__bf16)x;
+                    const float rem = x - (float)hh;
+                    const __bf16 mm = (__bf16)rem;
+                    h[i] = hh;
+                    m[i] = mm;
+                    l[i] = (__bf16)(rem - (float)mm);
+                }
+                *reinterpret_cast<bf4 *>(&Xs[0][r][kv * VE]) = h;
+                *reinterpret_cast<bf4 *>(&Xs[1][r][kv * VE]) = m;
+                *reinterpret_cast<bf4 *>(&Xs[2][r][kv * VE]) = l;
+            }
+            __syncthreads();
+            if (k0 + KC < naux) gload(k0 + KC);
+            bf8 ah[TM], am[TM], al[TM];
+#pragma unroll
+            for (int a = 0; a < TM; ++a) {
+                const int r = wm * WM + a * 16 + lr;
+                ah[a] = *reinterpret_cast<const bf8 *>(&Xs[0][r][lk * 8]);
+                am[a] = *reinterpret_cast<const bf8 *>(&Xs[1][r][lk * 8]);
+                al[a] = *reinterpret_cast<const bf8 *>(&Xs[2][r][lk * 8]);
+            }
+#pragma unroll
+            for (int b = 0; b < TN; ++b) {
+                const int r = BM + wn * WN + b * 16 + lr;
+                const bf8 bh = *reinterpret_cast<const bf8 *>(&Xs[0][r][lk * 8]);
+                const bf8 bm = *reinterpret_cast<const bf8 *>(&Xs[1][r][lk * 8]);
+                const bf8 bl = *reinterpret_cast<const bf8 *>(&Xs[2][r][lk * 8]);
+#pragma unroll
+                for (int a = 0; a < TM; ++a) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[a], bm, acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[a], bh, acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bm, acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
+                }
+            }
+        }
+        r4_store<float, R4Mfma<float>, r4_f4, TM, TN, WM, WN>(g, acc, wm, wn, mloc, nloc, ro, co, mro, mco);
     }
 }
 
@@ -418,8 +567,12 @@ struct CholState {
     // last chol_ensure: device ms of the factor products, rectangle products,
     // single blocks and exchange products; their algorithmic flops; single blocks
     double stats[10] = {};
+    // the block products by launch class (FP64 large tiles, FP64 narrow
+    // tiles, FP32 pairs' large tiles (FP32 or BF16 MFMA), FP32 pairs' narrow
+    // tiles): device ms, then algorithmic flops
+    double kstats[8] = {};
     std::vector<hipEvent_t> evs;  // stage boundaries of the last build_blocks
-    std::vector<int> ev_stage;
+    std::vector<int> ev_stage;    // stage, plus 16 (launch class + 1) for block products
     bool pending = false;  // stage times not yet read from the events
     hipEvent_t mark(int stage)
     {
@@ -519,46 +672,81 @@ size_t r4_tiles(const std::vector<R4Gemm> &gs, int bm, int bn)
     return n;
 }
 
-template <typename T, int BM, int BN, int KC, int OCC>
-void launch_r4_shape(nw_ctx *c, Uploader &up, const R4Gemm *dg, const std::vector<R4Gemm> &gs, int naux)
+std::vector<R4GTile> r4_tile_list(const std::vector<R4Gemm> &gs, int bm, int bn)
 {
     std::vector<R4GTile> tl;
-    tl.reserve(r4_tiles(gs, BM, BN));
+    tl.reserve(r4_tiles(gs, bm, bn));
     for (size_t i = 0; i < gs.size(); ++i)
-        for (int m0 = 0; m0 < gs[i].M; m0 += BM)
-            for (int n0 = 0; n0 < gs[i].N; n0 += BN) tl.push_back({(int)i, m0, n0, 0});
+        for (int m0 = 0; m0 < gs[i].M; m0 += bm)
+            for (int n0 = 0; n0 < gs[i].N; n0 += bn) tl.push_back({(int)i, m0, n0, 0});
+    return tl;
+}
+
+template <typename T, int BM, int BN, int KC, int OCC, bool GL>
+void launch_r4_shape(nw_ctx *c, Uploader &up, const R4Gemm *dg, const std::vector<R4Gemm> &gs, int naux)
+{
+    const std::vector<R4GTile> tl = r4_tile_list(gs, BM, BN);
     if (tl.empty()) return;
     const R4GTile *dt = up.put(tl);
-    hipLaunchKernelGGL((k_r4_gemm<T, BM, BN, KC, OCC>), dim3(r4_grid((int)tl.size())), dim3(256), 0, c->st, dg, dt,
+    hipLaunchKernelGGL((k_r4_gemm<T, BM, BN, KC, OCC, GL>), dim3(r4_grid((int)tl.size())), dim3(256), 0, c->st, dg, dt,
                        (int)tl.size(), c->nmo, naux);
 }
+
+template <int BM, int BN, int OCC, bool GL>
+void launch_r4_bf3_shape(nw_ctx *c, Uploader &up, const R4Gemm *dg, const std::vector<R4Gemm> &gs, int naux)
+{
+    const std::vector<R4GTile> tl = r4_tile_list(gs, BM, BN);
+    if (tl.empty()) return;
+    const R4GTile *dt = up.put(tl);
+    hipLaunchKernelGGL((k_r4_bf3<BM, BN, OCC, GL>), dim3(r4_grid((int)tl.size())), dim3(256), 0, c->st, dg, dt,
+                       (int)tl.size(), c->nmo, naux);
+}
+
+// FP32 pairs of an NW_PREC_BF16X3 context take their block products on BF16
+// MFMA (k_r4_bf3), those of NW_PREC_MIXED on FP32 MFMA.
+bool r4_bf3(const nw_ctx *c) { return c->precision == NW_PREC_BF16X3; }
 
 // One launch over a list of products: 128 x 128 output tiles in FP32 and
 // 128 x 64 in FP64 (measured 1.3-1.6x the rate of 64 x 64 tiles on the
 // n_bas = 100 products) when there are enough of them to fill the chip
-// (four per CU), else 64 x 64 tiles.
+// (four per CU), else 64 x 64 tiles.  The BF16 form runs 128 x 128 tiles at
+// two workgroups per CU (72 KiB of LDS each): 4.95 ms against 6.4 ms of
+// FP32 MFMA for the C2 step's products.  These tile shapes load through
+// generic pointers (GL = false): global-address loads measured slower on
+// them (FP64 128 x 64: 16.0 against 12.7 ms per step), faster or equal on
+// the narrow tiles.
 template <typename T>
 void launch_r4g(nw_ctx *c, Uploader &up, const std::vector<R4Gemm> &gs, int naux)
 {
     if (gs.empty()) return;
     const R4Gemm *dg = up.put(gs);
     constexpr bool f32 = sizeof(T) == 4;
-    if (r4_tiles(gs, 128, f32 ? 128 : 64) >= (size_t)4 * r4_ncu()) {
-        if constexpr (f32) launch_r4_shape<float, 128, 128, 32, 1>(c, up, dg, gs, naux);
-        else launch_r4_shape<double, 128, 64, 16, 2>(c, up, dg, gs, naux);
+    const bool big = r4_tiles(gs, 128, f32 ? 128 : 64) >= (size_t)4 * r4_ncu();
+    if constexpr (f32) {
+        if (r4_bf3(c)) {
+            if (big) launch_r4_bf3_shape<128, 128, 2, false>(c, up, dg, gs, naux);
+            else launch_r4_bf3_shape<64, 64, 2, false>(c, up, dg, gs, naux);
+            return;
+        }
+    }
+    if (big) {
+        if constexpr (f32) launch_r4_shape<float, 128, 128, 32, 1, false>(c, up, dg, gs, naux);
+        else launch_r4_shape<double, 128, 64, 16, 2, false>(c, up, dg, gs, naux);
     } else {
-        launch_r4_shape<T, 64, 64, 32, 2>(c, up, dg, gs, naux);
+        launch_r4_shape<T, 64, 64, 32, 2, false>(c, up, dg, gs, naux);
     }
 }
 
 // Products with a narrow side (at most a few dozen columns: ket-particle
-// pairs) in 128 x 32 tiles.
+// pairs) in 128 x 32 tiles, on FP32 (or FP64) MFMA in either reduced
+// precision: the BF16 form measured no faster on them (1.60 ms against
+// 1.2-1.6 ms per C2 step).
 void launch_r4g_narrow(nw_ctx *c, Uploader &up, const std::vector<R4Gemm> &gs, int naux, bool f32)
 {
     if (gs.empty()) return;
     const R4Gemm *dg = up.put(gs);
-    if (f32) launch_r4_shape<float, 128, 32, 32, 2>(c, up, dg, gs, naux);
-    else launch_r4_shape<double, 128, 32, 16, 2>(c, up, dg, gs, naux);
+    if (f32) launch_r4_shape<float, 128, 32, 32, 2, true>(c, up, dg, gs, naux);
+    else launch_r4_shape<double, 128, 32, 16, 2, true>(c, up, dg, gs, naux);
 }
 
 // Row index sets of one parent's determinants per spin, as bra (rows
@@ -1233,6 +1421,9 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
         for (const R4Gemm &g : gx[f]) s->stats[7] += 2.0 * g.M * (double)g.N * s->naux_in;
         for (const R4Gemm &g : gs1[f]) s->stats[6] += 2.0 * g.M * (double)g.N * s->naux_in;
         s->stats[9] += nsingle[f];
+        for (auto *v : {&gd[f], &gx[f], &gs1[f]})
+            for (const R4Gemm &g : *v) s->kstats[4 + 2 * f] += 2.0 * g.M * (double)g.N * s->naux_in;
+        for (const R4Gemm &g : gk[f]) s->kstats[5 + 2 * f] += 2.0 * g.M * (double)g.N * s->naux_in;
     }
     for (int f32 = 0; f32 < 2; ++f32) {
         // the exchange products run after the direct products of the same blocks (stream order)
@@ -1240,7 +1431,8 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
         std::vector<R4Gemm> *lists[4] = {&gd[f32], &gk[f32], &gs1[f32], &gx[f32]};
         for (int k = 0; k < 4; ++k) {
             if (lists[k]->empty()) continue;
-            if (k != 1) HIPCHK(hipEventRecord(s->mark(stage[k]), c->st));
+            const int cls = 2 * f32 + (k == 1 ? 1 : 0);
+            HIPCHK(hipEventRecord(s->mark(stage[k] + 16 * (cls + 1)), c->st));
             if (k == 1) launch_r4g_narrow(c, up, *lists[k], naux, f32 != 0);
             else if (f32) launch_r4g<float>(c, up, *lists[k], naux);
             else launch_r4g<double>(c, up, *lists[k], naux);
@@ -1284,8 +1476,10 @@ void chol_ensure(nw_ctx *c, const std::vector<PackedDet> &h, bool accumulate, co
     CholState *s = c->chol;
     if (!s) return;
     if (accumulate) chol_read_stages(s);
-    else
+    else {
         for (double &v : s->stats) v = 0.0;
+        for (double &v : s->kstats) v = 0.0;
+    }
     s->ms_blocks = 0.0;
     s->ev_stage.clear();
     s->pending = false;
@@ -1317,8 +1511,13 @@ void chol_read_stages(CholState *s)
     s->pending = false;
     HIPCHK(hipEventSynchronize(s->evs[s->ev_stage.size() - 1]));
     // stage k runs from its mark to the next mark
-    for (size_t k = 0; k + 1 < s->ev_stage.size(); ++k)
-        if (s->ev_stage[k] != 0) s->stats[s->ev_stage[k]] += elapsed(s->evs[k], s->evs[k + 1]);
+    for (size_t k = 0; k + 1 < s->ev_stage.size(); ++k) {
+        const int st = s->ev_stage[k] & 15, cls = (s->ev_stage[k] >> 4) - 1;
+        if (st == 0) continue;
+        const double ms = elapsed(s->evs[k], s->evs[k + 1]);
+        s->stats[st] += ms;
+        if (cls >= 0) s->kstats[cls] += ms;
+    }
     s->ms_blocks = elapsed(s->evs[0], s->evs[s->ev_stage.size() - 1]);
     s->stats[0] += s->ms_blocks;
 }
@@ -1449,6 +1648,16 @@ int nw_rank4_stats(nw_ctx *c, double *out10)
         if (!out10) return fail(c, NW_E_ARG, "null output");
         if (c->chol) chol_read_stages(c->chol);
         for (int k = 0; k < 10; ++k) out10[k] = c->chol ? c->chol->stats[k] : 0.0;
+        return NW_OK;
+    });
+}
+
+int nw_rank4_kernel_stats(nw_ctx *c, double *out8)
+{
+    return guarded(c, [&]() -> int {
+        if (!out8) return fail(c, NW_E_ARG, "null output");
+        if (c->chol) chol_read_stages(c->chol);
+        for (int k = 0; k < 8; ++k) out8[k] = c->chol ? c->chol->kstats[k] : 0.0;
         return NW_OK;
     });
 }

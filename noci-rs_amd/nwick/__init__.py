@@ -38,7 +38,8 @@ EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matri
            "nw_mlp_destroy", "nw_factor_create", "nw_factor_refresh", "nw_factor_apply", "nw_factor_diagonals",
            "nw_factor_info", "nw_factor_ids", "nw_factor_table", "nw_factor_last_ms", "nw_factor_destroy",
            "nw_factor_set_precision", "nw_factor_apply_pair", "nw_factor_apply_bytes", "nw_factor_block_groups", "nw_hs_batched_noalloc", "nw_s_same_batched", "nw_m_batched",
-           "nw_set_eri_factors", "nw_rank4_coverage", "nw_rank4_stats"]
+           "nw_set_eri_factors", "nw_rank4_coverage", "nw_rank4_stats",
+           "nw_rank4_kernel_stats"]
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_BF16X3 = 2
@@ -96,6 +97,7 @@ def load():
     L.nw_set_eri_factors.argtypes = [vp, vp, ct.c_int, ct.c_double, dp]
     L.nw_rank4_coverage.argtypes = [vp, ct.c_int, ct.c_int, dp]
     L.nw_rank4_stats.argtypes = [vp, dp]
+    L.nw_rank4_kernel_stats.argtypes = [vp, dp]
     L.nw_project_hs.argtypes = [vp, vp, ct.c_uint64, ct.c_uint64, vp, vp, vp, ct.c_int]
     L.nw_noci_fock.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_double, vp, vp, vp, vp]
     L.nw_save_wicks.argtypes = [vp, ct.c_char_p]
@@ -236,6 +238,14 @@ class Engine:
         self._check(self.L.nw_rank4_stats(self.ctx, out))
         keys = ("ms_total", "ms_factor", "ms_rect", "ms_single", "ms_exchange", "flops_rect", "flops_single",
                 "flops_exchange", "flops_factor", "single_blocks")
+        return dict(zip(keys, out))
+
+    def rank4_kernel_stats(self):
+        """The same call's block products by launch class (FP64 / FP32 pairs, large / narrow tiles): ms and flops."""
+        out = (ct.c_double * 8)()
+        self._check(self.L.nw_rank4_kernel_stats(self.ctx, out))
+        keys = ("ms_f64_large", "ms_f64_narrow", "ms_f32_large", "ms_f32_narrow",
+                "flops_f64_large", "flops_f64_narrow", "flops_f32_large", "flops_f32_narrow")
         return dict(zip(keys, out))
 
     def set_precision(self, prec):

@@ -108,6 +108,14 @@ def test_batched_project_and_streamed(small):
     eng.build()
     eng.hs_matrix(rec)
     whole = eng.rank4_stats()
+    # the block products by launch class partition the rectangle, single and
+    # exchange stages, in time and in flops
+    ks = eng.rank4_kernel_stats()
+    kms = ks["ms_f64_large"] + ks["ms_f64_narrow"] + ks["ms_f32_large"] + ks["ms_f32_narrow"]
+    kfl = ks["flops_f64_large"] + ks["flops_f64_narrow"] + ks["flops_f32_large"] + ks["flops_f32_narrow"]
+    assert kms == pytest.approx(whole["ms_rect"] + whole["ms_single"] + whole["ms_exchange"], rel=1e-9)
+    assert kfl == pytest.approx(whole["flops_rect"] + whole["flops_single"] + whole["flops_exchange"], rel=1e-12)
+    assert ks["flops_f64_large"] > 0.0 and ks["flops_f32_large"] == 0.0  # an FP64 context
     eng.close()
     eng = _pair(sysm)
     H, S, ng = eng.hs_matrix_streamed(rec, slab_bytes=5 << 19)
@@ -138,10 +146,13 @@ def test_export_completes_every_block(small):
     eng.close()
 
 
-def test_mixed_precision_factored(small):
-    """FP32 products of FP32-rounded factors for pairs above the sigma floor:
-    the error against FP64 is of the size of the dense FP32 transforms', at
-    the default floor and with every pair forced to FP32 (floor 0)."""
+@pytest.mark.parametrize("prec", [nwick.PREC_MIXED, nwick.PREC_BF16X3])
+def test_mixed_precision_factored(small, prec):
+    """FP32 products (NW_PREC_MIXED: FP32 MFMA; NW_PREC_BF16X3: BF16 MFMA on
+    three-way split operands) of FP32-rounded factors for pairs above the
+    sigma floor: the error against FP64 is of the size of the dense reduced-
+    precision transforms' of the same precision, at the default floor and with
+    every pair forced to FP32 (floor 0)."""
     sysm, dets, dense = small
     rec = basis.to_records(dets)
     Hd, Sd = dense.hs_matrix(rec)
@@ -149,7 +160,7 @@ def test_mixed_precision_factored(small):
     for floor in (0.1, 0.0):
         errs = {}
         for factors in (False, True):
-            eng = _pair(sysm, prec=nwick.PREC_MIXED, factors=factors)
+            eng = _pair(sysm, prec=prec, factors=factors)
             eng.set_mixed_floor(floor)
             meta = eng.build()
             n32 = sum(m.fp32 for m in meta.values())
@@ -157,8 +168,8 @@ def test_mixed_precision_factored(small):
             assert np.max(np.abs(S - Sd)) < TOL
             errs[factors] = np.max(np.abs(H - Hd)) / scale
             eng.close()
-        print(f"floor {floor}: {n32} FP32 pairs; max rel |dH| dense FP32 {errs[False]:.2e}, "
-              f"factored FP32 {errs[True]:.2e}", flush=True)
+        print(f"prec {prec} floor {floor}: {n32} FP32 pairs; max rel |dH| dense {errs[False]:.2e}, "
+              f"factored {errs[True]:.2e}", flush=True)
         assert n32 > 0
         assert errs[True] < max(10 * errs[False], 1e-9)
 
@@ -210,11 +221,12 @@ def n100():
     return sysm, dets
 
 
-@pytest.mark.parametrize("prec", [nwick.PREC_FP64, nwick.PREC_MIXED])
+@pytest.mark.parametrize("prec", [nwick.PREC_FP64, nwick.PREC_MIXED, nwick.PREC_BF16X3])
 def test_n100_factored_vs_dense(n100, prec):
     """The headline shape (n_bas = 100, 10+10 electrons, 300 factors): FP64
-    factored = dense to 1e-10; FP32 factored products against the FP64 dense
-    build as the FP32 transforms are (relative 1e-6)."""
+    factored = dense to 1e-10; FP32 factored products (FP32 MFMA, or BF16 MFMA
+    on split operands) against the FP64 dense build as the FP32 transforms are
+    (relative 1e-6)."""
     sysm, dets = n100
     rec = basis.to_records(dets)
     dense = nwick.Engine.from_system(sysm)
