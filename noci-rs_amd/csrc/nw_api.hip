@@ -343,10 +343,20 @@ int build_impl(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *me
             launch_lincomb(c->st, up.put(ly), (int)ly.size(), (long long)nm2);
         }
         // ---- J - K and J for every co-density (build.rs:179-191, 812-820) ----
+        // one launch for both products, K split when they have too few
+        // output tiles to fill the device
         {
-            std::vector<GemmDesc> g1 = {{M, c->d_easym, JK}}, g2 = {{M, c->d_eri, Jc}};
-            gemm(c, up, 0, 1, 4 * P, (int)nn, (int)nn, (int)nn, (int)nn, (int)nn, 1.0, 0.0, g1);
-            gemm(c, up, 0, 1, 4 * P, (int)nn, (int)nn, (int)nn, (int)nn, (int)nn, 1.0, 0.0, g2);
+            const std::vector<GemmDesc> g = {{M, c->d_easym, JK}, {M, c->d_eri, Jc}};
+            const int rows = 4 * P, nni = (int)nn;
+            int ncu = 0;
+            HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+            const int ks = nni >= 256 ? gemm_nt_splitk_count(rows, nni, nni, 2, std::max(1, ncu)) : 1;
+            if (ks > 1) {
+                double *part = c->splitk.get<double>((size_t)2 * ks * rows * nn);
+                launch_gemm_nt_splitk(c->st, rows, nni, nni, nni, nni, nni, up.put(g), 2, ks, part);
+            } else {
+                gemm(c, up, 0, 1, rows, nni, nni, nni, nni, nni, 1.0, 0.0, g);
+            }
         }
         // ---- scalar intermediates --------------------------------------------
         {

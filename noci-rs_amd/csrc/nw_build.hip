@@ -1108,15 +1108,22 @@ void launch_qtransform(hipStream_t st, const double *X, const double *c, double 
 // C = A B^T on FP64 MFMA (A: M x K, B: N x K, both K-contiguous), the shape
 // of the J - K and J contractions of every co-density with the n^2 x n^2
 // integral matrices.  A workgroup owns a 48 x 64 tile of C (4 waves, each 16
-// columns x 3 row blocks of 16), K staged through LDS 32 at a time.
+// columns x 3 row blocks of 16), K staged through LDS 32 at a time.  With
+// ksplit > 1 workgroup z = (product, split s) sums K in [s kc, (s + 1) kc)
+// only and writes its partial tile to part[(product ksplit + s)][M][N]
+// (k_splitk_reduce adds the splits in order: the result does not depend on
+// scheduling): at n_bas = 100 the products are 144 x 10^4 with K = 10^4, 471
+// tiles of 313 chunks each, too few workgroups for 256 CUs.
 __global__ __launch_bounds__(256) void k_gemm_nt_mfma(const GemmDesc *__restrict__ descs, int M, int N, int K, int lda,
-                                                      int ldb, int ldc)
+                                                      int ldb, int ldc, int ksplit, int kc, double *__restrict__ part)
 {
     constexpr int MT = 48, NTL = 64, BK = 32, LD = BK + 2;
     constexpr int PA = MT * BK / 2 / 256, PB = NTL * BK / 2 / 256;  // double2 per thread per chunk
     __shared__ double As[MT][LD];
     __shared__ double Bs[NTL][LD];
-    const GemmDesc g = descs[blockIdx.z];
+    const int zb = blockIdx.z / ksplit, sk = blockIdx.z % ksplit;
+    const GemmDesc g = descs[zb];
+    const int kbeg = sk * kc, kend = min(K, kbeg + kc);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
     // row blocks of M vary fastest over the grid: the workgroups sharing a B tile run together
     const int m0 = blockIdx.x * MT, n0 = blockIdx.y * NTL;
@@ -1129,10 +1136,10 @@ __global__ __launch_bounds__(256) void k_gemm_nt_mfma(const GemmDesc *__restrict
             double2 v{0.0, 0.0};
             if (gi < M) {
                 const double *src = g.A + (size_t)gi * lda + gk;
-                if (vec && gk + 1 < K) v = *reinterpret_cast<const double2 *>(src);
+                if (vec && gk + 1 < kend) v = *reinterpret_cast<const double2 *>(src);
                 else {
-                    if (gk < K) v.x = src[0];
-                    if (gk + 1 < K) v.y = src[1];
+                    if (gk < kend) v.x = src[0];
+                    if (gk + 1 < kend) v.y = src[1];
                 }
             }
             ra[j] = v;
@@ -1143,10 +1150,10 @@ __global__ __launch_bounds__(256) void k_gemm_nt_mfma(const GemmDesc *__restrict
             double2 v{0.0, 0.0};
             if (gj < N) {
                 const double *src = g.B + (size_t)gj * ldb + gk;
-                if (vec && gk + 1 < K) v = *reinterpret_cast<const double2 *>(src);
+                if (vec && gk + 1 < kend) v = *reinterpret_cast<const double2 *>(src);
                 else {
-                    if (gk < K) v.x = src[0];
-                    if (gk + 1 < K) v.y = src[1];
+                    if (gk < kend) v.x = src[0];
+                    if (gk + 1 < kend) v.y = src[1];
                 }
             }
             rb[j] = v;
@@ -1155,8 +1162,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt_mfma(const GemmDesc *__restrict
     nw_d4 acc[3];
 #pragma unroll
     for (int ms = 0; ms < 3; ++ms) acc[ms] = nw_d4{0.0, 0.0, 0.0, 0.0};
-    fetch(0);
-    for (int k0 = 0; k0 < K; k0 += BK) {
+    fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < PA; ++j) {
@@ -1171,7 +1178,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt_mfma(const GemmDesc *__restrict
             Bs[r][k + 1] = rb[j].y;
         }
         __syncthreads();
-        if (k0 + BK < K) fetch(k0 + BK);  // next chunk in flight during the products
+        if (k0 + BK < kend) fetch(k0 + BK);  // next chunk in flight during the products
 #pragma unroll
         for (int kk = 0; kk < BK / 4; ++kk) {
             const double b = Bs[wave * 16 + lr][kk * 4 + lk];
@@ -1182,13 +1189,30 @@ __global__ __launch_bounds__(256) void k_gemm_nt_mfma(const GemmDesc *__restrict
     }
     const int col = n0 + wave * 16 + lr;
     if (col >= N) return;
+    double *dst = part ? part + (size_t)blockIdx.z * M * N : g.C;
+    const size_t ld = part ? (size_t)N : (size_t)ldc;
 #pragma unroll
     for (int ms = 0; ms < 3; ++ms)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = m0 + ms * 16 + 4 * r + lk;
-            if (row < M) g.C[(size_t)row * ldc + col] = acc[ms][r];
+            if (row < M) dst[(size_t)row * ld + col] = acc[ms][r];
         }
+}
+
+// C[z] = sum over s = 0 .. ksplit-1, in order, of part[(z ksplit + s)].
+__global__ void k_splitk_reduce(const GemmDesc *__restrict__ descs, const double *__restrict__ part, int M, int N,
+                                int ldc, int ksplit)
+{
+    const int z = blockIdx.y;
+    double *C = descs[z].C;
+    const size_t mn = (size_t)M * N;
+    const double *p = part + (size_t)z * ksplit * mn;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < mn; e += (size_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int sk = 0; sk < ksplit; ++sk) v += p[sk * mn + e];
+        C[(e / N) * ldc + e % N] = v;
+    }
 }
 
 void launch_gemm_nt_mfma(hipStream_t st, int M, int N, int K, int lda, int ldb, int ldc, const GemmDesc *d_descs,
@@ -1198,8 +1222,29 @@ void launch_gemm_nt_mfma(hipStream_t st, int M, int N, int K, int lda, int ldb, 
     for (int z0 = 0; z0 < nbatch; z0 += 65535) {
         const int chunk = nbatch - z0 > 65535 ? 65535 : nbatch - z0;
         dim3 grid((M + 47) / 48, (N + 63) / 64, chunk);
-        hipLaunchKernelGGL(k_gemm_nt_mfma, grid, dim3(256), 0, st, d_descs + z0, M, N, K, lda, ldb, ldc);
+        hipLaunchKernelGGL(k_gemm_nt_mfma, grid, dim3(256), 0, st, d_descs + z0, M, N, K, lda, ldb, ldc, 1, K,
+                           (double *)nullptr);
     }
+}
+
+int gemm_nt_splitk_count(int M, int N, int K, int nbatch, int ncu)
+{
+    const long long tiles = (long long)((M + 47) / 48) * ((N + 63) / 64) * nbatch;
+    int ks = 1;
+    while (ks < 8 && tiles * ks < 8ll * ncu && K / (2 * ks) >= 512) ks *= 2;
+    return (long long)nbatch * ks <= 65535 ? ks : 1;
+}
+
+void launch_gemm_nt_splitk(hipStream_t st, int M, int N, int K, int lda, int ldb, int ldc, const GemmDesc *d_descs,
+                           int nbatch, int ksplit, double *part)
+{
+    if (nbatch <= 0 || M <= 0 || N <= 0) return;
+    const int kc = ((K + ksplit - 1) / ksplit + 31) / 32 * 32;
+    dim3 grid((M + 47) / 48, (N + 63) / 64, (unsigned)(nbatch * ksplit));
+    hipLaunchKernelGGL(k_gemm_nt_mfma, grid, dim3(256), 0, st, d_descs, M, N, K, lda, ldb, ldc, ksplit, kc, part);
+    const size_t mn = (size_t)M * N;
+    dim3 rgrid((unsigned)std::min<size_t>(2048, (mn + 255) / 256), (unsigned)nbatch);
+    hipLaunchKernelGGL(k_splitk_reduce, rgrid, dim3(256), 0, st, d_descs, part, M, N, ldc, ksplit);
 }
 
 void launch_gemm(hipStream_t st, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc,
@@ -1209,11 +1254,7 @@ void launch_gemm(hipStream_t st, int ta, int tb, int M, int N, int K, int lda, i
     if (ta == 0 && tb == 1 && alpha == 1.0 && beta == 0.0 && K >= 256 && (long long)N * K >= (1ll << 20) &&
         (N + 63) / 64 <= 65535) {
         // the large NT products (co-densities x integral matrices) on MFMA
-        for (int z0 = 0; z0 < nbatch; z0 += 65535) {
-            const int chunk = nbatch - z0 > 65535 ? 65535 : nbatch - z0;
-            dim3 grid((M + 47) / 48, (N + 63) / 64, chunk);
-            hipLaunchKernelGGL(k_gemm_nt_mfma, grid, dim3(256), 0, st, d_descs + z0, M, N, K, lda, ldb, ldc);
-        }
+        launch_gemm_nt_mfma(st, M, N, K, lda, ldb, ldc, d_descs, nbatch);
         return;
     }
     int nz = nbatch;

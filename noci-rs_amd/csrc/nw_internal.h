@@ -153,6 +153,7 @@ struct nw_ctx {
     // scratch
     nwi::DBuf buildscratch, arena_desc, scratchA, scratchB, cx32, xc32, keepM, keepCX, keepXC, fockbuf, dets_raw, dets_packed, tiles_buf, outH, outS, flag;
     nwi::DBuf req_in, req_key, req_bins, req_sorted;  // batched requests binned on the GPU
+    nwi::DBuf splitk;                                 // partial products of the split-K J - K contractions
     std::deque<std::vector<char>> keep;
     nwi::PinnedArena pinned;  // staging of Uploader copies
     nwi::CholState *chol = nullptr;  // integral factors (nw_set_eri_factors) and block coverage

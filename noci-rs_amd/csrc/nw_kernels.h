@@ -13,6 +13,13 @@ void launch_gemm(hipStream_t st, int ta, int tb, int M, int N, int K, int lda, i
 // C = A B^T on FP64 MFMA for any K (A: M x K, B: N x K, both K-contiguous).
 void launch_gemm_nt_mfma(hipStream_t st, int M, int N, int K, int lda, int ldb, int ldc, const GemmDesc *d_descs,
                          int nbatch);
+// The same products with K split ksplit ways (partials in part: nbatch x
+// ksplit x M x N doubles), summed in a fixed order into each C.
+// gemm_nt_splitk_count: the split for a launch of that shape on ncu CUs (1:
+// none needed).
+int gemm_nt_splitk_count(int M, int N, int K, int nbatch, int ncu);
+void launch_gemm_nt_splitk(hipStream_t st, int M, int N, int K, int lda, int ldb, int ldc, const GemmDesc *d_descs,
+                           int nbatch, int ksplit, double *part);
 void launch_svd(hipStream_t st, const double *A, int n, int nprob, double *U, double *V, double *sig,
                 double *phase, int *m, double *tsp, double tol);
 // Strides (in doubles) between consecutive problems of one launch.
