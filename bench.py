@@ -1003,6 +1003,14 @@ def wick_workload(args):
         dom = max(kern_roofs.items(), key=lambda kv: kv[1]["ms_per_step"])
         headline_roof = dict(dom[1], **{"traffic": None, "stage": dom[0],
                                         "profile": "profiles/r4_factored_c2_kernel_stats.txt"})
+        # HBM bytes per dispatch of k_r4_bf3 (copy-calibrated FETCH_SIZE + WRITE_SIZE, 3 dispatches per step)
+        tfile_r4 = os.path.join(ROOT, "profiles", "r4_factored_c2_counters.txt")
+        if best == "factored_bf16x3" and dom[0] == "kernel_f32_large" and os.path.exists(tfile_r4):
+            for line in open(tfile_r4):
+                if line.startswith("TRAFFIC_BYTES_PER_DISPATCH k_r4_bf3"):
+                    headline_roof["traffic"] = float(line.split()[2])
+                    headline_roof["traffic_source"] = ("profiles/r4_factored_c2_counters.txt (rocprofv3 FETCH_SIZE/"
+                                                       "WRITE_SIZE, copy-calibrated; per dispatch, 3 per step)")
     else:
         headline_roof = roof_dense
     cpu = None
