@@ -121,6 +121,28 @@ int nw_rank4_coverage(nw_ctx *ctx, int x, int w, double *frac3);
  * Diagnostics (measurement); no reference counterpart. */
 int nw_rank4_stats(nw_ctx *ctx, double *out10);
 
+/* Generalised Slater-Condon route for batched determinant pairs: replaces
+ * calculate_hs_pair_naive (src/noci/hs.rs:372-402: build_s_pair,
+ * one_electron, two_electron_same, two_electron_diff of
+ * src/noci/naive.rs:73-409) on the GPU, from the reference determinants of
+ * nw_set_refs and the integrals of nw_create (no nw_build needed).  req:
+ * nreq pairs (bra, ket) of indices into dets; h, s: nreq values each.  The
+ * determinants' occupations select their parent's occupied columns (ascending,
+ * occ_coeffs naive.rs:20-32); no excitation phase is applied, as in the
+ * reference.  O(n_bas^4) per pair (a cross-check route).  Host arrays only;
+ * NW_E_ARG for occupation counts that differ from the references'. */
+int nw_hs_naive_batched(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, const uint64_t *req, uint64_t nreq,
+                        double *h, double *s);
+
+/* wicks.compare: H, S of every pair by Wick's theorem as nw_hs_matrix
+ * (written to H, S), each upper-triangle pair also by the generalised
+ * Slater-Condon route (nw_hs_naive_batched); *total = sum over those pairs of
+ * |dH| + |dS|, *max_elem = the largest single |dH| or |dS|
+ * (compare_hs_pair_wicks_naive, src/noci/hs.rs:141-156, summed as in
+ * src/noci/matrix.rs:224-249).  Host arrays; up to 65536 determinants. */
+int nw_hs_compare(nw_ctx *ctx, const nw_det *dets, uint64_t ndet, double *H, double *S, int mode, double *total,
+                  double *max_elem);
+
 /* The same call's block products by kernel launch class: out8 = {device ms
  * of the FP64 pairs' large-tile products (k_r4_gemm, rectangles, single
  * blocks and exchange products), of their narrow-tile products (ket-particle

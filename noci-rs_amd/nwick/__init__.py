@@ -39,7 +39,7 @@ EXPORTS = ["nw_create", "nw_set_refs", "nw_build", "nw_hs_batched", "nw_hs_matri
            "nw_factor_info", "nw_factor_ids", "nw_factor_table", "nw_factor_last_ms", "nw_factor_destroy",
            "nw_factor_set_precision", "nw_factor_apply_pair", "nw_factor_apply_bytes", "nw_factor_block_groups", "nw_hs_batched_noalloc", "nw_s_same_batched", "nw_m_batched",
            "nw_set_eri_factors", "nw_rank4_coverage", "nw_rank4_stats",
-           "nw_rank4_kernel_stats"]
+           "nw_rank4_kernel_stats", "nw_hs_naive_batched", "nw_hs_compare"]
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_BF16X3 = 2
@@ -98,6 +98,8 @@ def load():
     L.nw_rank4_coverage.argtypes = [vp, ct.c_int, ct.c_int, dp]
     L.nw_rank4_stats.argtypes = [vp, dp]
     L.nw_rank4_kernel_stats.argtypes = [vp, dp]
+    L.nw_hs_naive_batched.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_uint64, vp, vp]
+    L.nw_hs_compare.argtypes = [vp, vp, ct.c_uint64, vp, vp, ct.c_int, dp, dp]
     L.nw_project_hs.argtypes = [vp, vp, ct.c_uint64, ct.c_uint64, vp, vp, vp, ct.c_int]
     L.nw_noci_fock.argtypes = [vp, vp, ct.c_uint64, vp, ct.c_double, vp, vp, vp, vp]
     L.nw_save_wicks.argtypes = [vp, ct.c_char_p]
@@ -320,6 +322,27 @@ class Engine:
         fn = self.L.nw_hs_batched_noalloc if noalloc else self.L.nw_hs_batched
         self._check(fn(self.ctx, _ptr(dets), ndet, _ptr(req), nreq, _ptr(h), _ptr(s), int(mode)))
         return h, s
+
+    def hs_naive_batched(self, dets, req):
+        """Generalised Slater-Condon H, S (calculate_hs_pair_naive) of request
+        pairs, from the references and integrals (no build needed)."""
+        req = np.ascontiguousarray(req, dtype=np.uint64).reshape(-1, 2)
+        dets = np.ascontiguousarray(dets)
+        h = np.empty(len(req)); s = np.empty(len(req))
+        self._check(self.L.nw_hs_naive_batched(self.ctx, _ptr(dets), len(dets), _ptr(req), len(req), _ptr(h),
+                                               _ptr(s)))
+        return h, s
+
+    def hs_compare(self, dets, mode=MODE_REFERENCE):
+        """wicks.compare: Wick's H, S of every pair plus the naive route's
+        discrepancy over the upper triangle: (H, S, total, max_element)."""
+        dets = np.ascontiguousarray(dets)
+        n = len(dets)
+        H = np.empty((n, n)); S = np.empty((n, n))
+        tot = ct.c_double(); mx = ct.c_double()
+        self._check(self.L.nw_hs_compare(self.ctx, _ptr(dets), n, _ptr(H), _ptr(S), int(mode), ct.byref(tot),
+                                         ct.byref(mx)))
+        return H, S, tot.value, mx.value
 
     def s_same_batched(self, dets, req, spin):
         """Per-spin overlap factors (excitation phases applied) of request pairs."""
