@@ -991,6 +991,137 @@ bool plan_missing(nw_ctx *c, const std::vector<PackedDet> &h, std::vector<PairWo
     return !work.empty();
 }
 
+// The FP32 factor products on BF16 MFMA with three-way split operands (the
+// split and the six products of k_r4_bf3): K staged 32 at a time as three
+// bf16 planes, one v_mfma_f32_16x16x32_bf16 per pair of parts; tiles of
+// 16 TM rows x 128 columns, each wave 16 TM x 32.  K must be a multiple of
+// 4 (16-byte loads; K past its end reads zeros).
+template <int TM>
+__global__ __launch_bounds__(256) void k_fac_gemm_bf3(const FacGemm *__restrict__ descs, int M, int N, int K, int lda,
+                                                      int ldb, int ldc)
+{
+    typedef float gf4 __attribute__((ext_vector_type(4)));
+    typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+    typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+    constexpr int TN = 2, KC = 32, VE = 4, KV = KC / VE, LDK = KC + 8, BM = 16 * TM, BN = 64 * TN;
+    constexpr int NV = (BM + BN) * KV, PER = (NV + 255) / 256;
+    __shared__ __attribute__((aligned(16))) __bf16 Xs[3][BM + BN][LDK];
+    const FacGemm g = descs[blockIdx.y];
+    const float *__restrict__ A = (const float *)g.A;
+    const float *__restrict__ B = (const float *)g.B;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+    const int n0 = blockIdx.x * BN, m0 = blockIdx.z * BM, mloc = min(BM, M - m0);
+    const float *src[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int e = tid + j * 256, r = e / KV, kv = e % KV;
+        const float *q = nullptr;
+        if (e < NV) {
+            if (r < BM) {
+                if (r < mloc) q = A + (size_t)(m0 + r) * lda + kv * VE;
+            } else if (n0 + r - BM < N) {
+                q = B + (size_t)(n0 + r - BM) * ldb + kv * VE;
+            }
+        }
+        src[j] = q;
+    }
+    gf4 buf[PER];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int kv = (tid + j * 256) % KV;
+            buf[j] = src[j] && k0 + kv * VE < K ? *reinterpret_cast<const gf4 *>(src[j] + k0) : gf4{0, 0, 0, 0};
+        }
+    };
+    r4_f4 acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = r4_f4{0, 0, 0, 0};
+    gload(0);
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = tid + j * 256, r = e / KV, kv = e % KV;
+            if (e >= NV) continue;
+            bf4 h, m, l;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float x = buf[j][i];
+                const __bf16 hh = (__bf16)x;
+                const float rem = x - (float)hh;
+                const __bf16 mm = (__bf16)rem;
+                h[i] = hh;
+                m[i] = mm;
+                l[i] = (__bf16)(rem - (float)mm);
+            }
+            *reinterpret_cast<bf4 *>(&Xs[0][r][kv * VE]) = h;
+            *reinterpret_cast<bf4 *>(&Xs[1][r][kv * VE]) = m;
+            *reinterpret_cast<bf4 *>(&Xs[2][r][kv * VE]) = l;
+        }
+        __syncthreads();
+        if (k0 + KC < K) gload(k0 + KC);  // next chunk in flight during the products
+        bf8 bh[TN], bm[TN], bl[TN];
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+            const int r = BM + wave * 16 * TN + b * 16 + lr;
+            bh[b] = *reinterpret_cast<const bf8 *>(&Xs[0][r][lk * 8]);
+            bm[b] = *reinterpret_cast<const bf8 *>(&Xs[1][r][lk * 8]);
+            bl[b] = *reinterpret_cast<const bf8 *>(&Xs[2][r][lk * 8]);
+        }
+#pragma unroll
+        for (int a = 0; a < TM; ++a) {
+            const int r = a * 16 + lr;
+            const bf8 ah = *reinterpret_cast<const bf8 *>(&Xs[0][r][lk * 8]);
+            const bf8 am = *reinterpret_cast<const bf8 *>(&Xs[1][r][lk * 8]);
+            const bf8 al = *reinterpret_cast<const bf8 *>(&Xs[2][r][lk * 8]);
+#pragma unroll
+            for (int b = 0; b < TN; ++b) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[b], acc[a][b], 0, 0, 0);
+            }
+        }
+    }
+    float *C = (float *)g.C;
+#pragma unroll
+    for (int b = 0; b < TN; ++b) {
+        const int col = n0 + wave * 16 * TN + b * 16 + lr;
+        if (col >= N) continue;
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = a * 16 + R4Mfma<float>::row(lk, r);
+                if (row < mloc) C[(size_t)(m0 + row) * ldc + col] = acc[a][b][r];
+            }
+    }
+}
+
+void launch_fac_gemm_bf3(hipStream_t st, int M, int N, int K, int lda, int ldb, int ldc, const FacGemm *d, int nb)
+{
+    const int tm = std::min(8, (M + 15) / 16);
+    const unsigned mblk = (unsigned)((M + 16 * tm - 1) / (16 * tm));
+    for (int z0 = 0; z0 < nb; z0 += 65535) {
+        const dim3 grid((unsigned)((N + 127) / 128), (unsigned)std::min(65535, nb - z0), mblk);
+        const FacGemm *dz = d + z0;
+        switch (tm) {
+            case 1: hipLaunchKernelGGL((k_fac_gemm_bf3<1>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 2: hipLaunchKernelGGL((k_fac_gemm_bf3<2>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 3: hipLaunchKernelGGL((k_fac_gemm_bf3<3>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 4: hipLaunchKernelGGL((k_fac_gemm_bf3<4>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 5: hipLaunchKernelGGL((k_fac_gemm_bf3<5>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 6: hipLaunchKernelGGL((k_fac_gemm_bf3<6>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            case 7: hipLaunchKernelGGL((k_fac_gemm_bf3<7>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+            default: hipLaunchKernelGGL((k_fac_gemm_bf3<8>), grid, dim3(256), 0, st, dz, M, N, K, lda, ldb, ldc); break;
+        }
+    }
+}
+
 template <typename T, int TN>
 void launch_fac_gemm(hipStream_t st, int M, int N, int K, int lda, int ldb, int ldc, const FacGemm *d, int nb)
 {
@@ -1076,8 +1207,13 @@ void factor_run(nw_ctx *c, const std::vector<FacJob> &jobs, Uploader &up)
             g2.push_back({trp[xi[j]], t1 + (size_t)(pos - (ys.begin() + g0)) * t1sz, jobs[j].out});
         }
         if (fast) {
-            launch_fac_gemm<T, f32 ? 2 : 1>(c->st, nm, naux * n, n, n, n, naux * n, up.put(g1), (int)g1.size());
-            launch_fac_gemm<T, f32 ? 2 : 1>(c->st, nm, nm * naux, n, n, n, nm * naux, up.put(g2), (int)g2.size());
+            if (f32 && r4_bf3(c)) {  // NW_PREC_BF16X3: the FP32 jobs' products on BF16 MFMA
+                launch_fac_gemm_bf3(c->st, nm, naux * n, n, n, n, naux * n, up.put(g1), (int)g1.size());
+                launch_fac_gemm_bf3(c->st, nm, nm * naux, n, n, n, nm * naux, up.put(g2), (int)g2.size());
+            } else {
+                launch_fac_gemm<T, f32 ? 2 : 1>(c->st, nm, naux * n, n, n, n, naux * n, up.put(g1), (int)g1.size());
+                launch_fac_gemm<T, f32 ? 2 : 1>(c->st, nm, nm * naux, n, n, n, nm * naux, up.put(g2), (int)g2.size());
+            }
         } else if constexpr (!f32) {
             auto gd = [](const std::vector<FacGemm> &v) {
                 std::vector<GemmDesc> o;

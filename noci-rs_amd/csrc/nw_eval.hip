@@ -187,8 +187,13 @@ __device__ inline void eval_tile(const EvalBlock *__restrict__ blocks, int nbloc
 // the runtime's limit of 2^32 - 1 work-items.
 constexpr int EVAL_XCD = 8;
 
+// Pairs without zero-overlap orbitals (M0) are held to two waves per SIMD
+// (256 VGPRs with the accumulation registers, a few bytes of spill): the
+// rank-4 bins then ran 22% faster than at one wave with AGPR spill space; the
+// m > 0 kernels keep the compiler's choice (at two waves they spill hundreds
+// of bytes per lane and ran 24% slower).
 template <int KIND, int LA, int LB, bool M0, bool GEN = false>
-__global__ __launch_bounds__(256) void k_eval_blocks(const EvalBlock *__restrict__ blocks, int nblocks, long long ntiles,
+__global__ __launch_bounds__(256, M0 ? 2 : 1) void k_eval_blocks(const EvalBlock *__restrict__ blocks, int nblocks, long long ntiles,
                                                      EvalArgs a)
 {
     const long long chunk = (ntiles + EVAL_XCD - 1) / EVAL_XCD;
