@@ -1267,27 +1267,56 @@ int nw_project_hs(nw_ctx *c, const nw_det *dets, uint64_t ndet, uint64_t ncur, c
         std::vector<PackedDet> h = pack(c, dets, ndet, dp);
         chol_ensure(c, h);
         auto bins = rect_blocks(c, h, (long long)ncur, (long long)ncur, true);
+        // Each block (candidate row run x current column run) sums its rows
+        // over its columns into its own slots; a row's projections are then its
+        // blocks' sums in column-run order (k_project_reduce), a fixed order.
+        std::vector<long long> ri0, cj0;  // row and column runs by first determinant
+        long long nslot = 0;
+        for (auto &kv : bins) {
+            long long t = 0;
+            for (EvalBlock &b : kv.second) {
+                b.tstart = t;  // strips of 16 rows
+                t += (b.ni + 15) / 16;
+                b.pslot = nslot;
+                nslot += b.ni;
+                ri0.push_back(b.i0);
+                cj0.push_back(b.j0);
+            }
+        }
+        for (auto *r : {&ri0, &cj0}) {
+            std::sort(r->begin(), r->end());
+            r->erase(std::unique(r->begin(), r->end()), r->end());
+        }
+        const int nrun = (int)ri0.size(), ncrun = (int)cj0.size();
+        std::vector<long long> table((size_t)nrun * ncrun, -1);
+        for (auto &kv : bins)
+            for (const EvalBlock &b : kv.second) {
+                const size_t k = std::lower_bound(ri0.begin(), ri0.end(), b.i0) - ri0.begin();
+                const size_t col = std::lower_bound(cj0.begin(), cj0.end(), b.j0) - cj0.begin();
+                table[k * ncrun + col] = b.pslot;
+            }
         const double *dx = (const double *)to_device(c, c->outH, coef, ncur * 8);
         const bool vdev = is_device_ptr(v), sdev = is_device_ptr(s);
         double *dv = vdev ? v : c->fockbuf.get<double>(2 * nc);
         double *ds = sdev ? s : (vdev ? c->fockbuf.get<double>(nc) : dv + nc);
-        std::vector<LinDesc> lz = {{nullptr, nullptr, nullptr, dv, 0, 0, 0}, {nullptr, nullptr, nullptr, ds, 0, 0, 0}};
-        Uploader up(c, 64u << 20);
-        launch_lincomb(c->st, up.put(lz), 2, (long long)nc);
+        double *part = c->scratchA.get<double>(2 * (size_t)std::max<long long>(1, nslot));
+        Uploader up(c, (64u << 20) + (ri0.size() + table.size()) * sizeof(long long));
         EvalArgs args = base_args(c, dp, mode);
         args.x = dx;
-        args.y = dv;
-        args.ys = ds;
-        args.ldo = (long long)ncur;  // row index offset of the outputs
+        args.y = part;
+        args.ys = part + nslot;
         HIPCHK(hipEventRecord(c->ev[1], c->st));
         for (auto &kv : bins) {
             const int key = kv.first;
             int la, lb;
             key_ranks(key, la, lb);
             EvalBlock *db = up.put(kv.second);
+            const EvalBlock &lastb = kv.second.back();
             launch_eval_blocks(c->st, EVAL_PROJECT, la, lb, key & 1, db, (int)kv.second.size(),
-                               tiles_of(kv.second), args);
+                               lastb.tstart + (lastb.ni + 15) / 16, args);
         }
+        launch_project_reduce(c->st, part, part + nslot, up.put(ri0), nrun, up.put(table), ncrun, (long long)ncur,
+                              (long long)nc, dv, ds);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c->ev[2], c->st));
         if (!vdev) HIPCHK(hipMemcpyAsync(v, dv, nc * 8, hipMemcpyDeviceToHost, c->st));

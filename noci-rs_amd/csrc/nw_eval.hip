@@ -108,34 +108,21 @@ __device__ inline void eval_tile(const EvalBlock *__restrict__ blocks, int nbloc
         const PackedDet bra = a.dets[i], ket = a.dets[j];
         if constexpr (GEN) {
             // contraction rank above four on a spin: the rank is the lane's own
-            if constexpr (KIND == EVAL_HS || KIND == EVAL_PROJECT) {
+            if constexpr (KIND == EVAL_HS) {
                 if constexpr (M0) hs_m0_rt(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);
                 else hs_gen_rt(P, slab, bra, ket, a.nmo, a.enuc, a.tol, a.mode, v0, v1);
             } else {
                 fs_elem_rt(P, slab, bra, ket, a.nmo, a.tol, a.mode, v0, v1);
             }
-        } else if constexpr (KIND == EVAL_HS || KIND == EVAL_PROJECT) {
+        } else if constexpr (KIND == EVAL_HS) {
             if constexpr (M0) hs_m0<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);
             else hs_gen<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, a.tol, a.mode, v0, v1);
         } else {
             fs_elem<LA, LB>(P, slab, bra, ket, a.nmo, a.tol, a.mode, v0, v1);
         }
     }
-    if constexpr (KIND == EVAL_PROJECT) {
-        // build_candidate_current_h + build_candidate_v (src/snoci/operators.rs:151, 991):
-        // v_a += H_ai c_i and s_a += S_ai c_i over the current-space columns.
-        const double xj = valid ? a.x[j] : 0.0;
-        double rh = valid ? v0 * xj : 0.0, rs = valid ? v1 * xj : 0.0;
-#pragma unroll
-        for (int o = 8; o >= 1; o >>= 1) {
-            rh += __shfl_xor(rh, o, 16);
-            rs += __shfl_xor(rs, o, 16);
-        }
-        if (lj == 0 && ti * 16 + li < B.ni) {
-            unsafeAtomicAdd(a.y + (i - a.ldo), rh);
-            unsafeAtomicAdd(a.ys + (i - a.ldo), rs);
-        }
-    } else if constexpr (KIND != EVAL_MATVEC) {
+    static_assert(KIND != EVAL_PROJECT, "projections run by strips (eval_strip_project)");
+    if constexpr (KIND != EVAL_MATVEC) {
         a.A[i * a.ldo + j] = v0;
         a.B[i * a.ldo + j] = v1;
         if (i != j) {
@@ -177,6 +164,86 @@ __device__ inline void eval_tile(const EvalBlock *__restrict__ blocks, int nbloc
     }
 }
 
+// build_candidate_current_h + build_candidate_v (src/snoci/operators.rs:151,
+// 991) over one 16-row strip s of a block: the workgroup walks the strip's
+// column tiles in order, each lane summing H_ai c_i and S_ai c_i over its
+// columns, the 16 lanes of a row then reduce by shuffles; the row's sums go
+// to its slot of the block (no atomics: k_project_reduce adds the blocks of a
+// row in a fixed order, so the projections do not depend on scheduling).
+template <int LA, int LB, bool M0, bool GEN>
+__device__ inline void eval_strip_project(const EvalBlock *__restrict__ blocks, int nblocks, const EvalArgs &a,
+                                          long long s)
+{
+    const EvalBlock B = blocks[find_block(blocks, nblocks, s)];
+    const int ntj = (B.nj + 15) >> 4;
+    const int ti = (int)(s - B.tstart);
+    const int li = threadIdx.x >> 4, lj = threadIdx.x & 15;
+    const long long i = B.i0 + ti * 16 + li;
+    const bool row_ok = ti * 16 + li < B.ni;
+    const PairInfo &P = a.pairs[B.pair];
+    const slab_t slab = (slab_t)(a.slab + P.base);
+    double rh = 0.0, rs = 0.0;
+    for (int tj = 0; tj < ntj; ++tj) {
+        const long long j = B.j0 + tj * 16 + lj;
+        if (!row_ok || tj * 16 + lj >= B.nj) continue;
+        const PackedDet bra = a.dets[i], ket = a.dets[j];
+        double v0, v1;
+        if constexpr (GEN) {
+            if constexpr (M0) hs_m0_rt(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);
+            else hs_gen_rt(P, slab, bra, ket, a.nmo, a.enuc, a.tol, a.mode, v0, v1);
+        } else {
+            if constexpr (M0) hs_m0<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, v0, v1);
+            else hs_gen<LA, LB>(P, slab, bra, ket, a.nmo, a.enuc, a.tol, a.mode, v0, v1);
+        }
+        const double xj = a.x[j];
+        rh += v0 * xj;
+        rs += v1 * xj;
+    }
+#pragma unroll
+    for (int o = 8; o >= 1; o >>= 1) {
+        rh += __shfl_xor(rh, o, 16);
+        rs += __shfl_xor(rs, o, 16);
+    }
+    if (lj == 0 && row_ok) {
+        a.y[B.pslot + ti * 16 + li] = rh;
+        a.ys[B.pslot + ti * 16 + li] = rs;
+    }
+}
+
+__global__ void k_project_reduce(const double *__restrict__ part_h, const double *__restrict__ part_s,
+                                 const long long *__restrict__ run_i0, int nruns, const long long *__restrict__ table,
+                                 int ncol, long long nrow0, long long nrows, double *__restrict__ v,
+                                 double *__restrict__ s)
+{
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const long long i = nrow0 + r;
+    int lo = 0, hi = nruns - 1;  // the last run starting at or before row i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (run_i0[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    const long long off = i - run_i0[lo];
+    double h = 0.0, o = 0.0;
+    for (int c = 0; c < ncol; ++c) {
+        const long long slot = table[(size_t)lo * ncol + c];
+        if (slot < 0) continue;
+        h += part_h[slot + off];
+        o += part_s[slot + off];
+    }
+    v[r] = h;
+    s[r] = o;
+}
+
+void launch_project_reduce(hipStream_t st, const double *part_h, const double *part_s, const long long *run_i0,
+                           int nruns, const long long *table, int ncol, long long nrow0, long long nrows, double *v,
+                           double *s)
+{
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(k_project_reduce, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, st, part_h, part_s,
+                       run_i0, nruns, table, ncol, nrow0, nrows, v, s);
+}
+
 // XCD-aware tile order: workgroups are dealt round-robin over the 8 XCDs, so
 // workgroup b runs on XCD b % 8.  The tile list is cut into 8 contiguous
 // chunks and XCD x takes chunk x, in order: neighbouring tiles (one bra strip
@@ -200,7 +267,10 @@ __global__ __launch_bounds__(256, M0 ? 2 : 1) void k_eval_blocks(const EvalBlock
     const int xcd = blockIdx.x % EVAL_XCD;
     const long long lo = xcd * chunk, hi = min(ntiles, lo + chunk);
     const long long step = gridDim.x / EVAL_XCD;
-    for (long long t = lo + blockIdx.x / EVAL_XCD; t < hi; t += step) eval_tile<KIND, LA, LB, M0, GEN>(blocks, nblocks, a, t);
+    for (long long t = lo + blockIdx.x / EVAL_XCD; t < hi; t += step) {
+        if constexpr (KIND == EVAL_PROJECT) eval_strip_project<LA, LB, M0, GEN>(blocks, nblocks, a, t);
+        else eval_tile<KIND, LA, LB, M0, GEN>(blocks, nblocks, a, t);
+    }
 }
 
 template <int KIND, int LA, int LB, bool M0, bool GEN = false>

@@ -76,7 +76,8 @@ struct EvalBlock {
     long long i0, j0;
     int ni, nj;
     int pair, tri;
-    long long tstart;
+    long long tstart;  // first tile (EVAL_PROJECT: first 16-row strip) of the block in its bin
+    long long pslot;   // EVAL_PROJECT: the block's first row slot in the partial-sum buffers
 };
 struct EvalReq { int a, b; long long out; };
 
@@ -93,6 +94,14 @@ struct EvalArgs {
     const double *x;      // matvec input
     double *y, *ys;       // matvec outputs (ys may be null)
 };
+
+// Candidate projections from the per-(block, row) partial sums of
+// EVAL_PROJECT: row r (determinant nrow0 + r) of row run k (runs sorted by
+// first row, run_i0[k]) sums slot table[k * ncol + c] + (row - run_i0[k]) of
+// every column run c in order (-1: none), h and s parts.
+void launch_project_reduce(hipStream_t st, const double *part_h, const double *part_s, const long long *run_i0,
+                           int nruns, const long long *table, int ncol, long long nrow0, long long nrows, double *v,
+                           double *s);
 
 void launch_pack_dets(hipStream_t st, const void *dets, long long ndet, int nref, PackedDet *out, int *err);
 
