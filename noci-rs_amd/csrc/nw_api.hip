@@ -201,7 +201,29 @@ int build_impl(nw_ctx *c, double tol, const uint8_t *pair_mask, nw_pair_meta *me
             gemm(c, up, 1, 0, no, no, n, nm, no, no, 1.0, 0.0, g2);  // Cx,occ^T (S Cw,occ)
         }
         // The SVD kernel reads problem q at A + q*no*no; repack into U as staging.
-        for (int sp = 0; sp < 2; ++sp) {
+        // With equal electron counts per spin both spins' problems go in one
+        // launch (one wave per problem: twice the problems run side by side).
+        if (c->nocc[0] == c->nocc[1]) {
+            const int no = c->nocc[0];
+            std::vector<LinDesc> cp;
+            double *stage = R;  // 4P*n*nm >= 2P*no*no
+            for (int q = 0; q < 2 * P; ++q)
+                cp.push_back({Socc + (size_t)q * nomax * nomax, nullptr, nullptr, stage + (size_t)q * no * no, 1.0, 0.0,
+                              0.0});
+            launch_lincomb(c->st, up.put(cp), 2 * P, (long long)no * no);
+            // no == nomax: problem sp * P + p lands where the per-spin launches put it
+            launch_svd(c->st, stage, no, 2 * P, U, V, sig, phase, mdev, tsp, tol);
+            std::vector<int> mh(2 * P);
+            HIPCHK(hipMemcpyAsync(mh.data(), mdev, 2 * P * sizeof(int), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            c->pinfo.assign((size_t)nref * nref, PairInfo{});
+            for (int p = 0; p < P; ++p) {
+                PairInfo &pi = c->pinfo[(size_t)px[p] * nref + pw[p]];
+                pi.ma = mh[p];
+                pi.mb = mh[P + p];
+            }
+        }
+        for (int sp = 0; sp < 2 && c->nocc[0] != c->nocc[1]; ++sp) {
             const int no = c->nocc[sp];
             std::vector<LinDesc> cp;
             double *stage = R;  // reuse R as staging (size 4P*n*nm >= P*no*no)

@@ -672,13 +672,24 @@ size_t r4_tiles(const std::vector<R4Gemm> &gs, int bm, int bn)
     return n;
 }
 
+// Tiles of each product in column bands of R4_BAND tiles, every row tile of
+// a band before the next band: an XCD's chunk is a run of this list and its
+// concurrently resident tiles then share a few A and B row blocks (a band of
+// 8 x 128 columns and the product's rows, ~2.5 MB of FP32 factor rows at
+// naux = 300, within one XCD's L2) instead of one row tile against dozens of
+// column tiles.
+constexpr int R4_BAND = 8;
+
 std::vector<R4GTile> r4_tile_list(const std::vector<R4Gemm> &gs, int bm, int bn)
 {
     std::vector<R4GTile> tl;
     tl.reserve(r4_tiles(gs, bm, bn));
-    for (size_t i = 0; i < gs.size(); ++i)
-        for (int m0 = 0; m0 < gs[i].M; m0 += bm)
-            for (int n0 = 0; n0 < gs[i].N; n0 += bn) tl.push_back({(int)i, m0, n0, 0});
+    for (size_t i = 0; i < gs.size(); ++i) {
+        const int nt = (gs[i].N + bn - 1) / bn;
+        for (int b0 = 0; b0 < nt; b0 += R4_BAND)
+            for (int m0 = 0; m0 < gs[i].M; m0 += bm)
+                for (int b = b0; b < std::min(nt, b0 + R4_BAND); ++b) tl.push_back({(int)i, m0, b * bn, 0});
+    }
     return tl;
 }
 
