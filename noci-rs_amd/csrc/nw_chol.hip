@@ -154,6 +154,40 @@ __device__ inline void r4_tile_index(const R4Gemm &g, const R4GTile &tl, int nm,
     }
 }
 
+// Column-band width of the tile order (see r4_tile_at).
+constexpr int R4_BAND = 8;
+
+// Tile t of a launch over products gs: tstart[g] (nprod + 1 entries) counts
+// the tiles before product g; within a product the tiles go in column bands
+// of R4_BAND tiles, every row tile of a band before the next band, so an
+// XCD's run of tiles (its resident workgroups) shares a few A and B row
+// blocks in L2.  The host passes only tstart; no tile list is built.
+__device__ inline R4GTile r4_tile_at(const long long *__restrict__ tstart, int nprod, const R4Gemm *__restrict__ gs,
+                                     long long t, int bm, int bn)
+{
+    int lo = 0, hi = nprod - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tstart[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const R4Gemm &g = gs[lo];
+    const long long l = t - tstart[lo];
+    const int mt = (g.M + bm - 1) / bm, nt = (g.N + bn - 1) / bn, full = nt / R4_BAND;
+    const long long fb = (long long)R4_BAND * mt;
+    int m, n;
+    if (l < full * fb) {
+        const long long rem = l % fb;
+        m = (int)(rem / R4_BAND);
+        n = (int)(l / fb) * R4_BAND + (int)(rem % R4_BAND);
+    } else {
+        const int w = nt - full * R4_BAND;
+        const long long rem = l - full * fb;
+        m = (int)(rem / w);
+        n = full * R4_BAND + (int)(rem % w);
+    }
+    return R4GTile{lo, m * bm, n * bn, 0};
+}
+
 // Writes a wave's TM x TN 16 x 16 result tiles (register r4 of lane group lk
 // holds row L::row(lk, r4), lane & 15 the column) by the product's mode.
 template <typename T, typename L, typename Acc, int TM, int TN, int WM, int WN>
@@ -187,8 +221,8 @@ __device__ inline void r4_store(const R4Gemm &g, const Acc (&acc)[TM][TN], int w
 // does not depend on the order of k), so each lane's fragments of a chunk
 // are contiguous in LDS and read 16 bytes at a time.
 template <typename T, int BM, int BN, int KC, int OCC, bool GL>
-__global__ __launch_bounds__(256, OCC) void k_r4_gemm(const R4Gemm *__restrict__ gs, const R4GTile *__restrict__ tiles,
-                                                      int ntiles, int nm, int naux)
+__global__ __launch_bounds__(256, OCC) void k_r4_gemm(const R4Gemm *__restrict__ gs, const long long *__restrict__ tstart,
+                                                      int nprod, long long ntiles, int nm, int naux)
 {
     typedef typename R4Vec<T>::t VT;
     typedef const T __attribute__((address_space(1))) *gptr;
@@ -203,10 +237,10 @@ __global__ __launch_bounds__(256, OCC) void k_r4_gemm(const R4Gemm *__restrict__
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
     const int wm = wave >> 1, wn = wave & 1;
     const int gper = gridDim.x / 8, chunk = blockIdx.x % 8, lidx = blockIdx.x / 8;
-    const int per = (ntiles + 7) / 8;
-    const int t0 = chunk * per, t1 = min(ntiles, t0 + per);
-    for (int it = t0 + lidx; it < t1; it += gper) {
-        const R4GTile tl = tiles[it];
+    const long long per = (ntiles + 7) / 8;
+    const long long t0 = chunk * per, t1 = min(ntiles, t0 + per);
+    for (long long it = t0 + lidx; it < t1; it += gper) {
+        const R4GTile tl = r4_tile_at(tstart, nprod, gs, it, BM, BN);
         const R4Gemm g = gs[tl.g];
         const int mloc = min(BM, g.M - tl.m0), nloc = min(BN, g.N - tl.n0);
         __syncthreads();  // the previous tile's indices are no longer read
@@ -285,8 +319,8 @@ __global__ __launch_bounds__(256, OCC) void k_r4_gemm(const R4Gemm *__restrict__
 // rounding, so the result carries FP32-level error.  Lane group lk holds
 // k = 8 lk .. 8 lk + 7 of the chunk (one 16-byte LDS read per plane).
 template <int BM, int BN, int OCC, bool GL>
-__global__ __launch_bounds__(256, OCC) void k_r4_bf3(const R4Gemm *__restrict__ gs, const R4GTile *__restrict__ tiles,
-                                                     int ntiles, int nm, int naux)
+__global__ __launch_bounds__(256, OCC) void k_r4_bf3(const R4Gemm *__restrict__ gs, const long long *__restrict__ tstart,
+                                                     int nprod, long long ntiles, int nm, int naux)
 {
     typedef float gf4 __attribute__((ext_vector_type(4)));
     typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
@@ -301,10 +335,10 @@ __global__ __launch_bounds__(256, OCC) void k_r4_bf3(const R4Gemm *__restrict__ 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
     const int wm = wave >> 1, wn = wave & 1;
     const int gper = gridDim.x / 8, chunk = blockIdx.x % 8, lidx = blockIdx.x / 8;
-    const int per = (ntiles + 7) / 8;
-    const int t0 = chunk * per, t1 = min(ntiles, t0 + per);
-    for (int it = t0 + lidx; it < t1; it += gper) {
-        const R4GTile tl = tiles[it];
+    const long long per = (ntiles + 7) / 8;
+    const long long t0 = chunk * per, t1 = min(ntiles, t0 + per);
+    for (long long it = t0 + lidx; it < t1; it += gper) {
+        const R4GTile tl = r4_tile_at(tstart, nprod, gs, it, BM, BN);
         const R4Gemm g = gs[tl.g];
         const int mloc = min(BM, g.M - tl.m0), nloc = min(BN, g.N - tl.n0);
         __syncthreads();
@@ -663,7 +697,7 @@ int r4_ncu()
 
 // Workgroups of a product launch: up to eight per CU, in multiples of eight
 // (the XCD chunks); each runs its chunk's tiles in a grid-stride loop.
-int r4_grid(int nitems) { return std::max(8, std::min((nitems + 7) / 8 * 8, 8 * r4_ncu())); }
+int r4_grid(long long nitems) { return (int)std::max(8ll, std::min((nitems + 7) / 8 * 8, 8ll * r4_ncu())); }
 
 size_t r4_tiles(const std::vector<R4Gemm> &gs, int bm, int bn)
 {
@@ -672,45 +706,34 @@ size_t r4_tiles(const std::vector<R4Gemm> &gs, int bm, int bn)
     return n;
 }
 
-// Tiles of each product in column bands of R4_BAND tiles, every row tile of
-// a band before the next band: an XCD's chunk is a run of this list and its
-// concurrently resident tiles then share a few A and B row blocks (a band of
-// 8 x 128 columns and the product's rows, ~2.5 MB of FP32 factor rows at
-// naux = 300, within one XCD's L2) instead of one row tile against dozens of
-// column tiles.
-constexpr int R4_BAND = 8;
-
-std::vector<R4GTile> r4_tile_list(const std::vector<R4Gemm> &gs, int bm, int bn)
+// Tile offsets of each product (r4_tile_at maps a tile index to its product
+// and position on the device).
+std::vector<long long> r4_tstart(const std::vector<R4Gemm> &gs, int bm, int bn)
 {
-    std::vector<R4GTile> tl;
-    tl.reserve(r4_tiles(gs, bm, bn));
-    for (size_t i = 0; i < gs.size(); ++i) {
-        const int nt = (gs[i].N + bn - 1) / bn;
-        for (int b0 = 0; b0 < nt; b0 += R4_BAND)
-            for (int m0 = 0; m0 < gs[i].M; m0 += bm)
-                for (int b = b0; b < std::min(nt, b0 + R4_BAND); ++b) tl.push_back({(int)i, m0, b * bn, 0});
-    }
-    return tl;
+    std::vector<long long> ts(gs.size() + 1, 0);
+    for (size_t i = 0; i < gs.size(); ++i)
+        ts[i + 1] = ts[i] + (long long)((gs[i].M + bm - 1) / bm) * ((gs[i].N + bn - 1) / bn);
+    return ts;
 }
 
 template <typename T, int BM, int BN, int KC, int OCC, bool GL>
 void launch_r4_shape(nw_ctx *c, Uploader &up, const R4Gemm *dg, const std::vector<R4Gemm> &gs, int naux)
 {
-    const std::vector<R4GTile> tl = r4_tile_list(gs, BM, BN);
-    if (tl.empty()) return;
-    const R4GTile *dt = up.put(tl);
-    hipLaunchKernelGGL((k_r4_gemm<T, BM, BN, KC, OCC, GL>), dim3(r4_grid((int)tl.size())), dim3(256), 0, c->st, dg, dt,
-                       (int)tl.size(), c->nmo, naux);
+    const std::vector<long long> ts = r4_tstart(gs, BM, BN);
+    const long long nt = ts.back();
+    if (nt == 0) return;
+    hipLaunchKernelGGL((k_r4_gemm<T, BM, BN, KC, OCC, GL>), dim3(r4_grid(nt)), dim3(256), 0, c->st, dg, up.put(ts),
+                       (int)gs.size(), nt, c->nmo, naux);
 }
 
 template <int BM, int BN, int OCC, bool GL>
 void launch_r4_bf3_shape(nw_ctx *c, Uploader &up, const R4Gemm *dg, const std::vector<R4Gemm> &gs, int naux)
 {
-    const std::vector<R4GTile> tl = r4_tile_list(gs, BM, BN);
-    if (tl.empty()) return;
-    const R4GTile *dt = up.put(tl);
-    hipLaunchKernelGGL((k_r4_bf3<BM, BN, OCC, GL>), dim3(r4_grid((int)tl.size())), dim3(256), 0, c->st, dg, dt,
-                       (int)tl.size(), c->nmo, naux);
+    const std::vector<long long> ts = r4_tstart(gs, BM, BN);
+    const long long nt = ts.back();
+    if (nt == 0) return;
+    hipLaunchKernelGGL((k_r4_bf3<BM, BN, OCC, GL>), dim3(r4_grid(nt)), dim3(256), 0, c->st, dg, up.put(ts),
+                       (int)gs.size(), nt, c->nmo, naux);
 }
 
 // FP32 pairs of an NW_PREC_BF16X3 context take their block products on BF16
@@ -1549,7 +1572,7 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
     size_t bytes = ints.size() * sizeof(int) + 8192;
     for (int f = 0; f < 2; ++f)  // descriptors, and tiles of at most 64 x 64 elements
         for (auto *v : {&gd[f], &gk[f], &gx[f], &gs1[f]})
-            bytes += v->size() * sizeof(R4Gemm) + r4_tiles(*v, 64, 32) * sizeof(R4GTile) + 1024;
+            bytes += v->size() * (sizeof(R4Gemm) + 2 * sizeof(long long)) + 2048;
     Uploader up(c, bytes);
     dints = up.put(ints);
     auto fix = [&](const int *p) { return p ? dints + ((uintptr_t)p - 1) : nullptr; };
