@@ -1121,7 +1121,7 @@ def stream_workload(args):
 
     runs = {}
     forms = (("fp64", nwick.PREC_FP64, False), ("bf16x3", nwick.PREC_BF16X3, False),
-             ("factored_mixed", nwick.PREC_MIXED, True))
+             ("factored_mixed", nwick.PREC_MIXED, True), ("factored_bf16x3", nwick.PREC_BF16X3, True))
     for name, prec, factored in forms:
         if factored:  # the workload's 300 Cholesky vectors: rank-4 blocks built on demand per group
             eng.set_eri_factors(sysm.chol)
@@ -1140,7 +1140,7 @@ def stream_workload(args):
                 f" ms, evaluations {tm['eval_ms']:.1f} ms)")
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
-        stages = eng.rank4_stats() if factored else None
+        stages = {**eng.rank4_stats(), **eng.rank4_kernel_stats()} if factored else None
         if factored:
             eng.set_eri_factors(None)
         energy = float("nan") if args.no_energy else lowest_energy()
@@ -1159,11 +1159,12 @@ def stream_workload(args):
         log(f"[stream {name}] {runs[name]['rate']:.4e} pairs/s, {groups} groups, energy {energy:.12f}")
     eng.close()
     del Hd, Sd
-    r64, rbf, rfac = runs["fp64"], runs["bf16x3"], runs["factored_mixed"]
+    r64, rbf, rfac, rfbf = runs["fp64"], runs["bf16x3"], runs["factored_mixed"], runs["factored_bf16x3"]
     dE = None if args.no_energy else rbf["energy"] - r64["energy"]
     dE_fac = None if args.no_energy else rfac["energy"] - r64["energy"]
+    dE_fbf = None if args.no_energy else rfbf["energy"] - r64["energy"]
     # the fastest form within 1 uHa of FP64
-    ok = [n for n in ("bf16x3", "factored_mixed")
+    ok = [n for n in ("bf16x3", "factored_mixed", "factored_bf16x3")
           if args.no_energy or abs(runs[n]["energy"] - r64["energy"]) < 1e-6]
     best = max(ok, key=lambda n: runs[n]["rate"]) if ok else "fp64"
     use_bf = best == "bf16x3"
@@ -1171,7 +1172,7 @@ def stream_workload(args):
     # dominant kernel: the four-index transforms inside the builds; the build time
     # also holds the SVDs, co-densities and small GEMMs, so this is a lower bound
     n = 100
-    rd = r64 if best == "factored_mixed" else res  # the dense form measured
+    rd = r64 if best.startswith("factored") else res  # the dense form measured
     bytes_tensor = 32.0 * n ** 4  # four passes, each reading and writing n^4 floats
     alg = rd["fp32_tensors"] * bytes_tensor + (rd["rank4_tensors"] - rd["fp32_tensors"]) * 2 * bytes_tensor
     achieved = alg / (rd["build_ms"] * 1e-3) / 1e9
@@ -1181,8 +1182,16 @@ def stream_workload(args):
                   "bytes_meaning": "32 n^4 bytes per reduced-precision rank-4 tensor (four passes over FP32), "
                                    "64 n^4 per FP64 tensor", "form": "bf16x3" if rd is rbf else "fp64"}
     roof = roof_dense
-    st = rfac["rank4_stages_last_step"]
-    if best == "factored_mixed" and st:
+    st = runs[best]["rank4_stages_last_step"] if best.startswith("factored") else None
+    if best == "factored_bf16x3" and st and st["ms_f32_large"] > 0.0:
+        # the dominant kernel: the FP32 pairs' large-tile block products (k_r4_bf3), all groups of the last step
+        ach = st["flops_f32_large"] / (st["ms_f32_large"] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "k_r4_bf3 (block products of the FP32 pairs, BF16 MFMA on split operands)",
+                "achieved": ach, "peak": PEAK_BF16_TFLOPS / 6.0, "unit": "TFLOP/s",
+                "frac": ach / (PEAK_BF16_TFLOPS / 6.0), "traffic": None, "ms_per_step": st["ms_f32_large"],
+                "flops_per_step": st["flops_f32_large"],
+                "note": "FP32-equivalent flops; peak = BF16 dense peak / 6 (six BF16 products per FP32 product)"}
+    elif best == "factored_mixed" and st:
         # dominant stage of the on-demand rank-4 blocks (all groups of the last step)
         cand = [("factor", "ms_factor", "flops_factor", "k_fac_gemm (factor products X^T L_K Y)"),
                 ("rect", "ms_rect", "flops_rect", "k_r4_gemm (rectangle products)"),
@@ -1202,13 +1211,14 @@ def stream_workload(args):
                   "reuse per group of reference pairs)",
         "value": res["rate"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"bf16x3": "bf16x3+f64", "factored_mixed": "f32+f64", "fp64": "f64"}[best], "data": "synthetic",
+        "dtype": {"bf16x3": "bf16x3+f64", "factored_mixed": "f32+f64", "factored_bf16x3": "bf16x3+f32+f64",
+                  "fp64": "f64"}[best], "data": "synthetic",
         "config": {"workload": "batched Wick elements, n_bas=nmo=100, nocc=10+10, 16 refs (1 with an alpha orbital "
                                "orthogonal to 7 others), 512 singles/doubles per ref, 8192 dets, "
                                f"{npairs} pairs, 136 upper-triangle reference pairs", "mode": args.mode,
                    "form": best},
-        "fp64": r64, "bf16x3": rbf, "factored_mixed": rfac, "dE_bf16x3_minus_fp64": dE,
-        "dE_factored_mixed_minus_fp64": dE_fac,
+        "fp64": r64, "bf16x3": rbf, "factored_mixed": rfac, "factored_bf16x3": rfbf, "dE_bf16x3_minus_fp64": dE,
+        "dE_factored_mixed_minus_fp64": dE_fac, "dE_factored_bf16x3_minus_fp64": dE_fbf,
         "roofline": roof, "roofline_dense_build": roof_dense,
         "cpu_baseline": cpu,
     }
