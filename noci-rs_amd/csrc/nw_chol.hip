@@ -309,6 +309,19 @@ __global__ __launch_bounds__(256, OCC) void k_r4_gemm(const R4Gemm *__restrict__
     }
 }
 
+// Staging map of the BF16-split kernels: element e (a 16-byte piece of
+// four FP32 values, KV = 8 pieces per row of a K = 32 chunk) goes to row r
+// and piece kv so that each wave's 64 lanes hold 16 rows x 4 pieces, lane
+// pairs on adjacent pieces of one row: the 8-byte writes of one plane then
+// cover 16-byte slots of consecutive rows, which fall on distinct LDS banks
+// at a row pitch of 80 bytes (as do the 16-byte fragment reads).
+__device__ inline void bf3_piece(int e, int &r, int &kv)
+{
+    const int g = e >> 6, l = e & 63;
+    r = (g >> 1) * 16 + ((l >> 1) & 15);
+    kv = (g & 1) * 4 + 2 * (l >> 5) + (l & 1);
+}
+
 // The same FP32 products on BF16 MFMA: each FP32 operand v is split into
 // three BF16 parts h + m + l (h = bf16(v), m = bf16(v - h), l = bf16(v - h -
 // m), round to nearest even), which hold its 24-bit significand, as the
@@ -329,6 +342,7 @@ __global__ __launch_bounds__(256, OCC) void k_r4_bf3(const R4Gemm *__restrict__ 
     constexpr int KC = 32, VE = 4, KV = KC / VE, LDK = KC + 8, NT = 256, ROWS = BM + BN;
     constexpr int PER = ROWS * KV / NT;
     static_assert(ROWS * KV % NT == 0, "tile shape");
+    static_assert(KV == 8 && ROWS % 16 == 0, "bf3_piece layout");
     constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
     __shared__ __attribute__((aligned(16))) __bf16 Xs[3][ROWS][LDK];  // planes h, m, l; rows A then B
     __shared__ long long ra[BM], rb[BN], ro[BM], co[BN], mro[BM], mco[BN];
@@ -350,7 +364,8 @@ __global__ __launch_bounds__(256, OCC) void k_r4_bf3(const R4Gemm *__restrict__ 
         bool okr[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const int e = tid + j * NT, r = e / KV, kv = e % KV;
+            int r, kv;
+            bf3_piece(tid + j * NT, r, kv);
             const long long b = r < BM ? ra[r] : rb[r - BM];
             okr[j] = b >= 0;
             src[j] = (gptr)((r < BM ? A : B) + (b < 0 ? 0 : b) + kv * VE);
@@ -359,7 +374,8 @@ __global__ __launch_bounds__(256, OCC) void k_r4_bf3(const R4Gemm *__restrict__ 
         auto gload = [&](int k0) {
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
-                const int kv = (tid + j * NT) % KV;
+                int r, kv;
+                bf3_piece(tid + j * NT, r, kv);
                 const bool ok = okr[j] && k0 + kv * VE < naux;
                 if constexpr (GL) {
                     const gf4 v = *reinterpret_cast<const gf4 __attribute__((address_space(1))) *>(src[j] + (ok ? k0 : 0));
@@ -379,7 +395,8 @@ __global__ __launch_bounds__(256, OCC) void k_r4_bf3(const R4Gemm *__restrict__ 
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
-                const int e = tid + j * NT, r = e / KV, kv = e % KV;
+                int r, kv;
+                bf3_piece(tid + j * NT, r, kv);
                 bf4 h, m, l;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -411,15 +428,20 @@ __global__ __launch_bounds__(256, OCC) void k_r4_bf3(const R4Gemm *__restrict__ 
                 const bf8 bh = *reinterpret_cast<const bf8 *>(&Xs[0][r][lk * 8]);
                 const bf8 bm = *reinterpret_cast<const bf8 *>(&Xs[1][r][lk * 8]);
                 const bf8 bl = *reinterpret_cast<const bf8 *>(&Xs[2][r][lk * 8]);
+                // each part pair over the TM row tiles in turn, so that
+                // consecutive MFMAs write different accumulators
 #pragma unroll
-                for (int a = 0; a < TM; ++a) {
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[a], bm, acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[a], bh, acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bm, acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
-                }
+                for (int a = 0; a < TM; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
+#pragma unroll
+                for (int a = 0; a < TM; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
+#pragma unroll
+                for (int a = 0; a < TM; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[a], bm, acc[a][b], 0, 0, 0);
+#pragma unroll
+                for (int a = 0; a < TM; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[a], bh, acc[a][b], 0, 0, 0);
+#pragma unroll
+                for (int a = 0; a < TM; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bm, acc[a][b], 0, 0, 0);
+#pragma unroll
+                for (int a = 0; a < TM; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
             }
         }
         r4_store<float, R4Mfma<float>, r4_f4, TM, TN, WM, WN>(g, acc, wm, wn, mloc, nloc, ro, co, mro, mco);
@@ -1039,6 +1061,7 @@ __global__ __launch_bounds__(256) void k_fac_gemm_bf3(const FacGemm *__restrict_
     typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
     constexpr int TN = 2, KC = 32, VE = 4, KV = KC / VE, LDK = KC + 8, BM = 16 * TM, BN = 64 * TN;
     constexpr int NV = (BM + BN) * KV, PER = (NV + 255) / 256;
+    static_assert(KV == 8 && (BM + BN) % 16 == 0, "bf3_piece layout");
     __shared__ __attribute__((aligned(16))) __bf16 Xs[3][BM + BN][LDK];
     const FacGemm g = descs[blockIdx.y];
     const float *__restrict__ A = (const float *)g.A;
@@ -1048,7 +1071,9 @@ __global__ __launch_bounds__(256) void k_fac_gemm_bf3(const FacGemm *__restrict_
     const float *src[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-        const int e = tid + j * 256, r = e / KV, kv = e % KV;
+        const int e = tid + j * 256;
+        int r, kv;
+        bf3_piece(e, r, kv);
         const float *q = nullptr;
         if (e < NV) {
             if (r < BM) {
@@ -1063,7 +1088,8 @@ __global__ __launch_bounds__(256) void k_fac_gemm_bf3(const FacGemm *__restrict_
     auto gload = [&](int k0) {
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const int kv = (tid + j * 256) % KV;
+            int r, kv;
+            bf3_piece(tid + j * 256, r, kv);
             buf[j] = src[j] && k0 + kv * VE < K ? *reinterpret_cast<const gf4 *>(src[j] + k0) : gf4{0, 0, 0, 0};
         }
     };
@@ -1077,8 +1103,10 @@ __global__ __launch_bounds__(256) void k_fac_gemm_bf3(const FacGemm *__restrict_
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const int e = tid + j * 256, r = e / KV, kv = e % KV;
+            const int e = tid + j * 256;
             if (e >= NV) continue;
+            int r, kv;
+            bf3_piece(e, r, kv);
             bf4 h, m, l;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
