@@ -1121,12 +1121,20 @@ __global__ __launch_bounds__(256) void k_gemm_nt_mfma(const GemmDesc *__restrict
     constexpr int PA = MT * BK / 2 / 256, PB = NTL * BK / 2 / 256;  // double2 per thread per chunk
     __shared__ double As[MT][LD];
     __shared__ double Bs[NTL][LD];
-    const int zb = blockIdx.z / ksplit, sk = blockIdx.z % ksplit;
+    // Workgroups go to the 8 XCDs round-robin by linear id; tile T is given
+    // to linear id L so that each XCD runs a contiguous range of T, in which
+    // the row blocks sharing a B tile (x fastest) meet in that XCD's L2.
+    const unsigned total = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned L = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const unsigned xcd = L % 8, q8 = total / 8, r8 = total % 8;
+    const unsigned T = xcd * q8 + min(xcd, r8) + L / 8;
+    const int bx = (int)(T % gridDim.x), by = (int)(T / gridDim.x % gridDim.y), bz = (int)(T / (gridDim.x * gridDim.y));
+    const int zb = bz / ksplit, sk = bz % ksplit;
     const GemmDesc g = descs[zb];
     const int kbeg = sk * kc, kend = min(K, kbeg + kc);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
     // row blocks of M vary fastest over the grid: the workgroups sharing a B tile run together
-    const int m0 = blockIdx.x * MT, n0 = blockIdx.y * NTL;
+    const int m0 = bx * MT, n0 = by * NTL;
     const bool vec = (K % 2 == 0) && (lda % 2 == 0) && (ldb % 2 == 0);
     double2 ra[PA], rb[PB];
     auto fetch = [&](int k0) {
@@ -1189,7 +1197,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt_mfma(const GemmDesc *__restrict
     }
     const int col = n0 + wave * 16 + lr;
     if (col >= N) return;
-    double *dst = part ? part + (size_t)blockIdx.z * M * N : g.C;
+    double *dst = part ? part + (size_t)bz * M * N : g.C;
     const size_t ld = part ? (size_t)N : (size_t)ldc;
 #pragma unroll
     for (int ms = 0; ms < 3; ++ms)
