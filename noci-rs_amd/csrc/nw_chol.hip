@@ -1426,12 +1426,6 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
     // index lists shared by the products of a (pair, class), in one upload:
     // rectangle rows, per-q ket-particle partners, sparse lists of single
     // blocks, exchange rows and columns.  Offsets are patched after upload.
-    std::vector<int> ints;
-    auto put = [&](const std::vector<int> &v) {
-        const size_t o = ints.size();
-        ints.insert(ints.end(), v.begin(), v.end());
-        return o;
-    };
     struct ClassLists {
         std::vector<std::pair<size_t, size_t>> rect;  // offsets of P and R lists
         std::vector<std::pair<int, size_t>> kq;       // (q, offset of S(q)) with |S(q)| in kqn
@@ -1446,10 +1440,24 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
         int nxrow = 0, nxcol = 0;
     };
     std::vector<ClassLists> cls(work.size() * 3);
-    std::vector<uint8_t> is_single(nm2, 0);  // single blocks of the current J class (cleared after it)
-    ints.reserve(1 << 20);
-    for (size_t i = 0; i < work.size(); ++i)
-        for (int cl = 0; cl < 3; ++cl) {
+    // The (pair, class) lists are independent: a few host threads, each
+    // appending to an index array of its own; the arrays are then joined
+    // and each list's offsets moved by its array's start.
+    const size_t ntask = work.size() * 3;
+    const size_t nth = std::min<size_t>({8, ntask, std::max(1u, std::thread::hardware_concurrency())});
+    std::vector<std::vector<int>> tints(nth);
+    auto lists_range = [&](size_t t0) {
+        std::vector<int> &ints = tints[t0];
+        ints.reserve(1 << 18);
+        auto put = [&](const std::vector<int> &v) {
+            const size_t o = ints.size();
+            ints.insert(ints.end(), v.begin(), v.end());
+            return o;
+        };
+        std::vector<uint8_t> is_single(nm2, 0);  // single blocks of the current J class (cleared after it)
+        for (size_t task = t0; task < ntask; task += nth) {
+            const size_t i = task / 3;
+            const int cl = (int)(task % 3);
             const ClassWork &cw = work[i].cls[cl];
             if (cw.blocks.empty()) continue;
             ClassLists &L = cls[i * 3 + cl];
@@ -1531,6 +1539,38 @@ void build_blocks(nw_ctx *c, std::vector<PairWork> &work)
                 L.nxcol = (int)cq.size();
             }
         }
+    };
+    if (nth > 1) {
+        std::vector<std::thread> th;
+        for (size_t t = 1; t < nth; ++t) th.emplace_back(lists_range, t);
+        lists_range(0);
+        for (auto &t : th) t.join();
+    } else {
+        lists_range(0);
+    }
+    std::vector<int> ints;
+    std::vector<size_t> tbase(nth);
+    {
+        size_t tot = 0;
+        for (const auto &v : tints) tot += v.size();
+        ints.reserve(tot);
+        for (size_t t = 0; t < nth; ++t) {
+            tbase[t] = ints.size();
+            ints.insert(ints.end(), tints[t].begin(), tints[t].end());
+        }
+    }
+    for (size_t task = 0; task < ntask; ++task) {
+        ClassLists &L = cls[task];
+        const size_t b = tbase[task % nth];
+        for (auto &x : L.rect) x = {x.first + b, x.second + b};
+        for (auto &x : L.kq) x.second += b;
+        for (auto *v : {&L.sgl, &L.sglm, &L.sglp})
+            for (auto &x : *v) x = {x.first + b, x.second + b};
+        L.xp += b;
+        L.xr += b;
+        L.xq += b;
+        L.xs += b;
+    }
     // List pointers hold (offset + 1) into `ints` until the upload; 0 = none.
     const int *dints = nullptr;
     auto at = [](size_t off) { return (const int *)(uintptr_t)(off + 1); };
